@@ -1,0 +1,225 @@
+"""Benchmark: the 1-D Couette flow step on one MI355X.
+
+Workload (the reference's Couette case, test/test_1D_couette.jl:66-97, at the
+scale of the headline configuration): argon VHS (T_ref 273 K, omega 0.81),
+500 cells over L = 5e-4 m, diffuse walls at 300 K moving at -/+500 m/s, 200
+equal-weight particles per cell (100,000 particles) sampled Maxwellian at
+300 K, number density 1e23 m^-3, FP64, dt = 2e-10 s (CFL 0.17).  One step is
+ntc! in every cell, convection, the bit-exact counting sort, and
+compute_props_sorted!.  Inputs are synthetic (sampled here with numpy) and
+resident on the GPU before the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+K_B = 1.380649e-23
+N_CELLS, PPC, L = 500, 200, 5e-4
+T_WALL, V_WALL, NDENS, DT = 300.0, 500.0, 1e23, 2.0e-10
+
+# algorithmic bytes per particle (reference method) of each kernel family:
+# convection reads x, v_x and writes x (+ the cell key) 28 B; the sort reads
+# the key twice and reads/writes the index 16 B; props read w, v and the index
+# 36 B; NTC touches < 1 B per particle at this collision rate.
+ALG_BYTES = {"convect": 28.0, "sort": 16.0, "props": 36.0, "ntc": 1.0, "step": 80.0}
+
+
+def sample_case(seed=12345):
+    """Particles in the reference layout: cell c holds logical positions
+    [(c-1) ppc + 1, c ppc], ids 1..N, positions uniform in the cell,
+    Maxwellian velocities (src/distributions_and_sampling.jl:432-509)."""
+    rng = np.random.default_rng(seed)
+    n = N_CELLS * PPC
+    dx = L / N_CELLS
+    m = 66.3e-27
+    vs = np.sqrt(2 * K_B * T_WALL / m)
+    cells = np.repeat(np.arange(N_CELLS), PPC)
+    x = np.empty((n, 3))
+    x[:, 0] = (cells + rng.uniform(size=n)) * dx
+    x[:, 1:] = rng.uniform(size=(n, 2))
+    vn = np.sqrt(-np.log(1.0 - rng.uniform(size=n)))
+    vr = np.sqrt(-np.log(1.0 - rng.uniform(size=n)))
+    t1 = 2 * np.pi * rng.uniform(size=n)
+    t2 = 2 * np.pi * rng.uniform(size=n)
+    v = vs * np.stack([vn * np.cos(t1), vr * np.cos(t2), vr * np.sin(t2)], axis=1)
+    Fnum = NDENS * dx / PPC
+    w = np.full(n, Fnum)
+    pia = np.zeros((N_CELLS, 7), dtype=np.int64)
+    c1 = np.arange(N_CELLS)
+    pia[:, 0] = PPC
+    pia[:, 1] = c1 * PPC + 1
+    pia[:, 2] = (c1 + 1) * PPC
+    pia[:, 3] = PPC
+    pia[:, 4] = 0
+    pia[:, 5] = -1
+    return dict(x=x, v=v, w=w, ids=np.arange(1, n + 1), pia=pia, n=n, Fnum=Fnum, mass=m)
+
+
+def load_pkg():
+    sys.path.insert(0, ROOT)
+    import __graft_entry__ as ge
+    return ge.load_package()
+
+
+def setup_store(pkg, case):
+    data = os.path.join(ROOT, "merzbild.jl_amd", "data")
+    sp = pkg.load_species_data(os.path.join(data, "particles.toml"), ["Ar"])
+    it = pkg.load_interaction_data(os.path.join(data, "vhs.toml"), sp)
+    st = pkg.DeviceStore(N_CELLS, 1, case["n"] + 4096)
+    st.upload_species(1, case["x"], case["v"], case["w"], case["ids"], case["pia"], case["n"], True,
+                      np.zeros(0, dtype=np.int64), case["n"])
+    grid = pkg.Grid1DUniform(L, N_CELLS)
+    st.set_grid(grid)
+    st.set_walls(pkg.MaxwellWalls1D(sp, T_WALL, T_WALL, -V_WALL, V_WALL, 1.0, 1.0))
+    st.set_interactions(it)
+    st.set_species_data(sp)
+    # create_collision_factors_array(pia, interactions, species_data, T, Fnum)
+    st.set_sigma_g_w_max(1, 1, pkg.estimate_sigma_g_w_max(it[0][0], sp[0].mass, sp[0].mass, T_WALL, T_WALL,
+                                                           case["Fnum"]))
+    st.set_seed(20000)
+    return st, sp, it
+
+
+def cpu_baseline(case, it, target_s=12.0):
+    """The C oracle (a restatement of the reference method, not the Julia
+    reference) on this host's cores with OpenMP, same workload, bounded to a
+    sample of steps.  Returns (particle-timesteps/s, threads, steps)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mzo
+    mzo.build()
+    n = case["n"]
+    pv = mzo.ParticleVector(n)
+    pia = mzo.PIA(N_CELLS, 1)
+    for i in range(1, n + 1):
+        pv.pop_to(i)
+    pv.x[:n] = case["x"]
+    pv.v[:n] = case["v"]
+    pv.w[:n] = case["w"]
+    for c in range(N_CELLS):
+        pia.set(c + 1, 1, *case["pia"][c])
+    pia.set_n_total(1, n)
+    grid = mzo.grid1d(L, N_CELLS)
+    walls = mzo.walls([case["mass"]], T_WALL, T_WALL, -V_WALL, V_WALL, 1.0, 1.0)
+    oit = mzo.interactions(["Ar"])[0][0]
+    cfs = (mzo.CF * N_CELLS)()
+    sg = mzo.lib().mzo_estimate_sigma_g_w_max(C.byref(oit), case["mass"], case["mass"], T_WALL, T_WALL,
+                                              case["Fnum"], 1.0)
+    for c in range(N_CELLS):
+        cfs[c].sigma_g_w_max = sg
+    V = np.full(N_CELLS, L / N_CELLS)
+    pp = mzo.Props(N_CELLS, 1)
+    rng = mzo.RNG(seed=20000)
+    threads = mzo.lib().mzo_omp_max_threads()
+    args = lambda k, s0: (rng.ref, C.byref(grid), C.byref(walls), C.byref(oit), cfs, pv.p, pia.p, DT,
+                          V.ctypes.data_as(C.POINTER(C.c_double)), case["mass"], pp.ref, k, s0, 1, threads, 1)
+    t0 = time.perf_counter()
+    mzo.lib().mzo_couette_steps(*args(5, 1))
+    t_probe = (time.perf_counter() - t0) / 5
+    steps = max(10, int(target_s / max(t_probe, 1e-6)))
+    t0 = time.perf_counter()
+    mzo.lib().mzo_couette_steps(*args(steps, 6))
+    el = time.perf_counter() - t0
+    return n * steps / el, threads, steps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--timing-steps", type=int, default=400)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of a hipGraph")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": a.gpus}))
+        sys.exit(2)
+    pkg = load_pkg()
+    case = sample_case()
+    st, sp, it = setup_store(pkg, case)
+    graph = not a.eager
+    n = case["n"]
+
+    # warmup (also captures the step graph)
+    pkg.couette_steps(st, a.warmup, 1, DT, props_every=1, graph=graph)
+    st.sync()
+    step0 = 1 + a.warmup
+    t0 = time.perf_counter()
+    st.timing(True)
+    pkg.couette_steps(st, a.steps, step0, DT, props_every=1, graph=graph)
+    st.sync()
+    t1 = time.perf_counter()
+    step_ms_ev, step_cnt = st.timing_get("step")
+    st.timing(False)
+    elapsed = t1 - t0
+    value = n * a.steps / elapsed
+
+    # per-kernel durations: the same steps launched kernel by kernel with HIP
+    # events on the library's stream around every launch
+    st.timing_reset()
+    st.timing(True)
+    s1 = step0 + a.steps
+    for t in range(a.timing_steps):
+        stp = s1 + t
+        pkg.ntc(st, 1, 1, DT, equal_weight=True, tag=(stp << 4) | 1)
+        pkg.convect_particles(st, 1, DT, tag=(stp << 4) | 2)
+        pkg.sort_particles(st, 1)
+        pkg.compute_props_sorted(st, sp, fetch=False)
+    kern = {}
+    for fam in ("ntc", "convect", "sort", "props"):
+        ms, cnt = st.timing_get(fam)
+        kern[fam] = ms
+    st.timing(False)
+    sc = st.scalars(1)
+    assert sc["err"] == 0 and sc["n_total"] == n, sc
+    dom = max(kern, key=kern.get)
+    dur_s = kern[dom] * 1e-3
+    achieved = ALG_BYTES[dom] * n / dur_s / 1e9
+    peak = 8000.0
+    roof = {"bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s", "frac": achieved / peak,
+            "traffic": None, "kernel": dom, "kernel_ms": kern[dom],
+            "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
+            "step_alg_GBps": ALG_BYTES["step"] * n / (step_ms_ev * 1e-3) / 1e9 if step_ms_ev else None}
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        v_cpu, threads, steps = cpu_baseline(case, it)
+        cpu = {"value": v_cpu, "unit": "particle-timesteps/s", "cores": threads, "kind": "port",
+               "sample": f"{steps} Couette steps x {n} particles (C oracle, OpenMP)"}
+    out = {
+        "metric": "particle-timesteps/s (sort+NTC+props)",
+        "value": value,
+        "unit": "particle-timesteps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": elapsed * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "1D Couette, argon VHS, 500 cells x 200 particles (100k), dt 2e-10 s, FP64; "
+                               "per step ntc!, convection, counting sort, compute_props_sorted!",
+                   "n_particles": n, "n_cells": N_CELLS, "mode": "hipGraph" if graph else "eager",
+                   "step_ms_events": step_ms_ev},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    st.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,159 @@
+/*
+ * mzhip.h -- C ABI of libmzhip.so, the gfx950 (MI355X) implementation of
+ * Merzbild.jl's per-timestep particle path.
+ *
+ * The reference path is a set of Julia functions over in-place-mutated
+ * structs (src/Merzbild.jl:25-71).  A Julia shim maps those signatures onto
+ * these entry points with ccall; Python ctypes drives them in the tests.
+ *
+ * Conventions
+ *   - every function returns 0 on success, <0 on error; mz_last_error()
+ *     returns a message for the calling thread;
+ *   - cells and species are 1-based and PIA values are the reference's
+ *     1-based positions, so tables round-trip unchanged;
+ *   - host buffers are owned by the caller and copied; device state is owned
+ *     by the store handle; one HIP stream per handle; calls on one handle are
+ *     not re-entrant;
+ *   - rng_mode 0 = Philox4x32-10 keyed by (seed, tag, stream, draw#),
+ *     rng_mode 1 = replay of a caller-supplied table [stream][draw#];
+ *     streams are cell-1 for NTC and merges, physical id-1 for convection.
+ */
+#ifndef MZHIP_H
+#define MZHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mz_store mz_store;
+
+/* ------------------------------------------------------------------ core */
+const char *mz_last_error(void);
+int mz_device_info(char *name, int name_len, int *n_cu, double *hbm_gbps_peak);
+
+/* A store of n_species ParticleVectors + one ParticleIndexerArray over
+ * n_cells cells: ParticleVector(np) / ParticleIndexerArray(n_cells,
+ * n_species) (src/particles.jl:138-174, 209-212).  capacity[s] bounds the
+ * logical positions of species s+1. */
+int mz_create(int32_t n_cells, int32_t n_species, const int64_t *capacity, mz_store **out);
+int mz_destroy(mz_store *h);
+int mz_sync(mz_store *h);
+
+/* Upload one species in the reference layout: x, v ([n_pos][3]) and w of
+ * the particles at logical positions 1..n_pos, their physical ids
+ * (pv.index[1:n_pos]), the PIA table pia[n_cells][7] =
+ * (n_local, start1, end1, n_group1, start2, end2, n_group2), n_total,
+ * contiguous, the free-id buffer pv.buffer[1:nbuffer] and length(pv).
+ * Replaces building ParticleVector / ParticleIndexerArray on the host
+ * (src/particles.jl:56-212). */
+int mz_upload_species(mz_store *h, int32_t species, int64_t n_pos, const double *x, const double *v,
+                      const double *w, const int64_t *ids, const int64_t *pia, int64_t n_total,
+                      int32_t contiguous, const int64_t *buffer, int64_t nbuffer, int64_t length);
+
+/* Set pv.cell[1:n] (the cell of each logical position), as a caller does
+ * before sort_particles!(gridsort, pv, pia, species)
+ * (src/grids/grid_sorting.jl:128-182). */
+int mz_upload_cell_keys(mz_store *h, int32_t species, int64_t n, const int64_t *cells);
+
+/* Scalars of a species: out[0..5] = n_total, contiguous, nbuffer, length,
+ * n_pos, error bits. */
+int mz_get_scalars(mz_store *h, int32_t species, int64_t *out);
+
+/* Download one species (arrays sized as in mz_upload_species; any pointer
+ * may be NULL).  cell_keys receives pv.cell as the last sort left it. */
+int mz_download_species(mz_store *h, int32_t species, int64_t n_pos_max, double *x, double *v,
+                        double *w, int64_t *ids, int64_t *cell_keys, int64_t *pia,
+                        int64_t *buffer, int64_t buffer_max);
+
+/* ------------------------------------------------------------ parameters */
+/* Grid1DUniform(L, nx; wall_offset) (src/grids/grid_uniform1D.jl:49-86) */
+int mz_set_grid_1d(mz_store *h, double L, int64_t nx, double wall_offset);
+/* MaxwellWalls1D (src/convection/boundary_conditions.jl:29-53): T[2],
+ * v[2][3], accommodation[2]; vsq[2][n_species] = 2 k_B T_wall / m. */
+int mz_set_walls_1d(mz_store *h, const double *T, const double *v, const double *acc, const double *vsq);
+/* Interaction[s1,s2] (src/collisions/collision_utils.jl:73-82): mu1, mu2,
+ * vhs_factor, vhs_o. */
+int mz_set_interaction(mz_store *h, int32_t s1, int32_t s2, double mu1, double mu2, double vhs_factor,
+                       double vhs_o);
+/* CollisionFactors[s1,s2,cell].sigma_g_w_max round trip
+ * (src/collisions/collision_ntc.jl:18-32). */
+int mz_set_sigma_g_w_max(mz_store *h, int32_t s1, int32_t s2, const double *sgwm);
+/* sgwm and the counters n1, n2, n_coll, n_coll_performed,
+ * n_eq_w_coll_performed of the last ntc call (any pointer may be NULL). */
+int mz_get_collision_factors(mz_store *h, int32_t s1, int32_t s2, double *sgwm, int64_t *n1, int64_t *n2,
+                             int64_t *n_coll, int64_t *n_performed, int64_t *n_eq_w);
+int mz_set_seed(mz_store *h, uint64_t seed);
+/* Replay tables (device copies): kind 0 = NTC, 1 = convection, 2 = merges. */
+int mz_set_replay(mz_store *h, int32_t kind, const double *table, int64_t rows, int64_t cols);
+
+/* ------------------------------------------------------------ operations */
+/* sort_particles!(gridsort, grid, pv, pia, species) when compute_cells = 1
+ * and sort_particles!(gridsort, pv, pia, species) with the stored cell keys
+ * when 0 (src/grids/grid_sorting.jl:58-113, 128-182).  Squashes first if the
+ * PIA is not contiguous. */
+int mz_sort(mz_store *h, int32_t species, int32_t compute_cells);
+
+/* squash_pia!(pv, pia, species) (src/particles.jl:622-701) */
+int mz_squash(mz_store *h, int32_t species);
+
+/* convect_particles!(rng, grid, walls, pv, pia, species, species_data, dt)
+ * (src/convection/convection_1D.jl:130-157). */
+int mz_convect(mz_store *h, int32_t species, double dt, uint64_t tag, int32_t rng_mode);
+
+/* ntc! / ntc_equal_weight! for cells cell_lo..cell_hi, single species when
+ * s1 == s2, two species otherwise (src/collisions/collision_ntc.jl:338-595).
+ * V[n_cells] are the cell volumes.  equal_weight selects ntc_equal_weight!.
+ * Cells are processed in parallel; particles created by splits are appended
+ * to group 2 in cell order, as a sequential loop over cells would. */
+int mz_ntc(mz_store *h, int32_t s1, int32_t s2, double dt, const double *V, double dw_tol,
+           int32_t equal_weight, uint64_t tag, int32_t rng_mode, int32_t cell_lo, int32_t cell_hi);
+
+/* compute_props! (variant 0, groups 1 and 2) or compute_props_sorted!
+ * (variant 1, group 1; ndens_not_Np with inv_V from the grid)
+ * (src/properties/physical_props.jl:104-154, 317-454).  Results stay on the
+ * device until mz_get_props. masses[n_species]. */
+int mz_compute_props(mz_store *h, int32_t variant, int32_t ndens_not_Np, const double *masses);
+/* PhysProps arrays [n_species][n_cells] (v: [n_species][n_cells][3]); any may be NULL. */
+int mz_get_props(mz_store *h, double *lpa, double *np, double *n, double *v, double *T);
+
+/* merge_octree_N2_based!(rng, octree, pv, pia, cell, species, target_np[, grid])
+ * for every cell whose n_local exceeds `threshold`
+ * (src/merging/merging_octree_N2.jl:1060-1094), then squash_pia! if
+ * squash != 0.  split: 1 mid, 2 mean; init_bounds: 1 min/max, 2 symmetric,
+ * 3 +-c; bounds_mode: 1 inherit, 2 recompute. n_merged may be NULL. */
+int mz_merge_octree(mz_store *h, int32_t species, int32_t threshold, int32_t target_np, int32_t split,
+                    int32_t init_bounds, int32_t bounds_mode, int32_t max_Nbins, int32_t max_depth,
+                    int32_t grid_clamp, int32_t squash, uint64_t tag, int32_t rng_mode, int64_t *n_merged);
+
+/* merge_nnls_based! for every cell whose n_local exceeds `threshold`
+ * (src/merging/merging_nnls.jl:1457-1510).  mim[n_mim][3] are the requested
+ * multi-indices (base moments are added as the reference does).
+ * status[n_cells] receives 1, -1, or 0 for cells not merged; NULL allowed. */
+int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim, int32_t n_mim,
+                  double vref, int32_t scaling_vref, int32_t centered_at_mean, const double *v_multipliers,
+                  int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
+                  int32_t squash, int32_t *status);
+
+/* ----------------------------------------------------------- time stepping */
+/* One 1-D Couette step repeated n_steps times: ntc! in every cell,
+ * convection, sort, compute_props_sorted! every props_every steps
+ * (test/test_1D_couette.jl:66-97).  Step t uses tags (t << 4) | 1 for NTC
+ * and (t << 4) | 2 for convection.  mode 0 launches kernels one by one,
+ * mode 1 replays a captured hipGraph of the step. */
+int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *V,
+                     int32_t equal_weight, int32_t props_every, const double *masses, int32_t mode);
+
+/* ------------------------------------------------------------------ timing */
+/* When enabled, the library brackets each launch of the named kernel family
+ * with hipEvents on its stream.  Families: "sort", "convect", "ntc", "props",
+ * "step" (a whole step).  Returns the mean duration in ms and the count. */
+int mz_timing_enable(mz_store *h, int32_t on);
+int mz_timing_get(mz_store *h, const char *family, double *mean_ms, int64_t *count);
+int mz_timing_reset(mz_store *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,475 @@
+"""merzbild.jl_amd -- host side of the MI355X particle path.
+
+Python mirror of the Merzbild.jl operator API for the per-timestep particle
+path (sort_particles!, convect_particles!, ntc!, compute_props!,
+compute_props_sorted!, merge_octree_N2_based!, merge_nnls_based!), backed by
+the gfx950 HIP library libmzhip.so through its C ABI (include/mzhip.h).  The
+Julia names drop the trailing "!"; every operation mutates the device store in
+place as the Julia functions mutate their arguments.
+
+There is no CPU fallback: loading fails loudly if libmzhip.so is absent.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmzhip.so")
+
+K_B = 1.380649e-23          # src/constants.jl:4
+TWOPI = 2 * np.pi           # src/constants.jl:29
+DELTA_PARTICLES = 256       # src/Merzbild.jl:23
+
+i32, i64, f64, u64 = C.c_int32, C.c_int64, C.c_double, C.c_uint64
+P = C.POINTER
+
+_SIGS = {
+    "mz_last_error": (C.c_char_p, []),
+    "mz_device_info": (C.c_int, [C.c_char_p, C.c_int, P(C.c_int), P(f64)]),
+    "mz_create": (C.c_int, [i32, i32, P(i64), P(C.c_void_p)]),
+    "mz_destroy": (C.c_int, [C.c_void_p]),
+    "mz_sync": (C.c_int, [C.c_void_p]),
+    "mz_upload_species": (C.c_int, [C.c_void_p, i32, i64, P(f64), P(f64), P(f64), P(i64), P(i64), i64, i32,
+                                    P(i64), i64, i64]),
+    "mz_upload_cell_keys": (C.c_int, [C.c_void_p, i32, i64, P(i64)]),
+    "mz_get_scalars": (C.c_int, [C.c_void_p, i32, P(i64)]),
+    "mz_download_species": (C.c_int, [C.c_void_p, i32, i64, P(f64), P(f64), P(f64), P(i64), P(i64), P(i64),
+                                      P(i64), i64]),
+    "mz_set_grid_1d": (C.c_int, [C.c_void_p, f64, i64, f64]),
+    "mz_set_walls_1d": (C.c_int, [C.c_void_p, P(f64), P(f64), P(f64), P(f64)]),
+    "mz_set_interaction": (C.c_int, [C.c_void_p, i32, i32, f64, f64, f64, f64]),
+    "mz_set_sigma_g_w_max": (C.c_int, [C.c_void_p, i32, i32, P(f64)]),
+    "mz_get_collision_factors": (C.c_int, [C.c_void_p, i32, i32, P(f64), P(i64), P(i64), P(i64), P(i64),
+                                           P(i64)]),
+    "mz_set_seed": (C.c_int, [C.c_void_p, u64]),
+    "mz_set_replay": (C.c_int, [C.c_void_p, i32, P(f64), i64, i64]),
+    "mz_sort": (C.c_int, [C.c_void_p, i32, i32]),
+    "mz_squash": (C.c_int, [C.c_void_p, i32]),
+    "mz_convect": (C.c_int, [C.c_void_p, i32, f64, u64, i32]),
+    "mz_ntc": (C.c_int, [C.c_void_p, i32, i32, f64, P(f64), f64, i32, u64, i32, i32, i32]),
+    "mz_compute_props": (C.c_int, [C.c_void_p, i32, i32, P(f64)]),
+    "mz_get_props": (C.c_int, [C.c_void_p, P(f64), P(f64), P(f64), P(f64), P(f64)]),
+    "mz_merge_octree": (C.c_int, [C.c_void_p, i32, i32, i32, i32, i32, i32, i32, i32, i32, i32, u64, i32,
+                                  P(i64)]),
+    "mz_merge_nnls": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, f64, i32, i32, P(f64), i32, f64, i32, f64,
+                                i32, P(i32)]),
+    "mz_couette_steps": (C.c_int, [C.c_void_p, i32, u64, f64, P(f64), i32, i32, P(f64), i32]),
+    "mz_timing_enable": (C.c_int, [C.c_void_p, i32]),
+    "mz_timing_get": (C.c_int, [C.c_void_p, C.c_char_p, P(f64), P(i64)]),
+    "mz_timing_reset": (C.c_int, [C.c_void_p]),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGS)
+
+_lib = None
+
+
+class MzError(RuntimeError):
+    pass
+
+
+def load_library(path=LIB_PATH):
+    """Load libmzhip.so; raises if it has not been built (no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise MzError(f"libmzhip.so not found at {path}; build it with merzbild.jl_amd/build.py")
+        L = C.CDLL(path)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise MzError(f"libmzhip error {rc}: {load_library().mz_last_error().decode()}")
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(P(f64))
+
+
+def _l(a):
+    return None if a is None else a.ctypes.data_as(P(i64))
+
+
+def device_info():
+    name = C.create_string_buffer(64)
+    ncu = C.c_int(0)
+    bw = f64(0)
+    _check(load_library().mz_device_info(name, 64, C.byref(ncu), C.byref(bw)))
+    return {"arch": name.value.decode(), "n_cu": ncu.value, "hbm_peak_gbps": bw.value}
+
+
+# ---------------------------------------------------------------------------
+# Setup data (src/particles.jl:580-608, src/collisions/collision_utils.jl)
+
+class Species:
+    """Species (src/particles.jl:30-35)."""
+
+    def __init__(self, name, mass, charge=0.0):
+        self.name, self.mass, self.charge = name, mass, charge
+
+
+def _parse_toml(path):
+    """Minimal reader for the flat [section] key = value files of data/*.toml."""
+    out, cur = {}, None
+    with open(path) as f:
+        for line in f:
+            line = line.split("#", 1)[0].strip()
+            if not line:
+                continue
+            if line.startswith("["):
+                cur = line.strip("[]").strip().strip('"')
+                out[cur] = {}
+            elif "=" in line and cur is not None:
+                k, v = (t.strip() for t in line.split("=", 1))
+                try:
+                    out[cur][k] = float(v)
+                except ValueError:
+                    out[cur][k] = v.strip('"')
+    return out
+
+
+def load_species_data(path, names):
+    """load_species_data (src/particles.jl:580-608)."""
+    if isinstance(names, str):
+        names = [names]
+    data = _parse_toml(path)
+    return [Species(n, data[n]["mass"], data[n].get("charge", 0.0)) for n in names]
+
+
+class Interaction:
+    """Interaction (src/collisions/collision_utils.jl:73-101, 218-223)."""
+
+    def __init__(self, m1, m2, d, omega, Tref):
+        from math import gamma, pi, sqrt
+        self.m_r = m1 * m2 / (m1 + m2)
+        self.mu1 = m1 / (m1 + m2)
+        self.mu2 = m2 / (m1 + m2)
+        self.vhs_d, self.vhs_o, self.vhs_Tref = d, omega, Tref
+        mref = 0.5 * (m1 + m2)
+        self.vhs_muref = 30.0 * sqrt(mref * K_B * Tref) / (4.0 * sqrt(pi) * (5.0 - 2.0 * omega) *
+                                                            (7.0 - 2.0 * omega) * d * d)
+        self.vhs_factor = pi * d ** 2 * (2 * K_B * Tref / self.m_r) ** (omega - 0.5) / gamma(2.5 - omega)
+
+    def sigma_vhs(self, g):
+        """sigma_vhs (src/collisions/collision_cross_sections.jl:179-181)"""
+        return self.vhs_factor * g ** (1.0 - 2 * self.vhs_o)
+
+
+def load_interaction_data(path, species_data):
+    """load_interaction_data (src/collisions/collision_utils.jl:159-202): [i][k]
+    holds mu1 = m_i/(m_i+m_k); KeyError when a pair is missing."""
+    data = _parse_toml(path)
+    n = len(species_data)
+    out = [[None] * n for _ in range(n)]
+    for i, a in enumerate(species_data):
+        for k, b in enumerate(species_data):
+            if i >= k:
+                key = f"{a.name},{b.name}"
+                if key not in data:
+                    key = f"{b.name},{a.name}"
+                e = data[key]
+                out[i][k] = Interaction(a.mass, b.mass, e["vhs_d"], e["vhs_o"], e["vhs_Tref"])
+                out[k][i] = Interaction(b.mass, a.mass, e["vhs_d"], e["vhs_o"], e["vhs_Tref"])
+    return out
+
+
+def estimate_sigma_g_w_max(interaction, m1, m2, T1, T2, Fnum, mult_factor=1.0):
+    """estimate_sigma_g_w_max (src/collisions/collision_utils.jl:418-424)"""
+    g1 = np.sqrt(2 * T1 * K_B / m1)
+    g2 = np.sqrt(2 * T2 * K_B / m2)
+    g = 0.5 * (g1 + g2)
+    return mult_factor * interaction.sigma_vhs(g) * g * Fnum
+
+
+class Grid1DUniform:
+    """Grid1DUniform(L, nx; wall_offset) (src/grids/grid_uniform1D.jl:49-86)."""
+
+    def __init__(self, L, nx, wall_offset=1e-12):
+        self.L = float(L)
+        self.n_cells = int(nx)
+        self.dx = self.L / nx
+        self.inv_dx = 1.0 / self.dx
+        self.wall_offset = wall_offset
+        self.min_x = self.dx * wall_offset
+        self.max_x = self.L - self.dx * wall_offset
+        self.V = np.full(nx, self.dx)
+
+
+class MaxwellWalls1D:
+    """MaxwellWalls1D (src/convection/boundary_conditions.jl:49-53)."""
+
+    def __init__(self, species_data, T_l, T_r, vy_l, vy_r, acc_l, acc_r):
+        self.T = np.array([T_l, T_r], dtype=np.float64)
+        self.v = np.array([[0.0, vy_l, 0.0], [0.0, vy_r, 0.0]], dtype=np.float64)
+        self.acc = np.array([acc_l, acc_r], dtype=np.float64)
+        self.vsq = np.array([[2 * K_B * T / s.mass for s in species_data] for T in (T_l, T_r)], dtype=np.float64)
+
+
+# ---------------------------------------------------------------------------
+# Device store
+
+PIA_FIELDS = ("n_local", "start1", "end1", "n_group1", "start2", "end2", "n_group2")
+
+
+class DeviceStore:
+    """n_species ParticleVectors and their ParticleIndexerArray on the GPU.
+
+    Slot p of a species holds the particle at logical position p+1 of the
+    reference's index array, with its physical id; see include/mzhip.h.
+    """
+
+    def __init__(self, n_cells, n_species, capacity):
+        lib = load_library()
+        cap = np.ascontiguousarray(np.broadcast_to(np.asarray(capacity, dtype=np.int64), (n_species,)))
+        h = C.c_void_p()
+        _check(lib.mz_create(n_cells, n_species, _l(cap), C.byref(h)))
+        self.h = h
+        self.n_cells, self.n_species = n_cells, n_species
+        self.capacity = cap
+        self.grid = None
+        self.masses = None
+
+    def close(self):
+        if self.h:
+            load_library().mz_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- data movement ---------------------------------------------------------
+    def upload_species(self, species, x, v, w, ids, pia, n_total, contiguous=True, buffer=None, length=None):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        ids = np.ascontiguousarray(ids, dtype=np.int64)
+        pia = np.ascontiguousarray(pia, dtype=np.int64).reshape(self.n_cells, 7)
+        buf = np.ascontiguousarray(buffer if buffer is not None else np.zeros(0), dtype=np.int64)
+        n_pos = len(w)
+        if length is None:
+            length = max(n_pos, int(ids.max()) if n_pos else 0) + len(buf)
+        _check(load_library().mz_upload_species(self.h, species, n_pos, _d(x), _d(v), _d(w), _l(ids), _l(pia),
+                                                int(n_total), 1 if contiguous else 0, _l(buf), len(buf),
+                                                int(length)))
+
+    def upload_cell_keys(self, species, cells):
+        """pv.cell[1:n] for the precomputed-cell sort."""
+        k = np.ascontiguousarray(cells, dtype=np.int64)
+        _check(load_library().mz_upload_cell_keys(self.h, species, len(k), _l(k)))
+
+    def scalars(self, species=1):
+        out = np.zeros(6, dtype=np.int64)
+        _check(load_library().mz_get_scalars(self.h, species, _l(out)))
+        return dict(zip(("n_total", "contiguous", "nbuffer", "length", "n_pos", "err"), out.tolist()))
+
+    def download_species(self, species=1):
+        sc = self.scalars(species)
+        n = sc["n_pos"]
+        x = np.zeros((max(n, 1), 3)); v = np.zeros((max(n, 1), 3)); w = np.zeros(max(n, 1))
+        ids = np.zeros(max(n, 1), dtype=np.int64); keys = np.zeros(max(n, 1), dtype=np.int64)
+        pia = np.zeros((self.n_cells, 7), dtype=np.int64)
+        buf = np.zeros(max(sc["nbuffer"], 1), dtype=np.int64)
+        _check(load_library().mz_download_species(self.h, species, n, _d(x), _d(v), _d(w), _l(ids), _l(keys),
+                                                  _l(pia), _l(buf), len(buf)))
+        return {"x": x[:n], "v": v[:n], "w": w[:n], "index": ids[:n], "cell": keys[:n], "pia": pia,
+                "buffer": buf[:sc["nbuffer"]], **sc}
+
+    def pia(self, species=1):
+        """The ParticleIndexerArray rows of a species: int64 [n_cells, 7] in
+        PIA_FIELDS order."""
+        return self.download_species(species)["pia"]
+
+    # -- parameters ------------------------------------------------------------
+    def set_grid(self, grid):
+        _check(load_library().mz_set_grid_1d(self.h, grid.L, grid.n_cells, grid.wall_offset))
+        self.grid = grid
+
+    def set_walls(self, walls):
+        _check(load_library().mz_set_walls_1d(self.h, _d(walls.T), _d(np.ascontiguousarray(walls.v.ravel())),
+                                              _d(walls.acc), _d(np.ascontiguousarray(walls.vsq.ravel()))))
+
+    def set_interactions(self, interactions):
+        for i in range(self.n_species):
+            for k in range(self.n_species):
+                it = interactions[i][k]
+                _check(load_library().mz_set_interaction(self.h, i + 1, k + 1, it.mu1, it.mu2, it.vhs_factor,
+                                                         it.vhs_o))
+
+    def set_species_data(self, species_data):
+        self.masses = np.array([s.mass for s in species_data], dtype=np.float64)
+
+    def set_sigma_g_w_max(self, s1, s2, values):
+        vals = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), (self.n_cells,)))
+        _check(load_library().mz_set_sigma_g_w_max(self.h, s1, s2, _d(vals)))
+
+    def collision_factors(self, s1, s2):
+        """CollisionFactors fields per cell for the pair (s1, s2)."""
+        nc = self.n_cells
+        sg = np.zeros(nc)
+        arrs = [np.zeros(nc, dtype=np.int64) for _ in range(5)]
+        _check(load_library().mz_get_collision_factors(self.h, s1, s2, _d(sg), *[_l(a) for a in arrs]))
+        return dict(sigma_g_w_max=sg, n1=arrs[0], n2=arrs[1], n_coll=arrs[2], n_coll_performed=arrs[3],
+                    n_eq_w_coll_performed=arrs[4])
+
+    def set_seed(self, seed):
+        _check(load_library().mz_set_seed(self.h, int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    def set_replay(self, kind, table):
+        t = np.ascontiguousarray(table, dtype=np.float64)
+        rows, cols = t.shape if t.size else (0, 0)
+        _check(load_library().mz_set_replay(self.h, {"ntc": 0, "convect": 1, "merge": 2}[kind], _d(t), rows, cols))
+
+    def sync(self):
+        _check(load_library().mz_sync(self.h))
+
+    # -- timing ----------------------------------------------------------------
+    def timing(self, on=True):
+        _check(load_library().mz_timing_enable(self.h, 1 if on else 0))
+
+    def timing_get(self, family):
+        ms, n = f64(0), i64(0)
+        _check(load_library().mz_timing_get(self.h, family.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def timing_reset(self):
+        _check(load_library().mz_timing_reset(self.h))
+
+
+# ---------------------------------------------------------------------------
+# Operations with the reference's names
+
+RNG_PHILOX, RNG_REPLAY = 0, 1
+
+
+def sort_particles(store, species, compute_cells=True):
+    """sort_particles!(gridsort, grid, pv, pia, species) (compute_cells) or
+    sort_particles!(gridsort, pv, pia, species) (src/grids/grid_sorting.jl:58-182)."""
+    _check(load_library().mz_sort(store.h, species, 1 if compute_cells else 0))
+
+
+def squash_pia(store, species):
+    """squash_pia!(pv, pia, species) (src/particles.jl:622-701)"""
+    _check(load_library().mz_squash(store.h, species))
+
+
+def convect_particles(store, species, dt, tag=0, rng_mode=RNG_PHILOX):
+    """convect_particles!(rng, grid, walls, pv, pia, species, species_data, dt)
+    (src/convection/convection_1D.jl:130-157)"""
+    _check(load_library().mz_convect(store.h, species, dt, tag, rng_mode))
+
+
+def ntc(store, s1, s2, dt, V=None, dw_tol=1e-16, equal_weight=False, tag=0, rng_mode=RNG_PHILOX, cells=None):
+    """ntc! / ntc_equal_weight! over cells (1-based inclusive range)
+    (src/collisions/collision_ntc.jl:338-595)."""
+    lo, hi = cells if cells is not None else (1, store.n_cells)
+    Vd = None if V is None else np.ascontiguousarray(np.broadcast_to(np.asarray(V, dtype=np.float64),
+                                                                      (store.n_cells,)))
+    _check(load_library().mz_ntc(store.h, s1, s2, dt, _d(Vd), dw_tol, 1 if equal_weight else 0, tag, rng_mode,
+                                 lo, hi))
+
+
+def ntc_equal_weight(store, s1, s2, dt, V=None, tag=0, rng_mode=RNG_PHILOX, cells=None):
+    return ntc(store, s1, s2, dt, V=V, equal_weight=True, tag=tag, rng_mode=rng_mode, cells=cells)
+
+
+class PhysProps:
+    """PhysProps (src/properties/physical_props.jl:24-90), host copies."""
+
+    def __init__(self, n_cells, n_species, ndens_not_Np=False):
+        self.n_cells, self.n_species, self.ndens_not_Np = n_cells, n_species, ndens_not_Np
+        self.lpa = np.zeros(n_species)
+        self.np = np.zeros((n_species, n_cells))
+        self.n = np.zeros((n_species, n_cells))
+        self.v = np.zeros((n_species, n_cells, 3))
+        self.T = np.zeros((n_species, n_cells))
+
+    def fetch(self, store):
+        _check(load_library().mz_get_props(store.h, _d(self.lpa), _d(self.np), _d(self.n), _d(self.v), _d(self.T)))
+        return self
+
+
+def _masses(store, species_data):
+    if species_data is not None:
+        return np.array([s.mass for s in species_data], dtype=np.float64)
+    if store.masses is None:
+        raise MzError("species masses unknown: pass species_data or call set_species_data")
+    return store.masses
+
+
+def compute_props(store, species_data=None, phys_props=None):
+    """compute_props!(particles, pia, species_data, phys_props)
+    (src/properties/physical_props.jl:104-154)"""
+    m = _masses(store, species_data)
+    _check(load_library().mz_compute_props(store.h, 0, 0, _d(m)))
+    pp = phys_props or PhysProps(store.n_cells, store.n_species)
+    return pp.fetch(store)
+
+
+def compute_props_sorted(store, species_data=None, phys_props=None, ndens_not_Np=False, fetch=True):
+    """compute_props_sorted!(particles, pia, species_data, phys_props[, grid])
+    (src/properties/physical_props.jl:317-454)"""
+    m = _masses(store, species_data)
+    _check(load_library().mz_compute_props(store.h, 1, 1 if ndens_not_Np else 0, _d(m)))
+    if not fetch:
+        return None
+    pp = phys_props or PhysProps(store.n_cells, store.n_species, ndens_not_Np)
+    return pp.fetch(store)
+
+
+OCTREE_MID, OCTREE_MEAN = 1, 2
+OCTREE_INIT_MINMAX, OCTREE_INIT_MINMAX_SYM, OCTREE_INIT_C = 1, 2, 3
+OCTREE_BOUNDS_INHERIT, OCTREE_BOUNDS_RECOMPUTE = 1, 2
+
+
+def merge_octree_N2_based(store, species, threshold, target_np, split=OCTREE_MID,
+                          init_bin_bounds=OCTREE_INIT_MINMAX, bin_bounds_compute=OCTREE_BOUNDS_INHERIT,
+                          max_Nbins=4096, max_depth=10, grid_clamp=False, squash=True, tag=0,
+                          rng_mode=RNG_PHILOX):
+    """merge_octree_N2_based! on every cell with n_local > threshold
+    (src/merging/merging_octree_N2.jl:1060-1094); returns the merged-cell count."""
+    n = i64(0)
+    _check(load_library().mz_merge_octree(store.h, species, threshold, target_np, split, init_bin_bounds,
+                                          bin_bounds_compute, max_Nbins, max_depth, 1 if grid_clamp else 0,
+                                          1 if squash else 0, tag, rng_mode, C.byref(n)))
+    return n.value
+
+
+def merge_nnls_based(store, species, threshold, multi_index_moments, vref=1.0, scaling="variance",
+                     centered_at_mean=True, v_multipliers=(0.5, 1.0), max_err=1e-11, iteration_mult=2,
+                     w_threshold=0.0, squash=True):
+    """merge_nnls_based! on every cell with n_local > threshold
+    (src/merging/merging_nnls.jl:1457-1510); returns the per-cell status
+    (1 merged, -1 failed, 0 not attempted)."""
+    mim = np.ascontiguousarray(np.asarray(multi_index_moments, dtype=np.int32).reshape(-1, 3))
+    vm = np.ascontiguousarray(v_multipliers, dtype=np.float64)
+    status = np.zeros(store.n_cells, dtype=np.int32)
+    _check(load_library().mz_merge_nnls(store.h, species, threshold, mim.ctypes.data_as(P(i32)), len(mim), vref,
+                                        1 if scaling == "vref" else 0, 1 if centered_at_mean else 0, _d(vm),
+                                        len(vm), max_err, iteration_mult, w_threshold, 1 if squash else 0,
+                                        status.ctypes.data_as(P(i32))))
+    return status
+
+
+def couette_steps(store, n_steps, step0, dt, species_data=None, V=None, equal_weight=True, props_every=1,
+                  graph=True):
+    """n_steps of the 1-D Couette loop (test/test_1D_couette.jl:66-97): ntc! in
+    every cell, convect_particles!, sort_particles!, compute_props_sorted!."""
+    m = _masses(store, species_data)
+    Vd = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
+    _check(load_library().mz_couette_steps(store.h, n_steps, step0, dt, _d(Vd), 1 if equal_weight else 0,
+                                           props_every, _d(m), 1 if graph else 0))
+
+
+def multi_index_moments(n):
+    """compute_multi_index_moments(n) (src/merging/merging_nnls.jl:341-352)"""
+    return [(i, j, n - i - j) for i in range(n + 1) for j in range(n - i + 1) if n - i - j >= 0]

@@ -1,0 +1,98 @@
+// mz_internal.h -- host-side store handle and internal launch helpers.
+#pragma once
+
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/mzhip.h"
+#include "mz_common.h"
+
+#define MZ_MAX_SPECIES 4
+#define MZ_SORT_CHUNK 1024
+#define MZ_SORT_MAX_CELLS 16384
+#define MZ_MAX_BOUNCES 64
+#define MZ_NTC_MAX_CANDIDATES 100000000  // per cell and call; larger counts flag an error
+#define MZ_KB 1.380649e-23  // src/constants.jl:4
+
+struct MzGrid {
+    double L, dx, inv_dx, min_x, max_x;
+    int32_t nx;
+};
+
+struct MzWalls {
+    double T[2];
+    double v[2][3];
+    double acc[2];
+    double vsq[2][MZ_MAX_SPECIES];
+};
+
+struct MzInteraction {
+    double mu1, mu2, factor, expo;
+    int32_t set;
+};
+
+// All species views by value (kernel arguments, no device-side table).
+struct MzViews {
+    MzSpView v[MZ_MAX_SPECIES];
+};
+
+struct MzEventPair {
+    hipEvent_t a, b;
+    int64_t n = 1;
+};
+
+struct MzScratch {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct mz_store {
+    int n_cells = 0, n_species = 0;
+    std::vector<MzSpecies> sp;
+    hipStream_t stream = nullptr;
+    MzGrid grid{};
+    bool has_grid = false;
+    MzWalls walls{};
+    bool has_walls = false;
+    std::vector<MzInteraction> inter;
+    double *d_sgwm = nullptr;    // [s1][s2][cell]
+    int32_t *d_cfcnt = nullptr;  // [s1][s2][cell][5]: n1, n2, n_coll, performed, eq_w
+    uint64_t seed = 0;
+    double *d_replay[3] = {nullptr, nullptr, nullptr};
+    int64_t replay_rows[3] = {0, 0, 0}, replay_cols[3] = {0, 0, 0};
+    int32_t *d_err = nullptr;
+    double *d_np = nullptr, *d_n = nullptr, *d_v = nullptr, *d_T = nullptr;
+    double *d_V = nullptr, *d_invV = nullptr;
+    double *d_masses = nullptr;
+    std::map<std::string, MzScratch> scratch;
+    // timing
+    bool timing = false;
+    std::vector<std::pair<std::string, MzEventPair>> open_events;
+    std::map<std::string, std::vector<MzEventPair>> pending;
+    std::map<std::string, std::pair<double, int64_t>> timing_acc;
+    // captured step graphs
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::string graph_key;
+    uint64_t *d_step = nullptr;
+};
+
+MzViews mz_views(mz_store *h);
+void *mz_scratch(mz_store *h, const char *name, size_t bytes);
+int mz_read_counters(mz_store *h, int species, MzCounters *c);
+MzRngParams mz_rng_params(mz_store *h, int kind, uint64_t tag, int mode);
+int mz_timing_begin(mz_store *h, const char *family);
+int mz_timing_end(mz_store *h);
+void mz_swap_buffers(MzSpecies &s);
+void mz_graph_release(mz_store *h);
+
+int mz_launch_sort(mz_store *h, int species, int compute_cells);
+int mz_launch_convect(mz_store *h, int species, double dt, uint64_t tag, int rng_mode, bool contiguous,
+                      int32_t n_pos, const uint64_t *step_ptr = nullptr);
+int mz_launch_props(mz_store *h, int variant, int ndens);
+int mz_launch_ntc_equal(mz_store *h, int s1, int s2, double dt, const double *dV, double dw_tol,
+                        int equal_weight, uint64_t tag, int rng_mode, int cell_lo, int cell_hi,
+                        const uint64_t *step_ptr = nullptr);
+int mz_timing_end_n(mz_store *h, int64_t n);

@@ -1,0 +1,424 @@
+// mz_ntc.hip -- NTC pair selection with VHS collisions on gfx950.
+//
+// Reference (Merzbild.jl, src/collisions/):
+//   compute_n_coll_*                collision_ntc.jl:173-197
+//   collide_2particles_vhs!         collision_ntc.jl:223-270
+//   collide_2particles_vhs_equal_weight!  collision_ntc.jl:294-309
+//   ntc! / ntc_equal_weight!        collision_ntc.jl:338-595
+//   compute_com!, compute_g!        collision_utils.jl:374-391
+//   scatter_vhs!                    collision_scattering.jl:17-29
+//   sigma_vhs                       collision_cross_sections.jl:179-181
+//
+// The candidate loop of one cell is sequential in the reference: the running
+// (sigma g w)_max is updated before every acceptance test, and particles
+// created by a split become candidates of the same loop.  Each cell is
+// therefore one lane; cells run in parallel.  Draw order per cell stream:
+// [n_coll], then per candidate [i, k (, k redraws)], [accept] if g > eps,
+// [phi, cos theta] if accepted.
+//
+// Particle creation (weight splitting) follows the reference's append at the
+// global end: created particles are first written to per-cell scratch (the
+// loop sees them as group-2 members), then appended in cell order, which is
+// the order a sequential loop over cells produces; ids come from the emulated
+// free-id LIFO, growing `length` by DELTA_PARTICLES when it runs dry, as
+// resize! does (src/particles.jl:269-298, 311-332, 426-433).
+#include "mz_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+struct NtcArgs {
+    MzSpView a, b;       // species s1, s2 (same view when s1 == s2)
+    int32_t single;      // s1 == s2
+    int32_t equal_weight;
+    int32_t may_split;
+    int32_t cell_lo, cell_hi;  // 0-based inclusive
+    double dt, dw_tol;
+    double mu1, mu2, factor, expo;
+    const double *V;
+    double *sgwm;        // [n_cells] of this pair
+    int32_t *cnt;        // [n_cells][5]
+    MzRngParams rp;
+    // split path
+    const int32_t *ncoll_in;   // [n_cells] from the count pass (NULL: draw here)
+    const int64_t *soff;       // [n_cells] scratch offsets
+    double *sc[7];             // scratch SoA: x0 x1 x2 v0 v1 v2 w, per species half
+    int64_t sc_cap;            // entries per species half
+    int32_t *created;          // [2][n_cells]
+};
+
+__device__ inline int64_t slot_of(const MzSpView &sp, int c, int32_t li, int32_t n1, int32_t n2) {
+    // map_cont_index (src/particles.jl:364-366) for positions present before the call
+    return li < n1 ? (int64_t)sp.pia.start1[c] - 1 + li : (int64_t)sp.pia.start2[c] - 1 + (li - n1);
+}
+
+// Particle handle: either a slot of the store or a scratch entry.
+struct PRef {
+    double *x0, *x1, *x2, *v0, *v1, *v2, *w;
+    int64_t i;
+};
+
+__device__ inline PRef pref_slot(const MzSpView &sp, int64_t p) {
+    return PRef{sp.x0, sp.x1, sp.x2, sp.v0, sp.v1, sp.v2, sp.w, p};
+}
+
+__device__ inline PRef pref_scratch(const NtcArgs &A, int half, int64_t e) {
+    int64_t o = (int64_t)half * A.sc_cap;
+    return PRef{A.sc[0] + o, A.sc[1] + o, A.sc[2] + o, A.sc[3] + o, A.sc[4] + o, A.sc[5] + o, A.sc[6] + o, e};
+}
+
+__global__ void k_ntc(NtcArgs A) {
+    const int c = A.cell_lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > A.cell_hi) return;
+    A.rp = mz_rng_resolve(A.rp);
+    const int32_t n1a = A.a.pia.n1[c], n2a = A.a.pia.n2[c];
+    const int32_t n1b = A.b.pia.n1[c], n2b = A.b.pia.n2[c];
+    const int32_t nla = A.a.pia.nloc[c], nlb = A.b.pia.nloc[c];
+    double sgwm = A.sgwm[c];
+    MzStream st = mz_stream((uint64_t)c);
+    int64_t n_coll;
+    if (A.ncoll_in) {
+        n_coll = A.ncoll_in[c];
+        st.k = 1;
+    } else {
+        double R = mz_rand(A.rp, st);
+        double ncf = A.single ? 0.5 * A.dt * (double)nla * (double)(nla - 1) * sgwm / A.V[c] + R
+                              : A.dt * (double)nla * (double)nlb * sgwm / A.V[c] + R;
+        n_coll = (int64_t)floor(ncf);
+    }
+    if (!(n_coll <= MZ_NTC_MAX_CANDIDATES)) {   // also catches NaN estimates
+        atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH);
+        n_coll = 0;
+    }
+    int32_t na = nla, nb = nlb;     // current n_local (grows with splits)
+    int32_t ca = 0, cb = 0;         // created per species
+    int32_t performed = 0, eqw = 0;
+    const int64_t s_base = A.may_split ? A.soff[c] : 0;
+    for (int64_t cc = 0; cc < n_coll; ++cc) {
+        int32_t i = (int32_t)floor(mz_rand(A.rp, st) * (double)na);
+        int32_t k = (int32_t)floor(mz_rand(A.rp, st) * (double)(A.single ? na : nb));
+        if (A.single) {
+            while (i == k) {
+                k = (int32_t)floor(mz_rand(A.rp, st) * (double)na);
+                if (A.rp.mode == 1 && (int64_t)st.k > A.rp.cols) return;   // replay row exhausted
+            }
+        }
+        // resolve both candidates: existing positions or this call's scratch entries
+        PRef pi = i < nla ? pref_slot(A.a, slot_of(A.a, c, i, n1a, n2a)) : pref_scratch(A, 0, s_base + (i - nla));
+        PRef pk;
+        if (A.single) {
+            pk = k < nla ? pref_slot(A.a, slot_of(A.a, c, k, n1a, n2a)) : pref_scratch(A, 0, s_base + (k - nla));
+        } else {
+            pk = k < nlb ? pref_slot(A.b, slot_of(A.b, c, k, n1b, n2b)) : pref_scratch(A, 1, s_base + (k - nlb));
+        }
+        double vi0 = pi.v0[pi.i], vi1 = pi.v1[pi.i], vi2 = pi.v2[pi.i];
+        double vk0 = pk.v0[pk.i], vk1 = pk.v1[pk.i], vk2 = pk.v2[pk.i];
+        double g0 = vi0 - vk0, g1 = vi1 - vk1, g2 = vi2 - vk2;
+        double g = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+        if (!(g > 2.220446049250313e-16)) continue;
+        double wi = pi.w[pi.i], wk = pk.w[pk.i];
+        double sigma = A.factor * pow(g, A.expo);
+        double sgw = sigma * g * (wi > wk ? wi : wk);
+        sgwm = sgw > sgwm ? sgw : sgwm;
+        if (!(mz_rand(A.rp, st) < sgw / sgwm)) continue;
+        ++performed;
+        double vc0 = fma(A.mu1, vi0, A.mu2 * vk0);
+        double vc1 = fma(A.mu1, vi1, A.mu2 * vk1);
+        double vc2 = fma(A.mu1, vi2, A.mu2 * vk2);
+        if (A.equal_weight || fabs(wi - wk) < A.dw_tol) {
+            ++eqw;
+        } else if (!A.may_split) {
+            atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH);
+        } else if (wi > wk) {
+            // split i: the new particle keeps i's pre-collision v, x and weight wi - wk
+            int32_t q = A.single ? ca + cb : ca;
+            if (q >= n_coll) { atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH); continue; }
+            PRef nw = pref_scratch(A, 0, s_base + q);
+            nw.x0[nw.i] = pi.x0[pi.i]; nw.x1[nw.i] = pi.x1[pi.i]; nw.x2[nw.i] = pi.x2[pi.i];
+            nw.v0[nw.i] = vi0; nw.v1[nw.i] = vi1; nw.v2[nw.i] = vi2;
+            nw.w[nw.i] = wi - wk;
+            pi.w[pi.i] = wk;
+            ++ca; ++na;
+        } else {
+            if (A.single) {
+                int32_t q = ca + cb;
+                if (q >= n_coll) { atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH); continue; }
+                PRef nw = pref_scratch(A, 0, s_base + q);
+                nw.x0[nw.i] = pk.x0[pk.i]; nw.x1[nw.i] = pk.x1[pk.i]; nw.x2[nw.i] = pk.x2[pk.i];
+                nw.v0[nw.i] = vk0; nw.v1[nw.i] = vk1; nw.v2[nw.i] = vk2;
+                nw.w[nw.i] = wk - wi;
+                pk.w[pk.i] = wi;
+                ++ca; ++na;
+            } else {
+                if (cb >= n_coll) { atomicOr(&A.b.ctr->err, MZ_ERR_SCRATCH); continue; }
+                PRef nw = pref_scratch(A, 1, s_base + cb);
+                nw.x0[nw.i] = pk.x0[pk.i]; nw.x1[nw.i] = pk.x1[pk.i]; nw.x2[nw.i] = pk.x2[pk.i];
+                nw.v0[nw.i] = vk0; nw.v1[nw.i] = vk1; nw.v2[nw.i] = vk2;
+                nw.w[nw.i] = wk - wi;
+                pk.w[pk.i] = wi;
+                ++cb; ++nb;
+            }
+        }
+        // scatter_vhs!: phi first, then cos(theta)
+        double phi = 6.283185307179586 * mz_rand(A.rp, st);
+        double sphi, cphi;
+        sincos(phi, &sphi, &cphi);
+        double ct = 2.0 * mz_rand(A.rp, st) - 1.0;
+        double stt = sqrt(1.0 - ct * ct);
+        double gn0 = g * (stt * cphi), gn1 = g * (stt * sphi), gn2 = g * ct;
+        pi.v0[pi.i] = fma(A.mu2, gn0, vc0);
+        pi.v1[pi.i] = fma(A.mu2, gn1, vc1);
+        pi.v2[pi.i] = fma(A.mu2, gn2, vc2);
+        pk.v0[pk.i] = fma(-A.mu1, gn0, vc0);
+        pk.v1[pk.i] = fma(-A.mu1, gn1, vc1);
+        pk.v2[pk.i] = fma(-A.mu1, gn2, vc2);
+    }
+    A.sgwm[c] = sgwm;
+    int32_t *cn = A.cnt + (int64_t)c * 5;
+    cn[0] = nla;
+    cn[1] = A.single ? nla : nlb;
+    cn[2] = (int32_t)n_coll;
+    cn[3] = performed;
+    cn[4] = eqw;
+    if (A.may_split) {
+        A.created[c] = A.single ? ca + cb : ca;
+        A.created[(int64_t)(A.cell_hi + 1) + c] = A.single ? 0 : cb;
+    }
+}
+
+// Count pass of the split path: n_coll of every cell (draw 0 of its stream).
+__global__ void k_ntc_count(NtcArgs A, int32_t *__restrict__ ncoll) {
+    const int c = A.cell_lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > A.cell_hi) return;
+    const int32_t nla = A.a.pia.nloc[c], nlb = A.b.pia.nloc[c];
+    A.rp = mz_rng_resolve(A.rp);
+    MzStream st = mz_stream((uint64_t)c);
+    double R = mz_rand(A.rp, st);
+    double sgwm = A.sgwm[c];
+    double ncf = A.single ? 0.5 * A.dt * (double)nla * (double)(nla - 1) * sgwm / A.V[c] + R
+                          : A.dt * (double)nla * (double)nlb * sgwm / A.V[c] + R;
+    double f = floor(ncf);
+    if (!(f <= (double)MZ_NTC_MAX_CANDIDATES)) {
+        atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH);
+        f = 0.0;
+    }
+    ncoll[c] = (int32_t)f;
+}
+
+// Exclusive scan of n int32 values into int64 offsets (one 1024-thread block);
+// total goes to *total.
+__global__ void __launch_bounds__(1024) k_scan_i32(const int32_t *__restrict__ in, int64_t n,
+                                                   int64_t *__restrict__ out, int64_t *__restrict__ total) {
+    __shared__ int64_t wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t lo = t * per, hi = min(n, lo + per);
+    int64_t s = 0;
+    for (int64_t i = lo; i < hi; ++i) s += in[i];
+    int64_t x = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    if (t < 16) {
+        int64_t a = wsum[t], b = a;
+        for (int o = 1; o < 16; o <<= 1) {
+            int64_t y = __shfl_up(b, o, 16);
+            if ((t & 15) >= o) b += y;
+        }
+        wsum[t] = b - a;
+    }
+    __syncthreads();
+    int64_t run = x - s + wsum[wv];
+    for (int64_t i = lo; i < hi; ++i) {
+        int64_t v = in[i];
+        out[i] = run;
+        run += v;
+    }
+    if (t == 1023) *total = run;
+}
+
+// Append pass: created particles of cell c go to positions
+// n_total_old + coff[c] + j (1-based +1), ids from the LIFO in creation order.
+__global__ void k_ntc_append(MzSpView sp, int32_t cell_lo, int32_t cell_hi, const int32_t *__restrict__ created,
+                             const int64_t *__restrict__ coff, const int64_t *__restrict__ ctotal,
+                             const int64_t *__restrict__ soff, const double *const *sc, int64_t sc_off,
+                             MzCounters snap) {
+    const int c = cell_lo + blockIdx.x;
+    if (c > cell_hi) return;
+    const int32_t k = created[c];
+    if (k == 0) return;
+    const int64_t base = (int64_t)snap.n_total + coff[c];  // 0-based first new slot
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+        int64_t dst = base + j;
+        if (dst >= sp.cap) {
+            atomicOr(&sp.ctr->err, MZ_ERR_CAPACITY);
+            continue;
+        }
+        int64_t src = sc_off + soff[c] + j;
+        sp.x0[dst] = sc[0][src]; sp.x1[dst] = sc[1][src]; sp.x2[dst] = sc[2][src];
+        sp.v0[dst] = sc[3][src]; sp.v1[dst] = sc[4][src]; sp.v2[dst] = sc[5][src];
+        sp.w[dst] = sc[6][src];
+        int64_t q = coff[c] + j;   // global creation number
+        int32_t idv = q < snap.nbuffer ? sp.buffer[snap.nbuffer - 1 - q]
+                                       : (int32_t)(snap.length + 1 + (q - snap.nbuffer));
+        sp.id[dst] = idv;
+        sp.key[dst] = c + 1;
+    }
+    if (threadIdx.x == 0) {
+        // update_particle_indexer_new_particle! applied k times
+        sp.pia.nloc[c] += k;
+        sp.pia.n2[c] += k;
+        if (sp.pia.start2[c] <= 0) sp.pia.start2[c] = (int32_t)(base + 1);
+        sp.pia.end2[c] = (int32_t)(base + k);
+    }
+    (void)ctotal;
+}
+
+// Counters after the append: n_total, n_pos, and the LIFO / length as a run of
+// resize! + pops leaves them.
+__global__ void k_ntc_finish(MzSpView sp, const int64_t *__restrict__ ctotal, MzCounters snap) {
+    const int64_t tot = *ctotal;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        sp.ctr->created = (int32_t)tot;
+        if (tot == 0) return;
+        int64_t nt = (int64_t)snap.n_total + tot;
+        if (nt > sp.cap) {
+            atomicOr(&sp.ctr->err, MZ_ERR_CAPACITY);
+            nt = sp.cap;
+        }
+        sp.ctr->n_total = (int32_t)nt;
+        sp.ctr->n_pos = max(snap.n_pos, sp.ctr->n_total);
+        if (tot <= snap.nbuffer) {
+            sp.ctr->nbuffer = snap.nbuffer - (int32_t)tot;
+        } else {
+            int64_t R = tot - snap.nbuffer;
+            int64_t chunks = (R + MZ_DELTA_PARTICLES - 1) / MZ_DELTA_PARTICLES;
+            int64_t L2 = snap.length + chunks * MZ_DELTA_PARTICLES;
+            sp.ctr->length = (int32_t)L2;
+            sp.ctr->nbuffer = (int32_t)(L2 - snap.length - R);
+        }
+    }
+    // the remaining free ids after a resize: buffer[0..nbuffer) = L', L'-1, ...
+    if (tot > snap.nbuffer) {
+        int64_t R = tot - snap.nbuffer;
+        int64_t chunks = (R + MZ_DELTA_PARTICLES - 1) / MZ_DELTA_PARTICLES;
+        int64_t L2 = snap.length + chunks * MZ_DELTA_PARTICLES;
+        int64_t nbuf = L2 - snap.length - R;
+        if (nbuf > sp.bufcap) {
+            if (threadIdx.x == 0) atomicOr(&sp.ctr->err, MZ_ERR_BUFFER);
+            return;
+        }
+        for (int64_t i = threadIdx.x; i < nbuf; i += blockDim.x) sp.buffer[i] = (int32_t)(L2 - i);
+    }
+}
+
+static NtcArgs base_args(mz_store *h, int s1, int s2, double dt, const double *dV, double dw_tol, int equal_weight,
+                         uint64_t tag, int rng_mode, int cell_lo, int cell_hi) {
+    NtcArgs A{};
+    A.a = mz_view(h->sp[s1 - 1]);
+    A.b = mz_view(h->sp[s2 - 1]);
+    A.single = s1 == s2;
+    A.equal_weight = equal_weight;
+    A.cell_lo = cell_lo - 1;
+    A.cell_hi = cell_hi - 1;
+    A.dt = dt;
+    A.dw_tol = dw_tol;
+    const MzInteraction &it = h->inter[(s1 - 1) * h->n_species + (s2 - 1)];
+    A.mu1 = it.mu1; A.mu2 = it.mu2; A.factor = it.factor; A.expo = it.expo;
+    A.V = dV;
+    size_t pair = (size_t)(s1 - 1) * h->n_species + (s2 - 1);
+    A.sgwm = h->d_sgwm + pair * h->n_cells;
+    A.cnt = h->d_cfcnt + pair * h->n_cells * 5;
+    A.rp = mz_rng_params(h, 0, tag, rng_mode);
+    return A;
+}
+
+int mz_launch_ntc_equal(mz_store *h, int s1, int s2, double dt, const double *dV, double dw_tol, int equal_weight,
+                        uint64_t tag, int rng_mode, int cell_lo, int cell_hi, const uint64_t *step_ptr) {
+    NtcArgs A = base_args(h, s1, s2, dt, dV, dw_tol, equal_weight, tag, rng_mode, cell_lo, cell_hi);
+    A.may_split = 0;
+    A.rp.step_ptr = step_ptr;
+    A.rp.phase = 1;
+    int ncells = cell_hi - cell_lo + 1;
+    hipLaunchKernelGGL(k_ntc, dim3((ncells + 63) / 64), dim3(64), 0, h->stream, A);
+    MZ_HIP(hipGetLastError());
+    return 0;
+}
+
+static int launch_ntc_split(mz_store *h, int s1, int s2, double dt, const double *dV, double dw_tol, uint64_t tag,
+                            int rng_mode, int cell_lo, int cell_hi) {
+    NtcArgs A = base_args(h, s1, s2, dt, dV, dw_tol, 0, tag, rng_mode, cell_lo, cell_hi);
+    A.may_split = 1;
+    const int nc = h->n_cells;
+    const int ncells = cell_hi - cell_lo + 1;
+    int32_t *ncoll = (int32_t *)mz_scratch(h, "ntc_ncoll", (size_t)nc * sizeof(int32_t));
+    int64_t *soff = (int64_t *)mz_scratch(h, "ntc_soff", (size_t)(nc + 1) * sizeof(int64_t));
+    int32_t *created = (int32_t *)mz_scratch(h, "ntc_created", (size_t)2 * nc * sizeof(int32_t));
+    int64_t *coff = (int64_t *)mz_scratch(h, "ntc_coff", (size_t)2 * (nc + 1) * sizeof(int64_t));
+    MZ_CHECK(ncoll && soff && created && coff, "ntc: scratch allocation failed");
+    MZ_HIP(hipMemsetAsync(ncoll, 0, (size_t)nc * sizeof(int32_t), h->stream));
+    MZ_HIP(hipMemsetAsync(created, 0, (size_t)2 * nc * sizeof(int32_t), h->stream));
+    hipLaunchKernelGGL(k_ntc_count, dim3((ncells + 255) / 256), dim3(256), 0, h->stream, A, ncoll);
+    hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, h->stream, ncoll, (int64_t)nc, soff, soff + nc);
+    int64_t total = 0;
+    MZ_HIP(hipMemcpyAsync(&total, soff + nc, sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    int64_t cap = std::max<int64_t>(total, 1);
+    double *sc = (double *)mz_scratch(h, "ntc_sc", (size_t)7 * 2 * cap * sizeof(double));
+    MZ_CHECK(sc != nullptr, "ntc: creation scratch allocation failed");
+    for (int f = 0; f < 7; ++f) A.sc[f] = sc + (size_t)f * 2 * cap;
+    A.sc_cap = cap;
+    A.ncoll_in = ncoll;
+    A.soff = soff;
+    // created is indexed [species half][cell]; the kernel writes half 1 at cell_hi + 1 + c
+    A.created = created;
+    hipLaunchKernelGGL(k_ntc, dim3((ncells + 63) / 64), dim3(64), 0, h->stream, A);
+    MZ_HIP(hipGetLastError());
+    // append per species half
+    double **scp = (double **)mz_scratch(h, "ntc_scp", 7 * sizeof(double *));
+    MZ_CHECK(scp != nullptr, "ntc: scratch allocation failed");
+    MZ_HIP(hipMemcpy(scp, A.sc, 7 * sizeof(double *), hipMemcpyHostToDevice));
+    for (int half = 0; half < (A.single ? 1 : 2); ++half) {
+        int sidx = half == 0 ? s1 : s2;
+        const int32_t *cr = created + (half == 0 ? 0 : (int64_t)(A.cell_hi + 1));
+        int64_t *co = coff + (size_t)half * (nc + 1);
+        MzCounters snap;
+        int r = mz_read_counters(h, sidx, &snap);
+        if (r) return r;
+        // created[] for half 1 is offset by cell_hi + 1 entries; scan the used range
+        hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, h->stream, cr, (int64_t)(A.cell_hi + 1), co,
+                           co + nc);
+        hipLaunchKernelGGL(k_ntc_append, dim3(ncells), dim3(64), 0, h->stream, mz_view(h->sp[sidx - 1]), A.cell_lo,
+                           A.cell_hi, cr, co, co + nc, soff, (const double *const *)scp, (int64_t)half * cap, snap);
+        hipLaunchKernelGGL(k_ntc_finish, dim3(1), dim3(256), 0, h->stream, mz_view(h->sp[sidx - 1]), co + nc, snap);
+        MZ_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+extern "C" int mz_ntc(mz_store *h, int32_t s1, int32_t s2, double dt, const double *V, double dw_tol,
+                      int32_t equal_weight, uint64_t tag, int32_t rng_mode, int32_t cell_lo, int32_t cell_hi) {
+    MZ_CHECK(s1 >= 1 && s1 <= h->n_species && s2 >= 1 && s2 <= h->n_species, "ntc: bad species");
+    MZ_CHECK(cell_lo >= 1 && cell_hi <= h->n_cells && cell_lo <= cell_hi, "ntc: bad cell range");
+    MZ_CHECK(h->inter[(s1 - 1) * h->n_species + (s2 - 1)].set, "ntc: interaction not set");
+    MZ_CHECK(rng_mode == 0 || (rng_mode == 1 && h->d_replay[0]), "ntc: replay table not set");
+    const double *dV = h->d_V;
+    if (V) {
+        double *v = (double *)mz_scratch(h, "ntc_V", h->n_cells * sizeof(double));
+        MZ_CHECK(v != nullptr, "ntc: scratch allocation failed");
+        MZ_HIP(hipMemcpy(v, V, h->n_cells * sizeof(double), hipMemcpyHostToDevice));
+        dV = v;
+    } else {
+        MZ_CHECK(h->has_grid, "ntc: V is NULL and no grid is set");
+    }
+    mz_timing_begin(h, "ntc");
+    int r;
+    if (equal_weight) r = mz_launch_ntc_equal(h, s1, s2, dt, dV, dw_tol, 1, tag, rng_mode, cell_lo, cell_hi);
+    else r = launch_ntc_split(h, s1, s2, dt, dV, dw_tol, tag, rng_mode, cell_lo, cell_hi);
+    mz_timing_end(h);
+    return r;
+}

@@ -1,0 +1,133 @@
+// mz_step.hip -- multi-step drivers: the 1-D Couette step (ntc! in every
+// cell, convection, sort, compute_props_sorted!; test/test_1D_couette.jl:66-97)
+// launched kernel by kernel or replayed as a captured hipGraph.
+//
+// A captured graph holds two steps: the sort writes into the second particle
+// buffer, so two steps bring the buffers back to where the capture started.
+// The random tags of captured kernels come from a device step counter that the
+// graph's last node advances, so every replay draws fresh numbers in the same
+// (step << 4) | phase layout as eager launches.
+#include "mz_internal.h"
+
+#include <cstdio>
+#include <vector>
+
+__global__ void k_set_step(uint64_t *step, uint64_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step = v;
+}
+__global__ void k_step_inc(uint64_t *step) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+void mz_graph_release(mz_store *h) {
+    if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+    h->graph_key.clear();
+    if (h->d_step) hipFree(h->d_step);
+    h->d_step = nullptr;
+}
+
+// one step on the current buffers; the host-side buffer swap happens here
+static int enqueue_step(mz_store *h, const double *dV, double dt, int do_props, uint64_t step,
+                        const uint64_t *step_ptr) {
+    int r = mz_launch_ntc_equal(h, 1, 1, dt, dV, 1e-16, 1, (step << 4) | 1u, 0, 1, h->n_cells, step_ptr);
+    if (r) return r;
+    r = mz_launch_convect(h, 1, dt, (step << 4) | 2u, 0, true, 0, step_ptr);
+    if (r) return r;
+    r = mz_launch_sort(h, 1, 1);
+    if (r) return r;
+    mz_swap_buffers(h->sp[0]);
+    if (do_props) {
+        r = mz_launch_props(h, 1, 0);
+        if (r) return r;
+    }
+    return 0;
+}
+
+extern "C" int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *V,
+                                int32_t equal_weight, int32_t props_every, const double *masses, int32_t mode) {
+    MZ_CHECK(h->n_species == 1, "couette_steps: single-species stores only");
+    MZ_CHECK(h->has_grid && h->has_walls, "couette_steps: grid or walls not set");
+    MZ_CHECK(h->inter[0].set, "couette_steps: interaction not set");
+    MZ_CHECK(masses != nullptr, "couette_steps: masses is NULL");
+    MZ_CHECK(n_steps >= 0, "couette_steps: negative step count");
+    MZ_HIP(hipMemcpy(h->d_masses, masses, sizeof(double), hipMemcpyHostToDevice));
+    const double *dV = h->d_V;
+    if (V) {
+        double *v = (double *)mz_scratch(h, "step_V", h->n_cells * sizeof(double));
+        MZ_CHECK(v != nullptr, "couette_steps: scratch allocation failed");
+        MZ_HIP(hipMemcpy(v, V, h->n_cells * sizeof(double), hipMemcpyHostToDevice));
+        dV = v;
+    }
+    MzCounters c;
+    int r = mz_read_counters(h, 1, &c);
+    if (r) return r;
+    if (!c.contiguous) {
+        r = mz_squash(h, 1);
+        if (r) return r;
+    }
+    const bool graph_ok = mode == 1 && equal_weight && (props_every == 0 || props_every == 1);
+    if (!graph_ok) {
+        // eager launches (also the path for ntc! with weight splitting)
+        mz_timing_begin(h, "step");
+        for (int32_t t = 0; t < n_steps; ++t) {
+            uint64_t step = step0 + (uint64_t)t;
+            int do_props = props_every > 0 && ((t + 1) % props_every) == 0;
+            if (equal_weight) {
+                r = enqueue_step(h, dV, dt, do_props, step, nullptr);
+                if (r) return r;
+            } else {
+                r = mz_ntc(h, 1, 1, dt, V, 1e-16, 0, (step << 4) | 1u, 0, 1, h->n_cells);
+                if (r) return r;
+                r = mz_convect(h, 1, dt, (step << 4) | 2u, 0);
+                if (r) return r;
+                r = mz_sort(h, 1, 1);
+                if (r) return r;
+                if (do_props) {
+                    r = mz_launch_props(h, 1, 0);
+                    if (r) return r;
+                }
+            }
+        }
+        mz_timing_end_n(h, n_steps);
+        return 0;
+    }
+    // graph replay
+    char keybuf[256];
+    std::snprintf(keybuf, sizeof(keybuf), "couette:%a:%d:%p", dt, props_every, (const void *)dV);
+    if (h->graph_key != keybuf) {
+        mz_graph_release(h);
+        MZ_HIP(hipMalloc(&h->d_step, sizeof(uint64_t)));
+        // scratch is allocated before capture
+        int32_t nchunks = (int32_t)((h->sp[0].cap + MZ_SORT_CHUNK - 1) / MZ_SORT_CHUNK);
+        MZ_CHECK(mz_scratch(h, "sort_hist", (size_t)h->n_cells * nchunks * sizeof(int32_t)) != nullptr,
+                 "couette_steps: scratch allocation failed");
+        MZ_HIP(hipStreamSynchronize(h->stream));
+        MZ_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        for (int rep = 0; rep < 2; ++rep) {
+            r = enqueue_step(h, dV, dt, props_every == 1, 0, h->d_step);
+            if (r) {
+                hipGraph_t g;
+                hipStreamEndCapture(h->stream, &g);
+                if (g) hipGraphDestroy(g);
+                return r;
+            }
+            hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, h->stream, h->d_step);
+        }
+        MZ_HIP(hipStreamEndCapture(h->stream, &h->graph));
+        MZ_HIP(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+        h->graph_key = keybuf;
+    }
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, h->stream, h->d_step, step0);
+    mz_timing_begin(h, "step");
+    for (int32_t t = 0; t + 1 < n_steps; t += 2) MZ_HIP(hipGraphLaunch(h->graph_exec, h->stream));
+    if (n_steps & 1) {
+        uint64_t step = step0 + (uint64_t)(n_steps - 1);
+        r = enqueue_step(h, dV, dt, props_every == 1, step, nullptr);
+        if (r) return r;
+    }
+    mz_timing_end_n(h, n_steps);
+    return 0;
+}

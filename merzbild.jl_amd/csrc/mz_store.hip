@@ -1,0 +1,1016 @@
+// mz_store.hip -- device particle store, grid sorting, squash, 1-D
+// convection and physical properties for gfx950.
+//
+// Reference behaviour restated here (file:line in Merzbild.jl):
+//   sort_particles!        src/grids/grid_sorting.jl:58-182
+//   squash_pia!            src/particles.jl:622-701
+//   convect_particles!     src/convection/convection_1D.jl:17-157
+//   reflect/diffuse walls  src/convection/boundary_conditions.jl:79-121
+//   compute_props!         src/properties/physical_props.jl:104-154
+//   compute_props_sorted!  src/properties/physical_props.jl:317-454
+#include "mz_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+// ===========================================================================
+// error string
+static thread_local std::string g_err;
+void mz_set_error(const std::string &msg) { g_err = msg; }
+extern "C" const char *mz_last_error(void) { return g_err.c_str(); }
+
+extern "C" int mz_device_info(char *name, int name_len, int *n_cu, double *hbm_gbps_peak) {
+    int dev = 0;
+    MZ_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    MZ_HIP(hipGetDeviceProperties(&p, dev));
+    if (name && name_len > 0) {
+        std::strncpy(name, p.gcnArchName, name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (n_cu) *n_cu = p.multiProcessorCount;
+    // MI355X published HBM3E peak
+    if (hbm_gbps_peak) *hbm_gbps_peak = 8000.0;
+    return 0;
+}
+
+// ===========================================================================
+// timing
+int mz_timing_begin(mz_store *h, const char *family) {
+    if (!h->timing) return 0;
+    MzEventPair ev;
+    MZ_HIP(hipEventCreate(&ev.a));
+    MZ_HIP(hipEventCreate(&ev.b));
+    MZ_HIP(hipEventRecord(ev.a, h->stream));
+    h->open_events.push_back({std::string(family), ev});
+    return 0;
+}
+
+int mz_timing_end(mz_store *h) { return mz_timing_end_n(h, 1); }
+
+// closes the last open bracket; it counts as n launches of its family
+int mz_timing_end_n(mz_store *h, int64_t n) {
+    if (!h->timing || h->open_events.empty()) return 0;
+    auto fe = h->open_events.back();
+    h->open_events.pop_back();
+    MZ_HIP(hipEventRecord(fe.second.b, h->stream));
+    fe.second.n = n;
+    h->pending[fe.first].push_back(fe.second);
+    return 0;
+}
+
+static int resolve_timing(mz_store *h) {
+    if (h->pending.empty()) return 0;
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    for (auto &kv : h->pending) {
+        auto &acc = h->timing_acc[kv.first];
+        for (auto &ev : kv.second) {
+            float ms = 0.f;
+            MZ_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
+            acc.first += ms;
+            acc.second += ev.n;
+            hipEventDestroy(ev.a);
+            hipEventDestroy(ev.b);
+        }
+    }
+    h->pending.clear();
+    return 0;
+}
+
+extern "C" int mz_timing_enable(mz_store *h, int32_t on) {
+    h->timing = on != 0;
+    return 0;
+}
+
+extern "C" int mz_timing_get(mz_store *h, const char *family, double *mean_ms, int64_t *count) {
+    int r = resolve_timing(h);
+    if (r) return r;
+    auto it = h->timing_acc.find(family);
+    if (it == h->timing_acc.end() || it->second.second == 0) {
+        if (mean_ms) *mean_ms = 0.0;
+        if (count) *count = 0;
+        return 0;
+    }
+    if (mean_ms) *mean_ms = it->second.first / (double)it->second.second;
+    if (count) *count = it->second.second;
+    return 0;
+}
+
+extern "C" int mz_timing_reset(mz_store *h) {
+    int r = resolve_timing(h);
+    if (r) return r;
+    h->timing_acc.clear();
+    return 0;
+}
+
+// ===========================================================================
+// create / destroy
+static int alloc_species(MzSpecies &s, int64_t cap, int n_cells) {
+    s.cap = cap;
+    s.bufcap = cap + 4 * MZ_DELTA_PARTICLES;
+    for (int d = 0; d < 3; ++d) {
+        MZ_HIP(hipMalloc(&s.x[d], cap * sizeof(double)));
+        MZ_HIP(hipMalloc(&s.v[d], cap * sizeof(double)));
+        MZ_HIP(hipMalloc(&s.xb[d], cap * sizeof(double)));
+        MZ_HIP(hipMalloc(&s.vb[d], cap * sizeof(double)));
+    }
+    MZ_HIP(hipMalloc(&s.w, cap * sizeof(double)));
+    MZ_HIP(hipMalloc(&s.wb, cap * sizeof(double)));
+    MZ_HIP(hipMalloc(&s.id, cap * sizeof(int32_t)));
+    MZ_HIP(hipMalloc(&s.idb, cap * sizeof(int32_t)));
+    MZ_HIP(hipMalloc(&s.key, cap * sizeof(int32_t)));
+    MZ_HIP(hipMalloc(&s.keyb, cap * sizeof(int32_t)));
+    MZ_HIP(hipMalloc(&s.buffer, s.bufcap * sizeof(int32_t)));
+    int32_t **f[7] = {&s.pia.start1, &s.pia.end1, &s.pia.n1, &s.pia.start2, &s.pia.end2, &s.pia.n2, &s.pia.nloc};
+    for (auto p : f) MZ_HIP(hipMalloc(p, n_cells * sizeof(int32_t)));
+    MZ_HIP(hipMalloc(&s.ctr, sizeof(MzCounters)));
+    MZ_HIP(hipMemset(s.ctr, 0, sizeof(MzCounters)));
+    return 0;
+}
+
+extern "C" int mz_create(int32_t n_cells, int32_t n_species, const int64_t *capacity, mz_store **out) {
+    MZ_CHECK(n_cells > 0 && n_species > 0 && n_species <= MZ_MAX_SPECIES, "mz_create: bad sizes");
+    MZ_CHECK(capacity != nullptr, "mz_create: capacity is NULL");
+    mz_store *h = new mz_store();
+    h->n_cells = n_cells;
+    h->n_species = n_species;
+    h->sp.resize(n_species);
+    MZ_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    for (int s = 0; s < n_species; ++s) {
+        MZ_CHECK(capacity[s] > 0 && capacity[s] < (int64_t)INT32_MAX / 2, "mz_create: capacity out of range");
+        int r = alloc_species(h->sp[s], capacity[s], n_cells);
+        if (r) return r;
+        // empty PIA: ParticleIndexer() = (0, 0, -1, 0, 0, -1, 0)
+        std::vector<int64_t> pia((size_t)n_cells * 7, 0);
+        for (int c = 0; c < n_cells; ++c) { pia[c * 7 + 2] = -1; pia[c * 7 + 5] = -1; }
+        r = mz_upload_species(h, s + 1, 0, nullptr, nullptr, nullptr, nullptr, pia.data(), 0, 1, nullptr, 0, 0);
+        if (r) return r;
+    }
+    int nss = n_species * n_species;
+    MZ_HIP(hipMalloc(&h->d_sgwm, (size_t)nss * n_cells * sizeof(double)));
+    MZ_HIP(hipMemset(h->d_sgwm, 0, (size_t)nss * n_cells * sizeof(double)));
+    MZ_HIP(hipMalloc(&h->d_cfcnt, (size_t)nss * n_cells * 5 * sizeof(int32_t)));
+    MZ_HIP(hipMemset(h->d_cfcnt, 0, (size_t)nss * n_cells * 5 * sizeof(int32_t)));
+    MZ_HIP(hipMalloc(&h->d_err, sizeof(int32_t)));
+    MZ_HIP(hipMemset(h->d_err, 0, sizeof(int32_t)));
+    size_t pn = (size_t)n_species * n_cells;
+    MZ_HIP(hipMalloc(&h->d_np, pn * sizeof(double)));
+    MZ_HIP(hipMalloc(&h->d_n, pn * sizeof(double)));
+    MZ_HIP(hipMalloc(&h->d_T, pn * sizeof(double)));
+    MZ_HIP(hipMalloc(&h->d_v, pn * 3 * sizeof(double)));
+    MZ_HIP(hipMemset(h->d_np, 0, pn * sizeof(double)));
+    MZ_HIP(hipMemset(h->d_n, 0, pn * sizeof(double)));
+    MZ_HIP(hipMemset(h->d_T, 0, pn * sizeof(double)));
+    MZ_HIP(hipMemset(h->d_v, 0, pn * 3 * sizeof(double)));
+    MZ_HIP(hipMalloc(&h->d_V, n_cells * sizeof(double)));
+    MZ_HIP(hipMalloc(&h->d_invV, n_cells * sizeof(double)));
+    MZ_HIP(hipMalloc(&h->d_masses, MZ_MAX_SPECIES * sizeof(double)));
+    h->inter.resize(nss);
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    *out = h;
+    return 0;
+}
+
+static void free_species(MzSpecies &s) {
+    for (int d = 0; d < 3; ++d) {
+        hipFree(s.x[d]); hipFree(s.v[d]); hipFree(s.xb[d]); hipFree(s.vb[d]);
+    }
+    hipFree(s.w); hipFree(s.wb); hipFree(s.id); hipFree(s.idb); hipFree(s.key); hipFree(s.keyb);
+    hipFree(s.buffer);
+    hipFree(s.pia.start1); hipFree(s.pia.end1); hipFree(s.pia.n1); hipFree(s.pia.start2);
+    hipFree(s.pia.end2); hipFree(s.pia.n2); hipFree(s.pia.nloc);
+    hipFree(s.ctr);
+}
+
+extern "C" int mz_destroy(mz_store *h) {
+    if (!h) return 0;
+    hipStreamSynchronize(h->stream);
+    mz_graph_release(h);
+    for (auto &s : h->sp) free_species(s);
+    hipFree(h->d_sgwm); hipFree(h->d_cfcnt); hipFree(h->d_err);
+    hipFree(h->d_np); hipFree(h->d_n); hipFree(h->d_T); hipFree(h->d_v); hipFree(h->d_V);
+    hipFree(h->d_invV);
+    hipFree(h->d_masses);
+    for (int k = 0; k < 3; ++k) if (h->d_replay[k]) hipFree(h->d_replay[k]);
+    for (auto &sc : h->scratch) if (sc.second.ptr) hipFree(sc.second.ptr);
+    hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+extern "C" int mz_sync(mz_store *h) {
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+void *mz_scratch(mz_store *h, const char *name, size_t bytes) {
+    auto &sc = h->scratch[name];
+    if (sc.bytes < bytes) {
+        if (sc.ptr) {
+            hipStreamSynchronize(h->stream);
+            hipFree(sc.ptr);
+        }
+        sc.ptr = nullptr;
+        size_t nb = std::max(bytes, (size_t)256);
+        if (hipMalloc(&sc.ptr, nb) != hipSuccess) {
+            sc.bytes = 0;
+            return nullptr;
+        }
+        sc.bytes = nb;
+    }
+    return sc.ptr;
+}
+
+// ===========================================================================
+// upload / download
+extern "C" int mz_upload_species(mz_store *h, int32_t species, int64_t n_pos, const double *x,
+                                 const double *v, const double *w, const int64_t *ids,
+                                 const int64_t *pia, int64_t n_total, int32_t contiguous,
+                                 const int64_t *buffer, int64_t nbuffer, int64_t length) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "upload: bad species");
+    MzSpecies &s = h->sp[species - 1];
+    MZ_CHECK(n_pos >= 0 && n_pos <= s.cap, "upload: n_pos exceeds capacity");
+    MZ_CHECK(length >= 0 && length < (int64_t)INT32_MAX / 2, "upload: ParticleVector length out of range");
+    MZ_CHECK(nbuffer >= 0 && nbuffer <= length, "upload: bad nbuffer");
+    MZ_CHECK(pia != nullptr, "upload: pia is NULL");
+    // the free-id LIFO holds at most `length` ids; creations can grow length up
+    // to the slot capacity plus one resize! chunk
+    int64_t need = std::max<int64_t>(length, s.cap) + 2 * MZ_DELTA_PARTICLES;
+    if (need > s.bufcap) {
+        MZ_HIP(hipStreamSynchronize(h->stream));
+        hipFree(s.buffer);
+        s.bufcap = need;
+        MZ_HIP(hipMalloc(&s.buffer, s.bufcap * sizeof(int32_t)));
+    }
+    std::vector<double> tmp((size_t)std::max<int64_t>(n_pos, 1));
+    for (int d = 0; d < 3 && n_pos > 0; ++d) {
+        for (int64_t i = 0; i < n_pos; ++i) tmp[i] = x[3 * i + d];
+        MZ_HIP(hipMemcpy(s.x[d], tmp.data(), n_pos * sizeof(double), hipMemcpyHostToDevice));
+        for (int64_t i = 0; i < n_pos; ++i) tmp[i] = v[3 * i + d];
+        MZ_HIP(hipMemcpy(s.v[d], tmp.data(), n_pos * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (n_pos > 0) {
+        MZ_HIP(hipMemcpy(s.w, w, n_pos * sizeof(double), hipMemcpyHostToDevice));
+        std::vector<int32_t> iv(n_pos);
+        for (int64_t i = 0; i < n_pos; ++i) {
+            MZ_CHECK(ids[i] >= 1 && ids[i] <= INT32_MAX, "upload: id out of range");
+            iv[i] = (int32_t)ids[i];
+        }
+        MZ_HIP(hipMemcpy(s.id, iv.data(), n_pos * sizeof(int32_t), hipMemcpyHostToDevice));
+        std::fill(iv.begin(), iv.end(), 0);
+        MZ_HIP(hipMemcpy(s.key, iv.data(), n_pos * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    int nc = h->n_cells;
+    std::vector<int32_t> col((size_t)nc);
+    int32_t *dst[7] = {s.pia.nloc, s.pia.start1, s.pia.end1, s.pia.n1, s.pia.start2, s.pia.end2, s.pia.n2};
+    int64_t max_end = 0;
+    for (int f = 0; f < 7; ++f) {
+        for (int c = 0; c < nc; ++c) {
+            int64_t val = pia[(int64_t)c * 7 + f];
+            MZ_CHECK(val >= -1 && val <= s.cap, "upload: PIA value out of range");
+            col[c] = (int32_t)val;
+            if (f == 2 || f == 5) max_end = std::max(max_end, val);
+        }
+        MZ_HIP(hipMemcpy(dst[f], col.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    MZ_CHECK(max_end <= n_pos, "upload: PIA references positions beyond n_pos");
+    // the consistency that check_pia_is_correct demands (src/particles.jl:863-907),
+    // plus group ranges inside [1, n_pos], so no kernel can index past the store
+    int64_t sum_local = 0;
+    for (int c = 0; c < nc; ++c) {
+        const int64_t *row = pia + (int64_t)c * 7;
+        int64_t nl = row[0], s1 = row[1], e1 = row[2], g1 = row[3], s2 = row[4], e2 = row[5], g2 = row[6];
+        MZ_CHECK(g1 >= 0 && g2 >= 0 && nl == g1 + g2, "upload: PIA n_local != n_group1 + n_group2");
+        MZ_CHECK(g1 == 0 || (s1 >= 1 && e1 == s1 + g1 - 1 && e1 <= n_pos), "upload: PIA group 1 out of range");
+        MZ_CHECK(g2 == 0 || (s2 >= 1 && e2 == s2 + g2 - 1 && e2 <= n_pos), "upload: PIA group 2 out of range");
+        sum_local += nl;
+    }
+    MZ_CHECK(sum_local == n_total, "upload: n_total differs from the sum of n_local");
+    MZ_CHECK(n_total >= 0 && n_total <= s.cap && (!contiguous || n_total <= n_pos),
+             "upload: n_total out of range");
+    if (nbuffer > 0) {
+        std::vector<int32_t> b(nbuffer);
+        for (int64_t i = 0; i < nbuffer; ++i) b[i] = (int32_t)buffer[i];
+        MZ_HIP(hipMemcpy(s.buffer, b.data(), nbuffer * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    MzCounters c{};
+    c.n_total = (int32_t)n_total;
+    c.contiguous = contiguous ? 1 : 0;
+    c.nbuffer = (int32_t)nbuffer;
+    c.length = (int32_t)length;
+    c.n_pos = (int32_t)std::max<int64_t>(max_end, contiguous ? n_total : max_end);
+    MZ_HIP(hipMemcpy(s.ctr, &c, sizeof(c), hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int mz_upload_cell_keys(mz_store *h, int32_t species, int64_t n, const int64_t *cells) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "upload_cell_keys: bad species");
+    MzSpecies &s = h->sp[species - 1];
+    MZ_CHECK(n >= 0 && n <= s.cap, "upload_cell_keys: n exceeds capacity");
+    std::vector<int32_t> k((size_t)std::max<int64_t>(n, 1));
+    for (int64_t i = 0; i < n; ++i) {
+        MZ_CHECK(cells[i] >= 1 && cells[i] <= h->n_cells, "upload_cell_keys: cell out of range");
+        k[i] = (int32_t)cells[i];
+    }
+    if (n > 0) MZ_HIP(hipMemcpy(s.key, k.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int mz_read_counters(mz_store *h, int species, MzCounters *c) {
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    MZ_HIP(hipMemcpy(c, h->sp[species - 1].ctr, sizeof(MzCounters), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int mz_get_scalars(mz_store *h, int32_t species, int64_t *out) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "get_scalars: bad species");
+    MzCounters c;
+    int r = mz_read_counters(h, species, &c);
+    if (r) return r;
+    int32_t rerr = 0;
+    MZ_HIP(hipMemcpy(&rerr, h->d_err, sizeof(int32_t), hipMemcpyDeviceToHost));
+    out[0] = c.n_total; out[1] = c.contiguous; out[2] = c.nbuffer; out[3] = c.length;
+    out[4] = c.n_pos; out[5] = c.err | (rerr ? MZ_ERR_REPLAY : 0);
+    return 0;
+}
+
+extern "C" int mz_download_species(mz_store *h, int32_t species, int64_t n_pos_max, double *x, double *v,
+                                   double *w, int64_t *ids, int64_t *cell_keys, int64_t *pia,
+                                   int64_t *buffer, int64_t buffer_max) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "download: bad species");
+    MzSpecies &s = h->sp[species - 1];
+    MzCounters c;
+    int r = mz_read_counters(h, species, &c);
+    if (r) return r;
+    int64_t n = std::min<int64_t>(c.n_pos, n_pos_max);
+    std::vector<double> tmp((size_t)std::max<int64_t>(n, 1));
+    if (n > 0) {
+        for (int d = 0; d < 3; ++d) {
+            if (x) {
+                MZ_HIP(hipMemcpy(tmp.data(), s.x[d], n * sizeof(double), hipMemcpyDeviceToHost));
+                for (int64_t i = 0; i < n; ++i) x[3 * i + d] = tmp[i];
+            }
+            if (v) {
+                MZ_HIP(hipMemcpy(tmp.data(), s.v[d], n * sizeof(double), hipMemcpyDeviceToHost));
+                for (int64_t i = 0; i < n; ++i) v[3 * i + d] = tmp[i];
+            }
+        }
+        if (w) MZ_HIP(hipMemcpy(w, s.w, n * sizeof(double), hipMemcpyDeviceToHost));
+        std::vector<int32_t> iv(n);
+        if (ids) {
+            MZ_HIP(hipMemcpy(iv.data(), s.id, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) ids[i] = iv[i];
+        }
+        if (cell_keys) {
+            MZ_HIP(hipMemcpy(iv.data(), s.key, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) cell_keys[i] = iv[i];
+        }
+    }
+    if (pia) {
+        int nc = h->n_cells;
+        std::vector<int32_t> col(nc);
+        int32_t *src[7] = {s.pia.nloc, s.pia.start1, s.pia.end1, s.pia.n1, s.pia.start2, s.pia.end2, s.pia.n2};
+        for (int f = 0; f < 7; ++f) {
+            MZ_HIP(hipMemcpy(col.data(), src[f], nc * sizeof(int32_t), hipMemcpyDeviceToHost));
+            for (int cc = 0; cc < nc; ++cc) pia[(int64_t)cc * 7 + f] = col[cc];
+        }
+    }
+    if (buffer && c.nbuffer > 0) {
+        int64_t nb = std::min<int64_t>(c.nbuffer, buffer_max);
+        std::vector<int32_t> b(nb);
+        MZ_HIP(hipMemcpy(b.data(), s.buffer, nb * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < nb; ++i) buffer[i] = b[i];
+    }
+    return 0;
+}
+
+// ===========================================================================
+// parameters
+extern "C" int mz_set_grid_1d(mz_store *h, double L, int64_t nx, double wall_offset) {
+    MZ_CHECK(nx == h->n_cells, "set_grid_1d: nx must equal the store's cell count");
+    MZ_CHECK(L > 0.0, "set_grid_1d: L must be positive");
+    double dx = L / (double)nx;
+    h->grid.L = L;
+    h->grid.nx = (int32_t)nx;
+    h->grid.dx = dx;
+    h->grid.inv_dx = 1.0 / dx;
+    h->grid.min_x = dx * wall_offset;
+    h->grid.max_x = L - dx * wall_offset;
+    h->has_grid = true;
+    // Cell1D(xlo, xhi, V): V = dx, inv_V = 1/V (grid_uniform1D.jl:20-31, 76-82)
+    std::vector<double> V(nx, dx), iV(nx, 1.0 / dx);
+    MZ_HIP(hipMemcpy(h->d_V, V.data(), nx * sizeof(double), hipMemcpyHostToDevice));
+    MZ_HIP(hipMemcpy(h->d_invV, iV.data(), nx * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int mz_set_walls_1d(mz_store *h, const double *T, const double *v, const double *acc,
+                               const double *vsq) {
+    for (int b = 0; b < 2; ++b) {
+        h->walls.T[b] = T[b];
+        for (int d = 0; d < 3; ++d) h->walls.v[b][d] = v[3 * b + d];
+        h->walls.acc[b] = acc[b];
+        for (int s = 0; s < h->n_species; ++s) h->walls.vsq[b][s] = vsq[b * h->n_species + s];
+    }
+    h->has_walls = true;
+    return 0;
+}
+
+extern "C" int mz_set_interaction(mz_store *h, int32_t s1, int32_t s2, double mu1, double mu2,
+                                  double vhs_factor, double vhs_o) {
+    MZ_CHECK(s1 >= 1 && s1 <= h->n_species && s2 >= 1 && s2 <= h->n_species, "set_interaction: bad species");
+    MzInteraction &it = h->inter[(s1 - 1) * h->n_species + (s2 - 1)];
+    it.mu1 = mu1;
+    it.mu2 = mu2;
+    it.factor = vhs_factor;
+    it.expo = 1.0 - 2 * vhs_o;   // sigma_vhs = factor * g^(1 - 2 omega)
+    it.set = 1;
+    return 0;
+}
+
+extern "C" int mz_set_sigma_g_w_max(mz_store *h, int32_t s1, int32_t s2, const double *sgwm) {
+    MZ_CHECK(s1 >= 1 && s1 <= h->n_species && s2 >= 1 && s2 <= h->n_species, "set_sgwm: bad species");
+    size_t off = ((size_t)(s1 - 1) * h->n_species + (s2 - 1)) * h->n_cells;
+    MZ_HIP(hipMemcpy(h->d_sgwm + off, sgwm, h->n_cells * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int mz_get_collision_factors(mz_store *h, int32_t s1, int32_t s2, double *sgwm, int64_t *n1,
+                                        int64_t *n2, int64_t *n_coll, int64_t *n_performed,
+                                        int64_t *n_eq_w) {
+    MZ_CHECK(s1 >= 1 && s1 <= h->n_species && s2 >= 1 && s2 <= h->n_species, "get_cf: bad species");
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    size_t pair = (size_t)(s1 - 1) * h->n_species + (s2 - 1);
+    int nc = h->n_cells;
+    if (sgwm) MZ_HIP(hipMemcpy(sgwm, h->d_sgwm + pair * nc, nc * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<int32_t> cnt((size_t)nc * 5);
+    MZ_HIP(hipMemcpy(cnt.data(), h->d_cfcnt + pair * nc * 5, cnt.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    int64_t *outs[5] = {n1, n2, n_coll, n_performed, n_eq_w};
+    for (int f = 0; f < 5; ++f)
+        if (outs[f])
+            for (int c = 0; c < nc; ++c) outs[f][c] = cnt[(size_t)c * 5 + f];
+    return 0;
+}
+
+extern "C" int mz_set_seed(mz_store *h, uint64_t seed) {
+    h->seed = seed;
+    return 0;
+}
+
+extern "C" int mz_set_replay(mz_store *h, int32_t kind, const double *table, int64_t rows, int64_t cols) {
+    MZ_CHECK(kind >= 0 && kind < 3, "set_replay: bad kind");
+    if (h->d_replay[kind]) {
+        MZ_HIP(hipStreamSynchronize(h->stream));
+        hipFree(h->d_replay[kind]);
+        h->d_replay[kind] = nullptr;
+    }
+    h->replay_rows[kind] = rows;
+    h->replay_cols[kind] = cols;
+    if (rows > 0 && cols > 0) {
+        MZ_HIP(hipMalloc(&h->d_replay[kind], rows * cols * sizeof(double)));
+        MZ_HIP(hipMemcpy(h->d_replay[kind], table, rows * cols * sizeof(double), hipMemcpyHostToDevice));
+    }
+    MZ_HIP(hipMemset(h->d_err, 0, sizeof(int32_t)));
+    return 0;
+}
+
+MzRngParams mz_rng_params(mz_store *h, int kind, uint64_t tag, int mode) {
+    MzRngParams rp;
+    rp.seed = h->seed;
+    rp.tag = tag;
+    rp.mode = mode;
+    rp.pad = 0;
+    rp.table = h->d_replay[kind];
+    rp.rows = h->replay_rows[kind];
+    rp.cols = h->replay_cols[kind];
+    rp.err = h->d_err;
+    rp.step_ptr = nullptr;
+    rp.phase = 0;
+    return rp;
+}
+
+// ===========================================================================
+// Sorting: stable counting sort by cell (grid_sorting.jl:58-182).
+//   1. k_sort_hist: per chunk of MZ_SORT_CHUNK positions, cell keys (from x,
+//      or the stored keys) and a per-chunk histogram in LDS; the histogram is
+//      written cell-major [cell][chunk].
+//   2. k_sort_scan: exclusive scan of that matrix; its cell-major order makes
+//      offsets[cell][chunk] the first output slot of chunk's particles in
+//      cell, and the per-cell totals give the new PIA (group 1 only, as the
+//      reference writes it).
+//   3. k_sort_scatter: one wave per chunk walks its positions in order, 64 at
+//      a time; lanes with equal keys are ranked with a ballot, so equal keys
+//      keep their order (stable, as the reference's reverse scatter).
+// The data moves with the sort (x, v, w, id); the key array stays in the
+// pre-sort order, like the reference's pv.cell.
+
+__global__ void k_sort_hist(MzSpView sp, MzGrid g, int32_t compute_cells, int32_t n_cells, int32_t nchunks,
+                            int32_t *__restrict__ hist) {
+    extern __shared__ int32_t sh[];
+    const int32_t n = mz_ntotal(sp);
+    for (int c = threadIdx.x; c < n_cells; c += blockDim.x) sh[c] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * MZ_SORT_CHUNK;
+    for (int i = threadIdx.x; i < MZ_SORT_CHUNK; i += blockDim.x) {
+        int64_t p = base + i;
+        if (p < n) {
+            int32_t c;
+            if (compute_cells) {
+                // get_cell (grid_uniform1D.jl:97-99): floor(x * inv_dx) + 1
+                c = (int32_t)floor(sp.x0[p] * g.inv_dx) + 1;
+                if (c < 1 || c > n_cells) {
+                    atomicOr(&sp.ctr->err, MZ_ERR_CAPACITY);
+                    c = c < 1 ? 1 : n_cells;
+                }
+                sp.key[p] = c;
+            } else {
+                c = sp.key[p];
+                if (c < 1 || c > n_cells) {
+                    atomicOr(&sp.ctr->err, MZ_ERR_CAPACITY);
+                    c = c < 1 ? 1 : n_cells;
+                }
+            }
+            atomicAdd(&sh[c - 1], 1);
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < n_cells; c += blockDim.x) hist[(int64_t)c * nchunks + blockIdx.x] = sh[c];
+}
+
+// Exclusive scan of M ints in place with one 1024-thread block, then PIA.
+__global__ void __launch_bounds__(1024) k_sort_scan(MzSpView sp, int32_t n_cells, int32_t nchunks,
+                                                   int32_t *__restrict__ hist) {
+    __shared__ int32_t part[1024];
+    __shared__ int32_t wsum[16];
+    const int64_t M = (int64_t)n_cells * nchunks;
+    const int t = threadIdx.x;
+    const int64_t per = (M + 1023) / 1024;
+    const int64_t lo = t * per, hi = min(M, lo + per);
+    int32_t s = 0;
+    for (int64_t i = lo; i < hi; ++i) s += hist[i];
+    // block exclusive scan of s
+    int lane = t & 63, wv = t >> 6;
+    int32_t x = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    if (t < 16) {
+        int32_t a = wsum[t];
+        int32_t b = a;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            int32_t y = __shfl_up(b, o, 16);
+            if ((t & 15) >= o) b += y;
+        }
+        wsum[t] = b - a;
+    }
+    __syncthreads();
+    int32_t run = x - s + wsum[wv];
+    part[t] = run;
+    for (int64_t i = lo; i < hi; ++i) {
+        int32_t v = hist[i];
+        hist[i] = run;
+        run += v;
+    }
+    __syncthreads();
+    // PIA: cell c occupies [offsets[c][0], offsets[c+1][0]) in 0-based slots
+    const int32_t total = mz_ntotal(sp);
+    for (int c = t; c < n_cells; c += 1024) {
+        int32_t st = hist[(int64_t)c * nchunks];
+        int32_t en = (c + 1 < n_cells) ? hist[(int64_t)(c + 1) * nchunks] : total;
+        int32_t cnt = en - st;
+        sp.pia.start2[c] = 0;
+        sp.pia.end2[c] = -1;
+        sp.pia.n2[c] = 0;
+        sp.pia.start1[c] = cnt > 0 ? st + 1 : 0;
+        sp.pia.end1[c] = cnt > 0 ? en : -1;
+        sp.pia.n1[c] = cnt;
+        sp.pia.nloc[c] = cnt;
+    }
+    if (t == 0) {
+        sp.ctr->contiguous = 1;
+        sp.ctr->n_pos = total;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_sort_scatter(MzSpView a, MzSpView b, int32_t n_cells, int32_t nchunks,
+                                                    const int32_t *__restrict__ offs) {
+    extern __shared__ int32_t run[];
+    const int32_t n = mz_ntotal(a);
+    const int lane = threadIdx.x;
+    for (int c = lane; c < n_cells; c += 64) run[c] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * MZ_SORT_CHUNK;
+    const uint64_t lt = mz_lanemask_lt();
+    for (int bt = 0; bt < MZ_SORT_CHUNK; bt += 64) {
+        int64_t p = base + bt + lane;
+        bool valid = p < n;
+        int32_t key = valid ? a.key[p] : 0;
+        if (valid) key = key < 1 ? 1 : (key > n_cells ? n_cells : key);
+        uint64_t active = __ballot(valid);
+        if (active == 0) break;
+        int64_t dst = 0;
+        while (active) {
+            int leader = __ffsll((unsigned long long)active) - 1;
+            int32_t k = __shfl(key, leader, 64);
+            uint64_t mask = __ballot(valid && key == k);
+            int32_t basek = offs[(int64_t)(k - 1) * nchunks + blockIdx.x] + run[k - 1];
+            if (valid && key == k) dst = (int64_t)basek + __popcll(mask & lt);
+            __syncthreads();
+            if (lane == leader) run[k - 1] += __popcll(mask);
+            __syncthreads();
+            active &= ~mask;
+        }
+        if (valid) {
+            b.x0[dst] = a.x0[p]; b.x1[dst] = a.x1[p]; b.x2[dst] = a.x2[p];
+            b.v0[dst] = a.v0[p]; b.v1[dst] = a.v1[p]; b.v2[dst] = a.v2[p];
+            b.w[dst] = a.w[p];
+            b.id[dst] = a.id[p];
+        }
+    }
+}
+
+MzViews mz_views(mz_store *h) {
+    MzViews v{};
+    for (int s = 0; s < h->n_species; ++s) v.v[s] = mz_view(h->sp[s]);
+    return v;
+}
+
+void mz_swap_buffers(MzSpecies &s) {
+    for (int d = 0; d < 3; ++d) {
+        std::swap(s.x[d], s.xb[d]);
+        std::swap(s.v[d], s.vb[d]);
+    }
+    std::swap(s.w, s.wb);
+    std::swap(s.id, s.idb);
+}
+
+static MzSpView view_b(const MzSpecies &s) {
+    MzSpView v = mz_view(s);
+    v.x0 = s.xb[0]; v.x1 = s.xb[1]; v.x2 = s.xb[2];
+    v.v0 = s.vb[0]; v.v1 = s.vb[1]; v.v2 = s.vb[2];
+    v.w = s.wb; v.id = s.idb;
+    return v;
+}
+
+// launches the three sort kernels on the current A buffers; the caller swaps
+int mz_launch_sort(mz_store *h, int species, int compute_cells) {
+    MzSpecies &s = h->sp[species - 1];
+    int nc = h->n_cells;
+    MZ_CHECK(nc <= MZ_SORT_MAX_CELLS, "sort: too many cells for the LDS histogram");
+    MZ_CHECK(!compute_cells || h->has_grid, "sort: no grid set");
+    int32_t nchunks = (int32_t)((s.cap + MZ_SORT_CHUNK - 1) / MZ_SORT_CHUNK);
+    int32_t *hist = (int32_t *)mz_scratch(h, "sort_hist", (size_t)nc * nchunks * sizeof(int32_t));
+    MZ_CHECK(hist != nullptr, "sort: scratch allocation failed");
+    MzSpView a = mz_view(s), b = view_b(s);
+    size_t lds = (size_t)nc * sizeof(int32_t);
+    hipLaunchKernelGGL(k_sort_hist, dim3(nchunks), dim3(256), lds, h->stream, a, h->grid, compute_cells, nc,
+                       nchunks, hist);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, h->stream, a, nc, nchunks, hist);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nchunks), dim3(64), lds, h->stream, a, b, nc, nchunks, hist);
+    MZ_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int mz_sort(mz_store *h, int32_t species, int32_t compute_cells) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "sort: bad species");
+    MzCounters c;
+    int r = mz_read_counters(h, species, &c);
+    if (r) return r;
+    if (!c.contiguous) {
+        r = mz_squash(h, species);
+        if (r) return r;
+    }
+    mz_timing_begin(h, "sort");
+    r = mz_launch_sort(h, species, compute_cells);
+    mz_timing_end(h);
+    if (r) return r;
+    mz_swap_buffers(h->sp[species - 1]);
+    return 0;
+}
+
+// ===========================================================================
+// squash_pia! (particles.jl:622-701).  Group-1 segments are packed to the
+// front in cell order, then group-2 segments follow in cell order; each
+// segment keeps its internal order.  k_squash_plan computes the new starts
+// with a scan; k_squash_move copies each cell's segments into the B buffers.
+
+__global__ void __launch_bounds__(1024) k_squash_plan(MzSpView sp, int32_t n_cells, int32_t *__restrict__ news) {
+    // news[c] = new start1 (0-based), news[n_cells + c] = new start2 (0-based)
+    __shared__ int32_t wsum[16];
+    __shared__ int32_t carry;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (int pass = 0; pass < 2; ++pass) {
+        const int32_t *cnt = pass == 0 ? sp.pia.n1 : sp.pia.n2;
+        for (int c0 = 0; c0 < n_cells; c0 += 1024) {
+            int c = c0 + t;
+            int32_t v = c < n_cells ? cnt[c] : 0;
+            int32_t x = v;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                int32_t y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            if (lane == 63) wsum[wv] = x;
+            __syncthreads();
+            if (t < 16) {
+                int32_t a = wsum[t], b = a;
+                for (int o = 1; o < 16; o <<= 1) {
+                    int32_t y = __shfl_up(b, o, 16);
+                    if ((t & 15) >= o) b += y;
+                }
+                wsum[t] = b - a;
+            }
+            __syncthreads();
+            int32_t excl = carry + wsum[wv] + x - v;
+            if (c < n_cells) news[pass * n_cells + c] = excl;
+            __syncthreads();
+            if (t == 1023) carry = excl + v;
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void k_squash_move(MzSpView a, MzSpView b, int32_t n_cells, const int32_t *__restrict__ news) {
+    const int c = blockIdx.x;
+    for (int g = 0; g < 2; ++g) {
+        int32_t cnt = g == 0 ? a.pia.n1[c] : a.pia.n2[c];
+        int32_t src = (g == 0 ? a.pia.start1[c] : a.pia.start2[c]) - 1;
+        int32_t dst = news[g * n_cells + c];
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+            int64_t p = src + i, q = dst + i;
+            b.x0[q] = a.x0[p]; b.x1[q] = a.x1[p]; b.x2[q] = a.x2[p];
+            b.v0[q] = a.v0[p]; b.v1[q] = a.v1[p]; b.v2[q] = a.v2[p];
+            b.w[q] = a.w[p]; b.id[q] = a.id[p]; b.key[q] = a.key[p];
+        }
+    }
+}
+
+__global__ void k_squash_pia(MzSpView sp, int32_t n_cells, const int32_t *__restrict__ news) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n_cells) {
+        int32_t n1 = sp.pia.n1[c], n2 = sp.pia.n2[c];
+        if (n1 > 0) { sp.pia.start1[c] = news[c] + 1; sp.pia.end1[c] = news[c] + n1; }
+        if (n2 > 0) { sp.pia.start2[c] = news[n_cells + c] + 1; sp.pia.end2[c] = news[n_cells + c] + n2; }
+    }
+    if (c == 0) {
+        sp.ctr->contiguous = 1;
+        sp.ctr->n_pos = sp.ctr->n_total;
+    }
+}
+
+extern "C" int mz_squash(mz_store *h, int32_t species) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "squash: bad species");
+    MzCounters c;
+    int r = mz_read_counters(h, species, &c);
+    if (r) return r;
+    if (c.contiguous) return 0;
+    MzSpecies &s = h->sp[species - 1];
+    int nc = h->n_cells;
+    int32_t *news = (int32_t *)mz_scratch(h, "squash", (size_t)2 * nc * sizeof(int32_t));
+    MZ_CHECK(news != nullptr, "squash: scratch allocation failed");
+    MzSpView a = mz_view(s), b = view_b(s);
+    b.key = s.keyb;
+    hipLaunchKernelGGL(k_squash_plan, dim3(1), dim3(1024), 0, h->stream, a, nc, news);
+    hipLaunchKernelGGL(k_squash_move, dim3(nc), dim3(256), 0, h->stream, a, b, nc, news);
+    hipLaunchKernelGGL(k_squash_pia, dim3((nc + 255) / 256), dim3(256), 0, h->stream, a, nc, news);
+    MZ_HIP(hipGetLastError());
+    mz_swap_buffers(s);
+    std::swap(s.key, s.keyb);
+    return 0;
+}
+
+// ===========================================================================
+// Convection (convection_1D.jl:17-54, boundary_conditions.jl:79-121).
+// x + v t is an explicit fma, matching the reference's @muladd block, so the
+// cell keys that follow are reproducible bit for bit.  One thread per
+// particle; a particle draws from stream (id - 1).
+
+__device__ inline void diffuse_reflect(const MzRngParams &rp, MzStream &st, double &v0, double &v1, double &v2,
+                                       double vsq, double sgn, const double *wv) {
+    double R = fmax(1e-50, mz_rand(rp, st));
+    double vn = sgn * sqrt(-vsq * log(R));
+    R = fmax(1e-50, mz_rand(rp, st));
+    double vt = sqrt(-vsq * log(R));
+    R = 6.283185307179586 * mz_rand(rp, st);
+    double s, c;
+    sincos(R, &s, &c);
+    v0 = vn + wv[0];
+    v1 = s * vt + wv[1];
+    v2 = c * vt + wv[2];
+}
+
+__device__ inline void convect_one(const MzRngParams &rp, const MzGrid &g, const MzWalls &wl, int sidx, double dt,
+                                   int32_t id, double &x0, double &v0, double &v1, double &v2, int32_t *err) {
+    double t_rest = dt;
+    double x_old = x0;
+    double x_new = fma(v0, dt, x0);
+    int bounces = 0;
+    MzStream st = mz_stream((uint64_t)(id - 1));
+    while ((x_new >= g.L) || (x_new <= 0.0)) {
+        if (++bounces > MZ_MAX_BOUNCES) {
+            atomicOr(err, MZ_ERR_BOUNCES);
+            break;
+        }
+        int bc;
+        double wn;
+        if (x_new >= g.L) {
+            t_rest -= fabs((g.L - x_old) / v0);
+            bc = 1; wn = -1.0; x_old = g.L;
+        } else {
+            t_rest -= fabs(x_old / v0);
+            bc = 0; wn = 1.0; x_old = 0.0;
+        }
+        double acc = wl.acc[bc];
+        if (acc == 0.0) {
+            v0 = -v0;
+        } else if (acc == 1.0) {
+            diffuse_reflect(rp, st, v0, v1, v2, wl.vsq[bc][sidx], wn, wl.v[bc]);
+        } else {
+            double R = mz_rand(rp, st);
+            if (R < acc) diffuse_reflect(rp, st, v0, v1, v2, wl.vsq[bc][sidx], wn, wl.v[bc]);
+            else v0 = -v0;
+        }
+        x_new = fma(v0, t_rest, x_old);
+    }
+    if (x_new < g.min_x) x_new = g.min_x;
+    else if (x_new > g.max_x) x_new = g.max_x;
+    x0 = x_new;
+}
+
+__global__ void k_convect(MzSpView sp, MzGrid g, MzWalls wl, int sidx, double dt, MzRngParams rp,
+                          const uint8_t *__restrict__ mask) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    rp = mz_rng_resolve(rp);
+    const int32_t lim = mask ? mz_npos(sp) : mz_ntotal(sp);
+    if (p >= lim) return;
+    if (mask && !mask[p]) return;
+    double x0 = sp.x0[p], v0 = sp.v0[p];
+    double x_new = fma(v0, dt, x0);
+    if (x_new < g.L && x_new > 0.0) {
+        // free flight (the common case): only x changes
+        sp.x0[p] = x_new < g.min_x ? g.min_x : (x_new > g.max_x ? g.max_x : x_new);
+        return;
+    }
+    double v1 = sp.v1[p], v2 = sp.v2[p];
+    convect_one(rp, g, wl, sidx, dt, sp.id[p], x0, v0, v1, v2, &sp.ctr->err);
+    sp.x0[p] = x0;
+    sp.v0[p] = v0;
+    sp.v1[p] = v1;
+    sp.v2[p] = v2;
+}
+
+__global__ void k_mark_valid(MzSpView sp, int32_t n_cells, uint8_t *__restrict__ mask) {
+    const int c = blockIdx.x;
+    for (int g = 0; g < 2; ++g) {
+        int32_t cnt = g == 0 ? sp.pia.n1[c] : sp.pia.n2[c];
+        int32_t st = (g == 0 ? sp.pia.start1[c] : sp.pia.start2[c]) - 1;
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x)
+            if (st + i >= 0 && st + i < sp.cap) mask[st + i] = 1;
+    }
+}
+
+int mz_launch_convect(mz_store *h, int species, double dt, uint64_t tag, int rng_mode, bool contiguous,
+                      int32_t n_pos, const uint64_t *step_ptr) {
+    MzSpecies &s = h->sp[species - 1];
+    MZ_CHECK(h->has_grid && h->has_walls, "convect: grid or walls not set");
+    uint8_t *mask = nullptr;
+    if (!contiguous) {
+        mask = (uint8_t *)mz_scratch(h, "convect_mask", (size_t)s.cap);
+        MZ_CHECK(mask != nullptr, "convect: scratch allocation failed");
+        MZ_HIP(hipMemsetAsync(mask, 0, (size_t)s.cap, h->stream));
+        hipLaunchKernelGGL(k_mark_valid, dim3(h->n_cells), dim3(256), 0, h->stream, mz_view(s), h->n_cells, mask);
+    }
+    MzRngParams rp = mz_rng_params(h, 1, tag, rng_mode);
+    rp.step_ptr = step_ptr;
+    rp.phase = 2;
+    int64_t n = contiguous ? s.cap : std::max<int64_t>(n_pos, 1);
+    int blocks = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_convect, dim3(blocks), dim3(256), 0, h->stream, mz_view(s), h->grid, h->walls,
+                       species - 1, dt, rp, mask);
+    MZ_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int mz_convect(mz_store *h, int32_t species, double dt, uint64_t tag, int32_t rng_mode) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "convect: bad species");
+    MZ_CHECK(rng_mode == 0 || (rng_mode == 1 && h->d_replay[1]), "convect: replay table not set");
+    MzCounters c;
+    int r = mz_read_counters(h, species, &c);
+    if (r) return r;
+    mz_timing_begin(h, "convect");
+    r = mz_launch_convect(h, species, dt, tag, rng_mode, c.contiguous != 0, c.n_pos);
+    mz_timing_end(h);
+    return r;
+}
+
+// ===========================================================================
+// Physical properties: one wave per (cell, species), one pass over the cell's
+// particles.  The temperature uses shifted sums, E = sum w |v - s|^2 -
+// n |vbar - s|^2 with s the cell's first particle velocity, which keeps the
+// cancellation of the one-pass form at the level of the reference's
+// two-pass sum while reading each particle once.
+
+__global__ void __launch_bounds__(256) k_props(MzViews views, int32_t n_cells, int32_t variant,
+                                               int32_t ndens, const double *__restrict__ inv_V_cells,
+                                               const double *__restrict__ masses, double *__restrict__ out_np,
+                                               double *__restrict__ out_n, double *__restrict__ out_v,
+                                               double *__restrict__ out_T) {
+    const int sidx = blockIdx.y;
+    const MzSpView &sp = views.v[sidx];
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c >= n_cells) return;
+    int32_t st1 = sp.pia.start1[c] - 1, n1 = sp.pia.n1[c];
+    int32_t st2 = sp.pia.start2[c] - 1, n2 = variant == 0 ? sp.pia.n2[c] : 0;
+    if (variant == 1) {
+        int32_t s1 = sp.pia.start1[c], e1 = sp.pia.end1[c];
+        n1 = e1 >= s1 ? e1 - s1 + 1 : 0;
+    }
+    int32_t tot = n1 + (n2 > 0 ? n2 : 0);
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    if (tot > 0) {
+        int64_t q = n1 > 0 ? st1 : st2;
+        sx = sp.v0[q]; sy = sp.v1[q]; sz = sp.v2[q];
+    }
+    double n = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, e = 0.0;
+    for (int32_t i = lane; i < tot; i += 64) {
+        int64_t p = i < n1 ? (int64_t)st1 + i : (int64_t)st2 + (i - n1);
+        double w = sp.w[p];
+        double d0 = sp.v0[p] - sx, d1 = sp.v1[p] - sy, d2 = sp.v2[p] - sz;
+        n += w;
+        a0 = fma(w, d0, a0);
+        a1 = fma(w, d1, a1);
+        a2 = fma(w, d2, a2);
+        e = fma(w, d0 * d0 + d1 * d1 + d2 * d2, e);
+    }
+    n = mz_wave_sum(n);
+    a0 = mz_wave_sum(a0);
+    a1 = mz_wave_sum(a1);
+    a2 = mz_wave_sum(a2);
+    e = mz_wave_sum(e);
+    if (lane == 0) {
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0, T = 0.0;
+        if (n > 0.0) {
+            double m0 = a0 / n, m1 = a1 / n, m2 = a2 / n;
+            v0 = sx + m0; v1 = sy + m1; v2 = sz + m2;
+            double E = e - n * (m0 * m0 + m1 * m1 + m2 * m2);
+            if (E < 0.0) E = 0.0;
+            E *= 0.5 * masses[sidx] / (n * MZ_KB);
+            T = (2.0 / 3.0) * E;
+        }
+        int64_t ix = (int64_t)sidx * n_cells + c;
+        out_np[ix] = (double)tot;
+        out_n[ix] = (variant == 1 && ndens) ? n * inv_V_cells[c] : n;
+        out_v[3 * ix] = v0;
+        out_v[3 * ix + 1] = v1;
+        out_v[3 * ix + 2] = v2;
+        out_T[ix] = T;
+    }
+}
+
+int mz_launch_props(mz_store *h, int variant, int ndens) {
+    dim3 grid((h->n_cells + 3) / 4, h->n_species);
+    hipLaunchKernelGGL(k_props, grid, dim3(256), 0, h->stream, mz_views(h), h->n_cells, variant, ndens,
+                       h->d_invV, h->d_masses, h->d_np, h->d_n, h->d_v, h->d_T);
+    MZ_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int mz_compute_props(mz_store *h, int32_t variant, int32_t ndens_not_Np, const double *masses) {
+    MZ_CHECK(variant == 0 || variant == 1, "props: bad variant");
+    MZ_CHECK(masses != nullptr, "props: masses is NULL");
+    MZ_HIP(hipMemcpyAsync(h->d_masses, masses, h->n_species * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    mz_timing_begin(h, "props");
+    int r = mz_launch_props(h, variant, ndens_not_Np);
+    mz_timing_end(h);
+    if (r) return r;
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int mz_get_props(mz_store *h, double *lpa, double *np, double *n, double *v, double *T) {
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    size_t pn = (size_t)h->n_species * h->n_cells;
+    if (np) MZ_HIP(hipMemcpy(np, h->d_np, pn * sizeof(double), hipMemcpyDeviceToHost));
+    if (n) MZ_HIP(hipMemcpy(n, h->d_n, pn * sizeof(double), hipMemcpyDeviceToHost));
+    if (v) MZ_HIP(hipMemcpy(v, h->d_v, pn * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (T) MZ_HIP(hipMemcpy(T, h->d_T, pn * sizeof(double), hipMemcpyDeviceToHost));
+    if (lpa) {
+        for (int s = 1; s <= h->n_species; ++s) {
+            MzCounters c;
+            int r = mz_read_counters(h, s, &c);
+            if (r) return r;
+            lpa[s - 1] = (double)c.length;
+        }
+    }
+    return 0;
+}
