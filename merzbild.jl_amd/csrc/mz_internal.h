@@ -11,7 +11,7 @@
 
 #define MZ_MAX_SPECIES 4
 #define MZ_SORT_CHUNK 1024
-#define MZ_SORT_MAX_CELLS 16384
+#define MZ_SORT_MAX_CELLS 14336   // LDS: 8 KB per chunk + 4 B per cell within 64 KB
 #define MZ_MAX_BOUNCES 64
 #define MZ_NTC_MAX_CANDIDATES 100000000  // per cell and call; larger counts flag an error
 #define MZ_KB 1.380649e-23  // src/constants.jl:4
@@ -63,6 +63,7 @@ struct mz_store {
     double *d_replay[3] = {nullptr, nullptr, nullptr};
     int64_t replay_rows[3] = {0, 0, 0}, replay_cols[3] = {0, 0, 0};
     int32_t *d_err = nullptr;
+    uint32_t *d_ticket = nullptr;  // last-block counter of k_sort_rows (left at 0 by every launch)
     double *d_np = nullptr, *d_n = nullptr, *d_v = nullptr, *d_T = nullptr;
     double *d_V = nullptr, *d_invV = nullptr;
     double *d_masses = nullptr;

@@ -1,24 +1,1198 @@
-// mz_merge.hip -- per-cell merging kernels (octree N:2 and NNLS).
+// mz_merge.hip -- per-cell merging kernels for gfx950: octree N:2
+// (src/merging/merging_octree_N2.jl) and NNLS moment matching
+// (src/merging/merging_nnls.jl, src/merging/nnls.jl).
+//
+// Octree: one workgroup per merged cell.  The particle order list (the
+// reference's particle_indexes_sorted) and the bins live in LDS.  Decisions
+// that the reference takes sequentially -- which bin to refine, the order of
+// the stable 8-way counting sort, the per-octant weight sums, the bin ids
+// and the random sign draws -- are reproduced exactly:
+//   * bounds come from min/max (order independent) and the split velocity
+//     from 0.5 (v_min + v_max), so octants are bit-exact;
+//   * the octant scatter ranks lanes with ballots in position order;
+//   * each child's weight is summed sequentially over its (stably sorted)
+//     segment, which is the reference's accumulation order;
+//   * the refine pick is the first maximum over bins 1..Nbins.
+// Bin means and variances are block reductions (they only feed the new
+// particle values, compared to 1e-12).
 #include "mz_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+#define OCT_MAX_N 2048
+#define OCT_MAX_BINS 512
+
+struct OctArgs {
+    MzSpView sp;
+    MzSpView tmp;             // second particle buffer, scratch for new particle values
+    int32_t threshold, target_np, split, init_bounds, bounds_mode, max_Nbins, max_depth, grid_clamp;
+    double min_x, max_x;
+    MzRngParams rp;
+    int32_t *old_pia;         // [n_cells][5]: start1, n1, start2, n2, n_new (n_new = -1: not merged)
+    int32_t *n_merged;
+    int32_t n_cells;
+};
+
+struct OctBin {
+    double w;
+    double vmin[3], vmax[3];
+    int32_t np, start, end, depth, refine, pad;
+};
+
+__device__ inline int64_t oct_slot(int li, int32_t s1, int32_t n1, int32_t s2) {
+    return li < n1 ? (int64_t)s1 - 1 + li : (int64_t)s2 - 1 + (li - n1);
+}
+
+__device__ inline double blk_min(double v, double *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
+    __syncthreads();
+    return r;
+}
+__device__ inline double blk_max(double v, double *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    __syncthreads();
+    return r;
+}
+__device__ inline double blk_sum(double v, double *red) {
+    v = mz_wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = (red[0] + red[1]) + (red[2] + red[3]);
+    __syncthreads();
+    return r;
+}
+
+// min/max of v over pis[bs..be] (bin_bounds_recompute!, merging_octree_N2.jl:382-418)
+__device__ void oct_bounds(const OctArgs &A, const int16_t *pis, int bs, int be, int32_t s1, int32_t n1, int32_t s2,
+                           double *mn, double *mx, double *red) {
+    for (int d = 0; d < 3; ++d) {
+        const double *vv = d == 0 ? A.sp.v0 : (d == 1 ? A.sp.v1 : A.sp.v2);
+        double a = 9299792458.0, b = -9299792458.0;
+        for (int i = bs + threadIdx.x; i <= be; i += blockDim.x) {
+            double x = vv[oct_slot(pis[i], s1, n1, s2)];
+            a = fmin(a, x);
+            b = fmax(b, x);
+        }
+        mn[d] = blk_min(a, red);
+        mx[d] = blk_max(b, red);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
+    __shared__ int16_t pis[OCT_MAX_N];
+    __shared__ int16_t psort[OCT_MAX_N];
+    __shared__ uint8_t poct[OCT_MAX_N];
+    __shared__ OctBin bins[OCT_MAX_BINS];
+    __shared__ double red[4];
+    __shared__ double vel_mid[3];
+    __shared__ int32_t cnt8[9], start8[9], run8[9], nonempty[9], ne_cnt[9];
+    __shared__ double ndens8[9];
+    __shared__ int32_t Nbins_s, total_post_s, refine_s, stop_s;
+    __shared__ double pick_w[4];
+    __shared__ int32_t pick_id[4];
+
+    const int c = blockIdx.x;
+    A.rp = mz_rng_resolve(A.rp);
+    const int32_t N = A.sp.pia.nloc[c];
+    if (N <= A.threshold) {
+        if (threadIdx.x == 0) A.old_pia[c * 5 + 4] = -1;
+        return;
+    }
+    if (N > OCT_MAX_N) {
+        if (threadIdx.x == 0) {
+            A.old_pia[c * 5 + 4] = -1;
+            atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
+        }
+        return;
+    }
+    const int32_t s1 = A.sp.pia.start1[c], n1 = A.sp.pia.n1[c], s2 = A.sp.pia.start2[c], n2 = A.sp.pia.n2[c];
+    const int max_bins = min(A.max_Nbins, OCT_MAX_BINS);
+    // init_octree! (merging_octree_N2.jl:947-984)
+    for (int i = threadIdx.x; i < N; i += blockDim.x) pis[i] = (int16_t)i;
+    double mn[3], mx[3];
+    if (A.init_bounds == 3) {
+        for (int d = 0; d < 3; ++d) { mn[d] = -299792458.0; mx[d] = 299792458.0; }
+    } else {
+        __syncthreads();
+        oct_bounds(A, pis, 0, N - 1, s1, n1, s2, mn, mx, red);
+        if (A.init_bounds == 2) {
+            for (int d = 0; d < 3; ++d) {
+                double m = fmax(fabs(mn[d]), fabs(mx[d]));
+                mn[d] = -m;
+                mx[d] = m;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        OctBin &b = bins[0];
+        b.w = 1e50;
+        for (int d = 0; d < 3; ++d) { b.vmin[d] = mn[d]; b.vmax[d] = mx[d]; }
+        b.np = N; b.start = 0; b.end = N - 1; b.depth = 0;
+        b.refine = (N > 2) && (A.max_depth > 0);
+        Nbins_s = 1;
+        total_post_s = N >= 2 ? 2 : N;
+        for (int d = 0; d < 3; ++d) vel_mid[d] = 0.0;
+        stop_s = 0;
+    }
+    __syncthreads();
+
+    // compute_octree! (merging_octree_N2.jl:998-1039)
+    for (;;) {
+        // refine pick: first maximum of w over refinable bins
+        {
+            double bw = -1.0;
+            int bid = -1;
+            for (int b = threadIdx.x; b < Nbins_s; b += blockDim.x) {
+                if (bins[b].refine && bins[b].w > bw) { bw = bins[b].w; bid = b; }
+            }
+            // wave argmax with lowest id on ties
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                double ow = __shfl_xor(bw, o, 64);
+                int oi = __shfl_xor(bid, o, 64);
+                if (oi >= 0 && (bid < 0 || ow > bw || (ow == bw && oi < bid))) { bw = ow; bid = oi; }
+            }
+            if ((threadIdx.x & 63) == 0) { pick_w[threadIdx.x >> 6] = bw; pick_id[threadIdx.x >> 6] = bid; }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double w0 = -1.0;
+                int i0 = -1;
+                for (int k = 0; k < 4; ++k) {
+                    int oi = pick_id[k];
+                    double ow = pick_w[k];
+                    if (oi >= 0 && (i0 < 0 || ow > w0 || (ow == w0 && oi < i0))) { w0 = ow; i0 = oi; }
+                }
+                refine_s = i0;
+                if (i0 < 0 || total_post_s + 14 > A.target_np) stop_s = 1;
+            }
+            __syncthreads();
+        }
+        if (stop_s) break;
+        // split_bin! (merging_octree_N2.jl:503-633)
+        const int b = refine_s;
+        const int bs = bins[b].start, be = bins[b].end;
+        const int depth = bins[b].depth;
+        if (A.bounds_mode == 2) {
+            double bmn[3], bmx[3];
+            oct_bounds(A, pis, bs, be, s1, n1, s2, bmn, bmx, red);
+            if (threadIdx.x == 0 && be >= bs)
+                for (int d = 0; d < 3; ++d) { bins[b].vmin[d] = bmn[d]; bins[b].vmax[d] = bmx[d]; }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            if (A.split == 1) {
+                for (int d = 0; d < 3; ++d) vel_mid[d] = 0.5 * (bins[b].vmin[d] + bins[b].vmax[d]);
+            } else {
+                // compute_v_mean! keeps the previous middle velocity in the sum
+                double nt = 0.0;
+                for (int i = bs; i <= be; ++i) {
+                    int64_t p = oct_slot(pis[i], s1, n1, s2);
+                    double w = A.sp.w[p];
+                    nt += w;
+                    vel_mid[0] = vel_mid[0] + w * A.sp.v0[p];
+                    vel_mid[1] = vel_mid[1] + w * A.sp.v1[p];
+                    vel_mid[2] = vel_mid[2] + w * A.sp.v2[p];
+                }
+                for (int d = 0; d < 3; ++d) vel_mid[d] = vel_mid[d] / nt;
+            }
+            for (int o = 0; o < 9; ++o) { cnt8[o] = 0; ndens8[o] = 0.0; }
+        }
+        __syncthreads();
+        for (int i = bs + threadIdx.x; i <= be; i += blockDim.x) {
+            int64_t p = oct_slot(pis[i], s1, n1, s2);
+            int o = 1;                        // compute_octant (:304-316): strict >
+            if (A.sp.v0[p] > vel_mid[0]) o += 1;
+            if (A.sp.v1[p] > vel_mid[1]) o += 2;
+            if (A.sp.v2[p] > vel_mid[2]) o += 4;
+            poct[i - bs] = (uint8_t)o;
+            atomicAdd(&cnt8[o], 1);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int acc = 0;
+            for (int o = 1; o <= 8; ++o) { start8[o] = bs + acc; run8[o] = 0; acc += cnt8[o]; }
+        }
+        __syncthreads();
+        // stable scatter by octant, in position order (wave 0)
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            const uint64_t lt = mz_lanemask_lt();
+            volatile int32_t *vrun = run8;
+            for (int i0 = bs; i0 <= be; i0 += 64) {
+                const int i = i0 + lane;
+                const bool valid = i <= be;
+                const int o = valid ? poct[i - bs] : 0;
+                for (int q = 1; q <= 8; ++q) {
+                    uint64_t m = __ballot(valid && o == q);
+                    if (m == 0) continue;
+                    const int32_t r0 = vrun[q];
+                    if (valid && o == q) psort[start8[q] + r0 + __popcll(m & lt)] = pis[i];
+                    vrun[q] = r0 + __popcll(m);
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = bs + threadIdx.x; i <= be; i += blockDim.x) pis[i] = psort[i];
+        __syncthreads();
+        // per-octant weights, summed in the reference's order (one lane per octant)
+        if (threadIdx.x >= 1 && threadIdx.x <= 8) {
+            const int o = threadIdx.x;
+            double s = 0.0;
+            for (int i = start8[o]; i < start8[o] + cnt8[o]; ++i) s += A.sp.w[oct_slot(pis[i], s1, n1, s2)];
+            ndens8[o] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int n_ne = 0;
+            for (int o = 1; o <= 8; ++o)
+                if (cnt8[o] > 0) { ++n_ne; nonempty[n_ne] = o; ne_cnt[n_ne] = cnt8[o]; }
+            const int Nb = Nbins_s;
+            double pmin[3], pmax[3];
+            for (int d = 0; d < 3; ++d) { pmin[d] = bins[b].vmin[d]; pmax[d] = bins[b].vmax[d]; }
+            int prev_end = bs - 1;
+            total_post_s -= 2;
+            for (int k = 1; k <= n_ne; ++k) {
+                const int bi = k == 1 ? b : Nb + k - 2;   // child 1 keeps the id (get_new_bin_id)
+                if (bi >= OCT_MAX_BINS) { atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE); stop_s = 1; break; }
+                OctBin &nb = bins[bi];
+                nb.start = prev_end + 1;
+                nb.end = nb.start + ne_cnt[k] - 1;
+                prev_end = nb.end;
+                const int oc = nonempty[k];
+                if (A.bounds_mode == 1) {   // bin_bounds_inherit! (:341-367)
+                    const int sx = (oc - 1) & 1, sy = ((oc - 1) >> 1) & 1, sz = ((oc - 1) >> 2) & 1;
+                    nb.vmin[0] = sx ? vel_mid[0] : pmin[0]; nb.vmax[0] = sx ? pmax[0] : vel_mid[0];
+                    nb.vmin[1] = sy ? vel_mid[1] : pmin[1]; nb.vmax[1] = sy ? pmax[1] : vel_mid[1];
+                    nb.vmin[2] = sz ? vel_mid[2] : pmin[2]; nb.vmax[2] = sz ? pmax[2] : vel_mid[2];
+                }
+                nb.np = ne_cnt[k];
+                nb.w = ndens8[oc];
+                nb.depth = depth + 1;
+                total_post_s += nb.np >= 2 ? 2 : nb.np;
+                nb.refine = (nb.np > 2) && (nb.depth < A.max_depth);
+            }
+            Nbins_s = Nb + n_ne - 1;
+            if (Nbins_s + 7 > A.max_Nbins) stop_s = 1;
+            else if (Nbins_s + 7 > max_bins) {   // LDS bin budget below the caller's max_Nbins
+                atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
+                stop_s = 1;
+            }
+        }
+        __syncthreads();
+        if (stop_s) break;
+    }
+    const int Nbins = Nbins_s;
+    if (Nbins == 1 && threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < N; ++i) s += A.sp.w[oct_slot(pis[i], s1, n1, s2)];
+        bins[0].w = s;
+    }
+    __syncthreads();
+
+    // compute_bin_props! (:646-700) and compute_new_particles! in bin order:
+    // block reductions give each bin's means and spreads, thread 0 draws the
+    // signs and writes the new particles to the second buffer at the cell's
+    // slots (the originals are still needed for later bins).
+    __shared__ int32_t cur_s;
+    if (threadIdx.x == 0) cur_s = 0;
+    __syncthreads();
+    MzStream st = mz_stream((uint64_t)c);
+    for (int b = 0; b < Nbins; ++b) {
+        const int32_t bnp = bins[b].w == 0.0 ? 0 : bins[b].np;
+        if (bnp <= 0) continue;
+        const int bs = bins[b].start, be = bins[b].end;
+        if (bnp <= 2) {
+            if (threadIdx.x == 0) {
+                for (int k = 0; k < bnp; ++k) {
+                    int64_t src = oct_slot(pis[bs + k], s1, n1, s2);
+                    int64_t dst = oct_slot(cur_s + k, s1, n1, s2);
+                    A.tmp.x0[dst] = A.sp.x0[src]; A.tmp.x1[dst] = A.sp.x1[src]; A.tmp.x2[dst] = A.sp.x2[src];
+                    A.tmp.v0[dst] = A.sp.v0[src]; A.tmp.v1[dst] = A.sp.v1[src]; A.tmp.v2[dst] = A.sp.v2[src];
+                    A.tmp.w[dst] = A.sp.w[src];
+                }
+                cur_s += bnp;
+            }
+            __syncthreads();
+            continue;
+        }
+        const double W = bins[b].w, inv_w = 1.0 / W;
+        double sv[3] = {0, 0, 0}, sx[3] = {0, 0, 0};
+        for (int i = bs + threadIdx.x; i <= be; i += blockDim.x) {
+            int64_t p = oct_slot(pis[i], s1, n1, s2);
+            double w = A.sp.w[p];
+            sv[0] = fma(w, A.sp.v0[p], sv[0]); sv[1] = fma(w, A.sp.v1[p], sv[1]); sv[2] = fma(w, A.sp.v2[p], sv[2]);
+            sx[0] = fma(w, A.sp.x0[p], sx[0]); sx[1] = fma(w, A.sp.x1[p], sx[1]); sx[2] = fma(w, A.sp.x2[p], sx[2]);
+        }
+        double vm[3], xm[3];
+        for (int d = 0; d < 3; ++d) {
+            vm[d] = blk_sum(sv[d], red) * inv_w;
+            xm[d] = blk_sum(sx[d], red) * inv_w;
+        }
+        double qv[3] = {0, 0, 0}, qx[3] = {0, 0, 0};
+        for (int i = bs + threadIdx.x; i <= be; i += blockDim.x) {
+            int64_t p = oct_slot(pis[i], s1, n1, s2);
+            double w = A.sp.w[p];
+            double a0 = A.sp.v0[p] - vm[0], a1 = A.sp.v1[p] - vm[1], a2 = A.sp.v2[p] - vm[2];
+            double b0 = A.sp.x0[p] - xm[0], b1 = A.sp.x1[p] - xm[1], b2 = A.sp.x2[p] - xm[2];
+            qv[0] = fma(w, a0 * a0, qv[0]); qv[1] = fma(w, a1 * a1, qv[1]); qv[2] = fma(w, a2 * a2, qv[2]);
+            qx[0] = fma(w, b0 * b0, qx[0]); qx[1] = fma(w, b1 * b1, qx[1]); qx[2] = fma(w, b2 * b2, qx[2]);
+        }
+        double sdv[3], sdx[3];
+        for (int d = 0; d < 3; ++d) {
+            sdv[d] = sqrt(blk_sum(qv[d], red) * inv_w);
+            sdx[d] = sqrt(blk_sum(qx[d], red) * inv_w);
+        }
+        if (threadIdx.x == 0) {
+            // compute_new_particles! (:736-813, 830-933): 3 + 3 sign draws
+            double dv[3], dx[3];
+            for (int d = 0; d < 3; ++d) dv[d] = mz_rand(A.rp, st) < 0.5 ? -1.0 : 1.0;
+            for (int d = 0; d < 3; ++d) dx[d] = mz_rand(A.rp, st) < 0.5 ? -1.0 : 1.0;
+            const double w1 = 0.5 * W;
+            for (int k = 0; k < 2; ++k) {
+                const double sg = k == 0 ? 1.0 : -1.0;
+                int64_t dst = oct_slot(cur_s + k, s1, n1, s2);
+                double x0 = xm[0] + sg * dx[0] * sdx[0];
+                if (A.grid_clamp) x0 = x0 < A.min_x ? A.min_x : (x0 > A.max_x ? A.max_x : x0);
+                A.tmp.x0[dst] = x0;
+                A.tmp.x1[dst] = xm[1] + sg * dx[1] * sdx[1];
+                A.tmp.x2[dst] = xm[2] + sg * dx[2] * sdx[2];
+                A.tmp.v0[dst] = vm[0] + sg * dv[0] * sdv[0];
+                A.tmp.v1[dst] = vm[1] + sg * dv[1] * sdv[1];
+                A.tmp.v2[dst] = vm[2] + sg * dv[2] * sdv[2];
+                A.tmp.w[dst] = w1;
+            }
+            cur_s += 2;
+        }
+        __syncthreads();
+    }
+    const int n_new = cur_s;
+    // move the new particles into place, zero the weights of deleted positions
+    for (int li = threadIdx.x; li < N; li += blockDim.x) {
+        int64_t p = oct_slot(li, s1, n1, s2);
+        if (li < n_new) {
+            A.sp.x0[p] = A.tmp.x0[p]; A.sp.x1[p] = A.tmp.x1[p]; A.sp.x2[p] = A.tmp.x2[p];
+            A.sp.v0[p] = A.tmp.v0[p]; A.sp.v1[p] = A.tmp.v1[p]; A.sp.v2[p] = A.tmp.v2[p];
+            A.sp.w[p] = A.tmp.w[p];
+        } else {
+            A.sp.w[p] = 0.0;
+        }
+    }
+    if (threadIdx.x == 0) {
+        int32_t *op = A.old_pia + c * 5;
+        op[0] = s1; op[1] = n1; op[2] = s2; op[3] = n2; op[4] = n_new;
+        // delete_particle_end! n_del times (particles.jl:478-566): group 2 first
+        int32_t n_del = N - n_new;
+        int32_t d2 = min(n_del, n2), d1 = n_del - d2;
+        int32_t nn2 = n2 - d2, nn1 = n1 - d1;
+        A.sp.pia.n2[c] = nn2;
+        A.sp.pia.end2[c] = nn2 > 0 ? s2 + nn2 - 1 : -1;
+        if (nn2 == 0) A.sp.pia.start2[c] = 0;
+        A.sp.pia.n1[c] = nn1;
+        A.sp.pia.end1[c] = nn1 > 0 ? s1 + nn1 - 1 : -1;
+        if (nn1 == 0) A.sp.pia.start1[c] = 0;
+        A.sp.pia.nloc[c] = nn1 + nn2;
+        // the reference clears contiguous unless the deletions stay inside the
+        // last cell's group 2 (merging_octree_N2.jl:801-805)
+        if (!(c == A.n_cells - 1) || (n_del > n2)) A.sp.ctr->contiguous = 0;
+        atomicAdd(A.n_merged, 1);
+    }
+}
+
+// Free-id pushes in cell order: cell c pushes the ids of its deleted
+// positions, last first (delete_particle_end! pushes pv.index[end]).
+__global__ void k_merge_push_ids(MzSpView sp, const int32_t *__restrict__ old_pia, const int64_t *__restrict__ off,
+                                 const int64_t *__restrict__ total, int32_t nbuf0, int32_t ntot0, int32_t n_cells) {
+    const int c = blockIdx.x;
+    if (c < n_cells) {
+        const int32_t *op = old_pia + c * 5;
+        const int32_t n_new = op[4];
+        if (n_new >= 0) {
+            const int32_t N = op[1] + op[3];
+            const int32_t n_del = N - n_new;
+            for (int j = threadIdx.x; j < n_del; j += blockDim.x) {
+                const int li = N - 1 - j;
+                const int64_t p = oct_slot(li, op[0], op[1], op[2]);
+                const int64_t q = (int64_t)nbuf0 + off[c] + j;
+                if (q < sp.bufcap) sp.buffer[q] = sp.id[p];
+                else atomicOr(&sp.ctr->err, MZ_ERR_BUFFER);
+            }
+        }
+    }
+    if (c == 0 && threadIdx.x == 0) {
+        sp.ctr->nbuffer = nbuf0 + (int32_t)*total;
+        sp.ctr->n_total = ntot0 - (int32_t)*total;
+        sp.ctr->deleted = (int32_t)*total;
+    }
+}
+
+__global__ void k_merge_ndel(const int32_t *__restrict__ old_pia, int32_t n_cells, int32_t *__restrict__ ndel) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n_cells) {
+        const int32_t *op = old_pia + c * 5;
+        ndel[c] = op[4] >= 0 ? op[1] + op[3] - op[4] : 0;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_i32m(const int32_t *__restrict__ in, int64_t n, int64_t *__restrict__ out,
+                                                    int64_t *__restrict__ total) {
+    __shared__ int64_t wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t lo = t * per, hi = min(n, lo + per);
+    int64_t s = 0;
+    for (int64_t i = lo; i < hi; ++i) s += in[i];
+    int64_t x = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    if (t < 16) {
+        int64_t a = wsum[t], b = a;
+        for (int o = 1; o < 16; o <<= 1) {
+            int64_t y = __shfl_up(b, o, 16);
+            if ((t & 15) >= o) b += y;
+        }
+        wsum[t] = b - a;
+    }
+    __syncthreads();
+    int64_t run = x - s + wsum[wv];
+    for (int64_t i = lo; i < hi; ++i) {
+        int64_t v = in[i];
+        out[i] = run;
+        run += v;
+    }
+    if (t == 1023) *total = run;
+}
 
 extern "C" int mz_merge_octree(mz_store *h, int32_t species, int32_t threshold, int32_t target_np, int32_t split,
                                int32_t init_bounds, int32_t bounds_mode, int32_t max_Nbins, int32_t max_depth,
                                int32_t grid_clamp, int32_t squash, uint64_t tag, int32_t rng_mode,
                                int64_t *n_merged) {
-    (void)h; (void)species; (void)threshold; (void)target_np; (void)split; (void)init_bounds;
-    (void)bounds_mode; (void)max_Nbins; (void)max_depth; (void)grid_clamp; (void)squash; (void)tag;
-    (void)rng_mode; (void)n_merged;
-    mz_set_error("merge_octree: not available in this build");
-    return -3;
+    MZ_CHECK(species >= 1 && species <= h->n_species, "merge_octree: bad species");
+    MZ_CHECK(split == 1 || split == 2, "merge_octree: split must be 1 (mid) or 2 (mean)");
+    MZ_CHECK(init_bounds >= 1 && init_bounds <= 3, "merge_octree: bad init_bounds");
+    MZ_CHECK(bounds_mode == 1 || bounds_mode == 2, "merge_octree: bad bounds_mode");
+    MZ_CHECK(max_Nbins >= 8, "merge_octree: max_Nbins must be at least 8");
+    MZ_CHECK(!grid_clamp || h->has_grid, "merge_octree: grid clamp needs a grid");
+    MZ_CHECK(rng_mode == 0 || (rng_mode == 1 && h->d_replay[2]), "merge_octree: replay table not set");
+    MzSpecies &s = h->sp[species - 1];
+    const int nc = h->n_cells;
+    int32_t *old_pia = (int32_t *)mz_scratch(h, "oct_oldpia", (size_t)nc * 5 * sizeof(int32_t));
+    int32_t *cnt = (int32_t *)mz_scratch(h, "oct_cnt", sizeof(int32_t));
+    int32_t *ndel = (int32_t *)mz_scratch(h, "oct_ndel", (size_t)nc * sizeof(int32_t));
+    int64_t *off = (int64_t *)mz_scratch(h, "oct_off", (size_t)(nc + 1) * sizeof(int64_t));
+    MZ_CHECK(old_pia && cnt && ndel && off, "merge_octree: scratch allocation failed");
+    MzCounters snap;
+    int r = mz_read_counters(h, species, &snap);
+    if (r) return r;
+    MZ_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t), h->stream));
+    OctArgs A{};
+    A.sp = mz_view(s);
+    A.tmp = A.sp;
+    A.tmp.x0 = s.xb[0]; A.tmp.x1 = s.xb[1]; A.tmp.x2 = s.xb[2];
+    A.tmp.v0 = s.vb[0]; A.tmp.v1 = s.vb[1]; A.tmp.v2 = s.vb[2];
+    A.tmp.w = s.wb;
+    A.threshold = threshold; A.target_np = target_np; A.split = split; A.init_bounds = init_bounds;
+    A.bounds_mode = bounds_mode; A.max_Nbins = max_Nbins; A.max_depth = max_depth; A.grid_clamp = grid_clamp;
+    A.min_x = h->grid.min_x; A.max_x = h->grid.max_x;
+    A.rp = mz_rng_params(h, 2, tag, rng_mode);
+    A.old_pia = old_pia;
+    A.n_merged = cnt;
+    A.n_cells = nc;
+    mz_timing_begin(h, "merge_octree");
+    hipLaunchKernelGGL(k_octree_merge, dim3(nc), dim3(256), 0, h->stream, A);
+    hipLaunchKernelGGL(k_merge_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, old_pia, nc, ndel);
+    hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
+    hipLaunchKernelGGL(k_merge_push_ids, dim3(nc), dim3(64), 0, h->stream, A.sp, old_pia, off, off + nc,
+                       snap.nbuffer, snap.n_total, nc);
+    mz_timing_end(h);
+    MZ_HIP(hipGetLastError());
+    if (n_merged) {
+        int32_t v = 0;
+        MZ_HIP(hipMemcpyAsync(&v, cnt, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+        MZ_HIP(hipStreamSynchronize(h->stream));
+        *n_merged = v;
+    }
+    if (squash) return mz_squash(h, species);
+    return 0;
 }
+
+// ===========================================================================
+// NNLS merging (merging_nnls.jl:367-613, 949-1038, 1178-1276, 1334-1510) with
+// the Lawson-Hanson solver (nnls.jl:26-548), one workgroup per merged cell.
+//
+// The solver's arithmetic is +, *, / and sqrt only.  Every loop the
+// reference runs over columns is spread over the threads with one thread per
+// column, and each column's loop over rows keeps the reference's order; the
+// sums over particles (total weight, mean, variances, right-hand side) are
+// sequential; contraction into fma is off.  The pivot choices and results
+// therefore equal the oracle's bit for bit.  The LHS matrix lives in global
+// scratch, row-major [row][column] so that the column threads read
+// consecutive addresses.
+
+#define NNLS_MAX_ROWS 64
+#define NNLS_MAX_VMULT 4
+
+struct NnlsArgs {
+    MzSpView sp, tmp;
+    int32_t m;                     // rows (velocity moments; no spatial moments)
+    int32_t mim[NNLS_MAX_ROWS][3];
+    int32_t tot_order[NNLS_MAX_ROWS];
+    double vref, inv_vref;
+    int32_t scaling_vref, centered, n_vmult, iteration_mult;
+    double vmult[NNLS_MAX_VMULT];
+    double max_err, w_threshold;
+    const int32_t *cells;          // batch: cells to merge (0-based)
+    const int64_t *aoff;           // batch: offset of each cell's scratch (doubles)
+    double *scratch;
+    int32_t *status;               // [n_cells]
+    int32_t *old_pia;              // [n_cells][5]
+    int32_t n_cells;
+};
+
+__device__ inline double nnls_ipow(double a, int p) {
+    double r = 1.0;
+    for (int i = 0; i < p; ++i) r *= a;
+    return r;
+}
+
+#pragma clang fp contract(off)
+
+__device__ inline void nnls_argmax(double v, int i, double *rw, int *ri, double &bw, int &bi) {
+    // first maximum: larger value wins, ties go to the smaller index
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        double ov = __shfl_xor(v, o, 64);
+        int oi = __shfl_xor(i, o, 64);
+        if (oi >= 0 && (i < 0 || ov > v || (ov == v && oi < i))) { v = ov; i = oi; }
+    }
+    if ((threadIdx.x & 63) == 0) { rw[threadIdx.x >> 6] = v; ri[threadIdx.x >> 6] = i; }
+    __syncthreads();
+    double w0 = -1.0;
+    int i0 = -1;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+        if (ri[k] >= 0 && (i0 < 0 || rw[k] > w0 || (rw[k] == w0 && ri[k] < i0))) { w0 = rw[k]; i0 = ri[k]; }
+    }
+    bw = w0;
+    bi = i0;
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_nnls_merge(NnlsArgs A) {
+    __shared__ double b[NNLS_MAX_ROWS], zz[NNLS_MAX_ROWS], rhs[NNLS_MAX_ROWS], rowscale[NNLS_MAX_ROWS];
+    __shared__ double st_d[24];
+    __shared__ int32_t st_i[16];
+    __shared__ double rw[4];
+    __shared__ int ri[4];
+    const int c = A.cells[blockIdx.x];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int32_t s1 = A.sp.pia.start1[c], n1 = A.sp.pia.n1[c], s2 = A.sp.pia.start2[c], n2 = A.sp.pia.n2[c];
+    const int32_t N = n1 + n2;
+    const int m = A.m;
+    const int n_add = (A.centered ? 1 : 0) + 8 * A.n_vmult;
+    const int n = N + n_add;
+    double *Am = A.scratch + A.aoff[blockIdx.x];       // [m][n] row-major
+    double *xs = Am + (int64_t)m * n;                   // x[n]
+    double *ws = xs + n;                                // w[n] (dual)
+    double *cn = ws + n;                                // column norms[n]
+    int32_t *idx = (int32_t *)(cn + n);                 // idx[n]
+#define AM(r, cc) Am[(int64_t)((r) - 1) * n + ((cc) - 1)]
+    // ---- compute_w_total_v0! (sequential sums, exact min/max)
+    if (tid == 0) {
+        double W = 0.0, v0[3] = {0, 0, 0}, x0[3] = {0, 0, 0};
+        double mn[3] = {299792458.0, 299792458.0, 299792458.0};
+        double mx[3] = {-299792458.0, -299792458.0, -299792458.0};
+        for (int li = 0; li < N; ++li) {
+            int64_t p = oct_slot(li, s1, n1, s2);
+            double w = A.sp.w[p], v[3] = {A.sp.v0[p], A.sp.v1[p], A.sp.v2[p]};
+            double xx[3] = {A.sp.x0[p], A.sp.x1[p], A.sp.x2[p]};
+            W += w;
+            for (int d = 0; d < 3; ++d) {
+                v0[d] = v0[d] + w * v[d];
+                x0[d] = x0[d] + w * xx[d];
+                if (v[d] > mx[d]) mx[d] = v[d];
+                if (v[d] < mn[d]) mn[d] = v[d];
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            v0[d] = v0[d] / W;
+            x0[d] = x0[d] / W;
+            st_d[d] = v0[d];
+            st_d[3 + d] = x0[d];
+            st_d[6 + d] = mn[d] - v0[d];
+            st_d[9 + d] = mx[d] - v0[d];
+        }
+        st_d[12] = W;
+    }
+    __syncthreads();
+    const double W = st_d[12];
+    const double v0x = st_d[0], v0y = st_d[1], v0z = st_d[2];
+    // ---- variances (compute_lhs_and_rhs!, :539-548): one accumulator per thread
+    if (tid < 3) {
+        double E = 0.0;
+        for (int li = 0; li < N; ++li) {
+            int64_t p = oct_slot(li, s1, n1, s2);
+            double v = tid == 0 ? A.sp.v0[p] : (tid == 1 ? A.sp.v1[p] : A.sp.v2[p]);
+            double a = v - st_d[tid];
+            E = E + A.sp.w[p] * (a * a);
+        }
+        st_d[13 + tid] = sqrt(E / W);
+    }
+    // ---- right-hand side: one row per thread, particles in order
+    for (int q = tid; q < m; q += nt) {
+        double r = 0.0;
+        for (int li = 0; li < N; ++li) {
+            int64_t p = oct_slot(li, s1, n1, s2);
+            double t = nnls_ipow(A.sp.v0[p] - v0x, A.mim[q][0]) * nnls_ipow(A.sp.v1[p] - v0y, A.mim[q][1]) *
+                       nnls_ipow(A.sp.v2[p] - v0z, A.mim[q][2]);
+            r = r + A.sp.w[p] * t;
+        }
+        rhs[q] = r / W;
+    }
+    __syncthreads();
+    const double Ex = st_d[13], Ey = st_d[14], Ez = st_d[15];
+    // ---- row scaling (scale_lhs_rhs_variance! / _vref!, :1178-1215)
+    if (tid < m) {
+        double rv;
+        if (!A.scaling_vref) {
+            double iex = Ex > 1e-6 ? 1.0 / Ex : A.inv_vref;
+            double iey = Ey > 1e-6 ? 1.0 / Ey : A.inv_vref;
+            double iez = Ez > 1e-6 ? 1.0 / Ez : A.inv_vref;
+            rv = nnls_ipow(iex, A.mim[tid][0]) * nnls_ipow(iey, A.mim[tid][1]) * nnls_ipow(iez, A.mim[tid][2]);
+        } else {
+            rv = nnls_ipow(A.inv_vref, A.tot_order[tid]);
+        }
+        rowscale[tid] = rv;
+        b[tid] = rhs[tid] * rv;
+    }
+    __syncthreads();
+    // ---- columns: particles, centre, octant particles; scaled, unit norm
+    for (int j = tid; j < n; j += nt) {
+        double cx, cy, cz;
+        bool zero_col = false;
+        if (j < N) {
+            int64_t p = oct_slot(j, s1, n1, s2);
+            cx = A.sp.v0[p] - v0x; cy = A.sp.v1[p] - v0y; cz = A.sp.v2[p] - v0z;
+        } else {
+            int f = j - N;
+            if (A.centered && f == 0) {
+                zero_col = true;
+                cx = cy = cz = 0.0;
+            } else {
+                f -= A.centered ? 1 : 0;
+                const int vm = f / 8, o = f % 8 + 1;
+                const double mult = A.vmult[vm];
+                const double sx = (o % 2 == 1) ? -1.0 : 1.0;
+                const double sy = (o == 3 || o == 4 || o == 7 || o == 8) ? 1.0 : -1.0;
+                const double sz = o >= 5 ? 1.0 : -1.0;
+                double e3[3] = {Ex, Ey, Ez}, s3[3] = {sx, sy, sz}, r3[3];
+                for (int d = 0; d < 3; ++d) {
+                    double ms = mult * (s3[d] * e3[d]);
+                    r3[d] = ms > st_d[9 + d] ? st_d[9 + d] * mult : (ms < st_d[6 + d] ? st_d[6 + d] * mult : ms);
+                }
+                cx = r3[0]; cy = r3[1]; cz = r3[2];
+            }
+        }
+        double nn = 0.0;
+        for (int q = 0; q < m; ++q) {
+            double t = zero_col ? 0.0
+                                : nnls_ipow(cx, A.mim[q][0]) * nnls_ipow(cy, A.mim[q][1]) * nnls_ipow(cz, A.mim[q][2]);
+            t *= rowscale[q];
+            AM(q + 1, j + 1) = t;
+            nn = nn + t * t;
+        }
+        nn = sqrt(nn);
+        nn = nn > 1e-15 ? nn : 1.0;
+        const double inv = 1.0 / nn;
+        cn[j] = inv;
+        for (int q = 0; q < m; ++q) AM(q + 1, j + 1) *= inv;
+        xs[j] = 0.0;
+        ws[j] = 0.0;
+        idx[j] = j + 1;
+    }
+    __syncthreads();
+
+    // ---- Lawson-Hanson solve!(work, iteration_mult * n, 1e-14)
+    const double kkt_tol = 1e-14, factor = 0.01;
+    const int64_t max_iter = (int64_t)A.iteration_mult * n;
+    // shared solver state: 0 iz1, 1 iz2, 2 nsetp, 3 j, 4 jj, 5 iz, 6 terminated, 7 mode, 8 brk, 9 i
+    if (tid == 0) {
+        st_i[0] = 1; st_i[1] = n; st_i[2] = 0; st_i[3] = 0; st_i[4] = 0; st_i[5] = 0;
+        st_i[6] = 0; st_i[7] = 1; st_i[8] = 0; st_i[9] = 0;
+        st_d[16] = 0.0;   // up
+        st_d[17] = 0.0;   // iter (as double)
+    }
+    __syncthreads();
+    for (;;) {
+        if (st_i[0] > st_i[1] || st_i[2] >= m) break;
+        // dual vector over the zero set
+        {
+            const int iz1 = st_i[0], iz2 = st_i[1], nsetp = st_i[2];
+            for (int i = iz1 + tid; i <= iz2; i += nt) {
+                const int idxi = idx[i - 1];
+                double sm = 0.0;
+                for (int l = nsetp + 1; l <= m; ++l) sm = sm + AM(l, idxi) * b[l - 1];
+                ws[idxi - 1] = sm;
+            }
+        }
+        __syncthreads();
+        bool terminated = false;
+        for (;;) {
+            // largest positive dual (first maximum in idx order)
+            const int iz1 = st_i[0], iz2 = st_i[1];
+            double wmax = 0.0;
+            int izmax = -1;
+            for (int i = iz1 + tid; i <= iz2; i += nt) {
+                const double wv = ws[idx[i - 1] - 1];
+                if (wv > wmax || (izmax < 0 && wv > 0.0)) {
+                    if (izmax < 0 || wv > wmax) { wmax = wv; izmax = i; }
+                }
+            }
+            double bw;
+            int bi;
+            nnls_argmax(izmax >= 0 ? wmax : -1.0, izmax, rw, ri, bw, bi);
+            if (bi < 0 || bw <= kkt_tol) { terminated = true; break; }
+            if (tid == 0) {
+                const int nsetp = st_i[2];
+                const int iz = bi, j = idx[iz - 1];
+                st_i[5] = iz;
+                st_i[3] = j;
+                const double Asave = AM(nsetp + 1, j);
+                // construct_householder! on A[nsetp+1:m, j]
+                double up = st_d[16];
+                const int len = m - nsetp;
+                if (len > 1) {
+                    double cl = 0.0;
+                    for (int l = nsetp + 1; l <= m; ++l) cl = fmax(cl, fabs(AM(l, j)));
+                    double clinv = 1.0 / cl, sm = 0.0;
+                    for (int l = nsetp + 1; l <= m; ++l) { double t = AM(l, j) * clinv; sm += t * t; }
+                    cl *= sqrt(sm);
+                    if (AM(nsetp + 1, j) > 0) cl = -cl;
+                    up = AM(nsetp + 1, j) - cl;
+                    AM(nsetp + 1, j) = cl;
+                }
+                st_d[16] = up;
+                double unorm = 0.0;
+                for (int l = 1; l <= nsetp; ++l) unorm += AM(l, j) * AM(l, j);
+                unorm = sqrt(unorm);
+                int accept = 0;
+                if (((unorm + fabs(AM(nsetp + 1, j)) * factor) - unorm) > 0) {
+                    for (int q = 0; q < m; ++q) zz[q] = b[q];
+                    // apply_householder!(A[nsetp+1:m, j], up, zz[nsetp+1:m])
+                    if (len > 1) {
+                        double bb = up * AM(nsetp + 1, j);
+                        if (bb < 0) {
+                            bb = 1.0 / bb;
+                            double sm = zz[nsetp] * up;
+                            for (int l = 2; l <= len; ++l) sm = sm + zz[nsetp + l - 1] * AM(nsetp + l, j);
+                            if (sm != 0) {
+                                sm *= bb;
+                                zz[nsetp] = zz[nsetp] + sm * up;
+                                for (int l = 2; l <= len; ++l) zz[nsetp + l - 1] = zz[nsetp + l - 1] + sm * AM(nsetp + l, j);
+                            }
+                        }
+                    }
+                    const double ztest = zz[nsetp] / AM(nsetp + 1, j);
+                    if (ztest > 0) accept = 1;
+                }
+                if (!accept) {
+                    AM(nsetp + 1, j) = Asave;
+                    ws[j - 1] = 0;
+                }
+                st_i[8] = accept;
+            }
+            __syncthreads();
+            if (st_i[8]) break;
+        }
+        if (terminated) break;
+        // move j from Z to P, transform the remaining columns
+        if (tid == 0) {
+            for (int q = 0; q < m; ++q) b[q] = zz[q];
+            const int iz = st_i[5], iz1 = st_i[0], j = st_i[3];
+            idx[iz - 1] = idx[iz1 - 1];
+            idx[iz1 - 1] = j;
+            st_i[0] = iz1 + 1;
+            st_i[2] = st_i[2] + 1;
+        }
+        __syncthreads();
+        {
+            const int iz1 = st_i[0], iz2 = st_i[1], nsetp = st_i[2], j = st_i[3];
+            const double up = st_d[16];
+            const int len = m - nsetp + 1;
+            for (int jz = iz1 + tid; jz <= iz2; jz += nt) {
+                const int jj = idx[jz - 1];
+                if (len > 1) {
+                    double bb = up * AM(nsetp, j);
+                    if (bb < 0) {
+                        bb = 1.0 / bb;
+                        double sm = AM(nsetp, jj) * up;
+                        for (int l = 2; l <= len; ++l) sm = sm + AM(nsetp + l - 1, jj) * AM(nsetp + l - 1, j);
+                        if (sm != 0) {
+                            sm *= bb;
+                            AM(nsetp, jj) = AM(nsetp, jj) + sm * up;
+                            for (int l = 2; l <= len; ++l)
+                                AM(nsetp + l - 1, jj) = AM(nsetp + l - 1, jj) + sm * AM(nsetp + l - 1, j);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const int nsetp = st_i[2], j = st_i[3];
+            if (nsetp != m)
+                for (int l = nsetp + 1; l <= m; ++l) AM(l, j) = 0;
+            ws[j - 1] = 0;
+            // solve_triangular_system!
+            int jj = st_i[4];
+            for (int l = 1; l <= nsetp; ++l) {
+                const int ip = nsetp + 1 - l;
+                if (l != 1)
+                    for (int ii = 1; ii <= ip; ++ii) zz[ii - 1] = zz[ii - 1] - AM(ii, jj) * zz[ip];
+                jj = idx[ip - 1];
+                zz[ip - 1] /= AM(ip, jj);
+            }
+            st_i[4] = jj;
+        }
+        __syncthreads();
+        // secondary loop
+        for (;;) {
+            if (tid == 0) {
+                st_i[8] = 0;   // 1: leave the secondary loop, 2: iteration cap
+                st_d[17] += 1.0;
+                if (st_d[17] > (double)max_iter) {
+                    st_i[7] = 3;
+                    st_i[8] = 2;
+                } else {
+                    const int nsetp = st_i[2];
+                    double alpha = 2.0;
+                    int jj = st_i[4];
+                    for (int ip = 1; ip <= nsetp; ++ip) {
+                        const int l = idx[ip - 1];
+                        if (zz[ip - 1] <= 0) {
+                            const double t = -xs[l - 1] / (zz[ip - 1] - xs[l - 1]);
+                            if (alpha > t) { alpha = t; jj = ip; }
+                        }
+                    }
+                    st_i[4] = jj;
+                    if (alpha == 2) {
+                        st_i[8] = 1;
+                    } else {
+                        for (int ip = 1; ip <= nsetp; ++ip) {
+                            const int l = idx[ip - 1];
+                            xs[l - 1] = xs[l - 1] + alpha * (zz[ip - 1] - xs[l - 1]);
+                        }
+                        st_i[9] = idx[jj - 1];   // i
+                    }
+                }
+            }
+            __syncthreads();
+            if (st_i[8] == 2) { terminated = true; break; }
+            if (st_i[8] == 1) break;
+            // move coefficient i from P to Z (with Givens rotations), repeat
+            // while some remaining coefficient is not positive
+            for (;;) {
+                if (tid == 0) {
+                    xs[st_i[9] - 1] = 0;
+                    st_i[12] = st_i[4] != st_i[2];
+                    if (st_i[12]) st_i[4] += 1;   // jj += 1 persists across passes, as in nnls.jl:449-451
+                }
+                __syncthreads();
+                if (st_i[12]) {
+                    const int jj0 = st_i[4], nsetp = st_i[2];
+                    for (int jx = jj0; jx <= nsetp; ++jx) {
+                        if (tid == 0) {
+                            const int ii = idx[jx - 1];
+                            idx[jx - 2] = ii;
+                            const double a = AM(jx - 1, ii), bq = AM(jx, ii);
+                            double cc, ss, sig;
+                            if (fabs(a) > fabs(bq)) {
+                                double xr = bq / a, yr = sqrt(1 + xr * xr);
+                                cc = (1 / yr) * (a > 0 ? 1.0 : (a < 0 ? -1.0 : 0.0));
+                                ss = cc * xr;
+                                sig = fabs(a) * yr;
+                            } else if (bq != 0) {
+                                double xr = a / bq, yr = sqrt(1 + xr * xr);
+                                ss = (1 / yr) * (bq > 0 ? 1.0 : -1.0);
+                                cc = ss * xr;
+                                sig = fabs(bq) * yr;
+                            } else {
+                                sig = 0.0; cc = 0.0; ss = 1.0;
+                            }
+                            AM(jx - 1, ii) = sig;
+                            AM(jx, ii) = 0;
+                            st_d[18] = cc; st_d[19] = ss;
+                            st_i[10] = ii;
+                            double temp = b[jx - 2];
+                            b[jx - 2] = cc * temp + ss * b[jx - 1];
+                            b[jx - 1] = -ss * temp + cc * b[jx - 1];
+                        }
+                        __syncthreads();
+                        const double cc = st_d[18], ss = st_d[19];
+                        const int ii = st_i[10];
+                        for (int l = 1 + tid; l <= n; l += nt) {
+                            if (l != ii) {
+                                const double temp = AM(jx - 1, l);
+                                AM(jx - 1, l) = cc * temp + ss * AM(jx, l);
+                                AM(jx, l) = -ss * temp + cc * AM(jx, l);
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
+                if (tid == 0) {
+                    st_i[2] -= 1;
+                    st_i[0] -= 1;
+                    idx[st_i[0] - 1] = st_i[9];
+                    int allfeasible = 1;
+                    for (int jji = 1; jji <= st_i[2]; ++jji) {
+                        const int i = idx[jji - 1];
+                        st_i[9] = i;
+                        if (xs[i - 1] <= 0) { allfeasible = 0; break; }
+                    }
+                    st_i[11] = allfeasible;
+                }
+                __syncthreads();
+                if (st_i[11]) break;
+            }
+            if (tid == 0) {
+                const int nsetp = st_i[2];
+                for (int q = 0; q < m; ++q) zz[q] = b[q];
+                int jj = st_i[4];
+                for (int l = 1; l <= nsetp; ++l) {
+                    const int ip = nsetp + 1 - l;
+                    if (l != 1)
+                        for (int ii = 1; ii <= ip; ++ii) zz[ii - 1] = zz[ii - 1] - AM(ii, jj) * zz[ip];
+                    jj = idx[ip - 1];
+                    zz[ip - 1] /= AM(ip, jj);
+                }
+                st_i[4] = jj;
+            }
+            __syncthreads();
+        }
+        if (terminated) break;
+        if (tid == 0)
+            for (int i = 1; i <= st_i[2]; ++i) xs[idx[i - 1] - 1] = zz[i - 1];
+        __syncthreads();
+    }
+    // ---- compute_post_merge_particles_nnls! (:1334-1404)
+    __shared__ int32_t n_new_s, ok_s;
+    if (tid == 0) {
+        const int nsetp = st_i[2];
+        double sm = 0.0;
+        if (nsetp < m)
+            for (int i = nsetp + 1; i <= m; ++i) sm += b[i - 1] * b[i - 1];
+        const double rnorm = sqrt(sm);
+        int ok = !(rnorm > A.max_err);
+        int nonzero = 0;
+        double ndw = 0.0;
+        if (ok) {
+            for (int i = 0; i < n; ++i) {
+                xs[i] = xs[i] * cn[i];
+                if (xs[i] > A.w_threshold) { nonzero += 1; ndw += xs[i]; }
+            }
+            if (nonzero >= N) ok = 0;
+        }
+        ok_s = ok;
+        n_new_s = ok ? nonzero : N;
+        st_d[20] = W / ndw;
+        A.status[c] = ok ? 1 : -1;
+    }
+    __syncthreads();
+    const int32_t n_new = n_new_s;
+    if (ok_s) {
+        if (tid == 0) {
+            const double wt = st_d[20];
+            int cur = 0;
+            for (int j = 0; j < n; ++j) {
+                if (!(xs[j] > A.w_threshold)) continue;
+                const int64_t dst = oct_slot(cur, s1, n1, s2);
+                ++cur;
+                double vx, vy, vz;
+                if (j < N) {
+                    int64_t p = oct_slot(j, s1, n1, s2);
+                    vx = A.sp.v0[p]; vy = A.sp.v1[p]; vz = A.sp.v2[p];
+                } else {
+                    int f = j - N;
+                    if (A.centered && f == 0) {
+                        vx = v0x; vy = v0y; vz = v0z;
+                    } else {
+                        f -= A.centered ? 1 : 0;
+                        const int vm = f / 8, o = f % 8 + 1;
+                        const double mult = A.vmult[vm];
+                        const double sx = (o % 2 == 1) ? -1.0 : 1.0;
+                        const double sy = (o == 3 || o == 4 || o == 7 || o == 8) ? 1.0 : -1.0;
+                        const double sz = o >= 5 ? 1.0 : -1.0;
+                        double e3[3] = {Ex, Ey, Ez}, s3[3] = {sx, sy, sz}, r3[3];
+                        for (int d = 0; d < 3; ++d) {
+                            double ms = mult * (s3[d] * e3[d]);
+                            r3[d] = ms > st_d[9 + d] ? st_d[9 + d] * mult : (ms < st_d[6 + d] ? st_d[6 + d] * mult : ms);
+                        }
+                        vx = v0x + r3[0]; vy = v0y + r3[1]; vz = v0z + r3[2];
+                    }
+                }
+                A.tmp.w[dst] = xs[j] * wt;
+                A.tmp.v0[dst] = vx; A.tmp.v1[dst] = vy; A.tmp.v2[dst] = vz;
+                A.tmp.x0[dst] = st_d[3]; A.tmp.x1[dst] = st_d[4]; A.tmp.x2[dst] = st_d[5];
+            }
+        }
+        __syncthreads();
+        for (int li = tid; li < N; li += nt) {
+            int64_t p = oct_slot(li, s1, n1, s2);
+            if (li < n_new) {
+                A.sp.x0[p] = A.tmp.x0[p]; A.sp.x1[p] = A.tmp.x1[p]; A.sp.x2[p] = A.tmp.x2[p];
+                A.sp.v0[p] = A.tmp.v0[p]; A.sp.v1[p] = A.tmp.v1[p]; A.sp.v2[p] = A.tmp.v2[p];
+                A.sp.w[p] = A.tmp.w[p];
+            } else {
+                A.sp.w[p] = 0.0;
+            }
+        }
+    }
+    if (tid == 0) {
+        int32_t *op = A.old_pia + c * 5;
+        op[0] = s1; op[1] = n1; op[2] = s2; op[3] = n2; op[4] = ok_s ? n_new : -1;
+        if (ok_s) {
+            const int32_t n_del = N - n_new;
+            const int32_t d2 = min(n_del, n2), d1 = n_del - d2;
+            const int32_t nn2 = n2 - d2, nn1 = n1 - d1;
+            A.sp.pia.n2[c] = nn2;
+            A.sp.pia.end2[c] = nn2 > 0 ? s2 + nn2 - 1 : -1;
+            if (nn2 == 0) A.sp.pia.start2[c] = 0;
+            A.sp.pia.n1[c] = nn1;
+            A.sp.pia.end1[c] = nn1 > 0 ? s1 + nn1 - 1 : -1;
+            if (nn1 == 0) A.sp.pia.start1[c] = 0;
+            A.sp.pia.nloc[c] = nn1 + nn2;
+            if (!(c == A.n_cells - 1) || (n_del > n2)) A.sp.ctr->contiguous = 0;
+        }
+    }
+#undef AM
+}
+
+#pragma clang fp contract(on)
 
 extern "C" int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim, int32_t n_mim,
                              double vref, int32_t scaling_vref, int32_t centered_at_mean, const double *v_multipliers,
                              int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
                              int32_t squash, int32_t *status) {
-    (void)h; (void)species; (void)threshold; (void)mim; (void)n_mim; (void)vref; (void)scaling_vref;
-    (void)centered_at_mean; (void)v_multipliers; (void)n_vmult; (void)max_err; (void)iteration_mult;
-    (void)w_threshold; (void)squash; (void)status;
-    mz_set_error("merge_nnls: not available in this build");
-    return -3;
+    MZ_CHECK(species >= 1 && species <= h->n_species, "merge_nnls: bad species");
+    MZ_CHECK(n_vmult >= 0 && n_vmult <= NNLS_MAX_VMULT, "merge_nnls: too many v_multipliers");
+    MZ_CHECK(n_mim >= 0, "merge_nnls: bad multi-index count");
+    MzSpecies &s = h->sp[species - 1];
+    const int nc = h->n_cells;
+    NnlsArgs A{};
+    // NNLSMerge constructor (merging_nnls.jl:143-206): base moments first,
+    // then the requested ones not already present
+    static const int32_t base[7][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {0, 2, 0}, {0, 0, 2}};
+    int m = 0;
+    for (int i = 0; i < 7; ++i) { for (int d = 0; d < 3; ++d) A.mim[m][d] = base[i][d]; ++m; }
+    for (int i = 0; i < n_mim; ++i) {
+        bool dup = false;
+        for (int k = 0; k < 7; ++k)
+            if (mim[3 * i] == base[k][0] && mim[3 * i + 1] == base[k][1] && mim[3 * i + 2] == base[k][2]) dup = true;
+        if (dup) continue;
+        MZ_CHECK(m < NNLS_MAX_ROWS, "merge_nnls: too many moments");
+        for (int d = 0; d < 3; ++d) {
+            MZ_CHECK(mim[3 * i + d] >= 0 && mim[3 * i + d] <= 16, "merge_nnls: bad multi-index");
+            A.mim[m][d] = mim[3 * i + d];
+        }
+        ++m;
+    }
+    A.m = m;
+    for (int q = 0; q < m; ++q) A.tot_order[q] = A.mim[q][0] + A.mim[q][1] + A.mim[q][2];
+    A.vref = vref;
+    A.inv_vref = 1.0 / vref;
+    A.scaling_vref = scaling_vref;
+    A.centered = centered_at_mean ? 1 : 0;
+    A.n_vmult = n_vmult;
+    for (int i = 0; i < n_vmult; ++i) A.vmult[i] = v_multipliers[i];
+    A.max_err = max_err;
+    A.iteration_mult = iteration_mult;
+    A.w_threshold = w_threshold;
+    A.sp = mz_view(s);
+    A.tmp = A.sp;
+    A.tmp.x0 = s.xb[0]; A.tmp.x1 = s.xb[1]; A.tmp.x2 = s.xb[2];
+    A.tmp.v0 = s.vb[0]; A.tmp.v1 = s.vb[1]; A.tmp.v2 = s.vb[2];
+    A.tmp.w = s.wb;
+    A.n_cells = nc;
+    // cells to merge and their scratch sizes (host side)
+    MzCounters snap;
+    int r = mz_read_counters(h, species, &snap);
+    if (r) return r;
+    std::vector<int32_t> nloc(nc);
+    MZ_HIP(hipMemcpy(nloc.data(), s.pia.nloc, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
+    const int n_add = A.centered + 8 * n_vmult;
+    std::vector<int32_t> cells;
+    std::vector<int64_t> offs;
+    int64_t tot = 0;
+    for (int c = 0; c < nc; ++c) {
+        if (nloc[c] > threshold) {
+            const int64_t n = nloc[c] + n_add;
+            cells.push_back(c);
+            offs.push_back(tot);
+            tot += ((int64_t)m * n + 3 * n + (n + 1) / 2 + 32) & ~(int64_t)31;
+        }
+    }
+    int32_t *old_pia = (int32_t *)mz_scratch(h, "oct_oldpia", (size_t)nc * 5 * sizeof(int32_t));
+    int32_t *d_status = (int32_t *)mz_scratch(h, "nnls_status", (size_t)nc * sizeof(int32_t));
+    int32_t *ndel = (int32_t *)mz_scratch(h, "oct_ndel", (size_t)nc * sizeof(int32_t));
+    int64_t *off = (int64_t *)mz_scratch(h, "oct_off", (size_t)(nc + 1) * sizeof(int64_t));
+    MZ_CHECK(old_pia && d_status && ndel && off, "merge_nnls: scratch allocation failed");
+    MZ_HIP(hipMemsetAsync(d_status, 0, nc * sizeof(int32_t), h->stream));
+    // old_pia[c][4] = -1 marks "not merged" for the id pushes
+    {
+        std::vector<int32_t> init((size_t)nc * 5, 0);
+        for (int c = 0; c < nc; ++c) init[(size_t)c * 5 + 4] = -1;
+        MZ_HIP(hipMemcpy(old_pia, init.data(), init.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    mz_timing_begin(h, "merge_nnls");
+    if (!cells.empty()) {
+        // batches bounded to ~1 GiB of matrix scratch
+        const int64_t budget = (int64_t)1 << 27;   // doubles
+        size_t i0 = 0;
+        while (i0 < cells.size()) {
+            size_t i1 = i0;
+            const int64_t base_off = offs[i0];
+            while (i1 < cells.size() && (i1 == i0 || offs[i1] - base_off + 1 < budget)) ++i1;
+            const int64_t span = (i1 < cells.size() ? offs[i1] : tot) - base_off;
+            double *scr = (double *)mz_scratch(h, "nnls_A", (size_t)span * sizeof(double));
+            int32_t *dcells = (int32_t *)mz_scratch(h, "nnls_cells", (i1 - i0) * sizeof(int32_t));
+            int64_t *doffs = (int64_t *)mz_scratch(h, "nnls_offs", (i1 - i0) * sizeof(int64_t));
+            MZ_CHECK(scr && dcells && doffs, "merge_nnls: scratch allocation failed");
+            std::vector<int64_t> lo(offs.begin() + i0, offs.begin() + i1);
+            for (auto &v : lo) v -= base_off;
+            MZ_HIP(hipMemcpy(dcells, cells.data() + i0, (i1 - i0) * sizeof(int32_t), hipMemcpyHostToDevice));
+            MZ_HIP(hipMemcpy(doffs, lo.data(), (i1 - i0) * sizeof(int64_t), hipMemcpyHostToDevice));
+            A.cells = dcells;
+            A.aoff = doffs;
+            A.scratch = scr;
+            A.status = d_status;
+            A.old_pia = old_pia;
+            hipLaunchKernelGGL(k_nnls_merge, dim3((unsigned)(i1 - i0)), dim3(256), 0, h->stream, A);
+            MZ_HIP(hipGetLastError());
+            MZ_HIP(hipStreamSynchronize(h->stream));
+            i0 = i1;
+        }
+    }
+    hipLaunchKernelGGL(k_merge_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, old_pia, nc, ndel);
+    hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
+    hipLaunchKernelGGL(k_merge_push_ids, dim3(nc), dim3(64), 0, h->stream, A.sp, old_pia, off, off + nc,
+                       snap.nbuffer, snap.n_total, nc);
+    mz_timing_end(h);
+    MZ_HIP(hipGetLastError());
+    if (status) MZ_HIP(hipMemcpy(status, d_status, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    if (squash) return mz_squash(h, species);
+    return 0;
 }

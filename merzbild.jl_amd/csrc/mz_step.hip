@@ -102,7 +102,8 @@ extern "C" int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, do
         MZ_HIP(hipMalloc(&h->d_step, sizeof(uint64_t)));
         // scratch is allocated before capture
         int32_t nchunks = (int32_t)((h->sp[0].cap + MZ_SORT_CHUNK - 1) / MZ_SORT_CHUNK);
-        MZ_CHECK(mz_scratch(h, "sort_hist", (size_t)h->n_cells * nchunks * sizeof(int32_t)) != nullptr,
+        MZ_CHECK(mz_scratch(h, "sort_hist", (size_t)h->n_cells * nchunks * sizeof(int32_t)) != nullptr &&
+                     mz_scratch(h, "sort_cstart", (size_t)h->n_cells * sizeof(int32_t)) != nullptr,
                  "couette_steps: scratch allocation failed");
         MZ_HIP(hipStreamSynchronize(h->stream));
         MZ_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));

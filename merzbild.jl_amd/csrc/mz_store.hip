@@ -155,6 +155,8 @@ extern "C" int mz_create(int32_t n_cells, int32_t n_species, const int64_t *capa
     MZ_HIP(hipMemset(h->d_cfcnt, 0, (size_t)nss * n_cells * 5 * sizeof(int32_t)));
     MZ_HIP(hipMalloc(&h->d_err, sizeof(int32_t)));
     MZ_HIP(hipMemset(h->d_err, 0, sizeof(int32_t)));
+    MZ_HIP(hipMalloc(&h->d_ticket, sizeof(uint32_t)));
+    MZ_HIP(hipMemset(h->d_ticket, 0, sizeof(uint32_t)));
     size_t pn = (size_t)n_species * n_cells;
     MZ_HIP(hipMalloc(&h->d_np, pn * sizeof(double)));
     MZ_HIP(hipMalloc(&h->d_n, pn * sizeof(double)));
@@ -189,7 +191,7 @@ extern "C" int mz_destroy(mz_store *h) {
     hipStreamSynchronize(h->stream);
     mz_graph_release(h);
     for (auto &s : h->sp) free_species(s);
-    hipFree(h->d_sgwm); hipFree(h->d_cfcnt); hipFree(h->d_err);
+    hipFree(h->d_sgwm); hipFree(h->d_cfcnt); hipFree(h->d_err); hipFree(h->d_ticket);
     hipFree(h->d_np); hipFree(h->d_n); hipFree(h->d_T); hipFree(h->d_v); hipFree(h->d_V);
     hipFree(h->d_invV);
     hipFree(h->d_masses);
@@ -496,13 +498,12 @@ MzRngParams mz_rng_params(mz_store *h, int kind, uint64_t tag, int mode) {
 //   1. k_sort_hist: per chunk of MZ_SORT_CHUNK positions, cell keys (from x,
 //      or the stored keys) and a per-chunk histogram in LDS; the histogram is
 //      written cell-major [cell][chunk].
-//   2. k_sort_scan: exclusive scan of that matrix; its cell-major order makes
-//      offsets[cell][chunk] the first output slot of chunk's particles in
-//      cell, and the per-cell totals give the new PIA (group 1 only, as the
-//      reference writes it).
-//   3. k_sort_scatter: one wave per chunk walks its positions in order, 64 at
-//      a time; lanes with equal keys are ranked with a ballot, so equal keys
-//      keep their order (stable, as the reference's reverse scatter).
+//   2. k_sort_rows: per cell, an exclusive scan over chunks gives each
+//      chunk's offset inside the cell; the cell totals, scanned by the block
+//      that finishes last, give the cell starts and the new PIA.
+//   3. k_sort_scatter: per chunk, its slot of particle i is cell start +
+//      chunk offset + rank of i among the chunk's equal keys in position
+//      order, so the sort is stable like the reference's reverse scatter.
 // The data moves with the sort (x, v, w, id); the key array stays in the
 // pre-sort order, like the reference's pv.cell.
 
@@ -539,100 +540,129 @@ __global__ void k_sort_hist(MzSpView sp, MzGrid g, int32_t compute_cells, int32_
     for (int c = threadIdx.x; c < n_cells; c += blockDim.x) hist[(int64_t)c * nchunks + blockIdx.x] = sh[c];
 }
 
-// Exclusive scan of M ints in place with one 1024-thread block, then PIA.
-__global__ void __launch_bounds__(1024) k_sort_scan(MzSpView sp, int32_t n_cells, int32_t nchunks,
-                                                   int32_t *__restrict__ hist) {
-    __shared__ int32_t part[1024];
-    __shared__ int32_t wsum[16];
-    const int64_t M = (int64_t)n_cells * nchunks;
-    const int t = threadIdx.x;
-    const int64_t per = (M + 1023) / 1024;
-    const int64_t lo = t * per, hi = min(M, lo + per);
-    int32_t s = 0;
-    for (int64_t i = lo; i < hi; ++i) s += hist[i];
-    // block exclusive scan of s
-    int lane = t & 63, wv = t >> 6;
-    int32_t x = s;
+// One wave per cell scans that cell's row of the count matrix over chunks
+// (exclusive, in place) and leaves the cell total; the block that finishes
+// last turns the totals into cell starts and writes the PIA (group 1 only,
+// group 2 cleared, as grid_sorting.jl:76-98 writes it).
+__global__ void __launch_bounds__(256) k_sort_rows(MzSpView sp, int32_t n_cells, int32_t nchunks,
+                                                   int32_t *__restrict__ hist, int32_t *__restrict__ cstart,
+                                                   uint32_t *__restrict__ ticket) {
+    __shared__ int32_t is_last;
+    __shared__ int32_t wsum[4];
+    __shared__ int32_t carry_s;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c = blockIdx.x * 4 + wv;
+    if (c < n_cells) {
+        int32_t *row = hist + (int64_t)c * nchunks;
+        int32_t carry = 0;
+        for (int b0 = 0; b0 < nchunks; b0 += 64) {
+            int b = b0 + lane;
+            int32_t v = b < nchunks ? row[b] : 0;
+            int32_t x = v;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) wsum[wv] = x;
-    __syncthreads();
-    if (t < 16) {
-        int32_t a = wsum[t];
-        int32_t b = a;
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-            int32_t y = __shfl_up(b, o, 16);
-            if ((t & 15) >= o) b += y;
+            for (int o = 1; o < 64; o <<= 1) {
+                int32_t y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            if (b < nchunks) row[b] = carry + x - v;
+            carry += __shfl(x, 63, 64);
         }
-        wsum[t] = b - a;
+        if (lane == 0) __hip_atomic_store(&cstart[c], carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = atomicAdd(ticket, 1u);
+        is_last = (t == gridDim.x - 1);
+        carry_s = 0;
     }
     __syncthreads();
-    int32_t run = x - s + wsum[wv];
-    part[t] = run;
-    for (int64_t i = lo; i < hi; ++i) {
-        int32_t v = hist[i];
-        hist[i] = run;
-        run += v;
-    }
-    __syncthreads();
-    // PIA: cell c occupies [offsets[c][0], offsets[c+1][0]) in 0-based slots
+    if (!is_last) return;
+    __threadfence();
     const int32_t total = mz_ntotal(sp);
-    for (int c = t; c < n_cells; c += 1024) {
-        int32_t st = hist[(int64_t)c * nchunks];
-        int32_t en = (c + 1 < n_cells) ? hist[(int64_t)(c + 1) * nchunks] : total;
-        int32_t cnt = en - st;
-        sp.pia.start2[c] = 0;
-        sp.pia.end2[c] = -1;
-        sp.pia.n2[c] = 0;
-        sp.pia.start1[c] = cnt > 0 ? st + 1 : 0;
-        sp.pia.end1[c] = cnt > 0 ? en : -1;
-        sp.pia.n1[c] = cnt;
-        sp.pia.nloc[c] = cnt;
+    for (int c0 = 0; c0 < n_cells; c0 += 256) {
+        const int cc = c0 + threadIdx.x;
+        int32_t v = cc < n_cells ? __hip_atomic_load(&cstart[cc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        int32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
+        __syncthreads();
+        int32_t pre = carry_s;
+        for (int k = 0; k < wv; ++k) pre += wsum[k];
+        int32_t excl = pre + x - v;
+        if (cc < n_cells) {
+            cstart[cc] = excl;
+            sp.pia.start2[cc] = 0;
+            sp.pia.end2[cc] = -1;
+            sp.pia.n2[cc] = 0;
+            sp.pia.start1[cc] = v > 0 ? excl + 1 : 0;
+            sp.pia.end1[cc] = v > 0 ? excl + v : -1;
+            sp.pia.n1[cc] = v;
+            sp.pia.nloc[cc] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 255) carry_s = excl + v;
+        __syncthreads();
     }
-    if (t == 0) {
+    if (threadIdx.x == 0) {
+        *ticket = 0;
         sp.ctr->contiguous = 1;
         sp.ctr->n_pos = total;
     }
 }
 
-__global__ void __launch_bounds__(64) k_sort_scatter(MzSpView a, MzSpView b, int32_t n_cells, int32_t nchunks,
-                                                    const int32_t *__restrict__ offs) {
-    extern __shared__ int32_t run[];
+// One block per chunk: the chunk's keys go to LDS, wave 0 ranks them in
+// position order (64 at a time; lanes with equal keys are ranked with a
+// ballot, so equal keys keep their order: stable, as the reference's reverse
+// scatter), then all threads move the particles to their slots.
+__global__ void __launch_bounds__(256) k_sort_scatter(MzSpView a, MzSpView b, int32_t n_cells, int32_t nchunks,
+                                                     const int32_t *__restrict__ rows,
+                                                     const int32_t *__restrict__ cstart) {
+    extern __shared__ int32_t run[];   // [n_cells]: next slot of each cell for this chunk
+    __shared__ int32_t keys[MZ_SORT_CHUNK];
+    __shared__ int32_t dst[MZ_SORT_CHUNK];
     const int32_t n = mz_ntotal(a);
-    const int lane = threadIdx.x;
-    for (int c = lane; c < n_cells; c += 64) run[c] = 0;
+    const int64_t p0 = (int64_t)blockIdx.x * MZ_SORT_CHUNK;
+    if (p0 >= n) return;
+    const int cnt = (int)min((int64_t)MZ_SORT_CHUNK, (int64_t)n - p0);
+    for (int k = threadIdx.x; k < n_cells; k += 256) run[k] = cstart[k] + rows[(int64_t)k * nchunks + blockIdx.x];
+    for (int i = threadIdx.x; i < cnt; i += 256) {
+        int32_t k = a.key[p0 + i];
+        keys[i] = k < 1 ? 1 : (k > n_cells ? n_cells : k);
+    }
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * MZ_SORT_CHUNK;
-    const uint64_t lt = mz_lanemask_lt();
-    for (int bt = 0; bt < MZ_SORT_CHUNK; bt += 64) {
-        int64_t p = base + bt + lane;
-        bool valid = p < n;
-        int32_t key = valid ? a.key[p] : 0;
-        if (valid) key = key < 1 ? 1 : (key > n_cells ? n_cells : key);
-        uint64_t active = __ballot(valid);
-        if (active == 0) break;
-        int64_t dst = 0;
-        while (active) {
-            int leader = __ffsll((unsigned long long)active) - 1;
-            int32_t k = __shfl(key, leader, 64);
-            uint64_t mask = __ballot(valid && key == k);
-            int32_t basek = offs[(int64_t)(k - 1) * nchunks + blockIdx.x] + run[k - 1];
-            if (valid && key == k) dst = (int64_t)basek + __popcll(mask & lt);
-            __syncthreads();
-            if (lane == leader) run[k - 1] += __popcll(mask);
-            __syncthreads();
-            active &= ~mask;
+    if (threadIdx.x < 64) {
+        volatile int32_t *vrun = run;
+        const int lane = threadIdx.x;
+        const uint64_t lt = mz_lanemask_lt();
+        for (int bt = 0; bt < cnt; bt += 64) {
+            const int i = bt + lane;
+            const bool valid = i < cnt;
+            const int32_t key = valid ? keys[i] : 0;
+            uint64_t active = __ballot(valid);
+            while (active) {
+                const int leader = __ffsll((unsigned long long)active) - 1;
+                const int32_t k = __shfl(key, leader, 64);
+                const uint64_t mask = __ballot(valid && key == k);
+                const int32_t bk = vrun[k - 1];
+                if (valid && key == k) dst[i] = bk + __popcll(mask & lt);
+                vrun[k - 1] = bk + __popcll(mask);
+                active &= ~mask;
+            }
         }
-        if (valid) {
-            b.x0[dst] = a.x0[p]; b.x1[dst] = a.x1[p]; b.x2[dst] = a.x2[p];
-            b.v0[dst] = a.v0[p]; b.v1[dst] = a.v1[p]; b.v2[dst] = a.v2[p];
-            b.w[dst] = a.w[p];
-            b.id[dst] = a.id[p];
-        }
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int i = threadIdx.x; i < cnt; i += 256) {
+        const int64_t p = p0 + i, d = dst[i];
+        b.x0[d] = a.x0[p]; b.x1[d] = a.x1[p]; b.x2[d] = a.x2[p];
+        b.v0[d] = a.v0[p]; b.v1[d] = a.v1[p]; b.v2[d] = a.v2[p];
+        b.w[d] = a.w[p];
+        b.id[d] = a.id[p];
     }
 }
 
@@ -667,13 +697,15 @@ int mz_launch_sort(mz_store *h, int species, int compute_cells) {
     MZ_CHECK(!compute_cells || h->has_grid, "sort: no grid set");
     int32_t nchunks = (int32_t)((s.cap + MZ_SORT_CHUNK - 1) / MZ_SORT_CHUNK);
     int32_t *hist = (int32_t *)mz_scratch(h, "sort_hist", (size_t)nc * nchunks * sizeof(int32_t));
-    MZ_CHECK(hist != nullptr, "sort: scratch allocation failed");
+    int32_t *cstart = (int32_t *)mz_scratch(h, "sort_cstart", (size_t)nc * sizeof(int32_t));
+    MZ_CHECK(hist != nullptr && cstart != nullptr, "sort: scratch allocation failed");
     MzSpView a = mz_view(s), b = view_b(s);
     size_t lds = (size_t)nc * sizeof(int32_t);
     hipLaunchKernelGGL(k_sort_hist, dim3(nchunks), dim3(256), lds, h->stream, a, h->grid, compute_cells, nc,
                        nchunks, hist);
-    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, h->stream, a, nc, nchunks, hist);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(nchunks), dim3(64), lds, h->stream, a, b, nc, nchunks, hist);
+    hipLaunchKernelGGL(k_sort_rows, dim3((nc + 3) / 4), dim3(256), 0, h->stream, a, nc, nchunks, hist, cstart,
+                       h->d_ticket);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nchunks), dim3(256), lds, h->stream, a, b, nc, nchunks, hist, cstart);
     MZ_HIP(hipGetLastError());
     return 0;
 }

@@ -1,0 +1,215 @@
+"""GPU parity of the merging kernels against the CPU oracle.
+
+Octree N:2: bin assignment follows from exact arithmetic, so the merged
+particle count, the positions the new particles occupy, the PIA, the ids and
+the free-id LIFO must match bit for bit; values to 1e-12; mass, momentum and
+energy are conserved to 1e-12 relative."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def _var_weight_cells(mzo, counts, seed, L_cell=1e-6):
+    rng = np.random.default_rng(seed)
+    n = int(sum(counts))
+    nc = len(counts)
+    pv = mzo.ParticleVector(n)
+    pia = mzo.PIA(nc, 1)
+    vs = np.sqrt(helpers.KB * 300.0 / helpers.MASS_AR)
+    pos = 1
+    for c, k in enumerate(counts, start=1):
+        for j in range(k):
+            pv.pop_to(pos + j)
+            w = 1e14 * rng.lognormal(0.0, 1.0) / np.exp(0.5)
+            v = rng.normal(0, vs, 3) + np.array([0.0, 500.0 * (2 * c / nc - 1), 0.0])
+            x = [(c - 1 + rng.uniform()) * L_cell, rng.uniform(), rng.uniform()]
+            pv.set(pos + j, w, v, x)
+        if k:
+            pia.set(c, 1, k, pos, pos + k - 1, k)
+        pos += k
+    pia.set_n_total(1, n)
+    return pv, pia
+
+
+def _moments(w, v):
+    m = w.sum()
+    p = (w[:, None] * v).sum(0)
+    e = (w * (v ** 2).sum(1)).sum()
+    return m, p, e
+
+
+@pytest.mark.parametrize("split,init,bounds,clamp", [(1, 1, 1, True), (1, 2, 1, False), (1, 3, 2, False),
+                                                     (2, 1, 1, False)])
+def test_octree_merge_vs_oracle(hip, mzo, split, init, bounds, clamp):
+    counts = [120, 450, 401, 30, 800, 1500, 402, 0, 2000, 399]
+    nc = len(counts)
+    pv, pia = _var_weight_cells(mzo, counts, seed=31 + split + init)
+    st = hip.DeviceStore(nc, 1, sum(counts) + 512)
+    helpers.upload_from_oracle(st, mzo, pv, pia)
+    grid = hip.Grid1DUniform(1e-6 * nc, nc)
+    st.set_grid(grid)
+    st.set_seed(4242)
+    tag = (17 << 4) | 3
+    before = st.download_species(1)
+    n = before["n_total"]
+    m0, p0, e0 = _moments(before["w"][:n], before["v"][:n])
+    n_merged = hip.merge_octree_N2_based(st, 1, threshold=400, target_np=64, split=split, init_bin_bounds=init,
+                                         bin_bounds_compute=bounds, max_depth=8, grid_clamp=clamp, squash=False,
+                                         tag=tag)
+    assert n_merged == sum(1 for k in counts if k > 400)
+    og = mzo.grid1d(1e-6 * nc, nc)
+    oc = mzo.Octree(split=split, init_bin_bounds=init, bin_bounds_compute=bounds, max_Nbins=4096, max_depth=8)
+    r = mzo.RNG(seed=4242, tag=tag)
+    for c in range(1, nc + 1):
+        if pia.ind(c).n_local > 400:
+            if split == 2:
+                # the mean split carries vel_middle between calls on one
+                # OctreeN2Merge (merging_octree_N2.jl:431-439); the device
+                # merges cells independently, i.e. with a fresh octree per cell
+                oc = mzo.Octree(split=split, init_bin_bounds=init, bin_bounds_compute=bounds, max_depth=8)
+            mzo.lib().mzo_merge_octree_N2_based(r.ref, oc.p, pv.p, pia.p, c, 1, 64, C.byref(og) if clamp else None)
+    got = st.download_species(1)
+    assert got["err"] == 0
+    helpers.assert_state_matches(mzo, got, pv, pia, rtol=1e-12)
+    # conservation over the used positions
+    tab = got["pia"]
+    sel = np.concatenate([np.arange(r_[1] - 1, r_[2]) for r_ in tab if r_[3] > 0] +
+                         [np.arange(r_[4] - 1, r_[5]) for r_ in tab if r_[6] > 0])
+    m1, p1, e1 = _moments(got["w"][sel], got["v"][sel])
+    assert abs(m1 / m0 - 1) < 1e-12
+    assert np.all(np.abs(p1 - p0) <= 1e-12 * np.abs(p0).max())
+    assert abs(e1 / e0 - 1) < 1e-12
+    # squash afterwards, as the reference's driver does
+    hip.squash_pia(st, 1)
+    mzo.lib().mzo_squash_pia(pv.p, pia.p, 1)
+    helpers.assert_state_matches(mzo, st.download_species(1), pv, pia, rtol=1e-12)
+    st.close()
+
+
+def test_octree_merge_reference_kat(hip, mzo):
+    """test/test_octree_merging.jl:76-163 on the device: 8 octants x 3
+    particles merged to 16 (one split, depth 1), then to 2 (no split)."""
+    parts = []
+    for o in range(1, 9):
+        for dv in (-0.5, 0.5, 0.0):
+            val = 9.0 - o + dv
+            vz = val if o >= 5 else -val
+            vx = -val if o % 2 == 1 else val
+            vy = val if o in (3, 4, 7, 8) else -val
+            parts.append((float(o), [vx, vy, vz], [1.0, -10.0, 3.0]))
+    pv = mzo.ParticleVector(24)
+    for i, (w, v, x) in enumerate(parts, start=1):
+        pv.pop_to(i)
+        pv.set(i, w, v, x)
+    pia = mzo.PIA(1, 1)
+    pia.set(1, 1, 24, 1, 24, 24)
+    pia.set_n_total(1, 24)
+    st = hip.DeviceStore(1, 1, 64)
+    helpers.upload_from_oracle(st, mzo, pv, pia)
+    st.set_seed(1234)
+    sp = [hip.Species("Ar", helpers.MASS_AR)]
+    p0 = hip.compute_props(st, sp)
+    n0, T0, v0 = p0.n[0, 0], p0.T[0, 0], p0.v[0, 0].copy()
+    hip.merge_octree_N2_based(st, 1, threshold=2, target_np=16, init_bin_bounds=hip.OCTREE_INIT_C, squash=False)
+    got = st.download_species(1)
+    assert got["n_total"] == 16 and got["pia"][0][0] == 16
+    p1 = hip.compute_props(st, sp)
+    assert abs(p1.n[0, 0] - n0) < np.finfo(float).eps * n0 and abs(p1.T[0, 0] - T0) < 1e-14
+    assert np.all(np.abs(p1.v[0, 0] - v0) < 1e-14)
+    hip.merge_octree_N2_based(st, 1, threshold=2, target_np=2, init_bin_bounds=hip.OCTREE_INIT_C, squash=False)
+    got = st.download_species(1)
+    assert got["n_total"] == 2
+    assert got["w"][0] == 0.5 * sum(3 * i for i in range(1, 9)) == got["w"][1]
+    st.close()
+
+
+def _mim(order):
+    return [t for i in range(1, order + 1) for t in (
+        [(a, b, i - a - b) for a in range(i + 1) for b in range(i - a + 1)])]
+
+
+def test_nnls_merge_reference_case(hip, mzo):
+    """test/test_nnls_merging.jl:104-147 on the device: 50 particles, moments
+    to order 4, :vref scaling; the merged particles equal the oracle's (same
+    pivots) and conserve mass, momentum and energy."""
+    parts = [(1.0, [i, 5 - 2.0 * i, -100.0 + 0.25 * i * i], [-0.5 * i, 27 + i, 400.0 - 0.5 * i * i])
+             for i in range(1, 51)]
+    pv = mzo.ParticleVector(50)
+    for i, (w, v, x) in enumerate(parts, start=1):
+        pv.pop_to(i)
+        pv.set(i, w, v, x)
+    pia = mzo.PIA(1, 1)
+    pia.set(1, 1, 50, 1, 50, 50)
+    pia.set_n_total(1, 50)
+    st = hip.DeviceStore(1, 1, 128)
+    helpers.upload_from_oracle(st, mzo, pv, pia)
+    vref = np.sqrt(2 * helpers.KB * 300.0 / helpers.MASS_AR)
+    mim = _mim(4)
+    status = hip.merge_nnls_based(st, 1, threshold=1, multi_index_moments=mim, vref=vref, scaling="vref",
+                                  squash=False)
+    nm = mzo.NNLSMerge(mim)
+    nm.s.reset_x0 = 1
+    res = nm.merge(mzo.RNG(seed=1), pv, pia, 1, 1, vref=vref, scaling="vref")
+    assert status[0] == res == 1
+    got = st.download_species(1)
+    helpers.assert_state_matches(mzo, got, pv, pia, rtol=1e-13)
+    n = got["n_total"]
+    assert n < 50
+    w, v = got["w"][:n], got["v"][:n]
+    ws, vs = np.array([p[0] for p in parts]), np.array([p[1] for p in parts])
+    assert abs(w.sum() - ws.sum()) < 1.5e-14 * ws.sum()
+    assert np.all(np.abs((w[:, None] * v).sum(0) - (ws[:, None] * vs).sum(0)) < 1e-12 * np.abs(vs).max() * ws.sum())
+    assert abs((w * (v ** 2).sum(1)).sum() / (ws * (vs ** 2).sum(1)).sum() - 1) < 1e-12
+    st.close()
+
+
+@pytest.mark.parametrize("scaling", ["variance", "vref"])
+def test_nnls_merge_batch_vs_oracle(hip, mzo, scaling):
+    """Many cells of bi-Maxwellian particles with lognormal weights merged in
+    one launch; every cell's status and merged particles equal the oracle's
+    sequential merges; ids pushed to the free-id LIFO in cell order."""
+    rng = np.random.default_rng(9)
+    counts = [int(k) for k in rng.integers(40, 400, size=24)]
+    counts[3] = 20                      # below the threshold: untouched
+    nc = len(counts)
+    n = sum(counts)
+    pv = mzo.ParticleVector(n)
+    pia = mzo.PIA(nc, 1)
+    pos = 1
+    for c, k in enumerate(counts, start=1):
+        ratio = rng.uniform(0.5, 2.0)
+        sig = np.sqrt(helpers.KB * 300.0 / helpers.MASS_AR) * np.array([np.sqrt(ratio), 1.0, 1.0])
+        drift = rng.normal(size=3)
+        drift *= rng.uniform(0, 500.0) / np.linalg.norm(drift)
+        for j in range(k):
+            pv.pop_to(pos + j)
+            pv.set(pos + j, 1e12 * rng.lognormal(0, 0.5), drift + sig * rng.normal(size=3),
+                   [rng.uniform(), rng.uniform(), rng.uniform()])
+        pia.set(c, 1, k, pos, pos + k - 1, k)
+        pos += k
+    pia.set_n_total(1, n)
+    st = hip.DeviceStore(nc, 1, n + 256)
+    helpers.upload_from_oracle(st, mzo, pv, pia)
+    vref = np.sqrt(2 * helpers.KB * 300.0 / helpers.MASS_AR)
+    mim = _mim(4)
+    status = hip.merge_nnls_based(st, 1, threshold=30, multi_index_moments=mim, vref=vref, scaling=scaling,
+                                  squash=False)
+    nm = mzo.NNLSMerge(mim)
+    nm.s.reset_x0 = 1
+    ref_status = []
+    for c in range(1, nc + 1):
+        if pia.ind(c).n_local > 30:
+            ref_status.append(nm.merge(mzo.RNG(seed=1), pv, pia, c, 1, vref=vref, scaling=scaling))
+        else:
+            ref_status.append(0)
+    assert list(status) == ref_status
+    assert sum(s == 1 for s in ref_status) > 15
+    got = st.download_species(1)
+    assert got["err"] == 0
+    helpers.assert_state_matches(mzo, got, pv, pia, rtol=1e-13)
+    st.close()
