@@ -9,9 +9,10 @@
 // and the random sign draws -- are reproduced exactly:
 //   * bounds come from min/max (order independent) and the split velocity
 //     from 0.5 (v_min + v_max), so octants are bit-exact;
-//   * the octant scatter ranks lanes with ballots in position order;
-//   * each child's weight is summed sequentially over its (stably sorted)
-//     segment, which is the reference's accumulation order;
+//   * the octant scatter ranks lanes with ballots and reproduces the
+//     reference's placement (each child in reverse parent order);
+//   * each child's weight is summed sequentially in parent order, the
+//     reference's accumulation order;
 //   * the refine pick is the first maximum over bins 1..Nbins.
 // Bin means and variances are block reductions (they only feed the new
 // particle values, compared to 1e-12).
@@ -222,7 +223,9 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
             for (int o = 1; o <= 8; ++o) { start8[o] = bs + acc; run8[o] = 0; acc += cnt8[o]; }
         }
         __syncthreads();
-        // stable scatter by octant, in position order (wave 0)
+        // Scatter by octant (wave 0).  The reference walks the bin forward and
+        // fills each octant from its end (merging_octree_N2.jl:621-626), so
+        // every child holds its particles in reverse parent order.
         if (threadIdx.x < 64) {
             const int lane = threadIdx.x;
             const uint64_t lt = mz_lanemask_lt();
@@ -235,7 +238,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                     uint64_t m = __ballot(valid && o == q);
                     if (m == 0) continue;
                     const int32_t r0 = vrun[q];
-                    if (valid && o == q) psort[start8[q] + r0 + __popcll(m & lt)] = pis[i];
+                    if (valid && o == q) psort[start8[q] + cnt8[q] - 1 - (r0 + __popcll(m & lt))] = pis[i];
                     vrun[q] = r0 + __popcll(m);
                 }
             }
@@ -243,11 +246,12 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
         __syncthreads();
         for (int i = bs + threadIdx.x; i <= be; i += blockDim.x) pis[i] = psort[i];
         __syncthreads();
-        // per-octant weights, summed in the reference's order (one lane per octant)
+        // per-octant weights in the reference's accumulation order, i.e. parent
+        // order, which is each child's segment read backwards (one lane per octant)
         if (threadIdx.x >= 1 && threadIdx.x <= 8) {
             const int o = threadIdx.x;
             double s = 0.0;
-            for (int i = start8[o]; i < start8[o] + cnt8[o]; ++i) s += A.sp.w[oct_slot(pis[i], s1, n1, s2)];
+            for (int i = start8[o] + cnt8[o] - 1; i >= start8[o]; --i) s += A.sp.w[oct_slot(pis[i], s1, n1, s2)];
             ndens8[o] = s;
         }
         __syncthreads();
