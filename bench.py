@@ -9,7 +9,7 @@ ntc! in every cell, convection, the bit-exact counting sort, and
 compute_props_sorted!.  Inputs are synthetic (sampled here with numpy) and
 resident on the GPU before the timed region.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode slab|graph|eager] [--no-cpu-baseline]
 
 Prints one JSON line.
 """
@@ -136,11 +136,13 @@ def cpu_baseline(case, it, target_s=12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--timing-steps", type=int, default=400)
+    ap.add_argument("--mode", choices=("slab", "graph", "eager"), default="slab",
+                    help="slab: all steps in one launch, cells resident in LDS; graph: replayed step graph; "
+                         "eager: kernel by kernel")
+    ap.add_argument("--graph-steps", type=int, default=400, help="steps of the step-graph path timed alongside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of a hipGraph")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": a.gpus}))
@@ -148,49 +150,63 @@ def main():
     pkg = load_pkg()
     case = sample_case()
     st, sp, it = setup_store(pkg, case)
-    graph = not a.eager
     n = case["n"]
 
-    # warmup (also captures the step graph)
-    pkg.couette_steps(st, a.warmup, 1, DT, props_every=1, graph=graph)
+    # warmup (also captures the step graph / sizes the slab scratch)
+    pkg.couette_steps(st, a.warmup, 1, DT, props_every=1, mode=a.mode)
     st.sync()
     step0 = 1 + a.warmup
-    t0 = time.perf_counter()
+    st.timing_reset()
     st.timing(True)
-    pkg.couette_steps(st, a.steps, step0, DT, props_every=1, graph=graph)
+    t0 = time.perf_counter()
+    path = pkg.couette_steps(st, a.steps, step0, DT, props_every=1, mode=a.mode)
     st.sync()
     t1 = time.perf_counter()
-    step_ms_ev, step_cnt = st.timing_get("step")
+    fam = "slab" if path == "slab" else "step"
+    launch_ms, launches = st.timing_get(fam)
     st.timing(False)
+    slab_info = st.slab_info() if path == "slab" else None
     elapsed = t1 - t0
     value = n * a.steps / elapsed
+    sc = st.scalars(1)
+    assert sc["err"] == 0 and sc["n_total"] == n, sc
 
-    # per-kernel durations: the same steps launched kernel by kernel with HIP
-    # events on the library's stream around every launch
+    # the step-by-step graph path on the same store, for comparison
     st.timing_reset()
     st.timing(True)
     s1 = step0 + a.steps
-    for t in range(a.timing_steps):
-        stp = s1 + t
+    g0 = time.perf_counter()
+    pkg.couette_steps(st, a.graph_steps, s1, DT, props_every=1, mode="graph")
+    st.sync()
+    graph_ms = (time.perf_counter() - g0) * 1e3 / max(a.graph_steps, 1)
+    # per-kernel durations of the step-by-step kernels, HIP events around each
+    kern = {}
+    st.timing_reset()
+    s2 = s1 + a.graph_steps
+    for t in range(100):
+        stp = s2 + t
         pkg.ntc(st, 1, 1, DT, equal_weight=True, tag=(stp << 4) | 1)
         pkg.convect_particles(st, 1, DT, tag=(stp << 4) | 2)
         pkg.sort_particles(st, 1)
         pkg.compute_props_sorted(st, sp, fetch=False)
-    kern = {}
-    for fam in ("ntc", "convect", "sort", "props"):
-        ms, cnt = st.timing_get(fam)
-        kern[fam] = ms
+    for f in ("ntc", "convect", "sort", "props"):
+        kern[f] = round(st.timing_get(f)[0], 5)
     st.timing(False)
     sc = st.scalars(1)
     assert sc["err"] == 0 and sc["n_total"] == n, sc
-    dom = max(kern, key=kern.get)
-    dur_s = kern[dom] * 1e-3
-    achieved = ALG_BYTES[dom] * n / dur_s / 1e9
+
+    # roofline of the dominant kernel: algorithmic bytes of the steps one launch
+    # runs over that launch's duration (HIP events on the library's stream; the
+    # library reports the launch duration divided by the steps it ran)
+    steps_per_launch = a.steps if path == "slab" else 1
+    kernel = "k_slab_couette" if path == "slab" else "step graph"
+    step_s = launch_ms * 1e-3
+    achieved = ALG_BYTES["step"] * n / step_s / 1e9
     peak = 8000.0
     roof = {"bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s", "frac": achieved / peak,
-            "traffic": None, "kernel": dom, "kernel_ms": kern[dom],
-            "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
-            "step_alg_GBps": ALG_BYTES["step"] * n / (step_ms_ev * 1e-3) / 1e9 if step_ms_ev else None}
+            "traffic": None, "kernel": kernel, "launch_ms": launch_ms * steps_per_launch,
+            "kernel_ms_per_step": launch_ms, "steps_per_launch": steps_per_launch,
+            "alg_bytes_per_particle_step": ALG_BYTES["step"]}
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -212,8 +228,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "1D Couette, argon VHS, 500 cells x 200 particles (100k), dt 2e-10 s, FP64; "
                                "per step ntc!, convection, counting sort, compute_props_sorted!",
-                   "n_particles": n, "n_cells": N_CELLS, "mode": "hipGraph" if graph else "eager",
-                   "step_ms_events": step_ms_ev},
+                   "n_particles": n, "n_cells": N_CELLS, "mode": path, "slab_shape": slab_info,
+                   "graph_ms_per_step": graph_ms, "step_kernels_ms": kern},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -141,9 +141,24 @@ int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, const int32_t
  * convection, sort, compute_props_sorted! every props_every steps
  * (test/test_1D_couette.jl:66-97).  Step t uses tags (t << 4) | 1 for NTC
  * and (t << 4) | 2 for convection.  mode 0 launches kernels one by one,
- * mode 1 replays a captured hipGraph of the step. */
+ * mode 1 replays a captured hipGraph of the step, mode 2 runs all steps in
+ * one cooperative launch with each slab of cells resident in LDS (equal
+ * weights; it falls back to mode 1 when the state is not the sorted state a
+ * sort leaves, or when a particle crosses more than one slab in a step).
+ * The chunked loop this replaces (collide, convect, sort per chunk, exchange
+ * between chunks, re-sort and props) is simulations/1D/
+ * couette_multithreaded.jl:97-161. */
 int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *V,
                      int32_t equal_weight, int32_t props_every, const double *masses, int32_t mode);
+/* Which path the last mz_couette_steps call took: 0 eager, 1 graph, 2 slab. */
+int mz_couette_last_path(mz_store *h);
+/* Slab shape of mode 2 (the reference's n_chunks, couette_multithreaded.jl:
+ * 13-14, 31): cells per slab (1..32) and particles per thread (1, 2, 4, 8) of
+ * the LDS capacity 256 x ppt per slab; 0 lets the library choose. */
+int mz_set_slab_shape(mz_store *h, int32_t cells_per_slab, int32_t particles_per_thread);
+/* info[0..3] = particles per thread, cells per slab, slabs, outcome (1 ran,
+ * -1 aborted and replayed on mode 1) of the last mode-2 launch. */
+int mz_get_slab_info(mz_store *h, int32_t *info);
 
 /* ------------------------------------------------------------------ timing */
 /* When enabled, the library brackets each launch of the named kernel family

@@ -55,6 +55,9 @@ _SIGS = {
     "mz_merge_nnls": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, f64, i32, i32, P(f64), i32, f64, i32, f64,
                                 i32, P(i32)]),
     "mz_couette_steps": (C.c_int, [C.c_void_p, i32, u64, f64, P(f64), i32, i32, P(f64), i32]),
+    "mz_couette_last_path": (C.c_int, [C.c_void_p]),
+    "mz_set_slab_shape": (C.c_int, [C.c_void_p, i32, i32]),
+    "mz_get_slab_info": (C.c_int, [C.c_void_p, P(i32)]),
     "mz_timing_enable": (C.c_int, [C.c_void_p, i32]),
     "mz_timing_get": (C.c_int, [C.c_void_p, C.c_char_p, P(f64), P(i64)]),
     "mz_timing_reset": (C.c_int, [C.c_void_p]),
@@ -344,6 +347,17 @@ class DeviceStore:
     def timing_reset(self):
         _check(load_library().mz_timing_reset(self.h))
 
+    # -- slab-resident step shape ------------------------------------------------
+    def set_slab_shape(self, cells_per_slab=0, particles_per_thread=0):
+        _check(load_library().mz_set_slab_shape(self.h, cells_per_slab, particles_per_thread))
+
+    def slab_info(self):
+        """(particles per thread, cells per slab, slabs, outcome) of the last
+        slab launch; outcome 1 = ran, -1 = aborted and replayed on the graph."""
+        out = (i32 * 4)()
+        _check(load_library().mz_get_slab_info(self.h, out))
+        return tuple(out)
+
 
 # ---------------------------------------------------------------------------
 # Operations with the reference's names
@@ -460,14 +474,27 @@ def merge_nnls_based(store, species, threshold, multi_index_moments, vref=1.0, s
     return status
 
 
+COUETTE_MODES = {"eager": 0, "graph": 1, "slab": 2}
+
+
 def couette_steps(store, n_steps, step0, dt, species_data=None, V=None, equal_weight=True, props_every=1,
-                  graph=True):
+                  graph=True, mode=None):
     """n_steps of the 1-D Couette loop (test/test_1D_couette.jl:66-97): ntc! in
-    every cell, convect_particles!, sort_particles!, compute_props_sorted!."""
+    every cell, convect_particles!, sort_particles!, compute_props_sorted!.
+
+    mode "eager" launches kernel by kernel, "graph" replays a captured
+    hipGraph of the step, "slab" runs all steps in one launch with each slab
+    of cells resident in LDS (the chunked loop of
+    simulations/1D/couette_multithreaded.jl:97-161); "slab" falls back to
+    "graph" where it cannot run.  Without mode, graph=True/False picks
+    "graph"/"eager".  Returns the path taken."""
     m = _masses(store, species_data)
     Vd = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
-    _check(load_library().mz_couette_steps(store.h, n_steps, step0, dt, _d(Vd), 1 if equal_weight else 0,
-                                           props_every, _d(m), 1 if graph else 0))
+    code = COUETTE_MODES[mode] if mode is not None else (1 if graph else 0)
+    L = load_library()
+    _check(L.mz_couette_steps(store.h, n_steps, step0, dt, _d(Vd), 1 if equal_weight else 0, props_every, _d(m),
+                              code))
+    return {v: k for k, v in COUETTE_MODES.items()}.get(L.mz_couette_last_path(store.h))
 
 
 def multi_index_moments(n):

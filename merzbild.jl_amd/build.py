@@ -14,11 +14,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmzhip.so")
-SOURCES = ["mz_store.hip", "mz_ntc.hip", "mz_step.hip", "mz_merge.hip"]
+SOURCES = ["mz_store.hip", "mz_ntc.hip", "mz_step.hip", "mz_merge.hip", "mz_slab.hip"]
 ARCH = os.environ.get("MZ_OFFLOAD_ARCH", "gfx950")
 
-# Packed FP32 instructions stay off: hipcc is told not to form them.
+# Packed FP32 instructions stay off: hipcc is told not to form them.  No
+# implicit multiply-add contraction: fused operations appear only where the
+# reference's @muladd blocks fuse them (written as fma), so the device
+# arithmetic is the oracle's up to the transcendental functions.
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-result",
+            "-ffp-contract=off",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-I", os.path.join(HERE, "..", "include")]
 

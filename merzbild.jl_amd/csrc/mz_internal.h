@@ -78,6 +78,11 @@ struct mz_store {
     hipGraph_t graph = nullptr;
     std::string graph_key;
     uint64_t *d_step = nullptr;
+    // slab-resident Couette kernel: requested shape (0 = automatic) and the
+    // last launch's {particles per thread, cells per slab, slabs, outcome}
+    int32_t slab_spc = 0, slab_ppt = 0;
+    int32_t slab_info[4] = {0, 0, 0, 0};
+    int32_t last_path = -1;   // of mz_couette_steps: 0 eager, 1 graph, 2 slab
 };
 
 MzViews mz_views(mz_store *h);
@@ -97,3 +102,5 @@ int mz_launch_ntc_equal(mz_store *h, int s1, int s2, double dt, const double *dV
                         int equal_weight, uint64_t tag, int rng_mode, int cell_lo, int cell_hi,
                         const uint64_t *step_ptr = nullptr);
 int mz_timing_end_n(mz_store *h, int64_t n);
+int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *dV, int32_t props_every,
+                    int *ran);

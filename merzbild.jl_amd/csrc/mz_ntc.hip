@@ -22,7 +22,7 @@
 // the order a sequential loop over cells produces; ids come from the emulated
 // free-id LIFO, growing `length` by DELTA_PARTICLES when it runs dry, as
 // resize! does (src/particles.jl:269-298, 311-332, 426-433).
-#include "mz_internal.h"
+#include "mz_phys.h"
 
 #include <algorithm>
 #include <vector>
@@ -82,9 +82,7 @@ __global__ void k_ntc(NtcArgs A) {
         st.k = 1;
     } else {
         double R = mz_rand(A.rp, st);
-        double ncf = A.single ? 0.5 * A.dt * (double)nla * (double)(nla - 1) * sgwm / A.V[c] + R
-                              : A.dt * (double)nla * (double)nlb * sgwm / A.V[c] + R;
-        n_coll = (int64_t)floor(ncf);
+        n_coll = (int64_t)floor(mz_ncoll_est(A.single, A.dt, nla, nlb, sgwm, A.V[c], R));
     }
     if (!(n_coll <= MZ_NTC_MAX_CANDIDATES)) {   // also catches NaN estimates
         atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH);
@@ -113,18 +111,16 @@ __global__ void k_ntc(NtcArgs A) {
         }
         double vi0 = pi.v0[pi.i], vi1 = pi.v1[pi.i], vi2 = pi.v2[pi.i];
         double vk0 = pk.v0[pk.i], vk1 = pk.v1[pk.i], vk2 = pk.v2[pk.i];
-        double g0 = vi0 - vk0, g1 = vi1 - vk1, g2 = vi2 - vk2;
-        double g = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
-        if (!(g > 2.220446049250313e-16)) continue;
+        double g0, g1, g2;
+        double g = mz_rel_speed(vi0, vi1, vi2, vk0, vk1, vk2, g0, g1, g2);
+        if (!(g > MZ_G_EPS)) continue;
         double wi = pi.w[pi.i], wk = pk.w[pk.i];
-        double sigma = A.factor * pow(g, A.expo);
-        double sgw = sigma * g * (wi > wk ? wi : wk);
+        double sgw = mz_vhs_sgw(A.factor, A.expo, g, wi, wk);
         sgwm = sgw > sgwm ? sgw : sgwm;
         if (!(mz_rand(A.rp, st) < sgw / sgwm)) continue;
         ++performed;
-        double vc0 = fma(A.mu1, vi0, A.mu2 * vk0);
-        double vc1 = fma(A.mu1, vi1, A.mu2 * vk1);
-        double vc2 = fma(A.mu1, vi2, A.mu2 * vk2);
+        double vc0, vc1, vc2;
+        mz_com(A.mu1, A.mu2, vi0, vi1, vi2, vk0, vk1, vk2, vc0, vc1, vc2);
         if (A.equal_weight || fabs(wi - wk) < A.dw_tol) {
             ++eqw;
         } else if (!A.may_split) {
@@ -160,18 +156,10 @@ __global__ void k_ntc(NtcArgs A) {
             }
         }
         // scatter_vhs!: phi first, then cos(theta)
-        double phi = 6.283185307179586 * mz_rand(A.rp, st);
-        double sphi, cphi;
-        sincos(phi, &sphi, &cphi);
-        double ct = 2.0 * mz_rand(A.rp, st) - 1.0;
-        double stt = sqrt(1.0 - ct * ct);
-        double gn0 = g * (stt * cphi), gn1 = g * (stt * sphi), gn2 = g * ct;
-        pi.v0[pi.i] = fma(A.mu2, gn0, vc0);
-        pi.v1[pi.i] = fma(A.mu2, gn1, vc1);
-        pi.v2[pi.i] = fma(A.mu2, gn2, vc2);
-        pk.v0[pk.i] = fma(-A.mu1, gn0, vc0);
-        pk.v1[pk.i] = fma(-A.mu1, gn1, vc1);
-        pk.v2[pk.i] = fma(-A.mu1, gn2, vc2);
+        double ui0, ui1, ui2, uk0, uk1, uk2;
+        mz_vhs_scatter(A.rp, st, A.mu1, A.mu2, g, vc0, vc1, vc2, ui0, ui1, ui2, uk0, uk1, uk2);
+        pi.v0[pi.i] = ui0; pi.v1[pi.i] = ui1; pi.v2[pi.i] = ui2;
+        pk.v0[pk.i] = uk0; pk.v1[pk.i] = uk1; pk.v2[pk.i] = uk2;
     }
     A.sgwm[c] = sgwm;
     int32_t *cn = A.cnt + (int64_t)c * 5;
@@ -195,9 +183,7 @@ __global__ void k_ntc_count(NtcArgs A, int32_t *__restrict__ ncoll) {
     MzStream st = mz_stream((uint64_t)c);
     double R = mz_rand(A.rp, st);
     double sgwm = A.sgwm[c];
-    double ncf = A.single ? 0.5 * A.dt * (double)nla * (double)(nla - 1) * sgwm / A.V[c] + R
-                          : A.dt * (double)nla * (double)nlb * sgwm / A.V[c] + R;
-    double f = floor(ncf);
+    double f = floor(mz_ncoll_est(A.single, A.dt, nla, nlb, sgwm, A.V[c], R));
     if (!(f <= (double)MZ_NTC_MAX_CANDIDATES)) {
         atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH);
         f = 0.0;

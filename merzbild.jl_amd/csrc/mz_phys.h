@@ -1,0 +1,137 @@
+// mz_phys.h -- per-particle and per-pair physics shared by the kernels that
+// launch one step phase at a time and by the slab-resident step kernel, so
+// both evaluate exactly the same floating-point expressions.
+//
+// Reference (Merzbild.jl, src/):
+//   compute_n_coll_single_species / _two_species  collisions/collision_ntc.jl:173-197
+//   sigma_vhs                                      collisions/collision_cross_sections.jl:179-181
+//   collide_2particles_vhs! acceptance             collisions/collision_ntc.jl:223-246
+//   compute_com!                                   collisions/collision_utils.jl:374-376
+//   scatter_vhs!                                   collisions/collision_scattering.jl:17-29
+//   convect_single_particle!                       convection/convection_1D.jl:17-54
+//   diffuse_reflection_x!, reflect_particle_x!     convection/boundary_conditions.jl:79-121
+//   get_cell (uniform 1-D grid)                    grids/grid_uniform1D.jl:97-99
+#pragma once
+
+#include "mz_internal.h"
+
+#define MZ_G_EPS 2.220446049250313e-16  // eps(Float64): the g > eps() test of ntc!
+
+// n_coll before the floor: 0.5 dt N (N - 1) (sigma g w)_max / V + R for one
+// species, dt N1 N2 (sigma g w)_max / V + R for two.
+__device__ inline double mz_ncoll_est(bool single, double dt, int32_t nla, int32_t nlb, double sgwm, double V,
+                                      double R) {
+    return single ? 0.5 * dt * (double)nla * (double)(nla - 1) * sgwm / V + R
+                  : dt * (double)nla * (double)nlb * sgwm / V + R;
+}
+
+// |v_i - v_k| with g = v_i - v_k
+__device__ inline double mz_rel_speed(double vi0, double vi1, double vi2, double vk0, double vk1, double vk2,
+                                      double &g0, double &g1, double &g2) {
+    g0 = vi0 - vk0;
+    g1 = vi1 - vk1;
+    g2 = vi2 - vk2;
+    return sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+}
+
+// sigma(g) g max(w_i, w_k), sigma(g) = C g^(1 - 2 omega) (expo = 1 - 2 omega)
+__device__ inline double mz_vhs_sgw(double factor, double expo, double g, double wi, double wk) {
+    double sigma = factor * pow(g, expo);
+    return sigma * g * (wi > wk ? wi : wk);
+}
+
+// centre-of-mass velocity, mu1 v_i + mu2 v_k as one fused multiply-add
+__device__ inline void mz_com(double mu1, double mu2, double vi0, double vi1, double vi2, double vk0, double vk1,
+                              double vk2, double &vc0, double &vc1, double &vc2) {
+    vc0 = fma(mu1, vi0, mu2 * vk0);
+    vc1 = fma(mu1, vi1, mu2 * vk1);
+    vc2 = fma(mu1, vi2, mu2 * vk2);
+}
+
+// scatter_vhs!: two draws, phi first, then cos(theta); isotropic post-
+// collision relative velocity of magnitude g about the centre of mass.
+__device__ inline void mz_vhs_scatter(const MzRngParams &rp, MzStream &st, double mu1, double mu2, double g,
+                                      double vc0, double vc1, double vc2, double &vi0, double &vi1, double &vi2,
+                                      double &vk0, double &vk1, double &vk2) {
+    double phi = 6.283185307179586 * mz_rand(rp, st);
+    double sphi, cphi;
+    sincos(phi, &sphi, &cphi);
+    double ct = 2.0 * mz_rand(rp, st) - 1.0;
+    double stt = sqrt(1.0 - ct * ct);
+    double gn0 = g * (stt * cphi), gn1 = g * (stt * sphi), gn2 = g * ct;
+    vi0 = fma(mu2, gn0, vc0);
+    vi1 = fma(mu2, gn1, vc1);
+    vi2 = fma(mu2, gn2, vc2);
+    vk0 = fma(-mu1, gn0, vc0);
+    vk1 = fma(-mu1, gn1, vc1);
+    vk2 = fma(-mu1, gn2, vc2);
+}
+
+// Diffuse re-emission from a wall: three draws (normal, tangential speed
+// factors clamped at 1e-50, then the tangential angle).
+__device__ inline void mz_diffuse_reflect(const MzRngParams &rp, MzStream &st, double &v0, double &v1, double &v2,
+                                          double vsq, double sgn, const double *wv) {
+    double R = fmax(1e-50, mz_rand(rp, st));
+    double vn = sgn * sqrt(-vsq * log(R));
+    R = fmax(1e-50, mz_rand(rp, st));
+    double vt = sqrt(-vsq * log(R));
+    R = 6.283185307179586 * mz_rand(rp, st);
+    double s, c;
+    sincos(R, &s, &c);
+    v0 = vn + wv[0];
+    v1 = s * vt + wv[1];
+    v2 = c * vt + wv[2];
+}
+
+// Free flight x + v dt (an explicit fma, as the reference's @muladd block)
+// with wall interactions while the particle is outside (0, L); the result is
+// clamped to [min_x, max_x].  A particle draws from stream (id - 1).
+__device__ inline void mz_convect_one(const MzRngParams &rp, const MzGrid &g, const MzWalls &wl, int sidx,
+                                      double dt, int32_t id, double &x0, double &v0, double &v1, double &v2,
+                                      int32_t *err) {
+    double t_rest = dt;
+    double x_old = x0;
+    double x_new = fma(v0, dt, x0);
+    int bounces = 0;
+    MzStream st = mz_stream((uint64_t)(id - 1));
+    while ((x_new >= g.L) || (x_new <= 0.0)) {
+        if (++bounces > MZ_MAX_BOUNCES) {
+            atomicOr(err, MZ_ERR_BOUNCES);
+            break;
+        }
+        int bc;
+        double wn;
+        if (x_new >= g.L) {
+            t_rest -= fabs((g.L - x_old) / v0);
+            bc = 1; wn = -1.0; x_old = g.L;
+        } else {
+            t_rest -= fabs(x_old / v0);
+            bc = 0; wn = 1.0; x_old = 0.0;
+        }
+        double acc = wl.acc[bc];
+        if (acc == 0.0) {
+            v0 = -v0;
+        } else if (acc == 1.0) {
+            mz_diffuse_reflect(rp, st, v0, v1, v2, wl.vsq[bc][sidx], wn, wl.v[bc]);
+        } else {
+            double R = mz_rand(rp, st);
+            if (R < acc) mz_diffuse_reflect(rp, st, v0, v1, v2, wl.vsq[bc][sidx], wn, wl.v[bc]);
+            else v0 = -v0;
+        }
+        x_new = fma(v0, t_rest, x_old);
+    }
+    if (x_new < g.min_x) x_new = g.min_x;
+    else if (x_new > g.max_x) x_new = g.max_x;
+    x0 = x_new;
+}
+
+// get_cell: floor(x / dx) + 1 (1-based); cells outside [1, n_cells] flag a
+// capacity error and are clamped so no kernel indexes out of range.
+__device__ inline int32_t mz_cell_of(double x0, const MzGrid &g, int32_t n_cells, int32_t *err) {
+    int32_t c = (int32_t)floor(x0 * g.inv_dx) + 1;
+    if (c < 1 || c > n_cells) {
+        atomicOr(err, MZ_ERR_CAPACITY);
+        c = c < 1 ? 1 : n_cells;
+    }
+    return c;
+}

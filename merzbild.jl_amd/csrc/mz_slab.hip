@@ -1,0 +1,788 @@
+// mz_slab.hip -- the 1-D Couette step with the particles resident in LDS.
+//
+// The reference's step (test/test_1D_couette.jl:66-97; simulations/1D/
+// couette_multithreaded.jl:97-161 runs it per chunk of cells, exchanging
+// particles between chunks after the sort) is
+//   ntc! in every cell -> convect_particles! -> sort_particles! ->
+//   compute_props_sorted!.
+// Here one workgroup owns a slab of consecutive cells and keeps that slab's
+// particles in LDS for all steps of a launch; only particles that leave the
+// slab travel, through a mailbox in HBM, to the neighbouring slab.  The
+// launch is cooperative so every slab is resident and a slab waits only for
+// its two neighbours (a flag per slab side, bounded spins, no grid barrier).
+//
+// Bit-exact order.  After a sort, the particles of a slab occupy one
+// contiguous range of logical positions in cell order.  The stable counting
+// sort (grid_sorting.jl:58-113) orders the particles of a new cell by their
+// previous position; a particle that moved at most into an adjacent slab
+// therefore lands after the arrivals from the left slab and before the
+// arrivals from the right slab, in the order it had.  Each slab rebuilds its
+// cells as [left arrivals | own particles | right arrivals], each part in
+// previous order, which is exactly the global stable sort restricted to the
+// slab.  A move across more than one slab, a mailbox or LDS overflow, or a
+// wait that exceeds its spin limit aborts the launch; the host then restores
+// the state it saved and runs the same steps with the step-by-step kernels,
+// which give the same results (every random stream is keyed by cell or
+// particle id and step, not by launch layout).
+//
+// The physics is the same inline code the step-by-step kernels use
+// (mz_phys.h), and props use the same lane-strided sums as k_props, so the
+// two paths agree bit for bit.
+#include "mz_phys.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#define SLAB_THREADS 256
+#define SLAB_MAXC 32                  // cells per slab
+#define SLAB_KMAX (SLAB_MAXC + 2)     // keys: local cells, left, right
+#define SLAB_MB_CAP 256               // particles per mailbox (slab side, step parity)
+#define SLAB_SPIN_LIMIT (1u << 22)    // polls per wait before the launch aborts
+#define SLAB_LINE 16                  // 64-bit words per flag (one 128-byte line)
+
+struct SlabArgs {
+    MzSpView a;                 // input particles, PIA, counters
+    double *bx0, *bx1, *bx2, *bv0, *bv1, *bv2, *bw;   // output particles
+    int32_t *bid;
+    MzGrid g;
+    MzWalls wl;
+    double dt, mu1, mu2, factor, expo, mass;
+    const double *V;            // cell volumes
+    double *sgwm;               // (sigma g w)_max per cell
+    int32_t *cnt;               // collision counters [cell][5]
+    MzRngParams rp;
+    uint64_t step0;
+    int32_t n_steps, props_every;
+    double *out_np, *out_n, *out_v, *out_T;
+    int32_t n_cells, spc, n_slabs, spx;
+    uint64_t *mb;               // mailboxes [slab][side][parity][field][SLAB_MB_CAP]
+    uint64_t *flag;             // [slab][side] x SLAB_LINE: [0] = step + 1, [2 + parity] = count
+    uint64_t *fin;              // [slab] x SLAB_LINE: [0] = 1 when done, [1] = counts
+    int32_t *abort_flag;
+};
+
+__device__ inline uint64_t *slab_mb(const SlabArgs &A, int slab, int side, int parity, int field) {
+    return A.mb + ((((size_t)slab * 2 + side) * 2 + parity) * 8 + field) * SLAB_MB_CAP;
+}
+
+__device__ inline void st_agent(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t ld_agent(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Every word that one slab writes and another reads (mailboxes, counts,
+// flags) is accessed with agent-scope relaxed atomics, which the hardware
+// keeps coherent across the XCDs' L2 caches by themselves.  Publishing
+// therefore needs only the order "payload stores complete, then flag store"
+// and reading only "flag load, then payload loads"; no agent-scope fence
+// (which writes back or invalidates a whole L2) is issued per step.
+//
+// Wait until this wave's outstanding memory operations have completed
+// (s_waitcnt vmcnt(0); expcnt and lgkmcnt left at their no-wait maxima).
+__device__ inline void slab_wait_mem() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Poll *p until (*p >> shift) >= want; false on abort or after the spin limit.
+__device__ inline bool slab_wait(const uint64_t *p, uint64_t want, int shift, const int32_t *abort_flag,
+                                 uint64_t &val) {
+    for (uint32_t it = 0; it < SLAB_SPIN_LIMIT; ++it) {
+        uint64_t v = ld_agent(p);
+        if ((v >> shift) >= want) {
+            val = v;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            return true;
+        }
+        if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return false;
+}
+
+__device__ inline void slab_raise_abort(const SlabArgs &A) {
+    __hip_atomic_store(A.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ntc_equal_weight! of one cell over its LDS segment [base, base + nla).
+__device__ inline void slab_ntc_cell(const SlabArgs &A, const MzRngParams &rp, int c, double Vc, int32_t base,
+                                     int32_t nla, double *sv0, double *sv1, double *sv2, const double *sw,
+                                     double &sgwm, int32_t &n_coll_out, int32_t &performed, int32_t &eqw) {
+    MzStream st = mz_stream((uint64_t)c);
+    double R = mz_rand(rp, st);
+    int64_t n_coll = (int64_t)floor(mz_ncoll_est(true, A.dt, nla, nla, sgwm, Vc, R));
+    if (!(n_coll <= MZ_NTC_MAX_CANDIDATES)) {
+        atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH);
+        n_coll = 0;
+    }
+    performed = 0;
+    eqw = 0;
+    for (int64_t cc = 0; cc < n_coll; ++cc) {
+        int32_t i = (int32_t)floor(mz_rand(rp, st) * (double)nla);
+        int32_t k = (int32_t)floor(mz_rand(rp, st) * (double)nla);
+        while (i == k) k = (int32_t)floor(mz_rand(rp, st) * (double)nla);
+        const int pi = base + i, pk = base + k;
+        double vi0 = sv0[pi], vi1 = sv1[pi], vi2 = sv2[pi];
+        double vk0 = sv0[pk], vk1 = sv1[pk], vk2 = sv2[pk];
+        double g0, g1, g2;
+        double g = mz_rel_speed(vi0, vi1, vi2, vk0, vk1, vk2, g0, g1, g2);
+        if (!(g > MZ_G_EPS)) continue;
+        double sgw = mz_vhs_sgw(A.factor, A.expo, g, sw[pi], sw[pk]);
+        sgwm = sgw > sgwm ? sgw : sgwm;
+        if (!(mz_rand(rp, st) < sgw / sgwm)) continue;
+        ++performed;
+        ++eqw;
+        double vc0, vc1, vc2;
+        mz_com(A.mu1, A.mu2, vi0, vi1, vi2, vk0, vk1, vk2, vc0, vc1, vc2);
+        double ui0, ui1, ui2, uk0, uk1, uk2;
+        mz_vhs_scatter(rp, st, A.mu1, A.mu2, g, vc0, vc1, vc2, ui0, ui1, ui2, uk0, uk1, uk2);
+        sv0[pi] = ui0; sv1[pi] = ui1; sv2[pi] = ui2;
+        sv0[pk] = uk0; sv1[pk] = uk1; sv2[pk] = uk2;
+    }
+    n_coll_out = (int32_t)n_coll;
+}
+
+template <int R>
+__global__ void __launch_bounds__(SLAB_THREADS) k_slab_couette(SlabArgs A) {
+    constexpr int P = R * SLAB_THREADS;   // LDS particle capacity
+    constexpr int NCH = P / 64;           // 64-particle chunks
+    static_assert(NCH <= 64, "chunk scan uses one wave");
+    __shared__ double sx0[P], sx1[P], sx2[P], sv0[P], sv1[P], sv2[P], sw[P];
+    __shared__ int32_t sid[P];
+    __shared__ int32_t tab[NCH][SLAB_KMAX];   // per chunk and key: count, then exclusive prefix
+    __shared__ int32_t tot[SLAB_KMAX];
+    __shared__ int32_t cst[SLAB_MAXC], ccn[SLAB_MAXC], runL[SLAB_MAXC], runR[SLAB_MAXC];
+    __shared__ int32_t arank[2][SLAB_MB_CAP], akey[2][SLAB_MB_CAP];
+    __shared__ int32_t s_abort, s_n, s_na[2];
+    __shared__ int64_t s_red[4][4];
+
+    const int b = blockIdx.x;
+    const int g = (b & 7) * A.spx + (b >> 3);   // consecutive slabs on one XCD
+    if (g >= A.n_slabs) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int clo = g * A.spc;
+    const int chi = min(A.n_cells, clo + A.spc) - 1;
+    const int ncl = chi - clo + 1;
+    const int K = ncl + 2, KL = ncl, KR = ncl + 1;
+    const bool has_left = g > 0, has_right = g + 1 < A.n_slabs;
+    const int left_lo = clo - A.spc;
+    const int right_hi = min(A.n_cells, chi + 1 + A.spc) - 1;
+    const uint64_t lt = mz_lanemask_lt();
+
+    // ---- load the slab: cells clo..chi are one contiguous range of slots
+    int32_t first = 0;
+    if (wave == 0) {
+        int32_t n1 = lane < ncl ? A.a.pia.n1[clo + lane] : 0;
+        int32_t s1 = lane < ncl ? A.a.pia.start1[clo + lane] : 0;
+        int32_t x = n1;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane < ncl) { cst[lane] = x - n1; ccn[lane] = n1; }
+        uint64_t ne = __ballot(n1 > 0);
+        int fl = ne ? __ffsll((unsigned long long)ne) - 1 : 0;
+        int32_t s_first = __shfl(s1, fl, 64);
+        if (lane == 63) { s_n = x; s_abort = x > P; }
+        if (lane == 0) s_red[0][0] = ne ? s_first - 1 : 0;
+    }
+    __syncthreads();
+    first = (int32_t)s_red[0][0];
+    int32_t n_loc = s_n;
+    if (s_abort) {
+        if (tid == 0) slab_raise_abort(A);
+        return;
+    }
+    for (int i = tid; i < n_loc; i += SLAB_THREADS) {
+        const int64_t p = (int64_t)first + i;
+        sx0[i] = A.a.x0[p]; sx1[i] = A.a.x1[p]; sx2[i] = A.a.x2[p];
+        sv0[i] = A.a.v0[p]; sv1[i] = A.a.v1[p]; sv2[i] = A.a.v2[p];
+        sw[i] = A.a.w[p];
+        sid[i] = A.a.id[p];
+    }
+    // NTC lanes: cell k runs on lane k / 4 of wave k % 4
+    const int my_k = lane * 4 + wave;
+    const bool ntc_lane = lane < SLAB_MAXC / 4 && my_k < ncl;
+    double my_sgwm = ntc_lane ? A.sgwm[clo + my_k] : 0.0;
+    const double my_V = ntc_lane ? A.V[clo + my_k] : 1.0;
+    int32_t my_nla = 0, my_ncoll = 0, my_perf = 0, my_eqw = 0;
+
+    double px0[R], px1[R], px2[R], pv0[R], pv1[R], pv2[R], pw[R];
+    int32_t pid[R], pkey[R], prank[R], pcell[R];
+    int32_t n_prev = n_loc;
+    bool aborted = false;
+    __syncthreads();
+
+    for (int32_t t = 0; t < A.n_steps; ++t) {
+        const uint64_t step = A.step0 + (uint64_t)t;
+        const int q = t & 1;
+        n_prev = n_loc;
+        // ---- 1. ntc_equal_weight! per cell; clear the ranking tables
+        for (int j = tid; j < NCH * SLAB_KMAX; j += SLAB_THREADS) (&tab[0][0])[j] = 0;
+        if (tid < SLAB_MAXC) { runL[tid] = 0; runR[tid] = 0; }
+        if (ntc_lane) {
+            MzRngParams rp = A.rp;
+            rp.tag = (step << 4) | 1u;
+            my_nla = ccn[my_k];
+            slab_ntc_cell(A, rp, clo + my_k, my_V, cst[my_k], my_nla, sv0, sv1, sv2, sw, my_sgwm, my_ncoll,
+                          my_perf, my_eqw);
+        }
+        __syncthreads();
+        // ---- 2. convection, new cell, rank inside 64-particle chunks
+        {
+            MzRngParams rp = A.rp;
+            rp.tag = (step << 4) | 2u;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = r * SLAB_THREADS + tid;
+                const bool valid = i < n_loc;
+                pkey[r] = -1;
+                prank[r] = 0;
+                if (valid) {
+                    px0[r] = sx0[i]; px1[r] = sx1[i]; px2[r] = sx2[i];
+                    pv0[r] = sv0[i]; pv1[r] = sv1[i]; pv2[r] = sv2[i];
+                    pw[r] = sw[i];
+                    pid[r] = sid[i];
+                    double x_new = fma(pv0[r], A.dt, px0[r]);
+                    if (x_new < A.g.L && x_new > 0.0) {
+                        px0[r] = x_new < A.g.min_x ? A.g.min_x : (x_new > A.g.max_x ? A.g.max_x : x_new);
+                    } else {
+                        mz_convect_one(rp, A.g, A.wl, 0, A.dt, pid[r], px0[r], pv0[r], pv1[r], pv2[r],
+                                       &A.a.ctr->err);
+                    }
+                    const int c = mz_cell_of(px0[r], A.g, A.n_cells, &A.a.ctr->err) - 1;
+                    pcell[r] = c + 1;
+                    int key;
+                    if (c < clo) {
+                        key = KL;
+                        if (c < left_lo) s_abort = 1;
+                    } else if (c > chi) {
+                        key = KR;
+                        if (c > right_hi) s_abort = 1;
+                    } else {
+                        key = c - clo;
+                    }
+                    pkey[r] = key;
+                }
+                const int ch = r * 4 + wave;
+                uint64_t active = __ballot(valid);
+                while (active) {
+                    const int leader = __ffsll((unsigned long long)active) - 1;
+                    const int lk = __shfl(pkey[r], leader, 64);
+                    const uint64_t m = __ballot(valid && pkey[r] == lk);
+                    if (valid && pkey[r] == lk) prank[r] = __popcll(m & lt);
+                    if (lane == leader) tab[ch][lk] = __popcll(m);
+                    active &= ~m;
+                }
+            }
+        }
+        __syncthreads();
+        // ---- 3. per key, exclusive prefix over chunks
+        for (int kk = wave; kk < K; kk += 4) {
+            int32_t v = lane < NCH ? tab[lane][kk] : 0;
+            int32_t x = v;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                int32_t y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            if (lane < NCH) tab[lane][kk] = x - v;
+            if (lane == 63) tot[kk] = x;
+        }
+        __syncthreads();
+        // ---- 4. ranks in the slab's previous order; leavers go to the mailboxes
+        const bool bad = s_abort || tot[KL] > SLAB_MB_CAP || tot[KR] > SLAB_MB_CAP;
+        if (!bad) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (pkey[r] < 0) continue;
+                prank[r] += tab[r * 4 + wave][pkey[r]];
+                if (pkey[r] >= KL) {
+                    const int side = pkey[r] == KL ? 0 : 1;
+                    const int j = prank[r];
+                    st_agent(slab_mb(A, g, side, q, 0) + j, (uint64_t)__double_as_longlong(px0[r]));
+                    st_agent(slab_mb(A, g, side, q, 1) + j, (uint64_t)__double_as_longlong(px1[r]));
+                    st_agent(slab_mb(A, g, side, q, 2) + j, (uint64_t)__double_as_longlong(px2[r]));
+                    st_agent(slab_mb(A, g, side, q, 3) + j, (uint64_t)__double_as_longlong(pv0[r]));
+                    st_agent(slab_mb(A, g, side, q, 4) + j, (uint64_t)__double_as_longlong(pv1[r]));
+                    st_agent(slab_mb(A, g, side, q, 5) + j, (uint64_t)__double_as_longlong(pv2[r]));
+                    st_agent(slab_mb(A, g, side, q, 6) + j, (uint64_t)__double_as_longlong(pw[r]));
+                    st_agent(slab_mb(A, g, side, q, 7) + j, ((uint64_t)(uint32_t)pid[r] << 32) | (uint32_t)pcell[r]);
+                }
+            }
+        }
+        slab_wait_mem();
+        __syncthreads();
+        if (bad) {
+            if (tid == 0) slab_raise_abort(A);
+            aborted = true;
+            break;
+        }
+        // ---- 5. publish, then wait for the neighbours' mailboxes of this step.
+        // The count goes to the slot of this step's parity: a neighbour may
+        // already publish the next step before this slab reads the flag, but
+        // it cannot reach the step after that (which reuses the slot) before
+        // this slab has published the next step itself.
+        const uint64_t mark = (uint64_t)(t + 1);
+        if (tid == 0) {
+            uint64_t *fl = A.flag + ((size_t)g * 2 + 0) * SLAB_LINE;
+            uint64_t *fr = A.flag + ((size_t)g * 2 + 1) * SLAB_LINE;
+            if (has_left) st_agent(fl + 2 + q, (uint64_t)tot[KL]);
+            if (has_right) st_agent(fr + 2 + q, (uint64_t)tot[KR]);
+            slab_wait_mem();
+            if (has_left) st_agent(fl, mark);
+            if (has_right) st_agent(fr, mark);
+        }
+        if (lane == 0 && wave < 2) {
+            const bool has = wave == 0 ? has_left : has_right;
+            int32_t na = 0;
+            if (has) {
+                const int src = wave == 0 ? g - 1 : g + 1;
+                const int side = wave == 0 ? 1 : 0;
+                const uint64_t *f = A.flag + ((size_t)src * 2 + side) * SLAB_LINE;
+                uint64_t v = 0;
+                if (slab_wait(f, mark, 0, A.abort_flag, v)) {
+                    na = (int32_t)ld_agent(f + 2 + q);
+                    if (na < 0 || na > SLAB_MB_CAP) {
+                        s_abort = 1;
+                        na = 0;
+                    }
+                } else {
+                    s_abort = 1;
+                }
+            }
+            s_na[wave] = na;
+        }
+        __syncthreads();
+        if (s_abort) {
+            if (tid == 0) slab_raise_abort(A);
+            aborted = true;
+            break;
+        }
+        // ---- 6. rank the arrivals by cell (wave 0: from the left, wave 1: from
+        // the right); the first 64 of each side stay in registers until the scatter
+        uint64_t aw[8];
+        int a_key = 0, a_rank = 0;
+        if (wave < 2) {
+            const int na = s_na[wave];
+            const int src = wave == 0 ? g - 1 : g + 1;
+            const int side = wave == 0 ? 1 : 0;
+            volatile int32_t *run = wave == 0 ? runL : runR;
+            for (int j0 = 0; j0 < na; j0 += 64) {
+                const int j = j0 + lane;
+                const bool valid = j < na;
+                int key = 0;
+                if (valid) {
+                    uint64_t w7;
+                    if (j0 == 0) {
+#pragma unroll
+                        for (int f = 0; f < 8; ++f) aw[f] = ld_agent(slab_mb(A, src, side, q, f) + j);
+                        w7 = aw[7];
+                    } else {
+                        w7 = ld_agent(slab_mb(A, src, side, q, 7) + j);
+                    }
+                    key = (int32_t)(uint32_t)(w7 & 0xFFFFFFFFu) - 1 - clo;
+                    if (key < 0 || key >= ncl) {
+                        s_abort = 1;
+                        key = key < 0 ? 0 : ncl - 1;
+                    }
+                }
+                uint64_t active = __ballot(valid);
+                while (active) {
+                    const int leader = __ffsll((unsigned long long)active) - 1;
+                    const int lk = __shfl(key, leader, 64);
+                    const uint64_t m = __ballot(valid && key == lk);
+                    const int32_t bk = run[lk];
+                    if (valid && key == lk) {
+                        arank[wave][j] = bk + __popcll(m & lt);
+                        akey[wave][j] = lk;
+                        if (j0 == 0) { a_key = lk; a_rank = bk + __popcll(m & lt); }
+                    }
+                    if (lane == leader) run[lk] = bk + __popcll(m);
+                    active &= ~m;
+                }
+            }
+        }
+        __syncthreads();
+        // ---- 7. new cell starts: [left arrivals | own | right arrivals] per cell
+        if (wave == 0) {
+            int32_t cnum = lane < ncl ? runL[lane] + tot[lane] + runR[lane] : 0;
+            int32_t x = cnum;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                int32_t y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            if (lane < ncl) { cst[lane] = x - cnum; ccn[lane] = cnum; }
+            if (lane == 63) {
+                s_n = x;
+                if (x > P) s_abort = 1;
+            }
+        }
+        __syncthreads();
+        if (s_abort) {
+            if (tid == 0) slab_raise_abort(A);
+            aborted = true;
+            break;
+        }
+        n_loc = s_n;
+        // ---- 8. scatter into the new order
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int key = pkey[r];
+            if (key < 0 || key >= ncl) continue;
+            const int pos = cst[key] + runL[key] + prank[r];
+            sx0[pos] = px0[r]; sx1[pos] = px1[r]; sx2[pos] = px2[r];
+            sv0[pos] = pv0[r]; sv1[pos] = pv1[r]; sv2[pos] = pv2[r];
+            sw[pos] = pw[r];
+            sid[pos] = pid[r];
+        }
+        if (wave < 2 && lane < s_na[wave]) {
+            const int pos = cst[a_key] + (wave == 0 ? 0 : runL[a_key] + tot[a_key]) + a_rank;
+            sx0[pos] = __longlong_as_double((long long)aw[0]);
+            sx1[pos] = __longlong_as_double((long long)aw[1]);
+            sx2[pos] = __longlong_as_double((long long)aw[2]);
+            sv0[pos] = __longlong_as_double((long long)aw[3]);
+            sv1[pos] = __longlong_as_double((long long)aw[4]);
+            sv2[pos] = __longlong_as_double((long long)aw[5]);
+            sw[pos] = __longlong_as_double((long long)aw[6]);
+            sid[pos] = (int32_t)(aw[7] >> 32);
+        }
+        {
+            // arrivals beyond the first 64 of a side are read again
+            const int xl = max(s_na[0] - 64, 0), xr = max(s_na[1] - 64, 0);
+            for (int j = tid; j < xl + xr; j += SLAB_THREADS) {
+                const int s = j < xl ? 0 : 1;
+                const int jj = 64 + (s == 0 ? j : j - xl);
+                const int key = akey[s][jj];
+                const int pos = cst[key] + (s == 0 ? 0 : runL[key] + tot[key]) + arank[s][jj];
+                const int src = s == 0 ? g - 1 : g + 1;
+                const int side = s == 0 ? 1 : 0;
+                sx0[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 0) + jj));
+                sx1[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 1) + jj));
+                sx2[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 2) + jj));
+                sv0[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 3) + jj));
+                sv1[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 4) + jj));
+                sv2[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 5) + jj));
+                sw[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 6) + jj));
+                sid[pos] = (int32_t)(ld_agent(slab_mb(A, src, side, q, 7) + jj) >> 32);
+            }
+        }
+        __syncthreads();
+        // ---- 9. compute_props_sorted!: one wave per cell, the sums of k_props
+        if (A.props_every > 0 && ((t + 1) % A.props_every) == 0) {
+            for (int k = wave; k < ncl; k += 4) {
+                const int32_t st1 = cst[k], tn = ccn[k];
+                double sxs = 0.0, sys = 0.0, szs = 0.0;
+                if (tn > 0) { sxs = sv0[st1]; sys = sv1[st1]; szs = sv2[st1]; }
+                double n = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, e = 0.0;
+                for (int32_t i = lane; i < tn; i += 64) {
+                    const int p = st1 + i;
+                    double w = sw[p];
+                    double d0 = sv0[p] - sxs, d1 = sv1[p] - sys, d2 = sv2[p] - szs;
+                    n += w;
+                    a0 = fma(w, d0, a0);
+                    a1 = fma(w, d1, a1);
+                    a2 = fma(w, d2, a2);
+                    e = fma(w, d0 * d0 + d1 * d1 + d2 * d2, e);
+                }
+                n = mz_wave_sum(n);
+                a0 = mz_wave_sum(a0);
+                a1 = mz_wave_sum(a1);
+                a2 = mz_wave_sum(a2);
+                e = mz_wave_sum(e);
+                if (lane == 0) {
+                    double v0 = 0.0, v1 = 0.0, v2 = 0.0, T = 0.0;
+                    if (n > 0.0) {
+                        double m0 = a0 / n, m1 = a1 / n, m2 = a2 / n;
+                        v0 = sxs + m0; v1 = sys + m1; v2 = szs + m2;
+                        double E = e - n * (m0 * m0 + m1 * m1 + m2 * m2);
+                        if (E < 0.0) E = 0.0;
+                        E *= 0.5 * A.mass / (n * MZ_KB);
+                        T = (2.0 / 3.0) * E;
+                    }
+                    const int c = clo + k;
+                    A.out_np[c] = (double)tn;
+                    A.out_n[c] = n;
+                    A.out_v[3 * c] = v0;
+                    A.out_v[3 * c + 1] = v1;
+                    A.out_v[3 * c + 2] = v2;
+                    A.out_T[c] = T;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (aborted) return;
+
+    // ---- publish the slab's counts, then place it after all earlier slabs
+    if (tid == 0) {
+        st_agent(A.fin + (size_t)g * SLAB_LINE + 1, ((uint64_t)(uint32_t)n_prev << 32) | (uint32_t)n_loc);
+        slab_wait_mem();
+        st_agent(A.fin + (size_t)g * SLAB_LINE, (uint64_t)1);
+    }
+    int64_t pre_prev = 0, pre_new = 0, all_new = 0;
+    bool ok = true;
+    for (int s = tid; s < A.n_slabs; s += SLAB_THREADS) {
+        uint64_t v = 0;
+        if (!slab_wait(A.fin + (size_t)s * SLAB_LINE, 1, 0, A.abort_flag, v)) {
+            ok = false;
+            break;
+        }
+        const uint64_t cn = ld_agent(A.fin + (size_t)s * SLAB_LINE + 1);
+        const int64_t cp = (int64_t)(cn >> 32), cnew = (int64_t)(cn & 0xFFFFFFFFu);
+        if (s < g) { pre_prev += cp; pre_new += cnew; }
+        all_new += cnew;
+    }
+    if (!ok) s_abort = 1;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        pre_prev += __shfl_xor(pre_prev, o, 64);
+        pre_new += __shfl_xor(pre_new, o, 64);
+        all_new += __shfl_xor(all_new, o, 64);
+    }
+    if (lane == 0) { s_red[wave][0] = pre_prev; s_red[wave][1] = pre_new; s_red[wave][2] = all_new; }
+    __syncthreads();
+    if (s_abort) {
+        if (tid == 0) slab_raise_abort(A);
+        return;
+    }
+    const int64_t off_prev = s_red[0][0] + s_red[1][0] + s_red[2][0] + s_red[3][0];
+    const int64_t off_new = s_red[0][1] + s_red[1][1] + s_red[2][1] + s_red[3][1];
+    const int64_t total = s_red[0][2] + s_red[1][2] + s_red[2][2] + s_red[3][2];
+    if (off_new + n_loc > A.a.cap || off_prev + n_prev > A.a.cap) {
+        // counts that do not add up to the store: flag it, write nothing
+        if (tid == 0) atomicOr(&A.a.ctr->err, MZ_ERR_CAPACITY);
+        return;
+    }
+    for (int i = tid; i < n_loc; i += SLAB_THREADS) {
+        const int64_t p = off_new + i;
+        A.bx0[p] = sx0[i]; A.bx1[p] = sx1[i]; A.bx2[p] = sx2[i];
+        A.bv0[p] = sv0[i]; A.bv1[p] = sv1[i]; A.bv2[p] = sv2[i];
+        A.bw[p] = sw[i];
+        A.bid[p] = sid[i];
+    }
+    // pv.cell keeps the cells of the last sort's input order
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = r * SLAB_THREADS + tid;
+        if (i < n_prev) A.a.key[off_prev + i] = pcell[r];
+    }
+    if (tid < ncl) {
+        const int c = clo + tid;
+        const int32_t cnum = ccn[tid];
+        const int32_t s1 = (int32_t)off_new + cst[tid];
+        A.a.pia.start1[c] = cnum > 0 ? s1 + 1 : 0;
+        A.a.pia.end1[c] = cnum > 0 ? s1 + cnum : -1;
+        A.a.pia.n1[c] = cnum;
+        A.a.pia.nloc[c] = cnum;
+        A.a.pia.start2[c] = 0;
+        A.a.pia.end2[c] = -1;
+        A.a.pia.n2[c] = 0;
+    }
+    if (ntc_lane) {
+        const int c = clo + my_k;
+        A.sgwm[c] = my_sgwm;
+        int32_t *cn = A.cnt + (int64_t)c * 5;
+        cn[0] = my_nla;
+        cn[1] = my_nla;
+        cn[2] = my_ncoll;
+        cn[3] = my_perf;
+        cn[4] = my_eqw;
+    }
+    if (g == 0 && tid == 0) {
+        if (total != (int64_t)A.a.ctr->n_total) atomicOr(&A.a.ctr->err, MZ_ERR_CAPACITY);
+        A.a.ctr->contiguous = 1;
+        A.a.ctr->n_pos = (int32_t)total;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+namespace {
+
+typedef void (*SlabKernel)(SlabArgs);
+
+struct SlabShape {
+    int R = 0, spc = 0, n_slabs = 0, spx = 0, grid = 0;
+    SlabKernel fn = nullptr;
+};
+
+SlabKernel slab_kernel(int R) {
+    switch (R) {
+    case 1: return k_slab_couette<1>;
+    case 2: return k_slab_couette<2>;
+    case 4: return k_slab_couette<4>;
+    case 8: return k_slab_couette<8>;
+    default: return nullptr;
+    }
+}
+
+// Largest per-slab occupancy for a given number of cells per slab.
+int32_t max_slab_particles(const std::vector<int32_t> &n1, int spc) {
+    int32_t best = 0;
+    for (size_t c0 = 0; c0 < n1.size(); c0 += spc) {
+        int32_t s = 0;
+        for (size_t c = c0; c < std::min(n1.size(), c0 + (size_t)spc); ++c) s += n1[c];
+        best = std::max(best, s);
+    }
+    return best;
+}
+
+}  // namespace
+
+extern "C" int mz_set_slab_shape(mz_store *h, int32_t cells_per_slab, int32_t particles_per_thread) {
+    MZ_CHECK(cells_per_slab >= 0 && cells_per_slab <= SLAB_MAXC, "slab shape: cells per slab out of range");
+    MZ_CHECK(particles_per_thread == 0 || slab_kernel(particles_per_thread) != nullptr,
+             "slab shape: particles per thread must be 0, 1, 2, 4 or 8");
+    h->slab_spc = cells_per_slab;
+    h->slab_ppt = particles_per_thread;
+    return 0;
+}
+
+extern "C" int mz_get_slab_info(mz_store *h, int32_t *info) {
+    MZ_CHECK(info != nullptr, "slab info: NULL output");
+    for (int k = 0; k < 4; ++k) info[k] = h->slab_info[k];
+    return 0;
+}
+
+// Runs n_steps on the slab kernel.  *ran = 1 when it did; 0 when the state or
+// the shape does not allow it, or when the launch aborted (state restored).
+int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *dV, int32_t props_every,
+                    int *ran) {
+    *ran = 0;
+    MzSpecies &s = h->sp[0];
+    const int nc = h->n_cells;
+    if (n_steps <= 0 || n_steps >= (1 << 28)) return 0;
+    // precondition: the state a sort leaves (group 1 only, consecutive in cell order)
+    MzCounters c;
+    int r = mz_read_counters(h, 1, &c);
+    if (r) return r;
+    if (!c.contiguous || c.err) return 0;
+    std::vector<int32_t> n1(nc), st1(nc), n2(nc);
+    MZ_HIP(hipMemcpy(n1.data(), s.pia.n1, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
+    MZ_HIP(hipMemcpy(st1.data(), s.pia.start1, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
+    MZ_HIP(hipMemcpy(n2.data(), s.pia.n2, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
+    int64_t pos = 1;
+    for (int k = 0; k < nc; ++k) {
+        if (n2[k] != 0 || n1[k] < 0) return 0;
+        if (n1[k] > 0 && st1[k] != pos) return 0;
+        pos += n1[k];
+    }
+    if (pos - 1 != c.n_total) return 0;
+
+    // shape: the fewest particles per thread whose LDS slab holds twice the
+    // current largest slab, with as many slabs as can be resident
+    int dev = 0, n_cu = 0;
+    MZ_HIP(hipGetDevice(&dev));
+    MZ_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int Rs[4] = {1, 2, 4, 8};
+    SlabShape sh;
+    for (int R : Rs) {
+        if (h->slab_ppt && R != h->slab_ppt) continue;
+        SlabKernel fn = slab_kernel(R);
+        int per_cu = 0;
+        MZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, SLAB_THREADS, 0));
+        const int max_wg = per_cu * n_cu;
+        if (max_wg <= 0) continue;
+        int spc = h->slab_spc ? h->slab_spc : (nc + max_wg - 1) / max_wg;
+        if (spc > SLAB_MAXC) continue;
+        const int n_slabs = (nc + spc - 1) / spc;
+        const int spx = (n_slabs + 7) / 8;
+        if (8 * spx > max_wg) continue;
+        const int64_t P = (int64_t)R * SLAB_THREADS;
+        if (2 * (int64_t)max_slab_particles(n1, spc) > P && !h->slab_ppt) continue;
+        if ((int64_t)max_slab_particles(n1, spc) > P) continue;
+        sh.R = R; sh.spc = spc; sh.n_slabs = n_slabs; sh.spx = spx; sh.grid = 8 * spx; sh.fn = fn;
+        break;
+    }
+    if (!sh.fn) return 0;
+
+    // scratch: mailboxes, flags, abort word, and a copy of what the kernel
+    // overwrites before it can abort (PIA, counters, collision factors)
+    const size_t mb_words = (size_t)sh.n_slabs * 2 * 2 * 8 * SLAB_MB_CAP;
+    const size_t fl_words = (size_t)sh.n_slabs * 2 * SLAB_LINE;
+    const size_t fin_words = (size_t)sh.n_slabs * SLAB_LINE;
+    uint64_t *mb = (uint64_t *)mz_scratch(h, "slab_mb", mb_words * 8);
+    uint64_t *fl = (uint64_t *)mz_scratch(h, "slab_flags", (fl_words + fin_words + SLAB_LINE) * 8);
+    const size_t bak_bytes = (size_t)nc * (7 * sizeof(int32_t) + sizeof(double) + 5 * sizeof(int32_t)) +
+                             sizeof(MzCounters);
+    char *bak = (char *)mz_scratch(h, "slab_backup", bak_bytes);
+    MZ_CHECK(mb && fl && bak, "couette slab: scratch allocation failed");
+    uint64_t *fin = fl + fl_words;
+    int32_t *abort_flag = (int32_t *)(fin + fin_words);
+    MZ_HIP(hipMemsetAsync(fl, 0, (fl_words + fin_words + SLAB_LINE) * 8, h->stream));
+    int32_t *pia_arr[7] = {s.pia.start1, s.pia.end1, s.pia.n1, s.pia.start2, s.pia.end2, s.pia.n2, s.pia.nloc};
+    size_t off = 0;
+    for (auto p : pia_arr) {
+        MZ_HIP(hipMemcpyAsync(bak + off, p, nc * sizeof(int32_t), hipMemcpyDeviceToDevice, h->stream));
+        off += nc * sizeof(int32_t);
+    }
+    MZ_HIP(hipMemcpyAsync(bak + off, h->d_sgwm, nc * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    off += nc * sizeof(double);
+    MZ_HIP(hipMemcpyAsync(bak + off, h->d_cfcnt, nc * 5 * sizeof(int32_t), hipMemcpyDeviceToDevice, h->stream));
+    off += nc * 5 * sizeof(int32_t);
+    MZ_HIP(hipMemcpyAsync(bak + off, s.ctr, sizeof(MzCounters), hipMemcpyDeviceToDevice, h->stream));
+
+    SlabArgs A{};
+    A.a = mz_view(s);
+    A.bx0 = s.xb[0]; A.bx1 = s.xb[1]; A.bx2 = s.xb[2];
+    A.bv0 = s.vb[0]; A.bv1 = s.vb[1]; A.bv2 = s.vb[2];
+    A.bw = s.wb; A.bid = s.idb;
+    A.g = h->grid;
+    A.wl = h->walls;
+    A.dt = dt;
+    const MzInteraction &it = h->inter[0];
+    A.mu1 = it.mu1; A.mu2 = it.mu2; A.factor = it.factor; A.expo = it.expo;
+    MZ_HIP(hipMemcpy(&A.mass, h->d_masses, sizeof(double), hipMemcpyDeviceToHost));
+    A.V = dV;
+    A.sgwm = h->d_sgwm;
+    A.cnt = h->d_cfcnt;
+    A.rp = mz_rng_params(h, 0, 0, 0);
+    A.step0 = step0;
+    A.n_steps = n_steps;
+    A.props_every = props_every;
+    A.out_np = h->d_np; A.out_n = h->d_n; A.out_v = h->d_v; A.out_T = h->d_T;
+    A.n_cells = nc;
+    A.spc = sh.spc;
+    A.n_slabs = sh.n_slabs;
+    A.spx = sh.spx;
+    A.mb = mb;
+    A.flag = fl;
+    A.fin = fin;
+    A.abort_flag = abort_flag;
+    void *args[] = {&A};
+    mz_timing_begin(h, "slab");
+    MZ_HIP(hipLaunchCooperativeKernel((const void *)sh.fn, dim3(sh.grid), dim3(SLAB_THREADS), args, 0, h->stream));
+    mz_timing_end_n(h, n_steps);
+    int32_t aborted = 0;
+    MZ_HIP(hipMemcpyAsync(&aborted, abort_flag, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    h->slab_info[0] = sh.R;
+    h->slab_info[1] = sh.spc;
+    h->slab_info[2] = sh.n_slabs;
+    h->slab_info[3] = aborted ? -1 : 1;
+    if (aborted) {
+        off = 0;
+        for (auto p : pia_arr) {
+            MZ_HIP(hipMemcpyAsync(p, bak + off, nc * sizeof(int32_t), hipMemcpyDeviceToDevice, h->stream));
+            off += nc * sizeof(int32_t);
+        }
+        MZ_HIP(hipMemcpyAsync(h->d_sgwm, bak + off, nc * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        off += nc * sizeof(double);
+        MZ_HIP(hipMemcpyAsync(h->d_cfcnt, bak + off, nc * 5 * sizeof(int32_t), hipMemcpyDeviceToDevice, h->stream));
+        off += nc * 5 * sizeof(int32_t);
+        MZ_HIP(hipMemcpyAsync(s.ctr, bak + off, sizeof(MzCounters), hipMemcpyDeviceToDevice, h->stream));
+        MZ_HIP(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    mz_swap_buffers(s);
+    *ran = 1;
+    return 0;
+}

@@ -46,6 +46,8 @@ static int enqueue_step(mz_store *h, const double *dV, double dt, int do_props, 
     return 0;
 }
 
+extern "C" int mz_couette_last_path(mz_store *h) { return h->last_path; }
+
 extern "C" int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *V,
                                 int32_t equal_weight, int32_t props_every, const double *masses, int32_t mode) {
     MZ_CHECK(h->n_species == 1, "couette_steps: single-species stores only");
@@ -68,7 +70,20 @@ extern "C" int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, do
         r = mz_squash(h, 1);
         if (r) return r;
     }
+    if (mode == 2 && equal_weight) {
+        // slab-resident kernel; falls through to the graph when the state or
+        // shape does not allow it or when the launch aborted (state restored)
+        int ran = 0;
+        r = mz_couette_slab(h, n_steps, step0, dt, dV, props_every, &ran);
+        if (r) return r;
+        if (ran) {
+            h->last_path = 2;
+            return 0;
+        }
+        mode = 1;
+    }
     const bool graph_ok = mode == 1 && equal_weight && (props_every == 0 || props_every == 1);
+    h->last_path = graph_ok ? 1 : 0;
     if (!graph_ok) {
         // eager launches (also the path for ntc! with weight splitting)
         mz_timing_begin(h, "step");

@@ -8,7 +8,7 @@
 //   reflect/diffuse walls  src/convection/boundary_conditions.jl:79-121
 //   compute_props!         src/properties/physical_props.jl:104-154
 //   compute_props_sorted!  src/properties/physical_props.jl:317-454
-#include "mz_internal.h"
+#include "mz_phys.h"
 
 #include <algorithm>
 #include <cmath>
@@ -519,12 +519,7 @@ __global__ void k_sort_hist(MzSpView sp, MzGrid g, int32_t compute_cells, int32_
         if (p < n) {
             int32_t c;
             if (compute_cells) {
-                // get_cell (grid_uniform1D.jl:97-99): floor(x * inv_dx) + 1
-                c = (int32_t)floor(sp.x0[p] * g.inv_dx) + 1;
-                if (c < 1 || c > n_cells) {
-                    atomicOr(&sp.ctr->err, MZ_ERR_CAPACITY);
-                    c = c < 1 ? 1 : n_cells;
-                }
+                c = mz_cell_of(sp.x0[p], g, n_cells, &sp.ctr->err);
                 sp.key[p] = c;
             } else {
                 c = sp.key[p];
@@ -826,58 +821,6 @@ extern "C" int mz_squash(mz_store *h, int32_t species) {
 // cell keys that follow are reproducible bit for bit.  One thread per
 // particle; a particle draws from stream (id - 1).
 
-__device__ inline void diffuse_reflect(const MzRngParams &rp, MzStream &st, double &v0, double &v1, double &v2,
-                                       double vsq, double sgn, const double *wv) {
-    double R = fmax(1e-50, mz_rand(rp, st));
-    double vn = sgn * sqrt(-vsq * log(R));
-    R = fmax(1e-50, mz_rand(rp, st));
-    double vt = sqrt(-vsq * log(R));
-    R = 6.283185307179586 * mz_rand(rp, st);
-    double s, c;
-    sincos(R, &s, &c);
-    v0 = vn + wv[0];
-    v1 = s * vt + wv[1];
-    v2 = c * vt + wv[2];
-}
-
-__device__ inline void convect_one(const MzRngParams &rp, const MzGrid &g, const MzWalls &wl, int sidx, double dt,
-                                   int32_t id, double &x0, double &v0, double &v1, double &v2, int32_t *err) {
-    double t_rest = dt;
-    double x_old = x0;
-    double x_new = fma(v0, dt, x0);
-    int bounces = 0;
-    MzStream st = mz_stream((uint64_t)(id - 1));
-    while ((x_new >= g.L) || (x_new <= 0.0)) {
-        if (++bounces > MZ_MAX_BOUNCES) {
-            atomicOr(err, MZ_ERR_BOUNCES);
-            break;
-        }
-        int bc;
-        double wn;
-        if (x_new >= g.L) {
-            t_rest -= fabs((g.L - x_old) / v0);
-            bc = 1; wn = -1.0; x_old = g.L;
-        } else {
-            t_rest -= fabs(x_old / v0);
-            bc = 0; wn = 1.0; x_old = 0.0;
-        }
-        double acc = wl.acc[bc];
-        if (acc == 0.0) {
-            v0 = -v0;
-        } else if (acc == 1.0) {
-            diffuse_reflect(rp, st, v0, v1, v2, wl.vsq[bc][sidx], wn, wl.v[bc]);
-        } else {
-            double R = mz_rand(rp, st);
-            if (R < acc) diffuse_reflect(rp, st, v0, v1, v2, wl.vsq[bc][sidx], wn, wl.v[bc]);
-            else v0 = -v0;
-        }
-        x_new = fma(v0, t_rest, x_old);
-    }
-    if (x_new < g.min_x) x_new = g.min_x;
-    else if (x_new > g.max_x) x_new = g.max_x;
-    x0 = x_new;
-}
-
 __global__ void k_convect(MzSpView sp, MzGrid g, MzWalls wl, int sidx, double dt, MzRngParams rp,
                           const uint8_t *__restrict__ mask) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -893,7 +836,7 @@ __global__ void k_convect(MzSpView sp, MzGrid g, MzWalls wl, int sidx, double dt
         return;
     }
     double v1 = sp.v1[p], v2 = sp.v2[p];
-    convect_one(rp, g, wl, sidx, dt, sp.id[p], x0, v0, v1, v2, &sp.ctr->err);
+    mz_convect_one(rp, g, wl, sidx, dt, sp.id[p], x0, v0, v1, v2, &sp.ctr->err);
     sp.x0[p] = x0;
     sp.v0[p] = v0;
     sp.v1[p] = v1;
