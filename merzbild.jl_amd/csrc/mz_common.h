@@ -89,6 +89,18 @@ struct MzStream {
     int has_spare;
 };
 
+// Draws 2 blk and 2 blk + 1 of a Philox stream.
+__device__ inline void mz_philox_pair(const MzRngParams &rp, uint64_t stream, uint64_t blk, double &d0,
+                                      double &d1) {
+    uint32_t o[4];
+    mz_philox((uint32_t)blk, (uint32_t)stream, (uint32_t)rp.tag, (uint32_t)(rp.tag >> 32), (uint32_t)rp.seed,
+              (uint32_t)(rp.seed >> 32), o);
+    uint64_t b0 = ((uint64_t)o[1] << 32) | o[0];
+    uint64_t b1 = ((uint64_t)o[3] << 32) | o[2];
+    d0 = (double)(b0 >> 11) * 0x1.0p-53;
+    d1 = (double)(b1 >> 11) * 0x1.0p-53;
+}
+
 __device__ inline double mz_rand(const MzRngParams &rp, MzStream &st) {
     uint64_t k = st.k++;
     if (rp.mode == 1) {
@@ -102,13 +114,8 @@ __device__ inline double mz_rand(const MzRngParams &rp, MzStream &st) {
         st.has_spare = 0;
         return st.spare;
     }
-    uint32_t o[4];
-    mz_philox((uint32_t)(k >> 1), (uint32_t)st.stream, (uint32_t)rp.tag, (uint32_t)(rp.tag >> 32),
-              (uint32_t)rp.seed, (uint32_t)(rp.seed >> 32), o);
-    uint64_t b0 = ((uint64_t)o[1] << 32) | o[0];
-    uint64_t b1 = ((uint64_t)o[3] << 32) | o[2];
-    double d0 = (double)(b0 >> 11) * 0x1.0p-53;
-    double d1 = (double)(b1 >> 11) * 0x1.0p-53;
+    double d0, d1;
+    mz_philox_pair(rp, st.stream, k >> 1, d0, d1);
     if (k & 1u) return d1;
     st.spare = d1;
     st.has_spare = 1;

@@ -48,15 +48,15 @@ __device__ inline void mz_com(double mu1, double mu2, double vi0, double vi1, do
     vc2 = fma(mu1, vi2, mu2 * vk2);
 }
 
-// scatter_vhs!: two draws, phi first, then cos(theta); isotropic post-
-// collision relative velocity of magnitude g about the centre of mass.
-__device__ inline void mz_vhs_scatter(const MzRngParams &rp, MzStream &st, double mu1, double mu2, double g,
-                                      double vc0, double vc1, double vc2, double &vi0, double &vi1, double &vi2,
-                                      double &vk0, double &vk1, double &vk2) {
-    double phi = 6.283185307179586 * mz_rand(rp, st);
+// scatter_vhs! from its two draws (R_phi drawn first, then R_ct): isotropic
+// post-collision relative velocity of magnitude g about the centre of mass.
+__device__ inline void mz_vhs_post(double mu1, double mu2, double g, double vc0, double vc1, double vc2,
+                                   double R_phi, double R_ct, double &vi0, double &vi1, double &vi2, double &vk0,
+                                   double &vk1, double &vk2) {
+    double phi = 6.283185307179586 * R_phi;
     double sphi, cphi;
     sincos(phi, &sphi, &cphi);
-    double ct = 2.0 * mz_rand(rp, st) - 1.0;
+    double ct = 2.0 * R_ct - 1.0;
     double stt = sqrt(1.0 - ct * ct);
     double gn0 = g * (stt * cphi), gn1 = g * (stt * sphi), gn2 = g * ct;
     vi0 = fma(mu2, gn0, vc0);
@@ -65,6 +65,15 @@ __device__ inline void mz_vhs_scatter(const MzRngParams &rp, MzStream &st, doubl
     vk0 = fma(-mu1, gn0, vc0);
     vk1 = fma(-mu1, gn1, vc1);
     vk2 = fma(-mu1, gn2, vc2);
+}
+
+// scatter_vhs!: two draws, phi first, then cos(theta)
+__device__ inline void mz_vhs_scatter(const MzRngParams &rp, MzStream &st, double mu1, double mu2, double g,
+                                      double vc0, double vc1, double vc2, double &vi0, double &vi1, double &vi2,
+                                      double &vk0, double &vk1, double &vk2) {
+    double R_phi = mz_rand(rp, st);
+    double R_ct = mz_rand(rp, st);
+    mz_vhs_post(mu1, mu2, g, vc0, vc1, vc2, R_phi, R_ct, vi0, vi1, vi2, vk0, vk1, vk2);
 }
 
 // Diffuse re-emission from a wall: three draws (normal, tangential speed

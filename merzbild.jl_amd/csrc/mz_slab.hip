@@ -40,6 +40,7 @@
 #define SLAB_MB_CAP 256               // particles per mailbox (slab side, step parity)
 #define SLAB_SPIN_LIMIT (1u << 22)    // polls per wait before the launch aborts
 #define SLAB_LINE 16                  // 64-bit words per flag (one 128-byte line)
+#define SLAB_DRAWS 64                 // NTC draws per cell computed ahead, in parallel
 
 struct SlabArgs {
     MzSpView a;                 // input particles, PIA, counters
@@ -108,12 +109,21 @@ __device__ inline void slab_raise_abort(const SlabArgs &A) {
     __hip_atomic_store(A.abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// ntc_equal_weight! of one cell over its LDS segment [base, base + nla).
+// Draw k of the cell's stream: from the block of draws the workgroup
+// computed ahead, then from the stream itself.
+__device__ inline double slab_draw(const MzRngParams &rp, MzStream &st, const double *pre) {
+    if (st.k < SLAB_DRAWS) return pre[st.k++];
+    return mz_rand(rp, st);
+}
+
+// ntc_equal_weight! of one cell over its LDS segment [base, base + nla);
+// pre holds the first SLAB_DRAWS draws of the cell's stream for this step.
 __device__ inline void slab_ntc_cell(const SlabArgs &A, const MzRngParams &rp, int c, double Vc, int32_t base,
                                      int32_t nla, double *sv0, double *sv1, double *sv2, const double *sw,
-                                     double &sgwm, int32_t &n_coll_out, int32_t &performed, int32_t &eqw) {
+                                     const double *pre, double &sgwm, int32_t &n_coll_out, int32_t &performed,
+                                     int32_t &eqw) {
     MzStream st = mz_stream((uint64_t)c);
-    double R = mz_rand(rp, st);
+    double R = slab_draw(rp, st, pre);
     int64_t n_coll = (int64_t)floor(mz_ncoll_est(true, A.dt, nla, nla, sgwm, Vc, R));
     if (!(n_coll <= MZ_NTC_MAX_CANDIDATES)) {
         atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH);
@@ -122,9 +132,9 @@ __device__ inline void slab_ntc_cell(const SlabArgs &A, const MzRngParams &rp, i
     performed = 0;
     eqw = 0;
     for (int64_t cc = 0; cc < n_coll; ++cc) {
-        int32_t i = (int32_t)floor(mz_rand(rp, st) * (double)nla);
-        int32_t k = (int32_t)floor(mz_rand(rp, st) * (double)nla);
-        while (i == k) k = (int32_t)floor(mz_rand(rp, st) * (double)nla);
+        int32_t i = (int32_t)floor(slab_draw(rp, st, pre) * (double)nla);
+        int32_t k = (int32_t)floor(slab_draw(rp, st, pre) * (double)nla);
+        while (i == k) k = (int32_t)floor(slab_draw(rp, st, pre) * (double)nla);
         const int pi = base + i, pk = base + k;
         double vi0 = sv0[pi], vi1 = sv1[pi], vi2 = sv2[pi];
         double vk0 = sv0[pk], vk1 = sv1[pk], vk2 = sv2[pk];
@@ -133,13 +143,15 @@ __device__ inline void slab_ntc_cell(const SlabArgs &A, const MzRngParams &rp, i
         if (!(g > MZ_G_EPS)) continue;
         double sgw = mz_vhs_sgw(A.factor, A.expo, g, sw[pi], sw[pk]);
         sgwm = sgw > sgwm ? sgw : sgwm;
-        if (!(mz_rand(rp, st) < sgw / sgwm)) continue;
+        if (!(slab_draw(rp, st, pre) < sgw / sgwm)) continue;
         ++performed;
         ++eqw;
         double vc0, vc1, vc2;
         mz_com(A.mu1, A.mu2, vi0, vi1, vi2, vk0, vk1, vk2, vc0, vc1, vc2);
         double ui0, ui1, ui2, uk0, uk1, uk2;
-        mz_vhs_scatter(rp, st, A.mu1, A.mu2, g, vc0, vc1, vc2, ui0, ui1, ui2, uk0, uk1, uk2);
+        const double R_phi = slab_draw(rp, st, pre);
+        const double R_ct = slab_draw(rp, st, pre);
+        mz_vhs_post(A.mu1, A.mu2, g, vc0, vc1, vc2, R_phi, R_ct, ui0, ui1, ui2, uk0, uk1, uk2);
         sv0[pi] = ui0; sv1[pi] = ui1; sv2[pi] = ui2;
         sv0[pk] = uk0; sv1[pk] = uk1; sv2[pk] = uk2;
     }
@@ -147,7 +159,7 @@ __device__ inline void slab_ntc_cell(const SlabArgs &A, const MzRngParams &rp, i
 }
 
 template <int R>
-__global__ void __launch_bounds__(SLAB_THREADS) k_slab_couette(SlabArgs A) {
+__global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(SlabArgs A) {
     constexpr int P = R * SLAB_THREADS;   // LDS particle capacity
     constexpr int NCH = P / 64;           // 64-particle chunks
     static_assert(NCH <= 64, "chunk scan uses one wave");
@@ -157,6 +169,7 @@ __global__ void __launch_bounds__(SLAB_THREADS) k_slab_couette(SlabArgs A) {
     __shared__ int32_t tot[SLAB_KMAX];
     __shared__ int32_t cst[SLAB_MAXC], ccn[SLAB_MAXC], runL[SLAB_MAXC], runR[SLAB_MAXC];
     __shared__ int32_t arank[2][SLAB_MB_CAP], akey[2][SLAB_MB_CAP];
+    __shared__ double sdraw[SLAB_MAXC][SLAB_DRAWS];
     __shared__ int32_t s_abort, s_n, s_na[2];
     __shared__ int64_t s_red[4][4];
 
@@ -222,15 +235,22 @@ __global__ void __launch_bounds__(SLAB_THREADS) k_slab_couette(SlabArgs A) {
         const uint64_t step = A.step0 + (uint64_t)t;
         const int q = t & 1;
         n_prev = n_loc;
-        // ---- 1. ntc_equal_weight! per cell; clear the ranking tables
+        // ---- 1. ntc_equal_weight! per cell; clear the ranking tables.  The
+        // first draws of every cell's stream are computed by all threads first.
+        MzRngParams rp_ntc = A.rp;
+        rp_ntc.tag = (step << 4) | 1u;
+        for (int j = tid; j < ncl * (SLAB_DRAWS / 2); j += SLAB_THREADS) {
+            const int k = j / (SLAB_DRAWS / 2), blk = j % (SLAB_DRAWS / 2);
+            mz_philox_pair(rp_ntc, (uint64_t)(clo + k), (uint64_t)blk, sdraw[k][2 * blk], sdraw[k][2 * blk + 1]);
+        }
         for (int j = tid; j < NCH * SLAB_KMAX; j += SLAB_THREADS) (&tab[0][0])[j] = 0;
         if (tid < SLAB_MAXC) { runL[tid] = 0; runR[tid] = 0; }
+        __syncthreads();
         if (ntc_lane) {
-            MzRngParams rp = A.rp;
-            rp.tag = (step << 4) | 1u;
+            const MzRngParams &rp = rp_ntc;
             my_nla = ccn[my_k];
-            slab_ntc_cell(A, rp, clo + my_k, my_V, cst[my_k], my_nla, sv0, sv1, sv2, sw, my_sgwm, my_ncoll,
-                          my_perf, my_eqw);
+            slab_ntc_cell(A, rp, clo + my_k, my_V, cst[my_k], my_nla, sv0, sv1, sv2, sw, sdraw[my_k], my_sgwm,
+                          my_ncoll, my_perf, my_eqw);
         }
         __syncthreads();
         // ---- 2. convection, new cell, rank inside 64-particle chunks
