@@ -143,7 +143,20 @@ def main():
                          "eager: kernel by kernel")
     ap.add_argument("--graph-steps", type=int, default=400, help="steps of the step-graph path timed alongside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run a plain device copy of MB megabytes three times (to scale the HBM byte "
+                         "counters of a profiler run) and exit")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        n = a.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": n * 4, "launches": 3}))
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": a.gpus}))
         sys.exit(2)
