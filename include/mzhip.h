@@ -115,6 +115,13 @@ int mz_ntc(mz_store *h, int32_t s1, int32_t s2, double dt, const double *V, doub
  * (src/properties/physical_props.jl:104-154, 317-454).  Results stay on the
  * device until mz_get_props. masses[n_species]. */
 int mz_compute_props(mz_store *h, int32_t variant, int32_t ndens_not_Np, const double *masses);
+/* compute_props_with_total_moments! (src/properties/physical_props.jl:168-245):
+ * n, v, T as compute_props! plus the total moments for powers[n_moments]
+ * (at most 8), scaled by the Maxwellian values at Tref. */
+int mz_compute_props_moments(mz_store *h, const int32_t *powers, int32_t n_moments, double Tref,
+                             const double *masses);
+/* moments[n_species][n_cells][n_moments] of the last mz_compute_props_moments. */
+int mz_get_props_moments(mz_store *h, double *moments);
 /* PhysProps arrays [n_species][n_cells] (v: [n_species][n_cells][3]); any may be NULL. */
 int mz_get_props(mz_store *h, double *lpa, double *np, double *n, double *v, double *T);
 

@@ -50,6 +50,8 @@ _SIGS = {
     "mz_ntc": (C.c_int, [C.c_void_p, i32, i32, f64, P(f64), f64, i32, u64, i32, i32, i32]),
     "mz_compute_props": (C.c_int, [C.c_void_p, i32, i32, P(f64)]),
     "mz_get_props": (C.c_int, [C.c_void_p, P(f64), P(f64), P(f64), P(f64), P(f64)]),
+    "mz_compute_props_moments": (C.c_int, [C.c_void_p, P(i32), i32, f64, P(f64)]),
+    "mz_get_props_moments": (C.c_int, [C.c_void_p, P(f64)]),
     "mz_merge_octree": (C.c_int, [C.c_void_p, i32, i32, i32, i32, i32, i32, i32, i32, i32, i32, u64, i32,
                                   P(i64)]),
     "mz_merge_nnls": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, f64, i32, i32, P(f64), i32, f64, i32, f64,
@@ -397,18 +399,28 @@ def ntc_equal_weight(store, s1, s2, dt, V=None, tag=0, rng_mode=RNG_PHILOX, cell
 
 
 class PhysProps:
-    """PhysProps (src/properties/physical_props.jl:24-90), host copies."""
+    """PhysProps (src/properties/physical_props.jl:24-90), host copies.
+    moments[species, cell, k] holds the total moment of power
+    moment_powers[k] (compute_props_with_total_moments!)."""
 
-    def __init__(self, n_cells, n_species, ndens_not_Np=False):
+    def __init__(self, n_cells, n_species, moments_list=(), ndens_not_Np=False, Tref=1.0):
+        if isinstance(moments_list, bool):     # PhysProps(n_cells, n_species, ndens_not_Np)
+            moments_list, ndens_not_Np = (), moments_list
         self.n_cells, self.n_species, self.ndens_not_Np = n_cells, n_species, ndens_not_Np
+        self.moment_powers = np.ascontiguousarray(moments_list, dtype=np.int32)
+        self.n_moments = len(self.moment_powers)
+        self.Tref = Tref
         self.lpa = np.zeros(n_species)
         self.np = np.zeros((n_species, n_cells))
         self.n = np.zeros((n_species, n_cells))
         self.v = np.zeros((n_species, n_cells, 3))
         self.T = np.zeros((n_species, n_cells))
+        self.moments = np.zeros((n_species, n_cells, self.n_moments))
 
-    def fetch(self, store):
+    def fetch(self, store, moments=False):
         _check(load_library().mz_get_props(store.h, _d(self.lpa), _d(self.np), _d(self.n), _d(self.v), _d(self.T)))
+        if moments and self.n_moments:
+            _check(load_library().mz_get_props_moments(store.h, _d(self.moments)))
         return self
 
 
@@ -427,6 +439,18 @@ def compute_props(store, species_data=None, phys_props=None):
     _check(load_library().mz_compute_props(store.h, 0, 0, _d(m)))
     pp = phys_props or PhysProps(store.n_cells, store.n_species)
     return pp.fetch(store)
+
+
+def compute_props_with_total_moments(store, species_data, phys_props, fetch=True):
+    """compute_props_with_total_moments!(particles, pia, species_data, phys_props)
+    (src/properties/physical_props.jl:168-245); without moments it is
+    compute_props!, as in the reference."""
+    if phys_props.n_moments == 0:
+        return compute_props(store, species_data, phys_props)
+    m = _masses(store, species_data)
+    _check(load_library().mz_compute_props_moments(store.h, phys_props.moment_powers.ctypes.data_as(P(i32)),
+                                                    phys_props.n_moments, phys_props.Tref, _d(m)))
+    return phys_props.fetch(store, moments=True) if fetch else None
 
 
 def compute_props_sorted(store, species_data=None, phys_props=None, ndens_not_Np=False, fetch=True):

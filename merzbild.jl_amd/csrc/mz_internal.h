@@ -13,6 +13,7 @@
 #define MZ_SORT_CHUNK 1024
 #define MZ_SORT_MAX_CELLS 14336   // LDS: 8 KB per chunk + 4 B per cell within 64 KB
 #define MZ_MAX_BOUNCES 64
+#define MZ_MAX_MOMENTS 8   // total-moment powers per compute_props_with_total_moments! call
 #define MZ_NTC_MAX_CANDIDATES 100000000  // per cell and call; larger counts flag an error
 #define MZ_KB 1.380649e-23  // src/constants.jl:4
 
@@ -83,6 +84,7 @@ struct mz_store {
     int32_t slab_spc = 0, slab_ppt = 0;
     int32_t slab_info[4] = {0, 0, 0, 0};
     int32_t last_path = -1;   // of mz_couette_steps: 0 eager, 1 graph, 2 slab
+    int32_t n_moments = 0;    // moments of the last mz_compute_props_moments
 };
 
 MzViews mz_views(mz_store *h);

@@ -972,6 +972,115 @@ extern "C" int mz_compute_props(mz_store *h, int32_t variant, int32_t ndens_not_
     return 0;
 }
 
+// compute_props_with_total_moments! (physical_props.jl:168-245): n, v and T
+// over groups 1 and 2 as compute_props!, plus the total moments
+// sum w |v - vbar|^p / (scaling_p n) for the requested powers p.  One wave per
+// (cell, species); the mean is reduced first, then the centred sums.
+__global__ void __launch_bounds__(256) k_props_moments(MzViews views, int32_t n_cells, int32_t n_mom,
+                                                       const int32_t *__restrict__ powers,
+                                                       const double *__restrict__ scale,   // [species][n_mom]
+                                                       const double *__restrict__ masses, double *__restrict__ out_np,
+                                                       double *__restrict__ out_n, double *__restrict__ out_v,
+                                                       double *__restrict__ out_T, double *__restrict__ out_m) {
+    const int sidx = blockIdx.y;
+    const MzSpView &sp = views.v[sidx];
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c >= n_cells) return;
+    const int32_t st1 = sp.pia.start1[c] - 1, n1 = sp.pia.n1[c];
+    const int32_t st2 = sp.pia.start2[c] - 1, n2 = sp.pia.n2[c] > 0 ? sp.pia.n2[c] : 0;
+    const int32_t tot = n1 + n2;
+    double n = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int32_t i = lane; i < tot; i += 64) {
+        const int64_t p = i < n1 ? (int64_t)st1 + i : (int64_t)st2 + (i - n1);
+        const double w = sp.w[p];
+        n += w;
+        a0 = a0 + sp.v0[p] * w;
+        a1 = a1 + sp.v1[p] * w;
+        a2 = a2 + sp.v2[p] * w;
+    }
+    n = mz_wave_sum(n);
+    a0 = mz_wave_sum(a0);
+    a1 = mz_wave_sum(a1);
+    a2 = mz_wave_sum(a2);
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0, E = 0.0;
+    double mom[MZ_MAX_MOMENTS];
+#pragma unroll
+    for (int j = 0; j < MZ_MAX_MOMENTS; ++j) mom[j] = 0.0;
+    if (n > 0.0) {
+        m0 = a0 / n; m1 = a1 / n; m2 = a2 / n;
+        for (int32_t i = lane; i < tot; i += 64) {
+            const int64_t p = i < n1 ? (int64_t)st1 + i : (int64_t)st2 + (i - n1);
+            const double w = sp.w[p];
+            const double d0 = sp.v0[p] - m0, d1 = sp.v1[p] - m1, d2 = sp.v2[p] - m2;
+            const double normv = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            E = E + w * (normv * normv);
+#pragma unroll
+            for (int j = 0; j < MZ_MAX_MOMENTS; ++j)
+                if (j < n_mom) mom[j] += w * pow(normv, (double)powers[j]);
+        }
+        E = mz_wave_sum(E);
+#pragma unroll
+        for (int j = 0; j < MZ_MAX_MOMENTS; ++j)
+            if (j < n_mom) mom[j] = mz_wave_sum(mom[j]);
+    }
+    if (lane == 0) {
+        double T = 0.0;
+        if (n > 0.0) T = (2.0 / 3.0) * (E * (0.5 * masses[sidx] / (n * MZ_KB)));
+        const int64_t ix = (int64_t)sidx * n_cells + c;
+        out_np[ix] = (double)tot;
+        out_n[ix] = n;
+        out_v[3 * ix] = m0;
+        out_v[3 * ix + 1] = m1;
+        out_v[3 * ix + 2] = m2;
+        out_T[ix] = T;
+        for (int j = 0; j < n_mom; ++j) out_m[ix * n_mom + j] = mom[j] / (scale[(int64_t)sidx * n_mom + j] * n);
+    }
+}
+
+extern "C" int mz_compute_props_moments(mz_store *h, const int32_t *powers, int32_t n_moments, double Tref,
+                                        const double *masses) {
+    MZ_CHECK(n_moments >= 1 && n_moments <= MZ_MAX_MOMENTS, "props moments: 1..8 moments");
+    MZ_CHECK(powers != nullptr && masses != nullptr, "props moments: NULL argument");
+    MZ_CHECK(Tref > 0.0, "props moments: Tref must be positive");
+    const int ns = h->n_species;
+    // moment_factor * moment_vref^-(3 + p) * gamma((3 + p) / 2) per species and power
+    std::vector<double> sc((size_t)ns * n_moments);
+    for (int s = 0; s < ns; ++s) {
+        const double mf = 4 * M_PI * std::pow(masses[s] / (2 * M_PI * MZ_KB * Tref), 1.5) * 0.5;
+        const double mvref = std::pow(masses[s] / (2 * MZ_KB * Tref), 0.5);
+        for (int j = 0; j < n_moments; ++j)
+            sc[(size_t)s * n_moments + j] =
+                mf * std::pow(mvref, -(double)(3 + powers[j])) * std::tgamma((3.0 + powers[j]) / 2.0);
+    }
+    int32_t *d_pow = (int32_t *)mz_scratch(h, "props_pow", MZ_MAX_MOMENTS * sizeof(int32_t));
+    double *d_sc = (double *)mz_scratch(h, "props_scale", (size_t)ns * MZ_MAX_MOMENTS * sizeof(double));
+    double *d_m = (double *)mz_scratch(h, "props_moments", (size_t)ns * h->n_cells * MZ_MAX_MOMENTS * sizeof(double));
+    MZ_CHECK(d_pow && d_sc && d_m, "props moments: scratch allocation failed");
+    MZ_HIP(hipMemcpyAsync(d_pow, powers, n_moments * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    MZ_HIP(hipMemcpyAsync(d_sc, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    MZ_HIP(hipMemcpyAsync(h->d_masses, masses, ns * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    mz_timing_begin(h, "props");
+    dim3 grid((h->n_cells + 3) / 4, ns);
+    hipLaunchKernelGGL(k_props_moments, grid, dim3(256), 0, h->stream, mz_views(h), h->n_cells, n_moments, d_pow,
+                       d_sc, h->d_masses, h->d_np, h->d_n, h->d_v, h->d_T, d_m);
+    MZ_HIP(hipGetLastError());
+    mz_timing_end(h);
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    h->n_moments = n_moments;
+    return 0;
+}
+
+extern "C" int mz_get_props_moments(mz_store *h, double *moments) {
+    MZ_CHECK(h->n_moments > 0, "props moments: none computed yet");
+    MZ_CHECK(moments != nullptr, "props moments: NULL output");
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    double *d_m = (double *)mz_scratch(h, "props_moments", 0);
+    MZ_HIP(hipMemcpy(moments, d_m, (size_t)h->n_species * h->n_cells * h->n_moments * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    return 0;
+}
+
 extern "C" int mz_get_props(mz_store *h, double *lpa, double *np, double *n, double *v, double *T) {
     MZ_HIP(hipStreamSynchronize(h->stream));
     size_t pn = (size_t)h->n_species * h->n_cells;
