@@ -1,0 +1,136 @@
+"""End-to-end parity of the batched 0-D and variable-weight Couette
+workloads at reduced sizes, step by step against the oracle.
+
+Batched uneven 0-D (BASELINE config 3 shape): cells of log-uniform occupancy,
+pseudo-Maxwell argon, BKW velocities at 273 K, weights uniform in
+[0.5, 2] Fnum, ntc! with weight splitting in every cell, then the sort
+(which folds the created particles into their cells) and compute_props!.
+
+Variable-weight Couette (test/test_1D_couette_varweight.jl:78-101): weights
+lognormal(0, 1) around Fnum, ntc! with splitting, octree N:2 merging of every
+cell above the threshold (grid-clamped), squash_pia!, convection, sort,
+compute_props_sorted!.  The device runs each phase for all cells at once, so
+both sides here use that order (all NTC, then all merges, then one squash);
+the reference's driver interleaves NTC and merge per cell, which changes
+which free ids the splits reuse, so its trajectories agree statistically, not
+bit for bit."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import analytic_cases as ac
+import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def _interactions(hip, st, its):
+    n = len(its)
+    st.set_interactions([[type("I", (), dict(mu1=its[i][k].mu1, mu2=its[i][k].mu2, vhs_factor=its[i][k].vhs_factor,
+                                             vhs_o=its[i][k].vhs_o))() for k in range(n)] for i in range(n)])
+
+
+def test_batched_0d_cells_with_splitting(hip, mzo):
+    rng = np.random.default_rng(7)
+    nc = 192
+    counts = np.floor(np.exp(rng.uniform(math.log(20), math.log(2000), size=nc))).astype(int)
+    n = int(counts.sum())
+    T0, m = 273.0, ac.M_AR
+    grid = mzo.grid1d(float(nc), nc)              # one unit-length cell per 0-D cell, V = 1
+    pv, pia = mzo.ParticleVector(n), mzo.PIA(nc, 1)
+    r = mzo.RNG(seed=7, tag=55)
+    pos = 1
+    Fnum = 1e23 / counts
+    for c in range(1, nc + 1):
+        k = int(counts[c - 1])
+        for j in range(k):
+            pv.pop_to(pos + j)
+            pv.set(pos + j, Fnum[c - 1] * rng.uniform(0.5, 2.0), [0.0, 0.0, 0.0],
+                   [c - 1 + rng.uniform(0.01, 0.99), 0.5, 0.5])
+        r.s.stream, r.s.k = c, 0
+        mzo.lib().mzo_sample_bkw(r.ref, pv.p, k, pos - 1, m, T0, (C.c_double * 3)(0.0, 0.0, 0.0))
+        pia.set(c, 1, k, pos, pos + k - 1, k)
+        pos += k
+    pia.set_n_total(1, n)
+    its = mzo.interactions(["Ar"], table=mzo.PSEUDO_MAXWELL)
+    it = its[0][0]
+    d = mzo.PSEUDO_MAXWELL[("Ar", "Ar")][0]
+    tref = 1.0 / (1e23 * math.pi * d * d) / math.sqrt(2 * ac.KB * T0 / m)
+    dt = 0.025 * tref
+    cfs = (mzo.CF * nc)()
+    sg = np.array([mzo.lib().mzo_estimate_sigma_g_w_max(C.byref(it), m, m, T0, T0, float(f), 1.0) for f in Fnum])
+    for c in range(nc):
+        cfs[c].sigma_g_w_max = sg[c]
+    st = hip.DeviceStore(nc, 1, 2 * n + 1024)
+    helpers.upload_from_oracle(st, mzo, pv, pia)
+    st.set_grid(hip.Grid1DUniform(float(nc), nc))
+    _interactions(hip, st, its)
+    st.set_sigma_g_w_max(1, 1, sg)
+    st.set_seed(7)
+    sp = [hip.Species("Ar", m)]
+    V = np.ones(nc)
+    ref = mzo.Props(nc, 1)
+    mzo.compute_props([pv], pia, [m], ref)
+    n_cell0 = ref.n[0].copy()
+    n_prev = n
+    for step in range(1, 5):
+        tag = (step << 4) | 1
+        hip.ntc(st, 1, 1, dt, V=V, tag=tag)
+        rr = mzo.RNG(seed=7, tag=tag)
+        for c in range(1, nc + 1):
+            mzo.lib().mzo_ntc_single(rr.ref, C.byref(cfs[c - 1]), C.byref(it), pv.p, pia.p, c, 1, dt, 1.0, 1e-16, 0)
+        got = st.download_species(1)
+        assert got["n_total"] > n_prev
+        n_prev = got["n_total"]
+        helpers.assert_state_matches(mzo, got, pv, pia, rtol=1e-11)
+        hip.sort_particles(st, 1)
+        mzo.lib().mzo_sort_particles(C.byref(grid), pv.p, pia.p, 1)
+    got = st.download_species(1)
+    assert got["err"] == 0 and got["n_total"] > n
+    helpers.assert_state_matches(mzo, got, pv, pia, rtol=1e-11)
+    pp = hip.compute_props(st, sp)
+    mzo.compute_props([pv], pia, [m], ref)
+    assert np.array_equal(pp.np, ref.np)
+    assert np.allclose(pp.n, ref.n, rtol=1e-12) and np.allclose(pp.T, ref.T, rtol=1e-10)
+    # weight splitting conserves each cell's total weight (up to rounding)
+    assert np.allclose(pp.n[0], n_cell0, rtol=1e-12)
+    st.close()
+
+
+def test_varweight_couette_with_octree_merges(hip, mzo):
+    nc, ppc, thr, target = 20, 100, 102, 64
+    case = helpers.couette_case(mzo, n_cells=nc, ppc=ppc, seed=13, dt=2e-10, sgwm_mult=40.0,
+                                weights=lambda rng, k: rng.lognormal(0.0, 1.0, size=k) / math.exp(0.5))
+    st = helpers.store_from_case(hip, case, capacity=4 * nc * ppc + 4096)
+    pv, pia, g = case["pv"], case["pia"], case["grid"]
+    sp = case["species"]
+    merged = 0
+    for step in range(1, 11):
+        # all NTC, then all merges, then one squash (the device's phase order)
+        hip.ntc(st, 1, 1, case["dt"], tag=(step << 4) | 1)
+        r = mzo.RNG(seed=case["seed"], tag=(step << 4) | 1)
+        for c in range(1, nc + 1):
+            mzo.lib().mzo_ntc_single(r.ref, C.byref(case["cfs"][c - 1]), C.byref(case["it"]), pv.p, pia.p, c, 1,
+                                     case["dt"], g.dx, 1e-16, 0)
+        mtag = (step << 4) | 3
+        merged += hip.merge_octree_N2_based(st, 1, thr, target, grid_clamp=True, squash=True, tag=mtag)
+        r = mzo.RNG(seed=case["seed"], tag=mtag)
+        oc = mzo.Octree(split=1, init_bin_bounds=1, bin_bounds_compute=1, max_Nbins=4096, max_depth=10)
+        for c in range(1, nc + 1):
+            if pia.ind(c).n_local > thr:
+                mzo.lib().mzo_merge_octree_N2_based(r.ref, oc.p, pv.p, pia.p, c, 1, target, C.byref(g))
+        mzo.lib().mzo_squash_pia(pv.p, pia.p, 1)
+        hip.convect_particles(st, 1, case["dt"], tag=(step << 4) | 2)
+        r = mzo.RNG(seed=case["seed"], tag=(step << 4) | 2)
+        mzo.lib().mzo_convect_particles(r.ref, C.byref(g), C.byref(case["walls"]), pv.p, pia.p, 1, case["dt"])
+        hip.sort_particles(st, 1)
+        mzo.lib().mzo_sort_particles(C.byref(g), pv.p, pia.p, 1)
+        got = st.download_species(1)
+        assert got["err"] == 0
+        helpers.assert_state_matches(mzo, got, pv, pia, rtol=1e-11)
+    assert merged > 0
+    pp = hip.compute_props_sorted(st, sp)
+    assert np.allclose(pp.T[0], helpers.oracle_props_sorted(mzo, case), rtol=1e-10)
+    st.close()
