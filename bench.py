@@ -133,16 +133,109 @@ def cpu_baseline(case, it, target_s=12.0):
     return n * steps / el, threads, steps
 
 
+# Batched uneven 0-D cells: 65,536 independent cells of V = 1 with log-uniform
+# occupancy in [20, 20000] (Philox seed 7), argon pseudo-Maxwell molecules,
+# BKW velocities at 273 K (|v| ~ Chi(5) sqrt(0.3 2kT/m), polar angle uniform
+# on [0, pi] as the reference's sample_bkw! draws it), weights uniform in
+# [0.5, 2] Fnum with Fnum = 1e23 / N_cell, dt = 0.025 t_ref.  A step is
+# ntc_equal_weight! in every cell (no splitting, so the particle count stays
+# fixed) and compute_props!.
+B0_CELLS, B0_NMIN, B0_NMAX, B0_T0, B0_NDENS = 65536, 20, 20000, 273.0, 1e23
+
+
+def sample_batched_0d(seed=7):
+    g = np.random.Generator(np.random.Philox(seed))
+    counts = np.floor(np.exp(g.uniform(np.log(B0_NMIN), np.log(B0_NMAX), size=B0_CELLS))).astype(np.int64)
+    n = int(counts.sum())
+    m = 66.3e-27
+    vscale = np.sqrt(2 * K_B * B0_T0 / m) * np.sqrt(0.3)
+    v = np.empty((n, 3))
+    chunk = 1 << 24
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        k = b - a
+        vabs = np.sqrt(g.chisquare(5, size=k))
+        th = g.uniform(size=k) * np.pi
+        ph = g.uniform(size=k) * 2 * np.pi
+        st = np.sin(th)
+        v[a:b, 0] = vscale * vabs * st * np.cos(ph)
+        v[a:b, 1] = vscale * vabs * st * np.sin(ph)
+        v[a:b, 2] = vscale * vabs * np.cos(th)
+    fnum = B0_NDENS / counts
+    w = np.repeat(fnum, counts) * g.uniform(0.5, 2.0, size=n)
+    x = np.zeros((n, 3))
+    x[:, 0] = np.repeat(np.arange(B0_CELLS, dtype=np.float64), counts) + 0.5
+    starts = np.concatenate(([0], np.cumsum(counts)[:-1]))
+    pia = np.zeros((B0_CELLS, 7), dtype=np.int64)
+    pia[:, 0] = counts
+    pia[:, 1] = starts + 1
+    pia[:, 2] = starts + counts
+    pia[:, 3] = counts
+    pia[:, 5] = -1
+    return dict(x=x, v=v, w=w, ids=np.arange(1, n + 1, dtype=np.int64), pia=pia, n=n, counts=counts, fnum=fnum,
+                mass=m)
+
+
+def run_batched_0d(a, pkg):
+    case = sample_batched_0d()
+    n = case["n"]
+    data = os.path.join(ROOT, "merzbild.jl_amd", "data")
+    sp = pkg.load_species_data(os.path.join(data, "particles.toml"), ["Ar"])
+    it = pkg.load_interaction_data(os.path.join(data, "pseudo_maxwell.toml"), sp)
+    st = pkg.DeviceStore(B0_CELLS, 1, n + 1024)
+    st.upload_species(1, case["x"], case["v"], case["w"], case["ids"], case["pia"], n, True,
+                      np.zeros(0, dtype=np.int64), n)
+    st.set_grid(pkg.Grid1DUniform(float(B0_CELLS), B0_CELLS))     # V = 1 per cell
+    st.set_interactions(it)
+    st.set_species_data(sp)
+    ia = it[0][0]
+    sg = np.array([pkg.estimate_sigma_g_w_max(ia, sp[0].mass, sp[0].mass, B0_T0, B0_T0, f) for f in case["fnum"]])
+    st.set_sigma_g_w_max(1, 1, sg)
+    st.set_seed(7)
+    sigma_ref = np.pi * ia.vhs_d ** 2
+    tref = 1.0 / (B0_NDENS * sigma_ref) / np.sqrt(2 * K_B * B0_T0 / sp[0].mass)
+    dt = 0.025 * tref
+    del case["x"], case["v"], case["w"]
+    mode = a.mode if a.mode in ("batched", "plain") else "batched"
+    steps = a.steps if a.steps != 20000 else 200
+
+    def run(k, s0):
+        return pkg.batch0d_steps(st, k, s0, dt, sp, mode=mode)
+
+    run(a.warmup if a.warmup != 200 else 5, 1)
+    st.sync()
+    st.timing_reset()
+    st.timing(True)
+    t0 = time.perf_counter()
+    run(steps, 1000)
+    st.sync()
+    el = time.perf_counter() - t0
+    st.timing(False)
+    sc = st.scalars(1)
+    assert sc["err"] == 0 and sc["n_total"] == n, sc
+    out = {"metric": "particle-timesteps/s (NTC+props), batched uneven 0-D cells", "value": n * steps / el,
+           "unit": "particle-timesteps/s", "n_gpus": 1, "steps": steps, "ms_per_step": el * 1e3 / steps,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "65536 0-D cells, log-uniform 20..20000 particles (Philox seed 7), "
+                                  "pseudo-Maxwell Ar, BKW at 273 K, w ~ U(0.5, 2) Fnum; ntc_equal_weight! + "
+                                  "compute_props! per step", "n_particles": n, "mode": mode,
+                      "batch_shape": st.batch_info() if mode == "batched" else None}}
+    print(json.dumps(out))
+    st.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--mode", choices=("slab", "graph", "eager"), default="slab",
+    ap.add_argument("--mode", choices=("slab", "graph", "eager", "batched", "plain"), default="slab",
                     help="slab: all steps in one launch, cells resident in LDS; graph: replayed step graph; "
                          "eager: kernel by kernel")
     ap.add_argument("--graph-steps", type=int, default=400, help="steps of the step-graph path timed alongside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("couette", "batched-0d"), default="couette",
+                    help="couette: the headline 1-D Couette line; batched-0d: 65,536 uneven 0-D cells")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB megabytes three times (to scale the HBM byte "
                          "counters of a profiler run) and exit")
@@ -161,6 +254,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": a.gpus}))
         sys.exit(2)
     pkg = load_pkg()
+    if a.workload == "batched-0d":
+        run_batched_0d(a, pkg)
+        return
     case = sample_case()
     st, sp, it = setup_store(pkg, case)
     n = case["n"]

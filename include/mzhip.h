@@ -160,12 +160,27 @@ int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, double dt, co
 /* Which path the last mz_couette_steps call took: 0 eager, 1 graph, 2 slab. */
 int mz_couette_last_path(mz_store *h);
 /* Slab shape of mode 2 (the reference's n_chunks, couette_multithreaded.jl:
- * 13-14, 31): cells per slab (1..32) and particles per thread (1, 2, 4, 8) of
+ * 13-14, 31): cells per slab (1..32) and particles per thread (1, 2, 4) of
  * the LDS capacity 256 x ppt per slab; 0 lets the library choose. */
 int mz_set_slab_shape(mz_store *h, int32_t cells_per_slab, int32_t particles_per_thread);
 /* info[0..3] = particles per thread, cells per slab, slabs, outcome (1 ran,
  * -1 aborted and replayed on mode 1) of the last mode-2 launch. */
 int mz_get_slab_info(mz_store *h, int32_t *info);
+
+/* Many independent 0-D cells: n_steps of ntc_equal_weight! in every cell of
+ * `species` and compute_props! every props_every steps (the per-cell loops of
+ * test/test_bkw.jl:88-93, test/test_2species.jl:52-65 over a batch of
+ * cells).  mode 0 launches the NTC and props kernels once per step; mode 1
+ * runs all steps in one persistent launch whose wavefronts drain a queue of
+ * cell groups ordered by size.  Both give identical results.  Step t uses
+ * tag (t << 4) | 1.  V[n_cells] as for mz_ntc (NULL: the grid's cells). */
+int mz_batch0d_steps(mz_store *h, int32_t species, int32_t n_steps, uint64_t step0, double dt, const double *V,
+                     int32_t props_every, const double *masses, int32_t mode);
+/* Packing of mode 1: at most max_cells (1..64) cells and about
+ * target_particles particles per wavefront item; 0 keeps the default. */
+int mz_set_batch_shape(mz_store *h, int32_t max_cells, int32_t target_particles);
+/* info[0..3] = items, max cells per item, target particles, workgroups of the last mode-1 launch. */
+int mz_get_batch_info(mz_store *h, int32_t *info);
 
 /* ------------------------------------------------------------------ timing */
 /* When enabled, the library brackets each launch of the named kernel family

@@ -60,6 +60,9 @@ _SIGS = {
     "mz_couette_last_path": (C.c_int, [C.c_void_p]),
     "mz_set_slab_shape": (C.c_int, [C.c_void_p, i32, i32]),
     "mz_get_slab_info": (C.c_int, [C.c_void_p, P(i32)]),
+    "mz_batch0d_steps": (C.c_int, [C.c_void_p, i32, i32, u64, f64, P(f64), i32, P(f64), i32]),
+    "mz_set_batch_shape": (C.c_int, [C.c_void_p, i32, i32]),
+    "mz_get_batch_info": (C.c_int, [C.c_void_p, P(i32)]),
     "mz_timing_enable": (C.c_int, [C.c_void_p, i32]),
     "mz_timing_get": (C.c_int, [C.c_void_p, C.c_char_p, P(f64), P(i64)]),
     "mz_timing_reset": (C.c_int, [C.c_void_p]),
@@ -353,6 +356,16 @@ class DeviceStore:
     def set_slab_shape(self, cells_per_slab=0, particles_per_thread=0):
         _check(load_library().mz_set_slab_shape(self.h, cells_per_slab, particles_per_thread))
 
+    def set_batch_shape(self, max_cells=0, target_particles=0):
+        _check(load_library().mz_set_batch_shape(self.h, max_cells, target_particles))
+
+    def batch_info(self):
+        """(items, max cells per item, target particles per item, workgroups)
+        of the last persistent batched 0-D launch."""
+        out = (i32 * 4)()
+        _check(load_library().mz_get_batch_info(self.h, out))
+        return tuple(out)
+
     def slab_info(self):
         """(particles per thread, cells per slab, slabs, outcome) of the last
         slab launch; outcome 1 = ran, -1 = aborted and replayed on the graph."""
@@ -519,6 +532,18 @@ def couette_steps(store, n_steps, step0, dt, species_data=None, V=None, equal_we
     _check(L.mz_couette_steps(store.h, n_steps, step0, dt, _d(Vd), 1 if equal_weight else 0, props_every, _d(m),
                               code))
     return {v: k for k, v in COUETTE_MODES.items()}.get(L.mz_couette_last_path(store.h))
+
+
+def batch0d_steps(store, n_steps, step0, dt, species_data=None, V=None, props_every=1, species=1, mode="batched"):
+    """n_steps of ntc_equal_weight! + compute_props! in every 0-D cell (the
+    per-cell loops of test/test_bkw.jl:88-93 over a batch of cells).  mode
+    "plain" launches the NTC and props kernels per step, "batched" runs all
+    steps in one persistent launch; both give identical results."""
+    m = _masses(store, species_data)
+    Vd = None if V is None else np.ascontiguousarray(np.broadcast_to(np.asarray(V, dtype=np.float64),
+                                                                      (store.n_cells,)))
+    _check(load_library().mz_batch0d_steps(store.h, species, n_steps, step0, dt, _d(Vd), props_every, _d(m),
+                                           {"plain": 0, "batched": 1}[mode]))
 
 
 def multi_index_moments(n):

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmzhip.so")
-SOURCES = ["mz_store.hip", "mz_ntc.hip", "mz_step.hip", "mz_merge.hip", "mz_slab.hip"]
+SOURCES = ["mz_store.hip", "mz_ntc.hip", "mz_step.hip", "mz_merge.hip", "mz_slab.hip", "mz_batch0d.hip"]
 ARCH = os.environ.get("MZ_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions stay off: hipcc is told not to form them.  No

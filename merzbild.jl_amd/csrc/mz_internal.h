@@ -85,6 +85,10 @@ struct mz_store {
     int32_t slab_info[4] = {0, 0, 0, 0};
     int32_t last_path = -1;   // of mz_couette_steps: 0 eager, 1 graph, 2 slab
     int32_t n_moments = 0;    // moments of the last mz_compute_props_moments
+    // batched 0-D cells: item packing (0 = default) and the last launch's
+    // {items, max cells per item, target particles per item, workgroups}
+    int32_t b0_max_cells = 0, b0_target = 0;
+    int32_t b0_info[4] = {0, 0, 0, 0};
 };
 
 MzViews mz_views(mz_store *h);

@@ -144,3 +144,51 @@ __device__ inline int32_t mz_cell_of(double x0, const MzGrid &g, int32_t n_cells
     }
     return c;
 }
+
+// compute_props! of one cell on one wave (physical_props.jl:104-154 and
+// 317-454): n = sum w, vbar = sum w v / n, and T from the centred energy,
+// in one pass with shifted sums (shift = the cell's first particle), lanes
+// striding over the cell's logical positions.  get(i, w, v0, v1, v2) loads
+// logical particle i.  Every kernel that computes props goes through here,
+// so all paths sum in the same order.
+struct MzPropsOut {
+    double n, v0, v1, v2, T;
+};
+
+template <class Get>
+__device__ inline MzPropsOut mz_props_wave(const Get &get, int32_t tot, double mass) {
+    const int lane = threadIdx.x & 63;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    if (tot > 0) {
+        double w0;
+        get(0, w0, sx, sy, sz);
+    }
+    double n = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, e = 0.0;
+    for (int32_t i = lane; i < tot; i += 64) {
+        double w, v0, v1, v2;
+        get(i, w, v0, v1, v2);
+        const double d0 = v0 - sx, d1 = v1 - sy, d2 = v2 - sz;
+        n += w;
+        a0 = fma(w, d0, a0);
+        a1 = fma(w, d1, a1);
+        a2 = fma(w, d2, a2);
+        e = fma(w, d0 * d0 + d1 * d1 + d2 * d2, e);
+    }
+    n = mz_wave_sum(n);
+    a0 = mz_wave_sum(a0);
+    a1 = mz_wave_sum(a1);
+    a2 = mz_wave_sum(a2);
+    e = mz_wave_sum(e);
+    MzPropsOut o{n, 0.0, 0.0, 0.0, 0.0};
+    if (n > 0.0) {
+        const double m0 = a0 / n, m1 = a1 / n, m2 = a2 / n;
+        o.v0 = sx + m0;
+        o.v1 = sy + m1;
+        o.v2 = sz + m2;
+        double E = e - n * (m0 * m0 + m1 * m1 + m2 * m2);
+        if (E < 0.0) E = 0.0;
+        E *= 0.5 * mass / (n * MZ_KB);
+        o.T = (2.0 / 3.0) * E;
+    }
+    return o;
+}

@@ -170,6 +170,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
     __shared__ int32_t cst[SLAB_MAXC], ccn[SLAB_MAXC], runL[SLAB_MAXC], runR[SLAB_MAXC];
     __shared__ int32_t arank[2][SLAB_MB_CAP], akey[2][SLAB_MB_CAP];
     __shared__ double sdraw[SLAB_MAXC][SLAB_DRAWS];
+    __shared__ double ss0[P], ss1[P], ss2[P];   // velocities as the last sort left them (props input)
     __shared__ int32_t s_abort, s_n, s_na[2];
     __shared__ int64_t s_red[4][4];
 
@@ -229,25 +230,57 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
     int32_t pid[R], pkey[R], prank[R], pcell[R];
     int32_t n_prev = n_loc;
     bool aborted = false;
+
+    // The first SLAB_DRAWS draws of every cell's NTC stream for step stp,
+    // computed by threads t0, t0 + stride, ...
+    auto fill_draws = [&](uint64_t stp, int t0, int stride) {
+        MzRngParams rpf = A.rp;
+        rpf.tag = (stp << 4) | 1u;
+        for (int j = t0; j < ncl * (SLAB_DRAWS / 2); j += stride) {
+            const int k = j / (SLAB_DRAWS / 2), blk = j % (SLAB_DRAWS / 2);
+            mz_philox_pair(rpf, (uint64_t)(clo + k), (uint64_t)blk, sdraw[k][2 * blk], sdraw[k][2 * blk + 1]);
+        }
+    };
+    // compute_props_sorted! of the slab's cells from the snapshot the last
+    // scatter wrote: cell k on wave (k + 1) % 4, the sums of k_props.  It runs
+    // while the next step's NTC, on wave k % 4, changes the live velocities.
+    auto props_from_snapshot = [&]() {
+        for (int k = 0; k < ncl; ++k) {
+            if (((k + 1) & 3) != wave) continue;
+            const int32_t st1 = cst[k], tn = ccn[k];
+            auto get = [&](int32_t i, double &w, double &v0, double &v1, double &v2) {
+                const int p = st1 + i;
+                w = sw[p]; v0 = ss0[p]; v1 = ss1[p]; v2 = ss2[p];
+            };
+            const MzPropsOut o = mz_props_wave(get, tn, A.mass);
+            if (lane == 0) {
+                const int c = clo + k;
+                A.out_np[c] = (double)tn;
+                A.out_n[c] = o.n;
+                A.out_v[3 * c] = o.v0;
+                A.out_v[3 * c + 1] = o.v1;
+                A.out_v[3 * c + 2] = o.v2;
+                A.out_T[c] = o.T;
+            }
+        }
+    };
+    auto props_due = [&](int32_t t) { return A.props_every > 0 && ((t + 1) % A.props_every) == 0; };
+    if (A.n_steps > 0) fill_draws(A.step0, tid, SLAB_THREADS);
     __syncthreads();
 
     for (int32_t t = 0; t < A.n_steps; ++t) {
         const uint64_t step = A.step0 + (uint64_t)t;
         const int q = t & 1;
         n_prev = n_loc;
-        // ---- 1. ntc_equal_weight! per cell; clear the ranking tables.  The
-        // first draws of every cell's stream are computed by all threads first.
-        MzRngParams rp_ntc = A.rp;
-        rp_ntc.tag = (step << 4) | 1u;
-        for (int j = tid; j < ncl * (SLAB_DRAWS / 2); j += SLAB_THREADS) {
-            const int k = j / (SLAB_DRAWS / 2), blk = j % (SLAB_DRAWS / 2);
-            mz_philox_pair(rp_ntc, (uint64_t)(clo + k), (uint64_t)blk, sdraw[k][2 * blk], sdraw[k][2 * blk + 1]);
-        }
+        // ---- 1. ntc_equal_weight! per cell (its draws were computed during
+        // the previous step), next to the previous step's props; clear the
+        // ranking tables.
         for (int j = tid; j < NCH * SLAB_KMAX; j += SLAB_THREADS) (&tab[0][0])[j] = 0;
         if (tid < SLAB_MAXC) { runL[tid] = 0; runR[tid] = 0; }
-        __syncthreads();
+        if (t > 0 && props_due(t - 1)) props_from_snapshot();
         if (ntc_lane) {
-            const MzRngParams &rp = rp_ntc;
+            MzRngParams rp = A.rp;
+            rp.tag = (step << 4) | 1u;
             my_nla = ccn[my_k];
             slab_ntc_cell(A, rp, clo + my_k, my_V, cst[my_k], my_nla, sv0, sv1, sv2, sw, sdraw[my_k], my_sgwm,
                           my_ncoll, my_perf, my_eqw);
@@ -378,6 +411,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
             }
             s_na[wave] = na;
         }
+        if (wave >= 2 && t + 1 < A.n_steps) fill_draws(step + 1, tid - 128, SLAB_THREADS - 128);
         __syncthreads();
         if (s_abort) {
             if (tid == 0) slab_raise_abort(A);
@@ -459,6 +493,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
             const int pos = cst[key] + runL[key] + prank[r];
             sx0[pos] = px0[r]; sx1[pos] = px1[r]; sx2[pos] = px2[r];
             sv0[pos] = pv0[r]; sv1[pos] = pv1[r]; sv2[pos] = pv2[r];
+            ss0[pos] = pv0[r]; ss1[pos] = pv1[r]; ss2[pos] = pv2[r];
             sw[pos] = pw[r];
             sid[pos] = pid[r];
         }
@@ -470,6 +505,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
             sv0[pos] = __longlong_as_double((long long)aw[3]);
             sv1[pos] = __longlong_as_double((long long)aw[4]);
             sv2[pos] = __longlong_as_double((long long)aw[5]);
+            ss0[pos] = sv0[pos]; ss1[pos] = sv1[pos]; ss2[pos] = sv2[pos];
             sw[pos] = __longlong_as_double((long long)aw[6]);
             sid[pos] = (int32_t)(aw[7] >> 32);
         }
@@ -489,56 +525,15 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
                 sv0[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 3) + jj));
                 sv1[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 4) + jj));
                 sv2[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 5) + jj));
+                ss0[pos] = sv0[pos]; ss1[pos] = sv1[pos]; ss2[pos] = sv2[pos];
                 sw[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 6) + jj));
                 sid[pos] = (int32_t)(ld_agent(slab_mb(A, src, side, q, 7) + jj) >> 32);
             }
         }
         __syncthreads();
-        // ---- 9. compute_props_sorted!: one wave per cell, the sums of k_props
-        if (A.props_every > 0 && ((t + 1) % A.props_every) == 0) {
-            for (int k = wave; k < ncl; k += 4) {
-                const int32_t st1 = cst[k], tn = ccn[k];
-                double sxs = 0.0, sys = 0.0, szs = 0.0;
-                if (tn > 0) { sxs = sv0[st1]; sys = sv1[st1]; szs = sv2[st1]; }
-                double n = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, e = 0.0;
-                for (int32_t i = lane; i < tn; i += 64) {
-                    const int p = st1 + i;
-                    double w = sw[p];
-                    double d0 = sv0[p] - sxs, d1 = sv1[p] - sys, d2 = sv2[p] - szs;
-                    n += w;
-                    a0 = fma(w, d0, a0);
-                    a1 = fma(w, d1, a1);
-                    a2 = fma(w, d2, a2);
-                    e = fma(w, d0 * d0 + d1 * d1 + d2 * d2, e);
-                }
-                n = mz_wave_sum(n);
-                a0 = mz_wave_sum(a0);
-                a1 = mz_wave_sum(a1);
-                a2 = mz_wave_sum(a2);
-                e = mz_wave_sum(e);
-                if (lane == 0) {
-                    double v0 = 0.0, v1 = 0.0, v2 = 0.0, T = 0.0;
-                    if (n > 0.0) {
-                        double m0 = a0 / n, m1 = a1 / n, m2 = a2 / n;
-                        v0 = sxs + m0; v1 = sys + m1; v2 = szs + m2;
-                        double E = e - n * (m0 * m0 + m1 * m1 + m2 * m2);
-                        if (E < 0.0) E = 0.0;
-                        E *= 0.5 * A.mass / (n * MZ_KB);
-                        T = (2.0 / 3.0) * E;
-                    }
-                    const int c = clo + k;
-                    A.out_np[c] = (double)tn;
-                    A.out_n[c] = n;
-                    A.out_v[3 * c] = v0;
-                    A.out_v[3 * c + 1] = v1;
-                    A.out_v[3 * c + 2] = v2;
-                    A.out_T[c] = T;
-                }
-            }
-        }
-        __syncthreads();
     }
     if (aborted) return;
+    if (A.n_steps > 0 && props_due(A.n_steps - 1)) props_from_snapshot();
 
     // ---- publish the slab's counts, then place it after all earlier slabs
     if (tid == 0) {
@@ -639,7 +634,6 @@ SlabKernel slab_kernel(int R) {
     case 1: return k_slab_couette<1>;
     case 2: return k_slab_couette<2>;
     case 4: return k_slab_couette<4>;
-    case 8: return k_slab_couette<8>;
     default: return nullptr;
     }
 }
@@ -660,7 +654,7 @@ int32_t max_slab_particles(const std::vector<int32_t> &n1, int spc) {
 extern "C" int mz_set_slab_shape(mz_store *h, int32_t cells_per_slab, int32_t particles_per_thread) {
     MZ_CHECK(cells_per_slab >= 0 && cells_per_slab <= SLAB_MAXC, "slab shape: cells per slab out of range");
     MZ_CHECK(particles_per_thread == 0 || slab_kernel(particles_per_thread) != nullptr,
-             "slab shape: particles per thread must be 0, 1, 2, 4 or 8");
+             "slab shape: particles per thread must be 0, 1, 2 or 4");
     h->slab_spc = cells_per_slab;
     h->slab_ppt = particles_per_thread;
     return 0;
@@ -702,7 +696,7 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
     int dev = 0, n_cu = 0;
     MZ_HIP(hipGetDevice(&dev));
     MZ_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int Rs[4] = {1, 2, 4, 8};
+    const int Rs[3] = {1, 2, 4};
     SlabShape sh;
     for (int R : Rs) {
         if (h->slab_ppt && R != h->slab_ppt) continue;

@@ -910,45 +910,20 @@ __global__ void __launch_bounds__(256) k_props(MzViews views, int32_t n_cells, i
         int32_t s1 = sp.pia.start1[c], e1 = sp.pia.end1[c];
         n1 = e1 >= s1 ? e1 - s1 + 1 : 0;
     }
-    int32_t tot = n1 + (n2 > 0 ? n2 : 0);
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    if (tot > 0) {
-        int64_t q = n1 > 0 ? st1 : st2;
-        sx = sp.v0[q]; sy = sp.v1[q]; sz = sp.v2[q];
-    }
-    double n = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, e = 0.0;
-    for (int32_t i = lane; i < tot; i += 64) {
-        int64_t p = i < n1 ? (int64_t)st1 + i : (int64_t)st2 + (i - n1);
-        double w = sp.w[p];
-        double d0 = sp.v0[p] - sx, d1 = sp.v1[p] - sy, d2 = sp.v2[p] - sz;
-        n += w;
-        a0 = fma(w, d0, a0);
-        a1 = fma(w, d1, a1);
-        a2 = fma(w, d2, a2);
-        e = fma(w, d0 * d0 + d1 * d1 + d2 * d2, e);
-    }
-    n = mz_wave_sum(n);
-    a0 = mz_wave_sum(a0);
-    a1 = mz_wave_sum(a1);
-    a2 = mz_wave_sum(a2);
-    e = mz_wave_sum(e);
+    const int32_t tot = n1 + (n2 > 0 ? n2 : 0);
+    auto get = [&](int32_t i, double &w, double &v0, double &v1, double &v2) {
+        const int64_t p = i < n1 ? (int64_t)st1 + i : (int64_t)st2 + (i - n1);
+        w = sp.w[p]; v0 = sp.v0[p]; v1 = sp.v1[p]; v2 = sp.v2[p];
+    };
+    const MzPropsOut o = mz_props_wave(get, tot, masses[sidx]);
     if (lane == 0) {
-        double v0 = 0.0, v1 = 0.0, v2 = 0.0, T = 0.0;
-        if (n > 0.0) {
-            double m0 = a0 / n, m1 = a1 / n, m2 = a2 / n;
-            v0 = sx + m0; v1 = sy + m1; v2 = sz + m2;
-            double E = e - n * (m0 * m0 + m1 * m1 + m2 * m2);
-            if (E < 0.0) E = 0.0;
-            E *= 0.5 * masses[sidx] / (n * MZ_KB);
-            T = (2.0 / 3.0) * E;
-        }
-        int64_t ix = (int64_t)sidx * n_cells + c;
+        const int64_t ix = (int64_t)sidx * n_cells + c;
         out_np[ix] = (double)tot;
-        out_n[ix] = (variant == 1 && ndens) ? n * inv_V_cells[c] : n;
-        out_v[3 * ix] = v0;
-        out_v[3 * ix + 1] = v1;
-        out_v[3 * ix + 2] = v2;
-        out_T[ix] = T;
+        out_n[ix] = (variant == 1 && ndens) ? o.n * inv_V_cells[c] : o.n;
+        out_v[3 * ix] = o.v0;
+        out_v[3 * ix + 1] = o.v1;
+        out_v[3 * ix + 2] = o.v2;
+        out_T[ix] = o.T;
     }
 }
 

@@ -134,3 +134,83 @@ def test_varweight_couette_with_octree_merges(hip, mzo):
     pp = hip.compute_props_sorted(st, sp)
     assert np.allclose(pp.T[0], helpers.oracle_props_sorted(mzo, case), rtol=1e-10)
     st.close()
+
+
+def _batch_case(hip, mzo, nc, nmax, seed):
+    rng = np.random.default_rng(seed)
+    counts = np.floor(np.exp(rng.uniform(math.log(20), math.log(nmax), size=nc))).astype(int)
+    n = int(counts.sum())
+    T0, m = 273.0, ac.M_AR
+    pv, pia = mzo.ParticleVector(n), mzo.PIA(nc, 1)
+    r = mzo.RNG(seed=seed, tag=55)
+    pos = 1
+    Fnum = 1e23 / counts
+    for c in range(1, nc + 1):
+        k = int(counts[c - 1])
+        for j in range(k):
+            pv.pop_to(pos + j)
+            pv.set(pos + j, Fnum[c - 1] * rng.uniform(0.5, 2.0), [0.0, 0.0, 0.0], [c - 0.5, 0.5, 0.5])
+        r.s.stream, r.s.k = c, 0
+        mzo.lib().mzo_sample_bkw(r.ref, pv.p, k, pos - 1, m, T0, (C.c_double * 3)(0.0, 0.0, 0.0))
+        pia.set(c, 1, k, pos, pos + k - 1, k)
+        pos += k
+    pia.set_n_total(1, n)
+    its = mzo.interactions(["Ar"], table=mzo.PSEUDO_MAXWELL)
+    d = mzo.PSEUDO_MAXWELL[("Ar", "Ar")][0]
+    tref = 1.0 / (1e23 * math.pi * d * d) / math.sqrt(2 * ac.KB * T0 / m)
+    sg = np.array([mzo.lib().mzo_estimate_sigma_g_w_max(C.byref(its[0][0]), m, m, T0, T0, float(f), 1.0)
+                   for f in Fnum])
+
+    def store():
+        st = hip.DeviceStore(nc, 1, n + 256)
+        helpers.upload_from_oracle(st, mzo, pv, pia)
+        st.set_grid(hip.Grid1DUniform(float(nc), nc))
+        _interactions(hip, st, its)
+        st.set_sigma_g_w_max(1, 1, sg)
+        st.set_seed(seed)
+        st.set_species_data([hip.Species("Ar", m)])
+        return st
+    return dict(pv=pv, pia=pia, it=its[0][0], sg=sg, dt=0.025 * tref, n=n, nc=nc, store=store, m=m, seed=seed)
+
+
+@pytest.mark.parametrize("shape", [(0, 0), (1, 1), (8, 512), (64, 100000)])
+def test_batched_0d_persistent_equals_per_step(hip, mzo, shape):
+    """The persistent batched launch and the per-step kernels leave identical
+    particles, properties and collision factors; both equal the oracle's
+    ntc_equal_weight! + compute_props! to 1e-11."""
+    case = _batch_case(hip, mzo, nc=300, nmax=3000, seed=21)
+    a, b = case["store"](), case["store"]()
+    steps = 6
+    hip.batch0d_steps(a, steps, 1, case["dt"], mode="plain")
+    b.set_batch_shape(*shape)
+    hip.batch0d_steps(b, steps, 1, case["dt"], mode="batched")
+    info = b.batch_info()
+    assert info[0] >= 1
+    da, db = a.download_species(1), b.download_species(1)
+    assert da["err"] == 0 and db["err"] == 0
+    n = case["n"]
+    for k in ("x", "v", "w", "index"):
+        assert np.array_equal(da[k][:n], db[k][:n]), k
+    pa, pb = hip.PhysProps(case["nc"], 1).fetch(a), hip.PhysProps(case["nc"], 1).fetch(b)
+    for k in ("np", "n", "v", "T"):
+        assert np.array_equal(getattr(pa, k), getattr(pb, k)), k
+    ca, cb = a.collision_factors(1, 1), b.collision_factors(1, 1)
+    for k in ca:
+        assert np.array_equal(ca[k], cb[k]), k
+    # the oracle
+    pv, pia = case["pv"], case["pia"]
+    cfs = (mzo.CF * case["nc"])()
+    for c in range(case["nc"]):
+        cfs[c].sigma_g_w_max = case["sg"][c]
+    for step in range(1, steps + 1):
+        rr = mzo.RNG(seed=case["seed"], tag=(step << 4) | 1)
+        for c in range(1, case["nc"] + 1):
+            mzo.lib().mzo_ntc_single(rr.ref, C.byref(cfs[c - 1]), C.byref(case["it"]), pv.p, pia.p, c, 1,
+                                     case["dt"], 1.0, 1e-16, 1)
+    helpers.assert_state_matches(mzo, db, pv, pia, rtol=1e-11)
+    ref = mzo.Props(case["nc"], 1)
+    mzo.compute_props([pv], pia, [case["m"]], ref)
+    assert np.allclose(pb.T, ref.T, rtol=1e-10) and np.array_equal(pb.np, ref.np)
+    assert sum(cb["n_coll_performed"]) > 0
+    a.close()
+    b.close()

@@ -61,7 +61,7 @@ def test_slab_couette_vs_oracle(hip, mzo, shape):
     st.close()
 
 
-@pytest.mark.parametrize("n_cells,ppc,steps,shape", [(500, 200, 3000, (0, 0)), (97, 300, 400, (4, 8)),
+@pytest.mark.parametrize("n_cells,ppc,steps,shape", [(500, 200, 3000, (0, 0)), (97, 300, 400, (3, 4)),
                                                      (64, 50, 800, (2, 1))])
 def test_slab_equals_graph(hip, mzo, n_cells, ppc, steps, shape):
     """The slab kernel and the captured step graph leave identical state,
