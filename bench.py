@@ -202,12 +202,18 @@ def run_batched_0d(a, pkg):
     def run(k, s0):
         return pkg.batch0d_steps(st, k, s0, dt, sp, mode=mode)
 
-    run(a.warmup if a.warmup != 200 else 5, 1)
+    tune = None
+    s0 = 1000
+    if mode == "batched" and not a.no_autotune:
+        best, times, s0 = pkg.batch0d_autotune(st, 1, dt, sp)
+        tune = {"best": list(best), "ms_per_step": {f"{k[0]}x{k[1]}": round(v, 4) for k, v in times.items()}}
+    else:
+        run(a.warmup if a.warmup != 200 else 5, 1)
     st.sync()
     st.timing_reset()
     st.timing(True)
     t0 = time.perf_counter()
-    run(steps, 1000)
+    run(steps, s0)
     st.sync()
     el = time.perf_counter() - t0
     st.timing(False)
@@ -219,7 +225,7 @@ def run_batched_0d(a, pkg):
            "config": {"workload": "65536 0-D cells, log-uniform 20..20000 particles (Philox seed 7), "
                                   "pseudo-Maxwell Ar, BKW at 273 K, w ~ U(0.5, 2) Fnum; ntc_equal_weight! + "
                                   "compute_props! per step", "n_particles": n, "mode": mode,
-                      "batch_shape": st.batch_info() if mode == "batched" else None}}
+                      "batch_shape": st.batch_info() if mode == "batched" else None, "autotune": tune}}
     print(json.dumps(out))
     st.close()
 
@@ -234,6 +240,7 @@ def main():
                          "eager: kernel by kernel")
     ap.add_argument("--graph-steps", type=int, default=400, help="steps of the step-graph path timed alongside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="slab mode: keep the library's default shape")
     ap.add_argument("--workload", choices=("couette", "batched-0d"), default="couette",
                     help="couette: the headline 1-D Couette line; batched-0d: 65,536 uneven 0-D cells")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
@@ -261,10 +268,16 @@ def main():
     st, sp, it = setup_store(pkg, case)
     n = case["n"]
 
-    # warmup (also captures the step graph / sizes the slab scratch)
-    pkg.couette_steps(st, a.warmup, 1, DT, props_every=1, mode=a.mode)
+    # warmup (also captures the step graph / sizes the slab scratch); in slab
+    # mode the warmup is the shape autotuner's trial runs
+    tune = None
+    if a.mode == "slab" and not a.no_autotune:
+        best, times, step0 = pkg.couette_autotune(st, 1, DT, steps=a.warmup)
+        tune = {"best": list(best), "ms_per_step": {f"{k[0]}x{k[1]}": round(v, 5) for k, v in times.items()}}
+    else:
+        pkg.couette_steps(st, a.warmup, 1, DT, props_every=1, mode=a.mode)
+        step0 = 1 + a.warmup
     st.sync()
-    step0 = 1 + a.warmup
     st.timing_reset()
     st.timing(True)
     t0 = time.perf_counter()
@@ -337,7 +350,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "1D Couette, argon VHS, 500 cells x 200 particles (100k), dt 2e-10 s, FP64; "
                                "per step ntc!, convection, counting sort, compute_props_sorted!",
-                   "n_particles": n, "n_cells": N_CELLS, "mode": path, "slab_shape": slab_info,
+                   "n_particles": n, "n_cells": N_CELLS, "mode": path, "slab_shape": slab_info, "autotune": tune,
                    "graph_ms_per_step": graph_ms, "step_kernels_ms": kern},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -534,6 +534,58 @@ def couette_steps(store, n_steps, step0, dt, species_data=None, V=None, equal_we
     return {v: k for k, v in COUETTE_MODES.items()}.get(L.mz_couette_last_path(store.h))
 
 
+def couette_autotune(store, step0, dt, species_data=None, steps=200,
+                     shapes=((0, 0), (1, 1), (1, 2), (2, 2), (2, 4), (3, 4), (4, 4))):
+    """Pick the slab shape (cells per slab, particles per thread) of the
+    slab-resident Couette step by running `steps` steps with each candidate
+    on the store itself (the steps advance the simulation, as warmup steps
+    would) and keeping the fastest one that ran without falling back.
+    Returns (best shape, {shape: ms per step}, next step number)."""
+    times = {}
+    s = step0
+    for shp in shapes:
+        store.set_slab_shape(*shp)
+        store.sync()
+        store.timing_reset()
+        store.timing(True)
+        path = couette_steps(store, steps, s, dt, species_data, mode="slab")
+        store.sync()
+        ms, _ = store.timing_get("slab")
+        store.timing(False)
+        s += steps
+        if path == "slab" and ms > 0:
+            times[tuple(shp)] = ms
+    if not times:
+        store.set_slab_shape(0, 0)
+        return (0, 0), times, s
+    best = min(times, key=times.get)
+    store.set_slab_shape(*best)
+    return best, times, s
+
+
+def batch0d_autotune(store, step0, dt, species_data=None, steps=3,
+                     shapes=((64, 4096), (64, 1024), (16, 4096), (8, 8192), (32, 16384), (4, 2048))):
+    """Pick the item packing (max cells, target particles) of the persistent
+    batched 0-D path by timing `steps` steps with each candidate on the store
+    itself.  Returns (best shape, {shape: ms per step}, next step number)."""
+    times = {}
+    s = step0
+    for shp in shapes:
+        store.set_batch_shape(*shp)
+        store.sync()
+        store.timing_reset()
+        store.timing(True)
+        batch0d_steps(store, steps, s, dt, species_data, mode="batched")
+        store.sync()
+        ms, _ = store.timing_get("batch0d")
+        store.timing(False)
+        s += steps
+        times[tuple(shp)] = ms
+    best = min(times, key=times.get)
+    store.set_batch_shape(*best)
+    return best, times, s
+
+
 def batch0d_steps(store, n_steps, step0, dt, species_data=None, V=None, props_every=1, species=1, mode="batched"):
     """n_steps of ntc_equal_weight! + compute_props! in every 0-D cell (the
     per-cell loops of test/test_bkw.jl:88-93 over a batch of cells).  mode
