@@ -58,11 +58,11 @@ def assert_state_matches(mzo, got, pv, pia, species=1, rtol=1e-12, check_buffer=
 
 
 def couette_case(mzo, n_cells=8, ppc=64, seed=5, T=300.0, ndens=1e23, v_wall=500.0, dt=2.0e-10,
-                 weights=None, sgwm_mult=1.0):
+                 weights=None, sgwm_mult=1.0, dx=1e-6):
     """A 1-D Couette setup in the reference layout (test/test_1D_couette.jl):
-    argon VHS, cells of 1e-6 m, diffuse walls at T moving at -/+ v_wall,
+    argon VHS, cells of dx, diffuse walls at T moving at -/+ v_wall,
     ppc particles per cell sampled Maxwellian, Fnum = ndens dx / ppc."""
-    L = 1e-6 * n_cells
+    L = dx * n_cells
     grid = mzo.grid1d(L, n_cells)
     walls = mzo.walls([MASS_AR], T, T, -v_wall, v_wall, 1.0, 1.0)
     it = mzo.interactions(["Ar"])[0][0]

@@ -1,7 +1,7 @@
 """End-to-end parity of the batched 0-D and variable-weight Couette
 workloads at reduced sizes, step by step against the oracle.
 
-Batched uneven 0-D (BASELINE config 3 shape): cells of log-uniform occupancy,
+Batched uneven 0-D (65,536 cells at full size; here 192): cells of log-uniform occupancy,
 pseudo-Maxwell argon, BKW velocities at 273 K, weights uniform in
 [0.5, 2] Fnum, ntc! with weight splitting in every cell, then the sort
 (which folds the created particles into their cells) and compute_props!.

@@ -1,0 +1,149 @@
+"""Production (Philox) runs against the reference's own fixtures: the
+fixtures were produced with Julia's StableRNG, whose stream cannot be
+reproduced here, so a 16-seed ensemble of device runs defines a band
+(mean +- 3 sigma of single runs at each output) and the fixture trajectory
+must lie inside it: at least 95% of its points within 3 sigma and all within
+5 sigma.  Fixtures: tests/golden/reference_fixtures.npz, extracted from
+test/data/*.nc by tests/golden/make_reference_fixtures.py.
+
+  two-gas relaxation   test/test_2species.jl      T of Ar and He, 800 steps
+  BKW                  test/test_bkw.jl           total moments 4, 6, 8, 10, 500 steps
+  1-D Couette          test/test_1D_couette.jl    n, v_y, T of 50 cells every 1000 of 6000 steps
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+import analytic_cases as ac
+import helpers
+
+pytestmark = pytest.mark.gpu
+
+N_SEEDS = 16
+FIX = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_fixtures.npz"))
+
+
+def _band_check(runs, ref, name):
+    """runs [seed, ...], ref [...]: the fixture inside mean +- 3 sigma at >= 95%
+    of points and inside 5 sigma everywhere."""
+    mu, sd = runs.mean(axis=0), runs.std(axis=0, ddof=1)
+    z = np.abs(ref - mu) / np.maximum(sd, 1e-300)
+    frac = float(np.mean(z <= 3.0))
+    assert frac >= 0.95 and float(z.max()) <= 5.0, (name, frac, float(z.max()))
+    return frac
+
+
+def _interactions(hip, st, its):
+    n = len(its)
+    st.set_interactions([[type("I", (), dict(mu1=its[i][k].mu1, mu2=its[i][k].mu2, vhs_factor=its[i][k].vhs_factor,
+                                             vhs_o=its[i][k].vhs_o))() for k in range(n)] for i in range(n)])
+
+
+def test_two_gas_ensemble_brackets_reference(hip, mzo):
+    """Sixteen replicas as sixteen cells of one store (each cell draws its own
+    streams), ntc! for Ar-Ar, He-Ar, He-He and compute_props! every step."""
+    nr = N_SEEDS
+    nA, nH = ac.TWO_GAS["n"]
+    pvs = [mzo.ParticleVector(nr * nA), mzo.ParticleVector(nr * nH)]
+    pia = mzo.PIA(nr, 2)
+    r = mzo.RNG(seed=2024, tag=99)
+    for c in range(1, nr + 1):
+        for s, (n, m, T) in enumerate(zip(ac.TWO_GAS["n"], (ac.M_AR, ac.M_HE), ac.TWO_GAS["T0"]), start=1):
+            r.s.stream, r.s.k = 2 * c + s, 0
+            mzo.lib().mzo_sample_particles_equal_weight(r.ref, pvs[s - 1].p, pia.p, c, s, n, m, T, ac.TWO_GAS["Fnum"],
+                                                        0, 1, 0, 1, 0, 1)
+    its = mzo.interactions(["Ar", "He"])
+    st = hip.DeviceStore(nr, 2, [nr * nA + 256, nr * nH + 256])
+    for s in (1, 2):
+        helpers.upload_from_oracle(st, mzo, pvs[s - 1], pia, species=s)
+    _interactions(hip, st, its)
+    masses = (ac.M_AR, ac.M_HE)
+    for (i, k) in ((1, 1), (2, 1), (2, 2), (1, 2)):
+        sg = mzo.lib().mzo_estimate_sigma_g_w_max(C.byref(its[i - 1][k - 1]), masses[i - 1], masses[k - 1],
+                                                  ac.TWO_GAS["T0"][i - 1], ac.TWO_GAS["T0"][k - 1],
+                                                  ac.TWO_GAS["Fnum"], 1.0)
+        st.set_sigma_g_w_max(i, k, np.full(nr, sg))
+    st.set_seed(2024)
+    sp = [hip.Species("Ar", ac.M_AR), hip.Species("He", ac.M_HE)]
+    pp = hip.PhysProps(nr, 2)
+    T = [hip.compute_props(st, sp, pp).T.copy()]
+    V = np.ones(nr)
+    for step in range(1, ac.TWO_GAS["steps"] + 1):
+        for (s1, s2) in ac.TWO_GAS_PAIRS:
+            hip.ntc(st, s1, s2, ac.TWO_GAS["dt"], V=V, tag=ac.two_gas_tag(step, s1, s2))
+        T.append(hip.compute_props(st, sp, pp).T.copy())
+        assert np.all(pp.np[0] == nA) and np.all(pp.np[1] == nH)
+    T = np.array(T)                       # [step, species, replica]
+    runs = np.transpose(T, (2, 0, 1))     # [replica, step, species]
+    ref = FIX["2species_T"][:, :, 0]      # [step, species]
+    _band_check(runs, ref, "two-gas T")
+    st.close()
+
+
+def test_bkw_ensemble_brackets_reference(hip, mzo):
+    """Sixteen BKW replicas of 20,000 particles as sixteen cells; ntc! and
+    compute_props_with_total_moments! every step."""
+    nr, n = N_SEEDS, ac.BKW["n"]
+    T0, m = ac.BKW["T0"], ac.M_AR
+    Fnum = ac.BKW["ndens"] / n
+    pv, pia = mzo.ParticleVector(nr * n), mzo.PIA(nr, 1)
+    r = mzo.RNG(seed=77, tag=77)
+    for c in range(1, nr + 1):
+        r.s.stream, r.s.k = 2 * c, 0
+        mzo.lib().mzo_sample_particles_equal_weight(r.ref, pv.p, pia.p, c, 1, n, m, T0, Fnum, 0, 1, 0, 1, 0, 1)
+        r.s.stream, r.s.k = 2 * c + 1, 0
+        mzo.lib().mzo_sample_bkw(r.ref, pv.p, n, (c - 1) * n, m, T0, (C.c_double * 3)(0.0, 0.0, 0.0))
+    case = ac.bkw_case(mzo)
+    its = mzo.interactions(["Ar"], table=mzo.PSEUDO_MAXWELL)
+    st = hip.DeviceStore(nr, 1, nr * n + 256)
+    helpers.upload_from_oracle(st, mzo, pv, pia)
+    _interactions(hip, st, its)
+    st.set_sigma_g_w_max(1, 1, np.full(nr, case["cf"].sigma_g_w_max))
+    st.set_seed(77)
+    sp = [hip.Species("Ar", m)]
+    pp = hip.PhysProps(nr, 1, ac.BKW["moments"], Tref=T0)
+    mom = [hip.compute_props_with_total_moments(st, sp, pp).moments[0].copy()]
+    V = np.ones(nr)
+    for step in range(1, ac.BKW["steps"] + 1):
+        hip.ntc(st, 1, 1, case["dt"], V=V, tag=(step << 4) | 1)
+        mom.append(hip.compute_props_with_total_moments(st, sp, pp).moments[0].copy())
+    mom = np.array(mom)                   # [step, replica, moment]
+    runs = np.transpose(mom, (1, 0, 2))
+    ref = FIX["bkw_moments"][:, 0, 0, :]
+    _band_check(runs[:, :, :3], ref[:, :3], "BKW moments 4, 6, 8")
+    st.close()
+
+
+def test_couette_ensemble_brackets_reference(hip, mzo):
+    """Sixteen replicas of test/test_1D_couette.jl (50 cells of 1e-5 m, 1000
+    particles per cell, n = 5e22, dt = 2.59e-9 s) on the slab kernel, props
+    every 1000 steps."""
+    outs = {"n": [], "vy": [], "T": []}
+    for seed in range(1, N_SEEDS + 1):
+        case = helpers.couette_case(mzo, n_cells=50, ppc=1000, seed=seed, T=300.0, ndens=5e22, v_wall=500.0,
+                                    dt=2.59e-9, dx=1e-5)
+        st = helpers.store_from_case(hip, case)
+        sp = case["species"]
+        pp = hip.compute_props_sorted(st, sp)
+        n, vy, T = [pp.n[0].copy()], [pp.v[0, :, 1].copy()], [pp.T[0].copy()]
+        for k in range(6):
+            path = hip.couette_steps(st, 1000, 1 + 1000 * k, case["dt"], props_every=1000, mode="slab")
+            assert path == "slab"
+            pp = hip.PhysProps(50, 1).fetch(st)
+            n.append(pp.n[0].copy())
+            vy.append(pp.v[0, :, 1].copy())
+            T.append(pp.T[0].copy())
+        assert st.scalars(1)["err"] == 0
+        outs["n"].append(n)
+        outs["vy"].append(vy)
+        outs["T"].append(T)
+        st.close()
+    ref_n = FIX["couette_ndens"][:, 0, :]
+    ref_v = FIX["couette_v"][:, 0, :, 1]
+    ref_T = FIX["couette_T"][:, 0, :]
+    _band_check(np.array(outs["T"]), ref_T, "Couette T")
+    _band_check(np.array(outs["vy"])[:, 1:], ref_v[1:], "Couette v_y")
+    _band_check(np.array(outs["n"]), ref_n, "Couette n")
