@@ -160,7 +160,7 @@ int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, double dt, co
 /* Which path the last mz_couette_steps call took: 0 eager, 1 graph, 2 slab. */
 int mz_couette_last_path(mz_store *h);
 /* Slab shape of mode 2 (the reference's n_chunks, couette_multithreaded.jl:
- * 13-14, 31): cells per slab (1..32) and particles per thread (1, 2, 4) of
+ * 13-14, 31): cells per slab (1..32) and particles per thread (1, 2, 4, 8) of
  * the LDS capacity 256 x ppt per slab; 0 lets the library choose. */
 int mz_set_slab_shape(mz_store *h, int32_t cells_per_slab, int32_t particles_per_thread);
 /* info[0..3] = particles per thread, cells per slab, slabs, outcome (1 ran,

@@ -162,6 +162,11 @@ template <int R>
 __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(SlabArgs A) {
     constexpr int P = R * SLAB_THREADS;   // LDS particle capacity
     constexpr int NCH = P / 64;           // 64-particle chunks
+    // With R <= 4 the velocities the sort left are also kept in a snapshot,
+    // so a step's props run next to the following NTC; the largest slabs have
+    // no LDS room for it and compute props right after the scatter.
+    constexpr bool SNAP = R <= 4;
+    constexpr int PS = SNAP ? P : 1;
     static_assert(NCH <= 64, "chunk scan uses one wave");
     __shared__ double sx0[P], sx1[P], sx2[P], sv0[P], sv1[P], sv2[P], sw[P];
     __shared__ int32_t sid[P];
@@ -170,7 +175,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
     __shared__ int32_t cst[SLAB_MAXC], ccn[SLAB_MAXC], runL[SLAB_MAXC], runR[SLAB_MAXC];
     __shared__ int32_t arank[2][SLAB_MB_CAP], akey[2][SLAB_MB_CAP];
     __shared__ double sdraw[SLAB_MAXC][SLAB_DRAWS];
-    __shared__ double ss0[P], ss1[P], ss2[P];   // velocities as the last sort left them (props input)
+    __shared__ double ss0[PS], ss1[PS], ss2[PS];   // velocities as the last sort left them (props input)
     __shared__ int32_t s_abort, s_n, s_na[2];
     __shared__ int64_t s_red[4][4];
 
@@ -244,13 +249,14 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
     // compute_props_sorted! of the slab's cells from the snapshot the last
     // scatter wrote: cell k on wave (k + 1) % 4, the sums of k_props.  It runs
     // while the next step's NTC, on wave k % 4, changes the live velocities.
-    auto props_from_snapshot = [&]() {
+    auto props_cells = [&]() {
         for (int k = 0; k < ncl; ++k) {
             if (((k + 1) & 3) != wave) continue;
             const int32_t st1 = cst[k], tn = ccn[k];
             auto get = [&](int32_t i, double &w, double &v0, double &v1, double &v2) {
                 const int p = st1 + i;
-                w = sw[p]; v0 = ss0[p]; v1 = ss1[p]; v2 = ss2[p];
+                if (SNAP) { w = sw[p]; v0 = ss0[p]; v1 = ss1[p]; v2 = ss2[p]; }
+                else { w = sw[p]; v0 = sv0[p]; v1 = sv1[p]; v2 = sv2[p]; }
             };
             const MzPropsOut o = mz_props_wave(get, tn, A.mass);
             if (lane == 0) {
@@ -277,7 +283,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
         // ranking tables.
         for (int j = tid; j < NCH * SLAB_KMAX; j += SLAB_THREADS) (&tab[0][0])[j] = 0;
         if (tid < SLAB_MAXC) { runL[tid] = 0; runR[tid] = 0; }
-        if (t > 0 && props_due(t - 1)) props_from_snapshot();
+        if (SNAP && t > 0 && props_due(t - 1)) props_cells();
         if (ntc_lane) {
             MzRngParams rp = A.rp;
             rp.tag = (step << 4) | 1u;
@@ -493,7 +499,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
             const int pos = cst[key] + runL[key] + prank[r];
             sx0[pos] = px0[r]; sx1[pos] = px1[r]; sx2[pos] = px2[r];
             sv0[pos] = pv0[r]; sv1[pos] = pv1[r]; sv2[pos] = pv2[r];
-            ss0[pos] = pv0[r]; ss1[pos] = pv1[r]; ss2[pos] = pv2[r];
+            if (SNAP) { ss0[pos] = pv0[r]; ss1[pos] = pv1[r]; ss2[pos] = pv2[r]; }
             sw[pos] = pw[r];
             sid[pos] = pid[r];
         }
@@ -505,7 +511,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
             sv0[pos] = __longlong_as_double((long long)aw[3]);
             sv1[pos] = __longlong_as_double((long long)aw[4]);
             sv2[pos] = __longlong_as_double((long long)aw[5]);
-            ss0[pos] = sv0[pos]; ss1[pos] = sv1[pos]; ss2[pos] = sv2[pos];
+            if (SNAP) { ss0[pos] = sv0[pos]; ss1[pos] = sv1[pos]; ss2[pos] = sv2[pos]; }
             sw[pos] = __longlong_as_double((long long)aw[6]);
             sid[pos] = (int32_t)(aw[7] >> 32);
         }
@@ -525,15 +531,19 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
                 sv0[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 3) + jj));
                 sv1[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 4) + jj));
                 sv2[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 5) + jj));
-                ss0[pos] = sv0[pos]; ss1[pos] = sv1[pos]; ss2[pos] = sv2[pos];
+                if (SNAP) { ss0[pos] = sv0[pos]; ss1[pos] = sv1[pos]; ss2[pos] = sv2[pos]; }
                 sw[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 6) + jj));
                 sid[pos] = (int32_t)(ld_agent(slab_mb(A, src, side, q, 7) + jj) >> 32);
             }
         }
         __syncthreads();
+        if (!SNAP && props_due(t)) {
+            props_cells();
+            __syncthreads();
+        }
     }
     if (aborted) return;
-    if (A.n_steps > 0 && props_due(A.n_steps - 1)) props_from_snapshot();
+    if (SNAP && A.n_steps > 0 && props_due(A.n_steps - 1)) props_cells();
 
     // ---- publish the slab's counts, then place it after all earlier slabs
     if (tid == 0) {
@@ -634,6 +644,7 @@ SlabKernel slab_kernel(int R) {
     case 1: return k_slab_couette<1>;
     case 2: return k_slab_couette<2>;
     case 4: return k_slab_couette<4>;
+    case 8: return k_slab_couette<8>;
     default: return nullptr;
     }
 }
@@ -654,7 +665,7 @@ int32_t max_slab_particles(const std::vector<int32_t> &n1, int spc) {
 extern "C" int mz_set_slab_shape(mz_store *h, int32_t cells_per_slab, int32_t particles_per_thread) {
     MZ_CHECK(cells_per_slab >= 0 && cells_per_slab <= SLAB_MAXC, "slab shape: cells per slab out of range");
     MZ_CHECK(particles_per_thread == 0 || slab_kernel(particles_per_thread) != nullptr,
-             "slab shape: particles per thread must be 0, 1, 2 or 4");
+             "slab shape: particles per thread must be 0, 1, 2, 4 or 8");
     h->slab_spc = cells_per_slab;
     h->slab_ppt = particles_per_thread;
     return 0;
@@ -696,7 +707,7 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
     int dev = 0, n_cu = 0;
     MZ_HIP(hipGetDevice(&dev));
     MZ_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int Rs[3] = {1, 2, 4};
+    const int Rs[4] = {1, 2, 4, 8};
     SlabShape sh;
     for (int R : Rs) {
         if (h->slab_ppt && R != h->slab_ppt) continue;
