@@ -163,8 +163,12 @@ int mz_couette_last_path(mz_store *h);
  * 13-14, 31): cells per slab (1..32) and particles per thread (1, 2, 4, 8) of
  * the LDS capacity 256 x ppt per slab; 0 lets the library choose. */
 int mz_set_slab_shape(mz_store *h, int32_t cells_per_slab, int32_t particles_per_thread);
+/* Workgroups per CU the slab kernel with the given particles per thread can
+ * keep resident, and its LDS bytes per workgroup. */
+int mz_slab_occupancy(int32_t particles_per_thread, int32_t *blocks_per_cu, int32_t *lds_bytes);
 /* info[0..3] = particles per thread, cells per slab, slabs, outcome (1 ran,
- * -1 aborted and replayed on mode 1) of the last mode-2 launch. */
+ * -1 aborted and replayed on mode 1, -2 the state was not the sorted state,
+ * -3 no slab shape fits the cells in LDS) of the last mode-2 call. */
 int mz_get_slab_info(mz_store *h, int32_t *info);
 
 /* Many independent 0-D cells: n_steps of ntc_equal_weight! in every cell of

@@ -60,6 +60,7 @@ _SIGS = {
     "mz_couette_last_path": (C.c_int, [C.c_void_p]),
     "mz_set_slab_shape": (C.c_int, [C.c_void_p, i32, i32]),
     "mz_get_slab_info": (C.c_int, [C.c_void_p, P(i32)]),
+    "mz_slab_occupancy": (C.c_int, [i32, P(i32), P(i32)]),
     "mz_batch0d_steps": (C.c_int, [C.c_void_p, i32, i32, u64, f64, P(f64), i32, P(f64), i32]),
     "mz_set_batch_shape": (C.c_int, [C.c_void_p, i32, i32]),
     "mz_get_batch_info": (C.c_int, [C.c_void_p, P(i32)]),
@@ -368,7 +369,8 @@ class DeviceStore:
 
     def slab_info(self):
         """(particles per thread, cells per slab, slabs, outcome) of the last
-        slab launch; outcome 1 = ran, -1 = aborted and replayed on the graph."""
+        slab call; outcome 1 = ran, -1 = aborted and replayed on the graph,
+        -2 = the state was not the sorted state, -3 = no shape fits in LDS."""
         out = (i32 * 4)()
         _check(load_library().mz_get_slab_info(self.h, out))
         return tuple(out)
@@ -532,6 +534,13 @@ def couette_steps(store, n_steps, step0, dt, species_data=None, V=None, equal_we
     _check(L.mz_couette_steps(store.h, n_steps, step0, dt, _d(Vd), 1 if equal_weight else 0, props_every, _d(m),
                               code))
     return {v: k for k, v in COUETTE_MODES.items()}.get(L.mz_couette_last_path(store.h))
+
+
+def slab_occupancy(particles_per_thread):
+    """(workgroups per CU, LDS bytes per workgroup) of the slab kernel."""
+    b, l = i32(0), i32(0)
+    _check(load_library().mz_slab_occupancy(particles_per_thread, C.byref(b), C.byref(l)))
+    return b.value, l.value
 
 
 def couette_autotune(store, step0, dt, species_data=None, steps=200,

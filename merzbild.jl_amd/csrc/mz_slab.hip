@@ -58,7 +58,7 @@ struct SlabArgs {
     double *out_np, *out_n, *out_v, *out_T;
     int32_t n_cells, spc, n_slabs, spx;
     uint64_t *mb;               // mailboxes [slab][side][parity][field][SLAB_MB_CAP]
-    uint64_t *flag;             // [slab][side] x SLAB_LINE: [0] = step + 1, [2 + parity] = count
+    uint64_t *flag;             // [slab][side] x SLAB_LINE: [parity] = (step + 1) << 32 | count
     uint64_t *fin;              // [slab] x SLAB_LINE: [0] = 1 when done, [1] = counts
     int32_t *abort_flag;
 };
@@ -383,19 +383,18 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
             break;
         }
         // ---- 5. publish, then wait for the neighbours' mailboxes of this step.
-        // The count goes to the slot of this step's parity: a neighbour may
-        // already publish the next step before this slab reads the flag, but
-        // it cannot reach the step after that (which reuses the slot) before
-        // this slab has published the next step itself.
+        // Each slab side has one word per step parity holding the step mark
+        // and the count, so the poll that sees the mark also delivers the
+        // count.  A neighbour may already publish the next step (the other
+        // word) before this slab reads its word, but it cannot reach the step
+        // after that, which rewrites this word, before this slab has
+        // published the next step itself.
         const uint64_t mark = (uint64_t)(t + 1);
         if (tid == 0) {
-            uint64_t *fl = A.flag + ((size_t)g * 2 + 0) * SLAB_LINE;
-            uint64_t *fr = A.flag + ((size_t)g * 2 + 1) * SLAB_LINE;
-            if (has_left) st_agent(fl + 2 + q, (uint64_t)tot[KL]);
-            if (has_right) st_agent(fr + 2 + q, (uint64_t)tot[KR]);
-            slab_wait_mem();
-            if (has_left) st_agent(fl, mark);
-            if (has_right) st_agent(fr, mark);
+            // (step + 1) << 32 | count in the word of this step's parity; the
+            // payload stores of all waves completed before the barrier above
+            if (has_left) st_agent(A.flag + ((size_t)g * 2 + 0) * SLAB_LINE + q, (mark << 32) | (uint32_t)tot[KL]);
+            if (has_right) st_agent(A.flag + ((size_t)g * 2 + 1) * SLAB_LINE + q, (mark << 32) | (uint32_t)tot[KR]);
         }
         if (lane == 0 && wave < 2) {
             const bool has = wave == 0 ? has_left : has_right;
@@ -403,10 +402,10 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
             if (has) {
                 const int src = wave == 0 ? g - 1 : g + 1;
                 const int side = wave == 0 ? 1 : 0;
-                const uint64_t *f = A.flag + ((size_t)src * 2 + side) * SLAB_LINE;
+                const uint64_t *f = A.flag + ((size_t)src * 2 + side) * SLAB_LINE + q;
                 uint64_t v = 0;
-                if (slab_wait(f, mark, 0, A.abort_flag, v)) {
-                    na = (int32_t)ld_agent(f + 2 + q);
+                if (slab_wait(f, mark, 32, A.abort_flag, v)) {
+                    na = (int32_t)(uint32_t)(v & 0xFFFFFFFFu);
                     if (na < 0 || na > SLAB_MB_CAP) {
                         s_abort = 1;
                         na = 0;
@@ -671,6 +670,18 @@ extern "C" int mz_set_slab_shape(mz_store *h, int32_t cells_per_slab, int32_t pa
     return 0;
 }
 
+extern "C" int mz_slab_occupancy(int32_t particles_per_thread, int32_t *blocks_per_cu, int32_t *lds_bytes) {
+    SlabKernel fn = slab_kernel(particles_per_thread);
+    MZ_CHECK(fn != nullptr, "slab occupancy: particles per thread must be 1, 2, 4 or 8");
+    int per_cu = 0;
+    MZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, SLAB_THREADS, 0));
+    hipFuncAttributes fa;
+    MZ_HIP(hipFuncGetAttributes(&fa, (const void *)fn));
+    if (blocks_per_cu) *blocks_per_cu = per_cu;
+    if (lds_bytes) *lds_bytes = (int32_t)fa.sharedSizeBytes;
+    return 0;
+}
+
 extern "C" int mz_get_slab_info(mz_store *h, int32_t *info) {
     MZ_CHECK(info != nullptr, "slab info: NULL output");
     for (int k = 0; k < 4; ++k) info[k] = h->slab_info[k];
@@ -684,6 +695,10 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
     *ran = 0;
     MzSpecies &s = h->sp[0];
     const int nc = h->n_cells;
+    // outcome codes of slab_info[3] when the slab kernel does not run
+    const int32_t NOT_SORTED = -2, NO_SHAPE = -3;
+    h->slab_info[0] = h->slab_info[1] = h->slab_info[2] = 0;
+    h->slab_info[3] = NOT_SORTED;
     if (n_steps <= 0 || n_steps >= (1 << 28)) return 0;
     // precondition: the state a sort leaves (group 1 only, consecutive in cell order)
     MzCounters c;
@@ -701,6 +716,7 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
         pos += n1[k];
     }
     if (pos - 1 != c.n_total) return 0;
+    h->slab_info[3] = NO_SHAPE;
 
     // shape: the fewest particles per thread whose LDS slab holds twice the
     // current largest slab, with as many slabs as can be resident
