@@ -132,7 +132,7 @@ def test_couette_ensemble_brackets_reference(hip, mzo):
         n, vy, T = [pp.n[0].copy()], [pp.v[0, :, 1].copy()], [pp.T[0].copy()]
         for k in range(6):
             path = hip.couette_steps(st, 1000, 1 + 1000 * k, case["dt"], props_every=1000, mode="slab")
-            assert path == "slab", st.slab_info()
+            assert path == "slab", (st.slab_info(), [hip.slab_occupancy(r) for r in (1, 2, 4, 8)])
             pp = hip.PhysProps(50, 1).fetch(st)
             n.append(pp.n[0].copy())
             vy.append(pp.v[0, :, 1].copy())
