@@ -31,6 +31,7 @@
 #include "mz_phys.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -719,7 +720,7 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
     h->slab_info[3] = NO_SHAPE;
 
     // shape: the fewest particles per thread whose LDS slab holds the current
-    // largest slab with 25% to spare, with as many slabs as can be resident
+    // largest slab with room to spare, with as many slabs as can be resident
     int dev = 0, n_cu = 0;
     MZ_HIP(hipGetDevice(&dev));
     MZ_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -738,9 +739,11 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
         const int spx = (n_slabs + 7) / 8;
         if (8 * spx > max_wg) continue;
         const int64_t P = (int64_t)R * SLAB_THREADS;
-        // 25% headroom for the cells' occupancy to fluctuate; a slab that still
-        // overflows aborts the launch and the steps run on the step graph
-        if (5 * (int64_t)max_slab_particles(n1, spc) > 4 * P && !h->slab_ppt) continue;
+        // room for the slab's occupancy to fluctuate (25% plus four standard
+        // deviations of a Poisson count); a slab that still overflows aborts
+        // the launch and the steps run on the step graph
+        const double mx = (double)max_slab_particles(n1, spc);
+        if (1.25 * mx + 4.0 * std::sqrt(mx) > (double)P && !h->slab_ppt) continue;
         if ((int64_t)max_slab_particles(n1, spc) > P) continue;
         sh.R = R; sh.spc = spc; sh.n_slabs = n_slabs; sh.spx = spx; sh.grid = 8 * spx; sh.fn = fn;
         break;
