@@ -33,6 +33,11 @@ T_WALL, V_WALL, NDENS, DT = 300.0, 500.0, 1e23, 2.0e-10
 # the key twice and reads/writes the index 16 B; props read w, v and the index
 # 36 B; NTC touches < 1 B per particle at this collision rate.
 ALG_BYTES = {"convect": 28.0, "sort": 16.0, "props": 36.0, "ntc": 1.0, "step": 80.0}
+# HBM bytes the slab kernel moves per particle-step, from rocprofv3 FETCH_SIZE
+# (x 2.0) and WRITE_SIZE (x 1.0) counters of a 2000-step launch on this
+# workload, the factors calibrated on a 256 MiB device copy
+# (profiles/couette_100k_slab_step.md)
+SLAB_HBM_BYTES_PER_PARTICLE_STEP = 16.9
 
 
 def sample_case(seed=12345):
@@ -417,8 +422,10 @@ def main():
     step_s = launch_ms * 1e-3
     achieved = ALG_BYTES["step"] * n / step_s / 1e9
     peak = 8000.0
+    traffic = SLAB_HBM_BYTES_PER_PARTICLE_STEP * n * steps_per_launch if path == "slab" else None
     roof = {"bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s", "frac": achieved / peak,
-            "traffic": None, "kernel": kernel, "launch_ms": launch_ms * steps_per_launch,
+            "traffic": traffic, "traffic_bytes_per_particle_step": SLAB_HBM_BYTES_PER_PARTICLE_STEP
+            if path == "slab" else None, "kernel": kernel, "launch_ms": launch_ms * steps_per_launch,
             "kernel_ms_per_step": launch_ms, "steps_per_launch": steps_per_launch,
             "alg_bytes_per_particle_step": ALG_BYTES["step"]}
 
