@@ -95,9 +95,14 @@ __device__ inline void mz_diffuse_reflect(const MzRngParams &rp, MzStream &st, d
 // Free flight x + v dt (an explicit fma, as the reference's @muladd block)
 // with wall interactions while the particle is outside (0, L); the result is
 // clamped to [min_x, max_x].  A particle draws from stream (id - 1).
+// With pre_bc >= 0, (p0, p1, p2) is the velocity a first diffuse reflection
+// at wall pre_bc produces, computed ahead by mz_diffuse_reflect from the same
+// stream (its draws 0-2); later bounces continue the stream at draw 3, so the
+// result is the same as without it.
 __device__ inline void mz_convect_one(const MzRngParams &rp, const MzGrid &g, const MzWalls &wl, int sidx,
                                       double dt, int32_t id, double &x0, double &v0, double &v1, double &v2,
-                                      int32_t *err) {
+                                      int32_t *err, int pre_bc = -1, double p0 = 0.0, double p1 = 0.0,
+                                      double p2 = 0.0) {
     double t_rest = dt;
     double x_old = x0;
     double x_new = fma(v0, dt, x0);
@@ -121,7 +126,12 @@ __device__ inline void mz_convect_one(const MzRngParams &rp, const MzGrid &g, co
         if (acc == 0.0) {
             v0 = -v0;
         } else if (acc == 1.0) {
-            mz_diffuse_reflect(rp, st, v0, v1, v2, wl.vsq[bc][sidx], wn, wl.v[bc]);
+            if (bounces == 1 && bc == pre_bc) {
+                v0 = p0; v1 = p1; v2 = p2;
+                st = mz_stream((uint64_t)(id - 1), 3);
+            } else {
+                mz_diffuse_reflect(rp, st, v0, v1, v2, wl.vsq[bc][sidx], wn, wl.v[bc]);
+            }
         } else {
             double R = mz_rand(rp, st);
             if (R < acc) mz_diffuse_reflect(rp, st, v0, v1, v2, wl.vsq[bc][sidx], wn, wl.v[bc]);

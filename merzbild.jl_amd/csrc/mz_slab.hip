@@ -42,6 +42,8 @@
 #define SLAB_SPIN_LIMIT (1u << 22)    // polls per wait before the launch aborts
 #define SLAB_LINE 16                  // 64-bit words per flag (one 128-byte line)
 #define SLAB_DRAWS 64                 // NTC draws per cell computed ahead, in parallel
+#define SLAB_WALL_CAP 256             // wall-cell particles whose reflection is drawn ahead
+static_assert(SLAB_WALL_CAP <= SLAB_MB_CAP, "the wall hit list borrows an arrival-rank row");
 
 struct SlabArgs {
     MzSpView a;                 // input particles, PIA, counters
@@ -177,6 +179,9 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
     __shared__ int32_t arank[2][SLAB_MB_CAP], akey[2][SLAB_MB_CAP];
     __shared__ double sdraw[SLAB_MAXC][SLAB_DRAWS];
     __shared__ double ss0[PS], ss1[PS], ss2[PS];   // velocities as the last sort left them (props input)
+    constexpr int WC = SNAP ? SLAB_WALL_CAP : 1;
+    __shared__ double swl[3][WC];                  // reflected velocities drawn ahead (wall slab)
+    __shared__ int32_t swf[WC];                    // 1 where swl holds the entry
     __shared__ int32_t s_abort, s_n, s_na[2];
     __shared__ int64_t s_red[4][4];
 
@@ -224,6 +229,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
         sv0[i] = A.a.v0[p]; sv1[i] = A.a.v1[p]; sv2[i] = A.a.v2[p];
         sw[i] = A.a.w[p];
         sid[i] = A.a.id[p];
+        if (SNAP) { ss0[i] = sv0[i]; ss1[i] = sv1[i]; ss2[i] = sv2[i]; }
     }
     // NTC lanes: cell k runs on lane k / 4 of wave k % 4
     const int my_k = lane * 4 + wave;
@@ -272,6 +278,53 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
         }
     };
     auto props_due = [&](int32_t t) { return A.props_every > 0 && ((t + 1) % A.props_every) == 0; };
+    // A slab that ends at a fully diffuse wall (one wall only) draws, while
+    // NTC runs, the velocities that a first reflection gives the wall cell's
+    // particles headed for that wall.  The draws depend on particle id and
+    // step only, so they are the ones convection would draw; which particles
+    // get one is predicted from the velocities before NTC (a particle that a
+    // collision turns towards the wall draws in convection as before).  The
+    // waves that carry no NTC or props cell do it: waves 2-3 of a one-cell
+    // slab, wave 3 of a two-cell slab.
+    int wall_side = -1;
+    if (SNAP && has_left != has_right && ncl <= 2) {
+        const int s = has_left ? 1 : 0;
+        if (A.wl.acc[s] == 1.0) wall_side = s;
+    }
+    const int wall_k = wall_side == 1 ? ncl - 1 : 0;
+    const int wall_w0 = ncl == 1 ? 2 : 3;   // first free wave
+    // Each free wave first marks its share of the wall cell, listing the
+    // predicted hits in arank[wave & 1] (the arrival ranks are not in use
+    // during NTC), then draws them one per lane, so a wave spends one
+    // reflection's latency rather than one per entry it marked.
+    auto wall_draws = [&](uint64_t stp) {
+        MzRngParams rpw = A.rp;
+        rpw.tag = (stp << 4) | 2u;
+        const int32_t lo = cst[wall_k], n = min(ccn[wall_k], SLAB_WALL_CAP);
+        const int nthr = SLAB_THREADS - wall_w0 * 64;
+        int32_t *list = arank[wave & 1];
+        int nh = 0;
+        for (int j0 = (wave - wall_w0) * 64; j0 < n; j0 += nthr) {
+            const int j = j0 + lane;
+            bool hit = false;
+            if (j < n) {
+                const double xn = fma(ss0[lo + j], A.dt, sx0[lo + j]);
+                hit = wall_side == 0 ? (xn <= 0.0 && xn < A.g.L) : (xn >= A.g.L);
+                swf[j] = hit ? 1 : 0;
+            }
+            const uint64_t m = __ballot(hit);
+            if (hit) list[nh + __popcll(m & lt)] = j;
+            nh += __popcll(m);
+        }
+        const double wn = wall_side == 0 ? 1.0 : -1.0;
+        for (int h = lane; h < nh; h += 64) {
+            const int j = list[h];
+            MzStream sr = mz_stream((uint64_t)(sid[lo + j] - 1));
+            double u0, u1, u2;
+            mz_diffuse_reflect(rpw, sr, u0, u1, u2, A.wl.vsq[wall_side][0], wn, A.wl.v[wall_side]);
+            swl[0][j] = u0; swl[1][j] = u1; swl[2][j] = u2;
+        }
+    };
     if (A.n_steps > 0) fill_draws(A.step0, tid, SLAB_THREADS);
     __syncthreads();
 
@@ -292,6 +345,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
             slab_ntc_cell(A, rp, clo + my_k, my_V, cst[my_k], my_nla, sv0, sv1, sv2, sw, sdraw[my_k], my_sgwm,
                           my_ncoll, my_perf, my_eqw);
         }
+        if (wall_side >= 0 && wave >= wall_w0) wall_draws(step);
         __syncthreads();
         // ---- 2. convection, new cell, rank inside 64-particle chunks
         {
@@ -312,8 +366,17 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
                     if (x_new < A.g.L && x_new > 0.0) {
                         px0[r] = x_new < A.g.min_x ? A.g.min_x : (x_new > A.g.max_x ? A.g.max_x : x_new);
                     } else {
+                        double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+                        int pre_bc = -1;
+                        if (wall_side >= 0) {
+                            const int j = i - cst[wall_k];
+                            if (j >= 0 && j < min(ccn[wall_k], SLAB_WALL_CAP) && swf[j]) {
+                                p0 = swl[0][j]; p1 = swl[1][j]; p2 = swl[2][j];
+                                pre_bc = wall_side;
+                            }
+                        }
                         mz_convect_one(rp, A.g, A.wl, 0, A.dt, pid[r], px0[r], pv0[r], pv1[r], pv2[r],
-                                       &A.a.ctr->err);
+                                       &A.a.ctr->err, pre_bc, p0, p1, p2);
                     }
                     const int c = mz_cell_of(px0[r], A.g, A.n_cells, &A.a.ctr->err) - 1;
                     pcell[r] = c + 1;
