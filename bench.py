@@ -255,7 +255,9 @@ def _abs_moments(w, v, mim):
     return np.array([np.sum(w * np.abs(v[:, 0] ** a * v[:, 1] ** b * v[:, 2] ** c)) for (a, b, c) in mim])
 
 
-def run_nnls(a, pkg):
+def sample_nnls_cells():
+    """The batched NNLS workload on the host: counts, positions, velocities,
+    weights of all cells (Philox seed 5)."""
     g = np.random.Generator(np.random.Philox(5))
     counts = g.integers(NN_NMIN, NN_NMAX + 1, size=NN_CELLS)
     n = int(counts.sum())
@@ -275,6 +277,14 @@ def run_nnls(a, pkg):
     w = g.lognormal(0.0, 0.5, size=n)
     x = g.uniform(size=(n, 3))
     starts = np.concatenate(([0], np.cumsum(counts)[:-1]))
+    return dict(counts=counts, starts=starts, n=n, x=x, v=v, w=w)
+
+
+def nnls_merge_once(pkg, cs, max_pcols, method, n_check=256):
+    """Upload the cells, merge every cell once (timed by the library's event
+    timer around the merge kernels), then the largest relative moment error
+    |M_after - M_before| / sum w |v^k| over a sample of merged cells."""
+    n, counts, starts = cs["n"], cs["counts"], cs["starts"]
     pia = np.zeros((NN_CELLS, 7), dtype=np.int64)
     pia[:, 0] = counts
     pia[:, 1] = starts + 1
@@ -282,24 +292,29 @@ def run_nnls(a, pkg):
     pia[:, 3] = counts
     pia[:, 5] = -1
     st = pkg.DeviceStore(NN_CELLS, 1, n + 1024)
-    st.upload_species(1, x, v, w, np.arange(1, n + 1, dtype=np.int64), pia, n, True, np.zeros(0, dtype=np.int64), n)
+    st.upload_species(1, cs["x"], cs["v"], cs["w"], np.arange(1, n + 1, dtype=np.int64), pia, n, True,
+                      np.zeros(0, dtype=np.int64), n)
     mim = _mim(4)
-    # a warmup merge on a copy would change the state; the timed call is the first
     st.sync()
     st.timing_reset()
     st.timing(True)
     t0 = time.perf_counter()
-    status = pkg.merge_nnls_based(st, 1, 1, mim, scaling="variance", squash=False)
+    if max_pcols is None:
+        status = pkg.merge_nnls_based(st, 1, 1, mim, scaling="variance", squash=False)
+    else:
+        status = pkg.merge_nnls_candidates(st, 1, 1, mim, max_pcols, method=method, scaling="variance",
+                                           squash=False)
     st.sync()
     el = time.perf_counter() - t0
     ms, _ = st.timing_get("merge_nnls")
     st.timing(False)
     merged = int(np.sum(status == 1))
-    # moment error on a sample of cells, in FP64 on the host: |M_after - M_before| / sum w |v^k|
     got = st.download_species(1)
-    sample = np.linspace(0, NN_CELLS - 1, 256).astype(int)
+    st.close()
+    sample = np.linspace(0, NN_CELLS - 1, n_check).astype(int)
     full = _mim(4) + [(0, 0, 0)]
     worst = 0.0
+    w, v = cs["w"], cs["v"]
     for c in sample:
         if status[c] != 1:
             continue
@@ -307,20 +322,37 @@ def run_nnls(a, pkg):
         pre = _moments(w[a0:a1], v[a0:a1], full)
         scale = _abs_moments(w[a0:a1], v[a0:a1], full)
         r = got["pia"][c]
-        post_w = got["w"][r[1] - 1:r[2]]
-        post_v = got["v"][r[1] - 1:r[2]]
-        post = _moments(post_w, post_v, full)
+        post = _moments(got["w"][r[1] - 1:r[2]], got["v"][r[1] - 1:r[2]], full)
         worst = max(worst, float(np.max(np.abs(post - pre) / scale)))
-    out = {"metric": "NNLS merges/s at <= 1e-10 moment error", "value": merged / (ms * 1e-3),
+    return dict(merges_per_s=merged / (ms * 1e-3), kernel_ms=ms, wall_s=el, merged=merged,
+                failed=int(np.sum(status == -1)), max_rel_moment_error_sampled=worst)
+
+
+# The candidate set: the first 239 particles of a cell plus the reference's
+# 17 fictitious particles (centre, 8 octants at 0.5 and 1.0 sigma), 256
+# columns.  64 columns (47 particles) leave the 35-moment system infeasible
+# (the residual stays near 10% of |b|), so those merges fail.
+NN_PCOLS = 239
+
+
+def run_nnls(a, pkg):
+    cs = sample_nnls_cells()
+    lds = nnls_merge_once(pkg, cs, NN_PCOLS, "lds")
+    ref = nnls_merge_once(pkg, cs, NN_PCOLS, "reference")
+    k64 = nnls_merge_once(pkg, cs, 47, "lds", n_check=0)
+    out = {"metric": "NNLS merges/s at <= 1e-10 moment error", "value": lds["merges_per_s"],
            "unit": "merges/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
-           "wall_s": el, "kernel_ms": ms,
+           "wall_s": lds["wall_s"], "kernel_ms": lds["kernel_ms"], "vs_baseline": None,
            "config": {"workload": "32768 cells x U{200..3000} bi-Maxwellian particles, lognormal(0.5) weights; "
-                                  "merge_nnls_based! with base + order <= 4 multi-index moments, :variance",
-                      "n_particles": n, "merged": merged, "failed": int(np.sum(status == -1)),
-                      "max_rel_moment_error_sampled": worst,
-                      "rows": len(set(mim) | {(0, 0, 0), (2, 0, 0), (0, 2, 0), (0, 0, 2)})}}
+                                  "merge_nnls_based! with base + order <= 4 multi-index moments (35 rows), "
+                                  ":variance, columns = first 239 particles + 17 fictitious",
+                      "n_particles": cs["n"], "max_pcols": NN_PCOLS, "method": "lds",
+                      "merged": lds["merged"], "failed": lds["failed"],
+                      "max_rel_moment_error_sampled": lds["max_rel_moment_error_sampled"],
+                      "rows": len(set(_mim(4)) | {(0, 0, 0)})},
+           "reference_layout_fp64": ref, "speedup_vs_reference_layout": lds["merges_per_s"] / ref["merges_per_s"],
+           "k64_columns": {"merged": k64["merged"], "failed": k64["failed"]}}
     print(json.dumps(out))
-    st.close()
 
 
 def main():

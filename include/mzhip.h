@@ -143,6 +143,21 @@ int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, const int32_t
                   int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
                   int32_t squash, int32_t *status);
 
+/* merge_nnls_based! with a bounded candidate set: the columns are the first
+ * max_pcols particles of the cell (logical order) plus the reference's
+ * fictitious particles, while the moments to match are still those of all
+ * the cell's particles (src/merging/merging_nnls.jl:1457-1510 with the column
+ * set of compute_lhs_and_rhs!, :509-613, cut at max_pcols; max_pcols < 0 is
+ * mz_merge_nnls).  method 0 runs the reference's column-by-column layout
+ * (matrix in device memory, results equal to the CPU restatement bit for
+ * bit); method 1 keeps the matrix in LDS with one thread per column and
+ * block-parallel moment sums (needs max_pcols + fictitious columns <= 256,
+ * results equal to rounding). */
+int mz_merge_nnls_cand(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim, int32_t n_mim,
+                       double vref, int32_t scaling_vref, int32_t centered_at_mean, const double *v_multipliers,
+                       int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
+                       int32_t squash, int32_t max_pcols, int32_t method, int32_t *status);
+
 /* ----------------------------------------------------------- time stepping */
 /* One 1-D Couette step repeated n_steps times: ntc! in every cell,
  * convection, sort, compute_props_sorted! every props_every steps

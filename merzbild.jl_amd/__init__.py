@@ -56,6 +56,8 @@ _SIGS = {
                                   P(i64)]),
     "mz_merge_nnls": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, f64, i32, i32, P(f64), i32, f64, i32, f64,
                                 i32, P(i32)]),
+    "mz_merge_nnls_cand": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, f64, i32, i32, P(f64), i32, f64, i32,
+                                     f64, i32, i32, i32, P(i32)]),
     "mz_couette_steps": (C.c_int, [C.c_void_p, i32, u64, f64, P(f64), i32, i32, P(f64), i32]),
     "mz_couette_last_path": (C.c_int, [C.c_void_p]),
     "mz_set_slab_shape": (C.c_int, [C.c_void_p, i32, i32]),
@@ -510,6 +512,28 @@ def merge_nnls_based(store, species, threshold, multi_index_moments, vref=1.0, s
                                         1 if scaling == "vref" else 0, 1 if centered_at_mean else 0, _d(vm),
                                         len(vm), max_err, iteration_mult, w_threshold, 1 if squash else 0,
                                         status.ctypes.data_as(P(i32))))
+    return status
+
+
+NNLS_METHODS = {"reference": 0, "lds": 1}
+
+
+def merge_nnls_candidates(store, species, threshold, multi_index_moments, max_pcols, method="lds", vref=1.0,
+                          scaling="variance", centered_at_mean=True, v_multipliers=(0.5, 1.0), max_err=1e-11,
+                          iteration_mult=2, w_threshold=0.0, squash=True):
+    """merge_nnls_based! with the columns cut to the first max_pcols particles
+    plus the fictitious ones; the moments matched are those of all the cell's
+    particles.  method "reference" keeps the reference's layout (bit-exact
+    with the CPU restatement), "lds" runs one workgroup per cell with the
+    matrix in LDS.  Returns the per-cell status as merge_nnls_based."""
+    mim = np.ascontiguousarray(np.asarray(multi_index_moments, dtype=np.int32).reshape(-1, 3))
+    vm = np.ascontiguousarray(v_multipliers, dtype=np.float64)
+    status = np.zeros(store.n_cells, dtype=np.int32)
+    _check(load_library().mz_merge_nnls_cand(store.h, species, threshold, mim.ctypes.data_as(P(i32)), len(mim),
+                                             vref, 1 if scaling == "vref" else 0, 1 if centered_at_mean else 0,
+                                             _d(vm), len(vm), max_err, iteration_mult, w_threshold,
+                                             1 if squash else 0, max_pcols, NNLS_METHODS[method],
+                                             status.ctypes.data_as(P(i32))))
     return status
 
 

@@ -170,6 +170,7 @@ static int batch0d_launch(mz_store *h, int species, int32_t n_steps, uint64_t st
     const int nc = h->n_cells;
     // the queue: cells by decreasing occupancy, packed into items
     std::vector<int32_t> nloc(nc);
+    MZ_HIP(hipStreamSynchronize(h->stream));   // the handle's stream does not synchronise with hipMemcpy
     MZ_HIP(hipMemcpy(nloc.data(), s.pia.nloc, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
     std::vector<int32_t> order(nc);
     std::iota(order.begin(), order.end(), 0);

@@ -564,7 +564,13 @@ struct NnlsArgs {
     int32_t *status;               // [n_cells]
     int32_t *old_pia;              // [n_cells][5]
     int32_t n_cells;
+    int32_t max_pcols;             // particle columns: the first max_pcols (< 0: all)
 };
+
+// Particle columns of a cell with N particles.
+__device__ inline int32_t nnls_pcols(const NnlsArgs &A, int32_t N) {
+    return (A.max_pcols >= 0 && A.max_pcols < N) ? A.max_pcols : N;
+}
 
 __device__ inline double nnls_ipow(double a, int p) {
     double r = 1.0;
@@ -604,9 +610,10 @@ __global__ void __launch_bounds__(256) k_nnls_merge(NnlsArgs A) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int32_t s1 = A.sp.pia.start1[c], n1 = A.sp.pia.n1[c], s2 = A.sp.pia.start2[c], n2 = A.sp.pia.n2[c];
     const int32_t N = n1 + n2;
+    const int32_t Np = nnls_pcols(A, N);                // particle columns
     const int m = A.m;
     const int n_add = (A.centered ? 1 : 0) + 8 * A.n_vmult;
-    const int n = N + n_add;
+    const int n = Np + n_add;
     double *Am = A.scratch + A.aoff[blockIdx.x];       // [m][n] row-major
     double *xs = Am + (int64_t)m * n;                   // x[n]
     double *ws = xs + n;                                // w[n] (dual)
@@ -686,11 +693,11 @@ __global__ void __launch_bounds__(256) k_nnls_merge(NnlsArgs A) {
     for (int j = tid; j < n; j += nt) {
         double cx, cy, cz;
         bool zero_col = false;
-        if (j < N) {
+        if (j < Np) {
             int64_t p = oct_slot(j, s1, n1, s2);
             cx = A.sp.v0[p] - v0x; cy = A.sp.v1[p] - v0y; cz = A.sp.v2[p] - v0z;
         } else {
-            int f = j - N;
+            int f = j - Np;
             if (A.centered && f == 0) {
                 zero_col = true;
                 cx = cy = cz = 0.0;
@@ -1025,11 +1032,11 @@ __global__ void __launch_bounds__(256) k_nnls_merge(NnlsArgs A) {
                 const int64_t dst = oct_slot(cur, s1, n1, s2);
                 ++cur;
                 double vx, vy, vz;
-                if (j < N) {
+                if (j < Np) {
                     int64_t p = oct_slot(j, s1, n1, s2);
                     vx = A.sp.v0[p]; vy = A.sp.v1[p]; vz = A.sp.v2[p];
                 } else {
-                    int f = j - N;
+                    int f = j - Np;
                     if (A.centered && f == 0) {
                         vx = v0x; vy = v0y; vz = v0z;
                     } else {
@@ -1084,13 +1091,567 @@ __global__ void __launch_bounds__(256) k_nnls_merge(NnlsArgs A) {
 #undef AM
 }
 
+
+// ===========================================================================
+// Candidate NNLS merging, one workgroup per cell with thread j owning column
+// j: the same Lawson-Hanson iteration as k_nnls_merge (nnls.jl:273-548) on a
+// column set of at most NF_COLS (the first particles plus the fictitious
+// ones), with the scaled moment matrix resident in LDS.  The moments and the
+// right-hand side are block reductions over the particles (thread-strided
+// partial sums combined in a fixed order), and the steps the reference runs
+// as loops over rows -- building and applying the Householder vector, the
+// back substitution, the step length of the inner loop, the feasibility scan
+// -- run on wave 0 with one lane per row.  Results agree with the oracle to
+// rounding (sums are taken in another order), not bit for bit.
+#define NF_COLS 256
+
+__device__ inline double nf_readlane(double v, int l) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ inline double nf_wsum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double nf_wmax(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline double nf_wmin(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// The reference's (fictitious) column f >= 0 after the particles: the
+// centre particle (all-zero column) and the octant particles
+// (compute_lhs_particles_additional!, merging_nnls.jl:949-1038), relative to
+// the mean velocity.
+__device__ inline bool nf_fict(const NnlsArgs &A, int f, const double *e3, const double *mnv, const double *mxv,
+                               double &cx, double &cy, double &cz) {
+    if (A.centered && f == 0) {
+        cx = cy = cz = 0.0;
+        return true;
+    }
+    f -= A.centered ? 1 : 0;
+    const int vm = f / 8, o = f % 8 + 1;
+    const double mult = A.vmult[vm];
+    const double s3[3] = {(o % 2 == 1) ? -1.0 : 1.0, (o == 3 || o == 4 || o == 7 || o == 8) ? 1.0 : -1.0,
+                          o >= 5 ? 1.0 : -1.0};
+    double r3[3];
+    for (int d = 0; d < 3; ++d) {
+        const double ms = mult * (s3[d] * e3[d]);
+        r3[d] = ms > mxv[d] ? mxv[d] * mult : (ms < mnv[d] ? mnv[d] * mult : ms);
+    }
+    cx = r3[0]; cy = r3[1]; cz = r3[2];
+    return false;
+}
+
+template <int ROWS>
+__global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsArgs A) {
+    __shared__ double Al[ROWS * NF_COLS];        // scaled moment matrix [row][column]
+    __shared__ double bq[ROWS], zz[ROWS];
+    __shared__ double xs[NF_COLS], ws[NF_COLS];
+    __shared__ int32_t idx[NF_COLS];
+    __shared__ double red[4][ROWS + 8];
+    __shared__ double sd[24];
+    __shared__ int32_t si[16];
+    __shared__ double rw[4];
+    __shared__ int ri[4];
+    const int c = A.cells[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int32_t s1 = A.sp.pia.start1[c], n1 = A.sp.pia.n1[c], s2 = A.sp.pia.start2[c], n2 = A.sp.pia.n2[c];
+    const int32_t N = n1 + n2;
+    const int32_t Np = nnls_pcols(A, N);
+    const int m = A.m;
+    const int n_add = (A.centered ? 1 : 0) + 8 * A.n_vmult;
+    const int n = Np + n_add;
+#define AL(r, cc) Al[((r) - 1) * NF_COLS + ((cc) - 1)]
+    // ---- compute_w_total_v0!: W, mean velocity and position, velocity bounds
+    {
+        double W = 0.0, sv[3] = {0, 0, 0}, sx[3] = {0, 0, 0};
+        double mn[3] = {299792458.0, 299792458.0, 299792458.0};
+        double mx[3] = {-299792458.0, -299792458.0, -299792458.0};
+        for (int li = tid; li < N; li += NF_COLS) {
+            const int64_t p = oct_slot(li, s1, n1, s2);
+            const double w = A.sp.w[p], v[3] = {A.sp.v0[p], A.sp.v1[p], A.sp.v2[p]};
+            const double xx[3] = {A.sp.x0[p], A.sp.x1[p], A.sp.x2[p]};
+            W += w;
+            for (int d = 0; d < 3; ++d) {
+                sv[d] += w * v[d];
+                sx[d] += w * xx[d];
+                mn[d] = fmin(mn[d], v[d]);
+                mx[d] = fmax(mx[d], v[d]);
+            }
+        }
+        double t[13] = {W, sv[0], sv[1], sv[2], sx[0], sx[1], sx[2], mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]};
+#pragma unroll
+        for (int k = 0; k < 13; ++k) {
+            const double r = k < 7 ? nf_wsum(t[k]) : (k < 10 ? nf_wmin(t[k]) : nf_wmax(t[k]));
+            if (lane == 0) red[wave][k] = r;
+        }
+        __syncthreads();
+        if (tid < 13) {
+            const int k = tid;
+            double r;
+            if (k < 7) r = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+            else if (k < 10) r = fmin(fmin(red[0][k], red[1][k]), fmin(red[2][k], red[3][k]));
+            else r = fmax(fmax(red[0][k], red[1][k]), fmax(red[2][k], red[3][k]));
+            sd[k] = r;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const double Wt = sd[0];
+            for (int d = 0; d < 3; ++d) {
+                const double v0 = sd[1 + d] / Wt, x0 = sd[4 + d] / Wt;
+                const double lo = sd[7 + d] - v0, hi = sd[10 + d] - v0;
+                sd[1 + d] = v0;
+                sd[4 + d] = x0;
+                sd[7 + d] = lo;
+                sd[10 + d] = hi;
+            }
+        }
+        __syncthreads();
+    }
+    const double W = sd[0];
+    const double v0x = sd[1], v0y = sd[2], v0z = sd[3];
+    // ---- variances and right-hand side (compute_lhs_and_rhs!, :509-613)
+    {
+        double acc[ROWS], e[3] = {0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) acc[q] = 0.0;
+        for (int li = tid; li < N; li += NF_COLS) {
+            const int64_t p = oct_slot(li, s1, n1, s2);
+            const double w = A.sp.w[p];
+            const double cx = A.sp.v0[p] - v0x, cy = A.sp.v1[p] - v0y, cz = A.sp.v2[p] - v0z;
+            e[0] += w * (cx * cx);
+            e[1] += w * (cy * cy);
+            e[2] += w * (cz * cz);
+#pragma unroll
+            for (int q = 0; q < ROWS; ++q) {
+                if (q < m)
+                    acc[q] += w * (nnls_ipow(cx, A.mim[q][0]) * nnls_ipow(cy, A.mim[q][1]) * nnls_ipow(cz, A.mim[q][2]));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double r = nf_wsum(e[k]);
+            if (lane == 0) red[wave][k] = r;
+        }
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+            if (q < m) {
+                const double r = nf_wsum(acc[q]);
+                if (lane == 0) red[wave][3 + q] = r;
+            }
+        }
+        __syncthreads();
+        if (tid < 3) {
+            const double E = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+            sd[13 + tid] = sqrt(E / W);
+        }
+        if (tid >= 32 && tid < 32 + m) {
+            const int q = tid - 32;
+            zz[q] = (((red[0][3 + q] + red[1][3 + q]) + red[2][3 + q]) + red[3][3 + q]) / W;
+        }
+        __syncthreads();
+    }
+    const double Ex = sd[13], Ey = sd[14], Ez = sd[15];
+    // ---- row scaling (scale_lhs_rhs_variance! / _vref!, :1178-1215); zz holds the moments
+    __shared__ double rowscale[ROWS];
+    if (tid < m) {
+        double rv;
+        if (!A.scaling_vref) {
+            const double iex = Ex > 1e-6 ? 1.0 / Ex : A.inv_vref;
+            const double iey = Ey > 1e-6 ? 1.0 / Ey : A.inv_vref;
+            const double iez = Ez > 1e-6 ? 1.0 / Ez : A.inv_vref;
+            rv = nnls_ipow(iex, A.mim[tid][0]) * nnls_ipow(iey, A.mim[tid][1]) * nnls_ipow(iez, A.mim[tid][2]);
+        } else {
+            rv = nnls_ipow(A.inv_vref, A.tot_order[tid]);
+        }
+        rowscale[tid] = rv;
+        bq[tid] = zz[tid] * rv;
+    }
+    __syncthreads();
+    // ---- column j on thread j: moments of the candidate, row-scaled, unit norm
+    // (scale_columns!, utils.jl:62-77); the candidate's velocity stays in
+    // registers for the post-merge write
+    double cvx = 0.0, cvy = 0.0, cvz = 0.0, cinv = 1.0;
+    if (tid < n) {
+        const int j = tid;
+        double cx, cy, cz;
+        bool zero_col = false;
+        if (j < Np) {
+            const int64_t p = oct_slot(j, s1, n1, s2);
+            cvx = A.sp.v0[p]; cvy = A.sp.v1[p]; cvz = A.sp.v2[p];
+            cx = cvx - v0x; cy = cvy - v0y; cz = cvz - v0z;
+        } else {
+            const double e3[3] = {Ex, Ey, Ez};
+            const double mnv[3] = {sd[7], sd[8], sd[9]}, mxv[3] = {sd[10], sd[11], sd[12]};
+            zero_col = nf_fict(A, j - Np, e3, mnv, mxv, cx, cy, cz);
+            cvx = v0x + cx; cvy = v0y + cy; cvz = v0z + cz;
+        }
+        double nn = 0.0;
+        for (int q = 0; q < m; ++q) {
+            double t = zero_col ? 0.0
+                                : nnls_ipow(cx, A.mim[q][0]) * nnls_ipow(cy, A.mim[q][1]) * nnls_ipow(cz, A.mim[q][2]);
+            t *= rowscale[q];
+            AL(q + 1, j + 1) = t;
+            nn = nn + t * t;
+        }
+        nn = sqrt(nn);
+        nn = nn > 1e-15 ? nn : 1.0;
+        cinv = 1.0 / nn;
+        for (int q = 0; q < m; ++q) AL(q + 1, j + 1) *= cinv;
+        xs[j] = 0.0;
+        ws[j] = 0.0;
+        idx[j] = j + 1;
+    }
+
+    // ---- Lawson-Hanson solve!(work, iteration_mult * n, 1e-14)
+    const double kkt_tol = 1e-14, factor = 0.01;
+    const int64_t max_iter = (int64_t)A.iteration_mult * n;
+    // shared state: si 0 iz1, 1 iz2, 2 nsetp, 3 j, 4 jj, 5 iz, 7 mode, 8 flag, 9 i, 11 allfeasible, 12 rotate
+    // sd 16 up, 17 iteration count, 18/19 Givens c/s
+    if (tid == 0) {
+        si[0] = 1; si[1] = n; si[2] = 0; si[3] = 0; si[4] = 0; si[5] = 0;
+        si[7] = 1; si[8] = 0; si[9] = 0;
+        sd[16] = 0.0;
+        sd[17] = 0.0;
+    }
+    __syncthreads();
+    // back substitution (solve_triangular_system!) on wave 0, lane r = row r + 1:
+    // zz <- R^-1 zz over the nsetp columns of P
+    auto back_subst = [&]() {
+        const int nsetp = si[2];
+        int jj = si[4];
+        double zr = lane < m ? zz[lane] : 0.0;
+        for (int l = 1; l <= nsetp; ++l) {
+            const int ip = nsetp + 1 - l;
+            if (l != 1) {
+                const double zip = nf_readlane(zr, ip);
+                if (lane < ip) zr = zr - AL(lane + 1, jj) * zip;
+            }
+            jj = idx[ip - 1];
+            if (lane == ip - 1) zr = zr / AL(ip, jj);
+        }
+        if (lane < m) zz[lane] = zr;
+        if (lane == 0) si[4] = jj;
+    };
+    for (;;) {
+        if (si[0] > si[1] || si[2] >= m) break;
+        // dual vector over the zero set
+        {
+            const int iz1 = si[0], iz2 = si[1], nsetp = si[2];
+            for (int i = iz1 + tid; i <= iz2; i += NF_COLS) {
+                const int idxi = idx[i - 1];
+                double sm = 0.0;
+                for (int l = nsetp + 1; l <= m; ++l) sm = sm + AL(l, idxi) * bq[l - 1];
+                ws[idxi - 1] = sm;
+            }
+        }
+        __syncthreads();
+        bool terminated = false;
+        for (;;) {
+            // largest positive dual (first maximum in idx order)
+            const int iz1 = si[0], iz2 = si[1];
+            double wmax = 0.0;
+            int izmax = -1;
+            for (int i = iz1 + tid; i <= iz2; i += NF_COLS) {
+                const double wv = ws[idx[i - 1] - 1];
+                if (wv > wmax || (izmax < 0 && wv > 0.0)) {
+                    if (izmax < 0 || wv > wmax) { wmax = wv; izmax = i; }
+                }
+            }
+            double bw;
+            int bi;
+            nnls_argmax(izmax >= 0 ? wmax : -1.0, izmax, rw, ri, bw, bi);
+            if (bi < 0 || bw <= kkt_tol) { terminated = true; break; }
+            if (wave == 0) {
+                // construct_householder! on rows nsetp+1..m of column j, the
+                // column test and apply_householder! to a copy of b
+                const int nsetp = si[2];
+                const int j = idx[bi - 1];
+                const int len = m - nsetp;
+                const double a = lane < m ? AL(lane + 1, j) : 0.0;
+                const bool hh = lane >= nsetp && lane < m;
+                const double asave = nf_readlane(a, nsetp);
+                double up = sd[16], top = asave;
+                if (len > 1) {
+                    double cl = nf_wmax(hh ? fabs(a) : 0.0);
+                    const double clinv = 1.0 / cl;
+                    const double t = hh ? a * clinv : 0.0;
+                    cl *= sqrt(nf_wsum(t * t));
+                    if (asave > 0) cl = -cl;
+                    up = asave - cl;
+                    top = cl;
+                }
+                const double unorm = sqrt(nf_wsum(lane < nsetp ? a * a : 0.0));
+                int accept = 0;
+                double zr = lane < m ? bq[lane] : 0.0;
+                if (((unorm + fabs(top) * factor) - unorm) > 0) {
+                    if (len > 1) {
+                        double bb = up * top;
+                        if (bb < 0) {
+                            bb = 1.0 / bb;
+                            const double ur = lane == nsetp ? up : (hh ? a : 0.0);
+                            double sm = nf_wsum(zr * ur);
+                            if (sm != 0) {
+                                sm *= bb;
+                                if (hh) zr = zr + sm * ur;
+                            }
+                        }
+                    }
+                    const double ztest = nf_readlane(zr, nsetp) / top;
+                    if (ztest > 0) accept = 1;
+                }
+                if (accept) {
+                    if (lane < m) zz[lane] = zr;
+                    if (lane == nsetp) AL(nsetp + 1, j) = top;
+                } else if (lane == 0) {
+                    ws[j - 1] = 0;
+                }
+                if (lane == 0) {
+                    sd[16] = up;
+                    si[8] = accept;
+                    si[5] = bi;
+                    si[3] = j;
+                }
+            }
+            __syncthreads();
+            if (si[8]) break;
+        }
+        if (terminated) break;
+        // move j from Z to P, transform the remaining columns
+        if (tid == 0) {
+            const int iz = si[5], iz1 = si[0], j = si[3];
+            idx[iz - 1] = idx[iz1 - 1];
+            idx[iz1 - 1] = j;
+            si[0] = iz1 + 1;
+            si[2] = si[2] + 1;
+        }
+        if (tid < m) bq[tid] = zz[tid];
+        __syncthreads();
+        {
+            const int iz1 = si[0], iz2 = si[1], nsetp = si[2], j = si[3];
+            const double up = sd[16];
+            const int len = m - nsetp + 1;
+            for (int jz = iz1 + tid; jz <= iz2; jz += NF_COLS) {
+                const int jj = idx[jz - 1];
+                if (len > 1) {
+                    double bb = up * AL(nsetp, j);
+                    if (bb < 0) {
+                        bb = 1.0 / bb;
+                        double sm = AL(nsetp, jj) * up;
+                        for (int l = 2; l <= len; ++l) sm = sm + AL(nsetp + l - 1, jj) * AL(nsetp + l - 1, j);
+                        if (sm != 0) {
+                            sm *= bb;
+                            AL(nsetp, jj) = AL(nsetp, jj) + sm * up;
+                            for (int l = 2; l <= len; ++l)
+                                AL(nsetp + l - 1, jj) = AL(nsetp + l - 1, jj) + sm * AL(nsetp + l - 1, j);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        {
+            const int nsetp = si[2], j = si[3];
+            if (tid > nsetp && tid <= m && nsetp != m) AL(tid, j) = 0;
+            if (tid == 0) ws[j - 1] = 0;
+        }
+        __syncthreads();
+        if (wave == 0) back_subst();
+        __syncthreads();
+        // secondary loop
+        for (;;) {
+            if (wave == 0) {
+                const int nsetp = si[2];
+                const double iter = sd[17] + 1.0;
+                int flag = 0;   // 1: leave the secondary loop, 2: iteration cap
+                if (iter > (double)max_iter) {
+                    flag = 2;
+                } else {
+                    // alpha = the first minimum of -x / (z - x) over the
+                    // non-positive z in P order, if below 2
+                    double t = 2.0;
+                    int l = 0;
+                    if (lane < nsetp) {
+                        l = idx[lane];
+                        const double z = zz[lane];
+                        if (z <= 0) t = -xs[l - 1] / (z - xs[l - 1]);
+                    }
+                    const bool cand = t < 2.0;
+                    double tv = cand ? t : 2.0;
+                    const double alpha = nf_wmin(tv);
+                    const uint64_t hit = __ballot(cand && t == alpha);
+                    if (!hit) {
+                        flag = 1;
+                    } else {
+                        const int jmin = __ffsll((unsigned long long)hit) - 1;   // 0-based ip - 1
+                        if (lane < nsetp) xs[l - 1] = xs[l - 1] + alpha * (zz[lane] - xs[l - 1]);
+                        if (lane == 0) {
+                            si[4] = jmin + 1;
+                            si[9] = idx[jmin];
+                        }
+                    }
+                }
+                if (lane == 0) {
+                    sd[17] = iter;
+                    si[8] = flag;
+                    if (flag == 2) si[7] = 3;
+                }
+            }
+            __syncthreads();
+            if (si[8] == 2) { terminated = true; break; }
+            if (si[8] == 1) break;
+            // move coefficient i from P to Z (Givens rotations), repeat while
+            // some remaining coefficient is not positive
+            for (;;) {
+                if (tid == 0) {
+                    xs[si[9] - 1] = 0;
+                    si[12] = si[4] != si[2];
+                    if (si[12]) si[4] += 1;   // jj += 1 persists across passes, as in nnls.jl:449-451
+                }
+                __syncthreads();
+                if (si[12]) {
+                    const int jj0 = si[4], nsetp = si[2];
+                    for (int jx = jj0; jx <= nsetp; ++jx) {
+                        if (tid == 0) {
+                            const int ii = idx[jx - 1];
+                            idx[jx - 2] = ii;
+                            const double a = AL(jx - 1, ii), bqv = AL(jx, ii);
+                            double cc, ss, sig;
+                            if (fabs(a) > fabs(bqv)) {
+                                const double xr = bqv / a, yr = sqrt(1 + xr * xr);
+                                cc = (1 / yr) * (a > 0 ? 1.0 : (a < 0 ? -1.0 : 0.0));
+                                ss = cc * xr;
+                                sig = fabs(a) * yr;
+                            } else if (bqv != 0) {
+                                const double xr = a / bqv, yr = sqrt(1 + xr * xr);
+                                ss = (1 / yr) * (bqv > 0 ? 1.0 : -1.0);
+                                cc = ss * xr;
+                                sig = fabs(bqv) * yr;
+                            } else {
+                                sig = 0.0; cc = 0.0; ss = 1.0;
+                            }
+                            AL(jx - 1, ii) = sig;
+                            AL(jx, ii) = 0;
+                            sd[18] = cc; sd[19] = ss;
+                            si[10] = ii;
+                            const double temp = bq[jx - 2];
+                            bq[jx - 2] = cc * temp + ss * bq[jx - 1];
+                            bq[jx - 1] = -ss * temp + cc * bq[jx - 1];
+                        }
+                        __syncthreads();
+                        const double cc = sd[18], ss = sd[19];
+                        const int ii = si[10];
+                        for (int l = 1 + tid; l <= n; l += NF_COLS) {
+                            if (l != ii) {
+                                const double temp = AL(jx - 1, l);
+                                AL(jx - 1, l) = cc * temp + ss * AL(jx, l);
+                                AL(jx, l) = -ss * temp + cc * AL(jx, l);
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
+                if (wave == 0) {
+                    // nsetp -= 1, iz1 -= 1, the removed index back to Z; then
+                    // the first coefficient of P that is not positive, if any
+                    const int nsetp = si[2] - 1, iz1 = si[0] - 1;
+                    const int32_t i0 = si[9];
+                    const bool bad = lane < nsetp && xs[idx[lane] - 1] <= 0;
+                    const uint64_t badm = __ballot(bad);
+                    if (lane == 0) {
+                        si[2] = nsetp;
+                        si[0] = iz1;
+                        idx[iz1 - 1] = i0;
+                        si[11] = badm ? 0 : 1;
+                        if (badm) si[9] = idx[__ffsll((unsigned long long)badm) - 1];
+                        else if (nsetp >= 1) si[9] = idx[nsetp - 1];
+                    }
+                }
+                __syncthreads();
+                if (si[11]) break;
+            }
+            if (tid < m) zz[tid] = bq[tid];
+            __syncthreads();
+            if (wave == 0) back_subst();
+            __syncthreads();
+        }
+        if (terminated) break;
+        if (tid < si[2]) xs[idx[tid] - 1] = zz[tid];
+        __syncthreads();
+    }
+    // ---- compute_post_merge_particles_nnls! (:1334-1404)
+    __shared__ int32_t wcnt[4];
+    __shared__ double wsum[4];
+    if (wave == 0) {
+        const int nsetp = si[2];
+        const double r = (lane >= nsetp && lane < m) ? bq[lane] : 0.0;
+        const double rnorm = sqrt(nf_wsum(r * r));
+        if (lane == 0) si[13] = !(rnorm > A.max_err);
+    }
+    __syncthreads();
+    int ok = si[13];
+    const double xj = tid < n ? xs[tid] * cinv : 0.0;
+    const bool keep = ok && tid < n && xj > A.w_threshold;
+    const uint64_t km = __ballot(keep);
+    {
+        const double ksum = nf_wsum(keep ? xj : 0.0);
+        if (lane == 0) { wcnt[wave] = __popcll(km); wsum[wave] = ksum; }
+    }
+    __syncthreads();
+    const int nonzero = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+    const double ndw = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+    if (nonzero >= N) ok = 0;
+    const int32_t n_new = ok ? nonzero : N;
+    if (ok) {
+        int base = 0;
+        for (int k = 0; k < wave; ++k) base += wcnt[k];
+        if (keep) {
+            const int cur = base + __popcll(km & mz_lanemask_lt());
+            const int64_t dst = oct_slot(cur, s1, n1, s2);
+            A.sp.w[dst] = xj * (W / ndw);
+            A.sp.v0[dst] = cvx; A.sp.v1[dst] = cvy; A.sp.v2[dst] = cvz;
+            A.sp.x0[dst] = sd[4]; A.sp.x1[dst] = sd[5]; A.sp.x2[dst] = sd[6];
+        }
+        for (int li = n_new + tid; li < N; li += NF_COLS) A.sp.w[oct_slot(li, s1, n1, s2)] = 0.0;
+    }
+    if (tid == 0) {
+        A.status[c] = ok ? 1 : -1;
+        int32_t *op = A.old_pia + c * 5;
+        op[0] = s1; op[1] = n1; op[2] = s2; op[3] = n2; op[4] = ok ? n_new : -1;
+        if (ok) {
+            const int32_t n_del = N - n_new;
+            const int32_t d2 = min(n_del, n2), d1 = n_del - d2;
+            const int32_t nn2 = n2 - d2, nn1 = n1 - d1;
+            A.sp.pia.n2[c] = nn2;
+            A.sp.pia.end2[c] = nn2 > 0 ? s2 + nn2 - 1 : -1;
+            if (nn2 == 0) A.sp.pia.start2[c] = 0;
+            A.sp.pia.n1[c] = nn1;
+            A.sp.pia.end1[c] = nn1 > 0 ? s1 + nn1 - 1 : -1;
+            if (nn1 == 0) A.sp.pia.start1[c] = 0;
+            A.sp.pia.nloc[c] = nn1 + nn2;
+            if (!(c == A.n_cells - 1) || (n_del > n2)) A.sp.ctr->contiguous = 0;
+        }
+    }
+#undef AL
+}
+
 #pragma clang fp contract(on)
 
-extern "C" int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim, int32_t n_mim,
-                             double vref, int32_t scaling_vref, int32_t centered_at_mean, const double *v_multipliers,
-                             int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
-                             int32_t squash, int32_t *status) {
+static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim, int32_t n_mim,
+                           double vref, int32_t scaling_vref, int32_t centered_at_mean, const double *v_multipliers,
+                           int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
+                           int32_t squash, int32_t max_pcols, int32_t method, int32_t *status) {
     MZ_CHECK(species >= 1 && species <= h->n_species, "merge_nnls: bad species");
+    MZ_CHECK(method == 0 || method == 1, "merge_nnls: method must be 0 (reference layout) or 1 (LDS-resident)");
     MZ_CHECK(n_vmult >= 0 && n_vmult <= NNLS_MAX_VMULT, "merge_nnls: too many v_multipliers");
     MZ_CHECK(n_mim >= 0, "merge_nnls: bad multi-index count");
     MzSpecies &s = h->sp[species - 1];
@@ -1130,6 +1691,11 @@ extern "C" int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, co
     A.tmp.v0 = s.vb[0]; A.tmp.v1 = s.vb[1]; A.tmp.v2 = s.vb[2];
     A.tmp.w = s.wb;
     A.n_cells = nc;
+    A.max_pcols = max_pcols < 0 ? -1 : max_pcols;
+    if (method == 1) {
+        MZ_CHECK(max_pcols >= 0 && max_pcols + A.centered + 8 * n_vmult <= NF_COLS,
+                 "merge_nnls: the LDS-resident method needs max_pcols + fictitious columns <= 256");
+    }
     // cells to merge and their scratch sizes (host side)
     MzCounters snap;
     int r = mz_read_counters(h, species, &snap);
@@ -1142,7 +1708,7 @@ extern "C" int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, co
     int64_t tot = 0;
     for (int c = 0; c < nc; ++c) {
         if (nloc[c] > threshold) {
-            const int64_t n = nloc[c] + n_add;
+            const int64_t n = (A.max_pcols >= 0 ? std::min<int64_t>(nloc[c], A.max_pcols) : nloc[c]) + n_add;
             cells.push_back(c);
             offs.push_back(tot);
             tot += ((int64_t)m * n + 3 * n + (n + 1) / 2 + 32) & ~(int64_t)31;
@@ -1161,7 +1727,17 @@ extern "C" int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, co
         MZ_HIP(hipMemcpy(old_pia, init.data(), init.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     mz_timing_begin(h, "merge_nnls");
-    if (!cells.empty()) {
+    if (!cells.empty() && method == 1) {
+        int32_t *dcells = (int32_t *)mz_scratch(h, "nnls_cells", cells.size() * sizeof(int32_t));
+        MZ_CHECK(dcells, "merge_nnls: scratch allocation failed");
+        MZ_HIP(hipMemcpy(dcells, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        A.cells = dcells;
+        A.status = d_status;
+        A.old_pia = old_pia;
+        if (m <= 35) hipLaunchKernelGGL(k_nnls_fast<35>, dim3((unsigned)cells.size()), dim3(NF_COLS), 0, h->stream, A);
+        else hipLaunchKernelGGL(k_nnls_fast<NNLS_MAX_ROWS>, dim3((unsigned)cells.size()), dim3(NF_COLS), 0, h->stream, A);
+        MZ_HIP(hipGetLastError());
+    } else if (!cells.empty()) {
         // batches bounded to ~1 GiB of matrix scratch
         const int64_t budget = (int64_t)1 << 27;   // doubles
         size_t i0 = 0;
@@ -1195,8 +1771,26 @@ extern "C" int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, co
                        snap.nbuffer, snap.n_total, nc);
     mz_timing_end(h);
     MZ_HIP(hipGetLastError());
-    if (status) MZ_HIP(hipMemcpy(status, d_status, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
+    // the handle's stream does not synchronise with hipMemcpy: wait for it first
     MZ_HIP(hipStreamSynchronize(h->stream));
+    if (status) MZ_HIP(hipMemcpy(status, d_status, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (squash) return mz_squash(h, species);
     return 0;
+}
+
+extern "C" int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim, int32_t n_mim,
+                             double vref, int32_t scaling_vref, int32_t centered_at_mean, const double *v_multipliers,
+                             int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
+                             int32_t squash, int32_t *status) {
+    return nnls_merge_impl(h, species, threshold, mim, n_mim, vref, scaling_vref, centered_at_mean, v_multipliers,
+                           n_vmult, max_err, iteration_mult, w_threshold, squash, -1, 0, status);
+}
+
+extern "C" int mz_merge_nnls_cand(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim,
+                                  int32_t n_mim, double vref, int32_t scaling_vref, int32_t centered_at_mean,
+                                  const double *v_multipliers, int32_t n_vmult, double max_err,
+                                  int32_t iteration_mult, double w_threshold, int32_t squash, int32_t max_pcols,
+                                  int32_t method, int32_t *status) {
+    return nnls_merge_impl(h, species, threshold, mim, n_mim, vref, scaling_vref, centered_at_mean, v_multipliers,
+                           n_vmult, max_err, iteration_mult, w_threshold, squash, max_pcols, method, status);
 }

@@ -241,6 +241,10 @@ int mzo_merge_nnls_based(mzo_rng *rng, mzo_nnls_merge *nm, mzo_pv *pv, mzo_pia *
                          int64_t species, double vref, int32_t scaling_vref, int64_t n_rand_pairs,
                          double max_err, int32_t centered_at_mean, const double *v_multipliers,
                          int64_t n_vmult, int64_t iteration_mult, double w_threshold);
+int mzo_merge_nnls_cand(mzo_rng *rng, mzo_nnls_merge *nm, mzo_pv *pv, mzo_pia *pia, int64_t cell,
+                        int64_t species, double vref, int32_t scaling_vref, int64_t n_rand_pairs,
+                        double max_err, int32_t centered_at_mean, const double *v_multipliers,
+                        int64_t n_vmult, int64_t iteration_mult, double w_threshold, int64_t max_pcols);
 void mzo_scale_columns(double *A, int64_t m, int64_t n, double *column_norms);
 
 /* ---------------- multi-step drivers (CPU baseline) ---------------- */

@@ -158,6 +158,7 @@ def lib():
             "mzo_scale_columns": (None, [P(f64), i64, i64, P(f64)]),
             "mzo_nnls_merge_init": (None, [P(NNLSMergeS), P(i32), i64, P(i32), i64]),
             "mzo_merge_nnls_based": (C.c_int, [P(Rng), P(NNLSMergeS), P(PvS), P(PiaS), i64, i64, f64, i32, i64, f64, i32, P(f64), i64, i64, f64]),
+            "mzo_merge_nnls_cand": (C.c_int, [P(Rng), P(NNLSMergeS), P(PvS), P(PiaS), i64, i64, f64, i32, i64, f64, i32, P(f64), i64, i64, f64, i64]),
             "mzo_sample_maxwellian": (None, [P(Rng), P(PvS), i64, i64, f64, f64, P(f64)]),
             "mzo_sample_particles_equal_weight": (None, [P(Rng), P(PvS), P(PiaS), i64, i64, i64, f64, f64, f64, f64, f64, f64, f64, f64, f64]),
             "mzo_sample_bkw": (None, [P(Rng), P(PvS), i64, i64, f64, f64, P(f64)]),
@@ -438,12 +439,15 @@ class NNLSMerge:
 
     def merge(self, rng, pv, pia, cell, species, vref=1.0, scaling="variance", n_rand_pairs=0,
               max_err=1e-11, centered_at_mean=True, v_multipliers=(0.5, 1.0), iteration_mult=2,
-              w_threshold=0.0):
+              w_threshold=0.0, max_pcols=-1):
+        """merge_nnls_based!; max_pcols >= 0 keeps only the first max_pcols
+        particles (logical order) as columns, all particles still define the
+        moments."""
         vm = np.ascontiguousarray(v_multipliers, dtype=np.float64)
-        return lib().mzo_merge_nnls_based(rng.ref, C.byref(self.s), pv.p, pia.p, cell, species, vref,
-                                          1 if scaling == "vref" else 0, n_rand_pairs, max_err,
-                                          1 if centered_at_mean else 0, _dptr(vm), len(vm),
-                                          iteration_mult, w_threshold)
+        return lib().mzo_merge_nnls_cand(rng.ref, C.byref(self.s), pv.p, pia.p, cell, species, vref,
+                                         1 if scaling == "vref" else 0, n_rand_pairs, max_err,
+                                         1 if centered_at_mean else 0, _dptr(vm), len(vm),
+                                         iteration_mult, w_threshold, max_pcols)
 
 
 def nnls_solve(A, b, max_iter=None, kkt_tol=1e-16):

@@ -600,9 +600,21 @@ int mzo_merge_nnls_based(mzo_rng *rng, mzo_nnls_merge *nm, mzo_pv *pv, mzo_pia *
                          int64_t species, double vref, int32_t scaling_vref, int64_t n_rand_pairs,
                          double max_err, int32_t centered_at_mean, const double *v_multipliers,
                          int64_t n_vmult, int64_t iteration_mult, double w_threshold) {
+    return mzo_merge_nnls_cand(rng, nm, pv, pia, cell, species, vref, scaling_vref, n_rand_pairs, max_err,
+                               centered_at_mean, v_multipliers, n_vmult, iteration_mult, w_threshold, -1);
+}
+
+/* The same merge with at most max_pcols particle columns (the first ones in
+ * logical order; max_pcols < 0: all, which is merge_nnls_based! itself).
+ * Every particle still enters the moments (the right-hand side). */
+int mzo_merge_nnls_cand(mzo_rng *rng, mzo_nnls_merge *nm, mzo_pv *pv, mzo_pia *pia, int64_t cell,
+                        int64_t species, double vref, int32_t scaling_vref, int64_t n_rand_pairs,
+                        double max_err, int32_t centered_at_mean, const double *v_multipliers,
+                        int64_t n_vmult, int64_t iteration_mult, double w_threshold, int64_t max_pcols) {
     mzo_ind *d = mzo_pia_at(pia, cell, species);
     int64_t n_add = (centered_at_mean ? 1 : 0) + 8 * n_vmult;
-    int64_t ncols = d->n_local + n_add + n_rand_pairs;
+    int64_t npc = (max_pcols >= 0 && max_pcols < d->n_local) ? max_pcols : d->n_local;
+    int64_t ncols = npc + n_add + n_rand_pairs;
     int64_t m = nm->n_total_conserved, nmv = nm->n_moments_vel;
     double *A = (double *)calloc((size_t)(m * ncols), sizeof(double));
     double *vp = (double *)calloc((size_t)(6 * ncols), sizeof(double));
@@ -645,7 +657,7 @@ int mzo_merge_nnls_based(mzo_rng *rng, mzo_nnls_merge *nm, mzo_pv *pv, mzo_pia *
     nm->maxvx -= nm->v0[0]; nm->maxvy -= nm->v0[1]; nm->maxvz -= nm->v0[2];
 
     /* compute_lhs_and_rhs! */
-    int64_t col = 1;
+    int64_t col = 1, li = 0;
     for (int grp = 1; grp <= 2; ++grp) {
         if (grp == 2 && d->n_group2 <= 0) break;
         int64_t st = grp == 1 ? d->start1 : d->start2, en = grp == 1 ? d->end1 : d->end2;
@@ -659,19 +671,23 @@ int mzo_merge_nnls_based(mzo_rng *rng, mzo_nnls_merge *nm, mzo_pv *pv, mzo_pia *
             a = px[0] - nm->x0[0]; nm->Epx = nm->Epx + pw * (a * a);
             a = px[1] - nm->x0[1]; nm->Epy = nm->Epy + pw * (a * a);
             a = px[2] - nm->x0[2]; nm->Epz = nm->Epz + pw * (a * a);
-            double *vpc = &vp[6 * (col - 1)];
-            vpc[0] = v[0]; vpc[1] = v[1]; vpc[2] = v[2]; vpc[3] = px[0]; vpc[4] = px[1]; vpc[5] = px[2];
+            const int has_col = li < npc;
+            if (has_col) {
+                double *vpc = &vp[6 * (col - 1)];
+                vpc[0] = v[0]; vpc[1] = v[1]; vpc[2] = v[2]; vpc[3] = px[0]; vpc[4] = px[1]; vpc[5] = px[2];
+            }
             for (int64_t q = 0; q < nmv; ++q) {
                 double t = ccm3(v[0], v[1], v[2], nm->v0[0], nm->v0[1], nm->v0[2], nm->mim[q]);
                 nm->rhs[q] = nm->rhs[q] + pw * t;
-                A[(col - 1) * m + q] = t;
+                if (has_col) A[(col - 1) * m + q] = t;
             }
             for (int64_t q = 0; q < nm->n_moments_pos; ++q) {
                 double t = ccm3(px[0], px[1], px[2], nm->x0[0], nm->x0[1], nm->x0[2], nm->mim_pos[q]);
                 nm->rhs[nmv + q] = nm->rhs[nmv + q] + pw * t;
-                A[(col - 1) * m + nmv + q] = t;
+                if (has_col) A[(col - 1) * m + nmv + q] = t;
             }
-            col += 1;
+            if (has_col) col += 1;
+            li += 1;
         }
     }
     for (int64_t i = 0; i < m; ++i) nm->rhs[i] /= nm->w_total;

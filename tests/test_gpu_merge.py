@@ -168,14 +168,7 @@ def test_nnls_merge_reference_case(hip, mzo):
     st.close()
 
 
-@pytest.mark.parametrize("scaling", ["variance", "vref"])
-def test_nnls_merge_batch_vs_oracle(hip, mzo, scaling):
-    """Many cells of bi-Maxwellian particles with lognormal weights merged in
-    one launch; every cell's status and merged particles equal the oracle's
-    sequential merges; ids pushed to the free-id LIFO in cell order."""
-    rng = np.random.default_rng(9)
-    counts = [int(k) for k in rng.integers(40, 400, size=24)]
-    counts[3] = 20                      # below the threshold: untouched
+def _bimaxwell_cells(mzo, counts, rng):
     nc = len(counts)
     n = sum(counts)
     pv = mzo.ParticleVector(n)
@@ -193,6 +186,20 @@ def test_nnls_merge_batch_vs_oracle(hip, mzo, scaling):
         pia.set(c, 1, k, pos, pos + k - 1, k)
         pos += k
     pia.set_n_total(1, n)
+    return pv, pia
+
+
+@pytest.mark.parametrize("scaling", ["variance", "vref"])
+def test_nnls_merge_batch_vs_oracle(hip, mzo, scaling):
+    """Many cells of bi-Maxwellian particles with lognormal weights merged in
+    one launch; every cell's status and merged particles equal the oracle's
+    sequential merges; ids pushed to the free-id LIFO in cell order."""
+    rng = np.random.default_rng(9)
+    counts = [int(k) for k in rng.integers(40, 400, size=24)]
+    counts[3] = 20                      # below the threshold: untouched
+    nc = len(counts)
+    n = sum(counts)
+    pv, pia = _bimaxwell_cells(mzo, counts, rng)
     st = hip.DeviceStore(nc, 1, n + 256)
     helpers.upload_from_oracle(st, mzo, pv, pia)
     vref = np.sqrt(2 * helpers.KB * 300.0 / helpers.MASS_AR)
@@ -212,4 +219,91 @@ def test_nnls_merge_batch_vs_oracle(hip, mzo, scaling):
     got = st.download_species(1)
     assert got["err"] == 0
     helpers.assert_state_matches(mzo, got, pv, pia, rtol=1e-13)
+    st.close()
+
+
+def _cell_moments(w, v, rows):
+    """Weighted moments sum w vx^a vy^b vz^c and their scale sum w |.|."""
+    t = np.array([v[:, 0] ** a * v[:, 1] ** b * v[:, 2] ** c for (a, b, c) in rows])
+    return (t * w).sum(1), (np.abs(t) * w).sum(1)
+
+
+def _nnls_rows():
+    return [(0, 0, 0)] + _mim(4)
+
+
+@pytest.mark.parametrize("max_pcols", [239, 100, 20])
+def test_nnls_candidates_reference_layout_vs_oracle(hip, mzo, max_pcols):
+    """Columns cut to the first max_pcols particles (plus the 17 fictitious
+    ones), the moments of all particles matched: the reference-layout kernel
+    equals the oracle's merge_nnls_based! on the same column set bit for bit,
+    for merged and failed cells alike (20 columns are mostly infeasible)."""
+    rng = np.random.default_rng(21)
+    counts = [int(k) for k in rng.integers(40, 600, size=20)]
+    counts[5] = 25
+    nc, n = len(counts), sum(counts)
+    pv, pia = _bimaxwell_cells(mzo, counts, rng)
+    st = hip.DeviceStore(nc, 1, n + 256)
+    helpers.upload_from_oracle(st, mzo, pv, pia)
+    mim = _mim(4)
+    status = hip.merge_nnls_candidates(st, 1, 30, mim, max_pcols, method="reference", squash=False)
+    nm = mzo.NNLSMerge(mim)
+    nm.s.reset_x0 = 1
+    ref_status = [nm.merge(mzo.RNG(seed=1), pv, pia, c, 1, max_pcols=max_pcols) if pia.ind(c).n_local > 30 else 0
+                  for c in range(1, nc + 1)]
+    assert list(status) == ref_status
+    if max_pcols == 239:
+        assert sum(s == 1 for s in ref_status) >= 15
+    if max_pcols == 20:
+        assert sum(s == -1 for s in ref_status) >= 10
+    got = st.download_species(1)
+    assert got["err"] == 0
+    helpers.assert_state_matches(mzo, got, pv, pia, rtol=1e-13)
+    st.close()
+
+
+@pytest.mark.parametrize("scaling", ["variance", "vref"])
+def test_nnls_candidates_lds_vs_oracle(hip, mzo, scaling):
+    """The LDS-resident method on the same column set: every cell's status
+    equals the oracle's, a merged cell keeps the oracle's particle count, its
+    new particles match the oracle's to 1e-9, and all 35 moments of the cell
+    (mass, momentum and energy among them) are preserved to 1e-12 of their
+    scale; failed and skipped cells are untouched."""
+    rng = np.random.default_rng(22)
+    counts = [int(k) for k in rng.integers(40, 800, size=24)]
+    counts[2] = 25
+    nc, n = len(counts), sum(counts)
+    pv, pia = _bimaxwell_cells(mzo, counts, rng)
+    st = hip.DeviceStore(nc, 1, n + 256)
+    helpers.upload_from_oracle(st, mzo, pv, pia)
+    pre = st.download_species(1)
+    vref = np.sqrt(2 * helpers.KB * 300.0 / helpers.MASS_AR)
+    mim = _mim(4)
+    status = hip.merge_nnls_candidates(st, 1, 30, mim, 239, method="lds", vref=vref, scaling=scaling,
+                                       squash=False)
+    nm = mzo.NNLSMerge(mim)
+    nm.s.reset_x0 = 1
+    ref_status = [nm.merge(mzo.RNG(seed=1), pv, pia, c, 1, vref=vref, scaling=scaling, max_pcols=239)
+                  if pia.ind(c).n_local > 30 else 0 for c in range(1, nc + 1)]
+    assert list(status) == ref_status
+    assert sum(s == 1 for s in ref_status) >= 18
+    got = st.download_species(1)
+    assert got["err"] == 0
+    ref = helpers.oracle_upload_arrays(mzo, pv, pia)
+    rows = _nnls_rows()
+    for c in range(nc):
+        r0, r1, r2 = pre["pia"][c], got["pia"][c], ref["pia"][c]
+        assert list(r1) == list(r2), c
+        a0, b0 = r0[1] - 1, r0[2]
+        w_pre, v_pre = pre["w"][a0:b0], pre["v"][a0:b0]
+        a1, b1 = r1[1] - 1, r1[2]
+        w, v = got["w"][a1:b1], got["v"][a1:b1]
+        if status[c] != 1:
+            assert np.array_equal(w, w_pre) and np.array_equal(v, v_pre)
+            continue
+        assert np.allclose(w, ref["w"][a1:b1], rtol=1e-9, atol=0)
+        assert np.allclose(v, ref["v"][a1:b1], rtol=1e-9, atol=1e-9 * np.abs(v_pre).max())
+        m0, scale = _cell_moments(w_pre, v_pre, rows)
+        m1, _ = _cell_moments(w, v, rows)
+        assert np.max(np.abs(m1 - m0) / scale) < 1e-12, c
     st.close()
