@@ -224,13 +224,28 @@ def run_batched_0d(a, pkg):
     st.timing(False)
     sc = st.scalars(1)
     assert sc["err"] == 0 and sc["n_total"] == n, sc
+    # the same cells under the other launch patterns, continuing the run
+    # (a few steps each; the per-step cost barely changes as the state relaxes)
+    others = {}
+    if mode == "batched" and not a.no_compare:
+        s1 = s0 + steps
+        for other, k in (("plain", 20), ("per-cell", 2)):
+            st.sync()
+            t1 = time.perf_counter()
+            pkg.batch0d_steps(st, k, s1, dt, sp, mode=other)
+            st.sync()
+            others[other] = {"steps": k, "ms_per_step": (time.perf_counter() - t1) * 1e3 / k}
+            s1 += k
+        for other in others:
+            others[other]["batched_speedup"] = others[other]["ms_per_step"] / (el * 1e3 / steps)
     out = {"metric": "particle-timesteps/s (NTC+props), batched uneven 0-D cells", "value": n * steps / el,
            "unit": "particle-timesteps/s", "n_gpus": 1, "steps": steps, "ms_per_step": el * 1e3 / steps,
            "higher_is_better": True, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "65536 0-D cells, log-uniform 20..20000 particles (Philox seed 7), "
                                   "pseudo-Maxwell Ar, BKW at 273 K, w ~ U(0.5, 2) Fnum; ntc_equal_weight! + "
                                   "compute_props! per step", "n_particles": n, "mode": mode,
-                      "batch_shape": st.batch_info() if mode == "batched" else None, "autotune": tune}}
+                      "batch_shape": st.batch_info() if mode == "batched" else None, "autotune": tune},
+           "compare": others}
     print(json.dumps(out))
     st.close()
 
@@ -365,6 +380,8 @@ def main():
                          "eager: kernel by kernel")
     ap.add_argument("--graph-steps", type=int, default=400, help="steps of the step-graph path timed alongside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="batched-0d: skip timing the plain and per-cell launch patterns")
     ap.add_argument("--no-autotune", action="store_true", help="slab mode: keep the library's default shape")
     ap.add_argument("--workload", choices=("couette", "batched-0d", "nnls-merge"), default="couette",
                     help="couette: the headline 1-D Couette line; batched-0d: 65,536 uneven 0-D cells; "

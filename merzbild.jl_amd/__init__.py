@@ -622,13 +622,15 @@ def batch0d_autotune(store, step0, dt, species_data=None, steps=3,
 def batch0d_steps(store, n_steps, step0, dt, species_data=None, V=None, props_every=1, species=1, mode="batched"):
     """n_steps of ntc_equal_weight! + compute_props! in every 0-D cell (the
     per-cell loops of test/test_bkw.jl:88-93 over a batch of cells).  mode
-    "plain" launches the NTC and props kernels per step, "batched" runs all
-    steps in one persistent launch; both give identical results."""
+    "plain" launches the NTC kernel over all cells and the props kernel per
+    step, "per-cell" launches NTC once per cell (the reference's loop of
+    ntc! calls, one drop-in launch each) and then props, "batched" runs all
+    steps in one persistent launch; all give identical results."""
     m = _masses(store, species_data)
     Vd = None if V is None else np.ascontiguousarray(np.broadcast_to(np.asarray(V, dtype=np.float64),
                                                                       (store.n_cells,)))
     _check(load_library().mz_batch0d_steps(store.h, species, n_steps, step0, dt, _d(Vd), props_every, _d(m),
-                                           {"plain": 0, "batched": 1}[mode]))
+                                           {"plain": 0, "batched": 1, "per-cell": 2}[mode]))
 
 
 def multi_index_moments(n):

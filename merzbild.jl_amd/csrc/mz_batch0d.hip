@@ -42,7 +42,21 @@ struct B0Args {
     const B0Item *items;
     int32_t n_items;
     int32_t *next;
+    double *aos;               // [slot][4]: v0, v1, v2, w of the launch's cells
 };
+
+// The persistent launch works on a packed copy of the velocities and weights,
+// one 32-byte record per slot, so a collision candidate gathers two records
+// instead of eight scattered words; the copy is filled when a wavefront
+// takes an item and written back when it has run all steps.
+__device__ inline void b0_get(const double *aos, int64_t p, double &v0, double &v1, double &v2, double &w) {
+    const double4 t = reinterpret_cast<const double4 *>(aos)[p];
+    v0 = t.x; v1 = t.y; v2 = t.z; w = t.w;
+}
+__device__ inline void b0_put_v(double *aos, int64_t p, double v0, double v1, double v2) {
+    double *q = aos + 4 * p;
+    q[0] = v0; q[1] = v1; q[2] = v2;
+}
 
 // ntc_equal_weight! of one cell whose logical particle i sits at slot
 // i < n1 ? st1 + i : st2 + (i - n1); the candidate loop of k_ntc without
@@ -66,12 +80,13 @@ __device__ inline void b0_ntc_cell(const B0Args &A, const MzRngParams &rp, int c
         while (i == k) k = (int32_t)floor(mz_rand(rp, st) * (double)nla);
         const int64_t pi = i < n1 ? (int64_t)st1 + i : (int64_t)st2 + (i - n1);
         const int64_t pk = k < n1 ? (int64_t)st1 + k : (int64_t)st2 + (k - n1);
-        double vi0 = sp.v0[pi], vi1 = sp.v1[pi], vi2 = sp.v2[pi];
-        double vk0 = sp.v0[pk], vk1 = sp.v1[pk], vk2 = sp.v2[pk];
+        double vi0, vi1, vi2, wi, vk0, vk1, vk2, wk;
+        b0_get(A.aos, pi, vi0, vi1, vi2, wi);
+        b0_get(A.aos, pk, vk0, vk1, vk2, wk);
         double g0, g1, g2;
         double g = mz_rel_speed(vi0, vi1, vi2, vk0, vk1, vk2, g0, g1, g2);
         if (!(g > MZ_G_EPS)) continue;
-        double sgw = mz_vhs_sgw(A.factor, A.expo, g, sp.w[pi], sp.w[pk]);
+        double sgw = mz_vhs_sgw(A.factor, A.expo, g, wi, wk);
         sgwm = sgw > sgwm ? sgw : sgwm;
         if (!(mz_rand(rp, st) < sgw / sgwm)) continue;
         ++performed;
@@ -80,13 +95,206 @@ __device__ inline void b0_ntc_cell(const B0Args &A, const MzRngParams &rp, int c
         mz_com(A.mu1, A.mu2, vi0, vi1, vi2, vk0, vk1, vk2, vc0, vc1, vc2);
         double ui0, ui1, ui2, uk0, uk1, uk2;
         mz_vhs_scatter(rp, st, A.mu1, A.mu2, g, vc0, vc1, vc2, ui0, ui1, ui2, uk0, uk1, uk2);
-        sp.v0[pi] = ui0; sp.v1[pi] = ui1; sp.v2[pi] = ui2;
-        sp.v0[pk] = uk0; sp.v1[pk] = uk1; sp.v2[pk] = uk2;
+        b0_put_v(A.aos, pi, ui0, ui1, ui2);
+        b0_put_v(A.aos, pk, uk0, uk1, uk2);
     }
     n_coll_out = (int32_t)n_coll;
 }
 
-__global__ void __launch_bounds__(256, 3) k_batch0d(B0Args A) {
+// A cell alone in its item with at least this many particles collides on
+// the whole wavefront (b0_ntc_cell_wave); smaller ones use one lane.
+#define B0_WAVE_MIN 256
+#define B0_SPEC 2   // candidates evaluated ahead per lane and window
+
+// The candidate starting at draw o of the cell's stream, evaluated directly
+// from the current velocities (the loop body of b0_ntc_cell); returns the
+// draw after it.
+__device__ inline uint64_t b0_candidate(const B0Args &A, const MzRngParams &rp, int c, int32_t n1, int32_t st1,
+                                        int32_t st2, int32_t nla, uint64_t o, double &sgwm, int32_t &performed,
+                                        int32_t &eqw) {
+    MzStream st = mz_stream((uint64_t)c, o);
+    int32_t i = (int32_t)floor(mz_rand(rp, st) * (double)nla);
+    int32_t k = (int32_t)floor(mz_rand(rp, st) * (double)nla);
+    while (i == k) k = (int32_t)floor(mz_rand(rp, st) * (double)nla);
+    const int64_t pi = i < n1 ? (int64_t)st1 + i : (int64_t)st2 + (i - n1);
+    const int64_t pk = k < n1 ? (int64_t)st1 + k : (int64_t)st2 + (k - n1);
+    double vi0, vi1, vi2, wi, vk0, vk1, vk2, wk;
+    b0_get(A.aos, pi, vi0, vi1, vi2, wi);
+    b0_get(A.aos, pk, vk0, vk1, vk2, wk);
+    double g0, g1, g2;
+    double g = mz_rel_speed(vi0, vi1, vi2, vk0, vk1, vk2, g0, g1, g2);
+    if (!(g > MZ_G_EPS)) return st.k;
+    double sgw = mz_vhs_sgw(A.factor, A.expo, g, wi, wk);
+    sgwm = sgw > sgwm ? sgw : sgwm;
+    if (!(mz_rand(rp, st) < sgw / sgwm)) return st.k;
+    ++performed;
+    ++eqw;
+    double vc0, vc1, vc2;
+    mz_com(A.mu1, A.mu2, vi0, vi1, vi2, vk0, vk1, vk2, vc0, vc1, vc2);
+    double ui0, ui1, ui2, uk0, uk1, uk2;
+    mz_vhs_scatter(rp, st, A.mu1, A.mu2, g, vc0, vc1, vc2, ui0, ui1, ui2, uk0, uk1, uk2);
+    b0_put_v(A.aos, pi, ui0, ui1, ui2);
+    b0_put_v(A.aos, pk, uk0, uk1, uk2);
+    return st.k;
+}
+
+// ntc_equal_weight! of one large cell on a whole wavefront, with the same
+// draws, order and arithmetic as b0_ntc_cell.  The candidates are taken in
+// windows of 128 draws: at the start of a window at draw D, lane L evaluates
+// the candidates that would start at draws D + L and D + 64 + L from the
+// velocities as they are then -- its pair (draws D+L, D+L+1), relative speed, sigma g w, and the
+// post-collision velocities from draws D+L+3 and D+L+4.  The candidate loop
+// then runs on uniform values: each candidate starts where the previous one
+// ended, takes its lane's evaluation, updates (sigma g w)_max and draws its
+// acceptance number (draw o + 2) exactly as the one-lane loop does.  A
+// candidate whose pair needs a redraw of k, or shares a particle with a
+// collision performed earlier in the window, is evaluated directly
+// (b0_candidate) and the next window starts after it.  Every lane returns
+// the same sgwm and counts.
+__device__ inline void b0_ntc_cell_wave(const B0Args &A, const MzRngParams &rp, int c, double Vc, int32_t n1,
+                                        int32_t st1, int32_t st2, int32_t nla, double &sgwm, int32_t &n_coll_out,
+                                        int32_t &performed, int32_t &eqw) {
+    const MzSpView &sp = A.sp;
+    const int lane = threadIdx.x & 63;
+    double R;
+    {
+        MzStream st = mz_stream((uint64_t)c);
+        R = mz_rand(rp, st);
+    }
+    int64_t n_coll = (int64_t)floor(mz_ncoll_est(true, A.dt, nla, nla, sgwm, Vc, R));
+    if (!(n_coll <= MZ_NTC_MAX_CANDIDATES)) {
+        if (lane == 0) atomicOr(&sp.ctr->err, MZ_ERR_SCRATCH);
+        n_coll = 0;
+    }
+    performed = 0;
+    eqw = 0;
+    int64_t cc = 0;
+    uint64_t o = 1;   // the next draw of the stream
+    while (cc < n_coll) {
+        // ---- window at draw D: lane L holds Philox blocks D/2 + L and
+        // D/2 + 64 + L (draws 2 (D/2) .. 2 (D/2) + 255), then evaluates the
+        // candidates that would start at draws D + L and D + 64 + L
+        const uint64_t D = o, b0 = D >> 1;
+        const double sgwm0 = sgwm;
+        double h[4];
+        mz_philox_pair(rp, (uint64_t)c, b0 + (uint64_t)lane, h[0], h[1]);
+        mz_philox_pair(rp, (uint64_t)c, b0 + 64 + (uint64_t)lane, h[2], h[3]);
+        auto draw = [&](uint64_t d) {   // any lane's draw d, 2 b0 <= d < 2 b0 + 256
+            const int src = (int)((d >> 1) - b0);
+            const int sl = src & 63;
+            const double a0 = __shfl(h[0], sl, 64), a1 = __shfl(h[1], sl, 64);
+            const double a2 = __shfl(h[2], sl, 64), a3 = __shfl(h[3], sl, 64);
+            return src < 64 ? ((d & 1) ? a1 : a0) : ((d & 1) ? a3 : a2);
+        };
+        int32_t si[B0_SPEC], sk[B0_SPEC], sflag[B0_SPEC];
+        double sgw[B0_SPEC], ui[B0_SPEC][3], uk[B0_SPEC][3];
+#pragma unroll
+        for (int q = 0; q < B0_SPEC; ++q) {
+            const uint64_t oL = D + (uint64_t)(64 * q + lane);
+            const double d0 = draw(oL), d1 = draw(oL + 1), d2 = draw(oL + 2), d3 = draw(oL + 3), d4 = draw(oL + 4);
+            si[q] = (int32_t)floor(d0 * (double)nla);
+            sk[q] = (int32_t)floor(d1 * (double)nla);
+            const int64_t pi = si[q] < n1 ? (int64_t)st1 + si[q] : (int64_t)st2 + (si[q] - n1);
+            const int64_t pk = sk[q] < n1 ? (int64_t)st1 + sk[q] : (int64_t)st2 + (sk[q] - n1);
+            double vi0, vi1, vi2, wi, vk0, vk1, vk2, wk;
+            b0_get(A.aos, pi, vi0, vi1, vi2, wi);
+            b0_get(A.aos, pk, vk0, vk1, vk2, wk);
+            double g0, g1, g2;
+            const double g = mz_rel_speed(vi0, vi1, vi2, vk0, vk1, vk2, g0, g1, g2);
+            sgw[q] = mz_vhs_sgw(A.factor, A.expo, g, wi, wk);
+            // the acceptance test as it comes out while (sigma g w)_max stays
+            // at its value at the window's start: the same operands as below
+            sflag[q] = (si[q] == sk[q] ? 1 : 0) | (g > MZ_G_EPS ? 2 : 0) | (sgw[q] > sgwm0 ? 4 : 0) |
+                       (d2 < sgw[q] / sgwm0 ? 8 : 0);
+            double vc0, vc1, vc2;
+            mz_com(A.mu1, A.mu2, vi0, vi1, vi2, vk0, vk1, vk2, vc0, vc1, vc2);
+            mz_vhs_post(A.mu1, A.mu2, g, vc0, vc1, vc2, d3, d4, ui[q][0], ui[q][1], ui[q][2], uk[q][0], uk[q][1],
+                        uk[q][2]);
+        }
+        // ---- walk the window; lane j < nd holds a particle that collided in it
+        int32_t dirty = -1;
+        int nd = 0;
+        bool direct = false;
+        while (cc < n_coll) {
+            const int L = (int)(o - D);
+            if (L >= 64 * B0_SPEC || nd >= 62) break;
+            const int q = L >> 6, ln = L & 63;
+            int32_t i, k, f;
+            if (q == 0) {
+                i = __builtin_amdgcn_readlane(si[0], ln);
+                k = __builtin_amdgcn_readlane(sk[0], ln);
+                f = __builtin_amdgcn_readlane(sflag[0], ln);
+            } else {
+                i = __builtin_amdgcn_readlane(si[1], ln);
+                k = __builtin_amdgcn_readlane(sk[1], ln);
+                f = __builtin_amdgcn_readlane(sflag[1], ln);
+            }
+            if ((f & 1) || __ballot(lane < nd && (dirty == i || dirty == k))) {
+                direct = true;
+                break;
+            }
+            if (!(f & 2)) {   // g <= eps: no acceptance draw
+                o += 2;
+                ++cc;
+                continue;
+            }
+            bool acc;
+            if (!(f & 4) && sgwm == sgwm0) {
+                acc = (f & 8) != 0;   // (sigma g w)_max unchanged: the lane's test
+            } else {
+                const double s = mz_readlane_d(q == 0 ? sgw[0] : sgw[1], ln);
+                sgwm = s > sgwm ? s : sgwm;
+                const uint64_t du = o + 2;
+                const int src = (int)((du >> 1) - b0);
+                const int hs = (src >> 6) * 2 + (int)(du & 1);
+                const double U =
+                    mz_readlane_d(hs == 0 ? h[0] : (hs == 1 ? h[1] : (hs == 2 ? h[2] : h[3])), src & 63);
+                acc = U < s / sgwm;
+            }
+            if (!acc) {
+                o += 3;
+                ++cc;
+                continue;
+            }
+            ++performed;
+            ++eqw;
+            double a[3], e[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                a[d] = mz_readlane_d(q == 0 ? ui[0][d] : ui[1][d], ln);
+                e[d] = mz_readlane_d(q == 0 ? uk[0][d] : uk[1][d], ln);
+            }
+            if (lane == 0) {
+                const int64_t qi = i < n1 ? (int64_t)st1 + i : (int64_t)st2 + (i - n1);
+                const int64_t qk = k < n1 ? (int64_t)st1 + k : (int64_t)st2 + (k - n1);
+                b0_put_v(A.aos, qi, a[0], a[1], a[2]);
+                b0_put_v(A.aos, qk, e[0], e[1], e[2]);
+            }
+            if (lane == nd) dirty = i;
+            if (lane == nd + 1) dirty = k;
+            nd += 2;
+            o += 5;
+            ++cc;
+        }
+        if (direct) {
+            uint64_t on = 0;
+            double sg = sgwm;
+            int32_t pf = performed, eq = eqw;
+            if (lane == 0) on = b0_candidate(A, rp, c, n1, st1, st2, nla, o, sg, pf, eq);
+            o = (uint64_t)__shfl((long long)on, 0, 64);
+            sgwm = mz_readlane_d(sg, 0);
+            performed = __builtin_amdgcn_readlane(pf, 0);
+            eqw = __builtin_amdgcn_readlane(eq, 0);
+            ++cc;
+        }
+        // lane 0's velocity stores are complete and visible to the wave's
+        // loads of the next window
+        if (nd > 0 || direct) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    }
+    n_coll_out = (int32_t)n_coll;
+}
+
+__global__ void __launch_bounds__(256, 2) k_batch0d(B0Args A) {
     const int lane = threadIdx.x & 63;
     const MzSpView &sp = A.sp;
     for (;;) {
@@ -109,10 +317,32 @@ __global__ void __launch_bounds__(256, 3) k_batch0d(B0Args A) {
         }
         const int32_t nla = n1 + n2;
         int32_t n_coll = 0, perf = 0, eqw = 0;
+        // pack the item's cells into the launch's records
+        for (int j = 0; j < itm.ncell; ++j) {
+            const int32_t jn1 = __shfl(n1, j, 64), jst1 = __shfl(st1, j, 64), jst2 = __shfl(st2, j, 64);
+            const int32_t jtot = __shfl(nla, j, 64);
+            for (int32_t i = lane; i < jtot; i += 64) {
+                const int64_t p = i < jn1 ? (int64_t)jst1 + i : (int64_t)jst2 + (i - jn1);
+                reinterpret_cast<double4 *>(A.aos)[p] = make_double4(sp.v0[p], sp.v1[p], sp.v2[p], sp.w[p]);
+            }
+        }
+        // records written by one lane are read by others: order the wave's
+        // memory operations at each hand-over
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         for (int32_t t = 0; t < A.n_steps; ++t) {
             MzRngParams rp = A.rp;
             rp.tag = ((A.step0 + (uint64_t)t) << 4) | 1u;
-            if (own) b0_ntc_cell(A, rp, c, Vc, n1, st1, st2, nla, sgwm, n_coll, perf, eqw);
+            if (itm.ncell == 1 && __shfl(nla, 0, 64) >= B0_WAVE_MIN) {
+                const int cw = __shfl(c, 0, 64);
+                const int32_t wn1 = __shfl(n1, 0, 64), wst1 = __shfl(st1, 0, 64), wst2 = __shfl(st2, 0, 64);
+                double wsg = mz_readlane_d(sgwm, 0);
+                b0_ntc_cell_wave(A, rp, cw, mz_readlane_d(Vc, 0), wn1, wst1, wst2, __shfl(nla, 0, 64), wsg, n_coll,
+                                 perf, eqw);
+                sgwm = wsg;
+            } else if (own) {
+                b0_ntc_cell(A, rp, c, Vc, n1, st1, st2, nla, sgwm, n_coll, perf, eqw);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
             if (A.props_every > 0 && ((t + 1) % A.props_every) == 0) {
                 for (int j = 0; j < itm.ncell; ++j) {
                     const int cj = __shfl(c, j, 64);
@@ -120,7 +350,7 @@ __global__ void __launch_bounds__(256, 3) k_batch0d(B0Args A) {
                     const int32_t jtot = __shfl(nla, j, 64);
                     auto get = [&](int32_t i, double &w, double &v0, double &v1, double &v2) {
                         const int64_t p = i < jn1 ? (int64_t)jst1 + i : (int64_t)jst2 + (i - jn1);
-                        w = sp.w[p]; v0 = sp.v0[p]; v1 = sp.v1[p]; v2 = sp.v2[p];
+                        b0_get(A.aos, p, v0, v1, v2, w);
                     };
                     const MzPropsOut o = mz_props_wave(get, jtot, A.mass);
                     if (lane == 0) {
@@ -133,6 +363,16 @@ __global__ void __launch_bounds__(256, 3) k_batch0d(B0Args A) {
                         A.out_T[ix] = o.T;
                     }
                 }
+            }
+        }
+        // write the velocities back
+        for (int j = 0; j < itm.ncell; ++j) {
+            const int32_t jn1 = __shfl(n1, j, 64), jst1 = __shfl(st1, j, 64), jst2 = __shfl(st2, j, 64);
+            const int32_t jtot = __shfl(nla, j, 64);
+            for (int32_t i = lane; i < jtot; i += 64) {
+                const int64_t p = i < jn1 ? (int64_t)jst1 + i : (int64_t)jst2 + (i - jn1);
+                const double4 r = reinterpret_cast<const double4 *>(A.aos)[p];
+                sp.v0[p] = r.x; sp.v1[p] = r.y; sp.v2[p] = r.z;
             }
         }
         if (own) {
@@ -216,6 +456,8 @@ static int batch0d_launch(mz_store *h, int species, int32_t n_steps, uint64_t st
     A.items = d_items;
     A.n_items = (int32_t)items.size();
     A.next = d_next;
+    A.aos = (double *)mz_scratch(h, "b0_aos", (size_t)std::max<int64_t>(s.cap, 1) * 4 * sizeof(double));
+    MZ_CHECK(A.aos, "batch0d: record scratch allocation failed");
     int dev = 0, n_cu = 0, per_cu = 0;
     MZ_HIP(hipGetDevice(&dev));
     MZ_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -240,7 +482,8 @@ extern "C" int mz_batch0d_steps(mz_store *h, int32_t species, int32_t n_steps, u
     MZ_CHECK(h->inter[(species - 1) * h->n_species + (species - 1)].set, "batch0d: interaction not set");
     MZ_CHECK(masses != nullptr, "batch0d: masses is NULL");
     MZ_CHECK(n_steps >= 0, "batch0d: negative step count");
-    MZ_CHECK(mode == 0 || mode == 1, "batch0d: mode must be 0 (per-step kernels) or 1 (persistent)");
+    MZ_CHECK(mode >= 0 && mode <= 2,
+             "batch0d: mode must be 0 (per-step kernels), 1 (persistent) or 2 (one NTC launch per cell)");
     MZ_HIP(hipMemcpy(h->d_masses, masses, h->n_species * sizeof(double), hipMemcpyHostToDevice));
     const double *dV = h->d_V;
     if (V) {
@@ -253,14 +496,24 @@ extern "C" int mz_batch0d_steps(mz_store *h, int32_t species, int32_t n_steps, u
     }
     if (n_steps == 0) return 0;
     if (mode == 1) return batch0d_launch(h, species, n_steps, step0, dt, dV, props_every);
-    // per-step kernels: the NTC kernel and the props kernel (groups 1 and 2)
+    // mode 0: per step the NTC kernel over all cells and the props kernel;
+    // mode 2: per step one NTC launch per cell (the reference's loop of
+    // ntc!(..., cell, ...) calls, each a drop-in launch), then the props kernel
     mz_timing_begin(h, "batch0d");
     for (int32_t t = 0; t < n_steps; ++t) {
         const uint64_t step = step0 + (uint64_t)t;
-        int r = mz_launch_ntc_equal(h, species, species, dt, dV, 1e-16, 1, (step << 4) | 1u, 0, 1, h->n_cells);
-        if (r) return r;
+        if (mode == 2) {
+            for (int32_t c = 1; c <= h->n_cells; ++c) {
+                int r = mz_launch_ntc_equal(h, species, species, dt, dV, 1e-16, 1, (step << 4) | 1u, 0, c, c);
+                if (r) return r;
+            }
+        } else {
+            int r = mz_launch_ntc_equal(h, species, species, dt, dV, 1e-16, 1, (step << 4) | 1u, 0, 1,
+                                        h->n_cells);
+            if (r) return r;
+        }
         if (props_every > 0 && ((t + 1) % props_every) == 0) {
-            r = mz_launch_props(h, 0, 0);
+            int r = mz_launch_props(h, 0, 0);
             if (r) return r;
         }
     }

@@ -1105,12 +1105,6 @@ __global__ void __launch_bounds__(256) k_nnls_merge(NnlsArgs A) {
 // rounding (sums are taken in another order), not bit for bit.
 #define NF_COLS 256
 
-__device__ inline double nf_readlane(double v, int l) {
-    const uint64_t u = (uint64_t)__double_as_longlong(v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
 __device__ inline double nf_wsum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -1333,7 +1327,7 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
         for (int l = 1; l <= nsetp; ++l) {
             const int ip = nsetp + 1 - l;
             if (l != 1) {
-                const double zip = nf_readlane(zr, ip);
+                const double zip = mz_readlane_d(zr, ip);
                 if (lane < ip) zr = zr - AL(lane + 1, jj) * zip;
             }
             jj = idx[ip - 1];
@@ -1379,7 +1373,7 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
                 const int len = m - nsetp;
                 const double a = lane < m ? AL(lane + 1, j) : 0.0;
                 const bool hh = lane >= nsetp && lane < m;
-                const double asave = nf_readlane(a, nsetp);
+                const double asave = mz_readlane_d(a, nsetp);
                 double up = sd[16], top = asave;
                 if (len > 1) {
                     double cl = nf_wmax(hh ? fabs(a) : 0.0);
@@ -1406,7 +1400,7 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
                             }
                         }
                     }
-                    const double ztest = nf_readlane(zr, nsetp) / top;
+                    const double ztest = mz_readlane_d(zr, nsetp) / top;
                     if (ztest > 0) accept = 1;
                 }
                 if (accept) {

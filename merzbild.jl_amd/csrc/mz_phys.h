@@ -34,9 +34,10 @@ __device__ inline double mz_rel_speed(double vi0, double vi1, double vi2, double
     return sqrt(g0 * g0 + g1 * g1 + g2 * g2);
 }
 
-// sigma(g) g max(w_i, w_k), sigma(g) = C g^(1 - 2 omega) (expo = 1 - 2 omega)
+// sigma(g) g max(w_i, w_k), sigma(g) = C g^(1 - 2 omega) (expo = 1 - 2 omega);
+// g^0 is 1 for every g, so pseudo-Maxwell molecules (omega = 1/2) skip pow
 __device__ inline double mz_vhs_sgw(double factor, double expo, double g, double wi, double wk) {
-    double sigma = factor * pow(g, expo);
+    double sigma = factor * (expo == 0.0 ? 1.0 : pow(g, expo));
     return sigma * g * (wi > wk ? wi : wk);
 }
 

@@ -173,6 +173,28 @@ def _batch_case(hip, mzo, nc, nmax, seed):
     return dict(pv=pv, pia=pia, it=its[0][0], sg=sg, dt=0.025 * tref, n=n, nc=nc, store=store, m=m, seed=seed)
 
 
+def test_batched_0d_per_cell_launches_equal_plain(hip, mzo):
+    """One NTC launch per cell per step (the reference's ntc! loop as drop-in
+    calls) leaves the same particles, properties and collision factors as the
+    NTC kernel over all cells."""
+    case = _batch_case(hip, mzo, nc=40, nmax=2000, seed=23)
+    a, b = case["store"](), case["store"]()
+    hip.batch0d_steps(a, 3, 1, case["dt"], mode="plain")
+    hip.batch0d_steps(b, 3, 1, case["dt"], mode="per-cell")
+    da, db = a.download_species(1), b.download_species(1)
+    n = case["n"]
+    for k in ("v", "w", "index"):
+        assert np.array_equal(da[k][:n], db[k][:n]), k
+    pa, pb = hip.PhysProps(case["nc"], 1).fetch(a), hip.PhysProps(case["nc"], 1).fetch(b)
+    for k in ("np", "n", "v", "T"):
+        assert np.array_equal(getattr(pa, k), getattr(pb, k)), k
+    ca, cb = a.collision_factors(1, 1), b.collision_factors(1, 1)
+    for k in ca:
+        assert np.array_equal(ca[k], cb[k]), k
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("shape", [(0, 0), (1, 1), (8, 512), (64, 100000)])
 def test_batched_0d_persistent_equals_per_step(hip, mzo, shape):
     """The persistent batched launch and the per-step kernels leave identical
