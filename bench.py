@@ -246,7 +246,7 @@ def run_batched_0d(a, pkg):
                                   "compute_props! per step", "n_particles": n, "mode": mode,
                       "batch_shape": st.batch_info() if mode == "batched" else None, "autotune": tune},
            "compare": others}
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
     st.close()
 
 
@@ -367,7 +367,7 @@ def run_nnls(a, pkg):
                       "rows": len(set(_mim(4)) | {(0, 0, 0)})},
            "reference_layout_fp64": ref, "speedup_vs_reference_layout": lds["merges_per_s"] / ref["merges_per_s"],
            "k64_columns": {"merged": k64["merged"], "failed": k64["failed"]}}
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -503,7 +503,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
     st.close()
 
 

@@ -189,10 +189,13 @@ int mz_get_slab_info(mz_store *h, int32_t *info);
 /* Many independent 0-D cells: n_steps of ntc_equal_weight! in every cell of
  * `species` and compute_props! every props_every steps (the per-cell loops of
  * test/test_bkw.jl:88-93, test/test_2species.jl:52-65 over a batch of
- * cells).  mode 0 launches the NTC and props kernels once per step; mode 1
+ * cells).  mode 0 launches the NTC kernel over all cells and the props
+ * kernel once per step; mode 2 launches NTC once per cell per step (the
+ * reference's loop of ntc! calls, each a drop-in launch), then props; mode 1
  * runs all steps in one persistent launch whose wavefronts drain a queue of
- * cell groups ordered by size.  Both give identical results.  Step t uses
- * tag (t << 4) | 1.  V[n_cells] as for mz_ntc (NULL: the grid's cells). */
+ * cell groups ordered by size, large cells colliding on a whole wavefront.
+ * All give identical results.  Step t uses tag (t << 4) | 1.  V[n_cells] as
+ * for mz_ntc (NULL: the grid's cells). */
 int mz_batch0d_steps(mz_store *h, int32_t species, int32_t n_steps, uint64_t step0, double dt, const double *V,
                      int32_t props_every, const double *masses, int32_t mode);
 /* Packing of mode 1: at most max_cells (1..64) cells and about
