@@ -152,11 +152,18 @@ int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, const int32_t
  * (matrix in device memory, results equal to the CPU restatement bit for
  * bit); method 1 keeps the matrix in LDS with one thread per column and
  * block-parallel moment sums (needs max_pcols + fictitious columns <= 256,
- * results equal to rounding). */
+ * results equal to rounding); method 2 runs the active-set search on the
+ * matrix stored in FP32 and solves the final passive set again in FP64
+ * (normal equations, one refinement step), re-running in method 1 every
+ * cell whose FP64 solution is not positive or misses max_err (at most 35
+ * moments; more run method 1). */
 int mz_merge_nnls_cand(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim, int32_t n_mim,
                        double vref, int32_t scaling_vref, int32_t centered_at_mean, const double *v_multipliers,
                        int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
                        int32_t squash, int32_t max_pcols, int32_t method, int32_t *status);
+/* info[0] = cells the last mz_merge_nnls_cand call merged or tried,
+ * info[1] = cells its method 2 handed to the FP64 method. */
+int mz_get_nnls_info(mz_store *h, int32_t *info);
 
 /* ----------------------------------------------------------- time stepping */
 /* One 1-D Couette step repeated n_steps times: ntc! in every cell,

@@ -56,6 +56,7 @@ _SIGS = {
                                   P(i64)]),
     "mz_merge_nnls": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, f64, i32, i32, P(f64), i32, f64, i32, f64,
                                 i32, P(i32)]),
+    "mz_get_nnls_info": (C.c_int, [C.c_void_p, P(i32)]),
     "mz_merge_nnls_cand": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, f64, i32, i32, P(f64), i32, f64, i32,
                                      f64, i32, i32, i32, P(i32)]),
     "mz_couette_steps": (C.c_int, [C.c_void_p, i32, u64, f64, P(f64), i32, i32, P(f64), i32]),
@@ -369,6 +370,13 @@ class DeviceStore:
         _check(load_library().mz_get_batch_info(self.h, out))
         return tuple(out)
 
+    def nnls_info(self):
+        """(cells tried, cells handed to the FP64 method) of the last
+        merge_nnls_candidates call."""
+        out = (i32 * 2)()
+        _check(load_library().mz_get_nnls_info(self.h, out))
+        return tuple(out)
+
     def slab_info(self):
         """(particles per thread, cells per slab, slabs, outcome) of the last
         slab call; outcome 1 = ran, -1 = aborted and replayed on the graph,
@@ -515,7 +523,7 @@ def merge_nnls_based(store, species, threshold, multi_index_moments, vref=1.0, s
     return status
 
 
-NNLS_METHODS = {"reference": 0, "lds": 1}
+NNLS_METHODS = {"reference": 0, "lds": 1, "mixed": 2}
 
 
 def merge_nnls_candidates(store, species, threshold, multi_index_moments, max_pcols, method="lds", vref=1.0,
@@ -525,7 +533,9 @@ def merge_nnls_candidates(store, species, threshold, multi_index_moments, max_pc
     plus the fictitious ones; the moments matched are those of all the cell's
     particles.  method "reference" keeps the reference's layout (bit-exact
     with the CPU restatement), "lds" runs one workgroup per cell with the
-    matrix in LDS.  Returns the per-cell status as merge_nnls_based."""
+    matrix in LDS, "mixed" runs the active-set search on the matrix stored in
+    FP32 and solves the final passive set again in FP64 (cells it cannot
+    confirm run "lds").  Returns the per-cell status as merge_nnls_based."""
     mim = np.ascontiguousarray(np.asarray(multi_index_moments, dtype=np.int32).reshape(-1, 3))
     vm = np.ascontiguousarray(v_multipliers, dtype=np.float64)
     status = np.zeros(store.n_cells, dtype=np.int32)

@@ -207,11 +207,15 @@ inline MzSpView mz_view(const MzSpecies &s) {
 
 // ---------------------------------------------------------------------------
 // wave64 helpers
-__device__ inline double mz_wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+// Wavefront reductions over the active lanes, every lane receiving the
+// result: the ROCm device library's DPP row operations (a few VALU cycles
+// per step) rather than LDS permutes.  Callers keep all lanes active.
+extern "C" __device__ double __ockl_wfred_add_f64(double);
+extern "C" __device__ double __ockl_wfred_max_f64(double);
+extern "C" __device__ double __ockl_wfred_min_f64(double);
+__device__ inline double mz_wave_sum(double v) { return __ockl_wfred_add_f64(v); }
+__device__ inline double mz_wave_max(double v) { return __ockl_wfred_max_f64(v); }
+__device__ inline double mz_wave_min(double v) { return __ockl_wfred_min_f64(v); }
 __device__ inline int mz_wave_sum_i(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

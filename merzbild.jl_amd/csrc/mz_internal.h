@@ -89,6 +89,7 @@ struct mz_store {
     // {items, max cells per item, target particles per item, workgroups}
     int32_t b0_max_cells = 0, b0_target = 0;
     int32_t b0_info[4] = {0, 0, 0, 0};
+    int32_t nnls_info[2] = {0, 0};   // last candidate NNLS merge: cells run, cells re-run in FP64
 };
 
 MzViews mz_views(mz_store *h);

@@ -1105,21 +1105,6 @@ __global__ void __launch_bounds__(256) k_nnls_merge(NnlsArgs A) {
 // rounding (sums are taken in another order), not bit for bit.
 #define NF_COLS 256
 
-__device__ inline double nf_wsum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ inline double nf_wmax(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ inline double nf_wmin(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
 
 // The reference's (fictitious) column f >= 0 after the particles: the
 // centre particle (all-zero column) and the octant particles
@@ -1145,13 +1130,22 @@ __device__ inline bool nf_fict(const NnlsArgs &A, int f, const double *e3, const
     return false;
 }
 
-template <int ROWS>
-__global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsArgs A) {
-    __shared__ double Al[ROWS * NF_COLS];        // scaled moment matrix [row][column]
+// T is the storage type of the scaled moment matrix and of the solver's x
+// and dual vectors; the arithmetic is FP64.  With T = float the active-set
+// search runs on the rounded matrix and nf_polish re-solves the final
+// passive set in FP64 (status 2 asks the host for the FP64 method).
+template <int ROWS, typename T>
+__global__ void __launch_bounds__(NF_COLS, sizeof(T) == 4 ? 4 : (ROWS <= 36 ? 2 : 1)) k_nnls_fast(NnlsArgs A) {
+    constexpr bool MIXED = sizeof(T) == 4;
+    // the matrix area also holds the moment partial sums before the matrix is
+    // built, and the FP64 passive-set system after the search
+    __shared__ __align__(16) unsigned char lds_mat[ROWS * NF_COLS * sizeof(T)];
+    static_assert(4 * (ROWS + 8) * sizeof(double) <= sizeof(lds_mat), "moment partials fit the matrix area");
+    T *Al = reinterpret_cast<T *>(lds_mat);     // scaled moment matrix [row][column]
+    double *red = reinterpret_cast<double *>(lds_mat);   // [wave][ROWS + 8]
     __shared__ double bq[ROWS], zz[ROWS];
-    __shared__ double xs[NF_COLS], ws[NF_COLS];
-    __shared__ int32_t idx[NF_COLS];
-    __shared__ double red[4][ROWS + 8];
+    __shared__ T xs[NF_COLS], ws[NF_COLS];
+    __shared__ int16_t idx[NF_COLS];
     __shared__ double sd[24];
     __shared__ int32_t si[16];
     __shared__ double rw[4];
@@ -1165,6 +1159,7 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
     const int n_add = (A.centered ? 1 : 0) + 8 * A.n_vmult;
     const int n = Np + n_add;
 #define AL(r, cc) Al[((r) - 1) * NF_COLS + ((cc) - 1)]
+#define RED(w, k) red[(w) * (ROWS + 8) + (k)]
     // ---- compute_w_total_v0!: W, mean velocity and position, velocity bounds
     {
         double W = 0.0, sv[3] = {0, 0, 0}, sx[3] = {0, 0, 0};
@@ -1185,16 +1180,16 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
         double t[13] = {W, sv[0], sv[1], sv[2], sx[0], sx[1], sx[2], mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]};
 #pragma unroll
         for (int k = 0; k < 13; ++k) {
-            const double r = k < 7 ? nf_wsum(t[k]) : (k < 10 ? nf_wmin(t[k]) : nf_wmax(t[k]));
-            if (lane == 0) red[wave][k] = r;
+            const double r = k < 7 ? mz_wave_sum(t[k]) : (k < 10 ? mz_wave_min(t[k]) : mz_wave_max(t[k]));
+            if (lane == 0) RED(wave, k) = r;
         }
         __syncthreads();
         if (tid < 13) {
             const int k = tid;
             double r;
-            if (k < 7) r = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
-            else if (k < 10) r = fmin(fmin(red[0][k], red[1][k]), fmin(red[2][k], red[3][k]));
-            else r = fmax(fmax(red[0][k], red[1][k]), fmax(red[2][k], red[3][k]));
+            if (k < 7) r = ((RED(0, k) + RED(1, k)) + RED(2, k)) + RED(3, k);
+            else if (k < 10) r = fmin(fmin(RED(0, k), RED(1, k)), fmin(RED(2, k), RED(3, k)));
+            else r = fmax(fmax(RED(0, k), RED(1, k)), fmax(RED(2, k), RED(3, k)));
             sd[k] = r;
         }
         __syncthreads();
@@ -1233,30 +1228,30 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const double r = nf_wsum(e[k]);
-            if (lane == 0) red[wave][k] = r;
+            const double r = mz_wave_sum(e[k]);
+            if (lane == 0) RED(wave, k) = r;
         }
 #pragma unroll
         for (int q = 0; q < ROWS; ++q) {
             if (q < m) {
-                const double r = nf_wsum(acc[q]);
-                if (lane == 0) red[wave][3 + q] = r;
+                const double r = mz_wave_sum(acc[q]);
+                if (lane == 0) RED(wave, 3 + q) = r;
             }
         }
         __syncthreads();
         if (tid < 3) {
-            const double E = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+            const double E = ((RED(0, tid) + RED(1, tid)) + RED(2, tid)) + RED(3, tid);
             sd[13 + tid] = sqrt(E / W);
         }
         if (tid >= 32 && tid < 32 + m) {
             const int q = tid - 32;
-            zz[q] = (((red[0][3 + q] + red[1][3 + q]) + red[2][3 + q]) + red[3][3 + q]) / W;
+            zz[q] = (((RED(0, 3 + q) + RED(1, 3 + q)) + RED(2, 3 + q)) + RED(3, 3 + q)) / W;
         }
         __syncthreads();
     }
     const double Ex = sd[13], Ey = sd[14], Ez = sd[15];
     // ---- row scaling (scale_lhs_rhs_variance! / _vref!, :1178-1215); zz holds the moments
-    __shared__ double rowscale[ROWS];
+    __shared__ double rowscale[ROWS], b0[ROWS];
     if (tid < m) {
         double rv;
         if (!A.scaling_vref) {
@@ -1269,16 +1264,17 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
         }
         rowscale[tid] = rv;
         bq[tid] = zz[tid] * rv;
+        b0[tid] = bq[tid];
     }
     __syncthreads();
     // ---- column j on thread j: moments of the candidate, row-scaled, unit norm
     // (scale_columns!, utils.jl:62-77); the candidate's velocity stays in
     // registers for the post-merge write
     double cvx = 0.0, cvy = 0.0, cvz = 0.0, cinv = 1.0;
+    double cx = 0.0, cy = 0.0, cz = 0.0;   // relative to the mean velocity
+    bool zero_col = false;
     if (tid < n) {
         const int j = tid;
-        double cx, cy, cz;
-        bool zero_col = false;
         if (j < Np) {
             const int64_t p = oct_slot(j, s1, n1, s2);
             cvx = A.sp.v0[p]; cvy = A.sp.v1[p]; cvz = A.sp.v2[p];
@@ -1314,38 +1310,60 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
     if (tid == 0) {
         si[0] = 1; si[1] = n; si[2] = 0; si[3] = 0; si[4] = 0; si[5] = 0;
         si[7] = 1; si[8] = 0; si[9] = 0;
+        si[14] = 0;   // 1: the duals of the zero set are current
         sd[16] = 0.0;
         sd[17] = 0.0;
     }
     __syncthreads();
     // back substitution (solve_triangular_system!) on wave 0, lane r = row r + 1:
     // zz <- R^-1 zz over the nsetp columns of P
+    // Lane k holds the column at P position k + 1 and the reciprocal of its
+    // diagonal entry; each step's column of R is loaded one step ahead.
     auto back_subst = [&]() {
         const int nsetp = si[2];
-        int jj = si[4];
+        const int colk = lane < nsetp ? idx[lane] : 1;
+        const double rdk = lane < nsetp ? 1.0 / AL(lane + 1, colk) : 0.0;
         double zr = lane < m ? zz[lane] : 0.0;
+        // the column of R the step at position ip subtracts (rows above ip)
+        auto rcol = [&](int ip) {
+            return (ip >= 1 && lane < ip) ? AL(lane + 1, __builtin_amdgcn_readlane(colk, ip)) : 0.0;
+        };
+        double p1 = rcol(nsetp - 1), p2 = rcol(nsetp - 2);
         for (int l = 1; l <= nsetp; ++l) {
             const int ip = nsetp + 1 - l;
             if (l != 1) {
                 const double zip = mz_readlane_d(zr, ip);
-                if (lane < ip) zr = zr - AL(lane + 1, jj) * zip;
+                if (lane < ip) zr = zr - p1 * zip;
+                p1 = p2;
+                p2 = rcol(ip - 2);
             }
-            jj = idx[ip - 1];
-            if (lane == ip - 1) zr = zr / AL(ip, jj);
+            if (lane == ip - 1) zr = zr * rdk;
         }
         if (lane < m) zz[lane] = zr;
-        if (lane == 0) si[4] = jj;
+        if (lane == 0 && nsetp >= 1) si[4] = __builtin_amdgcn_readlane(colk, 0);
+    };
+    // sum over rows lo..m of column c times bq, in four partial sums
+    auto dual_of = [&](int c, int lo) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int l = lo;
+        for (; l + 3 <= m; l += 4) {
+            s0 = s0 + AL(l, c) * bq[l - 1];
+            s1 = s1 + AL(l + 1, c) * bq[l];
+            s2 = s2 + AL(l + 2, c) * bq[l + 1];
+            s3 = s3 + AL(l + 3, c) * bq[l + 2];
+        }
+        for (; l <= m; ++l) s0 = s0 + AL(l, c) * bq[l - 1];
+        return (s0 + s1) + (s2 + s3);
     };
     for (;;) {
         if (si[0] > si[1] || si[2] >= m) break;
-        // dual vector over the zero set
-        {
+        // dual vector over the zero set (already current after a column
+        // transform that removed nothing)
+        if (!si[14]) {
             const int iz1 = si[0], iz2 = si[1], nsetp = si[2];
             for (int i = iz1 + tid; i <= iz2; i += NF_COLS) {
                 const int idxi = idx[i - 1];
-                double sm = 0.0;
-                for (int l = nsetp + 1; l <= m; ++l) sm = sm + AL(l, idxi) * bq[l - 1];
-                ws[idxi - 1] = sm;
+                ws[idxi - 1] = dual_of(idxi, nsetp + 1);
             }
         }
         __syncthreads();
@@ -1376,15 +1394,15 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
                 const double asave = mz_readlane_d(a, nsetp);
                 double up = sd[16], top = asave;
                 if (len > 1) {
-                    double cl = nf_wmax(hh ? fabs(a) : 0.0);
+                    double cl = mz_wave_max(hh ? fabs(a) : 0.0);
                     const double clinv = 1.0 / cl;
                     const double t = hh ? a * clinv : 0.0;
-                    cl *= sqrt(nf_wsum(t * t));
+                    cl *= sqrt(mz_wave_sum(t * t));
                     if (asave > 0) cl = -cl;
                     up = asave - cl;
                     top = cl;
                 }
-                const double unorm = sqrt(nf_wsum(lane < nsetp ? a * a : 0.0));
+                const double unorm = sqrt(mz_wave_sum(lane < nsetp ? a * a : 0.0));
                 int accept = 0;
                 double zr = lane < m ? bq[lane] : 0.0;
                 if (((unorm + fabs(top) * factor) - unorm) > 0) {
@@ -1393,7 +1411,7 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
                         if (bb < 0) {
                             bb = 1.0 / bb;
                             const double ur = lane == nsetp ? up : (hh ? a : 0.0);
-                            double sm = nf_wsum(zr * ur);
+                            double sm = mz_wave_sum(zr * ur);
                             if (sm != 0) {
                                 sm *= bb;
                                 if (hh) zr = zr + sm * ur;
@@ -1440,17 +1458,27 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
                     double bb = up * AL(nsetp, j);
                     if (bb < 0) {
                         bb = 1.0 / bb;
-                        double sm = AL(nsetp, jj) * up;
-                        for (int l = 2; l <= len; ++l) sm = sm + AL(nsetp + l - 1, jj) * AL(nsetp + l - 1, j);
+                        double s0 = AL(nsetp, jj) * up, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+                        int l = 2;
+                        for (; l + 2 <= len; l += 3) {
+                            s1 = s1 + AL(nsetp + l - 1, jj) * AL(nsetp + l - 1, j);
+                            s2 = s2 + AL(nsetp + l, jj) * AL(nsetp + l, j);
+                            s3 = s3 + AL(nsetp + l + 1, jj) * AL(nsetp + l + 1, j);
+                        }
+                        for (; l <= len; ++l) s1 = s1 + AL(nsetp + l - 1, jj) * AL(nsetp + l - 1, j);
+                        double sm = (s0 + s1) + (s2 + s3);
                         if (sm != 0) {
                             sm *= bb;
                             AL(nsetp, jj) = AL(nsetp, jj) + sm * up;
-                            for (int l = 2; l <= len; ++l)
-                                AL(nsetp + l - 1, jj) = AL(nsetp + l - 1, jj) + sm * AL(nsetp + l - 1, j);
+                            for (int l2 = 2; l2 <= len; ++l2)
+                                AL(nsetp + l2 - 1, jj) = AL(nsetp + l2 - 1, jj) + sm * AL(nsetp + l2 - 1, j);
                         }
                     }
                 }
+                // the column's dual for the next pass (b is already the new one)
+                ws[jj - 1] = dual_of(jj, nsetp + 1);
             }
+            if (tid == 0) si[14] = 1;
         }
         __syncthreads();
         {
@@ -1481,7 +1509,7 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
                     }
                     const bool cand = t < 2.0;
                     double tv = cand ? t : 2.0;
-                    const double alpha = nf_wmin(tv);
+                    const double alpha = mz_wave_min(tv);
                     const uint64_t hit = __ballot(cand && t == alpha);
                     if (!hit) {
                         flag = 1;
@@ -1510,6 +1538,7 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
                     xs[si[9] - 1] = 0;
                     si[12] = si[4] != si[2];
                     if (si[12]) si[4] += 1;   // jj += 1 persists across passes, as in nnls.jl:449-451
+                    si[14] = 0;
                 }
                 __syncthreads();
                 if (si[12]) {
@@ -1585,19 +1614,118 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
     // ---- compute_post_merge_particles_nnls! (:1334-1404)
     __shared__ int32_t wcnt[4];
     __shared__ double wsum[4];
-    if (wave == 0) {
-        const int nsetp = si[2];
-        const double r = (lane >= nsetp && lane < m) ? bq[lane] : 0.0;
-        const double rnorm = sqrt(nf_wsum(r * r));
-        if (lane == 0) si[13] = !(rnorm > A.max_err);
+    double xj = 0.0;
+    if constexpr (MIXED) {
+        // The search's passive set P solved again in FP64 from the exact
+        // columns: normal equations G y = A_P^T b by Cholesky, one refinement
+        // step with the FP64 residual; the merge stands if y > 0 and the
+        // residual is within max_err, otherwise the cell goes to the FP64
+        // method (status 2).
+        __shared__ double gy[ROWS + 1], gr[ROWS + 1];
+        constexpr int NPP = ROWS + 1;
+        static_assert((ROWS * NPP + NPP * NPP) * sizeof(double) <= sizeof(lds_mat), "system fits");
+        double *AP = reinterpret_cast<double *>(lds_mat);   // [row][NPP]
+        double *G = AP + ROWS * NPP;                          // [NPP][NPP]
+        int32_t *ppos = reinterpret_cast<int32_t *>(ws);    // column -> position in P
+        const int p = si[2];
+        if (tid < n) ppos[tid] = -1;
+        __syncthreads();
+        if (tid < p) ppos[idx[tid] - 1] = tid;
+        __syncthreads();
+        const int pc = tid < n ? ppos[tid] : -1;
+        if (pc >= 0) {
+            for (int q = 0; q < m; ++q) {
+                const double t = zero_col ? 0.0
+                                          : nnls_ipow(cx, A.mim[q][0]) * nnls_ipow(cy, A.mim[q][1]) *
+                                                nnls_ipow(cz, A.mim[q][2]);
+                AP[q * NPP + pc] = (t * rowscale[q]) * cinv;
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < p * p; e += NF_COLS) {
+            const int i = e / p, jx = e % p;
+            if (jx < i) continue;
+            double sm = 0.0;
+            for (int q = 0; q < m; ++q) sm = sm + AP[q * NPP + i] * AP[q * NPP + jx];
+            G[i * NPP + jx] = sm;
+            G[jx * NPP + i] = sm;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            // Cholesky, lane = row, lower triangle in place
+            bool spd = true;
+            for (int k = 0; k < p; ++k) {
+                const double dkk = G[k * NPP + k];
+                if (!(dkk > 0.0)) spd = false;
+                const double d = sqrt(dkk);
+                if (lane == k) G[k * NPP + k] = d;
+                if (lane > k && lane < p) {
+                    const double lik = G[lane * NPP + k] / d;
+                    G[lane * NPP + k] = lik;
+                    for (int jx = k + 1; jx <= lane; ++jx) G[lane * NPP + jx] -= lik * G[jx * NPP + k];
+                }
+            }
+            // G y = A_P^T r by the two triangular solves; r starts as b
+            auto solve = [&](double rhs) {   // lane t < p holds rhs_t, returns the solution component
+                double zr = lane < p ? rhs : 0.0;
+                for (int k = 0; k < p; ++k) {
+                    const double zk = mz_readlane_d(zr, k) / G[k * NPP + k];
+                    if (lane == k) zr = zk;
+                    if (lane > k && lane < p) zr -= G[lane * NPP + k] * zk;
+                }
+                for (int k = p - 1; k >= 0; --k) {
+                    const double yk = mz_readlane_d(zr, k) / G[k * NPP + k];
+                    if (lane == k) zr = yk;
+                    if (lane < k) zr -= G[k * NPP + lane] * yk;
+                }
+                return zr;
+            };
+            // residual b - A_P y (lane = row) and A_P^T of it (lane = column of P)
+            auto resid = [&](double yl, double &rq) {
+                if (lane < p) gy[lane] = yl;
+                rq = 0.0;
+                if (lane < m) {
+                    double sm = b0[lane];
+                    for (int t = 0; t < p; ++t) sm = sm - AP[lane * NPP + t] * gy[t];
+                    rq = sm;
+                    gr[lane] = sm;
+                }
+                double g = 0.0;
+                if (lane < p)
+                    for (int q = 0; q < m; ++q) g = g + AP[q * NPP + lane] * gr[q];
+                return g;
+            };
+            double rq;
+            double y = 0.0;
+            if (spd) {
+                y = solve(resid(0.0, rq));
+                const double dy = solve(resid(y, rq));
+                y = y + dy;
+            }
+            (void)resid(y, rq);
+            const double rnorm = sqrt(mz_wave_sum(rq * rq));
+            const bool pos = __ballot(lane < p && !(y > 0.0)) == 0;
+            if (lane < p) gy[lane] = y;
+            if (lane == 0) si[13] = (spd && pos && !(rnorm > A.max_err)) ? 1 : 2;
+        }
+        __syncthreads();
+        if (pc >= 0) xj = gy[pc] * cinv;
+    } else {
+        if (wave == 0) {
+            const int nsetp = si[2];
+            const double r = (lane >= nsetp && lane < m) ? bq[lane] : 0.0;
+            const double rnorm = sqrt(mz_wave_sum(r * r));
+            if (lane == 0) si[13] = !(rnorm > A.max_err);
+        }
+        __syncthreads();
+        xj = tid < n ? (double)xs[tid] * cinv : 0.0;
     }
-    __syncthreads();
-    int ok = si[13];
-    const double xj = tid < n ? xs[tid] * cinv : 0.0;
+    int ok = si[13] == 1;
+    const bool retry = si[13] == 2;
     const bool keep = ok && tid < n && xj > A.w_threshold;
     const uint64_t km = __ballot(keep);
     {
-        const double ksum = nf_wsum(keep ? xj : 0.0);
+        const double ksum = mz_wave_sum(keep ? xj : 0.0);
         if (lane == 0) { wcnt[wave] = __popcll(km); wsum[wave] = ksum; }
     }
     __syncthreads();
@@ -1605,6 +1733,7 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
     const double ndw = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
     if (nonzero >= N) ok = 0;
     const int32_t n_new = ok ? nonzero : N;
+    const bool redo = MIXED && !ok && (retry || nonzero >= N);   // the FP64 method decides
     if (ok) {
         int base = 0;
         for (int k = 0; k < wave; ++k) base += wcnt[k];
@@ -1617,7 +1746,8 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
         }
         for (int li = n_new + tid; li < N; li += NF_COLS) A.sp.w[oct_slot(li, s1, n1, s2)] = 0.0;
     }
-    if (tid == 0) {
+    if (tid == 0 && redo) A.status[c] = 2;
+    if (tid == 0 && !redo) {
         A.status[c] = ok ? 1 : -1;
         int32_t *op = A.old_pia + c * 5;
         op[0] = s1; op[1] = n1; op[2] = s2; op[3] = n2; op[4] = ok ? n_new : -1;
@@ -1635,6 +1765,7 @@ __global__ void __launch_bounds__(NF_COLS, ROWS <= 36 ? 2 : 1) k_nnls_fast(NnlsA
             if (!(c == A.n_cells - 1) || (n_del > n2)) A.sp.ctr->contiguous = 0;
         }
     }
+#undef RED
 #undef AL
 }
 
@@ -1645,7 +1776,8 @@ static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, cons
                            int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
                            int32_t squash, int32_t max_pcols, int32_t method, int32_t *status) {
     MZ_CHECK(species >= 1 && species <= h->n_species, "merge_nnls: bad species");
-    MZ_CHECK(method == 0 || method == 1, "merge_nnls: method must be 0 (reference layout) or 1 (LDS-resident)");
+    MZ_CHECK(method >= 0 && method <= 2,
+             "merge_nnls: method must be 0 (reference layout), 1 (LDS-resident) or 2 (FP32 search, FP64 solve)");
     MZ_CHECK(n_vmult >= 0 && n_vmult <= NNLS_MAX_VMULT, "merge_nnls: too many v_multipliers");
     MZ_CHECK(n_mim >= 0, "merge_nnls: bad multi-index count");
     MzSpecies &s = h->sp[species - 1];
@@ -1686,10 +1818,12 @@ static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, cons
     A.tmp.w = s.wb;
     A.n_cells = nc;
     A.max_pcols = max_pcols < 0 ? -1 : max_pcols;
-    if (method == 1) {
+    if (method >= 1) {
         MZ_CHECK(max_pcols >= 0 && max_pcols + A.centered + 8 * n_vmult <= NF_COLS,
-                 "merge_nnls: the LDS-resident method needs max_pcols + fictitious columns <= 256");
+                 "merge_nnls: the LDS-resident methods need max_pcols + fictitious columns <= 256");
     }
+    h->nnls_info[0] = 0;
+    h->nnls_info[1] = 0;
     // cells to merge and their scratch sizes (host side)
     MzCounters snap;
     int r = mz_read_counters(h, species, &snap);
@@ -1721,16 +1855,42 @@ static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, cons
         MZ_HIP(hipMemcpy(old_pia, init.data(), init.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
     mz_timing_begin(h, "merge_nnls");
-    if (!cells.empty() && method == 1) {
+    if (!cells.empty() && method >= 1) {
         int32_t *dcells = (int32_t *)mz_scratch(h, "nnls_cells", cells.size() * sizeof(int32_t));
         MZ_CHECK(dcells, "merge_nnls: scratch allocation failed");
         MZ_HIP(hipMemcpy(dcells, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        A.cells = dcells;
         A.status = d_status;
         A.old_pia = old_pia;
-        if (m <= 35) hipLaunchKernelGGL(k_nnls_fast<35>, dim3((unsigned)cells.size()), dim3(NF_COLS), 0, h->stream, A);
-        else hipLaunchKernelGGL(k_nnls_fast<NNLS_MAX_ROWS>, dim3((unsigned)cells.size()), dim3(NF_COLS), 0, h->stream, A);
-        MZ_HIP(hipGetLastError());
+        auto launch = [&](bool mixed, const int32_t *list, size_t count) {
+            A.cells = list;
+            const dim3 g((unsigned)count), b(NF_COLS);
+            if (m <= 35) {
+                if (mixed) hipLaunchKernelGGL((k_nnls_fast<35, float>), g, b, 0, h->stream, A);
+                else hipLaunchKernelGGL((k_nnls_fast<35, double>), g, b, 0, h->stream, A);
+            } else {   // more than 35 moments: FP64 only (the FP32 matrix area cannot hold the FP64 system)
+                hipLaunchKernelGGL((k_nnls_fast<NNLS_MAX_ROWS, double>), g, b, 0, h->stream, A);
+            }
+            return hipGetLastError();
+        };
+        const bool mixed = method == 2 && m <= 35;
+        MZ_HIP(launch(mixed, dcells, cells.size()));
+        h->nnls_info[0] = (int32_t)cells.size();
+        if (mixed) {
+            // cells the FP64 re-solve did not confirm run the FP64 method
+            std::vector<int32_t> st(nc);
+            MZ_HIP(hipStreamSynchronize(h->stream));
+            MZ_HIP(hipMemcpy(st.data(), d_status, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
+            std::vector<int32_t> redo;
+            for (int32_t cc : cells)
+                if (st[cc] == 2) redo.push_back(cc);
+            h->nnls_info[1] = (int32_t)redo.size();
+            if (!redo.empty()) {
+                int32_t *dredo = (int32_t *)mz_scratch(h, "nnls_cells_fp64", redo.size() * sizeof(int32_t));
+                MZ_CHECK(dredo, "merge_nnls: scratch allocation failed");
+                MZ_HIP(hipMemcpy(dredo, redo.data(), redo.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+                MZ_HIP(launch(false, dredo, redo.size()));
+            }
+        }
     } else if (!cells.empty()) {
         // batches bounded to ~1 GiB of matrix scratch
         const int64_t budget = (int64_t)1 << 27;   // doubles
@@ -1778,6 +1938,13 @@ extern "C" int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, co
                              int32_t squash, int32_t *status) {
     return nnls_merge_impl(h, species, threshold, mim, n_mim, vref, scaling_vref, centered_at_mean, v_multipliers,
                            n_vmult, max_err, iteration_mult, w_threshold, squash, -1, 0, status);
+}
+
+extern "C" int mz_get_nnls_info(mz_store *h, int32_t *info) {
+    MZ_CHECK(info != nullptr, "nnls info: NULL output");
+    info[0] = h->nnls_info[0];
+    info[1] = h->nnls_info[1];
+    return 0;
 }
 
 extern "C" int mz_merge_nnls_cand(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim,

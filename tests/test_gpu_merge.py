@@ -307,3 +307,51 @@ def test_nnls_candidates_lds_vs_oracle(hip, mzo, scaling):
         m1, _ = _cell_moments(w, v, rows)
         assert np.max(np.abs(m1 - m0) / scale) < 1e-12, c
     st.close()
+
+
+@pytest.mark.parametrize("scaling", ["variance", "vref"])
+def test_nnls_candidates_mixed_vs_oracle(hip, mzo, scaling):
+    """The mixed method (active-set search on the FP32-stored matrix, final
+    passive set re-solved in FP64, unconfirmed cells re-run in FP64): every
+    cell's status equals the oracle's, and every merged cell preserves all 35
+    moments to 1e-12 of their scale; failed and skipped cells are untouched.
+    The merged particle set may differ from the oracle's (another non-negative
+    solution of the same moment system)."""
+    rng = np.random.default_rng(24)
+    counts = [int(k) for k in rng.integers(40, 800, size=24)]
+    counts[4] = 25
+    nc, n = len(counts), sum(counts)
+    pv, pia = _bimaxwell_cells(mzo, counts, rng)
+    st = hip.DeviceStore(nc, 1, n + 256)
+    helpers.upload_from_oracle(st, mzo, pv, pia)
+    pre = st.download_species(1)
+    vref = np.sqrt(2 * helpers.KB * 300.0 / helpers.MASS_AR)
+    mim = _mim(4)
+    status = hip.merge_nnls_candidates(st, 1, 30, mim, 239, method="mixed", vref=vref, scaling=scaling,
+                                       squash=False)
+    tried, redone = st.nnls_info()
+    assert tried == sum(k > 30 for k in counts) and 0 <= redone <= tried
+    nm = mzo.NNLSMerge(mim)
+    nm.s.reset_x0 = 1
+    ref_status = [nm.merge(mzo.RNG(seed=1), pv, pia, c, 1, vref=vref, scaling=scaling, max_pcols=239)
+                  if pia.ind(c).n_local > 30 else 0 for c in range(1, nc + 1)]
+    assert list(status) == ref_status
+    assert sum(s == 1 for s in ref_status) >= 18
+    got = st.download_species(1)
+    assert got["err"] == 0
+    rows = _nnls_rows()
+    for c in range(nc):
+        r0, r1 = pre["pia"][c], got["pia"][c]
+        a0, b0 = r0[1] - 1, r0[2]
+        w_pre, v_pre = pre["w"][a0:b0], pre["v"][a0:b0]
+        a1, b1 = r1[1] - 1, r1[2]
+        w, v = got["w"][a1:b1], got["v"][a1:b1]
+        if status[c] != 1:
+            assert list(r1) == list(r0)
+            assert np.array_equal(w, w_pre) and np.array_equal(v, v_pre)
+            continue
+        assert r1[0] < r0[0] and np.all(w > 0)
+        m0, scale = _cell_moments(w_pre, v_pre, rows)
+        m1, _ = _cell_moments(w, v, rows)
+        assert np.max(np.abs(m1 - m0) / scale) < 1e-12, c
+    st.close()
