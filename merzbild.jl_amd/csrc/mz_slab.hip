@@ -405,27 +405,46 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
             }
         }
         __syncthreads();
-        // ---- 3. per key, exclusive prefix over chunks
-        for (int kk = wave; kk < K; kk += 4) {
-            int32_t v = lane < NCH ? tab[lane][kk] : 0;
-            int32_t x = v;
+        // ---- 3. per key, exclusive prefix over chunks.  With at most 8
+        // chunks each thread adds up the earlier chunks of its own keys in the
+        // next phase and wave 0 forms the totals there, without a phase of
+        // its own.
+        constexpr bool SMALL_SCAN = NCH <= 8;
+        if (!SMALL_SCAN) {
+            for (int kk = wave; kk < K; kk += 4) {
+                int32_t v = lane < NCH ? tab[lane][kk] : 0;
+                int32_t x = v;
 #pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                int32_t y = __shfl_up(x, o, 64);
-                if (lane >= o) x += y;
+                for (int o = 1; o < 64; o <<= 1) {
+                    int32_t y = __shfl_up(x, o, 64);
+                    if (lane >= o) x += y;
+                }
+                if (lane < NCH) tab[lane][kk] = x - v;
+                if (lane == 63) tot[kk] = x;
             }
-            if (lane < NCH) tab[lane][kk] = x - v;
-            if (lane == 63) tot[kk] = x;
+            __syncthreads();
+        } else if (wave == 0 && lane < K) {
+            int32_t t = 0;
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) t += tab[ch][lane];
+            tot[lane] = t;
         }
-        __syncthreads();
-        // ---- 4. ranks in the slab's previous order; leavers go to the mailboxes
-        const bool bad = s_abort || tot[KL] > SLAB_MB_CAP || tot[KR] > SLAB_MB_CAP;
-        if (!bad) {
+        // ---- 4. ranks in the slab's previous order; leavers go to the
+        // mailboxes (a rank past the mailbox capacity aborts below)
+        const bool bad0 = !SMALL_SCAN && (s_abort || tot[KL] > SLAB_MB_CAP || tot[KR] > SLAB_MB_CAP);
+        if (!bad0) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if (pkey[r] < 0) continue;
-                prank[r] += tab[r * 4 + wave][pkey[r]];
-                if (pkey[r] >= KL) {
+                if (SMALL_SCAN) {
+                    const int ch = r * 4 + wave;
+                    int32_t base = 0;
+                    for (int c2 = 0; c2 < ch; ++c2) base += tab[c2][pkey[r]];
+                    prank[r] += base;
+                } else {
+                    prank[r] += tab[r * 4 + wave][pkey[r]];
+                }
+                if (pkey[r] >= KL && prank[r] < SLAB_MB_CAP) {
                     const int side = pkey[r] == KL ? 0 : 1;
                     const int j = prank[r];
                     st_agent(slab_mb(A, g, side, q, 0) + j, (uint64_t)__double_as_longlong(px0[r]));
@@ -441,6 +460,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
         }
         slab_wait_mem();
         __syncthreads();
+        const bool bad = bad0 || s_abort || tot[KL] > SLAB_MB_CAP || tot[KR] > SLAB_MB_CAP;
         if (bad) {
             if (tid == 0) slab_raise_abort(A);
             aborted = true;
