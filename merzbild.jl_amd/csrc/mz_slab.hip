@@ -480,39 +480,33 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
             if (has_left) st_agent(A.flag + ((size_t)g * 2 + 0) * SLAB_LINE + q, (mark << 32) | (uint32_t)tot[KL]);
             if (has_right) st_agent(A.flag + ((size_t)g * 2 + 1) * SLAB_LINE + q, (mark << 32) | (uint32_t)tot[KR]);
         }
-        if (lane == 0 && wave < 2) {
-            const bool has = wave == 0 ? has_left : has_right;
-            int32_t na = 0;
-            if (has) {
-                const int src = wave == 0 ? g - 1 : g + 1;
-                const int side = wave == 0 ? 1 : 0;
-                const uint64_t *f = A.flag + ((size_t)src * 2 + side) * SLAB_LINE + q;
-                uint64_t v = 0;
-                if (slab_wait(f, mark, 32, A.abort_flag, v)) {
-                    na = (int32_t)(uint32_t)(v & 0xFFFFFFFFu);
-                    if (na < 0 || na > SLAB_MB_CAP) {
-                        s_abort = 1;
-                        na = 0;
-                    }
-                } else {
-                    s_abort = 1;
-                }
-            }
-            s_na[wave] = na;
-        }
-        if (wave >= 2 && t + 1 < A.n_steps) fill_draws(step + 1, tid - 128, SLAB_THREADS - 128);
-        __syncthreads();
-        if (s_abort) {
-            if (tid == 0) slab_raise_abort(A);
-            aborted = true;
-            break;
-        }
-        // ---- 6. rank the arrivals by cell (wave 0: from the left, wave 1: from
-        // the right); the first 64 of each side stay in registers until the scatter
+        // ---- 6. as soon as its neighbour's word shows this step, wave 0 (left)
+        // or 1 (right) ranks that side's arrivals by cell; the first 64 of
+        // each side stay in registers until the scatter
         uint64_t aw[8];
         int a_key = 0, a_rank = 0;
         if (wave < 2) {
-            const int na = s_na[wave];
+            int32_t na = 0;
+            if (lane == 0) {
+                const bool has = wave == 0 ? has_left : has_right;
+                if (has) {
+                    const int src = wave == 0 ? g - 1 : g + 1;
+                    const int side = wave == 0 ? 1 : 0;
+                    const uint64_t *f = A.flag + ((size_t)src * 2 + side) * SLAB_LINE + q;
+                    uint64_t v = 0;
+                    if (slab_wait(f, mark, 32, A.abort_flag, v)) {
+                        na = (int32_t)(uint32_t)(v & 0xFFFFFFFFu);
+                        if (na < 0 || na > SLAB_MB_CAP) {
+                            s_abort = 1;
+                            na = 0;
+                        }
+                    } else {
+                        s_abort = 1;
+                    }
+                }
+                s_na[wave] = na;
+            }
+            na = __shfl(na, 0, 64);
             const int src = wave == 0 ? g - 1 : g + 1;
             const int side = wave == 0 ? 1 : 0;
             volatile int32_t *run = wave == 0 ? runL : runR;
@@ -551,9 +545,26 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
                 }
             }
         }
+        if (wave >= 2 && t + 1 < A.n_steps) fill_draws(step + 1, tid - 128, SLAB_THREADS - 128);
         __syncthreads();
-        // ---- 7. new cell starts: [left arrivals | own | right arrivals] per cell
-        if (wave == 0) {
+        if (s_abort) {
+            if (tid == 0) slab_raise_abort(A);
+            aborted = true;
+            break;
+        }
+        // ---- 7. new cell starts: [left arrivals | own | right arrivals] per
+        // cell.  A slab of at most 4 cells forms them in the scatter below,
+        // each thread for its own cells.
+        const bool small_bases = ncl <= 4;
+        if (small_bases) {
+            int32_t x = 0;
+            for (int k = 0; k < ncl; ++k) {
+                const int32_t cnum = runL[k] + tot[k] + runR[k];
+                if (wave == 0 && lane == k) { cst[k] = x; ccn[k] = cnum; }
+                x += cnum;
+            }
+            n_loc = x;
+        } else if (wave == 0) {
             int32_t cnum = lane < ncl ? runL[lane] + tot[lane] + runR[lane] : 0;
             int32_t x = cnum;
 #pragma unroll
@@ -567,19 +578,29 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
                 if (x > P) s_abort = 1;
             }
         }
-        __syncthreads();
+        if (!small_bases) {
+            __syncthreads();
+            n_loc = s_n;
+        }
+        if (n_loc > P) s_abort = 1;   // same value on every thread
         if (s_abort) {
             if (tid == 0) slab_raise_abort(A);
             aborted = true;
             break;
         }
-        n_loc = s_n;
+        // the start of cell k (its cst entry, formed here when small_bases)
+        auto cstart = [&](int k) {
+            if (!small_bases) return (int32_t)cst[k];
+            int32_t x = 0;
+            for (int k2 = 0; k2 < k; ++k2) x += runL[k2] + tot[k2] + runR[k2];
+            return x;
+        };
         // ---- 8. scatter into the new order
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int key = pkey[r];
             if (key < 0 || key >= ncl) continue;
-            const int pos = cst[key] + runL[key] + prank[r];
+            const int pos = cstart(key) + runL[key] + prank[r];
             sx0[pos] = px0[r]; sx1[pos] = px1[r]; sx2[pos] = px2[r];
             sv0[pos] = pv0[r]; sv1[pos] = pv1[r]; sv2[pos] = pv2[r];
             if (SNAP) { ss0[pos] = pv0[r]; ss1[pos] = pv1[r]; ss2[pos] = pv2[r]; }
@@ -587,7 +608,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
             sid[pos] = pid[r];
         }
         if (wave < 2 && lane < s_na[wave]) {
-            const int pos = cst[a_key] + (wave == 0 ? 0 : runL[a_key] + tot[a_key]) + a_rank;
+            const int pos = cstart(a_key) + (wave == 0 ? 0 : runL[a_key] + tot[a_key]) + a_rank;
             sx0[pos] = __longlong_as_double((long long)aw[0]);
             sx1[pos] = __longlong_as_double((long long)aw[1]);
             sx2[pos] = __longlong_as_double((long long)aw[2]);
@@ -605,7 +626,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
                 const int s = j < xl ? 0 : 1;
                 const int jj = 64 + (s == 0 ? j : j - xl);
                 const int key = akey[s][jj];
-                const int pos = cst[key] + (s == 0 ? 0 : runL[key] + tot[key]) + arank[s][jj];
+                const int pos = cstart(key) + (s == 0 ? 0 : runL[key] + tot[key]) + arank[s][jj];
                 const int src = s == 0 ? g - 1 : g + 1;
                 const int side = s == 0 ? 1 : 0;
                 sx0[pos] = __longlong_as_double((long long)ld_agent(slab_mb(A, src, side, q, 0) + jj));
