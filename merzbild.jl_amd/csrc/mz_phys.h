@@ -11,9 +11,13 @@
 //   convect_single_particle!                       convection/convection_1D.jl:17-54
 //   diffuse_reflection_x!, reflect_particle_x!     convection/boundary_conditions.jl:79-121
 //   get_cell (uniform 1-D grid)                    grids/grid_uniform1D.jl:97-99
+// The power of the relative speed, the logarithms of the wall draws and the
+// sine/cosine pairs are the series forms of mz_math.h (within 2 ulp of the C
+// library), which keep these one-lane chains short.
 #pragma once
 
 #include "mz_internal.h"
+#include "mz_math.h"
 
 #define MZ_G_EPS 2.220446049250313e-16  // eps(Float64): the g > eps() test of ntc!
 
@@ -37,7 +41,7 @@ __device__ inline double mz_rel_speed(double vi0, double vi1, double vi2, double
 // sigma(g) g max(w_i, w_k), sigma(g) = C g^(1 - 2 omega) (expo = 1 - 2 omega);
 // g^0 is 1 for every g, so pseudo-Maxwell molecules (omega = 1/2) skip pow
 __device__ inline double mz_vhs_sgw(double factor, double expo, double g, double wi, double wk) {
-    double sigma = factor * (expo == 0.0 ? 1.0 : pow(g, expo));
+    double sigma = factor * (expo == 0.0 ? 1.0 : mz_pow_pos(g, expo));
     return sigma * g * (wi > wk ? wi : wk);
 }
 
@@ -56,7 +60,7 @@ __device__ inline void mz_vhs_post(double mu1, double mu2, double g, double vc0,
                                    double &vk1, double &vk2) {
     double phi = 6.283185307179586 * R_phi;
     double sphi, cphi;
-    sincos(phi, &sphi, &cphi);
+    mz_sincos(phi, &sphi, &cphi);
     double ct = 2.0 * R_ct - 1.0;
     double stt = sqrt(1.0 - ct * ct);
     double gn0 = g * (stt * cphi), gn1 = g * (stt * sphi), gn2 = g * ct;
@@ -82,12 +86,12 @@ __device__ inline void mz_vhs_scatter(const MzRngParams &rp, MzStream &st, doubl
 __device__ inline void mz_diffuse_reflect(const MzRngParams &rp, MzStream &st, double &v0, double &v1, double &v2,
                                           double vsq, double sgn, const double *wv) {
     double R = fmax(1e-50, mz_rand(rp, st));
-    double vn = sgn * sqrt(-vsq * log(R));
+    double vn = sgn * sqrt(-vsq * mz_log(R));
     R = fmax(1e-50, mz_rand(rp, st));
-    double vt = sqrt(-vsq * log(R));
+    double vt = sqrt(-vsq * mz_log(R));
     R = 6.283185307179586 * mz_rand(rp, st);
     double s, c;
-    sincos(R, &s, &c);
+    mz_sincos(R, &s, &c);
     v0 = vn + wv[0];
     v1 = s * vt + wv[1];
     v2 = c * vt + wv[2];
