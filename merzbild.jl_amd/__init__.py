@@ -646,3 +646,14 @@ def batch0d_steps(store, n_steps, step0, dt, species_data=None, V=None, props_ev
 def multi_index_moments(n):
     """compute_multi_index_moments(n) (src/merging/merging_nnls.jl:341-352)"""
     return [(i, j, n - i - j) for i in range(n + 1) for j in range(n - i + 1) if n - i - j >= 0]
+
+
+# NetCDF output in the reference's NCDataHolder layout (io.py), loaded on use
+_IO_NAMES = ("NCDataHolder", "write_netcdf", "close_netcdf")
+
+
+def __getattr__(name):
+    if name in _IO_NAMES:
+        from . import io as _io
+        return getattr(_io, name)
+    raise AttributeError(name)

@@ -1,0 +1,103 @@
+"""NetCDF output of PhysProps time series in the layout of the reference's
+NCDataHolder (src/io.jl:116-318), so device runs can be set beside the
+reference's own output files (test/data/*.nc) and read by its
+post-processing scripts.
+
+Variables (netCDF order: the unlimited timestep dimension first):
+    timestep               [timestep]
+    species_names          [n_species, name_strlen]   (chars)
+    length_particle_array  [timestep, n_species]
+    moment_powers          [n_moments]                 (int32)
+    moments                [timestep, n_species, n_cells, n_moments]
+    np, ndens, T           [timestep, n_species, n_cells]
+    v                      [timestep, n_species, n_cells, vel_components]
+and the global attribute "COMMENT ndens" (io.jl:160-165).  `ndens` holds
+PhysProps.n: the number density when ndens_not_Np, otherwise the number of
+physical particles in the cell.
+
+The reference writes NetCDF-4; this writer produces NetCDF classic with
+64-bit offsets (scipy.io.netcdf_file), which the same readers (libnetcdf,
+NetCDF.jl, NCDatasets.jl, netCDF4-python, xarray) read with the same
+variable names and shapes.  Classic files allow one unlimited dimension, so
+the moment variables are written only when the PhysProps tracks moments
+(the reference writes them with a zero-length dimension otherwise).
+"""
+import numpy as np
+from scipy.io import netcdf_file
+
+_SKIPPABLE = ("length_particle_array", "moments", "np", "ndens", "v", "T")
+
+
+class NCDataHolder:
+    """NCDataHolder(nc_filename, species_data, phys_props[, names_skip_list];
+    global_attributes) (io.jl:116-150).  names_skip_list names variables
+    not to write, as IOSkipList does (io.jl:1-40)."""
+
+    def __init__(self, nc_filename, species_data, phys_props, names_skip_list=(), global_attributes=None):
+        for name in names_skip_list:
+            if name not in _SKIPPABLE:
+                raise ValueError(f"cannot skip {name!r}; skippable: {_SKIPPABLE}")
+        self.skip = set(names_skip_list)
+        self.ndens_not_Np = bool(phys_props.ndens_not_Np)
+        self.n_species, self.n_cells = phys_props.n_species, phys_props.n_cells
+        self.n_moments = int(phys_props.n_moments)
+        if len(species_data) != self.n_species:
+            raise ValueError("species_data and phys_props disagree on the number of species")
+        f = netcdf_file(nc_filename, "w", version=2)
+        for k, v in (global_attributes or {}).items():
+            setattr(f, str(k), v)
+        if self.ndens_not_Np:
+            comment = "In this file, the ndens variable stores the number density, NOT # of physical particles in cell"
+        else:
+            comment = "In this file, the ndens variable stores # of physical particles in cell, NOT number density"
+        setattr(f, "COMMENT ndens", comment)
+        names = [s.name.encode() for s in species_data]
+        strlen = max(1, max(len(n) for n in names))
+        f.createDimension("timestep", None)   # the record dimension comes first in classic files
+        f.createDimension("vel_components", 3)
+        f.createDimension("n_cells", self.n_cells)
+        f.createDimension("n_species", self.n_species)
+        f.createDimension("name_strlen", strlen)
+        self.with_moments = self.n_moments > 0 and "moments" not in self.skip
+        if self.with_moments:
+            f.createDimension("n_moments", self.n_moments)
+        spn = f.createVariable("species_names", "c", ("n_species", "name_strlen"))
+        spn[:] = np.array([list(n.ljust(strlen, b"\0").decode("latin-1")) for n in names], dtype="S1")
+        self.v_timestep = f.createVariable("timestep", "d", ("timestep",))
+        self.vars = {}
+        cells = ("timestep", "n_species", "n_cells")
+        if "length_particle_array" not in self.skip:
+            self.vars["length_particle_array"] = f.createVariable("length_particle_array", "d",
+                                                                  ("timestep", "n_species"))
+        if self.with_moments:
+            mp = f.createVariable("moment_powers", "i", ("n_moments",))
+            mp[:] = np.asarray(phys_props.moment_powers, dtype=np.int32)
+            self.vars["moments"] = f.createVariable("moments", "d", cells + ("n_moments",))
+        for name in ("np", "ndens", "T"):
+            if name not in self.skip:
+                self.vars[name] = f.createVariable(name, "d", cells)
+        if "v" not in self.skip:
+            self.vars["v"] = f.createVariable("v", "d", cells + ("vel_components",))
+        self.f = f
+        self.n_written = 0
+
+
+def write_netcdf(ds, phys_props, timestep, sync_freq=0):
+    """write_netcdf(ds, phys_props, timestep; sync_freq) (io.jl:157-219):
+    append one output time."""
+    if (phys_props.n_species, phys_props.n_cells) != (ds.n_species, ds.n_cells):
+        raise ValueError("phys_props does not match the file's dimensions")
+    t = ds.n_written
+    ds.v_timestep[t] = float(timestep)
+    src = {"length_particle_array": phys_props.lpa, "np": phys_props.np, "ndens": phys_props.n,
+           "T": phys_props.T, "v": phys_props.v, "moments": phys_props.moments}
+    for name, var in ds.vars.items():
+        var[t] = np.asarray(src[name], dtype=np.float64)
+    ds.n_written += 1
+    if sync_freq > 0 and ds.n_written % sync_freq == 0:
+        ds.f.flush()
+
+
+def close_netcdf(ds):
+    """close_netcdf(ds) (io.jl:151-155)"""
+    ds.f.close()
