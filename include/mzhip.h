@@ -102,6 +102,33 @@ int mz_squash(mz_store *h, int32_t species);
  * (src/convection/convection_1D.jl:130-157). */
 int mz_convect(mz_store *h, int32_t species, double dt, uint64_t tag, int32_t rng_mode);
 
+/* Surface properties of the 1-D grid's two walls (SurfProps(pia, grid),
+ * src/properties/surface_props.jl:22-64): element 1 = left wall (normal +x),
+ * element 2 = right wall (normal -x).  The store keeps two sets, which = 0
+ * (the last convection's values) and which = 1 (the time average), each
+ * double[n_species][2][11] with the quantities of one element in the order
+ * np, flux_incident, flux_reflected, force[3], normal_pressure,
+ * shear_pressure[3], kinetic_energy_flux.
+ *
+ * convect_particles!(rng, grid, walls, pv, pia, species, species_data,
+ * surf_props, dt) (src/convection/convection_1D.jl:176-207): convection as in
+ * mz_convect, plus clear_props! of set 0, the contacts of every particle that
+ * reaches a wall (update_surface_incident!/update_surface_reflected!,
+ * surface_props.jl:77-128) and surface_props_scale! with the given mass
+ * (surface_props.jl:144-160).  The sums are reproducible from run to run;
+ * they differ from the reference's particle-order sums by rounding only. */
+int mz_convect_surf(mz_store *h, int32_t species, double dt, uint64_t tag, int32_t rng_mode, double mass);
+/* SurfProps areas (inv_areas = 1 / areas) of the two elements; default 1. */
+int mz_set_surf_areas(mz_store *h, const double *areas);
+/* avg_props!(surf_props_avg, surf_props, n_avg_timesteps)
+ * (surface_props.jl:202-222): set 1 += set 0 / n_avg_timesteps, on device. */
+int mz_surf_avg(mz_store *h, double n_avg_timesteps);
+/* clear_props!(surf_props) (surface_props.jl:171-189) of set `which`. */
+int mz_surf_clear(mz_store *h, int32_t which);
+/* copy set `which` to / from host memory (double[n_species][2][11]). */
+int mz_get_surf(mz_store *h, int32_t which, double *out);
+int mz_set_surf(mz_store *h, int32_t which, const double *in);
+
 /* ntc! / ntc_equal_weight! for cells cell_lo..cell_hi, single species when
  * s1 == s2, two species otherwise (src/collisions/collision_ntc.jl:338-595).
  * V[n_cells] are the cell volumes.  equal_weight selects ntc_equal_weight!.

@@ -47,6 +47,12 @@ _SIGS = {
     "mz_sort": (C.c_int, [C.c_void_p, i32, i32]),
     "mz_squash": (C.c_int, [C.c_void_p, i32]),
     "mz_convect": (C.c_int, [C.c_void_p, i32, f64, u64, i32]),
+    "mz_convect_surf": (C.c_int, [C.c_void_p, i32, f64, u64, i32, f64]),
+    "mz_set_surf_areas": (C.c_int, [C.c_void_p, P(f64)]),
+    "mz_surf_avg": (C.c_int, [C.c_void_p, f64]),
+    "mz_surf_clear": (C.c_int, [C.c_void_p, i32]),
+    "mz_get_surf": (C.c_int, [C.c_void_p, i32, P(f64)]),
+    "mz_set_surf": (C.c_int, [C.c_void_p, i32, P(f64)]),
     "mz_ntc": (C.c_int, [C.c_void_p, i32, i32, f64, P(f64), f64, i32, u64, i32, i32, i32]),
     "mz_compute_props": (C.c_int, [C.c_void_p, i32, i32, P(f64)]),
     "mz_get_props": (C.c_int, [C.c_void_p, P(f64), P(f64), P(f64), P(f64), P(f64)]),
@@ -357,6 +363,14 @@ class DeviceStore:
         _check(load_library().mz_timing_reset(self.h))
 
     # -- slab-resident step shape ------------------------------------------------
+    def surf_avg(self, n_avg_timesteps):
+        """avg_props! on the device: time-average set += instantaneous set / n."""
+        _check(load_library().mz_surf_avg(self.h, float(n_avg_timesteps)))
+
+    def surf_clear(self, which=0):
+        """clear_props! of the device set `which` (0 instantaneous, 1 average)."""
+        _check(load_library().mz_surf_clear(self.h, which))
+
     def set_slab_shape(self, cells_per_slab=0, particles_per_thread=0):
         _check(load_library().mz_set_slab_shape(self.h, cells_per_slab, particles_per_thread))
 
@@ -403,10 +417,127 @@ def squash_pia(store, species):
     _check(load_library().mz_squash(store.h, species))
 
 
-def convect_particles(store, species, dt, tag=0, rng_mode=RNG_PHILOX):
+def convect_particles(store, species, dt, tag=0, rng_mode=RNG_PHILOX, species_data=None, surf_props=None,
+                      fetch=True):
     """convect_particles!(rng, grid, walls, pv, pia, species, species_data, dt)
-    (src/convection/convection_1D.jl:130-157)"""
-    _check(load_library().mz_convect(store.h, species, dt, tag, rng_mode))
+    (src/convection/convection_1D.jl:130-157), and with surf_props
+    convect_particles!(..., species_data, surf_props, dt) (176-207): the
+    store's instantaneous surface properties are cleared, accumulated over
+    the wall contacts and scaled on the device, then copied into surf_props
+    (unless fetch is False)."""
+    L = load_library()
+    if surf_props is None:
+        _check(L.mz_convect(store.h, species, dt, tag, rng_mode))
+        return None
+    if surf_props.n_elements != 2 or surf_props.n_species != store.n_species:
+        raise MzError("convect_particles: surf_props must be SurfProps(pia, grid) of the store's species")
+    mass = float(_masses(store, species_data)[species - 1])
+    areas = np.ascontiguousarray(surf_props.areas, dtype=np.float64)
+    _check(L.mz_set_surf_areas(store.h, _d(areas)))
+    _check(L.mz_convect_surf(store.h, species, dt, tag, rng_mode, mass))
+    return surf_props.fetch(store) if fetch else None
+
+
+class SurfProps:
+    """SurfProps(n_elements, n_species, areas, normals)
+    (src/properties/surface_props.jl:22-58), host copies in the reference's
+    shapes: np, flux_incident, flux_reflected, normal_pressure and
+    kinetic_energy_flux are (n_elements, n_species); force and
+    shear_pressure (3, n_elements, n_species); normals (3, n_elements).
+    SurfProps.grid1d(n_species) is SurfProps(pia, grid::Grid1DUniform)
+    (surface_props.jl:63-64), the layout the device computes."""
+
+    FIELDS = ("np", "flux_incident", "flux_reflected", "force", "normal_pressure", "shear_pressure",
+              "kinetic_energy_flux")
+
+    def __init__(self, n_elements, n_species, areas, normals):
+        self.n_elements, self.n_species = int(n_elements), int(n_species)
+        self.areas = np.array(areas, dtype=np.float64)
+        self.inv_areas = np.array([1.0 / a for a in self.areas])
+        self.normals = np.array(normals, dtype=np.float64).reshape(3, self.n_elements)
+        ne, ns = self.n_elements, self.n_species
+        self.np = np.zeros((ne, ns))
+        self.flux_incident = np.zeros((ne, ns))
+        self.flux_reflected = np.zeros((ne, ns))
+        self.force = np.zeros((3, ne, ns))
+        self.normal_pressure = np.zeros((ne, ns))
+        self.shear_pressure = np.zeros((3, ne, ns))
+        self.kinetic_energy_flux = np.zeros((ne, ns))
+
+    @classmethod
+    def grid1d(cls, n_species):
+        return cls(2, n_species, [1.0, 1.0], np.array([[1.0, 0.0, 0.0], [-1.0, 0.0, 0.0]]).T)
+
+    def _from_device(self, q):
+        q = q.reshape(self.n_species, 2, 11)
+        self.np[:] = q[:, :, 0].T
+        self.flux_incident[:] = q[:, :, 1].T
+        self.flux_reflected[:] = q[:, :, 2].T
+        self.force[:] = np.transpose(q[:, :, 3:6], (2, 1, 0))
+        self.normal_pressure[:] = q[:, :, 6].T
+        self.shear_pressure[:] = np.transpose(q[:, :, 7:10], (2, 1, 0))
+        self.kinetic_energy_flux[:] = q[:, :, 10].T
+        return self
+
+    def _to_device(self):
+        q = np.zeros((self.n_species, 2, 11))
+        q[:, :, 0] = self.np.T
+        q[:, :, 1] = self.flux_incident.T
+        q[:, :, 2] = self.flux_reflected.T
+        q[:, :, 3:6] = np.transpose(self.force, (2, 1, 0))
+        q[:, :, 6] = self.normal_pressure.T
+        q[:, :, 7:10] = np.transpose(self.shear_pressure, (2, 1, 0))
+        q[:, :, 10] = self.kinetic_energy_flux.T
+        return np.ascontiguousarray(q)
+
+    def fetch(self, store, which=0):
+        """Copy the store's set `which` (0: last convection, 1: time average)."""
+        q = np.zeros(self.n_species * 2 * 11)
+        _check(load_library().mz_get_surf(store.h, which, _d(q)))
+        return self._from_device(q)
+
+    def push(self, store, which=0):
+        q = self._to_device()
+        _check(load_library().mz_set_surf(store.h, which, _d(q)))
+
+
+def clear_props(surf_props):
+    """clear_props!(surf_props) (surface_props.jl:171-189); for the store's
+    device sets use DeviceStore.surf_clear."""
+    for f in SurfProps.FIELDS:
+        getattr(surf_props, f)[...] = 0.0
+
+
+def surface_props_scale(species, surf_props, species_data, dt):
+    """surface_props_scale!(species, surf_props, species_data, dt)
+    (surface_props.jl:144-160)"""
+    factor_base = species_data[species - 1].mass / dt
+    s = species - 1
+    for e in range(surf_props.n_elements):
+        factor = factor_base * surf_props.inv_areas[e]
+        for f in ("flux_incident", "flux_reflected", "normal_pressure", "kinetic_energy_flux"):
+            getattr(surf_props, f)[e, s] *= factor
+        surf_props.force[:, e, s] *= factor
+        surf_props.shear_pressure[:, e, s] *= factor
+
+
+def avg_props(surf_props_avg, surf_props, n_avg_timesteps):
+    """avg_props!(surf_props_avg, surf_props, n_avg_timesteps)
+    (surface_props.jl:202-222) on host copies; DeviceStore.surf_avg does the
+    same on the store's sets."""
+    inv = 1.0 / n_avg_timesteps
+    for f in SurfProps.FIELDS:
+        a = getattr(surf_props_avg, f)
+        a[...] = a + getattr(surf_props, f) * inv
+
+
+def reduce_surf_props(surf_props_target, surf_props_chunks):
+    """reduce_surf_props!(surf_props_target, surf_props_chunks)
+    (surface_props.jl:232-254)"""
+    clear_props(surf_props_target)
+    for sp in surf_props_chunks:
+        for f in SurfProps.FIELDS:
+            getattr(surf_props_target, f)[...] += getattr(sp, f)
 
 
 def ntc(store, s1, s2, dt, V=None, dw_tol=1e-16, equal_weight=False, tag=0, rng_mode=RNG_PHILOX, cells=None):

@@ -90,6 +90,10 @@ struct mz_store {
     int32_t b0_max_cells = 0, b0_target = 0;
     int32_t b0_info[4] = {0, 0, 0, 0};
     int32_t nnls_info[2] = {0, 0};   // last candidate NNLS merge: cells run, cells re-run in FP64
+    // surface properties of the two walls: [set][species][element][MZ_SURF_Q],
+    // set 0 = the last convection's values, 1 = the time average
+    double *d_surf = nullptr;
+    double surf_inv_area[2] = {1.0, 1.0};
 };
 
 MzViews mz_views(mz_store *h);

@@ -104,10 +104,56 @@ __device__ inline void mz_diffuse_reflect(const MzRngParams &rp, MzStream &st, d
 // at wall pre_bc produces, computed ahead by mz_diffuse_reflect from the same
 // stream (its draws 0-2); later bounces continue the stream at draw 3, so the
 // result is the same as without it.
-__device__ inline void mz_convect_one(const MzRngParams &rp, const MzGrid &g, const MzWalls &wl, int sidx,
-                                      double dt, int32_t id, double &x0, double &v0, double &v1, double &v2,
-                                      int32_t *err, int pre_bc = -1, double p0 = 0.0, double p1 = 0.0,
-                                      double p2 = 0.0) {
+// Surface quantities of the 1-D grid's two walls (surface_props.jl:22-128):
+// element 0 is the left wall (normal +x, boundary condition 1), element 1 the
+// right wall (normal -x).  Per element MZ_SURF_Q values, in the order np,
+// flux_incident, flux_reflected, force[3], normal_pressure,
+// shear_pressure[3], kinetic_energy_flux.  One thread sums the contacts of
+// its particle; hit() adds the incident particle (sgn = 1,
+// update_surface_incident!) or subtracts the reflected one (sgn = -1,
+// update_surface_reflected!), with the reference's fused forms: the normal
+// projection muladd(px, n1, muladd(py, n2, pz n3)), the shear components
+// muladd(-p_dot_n, n_i, p_i) and the energy 0.5 muladd(px, v1, muladd(py, v2,
+// pz v3)).
+#define MZ_SURF_Q 11
+#define MZ_SURF_EACH(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10)
+struct MzSurfAcc {
+    double w;
+    double q[2][MZ_SURF_Q];
+
+    __device__ __attribute__((always_inline)) void zero() {
+#define MZ_SURF_ZERO(k) q[0][k] = 0.0; q[1][k] = 0.0;
+        MZ_SURF_EACH(MZ_SURF_ZERO)
+#undef MZ_SURF_ZERO
+    }
+    __device__ __attribute__((always_inline)) void hit(int bc, double v0, double v1, double v2, double sgn) {
+        const double n0 = bc == 0 ? 1.0 : -1.0, n1 = 0.0, n2 = 0.0;
+        const double px = w * v0, py = w * v1, pz = w * v2;
+        const double pdn = fma(px, n0, fma(py, n1, pz * n2));
+        const double d0 = sgn > 0.0 ? 1.0 : 0.0;
+        const double d1 = sgn > 0.0 ? w : 0.0;
+        const double d2 = sgn > 0.0 ? 0.0 : -w;
+        const double d3 = sgn * px;
+        const double d4 = sgn * py;
+        const double d5 = sgn * pz;
+        const double d6 = -sgn * pdn;
+        const double d7 = sgn * fma(-pdn, n0, px);
+        const double d8 = sgn * fma(-pdn, n1, py);
+        const double d9 = sgn * fma(-pdn, n2, pz);
+        const double d10 = sgn * (0.5 * fma(px, v0, fma(py, v1, pz * v2)));
+        // both elements updated through selects (adding 0.0 changes no sum),
+        // every index spelled out, so q stays in registers
+        const bool left = bc == 0;
+#define MZ_SURF_ADD(k) q[0][k] += left ? d##k : 0.0; q[1][k] += left ? 0.0 : d##k;
+        MZ_SURF_EACH(MZ_SURF_ADD)
+#undef MZ_SURF_ADD
+    }
+};
+
+template <class Acc>
+__device__ inline void mz_convect_impl(const MzRngParams &rp, const MzGrid &g, const MzWalls &wl, int sidx,
+                                       double dt, int32_t id, double &x0, double &v0, double &v1, double &v2,
+                                       int32_t *err, int pre_bc, double p0, double p1, double p2, Acc &sa) {
     double t_rest = dt;
     double x_old = x0;
     double x_new = fma(v0, dt, x0);
@@ -127,6 +173,7 @@ __device__ inline void mz_convect_one(const MzRngParams &rp, const MzGrid &g, co
             t_rest -= fabs(x_old / v0);
             bc = 0; wn = 1.0; x_old = 0.0;
         }
+        sa.hit(bc, v0, v1, v2, 1.0);
         double acc = wl.acc[bc];
         if (acc == 0.0) {
             v0 = -v0;
@@ -142,11 +189,33 @@ __device__ inline void mz_convect_one(const MzRngParams &rp, const MzGrid &g, co
             if (R < acc) mz_diffuse_reflect(rp, st, v0, v1, v2, wl.vsq[bc][sidx], wn, wl.v[bc]);
             else v0 = -v0;
         }
+        sa.hit(bc, v0, v1, v2, -1.0);
         x_new = fma(v0, t_rest, x_old);
     }
     if (x_new < g.min_x) x_new = g.min_x;
     else if (x_new > g.max_x) x_new = g.max_x;
     x0 = x_new;
+}
+
+// Walls without surface properties.
+struct MzNoSurf {
+    __device__ void hit(int, double, double, double, double) {}
+};
+
+__device__ inline void mz_convect_one(const MzRngParams &rp, const MzGrid &g, const MzWalls &wl, int sidx,
+                                      double dt, int32_t id, double &x0, double &v0, double &v1, double &v2,
+                                      int32_t *err, int pre_bc = -1, double p0 = 0.0, double p1 = 0.0,
+                                      double p2 = 0.0) {
+    MzNoSurf ns;
+    mz_convect_impl(rp, g, wl, sidx, dt, id, x0, v0, v1, v2, err, pre_bc, p0, p1, p2, ns);
+}
+
+// The same flight with every wall contact added to sa (sa.w: the particle's weight).
+__device__ inline void mz_convect_one_surf(MzSurfAcc &sa, const MzRngParams &rp, const MzGrid &g, const MzWalls &wl,
+                                           int sidx, double dt, int32_t id, double &x0, double &v0, double &v1,
+                                           double &v2, int32_t *err, int pre_bc = -1, double p0 = 0.0,
+                                           double p1 = 0.0, double p2 = 0.0) {
+    mz_convect_impl(rp, g, wl, sidx, dt, id, x0, v0, v1, v2, err, pre_bc, p0, p1, p2, sa);
 }
 
 // get_cell: floor(x / dx) + 1 (1-based); cells outside [1, n_cells] flag a

@@ -169,6 +169,9 @@ extern "C" int mz_create(int32_t n_cells, int32_t n_species, const int64_t *capa
     MZ_HIP(hipMalloc(&h->d_V, n_cells * sizeof(double)));
     MZ_HIP(hipMalloc(&h->d_invV, n_cells * sizeof(double)));
     MZ_HIP(hipMalloc(&h->d_masses, MZ_MAX_SPECIES * sizeof(double)));
+    const size_t surf_bytes = (size_t)2 * n_species * 2 * MZ_SURF_Q * sizeof(double);
+    MZ_HIP(hipMalloc(&h->d_surf, surf_bytes));
+    MZ_HIP(hipMemset(h->d_surf, 0, surf_bytes));
     h->inter.resize(nss);
     MZ_HIP(hipStreamSynchronize(h->stream));
     *out = h;
@@ -195,6 +198,7 @@ extern "C" int mz_destroy(mz_store *h) {
     hipFree(h->d_np); hipFree(h->d_n); hipFree(h->d_T); hipFree(h->d_v); hipFree(h->d_V);
     hipFree(h->d_invV);
     hipFree(h->d_masses);
+    hipFree(h->d_surf);
     for (int k = 0; k < 3; ++k) if (h->d_replay[k]) hipFree(h->d_replay[k]);
     for (auto &sc : h->scratch) if (sc.second.ptr) hipFree(sc.second.ptr);
     hipStreamDestroy(h->stream);
@@ -885,6 +889,194 @@ extern "C" int mz_convect(mz_store *h, int32_t species, double dt, uint64_t tag,
     r = mz_launch_convect(h, species, dt, tag, rng_mode, c.contiguous != 0, c.n_pos);
     mz_timing_end(h);
     return r;
+}
+
+// ===========================================================================
+// Convection with surface properties (convection_1D.jl:72-112, 176-207;
+// surface_props.jl:77-189).  Only particles whose flight ends outside
+// (0, L) touch a wall; each such thread sums its contacts (MzSurfAcc), a
+// block with any contact reduces them (wave sums, then its waves in order)
+// into its row of partials and flags it, and k_surf_finalize adds the flagged
+// rows in block order.  The sums are thus the same on every run; they differ
+// from the reference's particle-order sum only by rounding.
+
+#define MZ_SURF_BLOCK 256
+
+__global__ void k_fill_u8(uint8_t *__restrict__ a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+// Wave sums of the 2 x MZ_SURF_Q accumulators, written by lane 0.  The
+// recursion spells out every index at compile time, so the accumulators stay
+// in registers (a counted loop around the reduction calls keeps them in
+// scratch memory).
+template <int I>
+__device__ inline void surf_wave_sums(const MzSurfAcc &sa, double *dst, int lane) {
+    if constexpr (I < 2 * MZ_SURF_Q) {
+        const double t = mz_wave_sum(sa.q[I / MZ_SURF_Q][I % MZ_SURF_Q]);
+        if (lane == 0) dst[I] = t;
+        surf_wave_sums<I + 1>(sa, dst, lane);
+    }
+}
+
+__global__ void __launch_bounds__(MZ_SURF_BLOCK) k_convect_surf(MzSpView sp, MzGrid g, MzWalls wl, int sidx,
+                                                                double dt, MzRngParams rp,
+                                                                const uint8_t *__restrict__ mask,
+                                                                double *__restrict__ part,
+                                                                int32_t *__restrict__ pflag) {
+    __shared__ double red[MZ_SURF_BLOCK / 64][2 * MZ_SURF_Q];
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    rp = mz_rng_resolve(rp);
+    const int32_t lim = mask ? mz_npos(sp) : mz_ntotal(sp);
+    const bool active = p < lim && (!mask || mask[p]);
+    MzSurfAcc sa;
+    sa.zero();
+    int hit = 0;
+    if (active) {
+        double x0 = sp.x0[p], v0 = sp.v0[p];
+        const double x_new = fma(v0, dt, x0);
+        if (x_new < g.L && x_new > 0.0) {
+            sp.x0[p] = x_new < g.min_x ? g.min_x : (x_new > g.max_x ? g.max_x : x_new);
+        } else {
+            hit = 1;
+            double v1 = sp.v1[p], v2 = sp.v2[p];
+            sa.w = sp.w[p];
+            mz_convect_one_surf(sa, rp, g, wl, sidx, dt, sp.id[p], x0, v0, v1, v2, &sp.ctr->err);
+            sp.x0[p] = x0;
+            sp.v0[p] = v0;
+            sp.v1[p] = v1;
+            sp.v2[p] = v2;
+        }
+    }
+    const int any = __syncthreads_or(hit);
+    if (any) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        surf_wave_sums<0>(sa, red[wave], lane);
+        __syncthreads();
+        if (threadIdx.x < 2 * MZ_SURF_Q) {
+            double t = red[0][threadIdx.x];
+            for (int w = 1; w < MZ_SURF_BLOCK / 64; ++w) t += red[w][threadIdx.x];
+            part[(size_t)blockIdx.x * 2 * MZ_SURF_Q + threadIdx.x] = t;
+        }
+    }
+    if (threadIdx.x == 0) pflag[blockIdx.x] = any;
+}
+
+// One wave per quantity (2 x MZ_SURF_Q blocks): the flagged blocks' partials
+// in block order, then surface_props_scale! (every quantity but np times
+// (mass / dt) / area); clear_props! leaves the other species at zero.
+__global__ void __launch_bounds__(64) k_surf_finalize(const double *__restrict__ part,
+                                                      const int32_t *__restrict__ pflag, int32_t nblk,
+                                                      int32_t sidx, int32_t n_species, double f0, double f1,
+                                                      double *__restrict__ out) {
+    const int t = blockIdx.x, lane = threadIdx.x;
+    double acc = 0.0;
+    for (int32_t b = lane; b < nblk; b += 64)
+        if (pflag[b]) acc += part[(size_t)b * 2 * MZ_SURF_Q + t];
+    acc = mz_wave_sum(acc);
+    if (lane == 0) {
+        const int e = t / MZ_SURF_Q, k = t % MZ_SURF_Q;
+        const double f = k == 0 ? 1.0 : (e == 0 ? f0 : f1);
+        for (int s = 0; s < n_species; ++s) out[((size_t)s * 2 + e) * MZ_SURF_Q + k] = s == sidx ? acc * f : 0.0;
+    }
+}
+
+// avg_props!(avg, inst, n) for every species: avg = muladd(inst, 1 / n, avg)
+__global__ void k_surf_avg(double *__restrict__ avg, const double *__restrict__ inst, int32_t n, double inv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) avg[i] = fma(inst[i], inv, avg[i]);
+}
+
+__global__ void k_surf_fill(double *__restrict__ a, int32_t n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+
+int mz_launch_convect_surf(mz_store *h, int species, double dt, uint64_t tag, int rng_mode, bool contiguous,
+                           int32_t n_pos, double mass, const uint64_t *step_ptr) {
+    MzSpecies &s = h->sp[species - 1];
+    MZ_CHECK(h->has_grid && h->has_walls, "convect: grid or walls not set");
+    MZ_CHECK(mass > 0.0 && dt > 0.0, "convect: surface properties need mass > 0 and dt > 0");
+    uint8_t *mask = nullptr;
+    if (!contiguous) {
+        mask = (uint8_t *)mz_scratch(h, "convect_mask", (size_t)s.cap);
+        MZ_CHECK(mask != nullptr, "convect: scratch allocation failed");
+        hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((s.cap + 255) / 256)), dim3(256), 0, h->stream, mask,
+                           (int64_t)s.cap);
+        hipLaunchKernelGGL(k_mark_valid, dim3(h->n_cells), dim3(256), 0, h->stream, mz_view(s), h->n_cells, mask);
+    }
+    MzRngParams rp = mz_rng_params(h, 1, tag, rng_mode);
+    rp.step_ptr = step_ptr;
+    rp.phase = 2;
+    const int64_t n = contiguous ? s.cap : std::max<int64_t>(n_pos, 1);
+    const int blocks = (int)((n + MZ_SURF_BLOCK - 1) / MZ_SURF_BLOCK);
+    double *part = (double *)mz_scratch(h, "surf_part", (size_t)blocks * 2 * MZ_SURF_Q * sizeof(double));
+    int32_t *pflag = (int32_t *)mz_scratch(h, "surf_flag", (size_t)blocks * sizeof(int32_t));
+    MZ_CHECK(part && pflag, "convect: scratch allocation failed");
+    hipLaunchKernelGGL(k_convect_surf, dim3(blocks), dim3(MZ_SURF_BLOCK), 0, h->stream, mz_view(s), h->grid,
+                       h->walls, species - 1, dt, rp, mask, part, pflag);
+    const double fb = mass / dt;
+    hipLaunchKernelGGL(k_surf_finalize, dim3(2 * MZ_SURF_Q), dim3(64), 0, h->stream, part, pflag, blocks,
+                       species - 1, h->n_species, fb * h->surf_inv_area[0], fb * h->surf_inv_area[1], h->d_surf);
+    MZ_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int mz_convect_surf(mz_store *h, int32_t species, double dt, uint64_t tag, int32_t rng_mode,
+                               double mass) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "convect: bad species");
+    MZ_CHECK(rng_mode == 0 || (rng_mode == 1 && h->d_replay[1]), "convect: replay table not set");
+    MzCounters c;
+    int r = mz_read_counters(h, species, &c);
+    if (r) return r;
+    mz_timing_begin(h, "convect");
+    r = mz_launch_convect_surf(h, species, dt, tag, rng_mode, c.contiguous != 0, c.n_pos, mass, nullptr);
+    mz_timing_end(h);
+    return r;
+}
+
+extern "C" int mz_set_surf_areas(mz_store *h, const double *areas) {
+    MZ_CHECK(areas != nullptr, "set_surf_areas: areas is NULL");
+    MZ_CHECK(areas[0] > 0.0 && areas[1] > 0.0, "set_surf_areas: areas must be positive");
+    h->surf_inv_area[0] = 1.0 / areas[0];
+    h->surf_inv_area[1] = 1.0 / areas[1];
+    return 0;
+}
+
+extern "C" int mz_surf_avg(mz_store *h, double n_avg_timesteps) {
+    MZ_CHECK(n_avg_timesteps != 0.0, "surf_avg: zero averaging count");
+    const int32_t n = h->n_species * 2 * MZ_SURF_Q;
+    hipLaunchKernelGGL(k_surf_avg, dim3(1), dim3(256), 0, h->stream, h->d_surf + n, h->d_surf, n,
+                       1.0 / n_avg_timesteps);
+    MZ_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int mz_surf_clear(mz_store *h, int32_t which) {
+    MZ_CHECK(which == 0 || which == 1, "surf_clear: which is 0 (instantaneous) or 1 (average)");
+    const int32_t n = h->n_species * 2 * MZ_SURF_Q;
+    hipLaunchKernelGGL(k_surf_fill, dim3(1), dim3(256), 0, h->stream, h->d_surf + (size_t)which * n, n);
+    MZ_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int mz_get_surf(mz_store *h, int32_t which, double *out) {
+    MZ_CHECK(which == 0 || which == 1, "get_surf: which is 0 (instantaneous) or 1 (average)");
+    MZ_CHECK(out != nullptr, "get_surf: out is NULL");
+    const size_t n = (size_t)h->n_species * 2 * MZ_SURF_Q;
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    MZ_HIP(hipMemcpy(out, h->d_surf + which * n, n * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int mz_set_surf(mz_store *h, int32_t which, const double *in) {
+    MZ_CHECK(which == 0 || which == 1, "set_surf: which is 0 (instantaneous) or 1 (average)");
+    MZ_CHECK(in != nullptr, "set_surf: in is NULL");
+    const size_t n = (size_t)h->n_species * 2 * MZ_SURF_Q;
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    MZ_HIP(hipMemcpy(h->d_surf + which * n, in, n * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
 }
 
 // ===========================================================================

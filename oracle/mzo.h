@@ -122,6 +122,33 @@ void mzo_convect_particles(mzo_rng *rng, const mzo_grid1d *g, const mzo_walls *w
 void mzo_convect_one(mzo_rng *rng, const mzo_grid1d *g, const mzo_walls *wl, mzo_pv *pv, int64_t id,
                      int64_t species, double dt);
 
+/* ---------------- surface properties (src/properties/surface_props.jl) ----
+ * SurfProps of n_elements surface elements and n_species species.  Element e
+ * (1-based) of species s keeps its MZO_SURF_Q quantities at
+ * q[((s - 1) * n_elements + e - 1) * MZO_SURF_Q + k], k in the order
+ * np, flux_incident, flux_reflected, force[3], normal_pressure,
+ * shear_pressure[3], kinetic_energy_flux.  normals[3 (e - 1) + i] is
+ * component i of element e's normal.  For the 1-D grid element 1 is the left
+ * wall (normal +x) and element 2 the right wall (normal -x), the elements of
+ * boundary condition 1 and 2 (surface_props.jl:63-64).
+ * Parity: pinned by the known answers of test/test_surface_props_1D_uniform.jl
+ * (tests/test_oracle_surf.py) and by the invariants of specular walls. */
+#define MZO_SURF_Q 11
+typedef struct {
+    int64_t n_elements, n_species;
+    double *inv_areas;   /* [n_elements] */
+    double *normals;     /* [3 * n_elements] */
+    double *q;           /* [n_species * n_elements * MZO_SURF_Q] */
+} mzo_surf;
+
+void mzo_surf_update_incident(mzo_surf *sp, double w, const double *v, int64_t species, int64_t element);
+void mzo_surf_update_reflected(mzo_surf *sp, double w, const double *v, int64_t species, int64_t element);
+void mzo_surf_scale(mzo_surf *sp, int64_t species, double mass, double dt);
+void mzo_surf_clear(mzo_surf *sp);
+void mzo_surf_avg(mzo_surf *avg, const mzo_surf *sp, double n_avg_timesteps);
+void mzo_convect_particles_surf(mzo_rng *rng, const mzo_grid1d *g, const mzo_walls *wl, mzo_pv *pv,
+                                mzo_pia *pia, int64_t species, double dt, mzo_surf *sp, double mass);
+
 /* ---------------- collisions ---------------- */
 typedef struct {
     double m_r, mu1, mu2, vhs_d, vhs_o, vhs_Tref, vhs_muref, vhs_factor;

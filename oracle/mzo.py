@@ -53,6 +53,11 @@ class Walls(C.Structure):
     _fields_ = [("T", f64 * 2), ("v", (f64 * 3) * 2), ("acc", f64 * 2), ("n_species", i64), ("vsq", f64 * 32)]
 
 
+class SurfS(C.Structure):
+    _fields_ = [("n_elements", i64), ("n_species", i64), ("inv_areas", P(f64)), ("normals", P(f64)),
+                ("q", P(f64))]
+
+
 class Interaction(C.Structure):
     _fields_ = [(n, f64) for n in ("m_r", "mu1", "mu2", "vhs_d", "vhs_o", "vhs_Tref", "vhs_muref", "vhs_factor")]
 
@@ -137,6 +142,12 @@ def lib():
             "mzo_sort_particles_precomputed": (None, [P(PvS), P(PiaS), i64]),
             "mzo_walls_init": (None, [P(Walls), P(f64), i64, f64, f64, f64, f64, f64, f64]),
             "mzo_convect_particles": (None, [P(Rng), P(Grid1D), P(Walls), P(PvS), P(PiaS), i64, f64]),
+            "mzo_surf_update_incident": (None, [P(SurfS), f64, P(f64), i64, i64]),
+            "mzo_surf_update_reflected": (None, [P(SurfS), f64, P(f64), i64, i64]),
+            "mzo_surf_scale": (None, [P(SurfS), i64, f64, f64]),
+            "mzo_surf_clear": (None, [P(SurfS)]),
+            "mzo_surf_avg": (None, [P(SurfS), P(SurfS), f64]),
+            "mzo_convect_particles_surf": (None, [P(Rng), P(Grid1D), P(Walls), P(PvS), P(PiaS), i64, f64, P(SurfS), f64]),
             "mzo_interaction_init": (None, [P(Interaction), P(Interaction), f64, f64, f64, f64, f64]),
             "mzo_sigma_vhs": (f64, [P(Interaction), f64]),
             "mzo_estimate_sigma_g_w_max": (f64, [P(Interaction), f64, f64, f64, f64, f64, f64]),
@@ -335,6 +346,65 @@ def walls(masses, T_l, T_r, vy_l, vy_r, acc_l, acc_r):
     m = np.ascontiguousarray(masses, dtype=np.float64)
     lib().mzo_walls_init(C.byref(w), _dptr(m), len(m), T_l, T_r, vy_l, vy_r, acc_l, acc_r)
     return w
+
+
+class Surf:
+    """SurfProps (surface_props.jl:22-64) over the oracle's flat layout
+    q[species-1, element-1, k], k = np, flux_incident, flux_reflected,
+    force x3, normal_pressure, shear_pressure x3, kinetic_energy_flux."""
+
+    Q = 11
+
+    def __init__(self, n_elements, n_species, areas, normals):
+        self.n_elements, self.n_species = n_elements, n_species
+        self.inv_areas = np.ascontiguousarray([1.0 / a for a in areas], dtype=np.float64)
+        self.normals = np.ascontiguousarray(np.asarray(normals, dtype=np.float64).reshape(n_elements, 3))
+        self.q = np.zeros((n_species, n_elements, self.Q))
+        self.s = SurfS(n_elements, n_species, _dptr(self.inv_areas), _dptr(self.normals), _dptr(self.q))
+
+    @classmethod
+    def grid1d(cls, n_species):
+        """SurfProps(pia, grid::Grid1DUniform): two unit-area walls, normals +x and -x."""
+        return cls(2, n_species, [1.0, 1.0], [[1.0, 0.0, 0.0], [-1.0, 0.0, 0.0]])
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+    # the reference's arrays: (n_elements, n_species) and (3, n_elements, n_species)
+    def _f(self, k):
+        return self.q[:, :, k].T
+
+    np = property(lambda self: self._f(0))
+    flux_incident = property(lambda self: self._f(1))
+    flux_reflected = property(lambda self: self._f(2))
+    force = property(lambda self: np.transpose(self.q[:, :, 3:6], (2, 1, 0)))
+    normal_pressure = property(lambda self: self._f(6))
+    shear_pressure = property(lambda self: np.transpose(self.q[:, :, 7:10], (2, 1, 0)))
+    kinetic_energy_flux = property(lambda self: self._f(10))
+
+    def update_incident(self, w, v, species, element):
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        lib().mzo_surf_update_incident(self.ref, w, _dptr(vv), species, element)
+
+    def update_reflected(self, w, v, species, element):
+        vv = np.ascontiguousarray(v, dtype=np.float64)
+        lib().mzo_surf_update_reflected(self.ref, w, _dptr(vv), species, element)
+
+    def scale(self, species, mass, dt):
+        lib().mzo_surf_scale(self.ref, species, mass, dt)
+
+    def clear(self):
+        lib().mzo_surf_clear(self.ref)
+
+    def avg_from(self, inst, n_avg_timesteps):
+        lib().mzo_surf_avg(self.ref, inst.ref, float(n_avg_timesteps))
+
+
+def convect_particles_surf(rng, grid, walls, pv, pia, species, dt, surf, mass):
+    """convect_particles!(..., surf_props, dt) (convection_1D.jl:176-207)"""
+    lib().mzo_convect_particles_surf(rng.ref, C.byref(grid), C.byref(walls), pv.p, pia.p, species, dt, surf.ref,
+                                     mass)
 
 
 def interaction_pair(m_i, m_k, d, omega, Tref):

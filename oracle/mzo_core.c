@@ -562,6 +562,132 @@ void mzo_convect_particles(mzo_rng *rng, const mzo_grid1d *g, const mzo_walls *w
     }
 }
 
+/* ---------------- surface properties ----------------
+ * update_surface_incident! / update_surface_reflected!
+ * (surface_props.jl:77-97, 111-128).  The reference's @muladd block turns the
+ * normal projection px n1 + py n2 + pz n3 into muladd(px, n1, muladd(py, n2,
+ * pz n3)), px - p_dot_n n1 into muladd(-p_dot_n, n1, px) and the energy term
+ * likewise; 0.5 times the energy term is exact, so fusing its accumulation
+ * changes nothing. */
+enum { SQ_NP, SQ_FI, SQ_FR, SQ_F0, SQ_F1, SQ_F2, SQ_PN, SQ_S0, SQ_S1, SQ_S2, SQ_KE };
+
+static double *surf_at(mzo_surf *sp, int64_t species, int64_t element) {
+    return sp->q + ((species - 1) * sp->n_elements + element - 1) * MZO_SURF_Q;
+}
+
+static void surf_update(mzo_surf *sp, double w, const double *v, int64_t species, int64_t element, double sgn) {
+    const double *nrm = sp->normals + 3 * (element - 1);
+    double *q = surf_at(sp, species, element);
+    const double px = w * v[0], py = w * v[1], pz = w * v[2];
+    const double p_dot_n = fma(px, nrm[0], fma(py, nrm[1], pz * nrm[2]));
+    const double ke = 0.5 * fma(px, v[0], fma(py, v[1], pz * v[2]));
+    if (sgn > 0.0) {
+        q[SQ_NP] += 1.0;
+        q[SQ_FI] += w;
+    } else {
+        q[SQ_FR] -= w;
+    }
+    q[SQ_F0] += sgn * px;
+    q[SQ_F1] += sgn * py;
+    q[SQ_F2] += sgn * pz;
+    q[SQ_PN] -= sgn * p_dot_n;
+    q[SQ_S0] += sgn * fma(-p_dot_n, nrm[0], px);
+    q[SQ_S1] += sgn * fma(-p_dot_n, nrm[1], py);
+    q[SQ_S2] += sgn * fma(-p_dot_n, nrm[2], pz);
+    q[SQ_KE] += sgn * ke;
+}
+
+void mzo_surf_update_incident(mzo_surf *sp, double w, const double *v, int64_t species, int64_t element) {
+    surf_update(sp, w, v, species, element, 1.0);
+}
+
+void mzo_surf_update_reflected(mzo_surf *sp, double w, const double *v, int64_t species, int64_t element) {
+    surf_update(sp, w, v, species, element, -1.0);
+}
+
+/* surface_props_scale! (surface_props.jl:144-160): every quantity but np
+ * times (mass / dt) / area. */
+void mzo_surf_scale(mzo_surf *sp, int64_t species, double mass, double dt) {
+    const double factor_base = mass / dt;
+    for (int64_t e = 1; e <= sp->n_elements; ++e) {
+        const double factor = factor_base * sp->inv_areas[e - 1];
+        double *q = surf_at(sp, species, e);
+        for (int k = SQ_FI; k < MZO_SURF_Q; ++k) q[k] *= factor;
+    }
+}
+
+/* clear_props!(surf_props) (surface_props.jl:171-189) */
+void mzo_surf_clear(mzo_surf *sp) {
+    const int64_t n = sp->n_species * sp->n_elements * MZO_SURF_Q;
+    for (int64_t i = 0; i < n; ++i) sp->q[i] = 0.0;
+}
+
+/* avg_props!(surf_props_avg, surf_props, n) (surface_props.jl:202-222):
+ * avg = muladd(inst, 1 / n, avg) for every quantity. */
+void mzo_surf_avg(mzo_surf *avg, const mzo_surf *sp, double n_avg_timesteps) {
+    const double inv = 1.0 / n_avg_timesteps;
+    const int64_t n = sp->n_species * sp->n_elements * MZO_SURF_Q;
+    for (int64_t i = 0; i < n; ++i) avg->q[i] = fma(sp->q[i], inv, avg->q[i]);
+}
+
+/* convect_single_particle! with surface properties (convection_1D.jl:72-112):
+ * each wall hit adds the incident particle, reflects it, then subtracts the
+ * reflected one, on element bc + 1. */
+static void convect_single_surf(mzo_rng *rng, const mzo_grid1d *g, const mzo_walls *wl, double w, double *x,
+                                double *v, int64_t species, double dt, mzo_surf *sp) {
+    double t_rest = dt;
+    double x_old = x[0];
+    double x_new = fma(v[0], dt, x[0]);
+    while ((x_new >= g->L) || (x_new <= 0.0)) {
+        int bc;
+        double wn;
+        if (x_new >= g->L) {
+            t_rest -= fabs((g->L - x_old) / v[0]);
+            bc = 1; wn = -1.0; x_old = g->L;
+        } else {
+            t_rest -= fabs(x_old / v[0]);
+            bc = 0; wn = 1.0; x_old = 0.0;
+        }
+        mzo_surf_update_incident(sp, w, v, species, bc + 1);
+        reflect_particle_x(rng, v, wl->vsq[bc * wl->n_species + species - 1], wn, wl->v[bc], wl->acc[bc]);
+        mzo_surf_update_reflected(sp, w, v, species, bc + 1);
+        x_new = fma(v[0], t_rest, x_old);
+    }
+    if (x_new < g->min_x) x_new = g->min_x;
+    else if (x_new > g->max_x) x_new = g->max_x;
+    x[0] = x_new;
+}
+
+/* convect_particles!(rng, grid, walls, pv, pia, species, species_data,
+ * surf_props, dt) (convection_1D.jl:176-207): clear_props! of every species,
+ * the particles in the reference's order, then surface_props_scale!. */
+void mzo_convect_particles_surf(mzo_rng *rng, const mzo_grid1d *g, const mzo_walls *wl, mzo_pv *pv,
+                                mzo_pia *pia, int64_t species, double dt, mzo_surf *sp, double mass) {
+    mzo_surf_clear(sp);
+    if (pia->contiguous[species - 1]) {
+        int64_t n_tot = pia->n_total[species - 1];
+        for (int64_t i = 1; i <= n_tot; ++i) {
+            int64_t id = PID(pv, i);
+            mzo_rng_set_stream(rng, (uint64_t)(id - 1));
+            convect_single_surf(rng, g, wl, PV_W(pv, id), PV_X(pv, id), PV_V(pv, id), species, dt, sp);
+        }
+    } else {
+        for (int64_t c = 1; c <= g->n_cells; ++c) {
+            mzo_ind *d = mzo_pia_at(pia, c, species);
+            for (int grp = 0; grp < 2; ++grp) {
+                if (grp == 1 && d->n_group2 <= 0) break;
+                const int64_t s = grp == 0 ? d->start1 : d->start2, e = grp == 0 ? d->end1 : d->end2;
+                for (int64_t i = s; i <= e; ++i) {
+                    int64_t id = PID(pv, i);
+                    mzo_rng_set_stream(rng, (uint64_t)(id - 1));
+                    convect_single_surf(rng, g, wl, PV_W(pv, id), PV_X(pv, id), PV_V(pv, id), species, dt, sp);
+                }
+            }
+        }
+    }
+    mzo_surf_scale(sp, species, mass, dt);
+}
+
 /* One particle (physical id) through its own stream; used by the parallel
  * drivers. */
 void mzo_convect_one(mzo_rng *rng, const mzo_grid1d *g, const mzo_walls *wl, mzo_pv *pv, int64_t id,
