@@ -82,9 +82,77 @@ class NCDataHolder:
         self.n_written = 0
 
 
+_SURF_SKIPS = {"np": "np", "nparticles": "np", "fluxes": "fluxes", "force": "force",
+               "normal_pressure": "normal_pressure", "shear_pressure": "shear_pressure",
+               "kinetic_energy_flux": "kinetic_energy_flux"}
+
+
+class NCDataHolderSurf:
+    """NCDataHolderSurf(nc_filename[, names_skip_list], species_data,
+    surf_props; global_attributes) (io.jl:400-549) for SurfProps time series:
+        timestep                                   [timestep]
+        species_names                              [n_species, name_strlen] (chars)
+        np, flux_incident, flux_reflected,
+        normal_pressure, kinetic_energy_flux       [timestep, n_species, n_elements]
+        force, shear_pressure                      [timestep, n_species, n_elements, vector_components]
+    names_skip_list takes the names IOSkipListSurf takes (io.jl:334-398):
+    "np" or "nparticles", "fluxes" (both fluxes), "force", "normal_pressure",
+    "shear_pressure", "kinetic_energy_flux"; other names are ignored there
+    and rejected here."""
+
+    def __init__(self, nc_filename, species_data, surf_props, names_skip_list=(), global_attributes=None):
+        skip = set()
+        for name in names_skip_list:
+            if name not in _SURF_SKIPS:
+                raise ValueError(f"cannot skip {name!r}; skippable: {sorted(_SURF_SKIPS)}")
+            skip.add(_SURF_SKIPS[name])
+        if len(species_data) != surf_props.n_species:
+            raise ValueError("species_data and surf_props disagree on the number of species")
+        self.n_elements, self.n_species = surf_props.n_elements, surf_props.n_species
+        f = netcdf_file(nc_filename, "w", version=2)
+        for k, v in (global_attributes or {}).items():
+            setattr(f, str(k), v)
+        names = [s.name.encode() for s in species_data]
+        strlen = max(1, max(len(n) for n in names))
+        f.createDimension("timestep", None)   # the record dimension comes first in classic files
+        f.createDimension("vector_components", 3)
+        f.createDimension("n_elements", self.n_elements)
+        f.createDimension("n_species", self.n_species)
+        f.createDimension("name_strlen", strlen)
+        spn = f.createVariable("species_names", "c", ("n_species", "name_strlen"))
+        spn[:] = np.array([list(n.ljust(strlen, b"\0").decode("latin-1")) for n in names], dtype="S1")
+        self.v_timestep = f.createVariable("timestep", "d", ("timestep",))
+        el = ("timestep", "n_species", "n_elements")
+        vec = el + ("vector_components",)
+        wanted = [("np", el, "np"), ("flux_incident", el, "fluxes"), ("flux_reflected", el, "fluxes"),
+                  ("force", vec, "force"), ("normal_pressure", el, "normal_pressure"),
+                  ("shear_pressure", vec, "shear_pressure"), ("kinetic_energy_flux", el, "kinetic_energy_flux")]
+        self.vars = {name: f.createVariable(name, "d", dims) for name, dims, key in wanted if key not in skip}
+        self.f = f
+        self.n_written = 0
+
+
+def _write_surf(ds, surf_props, timestep, sync_freq):
+    """write_netcdf(ds, surf_props::SurfProps, timestep; sync_freq) (io.jl:566-610)"""
+    if (surf_props.n_elements, surf_props.n_species) != (ds.n_elements, ds.n_species):
+        raise ValueError("surf_props does not match the file's dimensions")
+    t = ds.n_written
+    ds.v_timestep[t] = float(timestep)
+    for name, var in ds.vars.items():
+        a = np.asarray(getattr(surf_props, name), dtype=np.float64)
+        # SurfProps arrays are (element, species) and (component, element, species)
+        var[t] = a.T if a.ndim == 2 else np.transpose(a, (2, 1, 0))
+    ds.n_written += 1
+    if sync_freq > 0 and ds.n_written % sync_freq == 0:
+        ds.f.flush()
+
+
 def write_netcdf(ds, phys_props, timestep, sync_freq=0):
     """write_netcdf(ds, phys_props, timestep; sync_freq) (io.jl:157-219):
-    append one output time."""
+    append one output time; with an NCDataHolderSurf and a SurfProps, the
+    surface-properties method (io.jl:566-610)."""
+    if isinstance(ds, NCDataHolderSurf):
+        return _write_surf(ds, phys_props, timestep, sync_freq)
     if (phys_props.n_species, phys_props.n_cells) != (ds.n_species, ds.n_cells):
         raise ValueError("phys_props does not match the file's dimensions")
     t = ds.n_written

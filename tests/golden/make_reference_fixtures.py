@@ -24,6 +24,10 @@ FILES = {
     "bkw": ("bkw_20k_seed1234.nc", ["moments", "T", "np"]),
     # test/test_1D_couette.jl: 50 cells x 1000 particles, 6000 steps, output every 1000
     "couette": ("couette_0.0005_50_500.0_300.0_1000.nc", ["T", "ndens", "v", "np"]),
+    # test/test_1D_couette_varweight.jl: surface properties of the two walls, output every 1000 steps
+    "couette_surf": ("couette_0.0005_50_500.0_300.0_1000_vw200to150_surf.nc",
+                     ["timestep", "np", "flux_incident", "flux_reflected", "force", "normal_pressure",
+                      "shear_pressure", "kinetic_energy_flux"]),
 }
 
 

@@ -89,3 +89,57 @@ def test_rejects_unknown_skip_name(tmp_path):
     pkg, io = _pkg(), _io()
     with pytest.raises(ValueError):
         io.NCDataHolder(str(tmp_path / "x.nc"), [_Sp("Ar")], pkg.PhysProps(2, 1), names_skip_list=["bogus"])
+
+
+def test_surf_fixture_round_trip(tmp_path):
+    """NCDataHolderSurf / write_netcdf for SurfProps (src/io.jl:400-610): the
+    reference's wall-property series (test/data/couette_..._vw200to150_surf.nc,
+    six outputs) written one output at a time come back unchanged, in the
+    reference's variable names and dimension order."""
+    pkg, io = _pkg(), _io()
+    names = ("np", "flux_incident", "flux_reflected", "force", "normal_pressure", "shear_pressure",
+             "kinetic_energy_flux")
+    fx = {n: FIX["couette_surf_" + n] for n in names}
+    ts = FIX["couette_surf_timestep"]
+    nt, ns, ne = fx["np"].shape
+    sp = pkg.SurfProps.grid1d(ns)
+    assert ne == sp.n_elements
+    path = str(tmp_path / "surf.nc")
+    ds = io.NCDataHolderSurf(path, [_Sp("Ar")], sp, global_attributes={"seed": 1234})
+    for t in range(nt):
+        for n in names:
+            a = fx[n][t]
+            getattr(sp, n)[...] = a.T if a.ndim == 2 else np.transpose(a, (2, 1, 0))
+        io.write_netcdf(ds, sp, ts[t], sync_freq=2)
+    io.close_netcdf(ds)
+    with netcdf_file(path, "r", mmap=False) as f:
+        assert f.variables["np"].dimensions == ("timestep", "n_species", "n_elements")
+        assert f.variables["force"].dimensions == ("timestep", "n_species", "n_elements", "vector_components")
+        for n in names:
+            assert np.array_equal(f.variables[n][:], fx[n]), n
+        assert np.array_equal(f.variables["timestep"][:], ts)
+        assert [b"".join(r).rstrip(b"\0").decode() for r in f.variables["species_names"][:]] == ["Ar"]
+
+
+def test_surf_skip_list(tmp_path):
+    """IOSkipListSurf (src/io.jl:334-398): "np"/"nparticles", "fluxes" (both
+    fluxes), "force", "normal_pressure", "shear_pressure", "kinetic_energy_flux"."""
+    pkg, io = _pkg(), _io()
+    sp = pkg.SurfProps.grid1d(1)
+    path = str(tmp_path / "skip.nc")
+    ds = io.NCDataHolderSurf(path, [_Sp("Ar")], sp, names_skip_list=["nparticles", "force", "shear_pressure"])
+    io.write_netcdf(ds, sp, 0)
+    io.close_netcdf(ds)
+    with netcdf_file(path, "r", mmap=False) as f:
+        for n in ("np", "force", "shear_pressure"):
+            assert n not in f.variables
+        for n in ("flux_incident", "flux_reflected", "normal_pressure", "kinetic_energy_flux"):
+            assert n in f.variables
+    path = str(tmp_path / "skip2.nc")
+    ds = io.NCDataHolderSurf(path, [_Sp("Ar")], sp, names_skip_list=["fluxes", "normal_pressure",
+                                                                      "kinetic_energy_flux"])
+    io.close_netcdf(ds)
+    with netcdf_file(path, "r", mmap=False) as f:
+        assert set(f.variables) == {"timestep", "species_names", "np", "force", "shear_pressure"}
+    with pytest.raises(ValueError):
+        io.NCDataHolderSurf(str(tmp_path / "bad.nc"), [_Sp("Ar")], sp, names_skip_list=["T"])
