@@ -152,6 +152,15 @@ int mz_get_props_moments(mz_store *h, double *moments);
 /* PhysProps arrays [n_species][n_cells] (v: [n_species][n_cells][3]); any may be NULL. */
 int mz_get_props(mz_store *h, double *lpa, double *np, double *n, double *v, double *T);
 
+/* avg_props!(phys_props_avg, phys_props, n_avg_timesteps)
+ * (src/properties/physical_props.jl:281-299) on the device: the store's
+ * time-averaged props += the last computed props / n_avg_timesteps (np, n,
+ * v, T; lpa from the particle vector's length).  mz_props_avg_clear zeroes
+ * the average, mz_get_props_avg copies it out in mz_get_props's layout. */
+int mz_props_avg(mz_store *h, double n_avg_timesteps);
+int mz_props_avg_clear(mz_store *h);
+int mz_get_props_avg(mz_store *h, double *lpa, double *np, double *n, double *v, double *T);
+
 /* merge_octree_N2_based!(rng, octree, pv, pia, cell, species, target_np[, grid])
  * for every cell whose n_local exceeds `threshold`
  * (src/merging/merging_octree_N2.jl:1060-1094), then squash_pia! if
@@ -206,6 +215,14 @@ int mz_get_nnls_info(mz_store *h, int32_t *info);
  * couette_multithreaded.jl:97-161. */
 int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *V,
                      int32_t equal_weight, int32_t props_every, const double *masses, int32_t mode);
+/* The averaging phase of simulations/1D/couette_with_surface_quantities.jl:
+ * 73-115: steps as mz_couette_steps with equal weights and props every step,
+ * where convection also computes the wall properties (mz_convect_surf with
+ * masses[0]) and each step adds the wall and cell properties / n_avg_timesteps
+ * to the store's time averages (mz_surf_avg, mz_props_avg).  mode 0 launches
+ * kernels one by one; modes 1 and 2 replay a captured hipGraph of the step. */
+int mz_couette_avg_steps(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *V,
+                         const double *masses, int32_t mode, double n_avg_timesteps);
 /* Which path the last mz_couette_steps call took: 0 eager, 1 graph, 2 slab. */
 int mz_couette_last_path(mz_store *h);
 /* Slab shape of mode 2 (the reference's n_chunks, couette_multithreaded.jl:

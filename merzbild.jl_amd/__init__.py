@@ -66,6 +66,10 @@ _SIGS = {
     "mz_merge_nnls_cand": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, f64, i32, i32, P(f64), i32, f64, i32,
                                      f64, i32, i32, i32, P(i32)]),
     "mz_couette_steps": (C.c_int, [C.c_void_p, i32, u64, f64, P(f64), i32, i32, P(f64), i32]),
+    "mz_couette_avg_steps": (C.c_int, [C.c_void_p, i32, u64, f64, P(f64), P(f64), i32, f64]),
+    "mz_props_avg": (C.c_int, [C.c_void_p, f64]),
+    "mz_props_avg_clear": (C.c_int, [C.c_void_p]),
+    "mz_get_props_avg": (C.c_int, [C.c_void_p, P(f64), P(f64), P(f64), P(f64), P(f64)]),
     "mz_couette_last_path": (C.c_int, [C.c_void_p]),
     "mz_set_slab_shape": (C.c_int, [C.c_void_p, i32, i32]),
     "mz_get_slab_info": (C.c_int, [C.c_void_p, P(i32)]),
@@ -363,6 +367,13 @@ class DeviceStore:
         _check(load_library().mz_timing_reset(self.h))
 
     # -- slab-resident step shape ------------------------------------------------
+    def props_avg(self, n_avg_timesteps):
+        """avg_props! of PhysProps on the device: time average += props / n."""
+        _check(load_library().mz_props_avg(self.h, float(n_avg_timesteps)))
+
+    def props_avg_clear(self):
+        _check(load_library().mz_props_avg_clear(self.h))
+
     def surf_avg(self, n_avg_timesteps):
         """avg_props! on the device: time-average set += instantaneous set / n."""
         _check(load_library().mz_surf_avg(self.h, float(n_avg_timesteps)))
@@ -521,14 +532,21 @@ def surface_props_scale(species, surf_props, species_data, dt):
         surf_props.shear_pressure[:, e, s] *= factor
 
 
-def avg_props(surf_props_avg, surf_props, n_avg_timesteps):
-    """avg_props!(surf_props_avg, surf_props, n_avg_timesteps)
-    (surface_props.jl:202-222) on host copies; DeviceStore.surf_avg does the
-    same on the store's sets."""
+def avg_props(props_avg, props, n_avg_timesteps):
+    """avg_props! on host copies, for SurfProps (surface_props.jl:202-222)
+    and PhysProps (physical_props.jl:281-299): avg + props / n, within one
+    rounding of the reference's fused form.  DeviceStore.surf_avg and
+    DeviceStore.props_avg do the same on the store's device sets."""
     inv = 1.0 / n_avg_timesteps
-    for f in SurfProps.FIELDS:
-        a = getattr(surf_props_avg, f)
-        a[...] = a + getattr(surf_props, f) * inv
+    if isinstance(props, PhysProps):
+        if props_avg.ndens_not_Np != props.ndens_not_Np:
+            raise MzError("Inconsistent computation of ndens/number of physical particles in cell")
+        fields = ("lpa", "np", "n", "v", "T")
+    else:
+        fields = SurfProps.FIELDS
+    for f in fields:
+        a = getattr(props_avg, f)
+        a[...] = a + getattr(props, f) * inv
 
 
 def reduce_surf_props(surf_props_target, surf_props_chunks):
@@ -572,6 +590,12 @@ class PhysProps:
         self.v = np.zeros((n_species, n_cells, 3))
         self.T = np.zeros((n_species, n_cells))
         self.moments = np.zeros((n_species, n_cells, self.n_moments))
+
+    def fetch_avg(self, store):
+        """Copy the store's time-averaged props (DeviceStore.props_avg)."""
+        _check(load_library().mz_get_props_avg(store.h, _d(self.lpa), _d(self.np), _d(self.n), _d(self.v),
+                                               _d(self.T)))
+        return self
 
     def fetch(self, store, moments=False):
         _check(load_library().mz_get_props(store.h, _d(self.lpa), _d(self.np), _d(self.n), _d(self.v), _d(self.T)))
@@ -698,6 +722,25 @@ def couette_steps(store, n_steps, step0, dt, species_data=None, V=None, equal_we
     L = load_library()
     _check(L.mz_couette_steps(store.h, n_steps, step0, dt, _d(Vd), 1 if equal_weight else 0, props_every, _d(m),
                               code))
+    return {v: k for k, v in COUETTE_MODES.items()}.get(L.mz_couette_last_path(store.h))
+
+
+def couette_avg_steps(store, n_steps, step0, dt, n_avg_timesteps, species_data=None, V=None, surf_props=None,
+                      mode="graph"):
+    """n_steps of the averaging phase of
+    simulations/1D/couette_with_surface_quantities.jl:73-115: ntc!,
+    convect_particles! with surface properties, avg_props! of SurfProps,
+    sort_particles!, compute_props_sorted!, avg_props! of PhysProps.  The
+    averages stay on the device (PhysProps.fetch_avg, SurfProps.fetch(store,
+    1)); surf_props, if given, sets the element areas.  Returns the path taken."""
+    m = _masses(store, species_data)
+    if surf_props is not None:
+        areas = np.ascontiguousarray(surf_props.areas, dtype=np.float64)
+        _check(load_library().mz_set_surf_areas(store.h, _d(areas)))
+    Vd = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
+    L = load_library()
+    _check(L.mz_couette_avg_steps(store.h, n_steps, step0, dt, _d(Vd), _d(m), COUETTE_MODES[mode],
+                                  float(n_avg_timesteps)))
     return {v: k for k, v in COUETTE_MODES.items()}.get(L.mz_couette_last_path(store.h))
 
 

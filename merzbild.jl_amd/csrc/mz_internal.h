@@ -94,6 +94,8 @@ struct mz_store {
     // set 0 = the last convection's values, 1 = the time average
     double *d_surf = nullptr;
     double surf_inv_area[2] = {1.0, 1.0};
+    // time-averaged PhysProps: [np | n | v (x3) | T] x [species][cell], then lpa[species]
+    double *d_pavg = nullptr;
 };
 
 MzViews mz_views(mz_store *h);
@@ -109,6 +111,11 @@ int mz_launch_sort(mz_store *h, int species, int compute_cells);
 int mz_launch_convect(mz_store *h, int species, double dt, uint64_t tag, int rng_mode, bool contiguous,
                       int32_t n_pos, const uint64_t *step_ptr = nullptr);
 int mz_launch_props(mz_store *h, int variant, int ndens);
+int mz_launch_convect_surf(mz_store *h, int species, double dt, uint64_t tag, int rng_mode, bool contiguous,
+                           int32_t n_pos, double mass, const uint64_t *step_ptr);
+int mz_launch_surf_avg(mz_store *h, double inv);
+int mz_launch_props_avg(mz_store *h, double inv);
+int mz_surf_prealloc(mz_store *h, int species);
 int mz_launch_ntc_equal(mz_store *h, int s1, int s2, double dt, const double *dV, double dw_tol,
                         int equal_weight, uint64_t tag, int rng_mode, int cell_lo, int cell_hi,
                         const uint64_t *step_ptr = nullptr);

@@ -30,26 +30,55 @@ void mz_graph_release(mz_store *h) {
 }
 
 // one step on the current buffers; the host-side buffer swap happens here
+// With avg_inv > 0 the step is one of the averaging phase of
+// simulations/1D/couette_with_surface_quantities.jl:73-115: convection also
+// computes the wall properties and adds them to the time average
+// (avg_props! of SurfProps), and the cell properties are computed every step
+// and added to theirs (avg_props! of PhysProps), each with weight avg_inv.
 static int enqueue_step(mz_store *h, const double *dV, double dt, int do_props, uint64_t step,
-                        const uint64_t *step_ptr) {
+                        const uint64_t *step_ptr, double avg_inv = 0.0, double mass = 0.0) {
     int r = mz_launch_ntc_equal(h, 1, 1, dt, dV, 1e-16, 1, (step << 4) | 1u, 0, 1, h->n_cells, step_ptr);
     if (r) return r;
-    r = mz_launch_convect(h, 1, dt, (step << 4) | 2u, 0, true, 0, step_ptr);
+    if (avg_inv > 0.0) {
+        r = mz_launch_convect_surf(h, 1, dt, (step << 4) | 2u, 0, true, 0, mass, step_ptr);
+        if (r) return r;
+        r = mz_launch_surf_avg(h, avg_inv);
+    } else {
+        r = mz_launch_convect(h, 1, dt, (step << 4) | 2u, 0, true, 0, step_ptr);
+    }
     if (r) return r;
     r = mz_launch_sort(h, 1, 1);
     if (r) return r;
     mz_swap_buffers(h->sp[0]);
-    if (do_props) {
+    if (do_props || avg_inv > 0.0) {
         r = mz_launch_props(h, 1, 0);
         if (r) return r;
     }
+    if (avg_inv > 0.0) return mz_launch_props_avg(h, avg_inv);
     return 0;
 }
 
 extern "C" int mz_couette_last_path(mz_store *h) { return h->last_path; }
 
+static int couette_run(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *V,
+                       int32_t equal_weight, int32_t props_every, const double *masses, int32_t mode,
+                       double avg_inv);
+
 extern "C" int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *V,
                                 int32_t equal_weight, int32_t props_every, const double *masses, int32_t mode) {
+    return couette_run(h, n_steps, step0, dt, V, equal_weight, props_every, masses, mode, 0.0);
+}
+
+extern "C" int mz_couette_avg_steps(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *V,
+                                    const double *masses, int32_t mode, double n_avg_timesteps) {
+    MZ_CHECK(n_avg_timesteps > 0.0, "couette_avg_steps: n_avg_timesteps must be positive");
+    MZ_CHECK(mode == 0 || mode == 1 || mode == 2, "couette_avg_steps: mode is 0 (eager), 1 (graph) or 2 (slab)");
+    return couette_run(h, n_steps, step0, dt, V, 1, 1, masses, mode, 1.0 / n_avg_timesteps);
+}
+
+static int couette_run(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *V,
+                       int32_t equal_weight, int32_t props_every, const double *masses, int32_t mode,
+                       double avg_inv) {
     MZ_CHECK(h->n_species == 1, "couette_steps: single-species stores only");
     MZ_CHECK(h->has_grid && h->has_walls, "couette_steps: grid or walls not set");
     MZ_CHECK(h->inter[0].set, "couette_steps: interaction not set");
@@ -70,7 +99,13 @@ extern "C" int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, do
         r = mz_squash(h, 1);
         if (r) return r;
     }
-    if (mode == 2 && equal_weight) {
+    const double mass = masses[0];
+    if (avg_inv > 0.0) {
+        MZ_CHECK(mass > 0.0 && dt > 0.0, "couette_avg_steps: wall properties need mass > 0 and dt > 0");
+        r = mz_surf_prealloc(h, 1);
+        if (r) return r;
+    }
+    if (mode == 2 && equal_weight && avg_inv == 0.0) {
         // slab-resident kernel; falls through to the graph when the state or
         // shape does not allow it or when the launch aborted (state restored)
         int ran = 0;
@@ -82,6 +117,7 @@ extern "C" int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, do
         }
         mode = 1;
     }
+    if (mode == 2) mode = 1;   // the averaging phase runs on the step graph
     const bool graph_ok = mode == 1 && equal_weight && (props_every == 0 || props_every == 1);
     h->last_path = graph_ok ? 1 : 0;
     if (!graph_ok) {
@@ -91,7 +127,7 @@ extern "C" int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, do
             uint64_t step = step0 + (uint64_t)t;
             int do_props = props_every > 0 && ((t + 1) % props_every) == 0;
             if (equal_weight) {
-                r = enqueue_step(h, dV, dt, do_props, step, nullptr);
+                r = enqueue_step(h, dV, dt, do_props, step, nullptr, avg_inv, mass);
                 if (r) return r;
             } else {
                 r = mz_ntc(h, 1, 1, dt, V, 1e-16, 0, (step << 4) | 1u, 0, 1, h->n_cells);
@@ -111,7 +147,8 @@ extern "C" int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, do
     }
     // graph replay
     char keybuf[256];
-    std::snprintf(keybuf, sizeof(keybuf), "couette:%a:%d:%p", dt, props_every, (const void *)dV);
+    std::snprintf(keybuf, sizeof(keybuf), "couette:%a:%d:%p:%a:%a:%a:%a", dt, props_every, (const void *)dV,
+                  avg_inv, mass, h->surf_inv_area[0], h->surf_inv_area[1]);
     if (h->graph_key != keybuf) {
         mz_graph_release(h);
         MZ_HIP(hipMalloc(&h->d_step, sizeof(uint64_t)));
@@ -123,7 +160,7 @@ extern "C" int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, do
         MZ_HIP(hipStreamSynchronize(h->stream));
         MZ_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
         for (int rep = 0; rep < 2; ++rep) {
-            r = enqueue_step(h, dV, dt, props_every == 1, 0, h->d_step);
+            r = enqueue_step(h, dV, dt, props_every == 1, 0, h->d_step, avg_inv, mass);
             if (r) {
                 hipGraph_t g;
                 hipStreamEndCapture(h->stream, &g);
@@ -141,7 +178,7 @@ extern "C" int mz_couette_steps(mz_store *h, int32_t n_steps, uint64_t step0, do
     for (int32_t t = 0; t + 1 < n_steps; t += 2) MZ_HIP(hipGraphLaunch(h->graph_exec, h->stream));
     if (n_steps & 1) {
         uint64_t step = step0 + (uint64_t)(n_steps - 1);
-        r = enqueue_step(h, dV, dt, props_every == 1, step, nullptr);
+        r = enqueue_step(h, dV, dt, props_every == 1, step, nullptr, avg_inv, mass);
         if (r) return r;
     }
     mz_timing_end_n(h, n_steps);

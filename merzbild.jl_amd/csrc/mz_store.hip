@@ -172,6 +172,9 @@ extern "C" int mz_create(int32_t n_cells, int32_t n_species, const int64_t *capa
     const size_t surf_bytes = (size_t)2 * n_species * 2 * MZ_SURF_Q * sizeof(double);
     MZ_HIP(hipMalloc(&h->d_surf, surf_bytes));
     MZ_HIP(hipMemset(h->d_surf, 0, surf_bytes));
+    const size_t pavg_bytes = (pn * 6 + n_species) * sizeof(double);
+    MZ_HIP(hipMalloc(&h->d_pavg, pavg_bytes));
+    MZ_HIP(hipMemset(h->d_pavg, 0, pavg_bytes));
     h->inter.resize(nss);
     MZ_HIP(hipStreamSynchronize(h->stream));
     *out = h;
@@ -199,6 +202,7 @@ extern "C" int mz_destroy(mz_store *h) {
     hipFree(h->d_invV);
     hipFree(h->d_masses);
     hipFree(h->d_surf);
+    hipFree(h->d_pavg);
     for (int k = 0; k < 3; ++k) if (h->d_replay[k]) hipFree(h->d_replay[k]);
     for (auto &sc : h->scratch) if (sc.second.ptr) hipFree(sc.second.ptr);
     hipStreamDestroy(h->stream);
@@ -993,6 +997,16 @@ __global__ void k_surf_fill(double *__restrict__ a, int32_t n) {
     if (i < n) a[i] = 0.0;
 }
 
+// Partial sums for every block a launch over the species' slots can have
+// (allocated before a step graph is captured, which cannot allocate).
+int mz_surf_prealloc(mz_store *h, int species) {
+    const int64_t blocks = (h->sp[species - 1].cap + MZ_SURF_BLOCK - 1) / MZ_SURF_BLOCK;
+    MZ_CHECK(mz_scratch(h, "surf_part", (size_t)blocks * 2 * MZ_SURF_Q * sizeof(double)) != nullptr &&
+                 mz_scratch(h, "surf_flag", (size_t)blocks * sizeof(int32_t)) != nullptr,
+             "convect: scratch allocation failed");
+    return 0;
+}
+
 int mz_launch_convect_surf(mz_store *h, int species, double dt, uint64_t tag, int rng_mode, bool contiguous,
                            int32_t n_pos, double mass, const uint64_t *step_ptr) {
     MzSpecies &s = h->sp[species - 1];
@@ -1011,9 +1025,10 @@ int mz_launch_convect_surf(mz_store *h, int species, double dt, uint64_t tag, in
     rp.phase = 2;
     const int64_t n = contiguous ? s.cap : std::max<int64_t>(n_pos, 1);
     const int blocks = (int)((n + MZ_SURF_BLOCK - 1) / MZ_SURF_BLOCK);
-    double *part = (double *)mz_scratch(h, "surf_part", (size_t)blocks * 2 * MZ_SURF_Q * sizeof(double));
-    int32_t *pflag = (int32_t *)mz_scratch(h, "surf_flag", (size_t)blocks * sizeof(int32_t));
-    MZ_CHECK(part && pflag, "convect: scratch allocation failed");
+    int r = mz_surf_prealloc(h, species);
+    if (r) return r;
+    double *part = (double *)mz_scratch(h, "surf_part", 0);
+    int32_t *pflag = (int32_t *)mz_scratch(h, "surf_flag", 0);
     hipLaunchKernelGGL(k_convect_surf, dim3(blocks), dim3(MZ_SURF_BLOCK), 0, h->stream, mz_view(s), h->grid,
                        h->walls, species - 1, dt, rp, mask, part, pflag);
     const double fb = mass / dt;
@@ -1044,12 +1059,62 @@ extern "C" int mz_set_surf_areas(mz_store *h, const double *areas) {
     return 0;
 }
 
+int mz_launch_surf_avg(mz_store *h, double inv) {
+    const int32_t n = h->n_species * 2 * MZ_SURF_Q;
+    hipLaunchKernelGGL(k_surf_avg, dim3(1), dim3(256), 0, h->stream, h->d_surf + n, h->d_surf, n, inv);
+    MZ_HIP(hipGetLastError());
+    return 0;
+}
+
 extern "C" int mz_surf_avg(mz_store *h, double n_avg_timesteps) {
     MZ_CHECK(n_avg_timesteps != 0.0, "surf_avg: zero averaging count");
-    const int32_t n = h->n_species * 2 * MZ_SURF_Q;
-    hipLaunchKernelGGL(k_surf_avg, dim3(1), dim3(256), 0, h->stream, h->d_surf + n, h->d_surf, n,
-                       1.0 / n_avg_timesteps);
+    return mz_launch_surf_avg(h, 1.0 / n_avg_timesteps);
+}
+
+// avg_props!(phys_props_avg, phys_props, n) (physical_props.jl:281-299):
+// avg = muladd(props, 1 / n, avg) for np, n, v and T of every cell and
+// species; lpa is averaged from the particle vector's length.
+__global__ void k_props_avg(double *__restrict__ avg, const double *__restrict__ np, const double *__restrict__ nd,
+                            const double *__restrict__ v, const double *__restrict__ T, int32_t pn, double inv,
+                            MzViews views, int32_t n_species) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < pn) {
+        avg[i] = fma(np[i], inv, avg[i]);
+        avg[pn + i] = fma(nd[i], inv, avg[pn + i]);
+        for (int d = 0; d < 3; ++d) avg[2 * pn + 3 * i + d] = fma(v[3 * i + d], inv, avg[2 * pn + 3 * i + d]);
+        avg[5 * pn + i] = fma(T[i], inv, avg[5 * pn + i]);
+    }
+    if (i < n_species) avg[6 * pn + i] = fma((double)views.v[i].ctr->length, inv, avg[6 * pn + i]);
+}
+
+int mz_launch_props_avg(mz_store *h, double inv) {
+    const int32_t pn = h->n_species * h->n_cells;
+    hipLaunchKernelGGL(k_props_avg, dim3((pn + 255) / 256), dim3(256), 0, h->stream, h->d_pavg, h->d_np, h->d_n,
+                       h->d_v, h->d_T, pn, inv, mz_views(h), h->n_species);
     MZ_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int mz_props_avg(mz_store *h, double n_avg_timesteps) {
+    MZ_CHECK(n_avg_timesteps != 0.0, "props_avg: zero averaging count");
+    return mz_launch_props_avg(h, 1.0 / n_avg_timesteps);
+}
+
+extern "C" int mz_props_avg_clear(mz_store *h) {
+    const int32_t n = h->n_species * h->n_cells * 6 + h->n_species;
+    hipLaunchKernelGGL(k_surf_fill, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_pavg, n);
+    MZ_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int mz_get_props_avg(mz_store *h, double *lpa, double *np, double *n, double *v, double *T) {
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    const size_t pn = (size_t)h->n_species * h->n_cells;
+    if (np) MZ_HIP(hipMemcpy(np, h->d_pavg, pn * sizeof(double), hipMemcpyDeviceToHost));
+    if (n) MZ_HIP(hipMemcpy(n, h->d_pavg + pn, pn * sizeof(double), hipMemcpyDeviceToHost));
+    if (v) MZ_HIP(hipMemcpy(v, h->d_pavg + 2 * pn, 3 * pn * sizeof(double), hipMemcpyDeviceToHost));
+    if (T) MZ_HIP(hipMemcpy(T, h->d_pavg + 5 * pn, pn * sizeof(double), hipMemcpyDeviceToHost));
+    if (lpa) MZ_HIP(hipMemcpy(lpa, h->d_pavg + 6 * pn, h->n_species * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -189,6 +189,12 @@ typedef struct {
     const int32_t *moment_powers;
 } mzo_props;
 
+/* avg_props!(phys_props_avg, phys_props, n_avg_timesteps)
+ * (physical_props.jl:281-299): avg = muladd(props, 1 / n, avg) for lpa, np,
+ * n, v and T.  Parity: pinned by closed forms (averaging constant props gives
+ * them back; tests/test_oracle_surf.py). */
+void mzo_props_avg(mzo_props *avg, const mzo_props *p, double n_avg_timesteps);
+
 void mzo_compute_props(mzo_pv **pvs, mzo_pia *pia, const double *masses, mzo_props *pp);
 void mzo_compute_props_sorted(mzo_pv **pvs, mzo_pia *pia, const double *masses, mzo_props *pp,
                               const double *inv_V, int64_t cell_lo, int64_t cell_hi);

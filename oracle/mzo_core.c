@@ -688,6 +688,18 @@ void mzo_convect_particles_surf(mzo_rng *rng, const mzo_grid1d *g, const mzo_wal
     mzo_surf_scale(sp, species, mass, dt);
 }
 
+void mzo_props_avg(mzo_props *avg, const mzo_props *p, double n_avg_timesteps) {
+    const double inv = 1.0 / n_avg_timesteps;
+    const int64_t pn = p->n_cells * p->n_species;
+    for (int64_t s = 0; s < p->n_species; ++s) avg->lpa[s] = fma(p->lpa[s], inv, avg->lpa[s]);
+    for (int64_t i = 0; i < pn; ++i) {
+        avg->np[i] = fma(p->np[i], inv, avg->np[i]);
+        avg->n[i] = fma(p->n[i], inv, avg->n[i]);
+        for (int d = 0; d < 3; ++d) avg->v[3 * i + d] = fma(p->v[3 * i + d], inv, avg->v[3 * i + d]);
+        avg->T[i] = fma(p->T[i], inv, avg->T[i]);
+    }
+}
+
 /* One particle (physical id) through its own stream; used by the parallel
  * drivers. */
 void mzo_convect_one(mzo_rng *rng, const mzo_grid1d *g, const mzo_walls *wl, mzo_pv *pv, int64_t id,

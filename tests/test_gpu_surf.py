@@ -174,3 +174,87 @@ def test_surf_sets_on_device(hip, mzo):
     assert not got.np[:, 0].any() and not got.force[:, :, 0].any()
     assert got.np[:, 1].sum() > 0
     st.close()
+
+
+def test_equilibrium_wall_pressure_device(hip):
+    """200,000 particles, 500 steps of device convection between diffuse
+    walls at the gas temperature, averaged on the device (avg_props!): the
+    wall pressure, incident mass flux and exchanges match kinetic theory
+    (about 500,000 contacts per wall; 1 % is about ten standard deviations)."""
+    from test_oracle_surf import _equilibrium, check_equilibrium, equilibrium_expectations
+    n, L, T, dt, steps = 200000, 1e-3, 300.0, 5e-8, 500
+    w = 1e23 * L / n
+    x, v = _equilibrium(n, L, T, 33)
+    st = hip.DeviceStore(10, 1, n + 1024)
+    pia = np.zeros((10, 7), dtype=np.int64)
+    pia[:, 2] = pia[:, 5] = -1
+    pia[0] = (n, 1, n, n, 0, -1, 0)
+    xs = np.column_stack([x, np.zeros(n), np.zeros(n)])
+    st.upload_species(1, xs, v, np.full(n, w), np.arange(1, n + 1), pia, n, True)
+    sp = [hip.Species("Ar", helpers.MASS_AR)]
+    st.set_species_data(sp)
+    st.set_grid(hip.Grid1DUniform(L, 10))
+    st.set_walls(hip.MaxwellWalls1D(sp, T, T, 0.0, 0.0, 1.0, 1.0))
+    st.set_seed(17)
+    surf = hip.SurfProps.grid1d(1)
+    st.surf_clear(1)
+    for t in range(steps):
+        hip.convect_particles(st, 1, dt, tag=(t << 4) | 2, species_data=sp, surf_props=surf, fetch=False)
+        st.surf_avg(steps)
+    avg = hip.SurfProps.grid1d(1).fetch(st, 1)._to_device()[0]
+    assert st.scalars(1)["err"] == 0
+    check_equilibrium(avg, equilibrium_expectations(n, L, T, w), 0.01)
+    st.close()
+
+
+@pytest.mark.parametrize("mode", ["eager", "graph"])
+def test_couette_avg_steps_vs_oracle(hip, mzo, mode):
+    """Twelve steps of the averaging phase of
+    simulations/1D/couette_with_surface_quantities.jl (ntc!, convection with
+    wall properties, their time average, sort, compute_props_sorted!, the
+    props' time average), kernel by kernel or as a replayed hipGraph, against
+    the oracle: particle state as in the plain Couette parity test, averaged
+    np bit for bit, averaged n, v, T to 1e-11, averaged wall properties to
+    1e-12 of their magnitude sums."""
+    steps, n_avg = 12, 12
+    case = helpers.couette_case(mzo, n_cells=20, ppc=100, seed=3, dt=2e-9, sgwm_mult=50.0)
+    st = helpers.store_from_case(hip, case)
+    st.surf_clear(1)
+    st.props_avg_clear()
+    path = hip.couette_avg_steps(st, steps, 1, case["dt"], n_avg, species_data=case["species"], mode=mode)
+    assert path == mode
+    pv, pia, g = case["pv"], case["pia"], case["grid"]
+    inst, avg = mzo.Surf.grid1d(1), mzo.Surf.grid1d(1)
+    pp, pavg = mzo.Props(case["n_cells"], 1), mzo.Props(case["n_cells"], 1)
+    n = pia.n_total()
+    mom, ke, flux = 0.0, 0.0, 0.0
+    f = helpers.MASS_AR / case["dt"]
+    for step in range(1, steps + 1):
+        r = mzo.RNG(seed=case["seed"], tag=(step << 4) | 1)
+        for c in range(1, case["n_cells"] + 1):
+            mzo.lib().mzo_ntc_single(r.ref, C.byref(case["cfs"][c - 1]), C.byref(case["it"]), pv.p, pia.p, c, 1,
+                                     case["dt"], g.dx, 1e-16, 1)
+        _, vb, wb, _ = pv.logical_arrays(n)
+        mzo.convect_particles_surf(mzo.RNG(seed=case["seed"], tag=(step << 4) | 2), g, case["walls"], pv, pia, 1,
+                                   case["dt"], inst, helpers.MASS_AR)
+        _, va, _, _ = pv.logical_arrays(n)
+        s = _scales(vb, va, wb, f) / n_avg
+        flux, mom, ke = flux + s[1], mom + s[3], ke + s[10]
+        avg.avg_from(inst, n_avg)
+        mzo.lib().mzo_sort_particles(C.byref(g), pv.p, pia.p, 1)
+        mzo.compute_props_sorted([pv], pia, [helpers.MASS_AR], pp)
+        pp.lpa[0] = pv.length
+        mzo.lib().mzo_props_avg(pavg.ref, pp.ref, float(n_avg))
+    got = st.download_species(1)
+    assert got["err"] == 0
+    helpers.assert_state_matches(mzo, got, pv, pia, rtol=1e-11)
+    gp = hip.PhysProps(case["n_cells"], 1).fetch_avg(st)
+    assert np.array_equal(gp.np, pavg.np)
+    assert gp.lpa[0] == pavg.lpa[0]
+    for name in ("n", "v", "T"):
+        a, b = getattr(gp, name), getattr(pavg, name)
+        assert np.allclose(a, b, rtol=1e-11, atol=0), name
+    sa = hip.SurfProps.grid1d(1).fetch(st, 1)._to_device()
+    assert avg.np.sum() > 0
+    _compare(sa, avg.q, np.array([0.0, flux, flux] + [mom] * 7 + [ke]))
+    st.close()

@@ -259,3 +259,57 @@ def test_package_surfprops_helpers_kat():
     back = pkg.SurfProps.grid1d(1)._from_device(q.ravel())
     for k in pkg.SurfProps.FIELDS:
         assert np.array_equal(getattr(back, k), getattr(sp, k))
+
+
+def _equilibrium(n, L, T, seed):
+    """n argon particles uniform in (0, L), Maxwellian at T, at rest."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.0, L, n)
+    v = rng.normal(0.0, np.sqrt(1.380649e-23 * T / M_AR), (n, 3))
+    return x, v
+
+
+def equilibrium_expectations(n, L, T, w):
+    """Kinetic theory of a gas at rest and in equilibrium with diffuse walls
+    at its own temperature: pressure n k T on each wall, incident number flux
+    n sqrt(k T / (2 pi m)), no net energy or tangential momentum exchange."""
+    kb = 1.380649e-23
+    ndens = n * w / L
+    flux = ndens * np.sqrt(kb * T / (2 * np.pi * M_AR))
+    return dict(p=ndens * kb * T, mass_flux=M_AR * flux, energy_flux_in=2 * kb * T * flux)
+
+
+def check_equilibrium(avg, e, tol):
+    """avg: time-averaged (np, flux_incident, flux_reflected, force[3],
+    normal_pressure, shear[3], ke) per element, rows = elements."""
+    for el, sgn in ((0, -1.0), (1, 1.0)):
+        q = avg[el]
+        assert abs(q[6] / e["p"] - 1) < tol, (el, q[6] / e["p"])
+        assert abs(sgn * q[3] / e["p"] - 1) < tol, (el, q[3] / e["p"])
+        assert abs(q[1] / e["mass_flux"] - 1) < tol, (el, q[1] / e["mass_flux"])
+        assert q[2] == -q[1]
+        assert abs(q[10]) < 3 * tol * e["energy_flux_in"], (el, q[10] / e["energy_flux_in"])
+        for k in (4, 5, 8, 9):
+            assert abs(q[k]) < 2 * tol * e["p"], (el, k, q[k] / e["p"])
+
+
+def test_equilibrium_wall_pressure(mzo):
+    """20,000 particles, 200 steps of convection between diffuse walls at the
+    gas temperature: the time-averaged wall properties match kinetic theory
+    (about 20,000 contacts per wall; 4 % is ten standard deviations)."""
+    n, L, T, dt = 20000, 1e-3, 300.0, 5e-8
+    w = 1e23 * L / n
+    x, v = _equilibrium(n, L, T, 21)
+    pv = make_pv(mzo, [(w, v[i], [x[i], 0.0, 0.0]) for i in range(n)])
+    pia = mzo.PIA(10, 1)
+    pia.set(1, 1, n, 1, n, n)
+    pia.set_n_total(1, n)
+    grid = mzo.grid1d(L, 10)
+    mzo.lib().mzo_sort_particles(mzo.C.byref(grid), pv.p, pia.p, 1)
+    wl = mzo.walls([M_AR], T, T, 0.0, 0.0, 1.0, 1.0)
+    inst, avg = mzo.Surf.grid1d(1), mzo.Surf.grid1d(1)
+    steps = 200
+    for t in range(steps):
+        mzo.convect_particles_surf(mzo.RNG(seed=9, tag=(t << 4) | 2), grid, wl, pv, pia, 1, dt, inst, M_AR)
+        avg.avg_from(inst, steps)
+    check_equilibrium(avg.q[0], equilibrium_expectations(n, L, T, w), 0.04)
