@@ -121,4 +121,4 @@ int mz_launch_ntc_equal(mz_store *h, int s1, int s2, double dt, const double *dV
                         const uint64_t *step_ptr = nullptr);
 int mz_timing_end_n(mz_store *h, int64_t n);
 int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *dV, int32_t props_every,
-                    int *ran);
+                    int *ran, double avg_inv = 0.0, double mass = 0.0);

@@ -197,6 +197,26 @@ __device__ inline void mz_convect_impl(const MzRngParams &rp, const MzGrid &g, c
     x0 = x_new;
 }
 
+// Term k of one wall contact, as MzSurfAcc::hit forms it.
+__device__ inline double mz_surf_term(int k, int bc, double w, double v0, double v1, double v2, double sgn) {
+    const double n0 = bc == 0 ? 1.0 : -1.0, n1 = 0.0, n2 = 0.0;
+    const double px = w * v0, py = w * v1, pz = w * v2;
+    const double pdn = fma(px, n0, fma(py, n1, pz * n2));
+    switch (k) {
+    case 0: return sgn > 0.0 ? 1.0 : 0.0;
+    case 1: return sgn > 0.0 ? w : 0.0;
+    case 2: return sgn > 0.0 ? 0.0 : -w;
+    case 3: return sgn * px;
+    case 4: return sgn * py;
+    case 5: return sgn * pz;
+    case 6: return -sgn * pdn;
+    case 7: return sgn * fma(-pdn, n0, px);
+    case 8: return sgn * fma(-pdn, n1, py);
+    case 9: return sgn * fma(-pdn, n2, pz);
+    default: return sgn * (0.5 * fma(px, v0, fma(py, v1, pz * v2)));
+    }
+}
+
 // Walls without surface properties.
 struct MzNoSurf {
     __device__ void hit(int, double, double, double, double) {}
@@ -210,8 +230,10 @@ __device__ inline void mz_convect_one(const MzRngParams &rp, const MzGrid &g, co
     mz_convect_impl(rp, g, wl, sidx, dt, id, x0, v0, v1, v2, err, pre_bc, p0, p1, p2, ns);
 }
 
-// The same flight with every wall contact added to sa (sa.w: the particle's weight).
-__device__ inline void mz_convect_one_surf(MzSurfAcc &sa, const MzRngParams &rp, const MzGrid &g, const MzWalls &wl,
+// The same flight with every wall contact passed to sa.hit (an MzSurfAcc, or
+// a recorder that keeps the contacts for a later sum).
+template <class Acc>
+__device__ inline void mz_convect_one_surf(Acc &sa, const MzRngParams &rp, const MzGrid &g, const MzWalls &wl,
                                            int sidx, double dt, int32_t id, double &x0, double &v0, double &v1,
                                            double &v2, int32_t *err, int pre_bc = -1, double p0 = 0.0,
                                            double p1 = 0.0, double p2 = 0.0) {

@@ -64,6 +64,41 @@ struct SlabArgs {
     uint64_t *flag;             // [slab][side] x SLAB_LINE: [parity] = (step + 1) << 32 | count
     uint64_t *fin;              // [slab] x SLAB_LINE: [0] = 1 when done, [1] = counts
     int32_t *abort_flag;
+    // averaging phase (avg_inv > 0): wall and cell properties of every step
+    // added to their time averages with weight avg_inv
+    double avg_inv;
+    double surf_f[2];           // mass / dt / area of the two walls
+    double *surf_part;          // [slab][2][2 MZ_SURF_Q]: the slab's average increments, its last step's values
+    double *pavg;               // time-averaged props, store layout
+};
+
+// Wall contacts of one step in the averaging phase: each wave appends its
+// lanes' contacts (wall, weight, incident and reflected velocity) to its own
+// list in lane order, and one wave sums the lists in wave order during the
+// next step's collisions, so the sums are the same on every run.
+#define SLAB_SURF_CAP 16   // contacts per wave and step; more abort the launch
+
+struct SlabSurfRec {
+    double (*rec)[8];          // this wave's list
+    volatile int32_t *cnt;     // this wave's count
+    int32_t *abort;
+    double w, u0, u1, u2;      // weight, incident velocity
+    __device__ void hit(int bc, double v0, double v1, double v2, double sgn) {
+        if (sgn > 0.0) {
+            u0 = v0; u1 = v1; u2 = v2;
+            return;
+        }
+        const uint64_t m = __ballot(1);
+        const int32_t base = *cnt;
+        const int slot = base + __popcll(m & mz_lanemask_lt());
+        if (slot < SLAB_SURF_CAP) {
+            double *e = rec[slot];
+            e[0] = (double)bc; e[1] = w; e[2] = u0; e[3] = u1; e[4] = u2; e[5] = v0; e[6] = v1; e[7] = v2;
+        } else {
+            *abort = 1;
+        }
+        if ((int)(threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1) *cnt = base + __popcll(m);
+    }
 };
 
 __device__ inline uint64_t *slab_mb(const SlabArgs &A, int slab, int side, int parity, int field) {
@@ -161,7 +196,7 @@ __device__ inline void slab_ntc_cell(const SlabArgs &A, const MzRngParams &rp, i
     n_coll_out = (int32_t)n_coll;
 }
 
-template <int R>
+template <int R, bool AVG>
 __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(SlabArgs A) {
     constexpr int P = R * SLAB_THREADS;   // LDS particle capacity
     constexpr int NCH = P / 64;           // 64-particle chunks
@@ -184,6 +219,10 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
     __shared__ int32_t swf[WC];                    // 1 where swl holds the entry
     __shared__ int32_t s_abort, s_n, s_na[2];
     __shared__ int64_t s_red[4][4];
+    constexpr int SC = AVG ? SLAB_SURF_CAP : 1;
+    __shared__ double srec[AVG ? 4 : 1][SC][8];   // wall contacts of the last convection, per wave
+    __shared__ int32_t scnt[4];
+    __shared__ double pav[AVG ? SLAB_MAXC : 1][6];  // the slab's cells' time-averaged props
 
     const int b = blockIdx.x;
     const int g = (b & 7) * A.spx + (b >> 3);   // consecutive slabs on one XCD
@@ -274,6 +313,15 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
                 A.out_v[3 * c + 1] = o.v1;
                 A.out_v[3 * c + 2] = o.v2;
                 A.out_T[c] = o.T;
+                if (AVG) {
+                    const double iv = A.avg_inv;
+                    pav[k][0] = fma((double)tn, iv, pav[k][0]);
+                    pav[k][1] = fma(o.n, iv, pav[k][1]);
+                    pav[k][2] = fma(o.v0, iv, pav[k][2]);
+                    pav[k][3] = fma(o.v1, iv, pav[k][3]);
+                    pav[k][4] = fma(o.v2, iv, pav[k][4]);
+                    pav[k][5] = fma(o.T, iv, pav[k][5]);
+                }
             }
         }
     };
@@ -326,6 +374,38 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
         }
     };
     if (A.n_steps > 0) fill_draws(A.step0, tid, SLAB_THREADS);
+    const int32_t pn = A.n_cells;
+    if (AVG) {
+        if (tid < 4) scnt[tid] = 0;
+        if (tid < ncl * 6) {
+            const int k = tid / 6, f = tid % 6, c = clo + k;
+            pav[k][f] = f == 0 ? A.pavg[c] : f == 1 ? A.pavg[pn + c]
+                      : f < 5 ? A.pavg[2 * pn + 3 * c + (f - 2)] : A.pavg[5 * pn + c];
+        }
+    }
+    // wave 3, lanes 0 .. 2 MZ_SURF_Q - 1: quantity (lane / MZ_SURF_Q, lane %
+    // MZ_SURF_Q) of the slab's wall properties, its running time average and
+    // its value in the last step
+    double surf_acc = 0.0, surf_last = 0.0;
+    auto surf_reduce = [&]() {
+        if (lane < 2 * MZ_SURF_Q) {
+            const int e = lane / MZ_SURF_Q, k = lane % MZ_SURF_Q;
+            double sum = 0.0;
+            for (int w = 0; w < 4; ++w) {
+                const int n = min((int)scnt[w], SLAB_SURF_CAP);
+                for (int j = 0; j < n; ++j) {
+                    const double *r = srec[w][j];
+                    if ((int)r[0] != e) continue;
+                    sum += mz_surf_term(k, e, r[1], r[2], r[3], r[4], 1.0);
+                    sum += mz_surf_term(k, e, r[1], r[5], r[6], r[7], -1.0);
+                }
+            }
+            const double inst = sum * (k == 0 ? 1.0 : A.surf_f[e]);
+            surf_acc = fma(inst, A.avg_inv, surf_acc);
+            surf_last = inst;
+        }
+        if (lane < 4) scnt[lane] = 0;
+    };
     __syncthreads();
 
     for (int32_t t = 0; t < A.n_steps; ++t) {
@@ -346,6 +426,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
                           my_ncoll, my_perf, my_eqw);
         }
         if (wall_side >= 0 && wave >= wall_w0) wall_draws(step);
+        if (AVG && t > 0 && wave == 3) surf_reduce();
         __syncthreads();
         // ---- 2. convection, new cell, rank inside 64-particle chunks
         {
@@ -375,8 +456,14 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
                                 pre_bc = wall_side;
                             }
                         }
-                        mz_convect_one(rp, A.g, A.wl, 0, A.dt, pid[r], px0[r], pv0[r], pv1[r], pv2[r],
-                                       &A.a.ctr->err, pre_bc, p0, p1, p2);
+                        if (AVG) {
+                            SlabSurfRec sr{srec[wave], &scnt[wave], &s_abort, pw[r], 0.0, 0.0, 0.0};
+                            mz_convect_one_surf(sr, rp, A.g, A.wl, 0, A.dt, pid[r], px0[r], pv0[r], pv1[r], pv2[r],
+                                                &A.a.ctr->err, pre_bc, p0, p1, p2);
+                        } else {
+                            mz_convect_one(rp, A.g, A.wl, 0, A.dt, pid[r], px0[r], pv0[r], pv1[r], pv2[r],
+                                           &A.a.ctr->err, pre_bc, p0, p1, p2);
+                        }
                     }
                     const int c = mz_cell_of(px0[r], A.g, A.n_cells, &A.a.ctr->err) - 1;
                     pcell[r] = c + 1;
@@ -648,6 +735,7 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
     }
     if (aborted) return;
     if (SNAP && A.n_steps > 0 && props_due(A.n_steps - 1)) props_cells();
+    if (AVG && A.n_steps > 0 && wave == 3) surf_reduce();
 
     // ---- publish the slab's counts, then place it after all earlier slabs
     if (tid == 0) {
@@ -729,6 +817,41 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
         A.a.ctr->contiguous = 1;
         A.a.ctr->n_pos = (int32_t)total;
     }
+    if (AVG) {
+        // the averages: cells from LDS (each written by its props lane, the
+        // barrier above orders it), the wall increments per slab, lpa from
+        // the vector's length, unchanged by the slab steps
+        if (tid < ncl * 6) {
+            const int k = tid / 6, f = tid % 6, c = clo + k;
+            double *d = f == 0 ? A.pavg + c : f == 1 ? A.pavg + pn + c
+                      : f < 5 ? A.pavg + 2 * pn + 3 * c + (f - 2) : A.pavg + 5 * pn + c;
+            *d = pav[k][f];
+        }
+        if (wave == 3 && lane < 2 * MZ_SURF_Q) {
+            A.surf_part[((size_t)g * 2) * 2 * MZ_SURF_Q + lane] = surf_acc;
+            A.surf_part[((size_t)g * 2 + 1) * 2 * MZ_SURF_Q + lane] = surf_last;
+        }
+        if (g == 0 && tid == 0) {
+            const double len = (double)A.a.ctr->length;
+            double l = A.pavg[6 * pn];
+            for (int32_t t = 0; t < A.n_steps; ++t) l = fma(len, A.avg_inv, l);
+            A.pavg[6 * pn] = l;
+        }
+    }
+}
+
+// After an averaging-phase launch: the slabs' wall increments added to the
+// time average in slab order, and the last step's values summed the same way.
+__global__ void k_slab_surf_finalize(const double *__restrict__ part, int32_t n_slabs, double *__restrict__ surf) {
+    const int i = threadIdx.x;
+    if (i >= 2 * MZ_SURF_Q) return;
+    double acc = 0.0, last = 0.0;
+    for (int32_t g = 0; g < n_slabs; ++g) {
+        acc += part[((size_t)g * 2) * 2 * MZ_SURF_Q + i];
+        last += part[((size_t)g * 2 + 1) * 2 * MZ_SURF_Q + i];
+    }
+    surf[2 * MZ_SURF_Q + i] += acc;   // set 1 (one species)
+    surf[i] = last;                   // set 0
 }
 
 // ---------------------------------------------------------------------------
@@ -743,12 +866,12 @@ struct SlabShape {
     SlabKernel fn = nullptr;
 };
 
-SlabKernel slab_kernel(int R) {
+SlabKernel slab_kernel(int R, bool avg = false) {
     switch (R) {
-    case 1: return k_slab_couette<1>;
-    case 2: return k_slab_couette<2>;
-    case 4: return k_slab_couette<4>;
-    case 8: return k_slab_couette<8>;
+    case 1: return avg ? k_slab_couette<1, true> : k_slab_couette<1, false>;
+    case 2: return avg ? k_slab_couette<2, true> : k_slab_couette<2, false>;
+    case 4: return avg ? k_slab_couette<4, true> : k_slab_couette<4, false>;
+    case 8: return avg ? k_slab_couette<8, true> : k_slab_couette<8, false>;
     default: return nullptr;
     }
 }
@@ -796,7 +919,8 @@ extern "C" int mz_get_slab_info(mz_store *h, int32_t *info) {
 // Runs n_steps on the slab kernel.  *ran = 1 when it did; 0 when the state or
 // the shape does not allow it, or when the launch aborted (state restored).
 int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *dV, int32_t props_every,
-                    int *ran) {
+                    int *ran, double avg_inv, double mass) {
+    const bool avg = avg_inv > 0.0;
     *ran = 0;
     MzSpecies &s = h->sp[0];
     const int nc = h->n_cells;
@@ -832,7 +956,7 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
     SlabShape sh;
     for (int R : Rs) {
         if (h->slab_ppt && R != h->slab_ppt) continue;
-        SlabKernel fn = slab_kernel(R);
+        SlabKernel fn = slab_kernel(R, avg);
         int per_cu = 0;
         MZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, SLAB_THREADS, 0));
         const int max_wg = per_cu * n_cu;
@@ -864,7 +988,9 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
     const size_t bak_bytes = (size_t)nc * (7 * sizeof(int32_t) + sizeof(double) + 5 * sizeof(int32_t)) +
                              sizeof(MzCounters);
     char *bak = (char *)mz_scratch(h, "slab_backup", bak_bytes);
-    MZ_CHECK(mb && fl && bak, "couette slab: scratch allocation failed");
+    double *surf_part = nullptr;
+    if (avg) surf_part = (double *)mz_scratch(h, "slab_surf", (size_t)sh.n_slabs * 4 * MZ_SURF_Q * sizeof(double));
+    MZ_CHECK(mb && fl && bak && (!avg || surf_part), "couette slab: scratch allocation failed");
     uint64_t *fin = fl + fl_words;
     int32_t *abort_flag = (int32_t *)(fin + fin_words);
     MZ_HIP(hipMemsetAsync(fl, 0, (fl_words + fin_words + SLAB_LINE) * 8, h->stream));
@@ -907,6 +1033,11 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
     A.flag = fl;
     A.fin = fin;
     A.abort_flag = abort_flag;
+    A.avg_inv = avg_inv;
+    A.surf_f[0] = mass / dt * h->surf_inv_area[0];
+    A.surf_f[1] = mass / dt * h->surf_inv_area[1];
+    A.surf_part = surf_part;
+    A.pavg = h->d_pavg;
     void *args[] = {&A};
     mz_timing_begin(h, "slab");
     MZ_HIP(hipLaunchCooperativeKernel((const void *)sh.fn, dim3(sh.grid), dim3(SLAB_THREADS), args, 0, h->stream));
@@ -931,6 +1062,10 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
         MZ_HIP(hipMemcpyAsync(s.ctr, bak + off, sizeof(MzCounters), hipMemcpyDeviceToDevice, h->stream));
         MZ_HIP(hipStreamSynchronize(h->stream));
         return 0;
+    }
+    if (avg) {
+        hipLaunchKernelGGL(k_slab_surf_finalize, dim3(1), dim3(64), 0, h->stream, surf_part, sh.n_slabs, h->d_surf);
+        MZ_HIP(hipGetLastError());
     }
     mz_swap_buffers(s);
     *ran = 1;

@@ -105,11 +105,11 @@ static int couette_run(mz_store *h, int32_t n_steps, uint64_t step0, double dt, 
         r = mz_surf_prealloc(h, 1);
         if (r) return r;
     }
-    if (mode == 2 && equal_weight && avg_inv == 0.0) {
+    if (mode == 2 && equal_weight) {
         // slab-resident kernel; falls through to the graph when the state or
         // shape does not allow it or when the launch aborted (state restored)
         int ran = 0;
-        r = mz_couette_slab(h, n_steps, step0, dt, dV, props_every, &ran);
+        r = mz_couette_slab(h, n_steps, step0, dt, dV, props_every, &ran, avg_inv, mass);
         if (r) return r;
         if (ran) {
             h->last_path = 2;
@@ -117,7 +117,7 @@ static int couette_run(mz_store *h, int32_t n_steps, uint64_t step0, double dt, 
         }
         mode = 1;
     }
-    if (mode == 2) mode = 1;   // the averaging phase runs on the step graph
+    if (mode == 2) mode = 1;
     const bool graph_ok = mode == 1 && equal_weight && (props_every == 0 || props_every == 1);
     h->last_path = graph_ok ? 1 : 0;
     if (!graph_ok) {

@@ -207,6 +207,95 @@ def test_equilibrium_wall_pressure_device(hip):
     st.close()
 
 
+def _oracle_avg_run(mzo, case, steps, n_avg, step0=1):
+    """The averaging phase on the oracle: returns the averaged wall
+    properties, the averaged props and the per-quantity magnitude scales."""
+    pv, pia, g = case["pv"], case["pia"], case["grid"]
+    inst, avg = mzo.Surf.grid1d(1), mzo.Surf.grid1d(1)
+    pp, pavg = mzo.Props(case["n_cells"], 1), mzo.Props(case["n_cells"], 1)
+    n = pia.n_total()
+    scale = np.zeros(11)
+    f = helpers.MASS_AR / case["dt"]
+    for step in range(step0, step0 + steps):
+        r = mzo.RNG(seed=case["seed"], tag=(step << 4) | 1)
+        for c in range(1, case["n_cells"] + 1):
+            mzo.lib().mzo_ntc_single(r.ref, C.byref(case["cfs"][c - 1]), C.byref(case["it"]), pv.p, pia.p, c, 1,
+                                     case["dt"], g.dx, 1e-16, 1)
+        _, vb, wb, _ = pv.logical_arrays(n)
+        mzo.convect_particles_surf(mzo.RNG(seed=case["seed"], tag=(step << 4) | 2), g, case["walls"], pv, pia, 1,
+                                   case["dt"], inst, helpers.MASS_AR)
+        _, va, _, _ = pv.logical_arrays(n)
+        scale += _scales(vb, va, wb, f) / n_avg
+        avg.avg_from(inst, n_avg)
+        mzo.lib().mzo_sort_particles(C.byref(g), pv.p, pia.p, 1)
+        mzo.compute_props_sorted([pv], pia, [helpers.MASS_AR], pp)
+        pp.lpa[0] = pv.length
+        mzo.lib().mzo_props_avg(pavg.ref, pp.ref, float(n_avg))
+    return avg, pavg, scale
+
+
+@pytest.mark.parametrize("mode,shape", [("slab", (0, 0)), ("slab", (1, 1)), ("slab", (3, 2))])
+def test_slab_avg_steps_vs_oracle(hip, mzo, mode, shape):
+    """Thirty steps of the averaging phase on the slab kernel (CFL 0.17)
+    against the oracle, held to the same bars as the step-graph path."""
+    steps = 30
+    case = helpers.couette_case(mzo, n_cells=20, ppc=100, seed=3, dt=2e-10, sgwm_mult=50.0)
+    st = helpers.store_from_case(hip, case)
+    st.set_slab_shape(*shape)
+    st.surf_clear(1)
+    st.props_avg_clear()
+    path = hip.couette_avg_steps(st, steps, 1, case["dt"], steps, species_data=case["species"], mode=mode)
+    assert path == "slab", st.slab_info()
+    avg, pavg, scale = _oracle_avg_run(mzo, case, steps, steps)
+    got = st.download_species(1)
+    assert got["err"] == 0
+    helpers.assert_state_matches(mzo, got, case["pv"], case["pia"], rtol=1e-11)
+    gp = hip.PhysProps(case["n_cells"], 1).fetch_avg(st)
+    assert np.array_equal(gp.np, pavg.np) and gp.lpa[0] == pavg.lpa[0]
+    for name in ("n", "v", "T"):
+        assert np.allclose(getattr(gp, name), getattr(pavg, name), rtol=1e-11, atol=0), name
+    assert avg.np.sum() * steps > 50   # contacts over the run (np is averaged per step)
+    _compare(hip.SurfProps.grid1d(1).fetch(st, 1)._to_device(), avg.q, scale)
+    st.close()
+
+
+def test_slab_avg_equals_graph(hip, mzo):
+    """500 cells x 200 particles, 1,000 steps of the averaging phase: slab
+    kernel and step graph leave the same particles and the same averaged
+    props bit for bit, and wall averages equal to rounding."""
+    steps = 1000
+    case = helpers.couette_case(mzo, n_cells=500, ppc=200, seed=21, dt=2e-10)
+    a = helpers.store_from_case(hip, case)
+    b = helpers.store_from_case(hip, case)
+    for st in (a, b):
+        st.surf_clear(1)
+        st.props_avg_clear()
+    assert hip.couette_avg_steps(a, steps, 7, case["dt"], steps, species_data=case["species"], mode="graph") == "graph"
+    assert hip.couette_avg_steps(b, steps, 7, case["dt"], steps, species_data=case["species"], mode="slab") == "slab"
+    da, db = a.download_species(1), b.download_species(1)
+    n = da["n_total"]
+    assert da["err"] == 0 and db["err"] == 0
+    assert np.array_equal(da["pia"], db["pia"]) and np.array_equal(da["index"][:n], db["index"][:n])
+    for k in ("x", "v", "w"):
+        assert np.array_equal(da[k][:n], db[k][:n]), k
+    pa = hip.PhysProps(500, 1).fetch_avg(a)
+    pb = hip.PhysProps(500, 1).fetch_avg(b)
+    for k in ("lpa", "np", "n", "v", "T"):
+        assert np.array_equal(getattr(pa, k), getattr(pb, k)), k
+    sa = hip.SurfProps.grid1d(1).fetch(a, 1)._to_device()[0]
+    sb = hip.SurfProps.grid1d(1).fetch(b, 1)._to_device()[0]
+    assert np.array_equal(sa[:, 0], sb[:, 0]) and sa[:, 0].min() * steps > 1000
+    for e in range(2):
+        mom = np.abs(sa[e, 3:10]).sum()
+        bound = np.array([0.0, sa[e, 1], sa[e, 1]] + [mom] * 7 + [np.abs(sa[e, 10]) + sa[e, 6] * 1000.0])
+        assert np.all(np.abs(sa[e] - sb[e]) <= 1e-10 * bound), (e, sa[e], sb[e])
+    ia = hip.SurfProps.grid1d(1).fetch(a, 0)._to_device()[0]
+    ib = hip.SurfProps.grid1d(1).fetch(b, 0)._to_device()[0]
+    assert np.array_equal(ia[:, 0], ib[:, 0])
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("mode", ["eager", "graph"])
 def test_couette_avg_steps_vs_oracle(hip, mzo, mode):
     """Twelve steps of the averaging phase of
