@@ -40,16 +40,16 @@ ALG_BYTES = {"convect": 28.0, "sort": 16.0, "props": 36.0, "ntc": 1.0, "step": 8
 SLAB_HBM_BYTES_PER_PARTICLE_STEP = 16.9
 
 
-def sample_case(seed=12345):
+def sample_case(seed=12345, n_cells=N_CELLS, ppc=PPC, length=L, ndens=NDENS):
     """Particles in the reference layout: cell c holds logical positions
     [(c-1) ppc + 1, c ppc], ids 1..N, positions uniform in the cell,
     Maxwellian velocities (src/distributions_and_sampling.jl:432-509)."""
     rng = np.random.default_rng(seed)
-    n = N_CELLS * PPC
-    dx = L / N_CELLS
+    n = n_cells * ppc
+    dx = length / n_cells
     m = 66.3e-27
     vs = np.sqrt(2 * K_B * T_WALL / m)
-    cells = np.repeat(np.arange(N_CELLS), PPC)
+    cells = np.repeat(np.arange(n_cells), ppc)
     x = np.empty((n, 3))
     x[:, 0] = (cells + rng.uniform(size=n)) * dx
     x[:, 1:] = rng.uniform(size=(n, 2))
@@ -58,17 +58,17 @@ def sample_case(seed=12345):
     t1 = 2 * np.pi * rng.uniform(size=n)
     t2 = 2 * np.pi * rng.uniform(size=n)
     v = vs * np.stack([vn * np.cos(t1), vr * np.cos(t2), vr * np.sin(t2)], axis=1)
-    Fnum = NDENS * dx / PPC
+    Fnum = ndens * dx / ppc
     w = np.full(n, Fnum)
-    pia = np.zeros((N_CELLS, 7), dtype=np.int64)
-    c1 = np.arange(N_CELLS)
-    pia[:, 0] = PPC
-    pia[:, 1] = c1 * PPC + 1
-    pia[:, 2] = (c1 + 1) * PPC
-    pia[:, 3] = PPC
+    pia = np.zeros((n_cells, 7), dtype=np.int64)
+    c1 = np.arange(n_cells)
+    pia[:, 0] = ppc
+    pia[:, 1] = c1 * ppc + 1
+    pia[:, 2] = (c1 + 1) * ppc
+    pia[:, 3] = ppc
     pia[:, 4] = 0
     pia[:, 5] = -1
-    return dict(x=x, v=v, w=w, ids=np.arange(1, n + 1), pia=pia, n=n, Fnum=Fnum, mass=m)
+    return dict(x=x, v=v, w=w, ids=np.arange(1, n + 1), pia=pia, n=n, Fnum=Fnum, mass=m, n_cells=n_cells, L=length)
 
 
 def load_pkg():
@@ -81,10 +81,10 @@ def setup_store(pkg, case):
     data = os.path.join(ROOT, "merzbild.jl_amd", "data")
     sp = pkg.load_species_data(os.path.join(data, "particles.toml"), ["Ar"])
     it = pkg.load_interaction_data(os.path.join(data, "vhs.toml"), sp)
-    st = pkg.DeviceStore(N_CELLS, 1, case["n"] + 4096)
+    st = pkg.DeviceStore(case["n_cells"], 1, case["n"] + 4096)
     st.upload_species(1, case["x"], case["v"], case["w"], case["ids"], case["pia"], case["n"], True,
                       np.zeros(0, dtype=np.int64), case["n"])
-    grid = pkg.Grid1DUniform(L, N_CELLS)
+    grid = pkg.Grid1DUniform(case["L"], case["n_cells"])
     st.set_grid(grid)
     st.set_walls(pkg.MaxwellWalls1D(sp, T_WALL, T_WALL, -V_WALL, V_WALL, 1.0, 1.0))
     st.set_interactions(it)
@@ -250,6 +250,82 @@ def run_batched_0d(a, pkg):
     st.close()
 
 
+# The averaging phase of the reference's large Couette run with wall
+# properties (simulations/1D/couette_with_surface_quantities.jl:130-131):
+# argon, L = 5e-4 m in 2,000 cells, 250 particles per cell (500,000),
+# number density 5e22 m^-3, walls at 300 K moving at -/+500 m/s, dt =
+# 2.59e-9 s.  A step: ntc!, convection with wall properties, their time
+# average, sort, compute_props_sorted!, the props' time average.  Particles
+# cross up to ~15 cells per step, so the slab kernel (moves within one slab)
+# does not apply and the step graph runs it.
+CS_CELLS, CS_PPC, CS_NDENS, CS_DT = 2000, 250, 5e22, 2.59e-9
+
+
+def run_couette_surf(a, pkg):
+    """The averaging phase at the reference run's size, or (--surf-headline)
+    at the headline Couette configuration, where the slab kernel runs it."""
+    if a.surf_headline:
+        n_cells, ppc, ndens, dt = N_CELLS, PPC, NDENS, DT
+    else:
+        n_cells, ppc, ndens, dt = CS_CELLS, CS_PPC, CS_NDENS, CS_DT
+    case = sample_case(seed=1234, n_cells=n_cells, ppc=ppc, length=L, ndens=ndens)
+    st, sp, it = setup_store(pkg, case)
+    n = case["n"]
+    steps = a.steps if a.steps != 20000 else 2000
+    warm = a.warmup if a.warmup != 200 else 50
+    mode = a.mode if a.mode in ("slab", "graph", "eager") else "graph"
+    n_avg = warm + steps
+    s0, tune = 1, None
+    if mode == "slab" and not a.no_autotune:
+        # the slab shape, picked on plain steps (which serve as warmup)
+        best, times, s0 = pkg.couette_autotune(st, 1, dt, steps=20)
+        tune = {"best": list(best), "ms_per_step": {f"{k[0]}x{k[1]}": round(v, 5) for k, v in times.items()}}
+    st.surf_clear(1)
+    st.props_avg_clear()
+    pkg.couette_avg_steps(st, warm, s0, dt, n_avg, species_data=sp, mode=mode)
+    st.sync()
+    st.timing_reset()
+    st.timing(True)
+    t0 = time.perf_counter()
+    path = pkg.couette_avg_steps(st, steps, s0 + warm, dt, n_avg, species_data=sp, mode=mode)
+    st.sync()
+    el = time.perf_counter() - t0
+    fam = "slab" if path == "slab" else "step"
+    launch_ms, _ = st.timing_get(fam)
+    st.timing(False)
+    sc = st.scalars(1)
+    assert sc["err"] == 0 and sc["n_total"] == n, sc
+    # the convection-with-wall-properties kernel on its own (HIP events)
+    st.timing_reset()
+    st.timing(True)
+    for t in range(100):
+        pkg.convect_particles(st, 1, dt, tag=((s0 + warm + steps + t) << 4) | 2, species_data=sp,
+                              surf_props=pkg.SurfProps.grid1d(1), fetch=False)
+    conv_ms = st.timing_get("convect")[0]
+    st.timing(False)
+    surf = pkg.SurfProps.grid1d(1).fetch(st, 1)
+    pp = pkg.PhysProps(n_cells, 1).fetch_avg(st)
+    conv_gbs = ALG_BYTES["convect"] * n / (conv_ms * 1e-3) / 1e9
+    out = {"metric": "particle-timesteps/s (averaging phase with wall properties)", "value": n * steps / el,
+           "unit": "particle-timesteps/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+           "ms_per_step": el * 1e3 / steps, "higher_is_better": True, "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": f"1D Couette with wall properties, argon VHS, {n_cells} cells x {ppc} "
+                                  f"particles ({n}), ndens {ndens:g}, dt {dt:g} s, FP64; per step ntc!, "
+                                  "convection + wall properties, avg_props! (wall), counting sort, "
+                                  "compute_props_sorted!, avg_props! (cells)", "n_particles": n, "mode": path,
+                      "launch_ms_per_step": launch_ms, "slab_shape": st.slab_info() if path == "slab" else None,
+                      "autotune": tune},
+           "convect_surf": {"ms": conv_ms, "alg_bytes_per_particle": ALG_BYTES["convect"],
+                            "achieved_GBs": conv_gbs, "frac_of_8000": conv_gbs / 8000.0},
+           "wall_averages": {"normal_pressure_Pa": surf.normal_pressure[:, 0].tolist(),
+                             "shear_y_Pa": surf.force[1, :, 0].tolist(),
+                             "contacts_per_step": surf.np[:, 0].tolist(),
+                             "T_mid_K": float(pp.T[0, n_cells // 2])}}
+    print(json.dumps(out), flush=True)
+    st.close()
+
+
 # Batched NNLS merging: 32,768 cells of U{200..3000} particles (Philox seed
 # 5) from bi-Maxwellians (T_perp = 300 K, T_par / T_perp ~ U[0.5, 2] along a
 # random axis, drift uniform in the ball |u| <= 500 m/s), weights
@@ -387,9 +463,13 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="batched-0d: skip timing the plain and per-cell launch patterns")
     ap.add_argument("--no-autotune", action="store_true", help="slab mode: keep the library's default shape")
-    ap.add_argument("--workload", choices=("couette", "batched-0d", "nnls-merge"), default="couette",
+    ap.add_argument("--workload", choices=("couette", "batched-0d", "nnls-merge", "couette-surf"),
+                    default="couette",
                     help="couette: the headline 1-D Couette line; batched-0d: 65,536 uneven 0-D cells; "
-                         "nnls-merge: 32,768 cells merged by NNLS")
+                         "nnls-merge: 32,768 cells merged by NNLS; couette-surf: the averaging phase of the "
+                         "2,000-cell Couette run with wall properties")
+    ap.add_argument("--surf-headline", action="store_true",
+                    help="couette-surf: run the averaging phase on the headline 500-cell configuration")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB megabytes three times (to scale the HBM byte "
                          "counters of a profiler run) and exit")
@@ -413,6 +493,9 @@ def main():
         return
     if a.workload == "nnls-merge":
         run_nnls(a, pkg)
+        return
+    if a.workload == "couette-surf":
+        run_couette_surf(a, pkg)
         return
     case = sample_case()
     st, sp, it = setup_store(pkg, case)

@@ -213,6 +213,18 @@ inline MzSpView mz_view(const MzSpecies &s) {
 extern "C" __device__ double __ockl_wfred_add_f64(double);
 extern "C" __device__ double __ockl_wfred_max_f64(double);
 extern "C" __device__ double __ockl_wfred_min_f64(double);
+// Last-block reductions publish their partials with agent-scope relaxed
+// atomic stores, which the hardware keeps coherent across the XCDs' L2
+// caches; the writer then only has to wait for its stores to complete
+// (s_waitcnt vmcnt(0); expcnt and lgkmcnt left at their no-wait maxima)
+// before it takes a ticket, instead of an agent-scope fence, which writes back
+// a whole L2.  The last block reads the partials with agent-scope atomic loads.
+__device__ inline void mz_wait_stores() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 __device__ inline double mz_wave_sum(double v) { return __ockl_wfred_add_f64(v); }
 __device__ inline double mz_wave_max(double v) { return __ockl_wfred_max_f64(v); }
 __device__ inline double mz_wave_min(double v) { return __ockl_wfred_min_f64(v); }

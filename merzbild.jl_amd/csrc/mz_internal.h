@@ -93,6 +93,7 @@ struct mz_store {
     // surface properties of the two walls: [set][species][element][MZ_SURF_Q],
     // set 0 = the last convection's values, 1 = the time average
     double *d_surf = nullptr;
+    uint32_t *d_surf_ticket = nullptr;   // last-block counter of k_convect_surf (left at 0 by every launch)
     double surf_inv_area[2] = {1.0, 1.0};
     // time-averaged PhysProps: [np | n | v (x3) | T] x [species][cell], then lpa[species]
     double *d_pavg = nullptr;
@@ -110,9 +111,9 @@ void mz_graph_release(mz_store *h);
 int mz_launch_sort(mz_store *h, int species, int compute_cells);
 int mz_launch_convect(mz_store *h, int species, double dt, uint64_t tag, int rng_mode, bool contiguous,
                       int32_t n_pos, const uint64_t *step_ptr = nullptr);
-int mz_launch_props(mz_store *h, int variant, int ndens);
+int mz_launch_props(mz_store *h, int variant, int ndens, double avg_inv = 0.0);
 int mz_launch_convect_surf(mz_store *h, int species, double dt, uint64_t tag, int rng_mode, bool contiguous,
-                           int32_t n_pos, double mass, const uint64_t *step_ptr);
+                           int32_t n_pos, double mass, const uint64_t *step_ptr, double avg_inv);
 int mz_launch_surf_avg(mz_store *h, double inv);
 int mz_launch_props_avg(mz_store *h, double inv);
 int mz_surf_prealloc(mz_store *h, int species);

@@ -840,18 +840,24 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
     }
 }
 
-// After an averaging-phase launch: the slabs' wall increments added to the
-// time average in slab order, and the last step's values summed the same way.
-__global__ void k_slab_surf_finalize(const double *__restrict__ part, int32_t n_slabs, double *__restrict__ surf) {
-    const int i = threadIdx.x;
-    if (i >= 2 * MZ_SURF_Q) return;
+// After an averaging-phase launch: quantity blockIdx.x of the slabs' wall
+// increments added to the time average, and of their last step's values
+// summed into the instantaneous set; lanes stride over the slabs, then a
+// wave sum, so the order is fixed.
+__global__ void __launch_bounds__(64) k_slab_surf_finalize(const double *__restrict__ part, int32_t n_slabs,
+                                                           double *__restrict__ surf) {
+    const int i = blockIdx.x, lane = threadIdx.x;
     double acc = 0.0, last = 0.0;
-    for (int32_t g = 0; g < n_slabs; ++g) {
+    for (int32_t g = lane; g < n_slabs; g += 64) {
         acc += part[((size_t)g * 2) * 2 * MZ_SURF_Q + i];
         last += part[((size_t)g * 2 + 1) * 2 * MZ_SURF_Q + i];
     }
-    surf[2 * MZ_SURF_Q + i] += acc;   // set 1 (one species)
-    surf[i] = last;                   // set 0
+    acc = mz_wave_sum(acc);
+    last = mz_wave_sum(last);
+    if (lane == 0) {
+        surf[2 * MZ_SURF_Q + i] += acc;   // set 1 (one species)
+        surf[i] = last;                   // set 0
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1064,7 +1070,8 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
         return 0;
     }
     if (avg) {
-        hipLaunchKernelGGL(k_slab_surf_finalize, dim3(1), dim3(64), 0, h->stream, surf_part, sh.n_slabs, h->d_surf);
+        hipLaunchKernelGGL(k_slab_surf_finalize, dim3(2 * MZ_SURF_Q), dim3(64), 0, h->stream, surf_part, sh.n_slabs,
+                           h->d_surf);
         MZ_HIP(hipGetLastError());
     }
     mz_swap_buffers(s);

@@ -40,9 +40,7 @@ static int enqueue_step(mz_store *h, const double *dV, double dt, int do_props, 
     int r = mz_launch_ntc_equal(h, 1, 1, dt, dV, 1e-16, 1, (step << 4) | 1u, 0, 1, h->n_cells, step_ptr);
     if (r) return r;
     if (avg_inv > 0.0) {
-        r = mz_launch_convect_surf(h, 1, dt, (step << 4) | 2u, 0, true, 0, mass, step_ptr);
-        if (r) return r;
-        r = mz_launch_surf_avg(h, avg_inv);
+        r = mz_launch_convect_surf(h, 1, dt, (step << 4) | 2u, 0, true, 0, mass, step_ptr, avg_inv);
     } else {
         r = mz_launch_convect(h, 1, dt, (step << 4) | 2u, 0, true, 0, step_ptr);
     }
@@ -51,10 +49,9 @@ static int enqueue_step(mz_store *h, const double *dV, double dt, int do_props, 
     if (r) return r;
     mz_swap_buffers(h->sp[0]);
     if (do_props || avg_inv > 0.0) {
-        r = mz_launch_props(h, 1, 0);
+        r = mz_launch_props(h, 1, 0, avg_inv);
         if (r) return r;
     }
-    if (avg_inv > 0.0) return mz_launch_props_avg(h, avg_inv);
     return 0;
 }
 

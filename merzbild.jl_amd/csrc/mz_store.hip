@@ -172,6 +172,8 @@ extern "C" int mz_create(int32_t n_cells, int32_t n_species, const int64_t *capa
     const size_t surf_bytes = (size_t)2 * n_species * 2 * MZ_SURF_Q * sizeof(double);
     MZ_HIP(hipMalloc(&h->d_surf, surf_bytes));
     MZ_HIP(hipMemset(h->d_surf, 0, surf_bytes));
+    MZ_HIP(hipMalloc(&h->d_surf_ticket, sizeof(uint32_t)));
+    MZ_HIP(hipMemset(h->d_surf_ticket, 0, sizeof(uint32_t)));
     const size_t pavg_bytes = (pn * 6 + n_species) * sizeof(double);
     MZ_HIP(hipMalloc(&h->d_pavg, pavg_bytes));
     MZ_HIP(hipMemset(h->d_pavg, 0, pavg_bytes));
@@ -202,6 +204,7 @@ extern "C" int mz_destroy(mz_store *h) {
     hipFree(h->d_invV);
     hipFree(h->d_masses);
     hipFree(h->d_surf);
+    hipFree(h->d_surf_ticket);
     hipFree(h->d_pavg);
     for (int k = 0; k < 3; ++k) if (h->d_replay[k]) hipFree(h->d_replay[k]);
     for (auto &sc : h->scratch) if (sc.second.ptr) hipFree(sc.second.ptr);
@@ -572,7 +575,7 @@ __global__ void __launch_bounds__(256) k_sort_rows(MzSpView sp, int32_t n_cells,
         }
         if (lane == 0) __hip_atomic_store(&cstart[c], carry, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __threadfence();
+    mz_wait_stores();
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t t = atomicAdd(ticket, 1u);
@@ -581,7 +584,7 @@ __global__ void __launch_bounds__(256) k_sort_rows(MzSpView sp, int32_t n_cells,
     }
     __syncthreads();
     if (!is_last) return;
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     const int32_t total = mz_ntotal(sp);
     for (int c0 = 0; c0 < n_cells; c0 += 256) {
         const int cc = c0 + threadIdx.x;
@@ -900,7 +903,7 @@ extern "C" int mz_convect(mz_store *h, int32_t species, double dt, uint64_t tag,
 // surface_props.jl:77-189).  Only particles whose flight ends outside
 // (0, L) touch a wall; each such thread sums its contacts (MzSurfAcc), a
 // block with any contact reduces them (wave sums, then its waves in order)
-// into its row of partials and flags it, and k_surf_finalize adds the flagged
+// into its row of partials and flags it, and the block that finishes last adds the flagged
 // rows in block order.  The sums are thus the same on every run; they differ
 // from the reference's particle-order sum only by rounding.
 
@@ -924,12 +927,27 @@ __device__ inline void surf_wave_sums(const MzSurfAcc &sa, double *dst, int lane
     }
 }
 
+// The block that finishes last (a ticket counter, left at zero for the next
+// launch) adds the flagged blocks' partials in block order, scales them
+// (surface_props_scale!: every quantity but np times (mass / dt) / area),
+// writes them as the species' instantaneous set (clear_props! leaves the
+// other species at zero) and, with avg_inv > 0, adds them to the time
+// average (avg_props!).
+struct MzSurfOut {
+    double *surf;              // the store's sets
+    uint32_t *ticket;
+    int32_t n_species;
+    double f0, f1;             // (mass / dt) / area of the two walls
+    double avg_inv;
+};
+
 __global__ void __launch_bounds__(MZ_SURF_BLOCK) k_convect_surf(MzSpView sp, MzGrid g, MzWalls wl, int sidx,
                                                                 double dt, MzRngParams rp,
                                                                 const uint8_t *__restrict__ mask,
                                                                 double *__restrict__ part,
-                                                                int32_t *__restrict__ pflag) {
+                                                                int32_t *__restrict__ pflag, MzSurfOut so) {
     __shared__ double red[MZ_SURF_BLOCK / 64][2 * MZ_SURF_Q];
+    __shared__ int32_t s_last;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     rp = mz_rng_resolve(rp);
     const int32_t lim = mask ? mz_npos(sp) : mz_ntotal(sp);
@@ -961,29 +979,50 @@ __global__ void __launch_bounds__(MZ_SURF_BLOCK) k_convect_surf(MzSpView sp, MzG
         if (threadIdx.x < 2 * MZ_SURF_Q) {
             double t = red[0][threadIdx.x];
             for (int w = 1; w < MZ_SURF_BLOCK / 64; ++w) t += red[w][threadIdx.x];
-            part[(size_t)blockIdx.x * 2 * MZ_SURF_Q + threadIdx.x] = t;
+            // agent-scope stores and loads: the last block may run on another XCD
+            __hip_atomic_store((unsigned long long *)part + (size_t)blockIdx.x * 2 * MZ_SURF_Q + threadIdx.x,
+                               (unsigned long long)__double_as_longlong(t), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
     }
-    if (threadIdx.x == 0) pflag[blockIdx.x] = any;
-}
-
-// One wave per quantity (2 x MZ_SURF_Q blocks): the flagged blocks' partials
-// in block order, then surface_props_scale! (every quantity but np times
-// (mass / dt) / area); clear_props! leaves the other species at zero.
-__global__ void __launch_bounds__(64) k_surf_finalize(const double *__restrict__ part,
-                                                      const int32_t *__restrict__ pflag, int32_t nblk,
-                                                      int32_t sidx, int32_t n_species, double f0, double f1,
-                                                      double *__restrict__ out) {
-    const int t = blockIdx.x, lane = threadIdx.x;
-    double acc = 0.0;
-    for (int32_t b = lane; b < nblk; b += 64)
-        if (pflag[b]) acc += part[(size_t)b * 2 * MZ_SURF_Q + t];
-    acc = mz_wave_sum(acc);
-    if (lane == 0) {
-        const int e = t / MZ_SURF_Q, k = t % MZ_SURF_Q;
-        const double f = k == 0 ? 1.0 : (e == 0 ? f0 : f1);
-        for (int s = 0; s < n_species; ++s) out[((size_t)s * 2 + e) * MZ_SURF_Q + k] = s == sidx ? acc * f : 0.0;
+    if (threadIdx.x == 0) __hip_atomic_store(pflag + blockIdx.x, any, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    mz_wait_stores();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(so.ticket, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    // thread t adds blocks t, t + MZ_SURF_BLOCK, ... ; then wave sums and the
+    // waves in order, as the per-block partials were formed
+    MzSurfAcc tot;
+    tot.zero();
+    for (int32_t b = threadIdx.x; b < (int32_t)gridDim.x; b += MZ_SURF_BLOCK) {
+        if (!__hip_atomic_load(pflag + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+        const unsigned long long *pb = (const unsigned long long *)part + (size_t)b * 2 * MZ_SURF_Q;
+#define MZ_SURF_LD(i) __longlong_as_double((long long)__hip_atomic_load(pb + (i), __ATOMIC_RELAXED, \
+                                                                         __HIP_MEMORY_SCOPE_AGENT))
+#define MZ_SURF_LOAD(k) tot.q[0][k] += MZ_SURF_LD(k); tot.q[1][k] += MZ_SURF_LD(MZ_SURF_Q + k);
+        MZ_SURF_EACH(MZ_SURF_LOAD)
+#undef MZ_SURF_LOAD
+#undef MZ_SURF_LD
     }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    surf_wave_sums<0>(tot, red[wave], lane);
+    __syncthreads();
+    if (threadIdx.x < 2 * MZ_SURF_Q) {
+        const int t = threadIdx.x, e = t / MZ_SURF_Q, k = t % MZ_SURF_Q;
+        double v = red[0][t];
+        for (int w = 1; w < MZ_SURF_BLOCK / 64; ++w) v += red[w][t];
+        const double inst = v * (k == 0 ? 1.0 : (e == 0 ? so.f0 : so.f1));
+        const size_t nset = (size_t)so.n_species * 2 * MZ_SURF_Q;
+        for (int s = 0; s < so.n_species; ++s) {
+            const size_t ix = ((size_t)s * 2 + e) * MZ_SURF_Q + k;
+            so.surf[ix] = s == sidx ? inst : 0.0;
+            if (s == sidx && so.avg_inv > 0.0) so.surf[nset + ix] = fma(inst, so.avg_inv, so.surf[nset + ix]);
+        }
+    }
+    if (threadIdx.x == 0) *so.ticket = 0;
 }
 
 // avg_props!(avg, inst, n) for every species: avg = muladd(inst, 1 / n, avg)
@@ -1008,7 +1047,7 @@ int mz_surf_prealloc(mz_store *h, int species) {
 }
 
 int mz_launch_convect_surf(mz_store *h, int species, double dt, uint64_t tag, int rng_mode, bool contiguous,
-                           int32_t n_pos, double mass, const uint64_t *step_ptr) {
+                           int32_t n_pos, double mass, const uint64_t *step_ptr, double avg_inv) {
     MzSpecies &s = h->sp[species - 1];
     MZ_CHECK(h->has_grid && h->has_walls, "convect: grid or walls not set");
     MZ_CHECK(mass > 0.0 && dt > 0.0, "convect: surface properties need mass > 0 and dt > 0");
@@ -1029,11 +1068,11 @@ int mz_launch_convect_surf(mz_store *h, int species, double dt, uint64_t tag, in
     if (r) return r;
     double *part = (double *)mz_scratch(h, "surf_part", 0);
     int32_t *pflag = (int32_t *)mz_scratch(h, "surf_flag", 0);
-    hipLaunchKernelGGL(k_convect_surf, dim3(blocks), dim3(MZ_SURF_BLOCK), 0, h->stream, mz_view(s), h->grid,
-                       h->walls, species - 1, dt, rp, mask, part, pflag);
     const double fb = mass / dt;
-    hipLaunchKernelGGL(k_surf_finalize, dim3(2 * MZ_SURF_Q), dim3(64), 0, h->stream, part, pflag, blocks,
-                       species - 1, h->n_species, fb * h->surf_inv_area[0], fb * h->surf_inv_area[1], h->d_surf);
+    MzSurfOut so{h->d_surf, h->d_surf_ticket, h->n_species, fb * h->surf_inv_area[0], fb * h->surf_inv_area[1],
+                 avg_inv};
+    hipLaunchKernelGGL(k_convect_surf, dim3(blocks), dim3(MZ_SURF_BLOCK), 0, h->stream, mz_view(s), h->grid,
+                       h->walls, species - 1, dt, rp, mask, part, pflag, so);
     MZ_HIP(hipGetLastError());
     return 0;
 }
@@ -1046,7 +1085,7 @@ extern "C" int mz_convect_surf(mz_store *h, int32_t species, double dt, uint64_t
     int r = mz_read_counters(h, species, &c);
     if (r) return r;
     mz_timing_begin(h, "convect");
-    r = mz_launch_convect_surf(h, species, dt, tag, rng_mode, c.contiguous != 0, c.n_pos, mass, nullptr);
+    r = mz_launch_convect_surf(h, species, dt, tag, rng_mode, c.contiguous != 0, c.n_pos, mass, nullptr, 0.0);
     mz_timing_end(h);
     return r;
 }
@@ -1155,11 +1194,18 @@ __global__ void __launch_bounds__(256) k_props(MzViews views, int32_t n_cells, i
                                                int32_t ndens, const double *__restrict__ inv_V_cells,
                                                const double *__restrict__ masses, double *__restrict__ out_np,
                                                double *__restrict__ out_n, double *__restrict__ out_v,
-                                               double *__restrict__ out_T) {
+                                               double *__restrict__ out_T, double *__restrict__ avg,
+                                               double avg_inv) {
     const int sidx = blockIdx.y;
     const MzSpView &sp = views.v[sidx];
     const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
+    // avg_props! of the results when avg is set (avg = muladd(props, avg_inv,
+    // avg), physical_props.jl:281-299; lpa from the vector's length)
+    if (avg && blockIdx.x == 0 && threadIdx.x == 0) {
+        const size_t pn = (size_t)gridDim.y * n_cells;
+        avg[6 * pn + sidx] = fma((double)sp.ctr->length, avg_inv, avg[6 * pn + sidx]);
+    }
     if (c >= n_cells) return;
     int32_t st1 = sp.pia.start1[c] - 1, n1 = sp.pia.n1[c];
     int32_t st2 = sp.pia.start2[c] - 1, n2 = variant == 0 ? sp.pia.n2[c] : 0;
@@ -1181,13 +1227,23 @@ __global__ void __launch_bounds__(256) k_props(MzViews views, int32_t n_cells, i
         out_v[3 * ix + 1] = o.v1;
         out_v[3 * ix + 2] = o.v2;
         out_T[ix] = o.T;
+        if (avg) {
+            const size_t pn = (size_t)gridDim.y * n_cells;
+            avg[ix] = fma((double)tot, avg_inv, avg[ix]);
+            avg[pn + ix] = fma(out_n[ix], avg_inv, avg[pn + ix]);
+            avg[2 * pn + 3 * ix] = fma(o.v0, avg_inv, avg[2 * pn + 3 * ix]);
+            avg[2 * pn + 3 * ix + 1] = fma(o.v1, avg_inv, avg[2 * pn + 3 * ix + 1]);
+            avg[2 * pn + 3 * ix + 2] = fma(o.v2, avg_inv, avg[2 * pn + 3 * ix + 2]);
+            avg[5 * pn + ix] = fma(o.T, avg_inv, avg[5 * pn + ix]);
+        }
     }
 }
 
-int mz_launch_props(mz_store *h, int variant, int ndens) {
+int mz_launch_props(mz_store *h, int variant, int ndens, double avg_inv) {
     dim3 grid((h->n_cells + 3) / 4, h->n_species);
     hipLaunchKernelGGL(k_props, grid, dim3(256), 0, h->stream, mz_views(h), h->n_cells, variant, ndens,
-                       h->d_invV, h->d_masses, h->d_np, h->d_n, h->d_v, h->d_T);
+                       h->d_invV, h->d_masses, h->d_np, h->d_n, h->d_v, h->d_T,
+                       avg_inv > 0.0 ? h->d_pavg : nullptr, avg_inv);
     MZ_HIP(hipGetLastError());
     return 0;
 }

@@ -313,3 +313,29 @@ def test_equilibrium_wall_pressure(mzo):
         mzo.convect_particles_surf(mzo.RNG(seed=9, tag=(t << 4) | 2), grid, wl, pv, pia, 1, dt, inst, M_AR)
         avg.avg_from(inst, steps)
     check_equilibrium(avg.q[0], equilibrium_expectations(n, L, T, w), 0.04)
+
+
+def test_props_avg_kat(mzo):
+    """avg_props! of PhysProps, test/test_computes.jl:82-116: the average of
+    two PhysProps with n = 2, then clear; the package's host helper gives the
+    same and refuses mixed ndens conventions."""
+    nc, ns = 40, 3
+    p3, p4, avg = mzo.Props(nc, ns), mzo.Props(nc, ns), mzo.Props(nc, ns)
+    for p, T, n, npart, lpa, v in ((p3, 100.0, 2e11, 100, 1e7, 1e3), (p4, 200.0, 4e11, 200, 2e7, 2e3)):
+        p.T[:], p.n[:], p.np[:], p.lpa[:], p.v[:] = T, n, npart, lpa, v
+    mzo.lib().mzo_props_avg(avg.ref, p3.ref, 2.0)
+    mzo.lib().mzo_props_avg(avg.ref, p4.ref, 2.0)
+    assert np.max(np.abs(avg.T - 150.0)) < 2 * EPS and np.max(np.abs(avg.n - 3e11)) < 2 * EPS
+    assert np.max(np.abs(avg.np - 150)) < 2 * EPS and np.max(np.abs(avg.lpa - 1.5e7)) < 2 * EPS
+    assert np.max(np.abs(avg.v - 1.5e3)) < 2 * EPS
+
+    pkg = _pkg()
+    h3, h4, havg = pkg.PhysProps(nc, ns), pkg.PhysProps(nc, ns), pkg.PhysProps(nc, ns)
+    for p, T, n, npart, lpa, v in ((h3, 100.0, 2e11, 100, 1e7, 1e3), (h4, 200.0, 4e11, 200, 2e7, 2e3)):
+        p.T[:], p.n[:], p.np[:], p.lpa[:], p.v[:] = T, n, npart, lpa, v
+    pkg.avg_props(havg, h3, 2)
+    pkg.avg_props(havg, h4, 2)
+    for k in ("T", "n", "np", "lpa", "v"):
+        assert np.array_equal(getattr(havg, k), getattr(avg, k)), k
+    with pytest.raises(pkg.MzError):
+        pkg.avg_props(havg, pkg.PhysProps(nc, ns, ndens_not_Np=True), 2)
