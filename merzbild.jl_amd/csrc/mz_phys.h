@@ -197,7 +197,12 @@ __device__ inline void mz_convect_impl(const MzRngParams &rp, const MzGrid &g, c
     x0 = x_new;
 }
 
-// Term k of one wall contact, as MzSurfAcc::hit forms it.
+// Term k of one wall contact on wall bc: the incident particle (sgn = 1,
+// update_surface_incident!) adds, the reflected one (sgn = -1,
+// update_surface_reflected!) subtracts, with the reference's fused forms:
+// the normal projection muladd(px, n1, muladd(py, n2, pz n3)), the shear
+// components muladd(-p_dot_n, n_i, p_i) and the energy 0.5 muladd(px, v1,
+// muladd(py, v2, pz v3)).
 __device__ inline double mz_surf_term(int k, int bc, double w, double v0, double v1, double v2, double sgn) {
     const double n0 = bc == 0 ? 1.0 : -1.0, n1 = 0.0, n2 = 0.0;
     const double px = w * v0, py = w * v1, pz = w * v2;
@@ -230,8 +235,9 @@ __device__ inline void mz_convect_one(const MzRngParams &rp, const MzGrid &g, co
     mz_convect_impl(rp, g, wl, sidx, dt, id, x0, v0, v1, v2, err, pre_bc, p0, p1, p2, ns);
 }
 
-// The same flight with every wall contact passed to sa.hit (an MzSurfAcc, or
-// a recorder that keeps the contacts for a later sum).
+// The same flight with every wall contact passed to sa.hit(bc, v0, v1, v2,
+// sgn) (an MzSurfAcc, or a recorder that keeps the contacts for a later sum):
+// before the reflection with sgn = 1, after it with sgn = -1.
 template <class Acc>
 __device__ inline void mz_convect_one_surf(Acc &sa, const MzRngParams &rp, const MzGrid &g, const MzWalls &wl,
                                            int sidx, double dt, int32_t id, double &x0, double &v0, double &v1,

@@ -1045,6 +1045,11 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
     A.surf_part = surf_part;
     A.pavg = h->d_pavg;
     void *args[] = {&A};
+    // Every slab waits on its neighbours, so all workgroups must be resident
+    // at once.  Kernels still running on other streams of this process hold
+    // CUs and would start some slabs late, while their neighbours spin until
+    // the launch gives up and replays on the step graph: drain the device first.
+    MZ_HIP(hipDeviceSynchronize());
     mz_timing_begin(h, "slab");
     MZ_HIP(hipLaunchCooperativeKernel((const void *)sh.fn, dim3(sh.grid), dim3(SLAB_THREADS), args, 0, h->stream));
     mz_timing_end_n(h, n_steps);

@@ -172,8 +172,8 @@ extern "C" int mz_create(int32_t n_cells, int32_t n_species, const int64_t *capa
     const size_t surf_bytes = (size_t)2 * n_species * 2 * MZ_SURF_Q * sizeof(double);
     MZ_HIP(hipMalloc(&h->d_surf, surf_bytes));
     MZ_HIP(hipMemset(h->d_surf, 0, surf_bytes));
-    MZ_HIP(hipMalloc(&h->d_surf_ticket, sizeof(uint32_t)));
-    MZ_HIP(hipMemset(h->d_surf_ticket, 0, sizeof(uint32_t)));
+    MZ_HIP(hipMalloc(&h->d_surf_ticket, 2 * sizeof(uint32_t)));
+    MZ_HIP(hipMemset(h->d_surf_ticket, 0, 2 * sizeof(uint32_t)));
     const size_t pavg_bytes = (pn * 6 + n_species) * sizeof(double);
     MZ_HIP(hipMalloc(&h->d_pavg, pavg_bytes));
     MZ_HIP(hipMemset(h->d_pavg, 0, pavg_bytes));
@@ -903,9 +903,12 @@ extern "C" int mz_convect(mz_store *h, int32_t species, double dt, uint64_t tag,
 // surface_props.jl:77-189).  Only particles whose flight ends outside
 // (0, L) touch a wall; each such thread sums its contacts (MzSurfAcc), a
 // block with any contact reduces them (wave sums, then its waves in order)
-// into its row of partials and flags it, and the block that finishes last adds the flagged
-// rows in block order.  The sums are thus the same on every run; they differ
-// from the reference's particle-order sum only by rounding.
+// into its row of partials and lists itself, and k_surf_finalize adds the
+// listed rows in block order.  (A last-block pass inside the convection
+// kernel costs one same-address atomic per block; at 2,000 blocks that takes
+// longer than a second launch.)  The sums are
+// thus the same on every run; they differ from the reference's particle-order
+// sum only by rounding.
 
 #define MZ_SURF_BLOCK 256
 
@@ -927,15 +930,10 @@ __device__ inline void surf_wave_sums(const MzSurfAcc &sa, double *dst, int lane
     }
 }
 
-// The block that finishes last (a ticket counter, left at zero for the next
-// launch) adds the flagged blocks' partials in block order, scales them
-// (surface_props_scale!: every quantity but np times (mass / dt) / area),
-// writes them as the species' instantaneous set (clear_props! leaves the
-// other species at zero) and, with avg_inv > 0, adds them to the time
-// average (avg_props!).
+// Where k_surf_finalize puts the wall properties.
 struct MzSurfOut {
     double *surf;              // the store's sets
-    uint32_t *ticket;
+    uint32_t *ticket;          // [1]: blocks with contacts listed by the last convection
     int32_t n_species;
     double f0, f1;             // (mass / dt) / area of the two walls
     double avg_inv;
@@ -945,10 +943,12 @@ __global__ void __launch_bounds__(MZ_SURF_BLOCK) k_convect_surf(MzSpView sp, MzG
                                                                 double dt, MzRngParams rp,
                                                                 const uint8_t *__restrict__ mask,
                                                                 double *__restrict__ part,
-                                                                int32_t *__restrict__ pflag, MzSurfOut so) {
-    __shared__ double red[MZ_SURF_BLOCK / 64][2 * MZ_SURF_Q];
-    __shared__ int32_t s_last;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                                int32_t *__restrict__ pflag,
+                                                                int32_t *__restrict__ plist, MzSurfOut so) {
+    constexpr int NW = MZ_SURF_BLOCK / 64, Q2 = 2 * MZ_SURF_Q;
+    __shared__ double red[NW][Q2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + tid;
     rp = mz_rng_resolve(rp);
     const int32_t lim = mask ? mz_npos(sp) : mz_ntotal(sp);
     const bool active = p < lim && (!mask || mask[p]);
@@ -973,56 +973,71 @@ __global__ void __launch_bounds__(MZ_SURF_BLOCK) k_convect_surf(MzSpView sp, MzG
     }
     const int any = __syncthreads_or(hit);
     if (any) {
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         surf_wave_sums<0>(sa, red[wave], lane);
         __syncthreads();
-        if (threadIdx.x < 2 * MZ_SURF_Q) {
-            double t = red[0][threadIdx.x];
-            for (int w = 1; w < MZ_SURF_BLOCK / 64; ++w) t += red[w][threadIdx.x];
-            // agent-scope stores and loads: the last block may run on another XCD
-            __hip_atomic_store((unsigned long long *)part + (size_t)blockIdx.x * 2 * MZ_SURF_Q + threadIdx.x,
-                               (unsigned long long)__double_as_longlong(t), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+        if (tid < Q2) {
+            double t = red[0][tid];
+            for (int w = 1; w < NW; ++w) t += red[w][tid];
+            part[(size_t)blockIdx.x * Q2 + tid] = t;
         }
+        if (tid == 0) plist[atomicAdd(so.ticket + 1, 1u)] = (int32_t)blockIdx.x;
     }
-    if (threadIdx.x == 0) __hip_atomic_store(pflag + blockIdx.x, any, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    mz_wait_stores();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(so.ticket, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!s_last) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    // thread t adds blocks t, t + MZ_SURF_BLOCK, ... ; then wave sums and the
-    // waves in order, as the per-block partials were formed
-    MzSurfAcc tot;
-    tot.zero();
-    for (int32_t b = threadIdx.x; b < (int32_t)gridDim.x; b += MZ_SURF_BLOCK) {
-        if (!__hip_atomic_load(pflag + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
-        const unsigned long long *pb = (const unsigned long long *)part + (size_t)b * 2 * MZ_SURF_Q;
-#define MZ_SURF_LD(i) __longlong_as_double((long long)__hip_atomic_load(pb + (i), __ATOMIC_RELAXED, \
-                                                                         __HIP_MEMORY_SCOPE_AGENT))
-#define MZ_SURF_LOAD(k) tot.q[0][k] += MZ_SURF_LD(k); tot.q[1][k] += MZ_SURF_LD(MZ_SURF_Q + k);
-        MZ_SURF_EACH(MZ_SURF_LOAD)
-#undef MZ_SURF_LOAD
-#undef MZ_SURF_LD
+    if (tid == 0) pflag[blockIdx.x] = any;
+}
+
+// One block: the rows of the listed blocks in block order (the list is
+// sorted here; from the per-block flags if more than MZ_SURF_BLOCK blocks
+// had contacts), scaled (surface_props_scale!: every quantity but np times
+// (mass / dt) / area), written as the species' instantaneous set
+// (clear_props! leaves the other species at zero) and, with avg_inv > 0,
+// added to the time average (avg_props!).  The list count is left at zero.
+__global__ void __launch_bounds__(MZ_SURF_BLOCK) k_surf_finalize(const double *__restrict__ part,
+                                                                 const int32_t *__restrict__ pflag,
+                                                                 const int32_t *__restrict__ plist, int32_t nblk,
+                                                                 int32_t sidx, MzSurfOut so) {
+    constexpr int Q2 = 2 * MZ_SURF_Q, NJ = MZ_SURF_BLOCK / Q2;
+    __shared__ double red[NJ][Q2];
+    __shared__ int32_t order[MZ_SURF_BLOCK];
+    const int tid = threadIdx.x;
+    const int32_t nl = (int32_t)so.ticket[1];
+    const bool listed = nl <= MZ_SURF_BLOCK;
+    if (listed) {
+        // ids are distinct: rank by count
+        const int32_t id = tid < nl ? plist[tid] : 0;
+        order[tid] = id;
+        __syncthreads();
+        int rank = 0;
+        for (int j = 0; j < nl; ++j) rank += order[j] < id;
+        __syncthreads();
+        if (tid < nl) order[rank] = id;
+        __syncthreads();
     }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // thread (j, q): quantity q over rows j, j + NJ, ... ; then the NJ parts in order
+    if (tid < NJ * Q2) {
+        const int q = tid % Q2, j = tid / Q2;
+        double sum = 0.0;
+        if (listed) {
+            for (int i = j; i < nl; i += NJ) sum += part[(size_t)order[i] * Q2 + q];
+        } else {
+            for (int32_t b = j; b < nblk; b += NJ)
+                if (pflag[b]) sum += part[(size_t)b * Q2 + q];
+        }
+        red[j][q] = sum;
+    }
     __syncthreads();
-    surf_wave_sums<0>(tot, red[wave], lane);
-    __syncthreads();
-    if (threadIdx.x < 2 * MZ_SURF_Q) {
-        const int t = threadIdx.x, e = t / MZ_SURF_Q, k = t % MZ_SURF_Q;
-        double v = red[0][t];
-        for (int w = 1; w < MZ_SURF_BLOCK / 64; ++w) v += red[w][t];
+    if (tid < Q2) {
+        const int e = tid / MZ_SURF_Q, k = tid % MZ_SURF_Q;
+        double v = red[0][tid];
+        for (int j = 1; j < NJ; ++j) v += red[j][tid];
         const double inst = v * (k == 0 ? 1.0 : (e == 0 ? so.f0 : so.f1));
-        const size_t nset = (size_t)so.n_species * 2 * MZ_SURF_Q;
+        const size_t nset = (size_t)so.n_species * Q2;
         for (int s = 0; s < so.n_species; ++s) {
             const size_t ix = ((size_t)s * 2 + e) * MZ_SURF_Q + k;
             so.surf[ix] = s == sidx ? inst : 0.0;
             if (s == sidx && so.avg_inv > 0.0) so.surf[nset + ix] = fma(inst, so.avg_inv, so.surf[nset + ix]);
         }
     }
-    if (threadIdx.x == 0) *so.ticket = 0;
+    if (tid == 0) so.ticket[1] = 0;
 }
 
 // avg_props!(avg, inst, n) for every species: avg = muladd(inst, 1 / n, avg)
@@ -1041,7 +1056,8 @@ __global__ void k_surf_fill(double *__restrict__ a, int32_t n) {
 int mz_surf_prealloc(mz_store *h, int species) {
     const int64_t blocks = (h->sp[species - 1].cap + MZ_SURF_BLOCK - 1) / MZ_SURF_BLOCK;
     MZ_CHECK(mz_scratch(h, "surf_part", (size_t)blocks * 2 * MZ_SURF_Q * sizeof(double)) != nullptr &&
-                 mz_scratch(h, "surf_flag", (size_t)blocks * sizeof(int32_t)) != nullptr,
+                 mz_scratch(h, "surf_flag", (size_t)blocks * sizeof(int32_t)) != nullptr &&
+                 mz_scratch(h, "surf_list", (size_t)blocks * sizeof(int32_t)) != nullptr,
              "convect: scratch allocation failed");
     return 0;
 }
@@ -1071,8 +1087,11 @@ int mz_launch_convect_surf(mz_store *h, int species, double dt, uint64_t tag, in
     const double fb = mass / dt;
     MzSurfOut so{h->d_surf, h->d_surf_ticket, h->n_species, fb * h->surf_inv_area[0], fb * h->surf_inv_area[1],
                  avg_inv};
+    int32_t *plist = (int32_t *)mz_scratch(h, "surf_list", 0);
     hipLaunchKernelGGL(k_convect_surf, dim3(blocks), dim3(MZ_SURF_BLOCK), 0, h->stream, mz_view(s), h->grid,
-                       h->walls, species - 1, dt, rp, mask, part, pflag, so);
+                       h->walls, species - 1, dt, rp, mask, part, pflag, plist, so);
+    hipLaunchKernelGGL(k_surf_finalize, dim3(1), dim3(MZ_SURF_BLOCK), 0, h->stream, part, pflag, plist, blocks,
+                       species - 1, so);
     MZ_HIP(hipGetLastError());
     return 0;
 }

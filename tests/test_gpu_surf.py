@@ -271,7 +271,8 @@ def test_slab_avg_equals_graph(hip, mzo):
         st.surf_clear(1)
         st.props_avg_clear()
     assert hip.couette_avg_steps(a, steps, 7, case["dt"], steps, species_data=case["species"], mode="graph") == "graph"
-    assert hip.couette_avg_steps(b, steps, 7, case["dt"], steps, species_data=case["species"], mode="slab") == "slab"
+    path = hip.couette_avg_steps(b, steps, 7, case["dt"], steps, species_data=case["species"], mode="slab")
+    assert path == "slab", b.slab_info()
     da, db = a.download_species(1), b.download_species(1)
     n = da["n_total"]
     assert da["err"] == 0 and db["err"] == 0
