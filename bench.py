@@ -450,6 +450,46 @@ def run_nnls(a, pkg):
     print(json.dumps(out), flush=True)
 
 
+def run_roulette(a, pkg):
+    """Roulette merging (merge_roulette!) of the NNLS workload's 32,768 cells
+    (U{200..3000} particles) to 64 particles each, plain and conservative."""
+    cs = sample_nnls_cells()
+    n, counts, starts = cs["n"], cs["counts"], cs["starts"]
+    pia = np.zeros((NN_CELLS, 7), dtype=np.int64)
+    pia[:, 0] = counts
+    pia[:, 1] = starts + 1
+    pia[:, 2] = starts + counts
+    pia[:, 3] = counts
+    pia[:, 5] = -1
+    res = {}
+    for conservative in (False, True):
+        st = pkg.DeviceStore(NN_CELLS, 1, n + 1024)
+        st.upload_species(1, cs["x"], cs["v"], cs["w"], np.arange(1, n + 1, dtype=np.int64), pia, n, True,
+                          np.zeros(0, dtype=np.int64), n)
+        st.set_seed(5)
+        st.sync()
+        st.timing_reset()
+        st.timing(True)
+        t0 = time.perf_counter()
+        merged = pkg.merge_roulette(st, 1, 64, 64, conservative=conservative, tag=3)
+        st.sync()
+        el = time.perf_counter() - t0
+        ms, _ = st.timing_get("merge_roulette")
+        st.timing(False)
+        sc = st.scalars(1)
+        assert sc["err"] == 0 and sc["n_total"] == 64 * NN_CELLS, sc
+        res["conservative" if conservative else "plain"] = {
+            "merged": merged, "kernel_ms": ms, "wall_s": el, "merges_per_s": merged / (ms * 1e-3),
+            "deleted_per_s": (n - 64 * NN_CELLS) / (ms * 1e-3)}
+        st.close()
+    out = {"metric": "roulette merges/s", "value": res["plain"]["merges_per_s"], "unit": "merges/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic", "vs_baseline": None,
+           "config": {"workload": "32768 cells x U{200..3000} bi-Maxwellian particles, lognormal(0.5) weights; "
+                                  "merge_roulette! of every cell to 64 particles", "n_particles": n},
+           "plain": res["plain"], "conservative": res["conservative"]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -463,7 +503,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="batched-0d: skip timing the plain and per-cell launch patterns")
     ap.add_argument("--no-autotune", action="store_true", help="slab mode: keep the library's default shape")
-    ap.add_argument("--workload", choices=("couette", "batched-0d", "nnls-merge", "couette-surf"),
+    ap.add_argument("--workload", choices=("couette", "batched-0d", "nnls-merge", "couette-surf", "roulette-merge"),
                     default="couette",
                     help="couette: the headline 1-D Couette line; batched-0d: 65,536 uneven 0-D cells; "
                          "nnls-merge: 32,768 cells merged by NNLS; couette-surf: the averaging phase of the "
@@ -496,6 +536,9 @@ def main():
         return
     if a.workload == "couette-surf":
         run_couette_surf(a, pkg)
+        return
+    if a.workload == "roulette-merge":
+        run_roulette(a, pkg)
         return
     case = sample_case()
     st, sp, it = setup_store(pkg, case)

@@ -179,6 +179,17 @@ int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, const int32_t
                   int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
                   int32_t squash, int32_t *status);
 
+/* merge_roulette!(rng, pv, pia, cell, species, target_np; conservative)
+ * (src/merging/merging_roulette.jl:25-184) on every cell holding more than
+ * `threshold` particles (at most 8192 per cell; larger cells flag a capacity
+ * error and are left alone), each drawing from its own stream (cell - 1)
+ * under `tag`.  Deleted ids are pushed to the free-id buffer in cell order,
+ * in draw order within a cell, as the reference's cell loop pushes them; the
+ * PIA is left not contiguous.  *n_merged (may be NULL) receives the number
+ * of cells merged. */
+int mz_merge_roulette(mz_store *h, int32_t species, int32_t threshold, int32_t target_np, int32_t conservative,
+                      uint64_t tag, int32_t rng_mode, int64_t *n_merged);
+
 /* merge_nnls_based! with a bounded candidate set: the columns are the first
  * max_pcols particles of the cell (logical order) plus the reference's
  * fictitious particles, while the moments to match are still those of all

@@ -63,6 +63,7 @@ _SIGS = {
     "mz_merge_nnls": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, f64, i32, i32, P(f64), i32, f64, i32, f64,
                                 i32, P(i32)]),
     "mz_get_nnls_info": (C.c_int, [C.c_void_p, P(i32)]),
+    "mz_merge_roulette": (C.c_int, [C.c_void_p, i32, i32, i32, i32, u64, i32, P(i64)]),
     "mz_merge_nnls_cand": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, f64, i32, i32, P(f64), i32, f64, i32,
                                      f64, i32, i32, i32, P(i32)]),
     "mz_couette_steps": (C.c_int, [C.c_void_p, i32, u64, f64, P(f64), i32, i32, P(f64), i32]),
@@ -659,6 +660,16 @@ def merge_octree_N2_based(store, species, threshold, target_np, split=OCTREE_MID
     _check(load_library().mz_merge_octree(store.h, species, threshold, target_np, split, init_bin_bounds,
                                           bin_bounds_compute, max_Nbins, max_depth, 1 if grid_clamp else 0,
                                           1 if squash else 0, tag, rng_mode, C.byref(n)))
+    return n.value
+
+
+def merge_roulette(store, species, threshold, target_np, conservative=False, tag=0, rng_mode=RNG_PHILOX):
+    """merge_roulette!(rng, pv, pia, cell, species, target_np; conservative)
+    (src/merging/merging_roulette.jl:25-184) on every cell above `threshold`;
+    returns the number of cells merged."""
+    n = i64(0)
+    _check(load_library().mz_merge_roulette(store.h, species, threshold, target_np, 1 if conservative else 0, tag,
+                                            rng_mode, C.byref(n)))
     return n.value
 
 

@@ -1955,3 +1955,230 @@ extern "C" int mz_merge_nnls_cand(mz_store *h, int32_t species, int32_t threshol
     return nnls_merge_impl(h, species, threshold, mim, n_mim, vref, scaling_vref, centered_at_mean, v_multipliers,
                            n_vmult, max_err, iteration_mult, w_threshold, squash, max_pcols, method, status);
 }
+
+// ===========================================================================
+// merge_roulette!(rng, pv, pia, cell, species, target_np; conservative)
+// (merging_roulette.jl:25-184) on every cell holding more than `threshold`
+// particles, one workgroup per cell.  Thread 0 replays the reference's
+// deletion loop on the cell's positions in LDS: draw k of the cell's stream
+// picks i = floor(R n_local), the particle at i trades places with the last of
+// its group, and that last one is deleted.  The outcome is a permutation of
+// the cell's slots (survivors first in each group, the deleted particles in
+// the group tails, as the reference's index array holds them), applied to the
+// data in one gather; then the re-weighting and, when conservative, the
+// velocity correction.  Sums are the reference's sequential ones.
+#define MZ_ROUL_MAX 8192   // particles per merged cell (the LDS position lists)
+
+struct RoulArgs {
+    MzSpView sp;
+    MzSpView tmp;              // second particle buffer: the cell's data before the merge
+    int32_t threshold, target, conservative, n_cells;
+    MzRngParams rp;
+    const int64_t *off;        // deleted particles before each cell (buffer order)
+    int32_t nbuf0;
+    int32_t *n_merged;
+};
+
+__global__ void __launch_bounds__(256) k_roulette_merge(RoulArgs A) {
+    __shared__ int32_t lst[MZ_ROUL_MAX];   // position k of the cell holds the particle first at lst[k]
+    __shared__ int32_t dl[MZ_ROUL_MAX];    // deleted particles in draw order
+    __shared__ int32_t s_m1, s_m2;
+    __shared__ double s_sc[5];              // scale factor; conservative: E_scale, v0 (3)
+    __shared__ double s_vm[3];
+    const int c = blockIdx.x, tid = threadIdx.x;
+    const int32_t s1 = A.sp.pia.start1[c], n1 = A.sp.pia.n1[c];
+    const int32_t s2 = A.sp.pia.start2[c], n2 = A.sp.pia.n2[c] > 0 ? A.sp.pia.n2[c] : 0;
+    const int32_t N = n1 + n2;
+    if (N <= A.threshold) return;
+    if (N > MZ_ROUL_MAX) {
+        if (tid == 0) atomicOr(&A.sp.ctr->err, MZ_ERR_CAPACITY);
+        return;
+    }
+    const int32_t n_del = N - A.target > 0 ? N - A.target : 0;
+    auto slot = [&](int32_t li) { return oct_slot(li, s1, n1, s2); };
+    for (int k = tid; k < N; k += blockDim.x) {
+        lst[k] = k;
+        const int64_t p = slot(k);
+        A.tmp.x0[p] = A.sp.x0[p]; A.tmp.x1[p] = A.sp.x1[p]; A.tmp.x2[p] = A.sp.x2[p];
+        A.tmp.v0[p] = A.sp.v0[p]; A.tmp.v1[p] = A.sp.v1[p]; A.tmp.v2[p] = A.sp.v2[p];
+        A.tmp.w[p] = A.sp.w[p];
+        A.tmp.id[p] = A.sp.id[p];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const MzRngParams rp = mz_rng_resolve(A.rp);
+        MzStream st = mz_stream((uint64_t)c);
+        int32_t m1 = n1, m2 = n2;
+        for (int32_t n = 0; n < n_del; ++n) {
+            const int32_t i = (int32_t)floor(mz_rand(rp, st) * (double)(m1 + m2));
+            int32_t at, last;
+            if (m2 > 0 && i >= m1) {
+                at = n1 + (i - m1);
+                last = n1 + m2 - 1;
+                --m2;
+            } else {
+                at = i;
+                last = m1 - 1;
+                --m1;
+            }
+            const int32_t chosen = lst[at];
+            lst[at] = lst[last];
+            lst[last] = chosen;
+            dl[n] = chosen;
+        }
+        s_m1 = m1;
+        s_m2 = m2;
+        // the reference's sums, in its order (group 1, then group 2)
+        double w_deleted = 0.0;
+        for (int32_t n = 0; n < n_del; ++n) w_deleted += A.tmp.w[slot(dl[n])];
+        double sc;
+        if (A.conservative) {
+            double wt = 0.0, v0[3] = {0.0, 0.0, 0.0}, E0 = 0.0;
+            for (int32_t k = 0; k < N; ++k) {
+                const int64_t p = slot(k);
+                const double w = A.tmp.w[p];
+                wt += w;
+                v0[0] += w * A.tmp.v0[p]; v0[1] += w * A.tmp.v1[p]; v0[2] += w * A.tmp.v2[p];
+            }
+            for (int d = 0; d < 3; ++d) v0[d] /= wt;
+            for (int32_t k = 0; k < N; ++k) {
+                const int64_t p = slot(k);
+                const double a = A.tmp.v0[p] - v0[0], b = A.tmp.v1[p] - v0[1], e = A.tmp.v2[p] - v0[2];
+                E0 += A.tmp.w[p] * ((a * a + b * b) + e * e);
+            }
+            E0 /= wt;
+            sc = wt / (wt - w_deleted);
+            // survivors after re-weighting: new mean and energy
+            double vm[3] = {0.0, 0.0, 0.0}, En = 0.0;
+            for (int g = 0; g < 2; ++g) {
+                const int32_t k0 = g == 0 ? 0 : n1, k1 = g == 0 ? m1 : n1 + m2;
+                for (int32_t k = k0; k < k1; ++k) {
+                    const int64_t p = slot(lst[k]);
+                    const double w = A.tmp.w[p] * sc;
+                    vm[0] += w * A.tmp.v0[p]; vm[1] += w * A.tmp.v1[p]; vm[2] += w * A.tmp.v2[p];
+                }
+            }
+            for (int d = 0; d < 3; ++d) vm[d] /= wt;
+            for (int g = 0; g < 2; ++g) {
+                const int32_t k0 = g == 0 ? 0 : n1, k1 = g == 0 ? m1 : n1 + m2;
+                for (int32_t k = k0; k < k1; ++k) {
+                    const int64_t p = slot(lst[k]);
+                    const double a = A.tmp.v0[p] - vm[0], b = A.tmp.v1[p] - vm[1], e = A.tmp.v2[p] - vm[2];
+                    En += A.tmp.w[p] * sc * ((a * a + b * b) + e * e);
+                }
+            }
+            En /= wt;
+            s_sc[1] = sqrt(E0 / En);
+            s_sc[2] = v0[0]; s_sc[3] = v0[1]; s_sc[4] = v0[2];
+            s_vm[0] = vm[0]; s_vm[1] = vm[1]; s_vm[2] = vm[2];
+        } else {
+            double wn = 0.0;
+            for (int g = 0; g < 2; ++g) {
+                const int32_t k0 = g == 0 ? 0 : n1, k1 = g == 0 ? m1 : n1 + m2;
+                for (int32_t k = k0; k < k1; ++k) wn += A.tmp.w[slot(lst[k])];
+            }
+            sc = (wn + w_deleted) / wn;
+        }
+        s_sc[0] = sc;
+    }
+    __syncthreads();
+    const int32_t m1 = s_m1, m2 = s_m2;
+    const double sc = s_sc[0];
+    // the gather: position k takes particle lst[k]; deleted ones weigh 0
+    for (int k = tid; k < N; k += blockDim.x) {
+        const int64_t q = slot(k), p = slot(lst[k]);
+        const bool alive = k < n1 ? k < m1 : k < n1 + m2;
+        double v0 = A.tmp.v0[p], v1 = A.tmp.v1[p], v2 = A.tmp.v2[p];
+        if (alive && A.conservative) {
+            const double es = s_sc[1];
+            v0 = es * (v0 - s_vm[0]) + s_sc[2];
+            v1 = es * (v1 - s_vm[1]) + s_sc[3];
+            v2 = es * (v2 - s_vm[2]) + s_sc[4];
+        }
+        A.sp.x0[q] = A.tmp.x0[p]; A.sp.x1[q] = A.tmp.x1[p]; A.sp.x2[q] = A.tmp.x2[p];
+        A.sp.v0[q] = v0; A.sp.v1[q] = v1; A.sp.v2[q] = v2;
+        A.sp.w[q] = alive ? A.tmp.w[p] * sc : 0.0;
+        A.sp.id[q] = A.tmp.id[p];
+    }
+    // free ids in draw order, after all earlier cells' (the reference's cell loop)
+    for (int n = tid; n < n_del; n += blockDim.x) {
+        const int64_t q = (int64_t)A.nbuf0 + A.off[c] + n;
+        if (q < A.sp.bufcap) A.sp.buffer[q] = A.tmp.id[slot(dl[n])];
+        else atomicOr(&A.sp.ctr->err, MZ_ERR_BUFFER);
+    }
+    if (tid == 0) {
+        // delete_particle_end_group1!/2! leave (0, -1) for an emptied group
+        A.sp.pia.n1[c] = m1;
+        A.sp.pia.end1[c] = m1 > 0 ? s1 + m1 - 1 : -1;
+        if (m1 == 0) A.sp.pia.start1[c] = 0;
+        A.sp.pia.n2[c] = m2;
+        A.sp.pia.end2[c] = m2 > 0 ? s2 + m2 - 1 : -1;
+        if (m2 == 0) A.sp.pia.start2[c] = 0;
+        A.sp.pia.nloc[c] = m1 + m2;
+        A.sp.ctr->contiguous = 0;
+        atomicAdd(A.n_merged, 1);
+    }
+}
+
+__global__ void k_roulette_ndel(MzSpView sp, int32_t n_cells, int32_t threshold, int32_t target,
+                                int32_t *__restrict__ ndel) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n_cells) {
+        const int32_t N = sp.pia.n1[c] + (sp.pia.n2[c] > 0 ? sp.pia.n2[c] : 0);
+        ndel[c] = (N > threshold && N <= MZ_ROUL_MAX && N > target) ? N - target : 0;
+    }
+}
+
+__global__ void k_roulette_counters(MzSpView sp, const int64_t *__restrict__ total, int32_t nbuf0, int32_t ntot0) {
+    sp.ctr->nbuffer = nbuf0 + (int32_t)*total;
+    sp.ctr->n_total = ntot0 - (int32_t)*total;
+    sp.ctr->deleted = (int32_t)*total;
+}
+
+extern "C" int mz_merge_roulette(mz_store *h, int32_t species, int32_t threshold, int32_t target_np,
+                                 int32_t conservative, uint64_t tag, int32_t rng_mode, int64_t *n_merged) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "merge_roulette: bad species");
+    MZ_CHECK(target_np >= 0 && threshold >= 0, "merge_roulette: negative threshold or target");
+    MZ_CHECK(rng_mode == 0 || (rng_mode == 1 && h->d_replay[2]), "merge_roulette: replay table not set");
+    MzSpecies &s = h->sp[species - 1];
+    const int nc = h->n_cells;
+    int32_t *cnt = (int32_t *)mz_scratch(h, "oct_cnt", sizeof(int32_t));
+    int32_t *ndel = (int32_t *)mz_scratch(h, "oct_ndel", (size_t)nc * sizeof(int32_t));
+    int64_t *off = (int64_t *)mz_scratch(h, "oct_off", (size_t)(nc + 1) * sizeof(int64_t));
+    MZ_CHECK(cnt && ndel && off, "merge_roulette: scratch allocation failed");
+    MzCounters snap;
+    int r = mz_read_counters(h, species, &snap);
+    if (r) return r;
+    MZ_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t), h->stream));
+    RoulArgs A{};
+    A.sp = mz_view(s);
+    A.tmp = A.sp;
+    A.tmp.x0 = s.xb[0]; A.tmp.x1 = s.xb[1]; A.tmp.x2 = s.xb[2];
+    A.tmp.v0 = s.vb[0]; A.tmp.v1 = s.vb[1]; A.tmp.v2 = s.vb[2];
+    A.tmp.w = s.wb;
+    A.tmp.id = s.idb;
+    A.threshold = threshold;
+    A.target = target_np;
+    A.conservative = conservative ? 1 : 0;
+    A.n_cells = nc;
+    A.rp = mz_rng_params(h, 2, tag, rng_mode);
+    A.off = off;
+    A.nbuf0 = snap.nbuffer;
+    A.n_merged = cnt;
+    mz_timing_begin(h, "merge_roulette");
+    hipLaunchKernelGGL(k_roulette_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, A.sp, nc, threshold,
+                       target_np, ndel);
+    hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
+    hipLaunchKernelGGL(k_roulette_merge, dim3(nc), dim3(256), 0, h->stream, A);
+    hipLaunchKernelGGL(k_roulette_counters, dim3(1), dim3(1), 0, h->stream, A.sp, off + nc, snap.nbuffer,
+                       snap.n_total);
+    mz_timing_end(h);
+    MZ_HIP(hipGetLastError());
+    if (n_merged) {
+        int32_t v = 0;
+        MZ_HIP(hipMemcpyAsync(&v, cnt, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+        MZ_HIP(hipStreamSynchronize(h->stream));
+        *n_merged = v;
+    }
+    return 0;
+}

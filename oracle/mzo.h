@@ -88,6 +88,7 @@ void mzo_update_indexer_new_particle(mzo_pia *pia, int64_t cell, int64_t species
 void mzo_update_indexer_new_lower_count(mzo_pia *pia, int64_t cell, int64_t species, int64_t new_count);
 void mzo_update_buffer_index_new_particle(mzo_pv *pv, mzo_pia *pia, int64_t cell, int64_t species);
 void mzo_delete_particle_end(mzo_pv *pv, mzo_pia *pia, int64_t cell, int64_t species);
+void mzo_delete_particle(mzo_pv *pv, mzo_pia *pia, int64_t cell, int64_t species, int64_t i);
 void mzo_squash_pia(mzo_pv *pv, mzo_pia *pia, int64_t species);
 int mzo_check_pia_is_correct(mzo_pia *pia, int64_t species, int64_t *where);
 int mzo_check_unique_index(mzo_pv *pv, mzo_pia *pia, int64_t species, int64_t *code);
@@ -244,6 +245,16 @@ void mzo_octree_compute_bin_props(mzo_octree *oc, int64_t bin_id, mzo_pv *pv);
 void mzo_octree_compute(mzo_octree *oc, mzo_pv *pv, int64_t target_np);
 void mzo_merge_octree_N2_based(mzo_rng *rng, mzo_octree *oc, mzo_pv *pv, mzo_pia *pia, int64_t cell,
                                int64_t species, int64_t target_np, const mzo_grid1d *grid);
+
+/* merge_roulette!(rng, pv, pia, cell, species, target_np; conservative)
+ * (src/merging/merging_roulette.jl:25-184): delete uniformly drawn particles
+ * (one draw each, from the cell's stream) until target_np remain, then
+ * re-weight the rest to keep the cell's total weight; conservative also
+ * restores the mean velocity and the temperature.  Always clears
+ * pia.contiguous.  Parity: pinned by test/test_roulette_merging.jl's answers
+ * and the conservation laws (tests/test_oracle_roulette.py). */
+void mzo_merge_roulette(mzo_rng *rng, mzo_pv *pv, mzo_pia *pia, int64_t cell, int64_t species, int64_t target_np,
+                        int conservative);
 
 /* ---------------- NNLS merging (src/merging/merging_nnls.jl, nnls.jl) ---------------- */
 int mzo_nnls_solve(double *A, int64_t m, int64_t n, double *b, double *x, double *w, double *zz,

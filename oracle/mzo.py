@@ -143,6 +143,8 @@ def lib():
             "mzo_walls_init": (None, [P(Walls), P(f64), i64, f64, f64, f64, f64, f64, f64]),
             "mzo_convect_particles": (None, [P(Rng), P(Grid1D), P(Walls), P(PvS), P(PiaS), i64, f64]),
             "mzo_props_avg": (None, [P(PropsS), P(PropsS), f64]),
+            "mzo_delete_particle": (None, [P(PvS), P(PiaS), i64, i64, i64]),
+            "mzo_merge_roulette": (None, [P(Rng), P(PvS), P(PiaS), i64, i64, i64, C.c_int]),
             "mzo_surf_update_incident": (None, [P(SurfS), f64, P(f64), i64, i64]),
             "mzo_surf_update_reflected": (None, [P(SurfS), f64, P(f64), i64, i64]),
             "mzo_surf_scale": (None, [P(SurfS), i64, f64, f64]),

@@ -229,6 +229,24 @@ static void delete_end_group(mzo_pv *pv, mzo_pia *pia, int64_t cell, int64_t spe
     pv->buffer[pv->nbuffer - 1] = PID(pv, pos);
 }
 
+/* delete_particle!(pv, pia, cell, species, i) (src/particles.jl:448-461): the
+ * particle at position i trades places with the last one of its group, which
+ * is then deleted. */
+void mzo_delete_particle(mzo_pv *pv, mzo_pia *pia, int64_t cell, int64_t species, int64_t i) {
+    mzo_ind *ind = mzo_pia_at(pia, cell, species);
+    if (ind->n_group2 > 0 && i >= ind->start2 && i <= ind->end2) {
+        int64_t last = pv->index[ind->end2 - 1];
+        pv->index[ind->end2 - 1] = pv->index[i - 1];
+        pv->index[i - 1] = last;
+        delete_end_group(pv, pia, cell, species, 2);
+    } else {
+        int64_t last = pv->index[ind->end1 - 1];
+        pv->index[ind->end1 - 1] = pv->index[i - 1];
+        pv->index[i - 1] = last;
+        delete_end_group(pv, pia, cell, species, 1);
+    }
+}
+
 void mzo_delete_particle_end(mzo_pv *pv, mzo_pia *pia, int64_t cell, int64_t species) {
     mzo_ind *ind = mzo_pia_at(pia, cell, species);
     if (ind->n_group2 > 0) delete_end_group(pv, pia, cell, species, 2);

@@ -812,3 +812,76 @@ int mzo_merge_nnls_cand(mzo_rng *rng, mzo_nnls_merge *nm, mzo_pv *pv, mzo_pia *p
     free(A); free(vp); free(cn); free(x); free(w); free(idx);
     return ret;
 }
+
+/* ========================================================================= */
+/* merge_roulette! (src/merging/merging_roulette.jl:25-184).  No @muladd in
+ * that file: sums and products are plain, in the reference's order (group 1
+ * then group 2). */
+static void roulette_sums(const mzo_pv *pv, const mzo_ind *d, const double *vm, double *w_sum, double *v_sum,
+                          double *e_sum) {
+    for (int g = 0; g < 2; ++g) {
+        if (g == 1 && d->n_group2 <= 0) break;
+        const int64_t s = g == 0 ? d->start1 : d->start2, e = g == 0 ? d->end1 : d->end2;
+        for (int64_t i = s; i <= e; ++i) {
+            const int64_t id = pv->index[i - 1];
+            const double w = pv->w[id - 1];
+            const double *v = pv->v + 3 * (id - 1);
+            if (w_sum) *w_sum += w;
+            if (v_sum) for (int k = 0; k < 3; ++k) v_sum[k] += w * v[k];
+            if (e_sum) {
+                const double a = v[0] - vm[0], b = v[1] - vm[1], c = v[2] - vm[2];
+                *e_sum += w * ((a * a + b * b) + c * c);
+            }
+        }
+    }
+}
+
+void mzo_merge_roulette(mzo_rng *rng, mzo_pv *pv, mzo_pia *pia, int64_t cell, int64_t species, int64_t target_np,
+                        int conservative) {
+    mzo_ind *d = mzo_pia_at(pia, cell, species);
+    const int64_t n_to_delete = d->n_local - target_np;
+    double w_total0 = 0.0, v0[3] = {0.0, 0.0, 0.0}, E0 = 0.0;
+    if (conservative) {
+        roulette_sums(pv, d, NULL, &w_total0, v0, NULL);
+        for (int k = 0; k < 3; ++k) v0[k] /= w_total0;
+        roulette_sums(pv, d, v0, NULL, NULL, &E0);
+        E0 /= w_total0;
+    }
+    double w_deleted = 0.0;
+    for (int64_t n = 0; n < n_to_delete; ++n) {
+        int64_t i_delete = (int64_t)floor(mzo_rand(rng) * (double)d->n_local);
+        i_delete = mzo_map_cont_index(d, i_delete);
+        w_deleted += pv->w[pv->index[i_delete - 1] - 1];
+        mzo_delete_particle(pv, pia, cell, species, i_delete);
+    }
+    pia->contiguous[species - 1] = 0;
+    double scale_factor;
+    if (conservative) {
+        scale_factor = w_total0 / (w_total0 - w_deleted);
+    } else {
+        double w_total_new = 0.0;
+        roulette_sums(pv, d, NULL, &w_total_new, NULL, NULL);
+        scale_factor = (w_total_new + w_deleted) / w_total_new;
+    }
+    for (int g = 0; g < 2; ++g) {
+        if (g == 1 && d->n_group2 <= 0) break;
+        const int64_t s = g == 0 ? d->start1 : d->start2, e = g == 0 ? d->end1 : d->end2;
+        for (int64_t i = s; i <= e; ++i) pv->w[pv->index[i - 1] - 1] *= scale_factor;
+    }
+    if (conservative) {
+        double vm[3] = {0.0, 0.0, 0.0}, E_new = 0.0;
+        roulette_sums(pv, d, NULL, NULL, vm, NULL);
+        for (int k = 0; k < 3; ++k) vm[k] /= w_total0;
+        roulette_sums(pv, d, vm, NULL, NULL, &E_new);
+        E_new /= w_total0;
+        const double E_scale = sqrt(E0 / E_new);
+        for (int g = 0; g < 2; ++g) {
+            if (g == 1 && d->n_group2 <= 0) break;
+            const int64_t s = g == 0 ? d->start1 : d->start2, e = g == 0 ? d->end1 : d->end2;
+            for (int64_t i = s; i <= e; ++i) {
+                double *v = pv->v + 3 * (pv->index[i - 1] - 1);
+                for (int k = 0; k < 3; ++k) v[k] = E_scale * (v[k] - vm[k]) + v0[k];
+            }
+        }
+    }
+}
