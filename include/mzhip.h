@@ -137,6 +137,18 @@ int mz_set_surf(mz_store *h, int32_t which, const double *in);
 int mz_ntc(mz_store *h, int32_t s1, int32_t s2, double dt, const double *V, double dw_tol,
            int32_t equal_weight, uint64_t tag, int32_t rng_mode, int32_t cell_lo, int32_t cell_hi);
 
+/* swpm!(rng, collision_factors_swpm, collision_data, interaction, pv, pia,
+ * cell, species, G, dt, V) (src/collisions/collision_swpm.jl:201-288) for
+ * cells cell_lo..cell_hi of one species, cells in parallel, each on its own
+ * stream (cell - 1) under `tag`.  For this method the pair's array of
+ * mz_set_sigma_g_w_max / mz_get_collision_factors holds (sigma g)_max (the
+ * reference's CollisionFactorsSWPM.sigma_g_max), and the counters n1, n2,
+ * n_coll, n_coll_performed (n_eq_w = 0).  Created particles (two per
+ * performed collision) are appended to group 2 in cell order, as a sequential
+ * loop over cells would append them. */
+int mz_swpm(mz_store *h, int32_t species, double G, double dt, const double *V, uint64_t tag, int32_t rng_mode,
+            int32_t cell_lo, int32_t cell_hi);
+
 /* compute_props! (variant 0, groups 1 and 2) or compute_props_sorted!
  * (variant 1, group 1; ndens_not_Np with inv_V from the grid)
  * (src/properties/physical_props.jl:104-154, 317-454).  Results stay on the

@@ -54,6 +54,7 @@ _SIGS = {
     "mz_get_surf": (C.c_int, [C.c_void_p, i32, P(f64)]),
     "mz_set_surf": (C.c_int, [C.c_void_p, i32, P(f64)]),
     "mz_ntc": (C.c_int, [C.c_void_p, i32, i32, f64, P(f64), f64, i32, u64, i32, i32, i32]),
+    "mz_swpm": (C.c_int, [C.c_void_p, i32, f64, f64, P(f64), u64, i32, i32, i32]),
     "mz_compute_props": (C.c_int, [C.c_void_p, i32, i32, P(f64)]),
     "mz_get_props": (C.c_int, [C.c_void_p, P(f64), P(f64), P(f64), P(f64), P(f64)]),
     "mz_compute_props_moments": (C.c_int, [C.c_void_p, P(i32), i32, f64, P(f64)]),
@@ -567,6 +568,18 @@ def ntc(store, s1, s2, dt, V=None, dw_tol=1e-16, equal_weight=False, tag=0, rng_
                                                                       (store.n_cells,)))
     _check(load_library().mz_ntc(store.h, s1, s2, dt, _d(Vd), dw_tol, 1 if equal_weight else 0, tag, rng_mode,
                                  lo, hi))
+
+
+def swpm(store, species, G, dt, V=None, tag=0, rng_mode=RNG_PHILOX, cells=None):
+    """swpm!(rng, collision_factors_swpm, collision_data, interaction, pv,
+    pia, cell, species, G, dt, V) over cells (1-based inclusive range)
+    (src/collisions/collision_swpm.jl:201-288).  (sigma g)_max per cell is
+    set with DeviceStore.set_sigma_g_w_max and read back by
+    collision_factors (its "sigma_g_w_max" entry)."""
+    lo, hi = cells if cells is not None else (1, store.n_cells)
+    Vd = None if V is None else np.ascontiguousarray(np.broadcast_to(np.asarray(V, dtype=np.float64),
+                                                                      (store.n_cells,)))
+    _check(load_library().mz_swpm(store.h, species, float(G), dt, _d(Vd), tag, rng_mode, lo, hi))
 
 
 def ntc_equal_weight(store, s1, s2, dt, V=None, tag=0, rng_mode=RNG_PHILOX, cells=None):

@@ -408,3 +408,167 @@ extern "C" int mz_ntc(mz_store *h, int32_t s1, int32_t s2, double dt, const doub
     mz_timing_end(h);
     return r;
 }
+
+// ===========================================================================
+// swpm! (collision_swpm.jl:201-288): single-species collisions of
+// variable-weight particles.  One lane per cell, as NTC: the loop is
+// sequential (the running (sigma g)_max, and particles created by the loop
+// are candidates of the same loop).  An accepted pair gives up
+// min(w_i, w_k) / (1 + G) of each weight to two new particles at the pair's
+// positions, which then take the pair's post-collision velocities
+// (scatter_vhs!); the pair keeps its velocities.  Created particles go to
+// per-cell scratch and are appended in cell order as for NTC splits.  For
+// this method the pair's (sigma g w)_max array holds (sigma g)_max.
+
+// Count pass: w_max of every cell, its candidate count (draw 0 of its
+// stream, compute_n_coll_single_species, collision_swpm.jl:171-174) and the
+// scratch it may need (two particles per candidate).
+__global__ void k_swpm_count(NtcArgs A, double G, int32_t *__restrict__ ncoll, int32_t *__restrict__ ssz,
+                             double *__restrict__ wmax) {
+    const int c = A.cell_lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > A.cell_hi) return;
+    const int32_t n1 = A.a.pia.n1[c], n2 = A.a.pia.n2[c] > 0 ? A.a.pia.n2[c] : 0, np = A.a.pia.nloc[c];
+    double w_max = 0.0;
+    for (int32_t li = 0; li < n1 + n2; ++li) {
+        const double w = A.a.w[slot_of(A.a, c, li, n1, n2)];
+        w_max = w > w_max ? w : w_max;
+    }
+    A.rp = mz_rng_resolve(A.rp);
+    MzStream st = mz_stream((uint64_t)c);
+    const double R = mz_rand(A.rp, st);
+    double f = floor(0.5 * A.dt * (double)np * (double)(np - 1) * A.sgwm[c] * w_max * (G + 1) / A.V[c] + R);
+    if (!(f <= (double)MZ_NTC_MAX_CANDIDATES / 2)) {   // also catches NaN estimates
+        atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH);
+        f = 0.0;
+    }
+    ncoll[c] = (int32_t)f;
+    ssz[c] = 2 * (int32_t)f;
+    wmax[c] = w_max;
+}
+
+__global__ void k_swpm(NtcArgs A, double G, const double *__restrict__ wmax) {
+    const int c = A.cell_lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > A.cell_hi) return;
+    A.rp = mz_rng_resolve(A.rp);
+    const int32_t n1 = A.a.pia.n1[c], n2 = A.a.pia.n2[c] > 0 ? A.a.pia.n2[c] : 0;
+    const int32_t nl = A.a.pia.nloc[c];
+    const double wtf = 1.0 / (1.0 + G);
+    const double inv_w_max = 1.0 / wmax[c];
+    double sgmax = A.sgwm[c];
+    MzStream st = mz_stream((uint64_t)c);
+    st.k = 1;
+    const int64_t n_coll = A.ncoll_in[c];
+    const int64_t s_base = A.soff[c];
+    int32_t na = nl, created = 0, performed = 0;
+    for (int64_t cc = 0; cc < n_coll; ++cc) {
+        const int32_t i = (int32_t)floor(mz_rand(A.rp, st) * (double)na);
+        int32_t k = (int32_t)floor(mz_rand(A.rp, st) * (double)na);
+        while (i == k) {
+            k = (int32_t)floor(mz_rand(A.rp, st) * (double)na);
+            if (A.rp.mode == 1 && (int64_t)st.k > A.rp.cols) return;   // replay row exhausted
+        }
+        PRef pi = i < nl ? pref_slot(A.a, slot_of(A.a, c, i, n1, n2)) : pref_scratch(A, 0, s_base + (i - nl));
+        PRef pk = k < nl ? pref_slot(A.a, slot_of(A.a, c, k, n1, n2)) : pref_scratch(A, 0, s_base + (k - nl));
+        const double vi0 = pi.v0[pi.i], vi1 = pi.v1[pi.i], vi2 = pi.v2[pi.i];
+        const double vk0 = pk.v0[pk.i], vk1 = pk.v1[pk.i], vk2 = pk.v2[pk.i];
+        double g0, g1, g2;
+        const double g = mz_rel_speed(vi0, vi1, vi2, vk0, vk1, vk2, g0, g1, g2);
+        if (!(g > MZ_G_EPS)) continue;
+        const double sgm = mz_vhs_sgw(A.factor, A.expo, g, 1.0, 1.0);   // sigma g
+        sgmax = sgm > sgmax ? sgm : sgmax;
+        const double wi = pi.w[pi.i], wk = pk.w[pk.i];
+        if (!(mz_rand(A.rp, st) < sgm * (wi > wk ? wi : wk) * inv_w_max / sgmax)) continue;
+        ++performed;
+        double vc0, vc1, vc2;
+        mz_com(A.mu1, A.mu2, vi0, vi1, vi2, vk0, vk1, vk2, vc0, vc1, vc2);
+        const double dw = (wi < wk ? wi : wk) * wtf;
+        pi.w[pi.i] = wi - dw;
+        pk.w[pk.i] = wk - dw;
+        if (created + 1 >= 2 * n_coll) {
+            atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH);
+            continue;
+        }
+        PRef n_i = pref_scratch(A, 0, s_base + created), n_k = pref_scratch(A, 0, s_base + created + 1);
+        n_i.x0[n_i.i] = pi.x0[pi.i]; n_i.x1[n_i.i] = pi.x1[pi.i]; n_i.x2[n_i.i] = pi.x2[pi.i];
+        n_k.x0[n_k.i] = pk.x0[pk.i]; n_k.x1[n_k.i] = pk.x1[pk.i]; n_k.x2[n_k.i] = pk.x2[pk.i];
+        n_i.w[n_i.i] = dw;
+        n_k.w[n_k.i] = dw;
+        created += 2;
+        na += 2;
+        // scatter_vhs! on the two new particles: phi first, then cos(theta)
+        double ui0, ui1, ui2, uk0, uk1, uk2;
+        mz_vhs_scatter(A.rp, st, A.mu1, A.mu2, g, vc0, vc1, vc2, ui0, ui1, ui2, uk0, uk1, uk2);
+        n_i.v0[n_i.i] = ui0; n_i.v1[n_i.i] = ui1; n_i.v2[n_i.i] = ui2;
+        n_k.v0[n_k.i] = uk0; n_k.v1[n_k.i] = uk1; n_k.v2[n_k.i] = uk2;
+    }
+    A.sgwm[c] = sgmax;
+    int32_t *cn = A.cnt + (int64_t)c * 5;
+    cn[0] = nl;
+    cn[1] = nl;
+    cn[2] = (int32_t)n_coll;
+    cn[3] = performed;
+    cn[4] = 0;
+    A.created[c] = created;
+}
+
+extern "C" int mz_swpm(mz_store *h, int32_t species, double G, double dt, const double *V, uint64_t tag,
+                       int32_t rng_mode, int32_t cell_lo, int32_t cell_hi) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "swpm: bad species");
+    MZ_CHECK(cell_lo >= 1 && cell_hi <= h->n_cells && cell_lo <= cell_hi, "swpm: bad cell range");
+    MZ_CHECK(G >= 0.0, "swpm: G must be non-negative");
+    MZ_CHECK(h->inter[(species - 1) * h->n_species + (species - 1)].set, "swpm: interaction not set");
+    MZ_CHECK(rng_mode == 0 || (rng_mode == 1 && h->d_replay[0]), "swpm: replay table not set");
+    const double *dV = h->d_V;
+    if (V) {
+        double *v = (double *)mz_scratch(h, "ntc_V", h->n_cells * sizeof(double));
+        MZ_CHECK(v != nullptr, "swpm: scratch allocation failed");
+        MZ_HIP(hipMemcpy(v, V, h->n_cells * sizeof(double), hipMemcpyHostToDevice));
+        dV = v;
+    } else {
+        MZ_CHECK(h->has_grid, "swpm: V is NULL and no grid is set");
+    }
+    NtcArgs A = base_args(h, species, species, dt, dV, 0.0, 0, tag, rng_mode, cell_lo, cell_hi);
+    A.may_split = 1;
+    const int nc = h->n_cells;
+    const int ncells = cell_hi - cell_lo + 1;
+    int32_t *ncoll = (int32_t *)mz_scratch(h, "ntc_ncoll", (size_t)nc * sizeof(int32_t));
+    int32_t *ssz = (int32_t *)mz_scratch(h, "swpm_ssz", (size_t)nc * sizeof(int32_t));
+    double *wmax = (double *)mz_scratch(h, "swpm_wmax", (size_t)nc * sizeof(double));
+    int64_t *soff = (int64_t *)mz_scratch(h, "ntc_soff", (size_t)(nc + 1) * sizeof(int64_t));
+    int32_t *created = (int32_t *)mz_scratch(h, "ntc_created", (size_t)2 * nc * sizeof(int32_t));
+    int64_t *coff = (int64_t *)mz_scratch(h, "ntc_coff", (size_t)2 * (nc + 1) * sizeof(int64_t));
+    MZ_CHECK(ncoll && ssz && wmax && soff && created && coff, "swpm: scratch allocation failed");
+    MZ_HIP(hipMemsetAsync(ncoll, 0, (size_t)nc * sizeof(int32_t), h->stream));
+    MZ_HIP(hipMemsetAsync(ssz, 0, (size_t)nc * sizeof(int32_t), h->stream));
+    MZ_HIP(hipMemsetAsync(created, 0, (size_t)2 * nc * sizeof(int32_t), h->stream));
+    mz_timing_begin(h, "swpm");
+    hipLaunchKernelGGL(k_swpm_count, dim3((ncells + 255) / 256), dim3(256), 0, h->stream, A, G, ncoll, ssz, wmax);
+    hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, h->stream, ssz, (int64_t)nc, soff, soff + nc);
+    int64_t total = 0;
+    MZ_HIP(hipMemcpyAsync(&total, soff + nc, sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    const int64_t cap = std::max<int64_t>(total, 1);
+    double *sc = (double *)mz_scratch(h, "ntc_sc", (size_t)7 * cap * sizeof(double));
+    MZ_CHECK(sc != nullptr, "swpm: creation scratch allocation failed");
+    for (int f = 0; f < 7; ++f) A.sc[f] = sc + (size_t)f * cap;
+    A.sc_cap = cap;
+    A.ncoll_in = ncoll;
+    A.soff = soff;
+    A.created = created;
+    hipLaunchKernelGGL(k_swpm, dim3((ncells + 63) / 64), dim3(64), 0, h->stream, A, G, (const double *)wmax);
+    MZ_HIP(hipGetLastError());
+    double **scp = (double **)mz_scratch(h, "ntc_scp", 7 * sizeof(double *));
+    MZ_CHECK(scp != nullptr, "swpm: scratch allocation failed");
+    MZ_HIP(hipMemcpy(scp, A.sc, 7 * sizeof(double *), hipMemcpyHostToDevice));
+    MzCounters snap;
+    int r = mz_read_counters(h, species, &snap);
+    if (r) return r;
+    hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, h->stream, created, (int64_t)(A.cell_hi + 1), coff,
+                       coff + nc);
+    hipLaunchKernelGGL(k_ntc_append, dim3(ncells), dim3(64), 0, h->stream, mz_view(h->sp[species - 1]), A.cell_lo,
+                       A.cell_hi, created, coff, coff + nc, soff, (const double *const *)scp, (int64_t)0, snap);
+    hipLaunchKernelGGL(k_ntc_finish, dim3(1), dim3(256), 0, h->stream, mz_view(h->sp[species - 1]), coff + nc, snap);
+    mz_timing_end(h);
+    MZ_HIP(hipGetLastError());
+    return 0;
+}

@@ -176,6 +176,25 @@ void mzo_collide_pair(mzo_rng *rng, mzo_cf *cf, const mzo_interaction *it, mzo_p
                       mzo_pia *pia, int64_t cell, int64_t s1, int64_t s2, int64_t id_i, int64_t id_k,
                       double dw_tol, int equal_weight);
 
+/* CollisionFactorsSWPM (src/collisions/collision_swpm.jl:16-23) */
+typedef struct {
+    int64_t n1, n2;
+    double sigma_g_max;
+    int64_t n_coll, n_coll_performed;
+} mzo_cf_swpm;
+
+/* swpm!(rng, cf, collision_data, interaction, particles, pia, cell, species, G,
+ * dt, V) (src/collisions/collision_swpm.jl:201-288) with
+ * compute_n_coll_single_species (:171-174): single-species collisions of
+ * variable-weight particles; an accepted pair gives up min(w_i, w_k) / (1 + G)
+ * of each weight to two new particles, which carry the pair's post-collision
+ * velocities.  Draw order per cell stream: [n_coll], per candidate [i, k (, k
+ * redraws)], [accept] if g > eps, [phi, cos theta] if accepted.  Parity:
+ * pinned by closed forms on replayed draws and the conservation laws
+ * (tests/test_oracle_swpm.py). */
+void mzo_swpm(mzo_rng *rng, mzo_cf_swpm *cf, const mzo_interaction *it, mzo_pv *pv, mzo_pia *pia, int64_t cell,
+              int64_t species, double G, double dt, double V);
+
 /* ---------------- properties (src/properties/physical_props.jl) ---------------- */
 typedef struct {
     int64_t n_cells, n_species, n_moments;
@@ -203,7 +222,6 @@ void mzo_compute_props_with_total_moments(mzo_pv **pvs, mzo_pia *pia, const doub
                                           mzo_props *pp);
 double mzo_compute_mixed_moment(mzo_pv *pv, mzo_pia *pia, int64_t cell, int64_t species,
                                 int px, int py, int pz, double sum_scaler, double res_scaler);
-void mzo_avg_props(mzo_props *avg, const mzo_props *pp, double n_avg_timesteps);
 void mzo_clear_props(mzo_props *pp);
 
 /* ---------------- octree N:2 merging (src/merging/merging_octree_N2.jl) ---------------- */

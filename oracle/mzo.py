@@ -58,6 +58,10 @@ class SurfS(C.Structure):
                 ("q", P(f64))]
 
 
+class CFSwpm(C.Structure):
+    _fields_ = [("n1", i64), ("n2", i64), ("sigma_g_max", f64), ("n_coll", i64), ("n_coll_performed", i64)]
+
+
 class Interaction(C.Structure):
     _fields_ = [(n, f64) for n in ("m_r", "mu1", "mu2", "vhs_d", "vhs_o", "vhs_Tref", "vhs_muref", "vhs_factor")]
 
@@ -144,6 +148,7 @@ def lib():
             "mzo_convect_particles": (None, [P(Rng), P(Grid1D), P(Walls), P(PvS), P(PiaS), i64, f64]),
             "mzo_props_avg": (None, [P(PropsS), P(PropsS), f64]),
             "mzo_delete_particle": (None, [P(PvS), P(PiaS), i64, i64, i64]),
+            "mzo_swpm": (None, [P(Rng), P(CFSwpm), P(Interaction), P(PvS), P(PiaS), i64, i64, f64, f64, f64]),
             "mzo_merge_roulette": (None, [P(Rng), P(PvS), P(PiaS), i64, i64, i64, C.c_int]),
             "mzo_surf_update_incident": (None, [P(SurfS), f64, P(f64), i64, i64]),
             "mzo_surf_update_reflected": (None, [P(SurfS), f64, P(f64), i64, i64]),

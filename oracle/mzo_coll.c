@@ -153,6 +153,82 @@ void mzo_ntc_single(mzo_rng *rng, mzo_cf *cf, const mzo_interaction *it, mzo_pv 
     }
 }
 
+/* swpm! (src/collisions/collision_swpm.jl:201-288).  The collision count
+ * (compute_n_coll_single_species, :171-174) has no fusable product-sum: its
+ * last operation is a division, added to the draw. */
+void mzo_swpm(mzo_rng *rng, mzo_cf_swpm *cf, const mzo_interaction *it, mzo_pv *pv, mzo_pia *pia, int64_t cell,
+              int64_t species, double G, double dt, double V) {
+    mzo_ind *ind = mzo_pia_at(pia, cell, species);
+    double w_max = 0.0;
+    for (int64_t i = ind->start1; i <= ind->end1; ++i) {
+        double w = PV_W(pv, PID(pv, i));
+        w_max = w > w_max ? w : w_max;
+    }
+    if (ind->n_group2 > 0) {
+        for (int64_t i = ind->start2; i <= ind->end2; ++i) {
+            double w = PV_W(pv, PID(pv, i));
+            w_max = w > w_max ? w : w_max;
+        }
+    }
+    const double wtf = 1.0 / (1.0 + G);
+    const double inv_w_max = 1.0 / w_max;
+    mzo_rng_set_stream(rng, (uint64_t)(cell - 1));
+    cf->n1 = ind->n_local;
+    cf->n2 = ind->n_local;
+    const int64_t np = ind->n_local;
+    const double ncf = 0.5 * dt * (double)np * (double)(np - 1) * cf->sigma_g_max * w_max * (G + 1) / V +
+                       mzo_rand(rng);
+    const int64_t n_coll = (int64_t)floor(ncf);
+    cf->n_coll = n_coll;
+    cf->n_coll_performed = 0;
+    for (int64_t c = 0; c < n_coll; ++c) {
+        int64_t i = (int64_t)floor(mzo_rand(rng) * (double)ind->n_local);
+        int64_t k = (int64_t)floor(mzo_rand(rng) * (double)ind->n_local);
+        while (i == k && !rng->err) k = (int64_t)floor(mzo_rand(rng) * (double)ind->n_local);
+        if (rng->err) return;   /* replay table exhausted */
+        i = mzo_map_cont_index(ind, i);
+        k = mzo_map_cont_index(ind, k);
+        const int64_t id_i = PID(pv, i), id_k = PID(pv, k);
+        const double g = rel_speed(PV_V(pv, id_i), PV_V(pv, id_k));
+        if (!(g > 2.220446049250313e-16)) continue;
+        const double sgm = mzo_sigma_vhs(it, g) * g;
+        cf->sigma_g_max = sgm > cf->sigma_g_max ? sgm : cf->sigma_g_max;
+        const double wi = PV_W(pv, id_i), wk = PV_W(pv, id_k);
+        if (!(mzo_rand(rng) < sgm * (wi > wk ? wi : wk) * inv_w_max / cf->sigma_g_max)) continue;
+        cf->n_coll_performed += 1;
+        double vcom[3];
+        {
+            const double *vi = PV_V(pv, id_i), *vk = PV_V(pv, id_k);
+            for (int d = 0; d < 3; ++d) vcom[d] = fma(it->mu1, vi[d], it->mu2 * vk[d]);
+        }
+        if (pv->length <= pia->n_total[species - 1]) mzo_pv_resize(pv, pv->length + MZO_DELTA_PARTICLES);
+        const double dw = (wi < wk ? wi : wk) * wtf;
+        PV_W(pv, id_i) = wi - dw;
+        PV_W(pv, id_k) = wk - dw;
+        int64_t nid[2];
+        const int64_t src[2] = {id_i, id_k};
+        for (int j = 0; j < 2; ++j) {
+            mzo_update_buffer_index_new_particle(pv, pia, cell, species);
+            nid[j] = PID(pv, pia->n_total[species - 1]);
+            PV_W(pv, nid[j]) = dw;
+            memcpy(PV_V(pv, nid[j]), PV_V(pv, src[j]), 3 * sizeof(double));
+            memcpy(PV_X(pv, nid[j]), PV_X(pv, src[j]), 3 * sizeof(double));
+        }
+        ind = mzo_pia_at(pia, cell, species);
+        /* scatter_vhs! on the two new particles: phi first, then cos(theta) */
+        const double phi = TWOPI * mzo_rand(rng);
+        const double cphi = cos(phi), sphi = sin(phi);
+        const double ct = 2.0 * mzo_rand(rng) - 1.0;
+        const double st = sqrt(1.0 - ct * ct);
+        const double gn[3] = {g * (st * cphi), g * (st * sphi), g * ct};
+        double *v1 = PV_V(pv, nid[0]), *v2 = PV_V(pv, nid[1]);
+        for (int d = 0; d < 3; ++d) {
+            v1[d] = fma(it->mu2, gn[d], vcom[d]);
+            v2[d] = fma(-it->mu1, gn[d], vcom[d]);
+        }
+    }
+}
+
 /* ntc! two species (src/collisions/collision_ntc.jl:412-453) and
  * ntc_equal_weight! (:554-595). */
 void mzo_ntc_two(mzo_rng *rng, mzo_cf *cf, const mzo_interaction *it, mzo_pv *pv1, mzo_pv *pv2,
@@ -323,22 +399,6 @@ double mzo_compute_mixed_moment(mzo_pv *pv, mzo_pia *pia, int64_t cell, int64_t 
         }
     }
     return r * res_scaler;
-}
-
-/* avg_props! (physical_props.jl:281-299); the ndens/Np consistency check is
- * the caller's. */
-void mzo_avg_props(mzo_props *avg, const mzo_props *pp, double n_avg_timesteps) {
-    double inv = 1.0 / n_avg_timesteps;
-    for (int64_t s = 0; s < pp->n_species; ++s) {
-        avg->lpa[s] = avg->lpa[s] + pp->lpa[s] * inv;
-        for (int64_t c = 0; c < pp->n_cells; ++c) {
-            int64_t ix = s * pp->n_cells + c;
-            avg->np[ix] = avg->np[ix] + pp->np[ix] * inv;
-            avg->n[ix] = avg->n[ix] + pp->n[ix] * inv;
-            for (int k = 0; k < 3; ++k) avg->v[3 * ix + k] = avg->v[3 * ix + k] + pp->v[3 * ix + k] * inv;
-            avg->T[ix] = avg->T[ix] + pp->T[ix] * inv;
-        }
-    }
 }
 
 /* clear_props! (physical_props.jl:256-262) */
