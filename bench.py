@@ -490,6 +490,148 @@ def run_roulette(a, pkg):
     print(json.dumps(out), flush=True)
 
 
+# SWPM collisions with octree N:2 merging (the loop of
+# test/test_bkw_varweight_octree_swpm.jl) over many independent BKW boxes:
+# 12,288 cells of V = 1, each 1,000 equal-weight argon particles (Fnum =
+# 1e23 / 1,000, pseudo-Maxwell molecules) with BKW velocities at 273 K, at
+# the cell centre.  A step: swpm! with G = 1 in every cell, octree N:2
+# merging of every cell above 1,250 particles to 1,000 (512 bins), and the
+# sort that puts the new particles back in their cells' runs.
+SW_CELLS, SW_PPC, SW_THR, SW_TARGET, SW_T0, SW_NDENS = 12288, 1000, 1250, 1000, 273.0, 1e23
+
+
+def sample_bkw_boxes(n_cells, ppc, seed=11):
+    g = np.random.Generator(np.random.Philox(seed))
+    n = n_cells * ppc
+    m = 66.3e-27
+    vscale = np.sqrt(2 * K_B * SW_T0 / m) * np.sqrt(0.3)
+    vabs = np.sqrt(g.chisquare(5, size=n))
+    th = g.uniform(size=n) * np.pi
+    ph = g.uniform(size=n) * 2 * np.pi
+    v = np.empty((n, 3))
+    v[:, 0] = vscale * vabs * np.sin(th) * np.cos(ph)
+    v[:, 1] = vscale * vabs * np.sin(th) * np.sin(ph)
+    v[:, 2] = vscale * vabs * np.cos(th)
+    x = np.full((n, 3), 0.5)
+    x[:, 0] = np.repeat(np.arange(n_cells, dtype=np.float64), ppc) + 0.5
+    w = np.full(n, SW_NDENS / ppc)
+    pia = np.zeros((n_cells, 7), dtype=np.int64)
+    pia[:, 0] = ppc
+    pia[:, 1] = np.arange(n_cells) * ppc + 1
+    pia[:, 2] = np.arange(1, n_cells + 1) * ppc
+    pia[:, 3] = ppc
+    pia[:, 5] = -1
+    return dict(x=x, v=v, w=w, pia=pia, n=n, mass=m)
+
+
+def swpm_cpu_baseline(sp, ia, dt, n_cells=128, target_s=12.0):
+    """The C oracle's swpm!, octree merge and sort on one core, over n_cells
+    boxes of the same workload; returns (particle-timesteps/s, steps)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mzo
+    mzo.build()
+    case = sample_bkw_boxes(n_cells, SW_PPC, seed=12)
+    n = case["n"]
+    pv, pia = mzo.ParticleVector(n), mzo.PIA(n_cells, 1)
+    for i in range(1, n + 1):
+        pv.pop_to(i)
+    pv.x[:n], pv.v[:n], pv.w[:n] = case["x"], case["v"], case["w"]
+    for c in range(n_cells):
+        pia.set(c + 1, 1, *case["pia"][c])
+    pia.set_n_total(1, n)
+    grid = mzo.grid1d(float(n_cells), n_cells)
+    oit = mzo.interactions(["Ar"], table=mzo.PSEUDO_MAXWELL)[0][0]
+    cfs = [mzo.CFSwpm() for _ in range(n_cells)]
+    for cf in cfs:
+        cf.sigma_g_max = mzo.lib().mzo_estimate_sigma_g_w_max(C.byref(oit), sp[0].mass, sp[0].mass, SW_T0, SW_T0,
+                                                              1.0, 1.0)
+    oc = mzo.Octree(split=1, init_bin_bounds=1, bin_bounds_compute=1, max_Nbins=512, max_depth=10)
+
+    def step(k):
+        r = mzo.RNG(seed=11, tag=(k << 4) | 1)
+        done = 0
+        for c in range(1, n_cells + 1):
+            done += pia.ind(c).n_local
+            mzo.lib().mzo_swpm(r.ref, C.byref(cfs[c - 1]), C.byref(oit), pv.p, pia.p, c, 1, 1.0, dt, 1.0)
+        r = mzo.RNG(seed=11, tag=(k << 4) | 3)
+        for c in range(1, n_cells + 1):
+            if pia.ind(c).n_local > SW_THR:
+                mzo.lib().mzo_merge_octree_N2_based(r.ref, oc.p, pv.p, pia.p, c, 1, SW_TARGET, None)
+        mzo.lib().mzo_squash_pia(pv.p, pia.p, 1)
+        mzo.lib().mzo_sort_particles(C.byref(grid), pv.p, pia.p, 1)
+        return done
+
+    for k in range(1, 6):
+        step(k)
+    t0 = time.perf_counter()
+    done, k = 0, 6
+    while time.perf_counter() - t0 < target_s:
+        done += step(k)
+        k += 1
+    return done / (time.perf_counter() - t0), k - 6
+
+
+def run_bkw_swpm(a, pkg):
+    case = sample_bkw_boxes(SW_CELLS, SW_PPC)
+    n = case["n"]
+    data = os.path.join(ROOT, "merzbild.jl_amd", "data")
+    sp = pkg.load_species_data(os.path.join(data, "particles.toml"), ["Ar"])
+    it = pkg.load_interaction_data(os.path.join(data, "pseudo_maxwell.toml"), sp)
+    st = pkg.DeviceStore(SW_CELLS, 1, 2 * n)
+    st.upload_species(1, case["x"], case["v"], case["w"], np.arange(1, n + 1, dtype=np.int64), case["pia"], n, True,
+                      np.zeros(0, dtype=np.int64), n)
+    del case["x"], case["v"], case["w"]
+    st.set_grid(pkg.Grid1DUniform(float(SW_CELLS), SW_CELLS))     # V = 1 per cell
+    st.set_interactions(it)
+    st.set_species_data(sp)
+    ia = it[0][0]
+    st.set_sigma_g_w_max(1, 1, np.full(SW_CELLS, pkg.estimate_sigma_g_w_max(ia, sp[0].mass, sp[0].mass, SW_T0,
+                                                                            SW_T0, 1.0)))
+    st.set_seed(11)
+    tref = 1.0 / (SW_NDENS * np.pi * ia.vhs_d ** 2) / np.sqrt(2 * K_B * SW_T0 / sp[0].mass)
+    dt = 0.025 * tref
+    steps = a.steps if a.steps != 20000 else 40
+    warm = a.warmup if a.warmup != 200 else 10
+
+    def step(k):
+        pkg.swpm(st, 1, 1.0, dt, tag=(k << 4) | 1)
+        merged = pkg.merge_octree_N2_based(st, 1, SW_THR, SW_TARGET, max_Nbins=512, squash=True, tag=(k << 4) | 3)
+        pkg.sort_particles(st, 1)
+        return merged
+
+    for k in range(1, warm + 1):
+        step(k)
+    st.sync()
+    st.timing_reset()
+    st.timing(True)
+    done, merged = 0, 0
+    t0 = time.perf_counter()
+    for k in range(warm + 1, warm + steps + 1):
+        done += st.scalars(1)["n_total"]
+        merged += step(k)
+    st.sync()
+    el = time.perf_counter() - t0
+    fam = {f: st.timing_get(f)[0] for f in ("swpm", "merge_octree", "sort")}
+    st.timing(False)
+    cf = st.collision_factors(1, 1)
+    sc = st.scalars(1)
+    assert sc["err"] == 0, sc
+    out = {"metric": "particle-timesteps/s (SWPM + octree merge + sort), BKW boxes", "value": done / el,
+           "unit": "particle-timesteps/s", "n_gpus": 1, "steps": steps, "warmup": warm, "ms_per_step": el * 1e3 / steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "12288 0-D BKW boxes x 1000 Ar particles (pseudo-Maxwell, 273 K), swpm! G = 1, "
+                                  "octree N:2 merge above 1250 to 1000 (512 bins), sort; dt = 0.025 t_ref",
+                      "n_particles_end": sc["n_total"], "cells_merged_per_step": merged / steps,
+                      "collisions_per_step": int(np.sum(cf["n_coll_performed"]))},
+           "kernel_ms_per_call": fam}
+    if not a.no_cpu_baseline:
+        v, k = swpm_cpu_baseline(sp, ia, dt)
+        out["cpu_baseline"] = {"value": v, "unit": "particle-timesteps/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} steps of the same loop over 128 of the boxes on the C oracle"}
+    print(json.dumps(out), flush=True)
+    st.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -503,7 +645,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="batched-0d: skip timing the plain and per-cell launch patterns")
     ap.add_argument("--no-autotune", action="store_true", help="slab mode: keep the library's default shape")
-    ap.add_argument("--workload", choices=("couette", "batched-0d", "nnls-merge", "couette-surf", "roulette-merge"),
+    ap.add_argument("--workload", choices=("couette", "batched-0d", "nnls-merge", "couette-surf", "roulette-merge",
+                                           "bkw-swpm"),
                     default="couette",
                     help="couette: the headline 1-D Couette line; batched-0d: 65,536 uneven 0-D cells; "
                          "nnls-merge: 32,768 cells merged by NNLS; couette-surf: the averaging phase of the "
@@ -539,6 +682,9 @@ def main():
         return
     if a.workload == "roulette-merge":
         run_roulette(a, pkg)
+        return
+    if a.workload == "bkw-swpm":
+        run_bkw_swpm(a, pkg)
         return
     case = sample_case()
     st, sp, it = setup_store(pkg, case)

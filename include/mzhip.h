@@ -94,6 +94,11 @@ int mz_set_replay(mz_store *h, int32_t kind, const double *table, int64_t rows, 
  * when 0 (src/grids/grid_sorting.jl:58-113, 128-182).  Squashes first if the
  * PIA is not contiguous. */
 int mz_sort(mz_store *h, int32_t species, int32_t compute_cells);
+/* Which sort runs (a tuning knob; the reference has one sort): 0 chooses by
+ * size, 1 the per-chunk count matrix over cells (at most 14,336 cells), 2 the
+ * 8-bit-digit radix sort, whose traffic does not grow with the cell count.
+ * Both give the same stable order. */
+int mz_set_sort_path(mz_store *h, int32_t path);
 
 /* squash_pia!(pv, pia, species) (src/particles.jl:622-701) */
 int mz_squash(mz_store *h, int32_t species);

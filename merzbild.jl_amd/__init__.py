@@ -45,6 +45,7 @@ _SIGS = {
     "mz_set_seed": (C.c_int, [C.c_void_p, u64]),
     "mz_set_replay": (C.c_int, [C.c_void_p, i32, P(f64), i64, i64]),
     "mz_sort": (C.c_int, [C.c_void_p, i32, i32]),
+    "mz_set_sort_path": (C.c_int, [C.c_void_p, i32]),
     "mz_squash": (C.c_int, [C.c_void_p, i32]),
     "mz_convect": (C.c_int, [C.c_void_p, i32, f64, u64, i32]),
     "mz_convect_surf": (C.c_int, [C.c_void_p, i32, f64, u64, i32, f64]),
@@ -383,6 +384,10 @@ class DeviceStore:
     def surf_clear(self, which=0):
         """clear_props! of the device set `which` (0 instantaneous, 1 average)."""
         _check(load_library().mz_surf_clear(self.h, which))
+
+    def set_sort_path(self, path=0):
+        """0 chooses by size, 1 the count-matrix sort, 2 the radix sort (same order)."""
+        _check(load_library().mz_set_sort_path(self.h, {"auto": 0, "matrix": 1, "radix": 2}.get(path, path)))
 
     def set_slab_shape(self, cells_per_slab=0, particles_per_thread=0):
         _check(load_library().mz_set_slab_shape(self.h, cells_per_slab, particles_per_thread))

@@ -85,6 +85,7 @@ struct mz_store {
     int32_t slab_info[4] = {0, 0, 0, 0};
     int32_t last_path = -1;   // of mz_couette_steps: 0 eager, 1 graph, 2 slab
     int32_t n_moments = 0;    // moments of the last mz_compute_props_moments
+    int32_t sort_path = 0;    // 0 automatic, 1 count matrix, 2 radix (mz_set_sort_path)
     // batched 0-D cells: item packing (0 = default) and the last launch's
     // {items, max cells per item, target particles per item, workgroups}
     int32_t b0_max_cells = 0, b0_target = 0;
@@ -109,6 +110,7 @@ void mz_swap_buffers(MzSpecies &s);
 void mz_graph_release(mz_store *h);
 
 int mz_launch_sort(mz_store *h, int species, int compute_cells);
+int mz_sort_prealloc(mz_store *h, int species);
 int mz_launch_convect(mz_store *h, int species, double dt, uint64_t tag, int rng_mode, bool contiguous,
                       int32_t n_pos, const uint64_t *step_ptr = nullptr);
 int mz_launch_props(mz_store *h, int variant, int ndens, double avg_inv = 0.0);

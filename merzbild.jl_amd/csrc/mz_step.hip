@@ -144,16 +144,14 @@ static int couette_run(mz_store *h, int32_t n_steps, uint64_t step0, double dt, 
     }
     // graph replay
     char keybuf[256];
-    std::snprintf(keybuf, sizeof(keybuf), "couette:%a:%d:%p:%a:%a:%a:%a", dt, props_every, (const void *)dV,
-                  avg_inv, mass, h->surf_inv_area[0], h->surf_inv_area[1]);
+    std::snprintf(keybuf, sizeof(keybuf), "couette:%a:%d:%p:%a:%a:%a:%a:%d", dt, props_every, (const void *)dV,
+                  avg_inv, mass, h->surf_inv_area[0], h->surf_inv_area[1], h->sort_path);
     if (h->graph_key != keybuf) {
         mz_graph_release(h);
         MZ_HIP(hipMalloc(&h->d_step, sizeof(uint64_t)));
         // scratch is allocated before capture
-        int32_t nchunks = (int32_t)((h->sp[0].cap + MZ_SORT_CHUNK - 1) / MZ_SORT_CHUNK);
-        MZ_CHECK(mz_scratch(h, "sort_hist", (size_t)h->n_cells * nchunks * sizeof(int32_t)) != nullptr &&
-                     mz_scratch(h, "sort_cstart", (size_t)h->n_cells * sizeof(int32_t)) != nullptr,
-                 "couette_steps: scratch allocation failed");
+        r = mz_sort_prealloc(h, 1);
+        if (r) return r;
         MZ_HIP(hipStreamSynchronize(h->stream));
         MZ_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
         for (int rep = 0; rep < 2; ++rep) {

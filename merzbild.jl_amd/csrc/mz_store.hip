@@ -672,6 +672,292 @@ __global__ void __launch_bounds__(256) k_sort_scatter(MzSpView a, MzSpView b, in
     }
 }
 
+// Sorting with many cells.  The count matrix of the path above holds
+// n_cells x chunks entries, so its traffic grows with the cell count (12,288
+// cells over 12.8M particles: 1.2 GB per sort).  Past a bound the sort runs
+// as a least-significant-digit radix sort of the (key, position) pairs on
+// 8-bit digits of key - 1, whose matrices hold 256 x chunks entries:
+//   k_rs_keys     keys (as k_sort_hist computes them), the clamped copy the
+//                 passes read, and the cell totals (wave-aggregated atomics);
+//   per digit:    k_rs_hist (per-chunk digit histogram), k_rs_rows (one block
+//                 per digit scans its row over the chunks), k_rs_scatter
+//                 (stable rank inside the chunk, as k_sort_scatter ranks);
+//   k_rs_gather   moves the particle data by the final permutation;
+//   k_rs_pia      scans the cell totals into the PIA.
+// Every pass is stable, so the result is the stable sort by cell, the same
+// permutation as the count-matrix path and the reference's reverse scatter.
+
+#define MZ_RS_CHUNK 4096
+
+__global__ void k_rs_zero(int32_t *__restrict__ a, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_rs_keys(MzSpView sp, MzGrid g, int32_t compute_cells, int32_t n_cells,
+                                                 int32_t *__restrict__ kout, int32_t *__restrict__ counts) {
+    const int32_t n = mz_ntotal(sp);
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p0 = (int64_t)blockIdx.x * blockDim.x; p0 < n; p0 += stride) {
+        const int64_t p = p0 + threadIdx.x;
+        const bool valid = p < n;
+        int32_t c = 0;
+        if (valid) {
+            if (compute_cells) {
+                c = mz_cell_of(sp.x0[p], g, n_cells, &sp.ctr->err);
+                sp.key[p] = c;
+            } else {
+                c = sp.key[p];
+                if (c < 1 || c > n_cells) {
+                    atomicOr(&sp.ctr->err, MZ_ERR_CAPACITY);
+                    c = c < 1 ? 1 : n_cells;
+                }
+            }
+            kout[p] = c;
+        }
+        uint64_t active = __ballot(valid);
+        while (active) {
+            const int leader = __ffsll((unsigned long long)active) - 1;
+            const int32_t k = __shfl(c, leader, 64);
+            const uint64_t m = __ballot(valid && c == k);
+            if (lane == leader) atomicAdd(&counts[k - 1], (int32_t)__popcll(m));
+            active &= ~m;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_rs_hist(MzSpView sp, const int32_t *__restrict__ ksrc, int32_t shift,
+                                                 int32_t nchunks, int32_t *__restrict__ mat) {
+    __shared__ int32_t h[256];
+    const int32_t n = mz_ntotal(sp);
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t p0 = (int64_t)blockIdx.x * MZ_RS_CHUNK;
+    for (int i = threadIdx.x; i < MZ_RS_CHUNK; i += 256) {
+        const int64_t p = p0 + i;
+        if (p < n) atomicAdd(&h[((ksrc[p] - 1) >> shift) & 255], 1);
+    }
+    __syncthreads();
+    mat[(int64_t)threadIdx.x * nchunks + blockIdx.x] = h[threadIdx.x];
+}
+
+// One block per digit: exclusive scan of its row over the chunks, in place;
+// the row total goes to tot[digit].
+__global__ void __launch_bounds__(256) k_rs_rows(int32_t nchunks, int32_t *__restrict__ mat,
+                                                 int32_t *__restrict__ tot) {
+    __shared__ int32_t wsum[4];
+    __shared__ int32_t carry_s;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int32_t *row = mat + (int64_t)blockIdx.x * nchunks;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nchunks; b0 += 256) {
+        const int b = b0 + threadIdx.x;
+        const int32_t v = b < nchunks ? row[b] : 0;
+        int32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
+        __syncthreads();
+        int32_t pre = carry_s;
+        for (int k = 0; k < wv; ++k) pre += wsum[k];
+        if (b < nchunks) row[b] = pre + x - v;
+        __syncthreads();
+        if (threadIdx.x == 255) carry_s = pre + x;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry_s;
+}
+
+// One block per chunk.  psrc == nullptr means the identity permutation.
+// kdst == nullptr skips the keys (the last pass needs only the permutation).
+__global__ void __launch_bounds__(256) k_rs_scatter(MzSpView sp, const int32_t *__restrict__ ksrc,
+                                                    const int32_t *__restrict__ psrc, int32_t *__restrict__ kdst,
+                                                    int32_t *__restrict__ pdst, int32_t shift, int32_t nchunks,
+                                                    const int32_t *__restrict__ mat,
+                                                    const int32_t *__restrict__ tot) {
+    __shared__ int32_t run[256];
+    __shared__ int32_t dig[MZ_RS_CHUNK];
+    __shared__ int32_t dst[MZ_RS_CHUNK];
+    __shared__ int32_t wsum[4];
+    const int32_t n = mz_ntotal(sp);
+    const int64_t p0 = (int64_t)blockIdx.x * MZ_RS_CHUNK;
+    if (p0 >= n) return;
+    const int cnt = (int)min((int64_t)MZ_RS_CHUNK, (int64_t)n - p0);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    {   // digit starts: exclusive scan of the 256 row totals, plus this chunk's offset
+        const int32_t v = tot[threadIdx.x];
+        int32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
+        __syncthreads();
+        int32_t pre = 0;
+        for (int k = 0; k < wv; ++k) pre += wsum[k];
+        run[threadIdx.x] = pre + x - v + mat[(int64_t)threadIdx.x * nchunks + blockIdx.x];
+    }
+    for (int i = threadIdx.x; i < cnt; i += 256) dig[i] = ((ksrc[p0 + i] - 1) >> shift) & 255;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        volatile int32_t *vrun = run;
+        const uint64_t lt = mz_lanemask_lt();
+        for (int bt = 0; bt < cnt; bt += 64) {
+            const int i = bt + lane;
+            const bool valid = i < cnt;
+            const int32_t d = valid ? dig[i] : -1;
+            uint64_t active = __ballot(valid);
+            while (active) {
+                const int leader = __ffsll((unsigned long long)active) - 1;
+                const int32_t k = __shfl(d, leader, 64);
+                const uint64_t m = __ballot(valid && d == k);
+                const int32_t bk = vrun[k];
+                if (valid && d == k) dst[i] = bk + __popcll(m & lt);
+                vrun[k] = bk + __popcll(m);
+                active &= ~m;
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += 256) {
+        const int64_t p = p0 + i;
+        const int32_t d = dst[i];
+        if (kdst) kdst[d] = ksrc[p];
+        pdst[d] = psrc ? psrc[p] : (int32_t)p;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_rs_gather(MzSpView a, MzSpView b, const int32_t *__restrict__ perm) {
+    const int32_t n = mz_ntotal(a);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < n; d += stride) {
+        const int64_t p = perm[d];
+        b.x0[d] = a.x0[p]; b.x1[d] = a.x1[p]; b.x2[d] = a.x2[p];
+        b.v0[d] = a.v0[p]; b.v1[d] = a.v1[p]; b.v2[d] = a.v2[p];
+        b.w[d] = a.w[p];
+        b.id[d] = a.id[p];
+    }
+}
+
+// One block: the cell totals scanned into the PIA (group 1 only, group 2
+// cleared, as grid_sorting.jl:76-98 writes it).
+__global__ void __launch_bounds__(1024) k_rs_pia(MzSpView sp, int32_t n_cells, const int32_t *__restrict__ counts) {
+    __shared__ int32_t wsum[16];
+    __shared__ int32_t carry_s;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    if (t == 0) carry_s = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < n_cells; c0 += 1024) {
+        const int c = c0 + t;
+        const int32_t v = c < n_cells ? counts[c] : 0;
+        int32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
+        __syncthreads();
+        int32_t pre = carry_s;
+        for (int k = 0; k < wv; ++k) pre += wsum[k];
+        const int32_t excl = pre + x - v;
+        if (c < n_cells) {
+            sp.pia.start2[c] = 0;
+            sp.pia.end2[c] = -1;
+            sp.pia.n2[c] = 0;
+            sp.pia.start1[c] = v > 0 ? excl + 1 : 0;
+            sp.pia.end1[c] = v > 0 ? excl + v : -1;
+            sp.pia.n1[c] = v;
+            sp.pia.nloc[c] = v;
+        }
+        __syncthreads();
+        if (t == 1023) carry_s = excl + v;
+        __syncthreads();
+    }
+    if (t == 0) {
+        sp.ctr->contiguous = 1;
+        sp.ctr->n_pos = mz_ntotal(sp);
+    }
+}
+
+// the count matrix costs 4 B x cells x chunks; past 32 B per particle slot
+// (or past the LDS histogram's cell bound) the radix path moves less
+static bool sort_uses_radix(const mz_store *h, const MzSpecies &s) {
+    const int64_t nchunks = (s.cap + MZ_SORT_CHUNK - 1) / MZ_SORT_CHUNK;
+    return h->sort_path == 2 ||
+           (h->sort_path == 0 && (h->n_cells > MZ_SORT_MAX_CELLS || (int64_t)h->n_cells * nchunks > 8 * s.cap));
+}
+
+struct RsScratch {
+    int32_t *kA, *kB, *pA, *pB, *mat, *tot, *counts;
+};
+
+static bool rs_scratch(mz_store *h, const MzSpecies &s, RsScratch &r) {
+    const int64_t cap = s.cap;
+    const int64_t nchunks = (cap + MZ_RS_CHUNK - 1) / MZ_RS_CHUNK;
+    r.kA = (int32_t *)mz_scratch(h, "rs_kA", (size_t)cap * sizeof(int32_t));
+    r.kB = (int32_t *)mz_scratch(h, "rs_kB", (size_t)cap * sizeof(int32_t));
+    r.pA = (int32_t *)mz_scratch(h, "rs_pA", (size_t)cap * sizeof(int32_t));
+    r.pB = (int32_t *)mz_scratch(h, "rs_pB", (size_t)cap * sizeof(int32_t));
+    r.mat = (int32_t *)mz_scratch(h, "rs_mat", (size_t)256 * nchunks * sizeof(int32_t));
+    r.tot = (int32_t *)mz_scratch(h, "rs_tot", 256 * sizeof(int32_t));
+    r.counts = (int32_t *)mz_scratch(h, "rs_counts", (size_t)h->n_cells * sizeof(int32_t));
+    return r.kA && r.kB && r.pA && r.pB && r.mat && r.tot && r.counts;
+}
+
+// allocates the scratch the next sort of this species uses (before a graph
+// capture, which must not allocate)
+int mz_sort_prealloc(mz_store *h, int species) {
+    const MzSpecies &s = h->sp[species - 1];
+    if (sort_uses_radix(h, s)) {
+        RsScratch r;
+        MZ_CHECK(rs_scratch(h, s, r), "sort: scratch allocation failed");
+        return 0;
+    }
+    const int64_t nchunks = (s.cap + MZ_SORT_CHUNK - 1) / MZ_SORT_CHUNK;
+    MZ_CHECK(mz_scratch(h, "sort_hist", (size_t)h->n_cells * nchunks * sizeof(int32_t)) != nullptr &&
+                 mz_scratch(h, "sort_cstart", (size_t)h->n_cells * sizeof(int32_t)) != nullptr,
+             "sort: scratch allocation failed");
+    return 0;
+}
+
+static int launch_radix_sort(mz_store *h, MzSpecies &s, MzSpView a, MzSpView b, int compute_cells) {
+    const int nc = h->n_cells;
+    const int64_t cap = s.cap;
+    const int32_t nchunks = (int32_t)((cap + MZ_RS_CHUNK - 1) / MZ_RS_CHUNK);
+    int bits = 1;
+    while (bits < 31 && ((int64_t)1 << bits) < (int64_t)nc) ++bits;
+    const int passes = (bits + 7) / 8;
+    RsScratch rs;
+    MZ_CHECK(rs_scratch(h, s, rs), "sort: scratch allocation failed");
+    int32_t *kA = rs.kA, *kB = rs.kB, *pA = rs.pA, *pB = rs.pB, *mat = rs.mat, *tot = rs.tot, *counts = rs.counts;
+    const int gblocks = (int)std::min<int64_t>((cap + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_rs_zero, dim3((nc + 255) / 256), dim3(256), 0, h->stream, counts, (int64_t)nc);
+    hipLaunchKernelGGL(k_rs_keys, dim3(gblocks), dim3(256), 0, h->stream, a, h->grid, compute_cells, nc, kA, counts);
+    int32_t *ks = kA, *kd = kB, *ps = nullptr, *pd = pA;
+    for (int q = 0; q < passes; ++q) {
+        const int shift = 8 * q;
+        const bool last = q == passes - 1;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nchunks), dim3(256), 0, h->stream, a, ks, shift, nchunks, mat);
+        hipLaunchKernelGGL(k_rs_rows, dim3(256), dim3(256), 0, h->stream, nchunks, mat, tot);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nchunks), dim3(256), 0, h->stream, a, ks, ps, last ? nullptr : kd, pd,
+                           shift, nchunks, mat, tot);
+        ps = pd;
+        pd = pd == pA ? pB : pA;
+        std::swap(ks, kd);
+    }
+    hipLaunchKernelGGL(k_rs_gather, dim3(gblocks), dim3(256), 0, h->stream, a, b, ps);
+    hipLaunchKernelGGL(k_rs_pia, dim3(1), dim3(1024), 0, h->stream, a, nc, counts);
+    MZ_HIP(hipGetLastError());
+    return 0;
+}
+
 MzViews mz_views(mz_store *h) {
     MzViews v{};
     for (int s = 0; s < h->n_species; ++s) v.v[s] = mz_view(h->sp[s]);
@@ -699,9 +985,11 @@ static MzSpView view_b(const MzSpecies &s) {
 int mz_launch_sort(mz_store *h, int species, int compute_cells) {
     MzSpecies &s = h->sp[species - 1];
     int nc = h->n_cells;
-    MZ_CHECK(nc <= MZ_SORT_MAX_CELLS, "sort: too many cells for the LDS histogram");
     MZ_CHECK(!compute_cells || h->has_grid, "sort: no grid set");
     int32_t nchunks = (int32_t)((s.cap + MZ_SORT_CHUNK - 1) / MZ_SORT_CHUNK);
+    const bool radix = sort_uses_radix(h, s);
+    MZ_CHECK(radix || nc <= MZ_SORT_MAX_CELLS, "sort: too many cells for the LDS histogram");
+    if (radix) return launch_radix_sort(h, s, mz_view(s), view_b(s), compute_cells);
     int32_t *hist = (int32_t *)mz_scratch(h, "sort_hist", (size_t)nc * nchunks * sizeof(int32_t));
     int32_t *cstart = (int32_t *)mz_scratch(h, "sort_cstart", (size_t)nc * sizeof(int32_t));
     MZ_CHECK(hist != nullptr && cstart != nullptr, "sort: scratch allocation failed");
@@ -713,6 +1001,13 @@ int mz_launch_sort(mz_store *h, int species, int compute_cells) {
                        h->d_ticket);
     hipLaunchKernelGGL(k_sort_scatter, dim3(nchunks), dim3(256), lds, h->stream, a, b, nc, nchunks, hist, cstart);
     MZ_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int mz_set_sort_path(mz_store *h, int32_t path) {
+    MZ_CHECK(path >= 0 && path <= 2, "set_sort_path: path must be 0 (automatic), 1 (count matrix) or 2 (radix)");
+    h->sort_path = path;
+    mz_graph_release(h);
     return 0;
 }
 
