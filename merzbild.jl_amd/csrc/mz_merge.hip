@@ -90,7 +90,7 @@ __device__ void oct_bounds(const OctArgs &A, const int16_t *pis, int bs, int be,
 
 __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     __shared__ int16_t pis[OCT_MAX_N];
-    __shared__ int16_t psort[OCT_MAX_N];
+    __shared__ __align__(16) int16_t psort[OCT_MAX_N];   // after the splits: per-bin slots and draws
     __shared__ uint8_t poct[OCT_MAX_N];
     __shared__ OctBin bins[OCT_MAX_BINS];
     __shared__ double red[4];
@@ -249,9 +249,18 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
         // per-octant weights in the reference's accumulation order, i.e. parent
         // order, which is each child's segment read backwards (one lane per octant)
         if (threadIdx.x >= 1 && threadIdx.x <= 8) {
-            const int o = threadIdx.x;
+            const int o = threadIdx.x, lo = start8[o];
             double s = 0.0;
-            for (int i = start8[o] + cnt8[o] - 1; i >= start8[o]; --i) s += A.sp.w[oct_slot(pis[i], s1, n1, s2)];
+            // eight loads in flight, then the adds in the same order
+            for (int i = start8[o] + cnt8[o] - 1; i >= lo; i -= 8) {
+                const int m = min(8, i - lo + 1);
+                double wb[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) wb[j] = j < m ? A.sp.w[oct_slot(pis[i - j], s1, n1, s2)] : 0.0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (j < m) s += wb[j];
+            }
             ndens8[o] = s;
         }
         __syncthreads();
@@ -302,68 +311,153 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     }
     __syncthreads();
 
-    // compute_bin_props! (:646-700) and compute_new_particles! in bin order:
-    // block reductions give each bin's means and spreads, thread 0 draws the
-    // signs and writes the new particles to the second buffer at the cell's
-    // slots (the originals are still needed for later bins).
-    __shared__ int32_t cur_s;
-    if (threadIdx.x == 0) cur_s = 0;
-    __syncthreads();
-    MzStream st = mz_stream((uint64_t)c);
-    for (int b = 0; b < Nbins; ++b) {
+    // compute_bin_props! (:646-700) and compute_new_particles! (:736-813,
+    // 830-933).  The reference walks the bins in order; here a scan over the
+    // bins gives each its output slot and its place in the cell's draw
+    // sequence (six sign draws per bin of more than two particles), and the
+    // four waves take the bins in turn.  A bin's sums are those of a 256-wide
+    // block reduction over its particles (lane l holds the block threads l,
+    // l + 64, l + 128, l + 192), so the values do not depend on which wave
+    // runs the bin.
+    int32_t *bslot = reinterpret_cast<int32_t *>(psort);
+    int32_t *bdraw = bslot + OCT_MAX_BINS;
+    __shared__ int32_t wsc[4];
+    __shared__ int32_t n_new_s;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    {
+        int32_t oc[2], dr[2];
+        for (int j = 0; j < 2; ++j) {
+            const int b = 2 * threadIdx.x + j;
+            const int32_t bnp = b < Nbins ? (bins[b].w == 0.0 ? 0 : bins[b].np) : 0;
+            oc[j] = bnp <= 0 ? 0 : (bnp <= 2 ? bnp : 2);
+            dr[j] = bnp > 2 ? 1 : 0;
+        }
+        const int32_t v = (oc[0] + oc[1]) | ((dr[0] + dr[1]) << 16);
+        int32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsc[wv] = x;
+        __syncthreads();
+        int32_t pre = 0;
+        for (int k = 0; k < wv; ++k) pre += wsc[k];
+        const int32_t ex = pre + x - v;
+        const int b = 2 * threadIdx.x;
+        if (b < OCT_MAX_BINS) {
+            bslot[b] = ex & 0xFFFF;
+            bdraw[b] = ex >> 16;
+            if (b + 1 < OCT_MAX_BINS) {
+                bslot[b + 1] = (ex & 0xFFFF) + oc[0];
+                bdraw[b + 1] = (ex >> 16) + dr[0];
+            }
+        }
+        if (threadIdx.x == 255) n_new_s = (pre + x) & 0xFFFF;
+        __syncthreads();
+    }
+    for (int b = wv; b < Nbins; b += 4) {
         const int32_t bnp = bins[b].w == 0.0 ? 0 : bins[b].np;
         if (bnp <= 0) continue;
         const int bs = bins[b].start, be = bins[b].end;
+        const int slot = bslot[b];
         if (bnp <= 2) {
-            if (threadIdx.x == 0) {
-                for (int k = 0; k < bnp; ++k) {
-                    int64_t src = oct_slot(pis[bs + k], s1, n1, s2);
-                    int64_t dst = oct_slot(cur_s + k, s1, n1, s2);
-                    A.tmp.x0[dst] = A.sp.x0[src]; A.tmp.x1[dst] = A.sp.x1[src]; A.tmp.x2[dst] = A.sp.x2[src];
-                    A.tmp.v0[dst] = A.sp.v0[src]; A.tmp.v1[dst] = A.sp.v1[src]; A.tmp.v2[dst] = A.sp.v2[src];
-                    A.tmp.w[dst] = A.sp.w[src];
-                }
-                cur_s += bnp;
+            if (lane < bnp) {
+                const int64_t src = oct_slot(pis[bs + lane], s1, n1, s2);
+                const int64_t dst = oct_slot(slot + lane, s1, n1, s2);
+                A.tmp.x0[dst] = A.sp.x0[src]; A.tmp.x1[dst] = A.sp.x1[src]; A.tmp.x2[dst] = A.sp.x2[src];
+                A.tmp.v0[dst] = A.sp.v0[src]; A.tmp.v1[dst] = A.sp.v1[src]; A.tmp.v2[dst] = A.sp.v2[src];
+                A.tmp.w[dst] = A.sp.w[src];
             }
-            __syncthreads();
             continue;
         }
         const double W = bins[b].w, inv_w = 1.0 / W;
-        double sv[3] = {0, 0, 0}, sx[3] = {0, 0, 0};
-        for (int i = bs + threadIdx.x; i <= be; i += blockDim.x) {
-            int64_t p = oct_slot(pis[i], s1, n1, s2);
-            double w = A.sp.w[p];
-            sv[0] = fma(w, A.sp.v0[p], sv[0]); sv[1] = fma(w, A.sp.v1[p], sv[1]); sv[2] = fma(w, A.sp.v2[p], sv[2]);
-            sx[0] = fma(w, A.sp.x0[p], sx[0]); sx[1] = fma(w, A.sp.x1[p], sx[1]); sx[2] = fma(w, A.sp.x2[p], sx[2]);
+        double vm[3], xm[3], sdv[3], sdx[3];
+        if (bnp <= 64) {
+            // one particle per lane: the block reduction's other waves add zeros
+            double w = 0.0, pv0 = 0.0, pv1 = 0.0, pv2 = 0.0, px0 = 0.0, px1 = 0.0, px2 = 0.0;
+            if (lane < bnp) {
+                const int64_t p = oct_slot(pis[bs + lane], s1, n1, s2);
+                w = A.sp.w[p];
+                pv0 = A.sp.v0[p]; pv1 = A.sp.v1[p]; pv2 = A.sp.v2[p];
+                px0 = A.sp.x0[p]; px1 = A.sp.x1[p]; px2 = A.sp.x2[p];
+            }
+            const bool on = lane < bnp;
+            vm[0] = mz_wave_sum(on ? fma(w, pv0, 0.0) : 0.0) * inv_w;
+            vm[1] = mz_wave_sum(on ? fma(w, pv1, 0.0) : 0.0) * inv_w;
+            vm[2] = mz_wave_sum(on ? fma(w, pv2, 0.0) : 0.0) * inv_w;
+            xm[0] = mz_wave_sum(on ? fma(w, px0, 0.0) : 0.0) * inv_w;
+            xm[1] = mz_wave_sum(on ? fma(w, px1, 0.0) : 0.0) * inv_w;
+            xm[2] = mz_wave_sum(on ? fma(w, px2, 0.0) : 0.0) * inv_w;
+            const double a0 = pv0 - vm[0], a1 = pv1 - vm[1], a2 = pv2 - vm[2];
+            const double b0 = px0 - xm[0], b1 = px1 - xm[1], b2 = px2 - xm[2];
+            sdv[0] = sqrt(mz_wave_sum(on ? fma(w, a0 * a0, 0.0) : 0.0) * inv_w);
+            sdv[1] = sqrt(mz_wave_sum(on ? fma(w, a1 * a1, 0.0) : 0.0) * inv_w);
+            sdv[2] = sqrt(mz_wave_sum(on ? fma(w, a2 * a2, 0.0) : 0.0) * inv_w);
+            sdx[0] = sqrt(mz_wave_sum(on ? fma(w, b0 * b0, 0.0) : 0.0) * inv_w);
+            sdx[1] = sqrt(mz_wave_sum(on ? fma(w, b1 * b1, 0.0) : 0.0) * inv_w);
+            sdx[2] = sqrt(mz_wave_sum(on ? fma(w, b2 * b2, 0.0) : 0.0) * inv_w);
+        } else {
+            // lane l stands for the block threads l + 64 q, q = 0..3
+            double sv[4][3], sx[4][3];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) sv[q][d] = sx[q][d] = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                for (int i = bs + lane + 64 * q; i <= be; i += 256) {
+                    const int64_t p = oct_slot(pis[i], s1, n1, s2);
+                    const double w = A.sp.w[p];
+                    sv[q][0] = fma(w, A.sp.v0[p], sv[q][0]); sv[q][1] = fma(w, A.sp.v1[p], sv[q][1]);
+                    sv[q][2] = fma(w, A.sp.v2[p], sv[q][2]);
+                    sx[q][0] = fma(w, A.sp.x0[p], sx[q][0]); sx[q][1] = fma(w, A.sp.x1[p], sx[q][1]);
+                    sx[q][2] = fma(w, A.sp.x2[p], sx[q][2]);
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                vm[d] = ((mz_wave_sum(sv[0][d]) + mz_wave_sum(sv[1][d])) +
+                         (mz_wave_sum(sv[2][d]) + mz_wave_sum(sv[3][d]))) * inv_w;
+                xm[d] = ((mz_wave_sum(sx[0][d]) + mz_wave_sum(sx[1][d])) +
+                         (mz_wave_sum(sx[2][d]) + mz_wave_sum(sx[3][d]))) * inv_w;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) sv[q][d] = sx[q][d] = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                for (int i = bs + lane + 64 * q; i <= be; i += 256) {
+                    const int64_t p = oct_slot(pis[i], s1, n1, s2);
+                    const double w = A.sp.w[p];
+                    const double a0 = A.sp.v0[p] - vm[0], a1 = A.sp.v1[p] - vm[1], a2 = A.sp.v2[p] - vm[2];
+                    const double b0 = A.sp.x0[p] - xm[0], b1 = A.sp.x1[p] - xm[1], b2 = A.sp.x2[p] - xm[2];
+                    sv[q][0] = fma(w, a0 * a0, sv[q][0]); sv[q][1] = fma(w, a1 * a1, sv[q][1]);
+                    sv[q][2] = fma(w, a2 * a2, sv[q][2]);
+                    sx[q][0] = fma(w, b0 * b0, sx[q][0]); sx[q][1] = fma(w, b1 * b1, sx[q][1]);
+                    sx[q][2] = fma(w, b2 * b2, sx[q][2]);
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                sdv[d] = sqrt(((mz_wave_sum(sv[0][d]) + mz_wave_sum(sv[1][d])) +
+                               (mz_wave_sum(sv[2][d]) + mz_wave_sum(sv[3][d]))) * inv_w);
+                sdx[d] = sqrt(((mz_wave_sum(sx[0][d]) + mz_wave_sum(sx[1][d])) +
+                               (mz_wave_sum(sx[2][d]) + mz_wave_sum(sx[3][d]))) * inv_w);
+            }
         }
-        double vm[3], xm[3];
-        for (int d = 0; d < 3; ++d) {
-            vm[d] = blk_sum(sv[d], red) * inv_w;
-            xm[d] = blk_sum(sx[d], red) * inv_w;
-        }
-        double qv[3] = {0, 0, 0}, qx[3] = {0, 0, 0};
-        for (int i = bs + threadIdx.x; i <= be; i += blockDim.x) {
-            int64_t p = oct_slot(pis[i], s1, n1, s2);
-            double w = A.sp.w[p];
-            double a0 = A.sp.v0[p] - vm[0], a1 = A.sp.v1[p] - vm[1], a2 = A.sp.v2[p] - vm[2];
-            double b0 = A.sp.x0[p] - xm[0], b1 = A.sp.x1[p] - xm[1], b2 = A.sp.x2[p] - xm[2];
-            qv[0] = fma(w, a0 * a0, qv[0]); qv[1] = fma(w, a1 * a1, qv[1]); qv[2] = fma(w, a2 * a2, qv[2]);
-            qx[0] = fma(w, b0 * b0, qx[0]); qx[1] = fma(w, b1 * b1, qx[1]); qx[2] = fma(w, b2 * b2, qx[2]);
-        }
-        double sdv[3], sdx[3];
-        for (int d = 0; d < 3; ++d) {
-            sdv[d] = sqrt(blk_sum(qv[d], red) * inv_w);
-            sdx[d] = sqrt(blk_sum(qx[d], red) * inv_w);
-        }
-        if (threadIdx.x == 0) {
-            // compute_new_particles! (:736-813, 830-933): 3 + 3 sign draws
+        if (lane == 0) {
+            // 3 + 3 sign draws, at this bin's place in the cell's sequence
+            MzStream st = mz_stream((uint64_t)c);
+            st.k = 6 * (uint64_t)bdraw[b];
             double dv[3], dx[3];
             for (int d = 0; d < 3; ++d) dv[d] = mz_rand(A.rp, st) < 0.5 ? -1.0 : 1.0;
             for (int d = 0; d < 3; ++d) dx[d] = mz_rand(A.rp, st) < 0.5 ? -1.0 : 1.0;
             const double w1 = 0.5 * W;
             for (int k = 0; k < 2; ++k) {
                 const double sg = k == 0 ? 1.0 : -1.0;
-                int64_t dst = oct_slot(cur_s + k, s1, n1, s2);
+                const int64_t dst = oct_slot(slot + k, s1, n1, s2);
                 double x0 = xm[0] + sg * dx[0] * sdx[0];
                 if (A.grid_clamp) x0 = x0 < A.min_x ? A.min_x : (x0 > A.max_x ? A.max_x : x0);
                 A.tmp.x0[dst] = x0;
@@ -374,11 +468,10 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                 A.tmp.v2[dst] = vm[2] + sg * dv[2] * sdv[2];
                 A.tmp.w[dst] = w1;
             }
-            cur_s += 2;
         }
-        __syncthreads();
     }
-    const int n_new = cur_s;
+    __syncthreads();
+    const int n_new = n_new_s;
     // move the new particles into place, zero the weights of deleted positions
     for (int li = threadIdx.x; li < N; li += blockDim.x) {
         int64_t p = oct_slot(li, s1, n1, s2);
