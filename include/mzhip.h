@@ -207,6 +207,23 @@ int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, const int32_t
 int mz_merge_roulette(mz_store *h, int32_t species, int32_t threshold, int32_t target_np, int32_t conservative,
                       uint64_t tag, int32_t rng_mode, int64_t *n_merged);
 
+/* merge_grid_based!(rng, merging_grid, pv, pia, cell, species, species_data,
+ * phys_props[, grid]) and (..., vx_extent, vy_extent, vz_extent[, grid])
+ * (src/merging/merging_grid.jl:597-703) on every cell with n_local >
+ * threshold, for GridN2Merge(Nx, Ny, Nz, extent_multiplier) (:72-115,
+ * Nx Ny Nz + 8 <= 4104, at most 2,048 particles per merged cell).  With
+ * extents == NULL the velocity grid spans v -/+ extent_multiplier[d] *
+ * sqrt(2 k T / mass) from the cell's T and v of the last mz_compute_props of
+ * this species (compute_velocity_extent!, :190-197); otherwise extents =
+ * {vx_lo, vx_hi, vy_lo, vy_hi, vz_lo, vz_hi} (:210-220).  grid_clamp = 1
+ * puts x of the merged pairs back into the grid (:484-595).  Six sign draws
+ * per velocity cell of more than two particles, from stream cell - 1 of the
+ * merge family.  Deleted ids are pushed as by merge_octree; squash = 1 then
+ * runs squash_pia!.  *n_merged (may be NULL) receives the cells merged. */
+int mz_merge_grid(mz_store *h, int32_t species, int32_t threshold, int32_t Nx, int32_t Ny, int32_t Nz,
+                  const double *extent_multiplier, const double *extents, double mass, int32_t grid_clamp,
+                  int32_t squash, uint64_t tag, int32_t rng_mode, int64_t *n_merged);
+
 /* merge_nnls_based! with a bounded candidate set: the columns are the first
  * max_pcols particles of the cell (logical order) plus the reference's
  * fictitious particles, while the moments to match are still those of all

@@ -45,6 +45,8 @@ _SIGS = {
     "mz_set_seed": (C.c_int, [C.c_void_p, u64]),
     "mz_set_replay": (C.c_int, [C.c_void_p, i32, P(f64), i64, i64]),
     "mz_sort": (C.c_int, [C.c_void_p, i32, i32]),
+    "mz_merge_grid": (C.c_int, [C.c_void_p, i32, i32, i32, i32, i32, P(f64), P(f64), f64, i32, i32, u64, i32,
+                                 P(i64)]),
     "mz_set_sort_path": (C.c_int, [C.c_void_p, i32]),
     "mz_squash": (C.c_int, [C.c_void_p, i32]),
     "mz_convect": (C.c_int, [C.c_void_p, i32, f64, u64, i32]),
@@ -678,6 +680,41 @@ def merge_octree_N2_based(store, species, threshold, target_np, split=OCTREE_MID
     _check(load_library().mz_merge_octree(store.h, species, threshold, target_np, split, init_bin_bounds,
                                           bin_bounds_compute, max_Nbins, max_depth, 1 if grid_clamp else 0,
                                           1 if squash else 0, tag, rng_mode, C.byref(n)))
+    return n.value
+
+
+class GridN2Merge:
+    """GridN2Merge(Nx, Ny, Nz, extent_multiplier) (src/merging/merging_grid.jl:72-115)."""
+
+    def __init__(self, Nx, Ny=None, Nz=None, extent_multiplier=3.5):
+        Ny = Nx if Ny is None else Ny
+        Nz = Nx if Nz is None else Nz
+        self.Nx, self.Ny, self.Nz = int(Nx), int(Ny), int(Nz)
+        m = np.broadcast_to(np.asarray(extent_multiplier, dtype=np.float64), (3,))
+        self.extent_multiplier = np.ascontiguousarray(m)
+        self.Ntotal = self.Nx * self.Ny * self.Nz + 8
+
+
+def merge_grid_based(store, species, threshold, merging_grid, species_data=None, extents=None, grid_clamp=False,
+                     squash=True, tag=0, rng_mode=RNG_PHILOX):
+    """merge_grid_based! (src/merging/merging_grid.jl:597-703) on every cell
+    with n_local > threshold.  extents = (vx_extent, vy_extent, vz_extent)
+    gives the velocity grid's bounds; without it the bounds come from each
+    cell's T and v of the last compute_props call on the store (species_data
+    gives the mass).  Returns the number of cells merged."""
+    mg = merging_grid
+    n = i64(0)
+    if extents is not None:
+        ext = np.ascontiguousarray(np.asarray(extents, dtype=np.float64).reshape(6))
+        mass, mult = 0.0, None
+    else:
+        if species_data is None:
+            raise ValueError("merge_grid_based: extents from the props need species_data")
+        ext, mult, mass = None, mg.extent_multiplier, float(species_data[species - 1].mass)
+    _check(load_library().mz_merge_grid(store.h, species, int(threshold), mg.Nx, mg.Ny, mg.Nz,
+                                        _d(mult) if mult is not None else None, _d(ext) if ext is not None else None,
+                                        mass, 1 if grid_clamp else 0, 1 if squash else 0, tag, rng_mode,
+                                        C.byref(n)))
     return n.value
 
 

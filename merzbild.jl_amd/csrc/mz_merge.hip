@@ -88,6 +88,45 @@ __device__ void oct_bounds(const OctArgs &A, const int16_t *pis, int bs, int be,
     }
 }
 
+// The end of an N:2 merge of cell c (compute_new_particles!,
+// merging_octree_N2.jl:736-813 and merging_grid.jl:393-482): the n_new
+// particles staged in tmp at the cell's first n_new logical slots replace
+// the cell's data, the weights of the rest are zeroed, and the PIA loses
+// N - n_new particles from the end, group 2 first (delete_particle_end!,
+// particles.jl:478-566); old_pia keeps what k_merge_push_ids needs.  The
+// reference clears contiguous unless the deletions stay inside the last
+// cell's group 2.  Called by every thread of the block after a barrier.
+__device__ void merge_finish(const MzSpView &sp, const MzSpView &tmp, int c, int N, int n_new, int32_t s1,
+                             int32_t n1, int32_t s2, int32_t n2, int32_t *old_pia, int32_t *n_merged,
+                             int32_t n_cells) {
+    for (int li = threadIdx.x; li < N; li += blockDim.x) {
+        int64_t p = oct_slot(li, s1, n1, s2);
+        if (li < n_new) {
+            sp.x0[p] = tmp.x0[p]; sp.x1[p] = tmp.x1[p]; sp.x2[p] = tmp.x2[p];
+            sp.v0[p] = tmp.v0[p]; sp.v1[p] = tmp.v1[p]; sp.v2[p] = tmp.v2[p];
+            sp.w[p] = tmp.w[p];
+        } else {
+            sp.w[p] = 0.0;
+        }
+    }
+    if (threadIdx.x == 0) {
+        int32_t *op = old_pia + c * 5;
+        op[0] = s1; op[1] = n1; op[2] = s2; op[3] = n2; op[4] = n_new;
+        int32_t n_del = N - n_new;
+        int32_t d2 = min(n_del, n2), d1 = n_del - d2;
+        int32_t nn2 = n2 - d2, nn1 = n1 - d1;
+        sp.pia.n2[c] = nn2;
+        sp.pia.end2[c] = nn2 > 0 ? s2 + nn2 - 1 : -1;
+        if (nn2 == 0) sp.pia.start2[c] = 0;
+        sp.pia.n1[c] = nn1;
+        sp.pia.end1[c] = nn1 > 0 ? s1 + nn1 - 1 : -1;
+        if (nn1 == 0) sp.pia.start1[c] = 0;
+        sp.pia.nloc[c] = nn1 + nn2;
+        if (!(c == n_cells - 1) || (n_del > n2)) sp.ctr->contiguous = 0;
+        atomicAdd(n_merged, 1);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     __shared__ int16_t pis[OCT_MAX_N];
     __shared__ __align__(16) int16_t psort[OCT_MAX_N];   // after the splits: per-bin slots and draws
@@ -471,37 +510,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
         }
     }
     __syncthreads();
-    const int n_new = n_new_s;
-    // move the new particles into place, zero the weights of deleted positions
-    for (int li = threadIdx.x; li < N; li += blockDim.x) {
-        int64_t p = oct_slot(li, s1, n1, s2);
-        if (li < n_new) {
-            A.sp.x0[p] = A.tmp.x0[p]; A.sp.x1[p] = A.tmp.x1[p]; A.sp.x2[p] = A.tmp.x2[p];
-            A.sp.v0[p] = A.tmp.v0[p]; A.sp.v1[p] = A.tmp.v1[p]; A.sp.v2[p] = A.tmp.v2[p];
-            A.sp.w[p] = A.tmp.w[p];
-        } else {
-            A.sp.w[p] = 0.0;
-        }
-    }
-    if (threadIdx.x == 0) {
-        int32_t *op = A.old_pia + c * 5;
-        op[0] = s1; op[1] = n1; op[2] = s2; op[3] = n2; op[4] = n_new;
-        // delete_particle_end! n_del times (particles.jl:478-566): group 2 first
-        int32_t n_del = N - n_new;
-        int32_t d2 = min(n_del, n2), d1 = n_del - d2;
-        int32_t nn2 = n2 - d2, nn1 = n1 - d1;
-        A.sp.pia.n2[c] = nn2;
-        A.sp.pia.end2[c] = nn2 > 0 ? s2 + nn2 - 1 : -1;
-        if (nn2 == 0) A.sp.pia.start2[c] = 0;
-        A.sp.pia.n1[c] = nn1;
-        A.sp.pia.end1[c] = nn1 > 0 ? s1 + nn1 - 1 : -1;
-        if (nn1 == 0) A.sp.pia.start1[c] = 0;
-        A.sp.pia.nloc[c] = nn1 + nn2;
-        // the reference clears contiguous unless the deletions stay inside the
-        // last cell's group 2 (merging_octree_N2.jl:801-805)
-        if (!(c == A.n_cells - 1) || (n_del > n2)) A.sp.ctr->contiguous = 0;
-        atomicAdd(A.n_merged, 1);
-    }
+    merge_finish(A.sp, A.tmp, c, N, n_new_s, s1, n1, s2, n2, A.old_pia, A.n_merged, A.n_cells);
 }
 
 // Free-id pushes in cell order: cell c pushes the ids of its deleted
@@ -610,6 +619,296 @@ extern "C" int mz_merge_octree(mz_store *h, int32_t species, int32_t threshold, 
     A.n_cells = nc;
     mz_timing_begin(h, "merge_octree");
     hipLaunchKernelGGL(k_octree_merge, dim3(nc), dim3(256), 0, h->stream, A);
+    hipLaunchKernelGGL(k_merge_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, old_pia, nc, ndel);
+    hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
+    hipLaunchKernelGGL(k_merge_push_ids, dim3(nc), dim3(64), 0, h->stream, A.sp, old_pia, off, off + nc,
+                       snap.nbuffer, snap.n_total, nc);
+    mz_timing_end(h);
+    MZ_HIP(hipGetLastError());
+    if (n_merged) {
+        int32_t v = 0;
+        MZ_HIP(hipMemcpyAsync(&v, cnt, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+        MZ_HIP(hipStreamSynchronize(h->stream));
+        *n_merged = v;
+    }
+    if (squash) return mz_squash(h, species);
+    return 0;
+}
+
+// ===========================================================================
+// Grid N:2 merging (merging_grid.jl:190-703), one workgroup per cell above
+// the threshold.  The reference walks the cell's particles (group 1, then
+// group 2) accumulating into the velocity cells, then walks the velocity
+// cells in order.  Here the particles' velocity-cell indices are counted in
+// LDS, a stable ranking lists each velocity cell's particles in particle
+// order, and one thread per velocity cell forms its sums in that order
+// (x + v * w accumulations, then / w, unfused as the reference writes them),
+// so the values equal the oracle's bit for bit.  A scan over the velocity
+// cells gives each its output slots and its place in the cell's draw
+// sequence (six sign draws per velocity cell of more than two particles).
+
+#define GM_MAX_N 2048
+#define GM_MAX_CELLS 4104   // Nx Ny Nz + 8
+
+struct GridArgs {
+    MzSpView sp, tmp;
+    int32_t threshold, Nx, Ny, Nz, extent_mode, grid_clamp;
+    double mult[3], ext_lo[3], ext_hi[3];
+    const double *T, *v;      // extent_mode 0: this species' props, [cell] and [cell][3]
+    double mass, min_x, max_x;
+    MzRngParams rp;
+    int32_t *old_pia;
+    int32_t *n_merged;
+    int32_t n_cells;
+};
+
+__device__ inline int64_t gm_coord(double f, int32_t n) {
+    const double fl = floor(f);
+    return fl >= 0.0 ? (fl <= (double)n ? (int64_t)fl : (int64_t)n) : 0;
+}
+
+// block-wide exclusive scan of a[0..n) in place (contiguous chunks per
+// thread); returns the total to every thread
+__device__ int32_t gm_scan(int32_t *a, int n, int32_t *wsc) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int per = (n + 255) / 256;
+    const int lo = t * per, hi = min(n, lo + per);
+    int32_t loc = 0;
+    for (int i = lo; i < hi; ++i) loc += a[i];
+    int32_t x = loc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsc[wv] = x;
+    __syncthreads();
+    int32_t pre = 0;
+    for (int k = 0; k < wv; ++k) pre += wsc[k];
+    const int32_t total = wsc[0] + wsc[1] + wsc[2] + wsc[3];
+    int32_t run = pre + x - loc;
+    for (int i = lo; i < hi; ++i) {
+        const int32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+    __syncthreads();
+    return total;
+}
+
+__global__ void __launch_bounds__(256) k_grid_merge(GridArgs A) {
+    __shared__ int16_t gidx[GM_MAX_N];     // velocity cell (0-based) of logical particle li
+    __shared__ int16_t ord[GM_MAX_N];      // logical particles grouped by velocity cell, in particle order
+    __shared__ int32_t gstart[GM_MAX_CELLS + 1];
+    __shared__ int32_t gaux[GM_MAX_CELLS + 1];   // counts, then running slots, then packed outputs/draws
+    __shared__ int32_t wsc[4];
+    __shared__ int32_t n_new_s;
+    const int c = blockIdx.x;
+    const int32_t N = A.sp.pia.nloc[c];
+    if (N <= A.threshold) {
+        if (threadIdx.x == 0) A.old_pia[c * 5 + 4] = -1;
+        return;
+    }
+    if (N > GM_MAX_N) {
+        if (threadIdx.x == 0) {
+            A.old_pia[c * 5 + 4] = -1;
+            atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
+        }
+        return;
+    }
+    A.rp = mz_rng_resolve(A.rp);
+    const int32_t s1 = A.sp.pia.start1[c], n1 = A.sp.pia.n1[c], s2 = A.sp.pia.start2[c], n2 = A.sp.pia.n2[c];
+    const int Nx = A.Nx, Ny = A.Ny, Nz = A.Nz, NyNz = Ny * Nz, Nt = Nx * Ny * Nz + 8;
+    // compute_velocity_extent! (:190-220)
+    double lo[3], up[3], mid[3], dinv[3];
+    {
+        const int32_t Nd[3] = {Nx, Ny, Nz};
+        if (A.extent_mode == 0) {
+            const double vt = sqrt(2 * A.T[c] * MZ_KB / A.mass);
+            for (int d = 0; d < 3; ++d) {
+                const double dv = A.mult[d] * vt, vc = A.v[3 * c + d];
+                lo[d] = vc - dv;
+                up[d] = vc + dv;
+                mid[d] = vc;
+                dinv[d] = 1.0 / (2 * dv / (double)Nd[d]);
+            }
+        } else {
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = A.ext_lo[d];
+                up[d] = A.ext_hi[d];
+                mid[d] = 0.5 * (A.ext_lo[d] + A.ext_hi[d]);
+                dinv[d] = 1.0 / ((A.ext_hi[d] - A.ext_lo[d]) / (double)Nd[d]);
+            }
+        }
+    }
+    for (int g = threadIdx.x; g <= Nt; g += 256) gaux[g] = 0;
+    __syncthreads();
+    // compute_grid_index (:235-270) of every particle
+    for (int li = threadIdx.x; li < N; li += 256) {
+        const int64_t p = oct_slot(li, s1, n1, s2);
+        const double v0 = A.sp.v0[p], v1 = A.sp.v1[p], v2 = A.sp.v2[p];
+        const bool outside = v0 < lo[0] || v0 > up[0] || v1 < lo[1] || v1 > up[1] || v2 < lo[2] || v2 > up[2];
+        int64_t k;
+        if (!outside) {
+            k = gm_coord((v0 - lo[0]) * dinv[0], Nx) * NyNz + gm_coord((v1 - lo[1]) * dinv[1], Ny) * Nz +
+                gm_coord((v2 - lo[2]) * dinv[2], Nz);
+        } else {
+            k = Nt - 8 + (v0 > mid[0] ? 1 : 0) + (v1 > mid[1] ? 2 : 0) + (v2 > mid[2] ? 4 : 0);
+        }
+        if (k > Nt - 1) k = Nt - 1;
+        gidx[li] = (int16_t)k;
+        atomicAdd(&gaux[k], 1);
+    }
+    __syncthreads();
+    for (int g = threadIdx.x; g < Nt; g += 256) gstart[g] = gaux[g];
+    __syncthreads();
+    gm_scan(gstart, Nt, wsc);
+    if (threadIdx.x == 0) gstart[Nt] = N;
+    for (int g = threadIdx.x; g < Nt; g += 256) gaux[g] = gstart[g];
+    __syncthreads();
+    // stable ranking by velocity cell (wave 0, 64 particles at a time)
+    if (threadIdx.x < 64) {
+        volatile int32_t *vrun = gaux;
+        const int lane = threadIdx.x;
+        const uint64_t lt = mz_lanemask_lt();
+        for (int b0 = 0; b0 < N; b0 += 64) {
+            const int li = b0 + lane;
+            const bool valid = li < N;
+            const int32_t key = valid ? gidx[li] : -1;
+            uint64_t active = __ballot(valid);
+            while (active) {
+                const int leader = __ffsll((unsigned long long)active) - 1;
+                const int32_t k = __shfl(key, leader, 64);
+                const uint64_t m = __ballot(valid && key == k);
+                const int32_t bk = vrun[k];
+                if (valid && key == k) ord[bk + __popcll(m & lt)] = (int16_t)li;
+                vrun[k] = bk + __popcll(m);
+                active &= ~m;
+            }
+        }
+    }
+    __syncthreads();
+    // pass A: each velocity cell's weight in particle order; outputs
+    // min(np, 2) (none when the weight is not positive), draws 1 if np > 2
+    for (int g = threadIdx.x; g < Nt; g += 256) {
+        const int b = gstart[g], e = gstart[g + 1];
+        double w = 0.0;
+        for (int j = b; j < e; ++j) w += A.sp.w[oct_slot(ord[j], s1, n1, s2)];
+        const int np = w > 0.0 ? e - b : 0;
+        gaux[g] = (np >= 2 ? 2 : np) | ((np > 2 ? 1 : 0) << 16);
+    }
+    __syncthreads();
+    const int32_t tot = gm_scan(gaux, Nt, wsc);
+    if (threadIdx.x == 0) {
+        gaux[Nt] = tot;
+        n_new_s = tot & 0xFFFF;
+    }
+    __syncthreads();
+    // pass B: compute_grid!'s means and spreads (:304-384) and
+    // compute_new_particles! (:393-595), one thread per velocity cell
+    for (int g = threadIdx.x; g < Nt; g += 256) {
+        const int b = gstart[g], e = gstart[g + 1];
+        const int slot = gaux[g] & 0xFFFF, n_out = (gaux[g + 1] & 0xFFFF) - slot;
+        if (n_out == 0) continue;
+        const int np = e - b;
+        double ow[2], ov[2][3], ox[2][3];
+        if (np > 2) {
+            double w = 0.0, vm[3] = {0, 0, 0}, xm[3] = {0, 0, 0};
+            for (int j = b; j < e; ++j) {
+                const int64_t p = oct_slot(ord[j], s1, n1, s2);
+                const double pw = A.sp.w[p];
+                w += pw;
+                vm[0] = vm[0] + A.sp.v0[p] * pw; vm[1] = vm[1] + A.sp.v1[p] * pw; vm[2] = vm[2] + A.sp.v2[p] * pw;
+                xm[0] = xm[0] + A.sp.x0[p] * pw; xm[1] = xm[1] + A.sp.x1[p] * pw; xm[2] = xm[2] + A.sp.x2[p] * pw;
+            }
+            for (int d = 0; d < 3; ++d) { vm[d] = vm[d] / w; xm[d] = xm[d] / w; }
+            double vs[3] = {0, 0, 0}, xs[3] = {0, 0, 0};
+            for (int j = b; j < e; ++j) {
+                const int64_t p = oct_slot(ord[j], s1, n1, s2);
+                const double pw = A.sp.w[p];
+                const double a0 = A.sp.v0[p] - vm[0], a1 = A.sp.v1[p] - vm[1], a2 = A.sp.v2[p] - vm[2];
+                const double c0 = A.sp.x0[p] - xm[0], c1 = A.sp.x1[p] - xm[1], c2 = A.sp.x2[p] - xm[2];
+                vs[0] = vs[0] + (a0 * a0) * pw; vs[1] = vs[1] + (a1 * a1) * pw; vs[2] = vs[2] + (a2 * a2) * pw;
+                xs[0] = xs[0] + (c0 * c0) * pw; xs[1] = xs[1] + (c1 * c1) * pw; xs[2] = xs[2] + (c2 * c2) * pw;
+            }
+            for (int d = 0; d < 3; ++d) { vs[d] = sqrt(vs[d] / w); xs[d] = sqrt(xs[d] / w); }
+            MzStream st = mz_stream((uint64_t)c);
+            st.k = 6 * (uint64_t)(gaux[g] >> 16);
+            double dir[3];
+            for (int d = 0; d < 3; ++d) dir[d] = mz_rand(A.rp, st) < 0.5 ? -1.0 : 1.0;
+            for (int d = 0; d < 3; ++d) { ov[0][d] = vm[d] + dir[d] * vs[d]; ov[1][d] = vm[d] - dir[d] * vs[d]; }
+            for (int d = 0; d < 3; ++d) dir[d] = mz_rand(A.rp, st) < 0.5 ? -1.0 : 1.0;
+            for (int d = 0; d < 3; ++d) { ox[0][d] = xm[d] + dir[d] * xs[d]; ox[1][d] = xm[d] - dir[d] * xs[d]; }
+            ow[0] = 0.5 * w;
+            ow[1] = ow[0];
+        } else {
+            for (int h = 0; h < n_out; ++h) {
+                const int64_t p = oct_slot(ord[b + h], s1, n1, s2);
+                ow[h] = A.sp.w[p];
+                ov[h][0] = A.sp.v0[p]; ov[h][1] = A.sp.v1[p]; ov[h][2] = A.sp.v2[p];
+                ox[h][0] = A.sp.x0[p]; ox[h][1] = A.sp.x1[p]; ox[h][2] = A.sp.x2[p];
+            }
+        }
+        for (int h = 0; h < n_out; ++h) {
+            const int64_t dst = oct_slot(slot + h, s1, n1, s2);
+            double x0 = ox[h][0];
+            if (A.grid_clamp && n_out == 2) x0 = x0 < A.min_x ? A.min_x : (x0 > A.max_x ? A.max_x : x0);
+            A.tmp.x0[dst] = x0; A.tmp.x1[dst] = ox[h][1]; A.tmp.x2[dst] = ox[h][2];
+            A.tmp.v0[dst] = ov[h][0]; A.tmp.v1[dst] = ov[h][1]; A.tmp.v2[dst] = ov[h][2];
+            A.tmp.w[dst] = ow[h];
+        }
+    }
+    __syncthreads();
+    merge_finish(A.sp, A.tmp, c, N, n_new_s, s1, n1, s2, n2, A.old_pia, A.n_merged, A.n_cells);
+}
+
+extern "C" int mz_merge_grid(mz_store *h, int32_t species, int32_t threshold, int32_t Nx, int32_t Ny, int32_t Nz,
+                             const double *extent_multiplier, const double *extents, double mass,
+                             int32_t grid_clamp, int32_t squash, uint64_t tag, int32_t rng_mode,
+                             int64_t *n_merged) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "merge_grid: bad species");
+    MZ_CHECK(Nx >= 1 && Ny >= 1 && Nz >= 1 && (int64_t)Nx * Ny * Nz + 8 <= GM_MAX_CELLS,
+             "merge_grid: Nx Ny Nz must be positive with Nx Ny Nz + 8 <= 4104");
+    MZ_CHECK(extents != nullptr || extent_multiplier != nullptr,
+             "merge_grid: give the extent multipliers (extents from the props) or the explicit extents");
+    MZ_CHECK(!grid_clamp || h->has_grid, "merge_grid: grid clamp needs a grid");
+    MZ_CHECK(rng_mode == 0 || (rng_mode == 1 && h->d_replay[2]), "merge_grid: replay table not set");
+    MzSpecies &s = h->sp[species - 1];
+    const int nc = h->n_cells;
+    int32_t *old_pia = (int32_t *)mz_scratch(h, "oct_oldpia", (size_t)nc * 5 * sizeof(int32_t));
+    int32_t *cnt = (int32_t *)mz_scratch(h, "oct_cnt", sizeof(int32_t));
+    int32_t *ndel = (int32_t *)mz_scratch(h, "oct_ndel", (size_t)nc * sizeof(int32_t));
+    int64_t *off = (int64_t *)mz_scratch(h, "oct_off", (size_t)(nc + 1) * sizeof(int64_t));
+    MZ_CHECK(old_pia && cnt && ndel && off, "merge_grid: scratch allocation failed");
+    MzCounters snap;
+    int r = mz_read_counters(h, species, &snap);
+    if (r) return r;
+    MZ_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t), h->stream));
+    GridArgs A{};
+    A.sp = mz_view(s);
+    A.tmp = A.sp;
+    A.tmp.x0 = s.xb[0]; A.tmp.x1 = s.xb[1]; A.tmp.x2 = s.xb[2];
+    A.tmp.v0 = s.vb[0]; A.tmp.v1 = s.vb[1]; A.tmp.v2 = s.vb[2];
+    A.tmp.w = s.wb;
+    A.threshold = threshold; A.Nx = Nx; A.Ny = Ny; A.Nz = Nz; A.grid_clamp = grid_clamp;
+    if (extents) {   // [vx_lo, vx_hi, vy_lo, vy_hi, vz_lo, vz_hi]
+        A.extent_mode = 1;
+        for (int d = 0; d < 3; ++d) { A.ext_lo[d] = extents[2 * d]; A.ext_hi[d] = extents[2 * d + 1]; }
+    } else {
+        A.extent_mode = 0;
+        for (int d = 0; d < 3; ++d) A.mult[d] = extent_multiplier[d];
+        A.T = h->d_T + (size_t)(species - 1) * nc;
+        A.v = h->d_v + (size_t)(species - 1) * nc * 3;
+        MZ_CHECK(mass > 0.0, "merge_grid: the extents from the props need the species mass");
+        A.mass = mass;
+    }
+    A.min_x = h->grid.min_x; A.max_x = h->grid.max_x;
+    A.rp = mz_rng_params(h, 2, tag, rng_mode);
+    A.old_pia = old_pia;
+    A.n_merged = cnt;
+    A.n_cells = nc;
+    mz_timing_begin(h, "merge_grid");
+    hipLaunchKernelGGL(k_grid_merge, dim3(nc), dim3(256), 0, h->stream, A);
     hipLaunchKernelGGL(k_merge_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, old_pia, nc, ndel);
     hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
     hipLaunchKernelGGL(k_merge_push_ids, dim3(nc), dim3(64), 0, h->stream, A.sp, old_pia, off, off + nc,

@@ -422,17 +422,20 @@ extern "C" int mz_ntc(mz_store *h, int32_t s1, int32_t s2, double dt, const doub
 
 // Count pass: w_max of every cell, its candidate count (draw 0 of its
 // stream, compute_n_coll_single_species, collision_swpm.jl:171-174) and the
-// scratch it may need (two particles per candidate).
-__global__ void k_swpm_count(NtcArgs A, double G, int32_t *__restrict__ ncoll, int32_t *__restrict__ ssz,
-                             double *__restrict__ wmax) {
-    const int c = A.cell_lo + blockIdx.x * blockDim.x + threadIdx.x;
+// scratch it may need (two particles per candidate); one wave per cell.
+__global__ void __launch_bounds__(256) k_swpm_count(NtcArgs A, double G, int32_t *__restrict__ ncoll,
+                                                    int32_t *__restrict__ ssz, double *__restrict__ wmax) {
+    const int c = A.cell_lo + blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (c > A.cell_hi) return;
     const int32_t n1 = A.a.pia.n1[c], n2 = A.a.pia.n2[c] > 0 ? A.a.pia.n2[c] : 0, np = A.a.pia.nloc[c];
     double w_max = 0.0;
-    for (int32_t li = 0; li < n1 + n2; ++li) {
+    for (int32_t li = lane; li < n1 + n2; li += 64) {
         const double w = A.a.w[slot_of(A.a, c, li, n1, n2)];
         w_max = w > w_max ? w : w_max;
     }
+    w_max = mz_wave_max(w_max);
+    if (lane != 0) return;
     A.rp = mz_rng_resolve(A.rp);
     MzStream st = mz_stream((uint64_t)c);
     const double R = mz_rand(A.rp, st);
@@ -542,7 +545,7 @@ extern "C" int mz_swpm(mz_store *h, int32_t species, double G, double dt, const 
     MZ_HIP(hipMemsetAsync(ssz, 0, (size_t)nc * sizeof(int32_t), h->stream));
     MZ_HIP(hipMemsetAsync(created, 0, (size_t)2 * nc * sizeof(int32_t), h->stream));
     mz_timing_begin(h, "swpm");
-    hipLaunchKernelGGL(k_swpm_count, dim3((ncells + 255) / 256), dim3(256), 0, h->stream, A, G, ncoll, ssz, wmax);
+    hipLaunchKernelGGL(k_swpm_count, dim3((ncells + 3) / 4), dim3(256), 0, h->stream, A, G, ncoll, ssz, wmax);
     hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, h->stream, ssz, (int64_t)nc, soff, soff + nc);
     int64_t total = 0;
     MZ_HIP(hipMemcpyAsync(&total, soff + nc, sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));

@@ -169,6 +169,7 @@ extern "C" int mz_create(int32_t n_cells, int32_t n_species, const int64_t *capa
     MZ_HIP(hipMalloc(&h->d_V, n_cells * sizeof(double)));
     MZ_HIP(hipMalloc(&h->d_invV, n_cells * sizeof(double)));
     MZ_HIP(hipMalloc(&h->d_masses, MZ_MAX_SPECIES * sizeof(double)));
+    MZ_HIP(hipMemset(h->d_masses, 0, MZ_MAX_SPECIES * sizeof(double)));
     const size_t surf_bytes = (size_t)2 * n_species * 2 * MZ_SURF_Q * sizeof(double);
     MZ_HIP(hipMalloc(&h->d_surf, surf_bytes));
     MZ_HIP(hipMemset(h->d_surf, 0, surf_bytes));

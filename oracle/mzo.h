@@ -264,6 +264,36 @@ void mzo_octree_compute(mzo_octree *oc, mzo_pv *pv, int64_t target_np);
 void mzo_merge_octree_N2_based(mzo_rng *rng, mzo_octree *oc, mzo_pv *pv, mzo_pia *pia, int64_t cell,
                                int64_t species, int64_t target_np, const mzo_grid1d *grid);
 
+/* ---------------- grid N:2 merging (src/merging/merging_grid.jl) ----------------
+ * GridN2Merge (:72-115): Nx x Ny x Nz velocity cells plus the 8 octants
+ * outside the grid, indexed 1..Ntotal as compute_grid_index (:235-270) does.
+ * Parity: pinned by test/test_merging_grid_merging.jl's answers (extents,
+ * conservation, zero-weight octants, two index groups) and closed forms
+ * (tests/test_oracle_grid_merge.py). */
+typedef struct {
+    int64_t np;
+    double w, v_mean[3], v_std_sq[3], x_mean[3], x_std_sq[3];
+    int64_t particle_index1, particle_index2;
+} mzo_gcell;
+
+typedef struct {
+    int64_t Nx, Ny, Nz, NyNz, Ntotal;
+    double mult[3], lower[3], upper[3], mid[3], dv[3], dv_inv[3];
+    mzo_gcell *cells;   /* [Ntotal], velocity cell index k at cells[k - 1] */
+    int64_t out_of_range;   /* indices past Ntotal met (a velocity on an upper bound), clamped */
+} mzo_gridmerge;
+
+mzo_gridmerge *mzo_gridmerge_new(int64_t Nx, int64_t Ny, int64_t Nz, const double mult[3]);
+void mzo_gridmerge_free(mzo_gridmerge *mg);
+/* compute_velocity_extent! from a cell's T and mean velocity (:190-197) or from explicit bounds (:210-220) */
+void mzo_gridmerge_extent_props(mzo_gridmerge *mg, double T, const double v[3], double mass);
+void mzo_gridmerge_extent(mzo_gridmerge *mg, const double vx[2], const double vy[2], const double vz[2]);
+int64_t mzo_gridmerge_index(mzo_gridmerge *mg, const double v[3]);
+/* compute_grid! + compute_new_particles! (:304-595) on the extents set
+ * above; grid may be NULL (no clamping).  Draws from stream cell - 1. */
+void mzo_merge_grid_based(mzo_rng *rng, mzo_gridmerge *mg, mzo_pv *pv, mzo_pia *pia, int64_t cell, int64_t species,
+                          const mzo_grid1d *grid);
+
 /* merge_roulette!(rng, pv, pia, cell, species, target_np; conservative)
  * (src/merging/merging_roulette.jl:25-184): delete uniformly drawn particles
  * (one draw each, from the cell's stream) until target_np remain, then

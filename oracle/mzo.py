@@ -150,6 +150,12 @@ def lib():
             "mzo_delete_particle": (None, [P(PvS), P(PiaS), i64, i64, i64]),
             "mzo_swpm": (None, [P(Rng), P(CFSwpm), P(Interaction), P(PvS), P(PiaS), i64, i64, f64, f64, f64]),
             "mzo_merge_roulette": (None, [P(Rng), P(PvS), P(PiaS), i64, i64, i64, C.c_int]),
+            "mzo_gridmerge_new": (P(GridMergeS), [i64, i64, i64, P(f64)]),
+            "mzo_gridmerge_free": (None, [P(GridMergeS)]),
+            "mzo_gridmerge_extent_props": (None, [P(GridMergeS), f64, P(f64), f64]),
+            "mzo_gridmerge_extent": (None, [P(GridMergeS), P(f64), P(f64), P(f64)]),
+            "mzo_gridmerge_index": (i64, [P(GridMergeS), P(f64)]),
+            "mzo_merge_grid_based": (None, [P(Rng), P(GridMergeS), P(PvS), P(PiaS), i64, i64, P(Grid1D)]),
             "mzo_surf_update_incident": (None, [P(SurfS), f64, P(f64), i64, i64]),
             "mzo_surf_update_reflected": (None, [P(SurfS), f64, P(f64), i64, i64]),
             "mzo_surf_scale": (None, [P(SurfS), i64, f64, f64]),
@@ -461,6 +467,48 @@ def compute_props_sorted(pvs, pia, masses, props, inv_V=None, cells=None):
 def compute_props_with_total_moments(pvs, pia, masses, props):
     m = np.ascontiguousarray(masses, dtype=np.float64)
     lib().mzo_compute_props_with_total_moments(pv_array(pvs), pia.p, _dptr(m), props.ref)
+
+
+class GCell(C.Structure):
+    _fields_ = [("np", i64), ("w", f64), ("v_mean", f64 * 3), ("v_std_sq", f64 * 3), ("x_mean", f64 * 3),
+                ("x_std_sq", f64 * 3), ("particle_index1", i64), ("particle_index2", i64)]
+
+
+class GridMergeS(C.Structure):
+    _fields_ = [("Nx", i64), ("Ny", i64), ("Nz", i64), ("NyNz", i64), ("Ntotal", i64), ("mult", f64 * 3),
+                ("lower", f64 * 3), ("upper", f64 * 3), ("mid", f64 * 3), ("dv", f64 * 3), ("dv_inv", f64 * 3),
+                ("cells", P(GCell)), ("out_of_range", i64)]
+
+
+class GridMerge:
+    """GridN2Merge (src/merging/merging_grid.jl:72-115)."""
+
+    def __init__(self, Nx, Ny, Nz, mult):
+        m = (C.c_double * 3)(*([float(mult)] * 3 if np.isscalar(mult) else [float(x) for x in mult]))
+        self.p = lib().mzo_gridmerge_new(Nx, Ny, Nz, m)
+
+    def __del__(self):
+        try:
+            lib().mzo_gridmerge_free(self.p)
+        except Exception:
+            pass
+
+    @property
+    def s(self):
+        return self.p.contents
+
+    def extent_props(self, T, v, mass):
+        lib().mzo_gridmerge_extent_props(self.p, T, (C.c_double * 3)(*v), mass)
+
+    def extent(self, vx, vy, vz):
+        lib().mzo_gridmerge_extent(self.p, (C.c_double * 2)(*vx), (C.c_double * 2)(*vy), (C.c_double * 2)(*vz))
+
+    def index(self, v):
+        return lib().mzo_gridmerge_index(self.p, (C.c_double * 3)(*v))
+
+    def cell(self, k):
+        """velocity cell k (1-based)"""
+        return self.s.cells[k - 1]
 
 
 class Octree:

@@ -885,3 +885,181 @@ void mzo_merge_roulette(mzo_rng *rng, mzo_pv *pv, mzo_pia *pia, int64_t cell, in
         }
     }
 }
+
+/* ===================================================================== */
+/* Grid N:2 merging (src/merging/merging_grid.jl)                          */
+
+mzo_gridmerge *mzo_gridmerge_new(int64_t Nx, int64_t Ny, int64_t Nz, const double mult[3]) {
+    mzo_gridmerge *mg = (mzo_gridmerge *)calloc(1, sizeof(mzo_gridmerge));
+    mg->Nx = Nx; mg->Ny = Ny; mg->Nz = Nz;
+    mg->NyNz = Ny * Nz;
+    mg->Ntotal = Nx * Ny * Nz + 8;
+    for (int d = 0; d < 3; ++d) mg->mult[d] = mult[d];
+    mg->cells = (mzo_gcell *)calloc((size_t)mg->Ntotal, sizeof(mzo_gcell));
+    return mg;
+}
+
+void mzo_gridmerge_free(mzo_gridmerge *mg) {
+    if (!mg) return;
+    free(mg->cells);
+    free(mg);
+}
+
+/* compute_velocity_extent!(merging_grid, cell, species, species_data, phys_props) (:190-197) */
+void mzo_gridmerge_extent_props(mzo_gridmerge *mg, double T, const double v[3], double mass) {
+    const double vt = sqrt(2 * T * MZO_KB / mass);
+    const int64_t N[3] = {mg->Nx, mg->Ny, mg->Nz};
+    for (int d = 0; d < 3; ++d) {
+        const double dv = mg->mult[d] * vt;
+        mg->lower[d] = v[d] - dv;
+        mg->upper[d] = v[d] + dv;
+        mg->mid[d] = v[d];
+        mg->dv[d] = 2 * dv / (double)N[d];
+        mg->dv_inv[d] = 1.0 / mg->dv[d];
+    }
+}
+
+/* compute_velocity_extent!(merging_grid, vx_extent, vy_extent, vz_extent) (:210-220) */
+void mzo_gridmerge_extent(mzo_gridmerge *mg, const double vx[2], const double vy[2], const double vz[2]) {
+    const double *e[3] = {vx, vy, vz};
+    const int64_t N[3] = {mg->Nx, mg->Ny, mg->Nz};
+    for (int d = 0; d < 3; ++d) {
+        mg->lower[d] = e[d][0];
+        mg->upper[d] = e[d][1];
+        mg->mid[d] = 0.5 * (e[d][0] + e[d][1]);
+        mg->dv[d] = (e[d][1] - e[d][0]) / (double)N[d];
+        mg->dv_inv[d] = 1.0 / mg->dv[d];
+    }
+}
+
+/* compute_grid_index (:235-270).  A velocity exactly on an upper bound gives
+ * floor((upper - lower) / dv) = N, an index the reference does not bound
+ * (@inbounds); indices past Ntotal are clamped here and counted.  A grid of
+ * zero width (T = 0) gives NaN cell coordinates, where the reference's
+ * floor(Int64, NaN) throws; here each coordinate is taken into [0, N]
+ * (NaN -> 0) before the index is formed. */
+static int64_t grid_coord(double f, int64_t n) {
+    const double fl = floor(f);
+    return fl >= 0.0 ? (fl <= (double)n ? (int64_t)fl : n) : 0;
+}
+
+int64_t mzo_gridmerge_index(mzo_gridmerge *mg, const double v[3]) {
+    int outside = 0;
+    for (int d = 0; d < 3 && !outside; ++d)
+        if (v[d] < mg->lower[d] || v[d] > mg->upper[d]) outside = 1;
+    int64_t index;
+    if (!outside) {
+        const double ix = (v[0] - mg->lower[0]) * mg->dv_inv[0];
+        const double iy = (v[1] - mg->lower[1]) * mg->dv_inv[1];
+        const double iz = (v[2] - mg->lower[2]) * mg->dv_inv[2];
+        index = grid_coord(ix, mg->Nx) * mg->NyNz + grid_coord(iy, mg->Ny) * mg->Nz + grid_coord(iz, mg->Nz) + 1;
+    } else {
+        index = mg->Ntotal - 7;
+        if (v[0] > mg->mid[0]) index += 1;
+        if (v[1] > mg->mid[1]) index += 2;
+        if (v[2] > mg->mid[2]) index += 4;
+    }
+    if (index > mg->Ntotal) {
+        mg->out_of_range += 1;
+        index = mg->Ntotal;
+    }
+    return index;
+}
+
+/* compute_grid! (:304-384): sums in particle order, group 1 then group 2 */
+static void grid_accumulate(mzo_gridmerge *mg, mzo_pv *pv, int64_t pos, int pass) {
+    const int64_t id = PID(pv, pos);
+    const double w = PV_W(pv, id), *v = PV_V(pv, id), *x = PV_X(pv, id);
+    mzo_gcell *g = &mg->cells[mzo_gridmerge_index(mg, v) - 1];
+    if (pass == 0) {
+        g->np += 1;
+        g->w += w;
+        for (int d = 0; d < 3; ++d) {
+            g->v_mean[d] = g->v_mean[d] + v[d] * w;
+            g->x_mean[d] = g->x_mean[d] + x[d] * w;
+        }
+        if (g->np == 1) g->particle_index1 = pos;
+        else if (g->np == 2) g->particle_index2 = pos;
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            const double a = v[d] - g->v_mean[d], c = x[d] - g->x_mean[d];
+            g->v_std_sq[d] = g->v_std_sq[d] + (a * a) * w;
+            g->x_std_sq[d] = g->x_std_sq[d] + (c * c) * w;
+        }
+    }
+}
+
+static void grid_compute(mzo_gridmerge *mg, mzo_pv *pv, mzo_pia *pia, int64_t cell, int64_t species) {
+    memset(mg->cells, 0, sizeof(mzo_gcell) * (size_t)mg->Ntotal);   /* clear_merging_grid! (:278-291) */
+    const mzo_ind *d = mzo_pia_at(pia, cell, species);
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int64_t i = d->start1; i <= d->end1; ++i) grid_accumulate(mg, pv, i, pass);
+        if (d->start2 > 0)
+            for (int64_t i = d->start2; i <= d->end2; ++i) grid_accumulate(mg, pv, i, pass);
+        for (int64_t k = 0; k < mg->Ntotal; ++k) {
+            mzo_gcell *g = &mg->cells[k];
+            if (pass == 0) {
+                if (g->w > 0.0) {
+                    for (int q = 0; q < 3; ++q) { g->v_mean[q] = g->v_mean[q] / g->w; g->x_mean[q] = g->x_mean[q] / g->w; }
+                } else {
+                    g->np = 0;
+                }
+            } else if (g->w > 0.0) {
+                for (int q = 0; q < 3; ++q) { g->v_std_sq[q] = g->v_std_sq[q] / g->w; g->x_std_sq[q] = g->x_std_sq[q] / g->w; }
+            }
+        }
+    }
+}
+
+/* compute_new_particles! (:393-482) and its grid-clamped variant (:484-595) */
+void mzo_merge_grid_based(mzo_rng *rng, mzo_gridmerge *mg, mzo_pv *pv, mzo_pia *pia, int64_t cell, int64_t species,
+                          const mzo_grid1d *grid) {
+    mzo_rng_set_stream(rng, (uint64_t)(cell - 1));
+    grid_compute(mg, pv, pia, cell, species);
+    const int64_t Nt = mg->Ntotal;
+    double *ow = (double *)calloc((size_t)(2 * Nt), sizeof(double));
+    double *ov = (double *)calloc((size_t)(6 * Nt), sizeof(double));
+    double *ox = (double *)calloc((size_t)(6 * Nt), sizeof(double));
+    for (int64_t k = 0; k < Nt; ++k) {
+        mzo_gcell *g = &mg->cells[k];
+        double *w1 = &ow[2 * k], *v1 = &ov[6 * k], *x1 = &ox[6 * k];
+        if (g->np > 2) {
+            w1[0] = 0.5 * g->w;
+            w1[1] = w1[0];
+            double sv[3], sx[3], dir[3];
+            for (int d = 0; d < 3; ++d) { sv[d] = sqrt(g->v_std_sq[d]); sx[d] = sqrt(g->x_std_sq[d]); }
+            for (int d = 0; d < 3; ++d) dir[d] = draw_sign(rng);
+            for (int d = 0; d < 3; ++d) { v1[d] = g->v_mean[d] + dir[d] * sv[d]; v1[3 + d] = g->v_mean[d] - dir[d] * sv[d]; }
+            for (int d = 0; d < 3; ++d) dir[d] = draw_sign(rng);
+            for (int d = 0; d < 3; ++d) { x1[d] = g->x_mean[d] + dir[d] * sx[d]; x1[3 + d] = g->x_mean[d] - dir[d] * sx[d]; }
+        } else if (g->np >= 1) {
+            for (int h = 0; h < g->np; ++h) {
+                const int64_t id = PID(pv, h == 0 ? g->particle_index1 : g->particle_index2);
+                w1[h] = PV_W(pv, id);
+                memcpy(&v1[3 * h], PV_V(pv, id), 24);
+                memcpy(&x1[3 * h], PV_X(pv, id), 24);
+            }
+        }
+    }
+    mzo_ind *d = mzo_pia_at(pia, cell, species);
+    int64_t cur = 0;
+    for (int64_t k = 0; k < Nt; ++k) {
+        const int64_t np = mg->cells[k].np;
+        const int n_out = np >= 2 ? 2 : (int)np;
+        for (int h = 0; h < n_out; ++h) {
+            const int64_t id = PID(pv, mzo_map_cont_index(d, cur));
+            cur += 1;
+            PV_W(pv, id) = ow[2 * k + h];
+            memcpy(PV_V(pv, id), &ov[6 * k + 3 * h], 24);
+            memcpy(PV_X(pv, id), &ox[6 * k + 3 * h], 24);
+            if (grid && np >= 2) {
+                if (PV_X(pv, id)[0] < grid->min_x) PV_X(pv, id)[0] = grid->min_x;
+                else if (PV_X(pv, id)[0] > grid->max_x) PV_X(pv, id)[0] = grid->max_x;
+            }
+        }
+    }
+    free(ow); free(ov); free(ox);
+    const int64_t n_del = d->n_local - cur;
+    if (!(cell == pia->n_cells) || (n_del > d->n_group2)) pia->contiguous[species - 1] = 0;
+    for (int64_t i = 0; i < n_del; ++i) mzo_delete_particle_end(pv, pia, cell, species);
+}
