@@ -632,6 +632,81 @@ def run_bkw_swpm(a, pkg):
     st.close()
 
 
+def grid_merge_cpu_baseline(sp, n_cells=512, target_s=10.0):
+    """The C oracle's merge_grid_based! on one core over n_cells of the BKW
+    boxes, re-sampled between passes; returns the cpu_baseline object."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mzo
+    mzo.build()
+    case = sample_bkw_boxes(n_cells, SW_PPC, seed=12)
+    n = case["n"]
+    mg = mzo.GridMerge(4, 4, 4, 3.5)
+    done, el = 0, 0.0
+    while el < target_s:
+        pv, pia = mzo.ParticleVector(n), mzo.PIA(n_cells, 1)
+        for i in range(1, n + 1):
+            pv.pop_to(i)
+        pv.x[:n], pv.v[:n], pv.w[:n] = case["x"], case["v"], case["w"]
+        for c in range(n_cells):
+            pia.set(c + 1, 1, *case["pia"][c])
+        pia.set_n_total(1, n)
+        pp = mzo.Props(n_cells, 1)
+        mzo.compute_props([pv], pia, [sp[0].mass], pp)
+        r = mzo.RNG(seed=11, tag=3)
+        t0 = time.perf_counter()
+        for c in range(1, n_cells + 1):
+            mg.extent_props(pp.T[0, c - 1], pp.v[0, c - 1], sp[0].mass)
+            mzo.lib().mzo_merge_grid_based(r.ref, mg.p, pv.p, pia.p, c, 1, None)
+        el += time.perf_counter() - t0
+        done += n_cells
+    return {"value": done / el, "unit": "merges/s", "cores": 1, "kind": "port",
+            "sample": f"{done} box merges ({n_cells} boxes per pass) on the C oracle"}
+
+
+def run_grid_merge(a, pkg):
+    """merge_grid_based! (GridN2Merge(4, 4, 4, 3.5), extents from each cell's
+    props) of the 12,288 BKW boxes of 1,000 particles: one call merges every
+    box; the boxes are re-uploaded between repetitions."""
+    case = sample_bkw_boxes(SW_CELLS, SW_PPC)
+    n = case["n"]
+    data = os.path.join(ROOT, "merzbild.jl_amd", "data")
+    sp = pkg.load_species_data(os.path.join(data, "particles.toml"), ["Ar"])
+    st = pkg.DeviceStore(SW_CELLS, 1, n + 1024)
+    mg = pkg.GridN2Merge(4, extent_multiplier=3.5)
+    reps = 5
+    times, out_np = [], 0
+    for r in range(reps + 1):
+        st.upload_species(1, case["x"], case["v"], case["w"], np.arange(1, n + 1, dtype=np.int64), case["pia"], n,
+                          True, np.zeros(0, dtype=np.int64), n)
+        st.set_seed(11)
+        pkg.compute_props(st, sp)
+        st.sync()
+        st.timing_reset()
+        st.timing(True)
+        merged = pkg.merge_grid_based(st, 1, 2, mg, species_data=sp, squash=False, tag=(r << 4) | 3)
+        st.sync()
+        ms, _ = st.timing_get("merge_grid")
+        st.timing(False)
+        sc = st.scalars(1)
+        assert sc["err"] == 0 and merged == SW_CELLS, sc
+        out_np = sc["n_total"]
+        if r > 0:      # the first call allocates the scratch
+            times.append(ms)
+    ms = float(np.median(times))
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = grid_merge_cpu_baseline(sp)
+    out = {"metric": "grid N:2 merges/s", "value": SW_CELLS / (ms * 1e-3), "unit": "merges/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic", "vs_baseline": None,
+           "ms_per_call": ms, "particles_per_s": n / (ms * 1e-3),
+           "config": {"workload": "12288 BKW boxes x 1000 Ar particles, merge_grid_based! with GridN2Merge(4, 3.5), "
+                                  "extents from the props", "n_particles": n, "n_after": out_np}}
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+    st.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -646,7 +721,7 @@ def main():
                     help="batched-0d: skip timing the plain and per-cell launch patterns")
     ap.add_argument("--no-autotune", action="store_true", help="slab mode: keep the library's default shape")
     ap.add_argument("--workload", choices=("couette", "batched-0d", "nnls-merge", "couette-surf", "roulette-merge",
-                                           "bkw-swpm"),
+                                           "bkw-swpm", "grid-merge"),
                     default="couette",
                     help="couette: the headline 1-D Couette line; batched-0d: 65,536 uneven 0-D cells; "
                          "nnls-merge: 32,768 cells merged by NNLS; couette-surf: the averaging phase of the "
@@ -685,6 +760,9 @@ def main():
         return
     if a.workload == "bkw-swpm":
         run_bkw_swpm(a, pkg)
+        return
+    if a.workload == "grid-merge":
+        run_grid_merge(a, pkg)
         return
     case = sample_case()
     st, sp, it = setup_store(pkg, case)
