@@ -41,6 +41,14 @@ struct OctBin {
     int32_t np, start, end, depth, refine, pad;
 };
 
+// orders LDS accesses among the lanes of one wave (a barrier for code that a
+// single wave runs)
+__device__ inline void oct_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline int64_t oct_slot(int li, int32_t s1, int32_t n1, int32_t s2) {
     return li < n1 ? (int64_t)s1 - 1 + li : (int64_t)s2 - 1 + (li - n1);
 }
@@ -136,9 +144,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     __shared__ double vel_mid[3];
     __shared__ int32_t cnt8[9], start8[9], run8[9], nonempty[9], ne_cnt[9];
     __shared__ double ndens8[9];
-    __shared__ int32_t Nbins_s, total_post_s, refine_s, stop_s;
-    __shared__ double pick_w[4];
-    __shared__ int32_t pick_id[4];
+    __shared__ int32_t Nbins_s, total_post_s, stop_s;
 
     const int c = blockIdx.x;
     A.rp = mz_rng_resolve(A.rp);
@@ -185,163 +191,161 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     }
     __syncthreads();
 
-    // compute_octree! (merging_octree_N2.jl:998-1039)
-    for (;;) {
-        // refine pick: first maximum of w over refinable bins
-        {
+    // compute_octree! (merging_octree_N2.jl:998-1039).  Each split depends on
+    // the one before, so the loop runs on wave 0 alone, where a barrier is a
+    // wave-level fence; the other waves wait at the block barrier after it.
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        for (;;) {
+            // refine pick: first maximum of w over refinable bins (lowest id on ties)
             double bw = -1.0;
             int bid = -1;
-            for (int b = threadIdx.x; b < Nbins_s; b += blockDim.x) {
+            for (int b = lane; b < Nbins_s; b += 64) {
                 if (bins[b].refine && bins[b].w > bw) { bw = bins[b].w; bid = b; }
             }
-            // wave argmax with lowest id on ties
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) {
                 double ow = __shfl_xor(bw, o, 64);
                 int oi = __shfl_xor(bid, o, 64);
                 if (oi >= 0 && (bid < 0 || ow > bw || (ow == bw && oi < bid))) { bw = ow; bid = oi; }
             }
-            if ((threadIdx.x & 63) == 0) { pick_w[threadIdx.x >> 6] = bw; pick_id[threadIdx.x >> 6] = bid; }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                double w0 = -1.0;
-                int i0 = -1;
-                for (int k = 0; k < 4; ++k) {
-                    int oi = pick_id[k];
-                    double ow = pick_w[k];
-                    if (oi >= 0 && (i0 < 0 || ow > w0 || (ow == w0 && oi < i0))) { w0 = ow; i0 = oi; }
+            if (bid < 0 || total_post_s + 14 > A.target_np) break;
+            // split_bin! (merging_octree_N2.jl:503-633)
+            const int b = bid;
+            const int bs = bins[b].start, be = bins[b].end;
+            const int depth = bins[b].depth;
+            if (A.bounds_mode == 2) {
+                double bmn[3], bmx[3];
+                for (int d = 0; d < 3; ++d) {
+                    const double *vv = d == 0 ? A.sp.v0 : (d == 1 ? A.sp.v1 : A.sp.v2);
+                    double lo = 9299792458.0, hi = -9299792458.0;
+                    for (int i = bs + lane; i <= be; i += 64) {
+                        const double x = vv[oct_slot(pis[i], s1, n1, s2)];
+                        lo = fmin(lo, x);
+                        hi = fmax(hi, x);
+                    }
+                    bmn[d] = mz_wave_min(lo);
+                    bmx[d] = mz_wave_max(hi);
                 }
-                refine_s = i0;
-                if (i0 < 0 || total_post_s + 14 > A.target_np) stop_s = 1;
+                if (lane == 0 && be >= bs)
+                    for (int d = 0; d < 3; ++d) { bins[b].vmin[d] = bmn[d]; bins[b].vmax[d] = bmx[d]; }
+                oct_wave_sync();
             }
-            __syncthreads();
-        }
-        if (stop_s) break;
-        // split_bin! (merging_octree_N2.jl:503-633)
-        const int b = refine_s;
-        const int bs = bins[b].start, be = bins[b].end;
-        const int depth = bins[b].depth;
-        if (A.bounds_mode == 2) {
-            double bmn[3], bmx[3];
-            oct_bounds(A, pis, bs, be, s1, n1, s2, bmn, bmx, red);
-            if (threadIdx.x == 0 && be >= bs)
-                for (int d = 0; d < 3; ++d) { bins[b].vmin[d] = bmn[d]; bins[b].vmax[d] = bmx[d]; }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            if (A.split == 1) {
-                for (int d = 0; d < 3; ++d) vel_mid[d] = 0.5 * (bins[b].vmin[d] + bins[b].vmax[d]);
-            } else {
-                // compute_v_mean! keeps the previous middle velocity in the sum
-                double nt = 0.0;
-                for (int i = bs; i <= be; ++i) {
-                    int64_t p = oct_slot(pis[i], s1, n1, s2);
-                    double w = A.sp.w[p];
-                    nt += w;
-                    vel_mid[0] = vel_mid[0] + w * A.sp.v0[p];
-                    vel_mid[1] = vel_mid[1] + w * A.sp.v1[p];
-                    vel_mid[2] = vel_mid[2] + w * A.sp.v2[p];
+            if (lane == 0) {
+                if (A.split == 1) {
+                    for (int d = 0; d < 3; ++d) vel_mid[d] = 0.5 * (bins[b].vmin[d] + bins[b].vmax[d]);
+                } else {
+                    // compute_v_mean! keeps the previous middle velocity in the sum
+                    double nt = 0.0;
+                    for (int i = bs; i <= be; ++i) {
+                        int64_t p = oct_slot(pis[i], s1, n1, s2);
+                        double w = A.sp.w[p];
+                        nt += w;
+                        vel_mid[0] = vel_mid[0] + w * A.sp.v0[p];
+                        vel_mid[1] = vel_mid[1] + w * A.sp.v1[p];
+                        vel_mid[2] = vel_mid[2] + w * A.sp.v2[p];
+                    }
+                    for (int d = 0; d < 3; ++d) vel_mid[d] = vel_mid[d] / nt;
                 }
-                for (int d = 0; d < 3; ++d) vel_mid[d] = vel_mid[d] / nt;
+                for (int o = 0; o < 9; ++o) { cnt8[o] = 0; ndens8[o] = 0.0; }
             }
-            for (int o = 0; o < 9; ++o) { cnt8[o] = 0; ndens8[o] = 0.0; }
-        }
-        __syncthreads();
-        for (int i = bs + threadIdx.x; i <= be; i += blockDim.x) {
-            int64_t p = oct_slot(pis[i], s1, n1, s2);
-            int o = 1;                        // compute_octant (:304-316): strict >
-            if (A.sp.v0[p] > vel_mid[0]) o += 1;
-            if (A.sp.v1[p] > vel_mid[1]) o += 2;
-            if (A.sp.v2[p] > vel_mid[2]) o += 4;
-            poct[i - bs] = (uint8_t)o;
-            atomicAdd(&cnt8[o], 1);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int acc = 0;
-            for (int o = 1; o <= 8; ++o) { start8[o] = bs + acc; run8[o] = 0; acc += cnt8[o]; }
-        }
-        __syncthreads();
-        // Scatter by octant (wave 0).  The reference walks the bin forward and
-        // fills each octant from its end (merging_octree_N2.jl:621-626), so
-        // every child holds its particles in reverse parent order.
-        if (threadIdx.x < 64) {
-            const int lane = threadIdx.x;
-            const uint64_t lt = mz_lanemask_lt();
-            volatile int32_t *vrun = run8;
-            for (int i0 = bs; i0 <= be; i0 += 64) {
-                const int i = i0 + lane;
-                const bool valid = i <= be;
-                const int o = valid ? poct[i - bs] : 0;
-                for (int q = 1; q <= 8; ++q) {
-                    uint64_t m = __ballot(valid && o == q);
-                    if (m == 0) continue;
-                    const int32_t r0 = vrun[q];
-                    if (valid && o == q) psort[start8[q] + cnt8[q] - 1 - (r0 + __popcll(m & lt))] = pis[i];
-                    vrun[q] = r0 + __popcll(m);
+            oct_wave_sync();
+            for (int i = bs + lane; i <= be; i += 64) {
+                int64_t p = oct_slot(pis[i], s1, n1, s2);
+                int o = 1;                        // compute_octant (:304-316): strict >
+                if (A.sp.v0[p] > vel_mid[0]) o += 1;
+                if (A.sp.v1[p] > vel_mid[1]) o += 2;
+                if (A.sp.v2[p] > vel_mid[2]) o += 4;
+                poct[i - bs] = (uint8_t)o;
+                atomicAdd(&cnt8[o], 1);
+            }
+            oct_wave_sync();
+            if (lane == 0) {
+                int acc = 0;
+                for (int o = 1; o <= 8; ++o) { start8[o] = bs + acc; run8[o] = 0; acc += cnt8[o]; }
+            }
+            oct_wave_sync();
+            // Scatter by octant.  The reference walks the bin forward and fills
+            // each octant from its end (merging_octree_N2.jl:621-626), so every
+            // child holds its particles in reverse parent order.
+            {
+                const uint64_t lt = mz_lanemask_lt();
+                volatile int32_t *vrun = run8;
+                for (int i0 = bs; i0 <= be; i0 += 64) {
+                    const int i = i0 + lane;
+                    const bool valid = i <= be;
+                    const int o = valid ? poct[i - bs] : 0;
+                    for (int q = 1; q <= 8; ++q) {
+                        uint64_t m = __ballot(valid && o == q);
+                        if (m == 0) continue;
+                        const int32_t r0 = vrun[q];
+                        if (valid && o == q) psort[start8[q] + cnt8[q] - 1 - (r0 + __popcll(m & lt))] = pis[i];
+                        vrun[q] = r0 + __popcll(m);
+                    }
                 }
             }
-        }
-        __syncthreads();
-        for (int i = bs + threadIdx.x; i <= be; i += blockDim.x) pis[i] = psort[i];
-        __syncthreads();
-        // per-octant weights in the reference's accumulation order, i.e. parent
-        // order, which is each child's segment read backwards (one lane per octant)
-        if (threadIdx.x >= 1 && threadIdx.x <= 8) {
-            const int o = threadIdx.x, lo = start8[o];
-            double s = 0.0;
-            // eight loads in flight, then the adds in the same order
-            for (int i = start8[o] + cnt8[o] - 1; i >= lo; i -= 8) {
-                const int m = min(8, i - lo + 1);
-                double wb[8];
+            oct_wave_sync();
+            for (int i = bs + lane; i <= be; i += 64) pis[i] = psort[i];
+            oct_wave_sync();
+            // per-octant weights in the reference's accumulation order, i.e. parent
+            // order, which is each child's segment read backwards (one lane per octant)
+            if (lane >= 1 && lane <= 8) {
+                const int o = lane, lo = start8[o];
+                double sw = 0.0;
+                // eight loads in flight, then the adds in the same order
+                for (int i = start8[o] + cnt8[o] - 1; i >= lo; i -= 8) {
+                    const int m = min(8, i - lo + 1);
+                    double wb[8];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) wb[j] = j < m ? A.sp.w[oct_slot(pis[i - j], s1, n1, s2)] : 0.0;
+                    for (int j = 0; j < 8; ++j) wb[j] = j < m ? A.sp.w[oct_slot(pis[i - j], s1, n1, s2)] : 0.0;
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (j < m) s += wb[j];
-            }
-            ndens8[o] = s;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int n_ne = 0;
-            for (int o = 1; o <= 8; ++o)
-                if (cnt8[o] > 0) { ++n_ne; nonempty[n_ne] = o; ne_cnt[n_ne] = cnt8[o]; }
-            const int Nb = Nbins_s;
-            double pmin[3], pmax[3];
-            for (int d = 0; d < 3; ++d) { pmin[d] = bins[b].vmin[d]; pmax[d] = bins[b].vmax[d]; }
-            int prev_end = bs - 1;
-            total_post_s -= 2;
-            for (int k = 1; k <= n_ne; ++k) {
-                const int bi = k == 1 ? b : Nb + k - 2;   // child 1 keeps the id (get_new_bin_id)
-                if (bi >= OCT_MAX_BINS) { atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE); stop_s = 1; break; }
-                OctBin &nb = bins[bi];
-                nb.start = prev_end + 1;
-                nb.end = nb.start + ne_cnt[k] - 1;
-                prev_end = nb.end;
-                const int oc = nonempty[k];
-                if (A.bounds_mode == 1) {   // bin_bounds_inherit! (:341-367)
-                    const int sx = (oc - 1) & 1, sy = ((oc - 1) >> 1) & 1, sz = ((oc - 1) >> 2) & 1;
-                    nb.vmin[0] = sx ? vel_mid[0] : pmin[0]; nb.vmax[0] = sx ? pmax[0] : vel_mid[0];
-                    nb.vmin[1] = sy ? vel_mid[1] : pmin[1]; nb.vmax[1] = sy ? pmax[1] : vel_mid[1];
-                    nb.vmin[2] = sz ? vel_mid[2] : pmin[2]; nb.vmax[2] = sz ? pmax[2] : vel_mid[2];
+                    for (int j = 0; j < 8; ++j)
+                        if (j < m) sw += wb[j];
                 }
-                nb.np = ne_cnt[k];
-                nb.w = ndens8[oc];
-                nb.depth = depth + 1;
-                total_post_s += nb.np >= 2 ? 2 : nb.np;
-                nb.refine = (nb.np > 2) && (nb.depth < A.max_depth);
+                ndens8[o] = sw;
             }
-            Nbins_s = Nb + n_ne - 1;
-            if (Nbins_s + 7 > A.max_Nbins) stop_s = 1;
-            else if (Nbins_s + 7 > max_bins) {   // LDS bin budget below the caller's max_Nbins
-                atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
-                stop_s = 1;
+            oct_wave_sync();
+            if (lane == 0) {
+                int n_ne = 0;
+                for (int o = 1; o <= 8; ++o)
+                    if (cnt8[o] > 0) { ++n_ne; nonempty[n_ne] = o; ne_cnt[n_ne] = cnt8[o]; }
+                const int Nb = Nbins_s;
+                double pmin[3], pmax[3];
+                for (int d = 0; d < 3; ++d) { pmin[d] = bins[b].vmin[d]; pmax[d] = bins[b].vmax[d]; }
+                int prev_end = bs - 1;
+                total_post_s -= 2;
+                for (int k = 1; k <= n_ne; ++k) {
+                    const int bi = k == 1 ? b : Nb + k - 2;   // child 1 keeps the id (get_new_bin_id)
+                    if (bi >= OCT_MAX_BINS) { atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE); stop_s = 1; break; }
+                    OctBin &nb = bins[bi];
+                    nb.start = prev_end + 1;
+                    nb.end = nb.start + ne_cnt[k] - 1;
+                    prev_end = nb.end;
+                    const int oc = nonempty[k];
+                    if (A.bounds_mode == 1) {   // bin_bounds_inherit! (:341-367)
+                        const int sx = (oc - 1) & 1, sy = ((oc - 1) >> 1) & 1, sz = ((oc - 1) >> 2) & 1;
+                        nb.vmin[0] = sx ? vel_mid[0] : pmin[0]; nb.vmax[0] = sx ? pmax[0] : vel_mid[0];
+                        nb.vmin[1] = sy ? vel_mid[1] : pmin[1]; nb.vmax[1] = sy ? pmax[1] : vel_mid[1];
+                        nb.vmin[2] = sz ? vel_mid[2] : pmin[2]; nb.vmax[2] = sz ? pmax[2] : vel_mid[2];
+                    }
+                    nb.np = ne_cnt[k];
+                    nb.w = ndens8[oc];
+                    nb.depth = depth + 1;
+                    total_post_s += nb.np >= 2 ? 2 : nb.np;
+                    nb.refine = (nb.np > 2) && (nb.depth < A.max_depth);
+                }
+                Nbins_s = Nb + n_ne - 1;
+                if (Nbins_s + 7 > A.max_Nbins) stop_s = 1;
+                else if (Nbins_s + 7 > max_bins) {   // LDS bin budget below the caller's max_Nbins
+                    atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
+                    stop_s = 1;
+                }
             }
+            oct_wave_sync();
+            if (stop_s) break;
         }
-        __syncthreads();
-        if (stop_s) break;
     }
+    __syncthreads();
     const int Nbins = Nbins_s;
     if (Nbins == 1 && threadIdx.x == 0) {
         double s = 0.0;
