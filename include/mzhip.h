@@ -85,6 +85,22 @@ int mz_set_sigma_g_w_max(mz_store *h, int32_t s1, int32_t s2, const double *sgwm
 int mz_get_collision_factors(mz_store *h, int32_t s1, int32_t s2, double *sgwm, int64_t *n1, int64_t *n2,
                              int64_t *n_coll, int64_t *n_performed, int64_t *n_eq_w);
 int mz_set_seed(mz_store *h, uint64_t seed);
+/* compute_flux_props! (src/properties/flux_props.jl:77-131) over cells
+ * cell_lo..cell_hi (1-based; all cells is the whole-grid form, a range the
+ * cell_chunk form of compute_flux_props_sorted!, :199-249), groups 1 and 2,
+ * all species, with the mean velocities of the store's last mz_compute_props
+ * and the grid's cell volumes.  The store keeps an instantaneous and an
+ * averaged set (FluxProps, :27-58). */
+int mz_compute_flux_props(mz_store *h, const double *masses, int32_t cell_lo, int32_t cell_hi);
+/* avg_props!(flux_props_avg, flux_props, n_avg_timesteps) (:152-186) on the store's two sets */
+int mz_flux_avg(mz_store *h, double n_avg_timesteps);
+/* clear_props!(flux_props) (:133-137) of set which (0 instantaneous, 1 average) */
+int mz_flux_clear(mz_store *h, int32_t which);
+/* Copy set which out; each array is [species][cell][3] (the reference's
+ * (3, n_cells, n_species) arrays in column-major order).  NULL skips one. */
+int mz_get_flux(mz_store *h, int32_t which, double *kinetic_energy_flux, double *diagonal_momentum_flux,
+                double *off_diagonal_momentum_flux);
+
 /* Replay tables (device copies): kind 0 = NTC, 1 = convection, 2 = merges. */
 int mz_set_replay(mz_store *h, int32_t kind, const double *table, int64_t rows, int64_t cols);
 

@@ -45,6 +45,10 @@ _SIGS = {
     "mz_set_seed": (C.c_int, [C.c_void_p, u64]),
     "mz_set_replay": (C.c_int, [C.c_void_p, i32, P(f64), i64, i64]),
     "mz_sort": (C.c_int, [C.c_void_p, i32, i32]),
+    "mz_compute_flux_props": (C.c_int, [C.c_void_p, P(f64), i32, i32]),
+    "mz_flux_avg": (C.c_int, [C.c_void_p, f64]),
+    "mz_flux_clear": (C.c_int, [C.c_void_p, i32]),
+    "mz_get_flux": (C.c_int, [C.c_void_p, i32, P(f64), P(f64), P(f64)]),
     "mz_merge_grid": (C.c_int, [C.c_void_p, i32, i32, i32, i32, i32, P(f64), P(f64), f64, i32, i32, u64, i32,
                                  P(i64)]),
     "mz_set_sort_path": (C.c_int, [C.c_void_p, i32]),
@@ -383,6 +387,14 @@ class DeviceStore:
         """avg_props! on the device: time-average set += instantaneous set / n."""
         _check(load_library().mz_surf_avg(self.h, float(n_avg_timesteps)))
 
+    def flux_avg(self, n_avg_timesteps):
+        """avg_props! of the store's FluxProps sets (flux_props.jl:152-186)."""
+        _check(load_library().mz_flux_avg(self.h, float(n_avg_timesteps)))
+
+    def flux_clear(self, which=0):
+        """clear_props! of the FluxProps set `which` (0 instantaneous, 1 average)."""
+        _check(load_library().mz_flux_clear(self.h, which))
+
     def surf_clear(self, which=0):
         """clear_props! of the device set `which` (0 instantaneous, 1 average)."""
         _check(load_library().mz_surf_clear(self.h, which))
@@ -521,11 +533,15 @@ class SurfProps:
         _check(load_library().mz_set_surf(store.h, which, _d(q)))
 
 
-def clear_props(surf_props):
-    """clear_props!(surf_props) (surface_props.jl:171-189); for the store's
-    device sets use DeviceStore.surf_clear."""
-    for f in SurfProps.FIELDS:
-        getattr(surf_props, f)[...] = 0.0
+_FLUX_FIELDS = ("kinetic_energy_flux", "diagonal_momentum_flux", "off_diagonal_momentum_flux")
+
+
+def clear_props(props):
+    """clear_props! of host copies: SurfProps (surface_props.jl:171-189) or
+    FluxProps (flux_props.jl:133-137); for the store's device sets use
+    DeviceStore.surf_clear / flux_clear."""
+    for f in (_FLUX_FIELDS if isinstance(props, FluxProps) else SurfProps.FIELDS):
+        getattr(props, f)[...] = 0.0
 
 
 def surface_props_scale(species, surf_props, species_data, dt):
@@ -551,6 +567,8 @@ def avg_props(props_avg, props, n_avg_timesteps):
         if props_avg.ndens_not_Np != props.ndens_not_Np:
             raise MzError("Inconsistent computation of ndens/number of physical particles in cell")
         fields = ("lpa", "np", "n", "v", "T")
+    elif isinstance(props, FluxProps):
+        fields = _FLUX_FIELDS           # flux_props.jl:152-186
     else:
         fields = SurfProps.FIELDS
     for f in fields:
@@ -681,6 +699,44 @@ def merge_octree_N2_based(store, species, threshold, target_np, split=OCTREE_MID
                                           bin_bounds_compute, max_Nbins, max_depth, 1 if grid_clamp else 0,
                                           1 if squash else 0, tag, rng_mode, C.byref(n)))
     return n.value
+
+
+class FluxProps:
+    """FluxProps (src/properties/flux_props.jl:27-58), host copies shaped
+    [species, cell, component] (the reference's arrays are (component, cell,
+    species) in column-major order, the same memory order)."""
+
+    def __init__(self, n_cells, n_species):
+        self.n_cells, self.n_species = n_cells, n_species
+        self.kinetic_energy_flux = np.zeros((n_species, n_cells, 3))
+        self.diagonal_momentum_flux = np.zeros((n_species, n_cells, 3))
+        self.off_diagonal_momentum_flux = np.zeros((n_species, n_cells, 3))
+
+    def fetch(self, store, which=0):
+        """Copy the store's instantaneous (which=0) or averaged (1) set."""
+        _check(load_library().mz_get_flux(store.h, which, _d(self.kinetic_energy_flux),
+                                          _d(self.diagonal_momentum_flux), _d(self.off_diagonal_momentum_flux)))
+        return self
+
+
+def compute_flux_props(store, species_data=None, flux_props=None, cells=None, fetch=True):
+    """compute_flux_props!(particles, pia, species_data, phys_props,
+    flux_props, grid) (src/properties/flux_props.jl:77-131), or with cells =
+    (lo, hi) the cell_chunk form of compute_flux_props_sorted! (:199-249).
+    Uses the mean velocities of the store's last compute_props call."""
+    m = _masses(store, species_data)
+    lo, hi = (1, store.n_cells) if cells is None else cells
+    _check(load_library().mz_compute_flux_props(store.h, _d(m), int(lo), int(hi)))
+    if not fetch:
+        return None
+    fp = flux_props if flux_props is not None else FluxProps(store.n_cells, store.n_species)
+    return fp.fetch(store, 0)
+
+
+def compute_flux_props_sorted(store, species_data=None, flux_props=None, cells=None, fetch=True):
+    """compute_flux_props_sorted! (src/properties/flux_props.jl:199-249); the
+    device pass reads groups 1 and 2, so it serves both forms."""
+    return compute_flux_props(store, species_data, flux_props, cells, fetch)
 
 
 class GridN2Merge:

@@ -98,6 +98,8 @@ struct mz_store {
     double surf_inv_area[2] = {1.0, 1.0};
     // time-averaged PhysProps: [np | n | v (x3) | T] x [species][cell], then lpa[species]
     double *d_pavg = nullptr;
+    // FluxProps: [set][species][cell][9] (energy xyz, diagonal xyz, off-diagonal xy xz yz), set 1 = average
+    double *d_flux = nullptr;
 };
 
 MzViews mz_views(mz_store *h);

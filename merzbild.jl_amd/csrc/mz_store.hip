@@ -170,6 +170,8 @@ extern "C" int mz_create(int32_t n_cells, int32_t n_species, const int64_t *capa
     MZ_HIP(hipMalloc(&h->d_invV, n_cells * sizeof(double)));
     MZ_HIP(hipMalloc(&h->d_masses, MZ_MAX_SPECIES * sizeof(double)));
     MZ_HIP(hipMemset(h->d_masses, 0, MZ_MAX_SPECIES * sizeof(double)));
+    MZ_HIP(hipMalloc(&h->d_flux, (size_t)2 * 9 * pn * sizeof(double)));
+    MZ_HIP(hipMemset(h->d_flux, 0, (size_t)2 * 9 * pn * sizeof(double)));
     const size_t surf_bytes = (size_t)2 * n_species * 2 * MZ_SURF_Q * sizeof(double);
     MZ_HIP(hipMalloc(&h->d_surf, surf_bytes));
     MZ_HIP(hipMemset(h->d_surf, 0, surf_bytes));
@@ -204,6 +206,7 @@ extern "C" int mz_destroy(mz_store *h) {
     hipFree(h->d_np); hipFree(h->d_n); hipFree(h->d_T); hipFree(h->d_v); hipFree(h->d_V);
     hipFree(h->d_invV);
     hipFree(h->d_masses);
+    hipFree(h->d_flux);
     hipFree(h->d_surf);
     hipFree(h->d_surf_ticket);
     hipFree(h->d_pavg);
@@ -1572,6 +1575,102 @@ extern "C" int mz_compute_props(mz_store *h, int32_t variant, int32_t ndens_not_
     mz_timing_end(h);
     if (r) return r;
     MZ_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// compute_flux_props! / compute_flux_props_sorted! (flux_props.jl:77-131,
+// 199-249): per (cell, species), with C = v - vbar from the store's last
+// props, sum w C |C|^2, w (Cx^2, Cy^2, Cz^2) and w (Cx Cy, Cx Cz, Cy Cz)
+// over groups 1 and 2, scaled by m / V (and 1/2 for the energy flux).  One
+// wave per (cell, species) of cells [lo, hi]; other cells are left as they
+// are, as the reference's cell_chunk form leaves them.
+__global__ void __launch_bounds__(256) k_flux(MzViews views, int32_t n_cells, int32_t lo, int32_t hi,
+                                              const double *__restrict__ masses, const double *__restrict__ inv_V,
+                                              const double *__restrict__ vbar, double *__restrict__ out) {
+    const int sidx = blockIdx.y;
+    const MzSpView &sp = views.v[sidx];
+    const int c = lo + blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c > hi) return;
+    const int32_t st1 = sp.pia.start1[c] - 1, n1 = sp.pia.n1[c];
+    const int32_t st2 = sp.pia.start2[c] - 1, n2 = sp.pia.n2[c] > 0 ? sp.pia.n2[c] : 0;
+    const int32_t tot = n1 + n2;
+    const int64_t ix = (int64_t)sidx * n_cells + c;
+    const double u0 = vbar[3 * ix], u1 = vbar[3 * ix + 1], u2 = vbar[3 * ix + 2];
+    double q[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) q[j] = 0.0;
+    for (int32_t i = lane; i < tot; i += 64) {
+        const int64_t p = i < n1 ? (int64_t)st1 + i : (int64_t)st2 + (i - n1);
+        const double w = sp.w[p];
+        const double c0 = sp.v0[p] - u0, c1 = sp.v1[p] - u1, c2 = sp.v2[p] - u2;
+        const double cx = c0 * c0, cy = c1 * c1, cz = c2 * c2;
+        const double csq = cx + cy + cz;
+        q[0] = q[0] + w * c0 * csq; q[1] = q[1] + w * c1 * csq; q[2] = q[2] + w * c2 * csq;
+        q[3] = q[3] + w * cx; q[4] = q[4] + w * cy; q[5] = q[5] + w * cz;
+        q[6] = q[6] + w * (c0 * c1); q[7] = q[7] + w * (c0 * c2); q[8] = q[8] + w * (c1 * c2);
+    }
+#pragma unroll
+    for (int j = 0; j < 9; ++j) q[j] = mz_wave_sum(q[j]);
+    if (lane == 0) {
+        const double m = masses[sidx], iv = inv_V[c];
+        double *o = out + 9 * ix;   // [species][cell][energy xyz | diagonal xyz | off-diagonal xy xz yz]
+        for (int j = 0; j < 3; ++j) o[j] = 0.5 * q[j] * m * iv;
+        for (int j = 3; j < 9; ++j) o[j] = q[j] * m * iv;
+    }
+}
+
+// avg_props! of FluxProps (flux_props.jl:152-186): avg = avg + inst / n
+__global__ void k_flux_avg(const double *__restrict__ inst, double *__restrict__ avg, int64_t n, double inv_n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) avg[i] = avg[i] + inst[i] * inv_n;
+}
+
+extern "C" int mz_compute_flux_props(mz_store *h, const double *masses, int32_t cell_lo, int32_t cell_hi) {
+    MZ_CHECK(masses != nullptr, "flux_props: masses is NULL");
+    MZ_CHECK(h->has_grid, "flux_props: no grid set (the cell volumes come from it)");
+    MZ_CHECK(cell_lo >= 1 && cell_hi <= h->n_cells && cell_lo <= cell_hi, "flux_props: bad cell range");
+    MZ_HIP(hipMemcpyAsync(h->d_masses, masses, h->n_species * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    const int ncell = cell_hi - cell_lo + 1;
+    mz_timing_begin(h, "flux");
+    hipLaunchKernelGGL(k_flux, dim3((ncell + 3) / 4, h->n_species), dim3(256), 0, h->stream, mz_views(h), h->n_cells,
+                       cell_lo - 1, cell_hi - 1, h->d_masses, h->d_invV, h->d_v, h->d_flux);
+    mz_timing_end(h);
+    MZ_HIP(hipGetLastError());
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int mz_flux_avg(mz_store *h, double n_avg_timesteps) {
+    MZ_CHECK(n_avg_timesteps > 0, "flux_avg: n_avg_timesteps must be positive");
+    const int64_t n = (int64_t)9 * h->n_species * h->n_cells;
+    hipLaunchKernelGGL(k_flux_avg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->d_flux,
+                       h->d_flux + n, n, 1.0 / n_avg_timesteps);
+    MZ_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int mz_flux_clear(mz_store *h, int32_t which) {
+    MZ_CHECK(which == 0 || which == 1, "flux_clear: which must be 0 (instantaneous) or 1 (average)");
+    const int64_t n = (int64_t)9 * h->n_species * h->n_cells;
+    MZ_HIP(hipMemsetAsync(h->d_flux + which * n, 0, n * sizeof(double), h->stream));
+    return 0;
+}
+
+extern "C" int mz_get_flux(mz_store *h, int32_t which, double *kinetic_energy_flux, double *diagonal_momentum_flux,
+                           double *off_diagonal_momentum_flux) {
+    MZ_CHECK(which == 0 || which == 1, "get_flux: which must be 0 (instantaneous) or 1 (average)");
+    const int64_t n = (int64_t)9 * h->n_species * h->n_cells;
+    std::vector<double> buf(n);
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    MZ_HIP(hipMemcpy(buf.data(), h->d_flux + which * n, n * sizeof(double), hipMemcpyDeviceToHost));
+    const int64_t m = (int64_t)h->n_species * h->n_cells;
+    for (int64_t k = 0; k < m; ++k)
+        for (int j = 0; j < 3; ++j) {
+            if (kinetic_energy_flux) kinetic_energy_flux[3 * k + j] = buf[9 * k + j];
+            if (diagonal_momentum_flux) diagonal_momentum_flux[3 * k + j] = buf[9 * k + 3 + j];
+            if (off_diagonal_momentum_flux) off_diagonal_momentum_flux[3 * k + j] = buf[9 * k + 6 + j];
+        }
     return 0;
 }
 

@@ -223,6 +223,14 @@ void mzo_compute_props_with_total_moments(mzo_pv **pvs, mzo_pia *pia, const doub
 double mzo_compute_mixed_moment(mzo_pv *pv, mzo_pia *pia, int64_t cell, int64_t species,
                                 int px, int py, int pz, double sum_scaler, double res_scaler);
 void mzo_clear_props(mzo_props *pp);
+/* compute_flux_props! (src/properties/flux_props.jl:77-131) over cells
+ * cell_lo..cell_hi (1-based), groups 1 and 2, in particle order, with the
+ * mean velocities in pp->v; flux is [species][cell][9]: energy flux xyz,
+ * diagonal momentum flux xyz, off-diagonal xy xz yz.  Parity: pinned by
+ * test/test_flux_computes.jl's values and by Maxwellian closed forms
+ * (tests/test_oracle_flux.py). */
+void mzo_compute_flux_props(mzo_pv **pvs, const mzo_pia *pia, const double *masses, const mzo_props *pp,
+                            const double *inv_V, double *flux, int64_t cell_lo, int64_t cell_hi);
 
 /* ---------------- octree N:2 merging (src/merging/merging_octree_N2.jl) ---------------- */
 typedef struct {

@@ -169,6 +169,7 @@ def lib():
             "mzo_ntc_two": (None, [P(Rng), P(CF), P(Interaction), P(PvS), P(PvS), P(PiaS), i64, i64, i64, f64, f64, f64, C.c_int]),
             "mzo_collide_pair": (None, [P(Rng), P(CF), P(Interaction), P(PvS), P(PvS), P(PiaS), i64, i64, i64, i64, i64, f64, C.c_int]),
             "mzo_compute_props": (None, [P(P(PvS)), P(PiaS), P(f64), P(PropsS)]),
+            "mzo_compute_flux_props": (None, [P(P(PvS)), P(PiaS), P(f64), P(PropsS), P(f64), P(f64), i64, i64]),
             "mzo_compute_props_sorted": (None, [P(P(PvS)), P(PiaS), P(f64), P(PropsS), P(f64), i64, i64]),
             "mzo_compute_props_with_total_moments": (None, [P(P(PvS)), P(PiaS), P(f64), P(PropsS)]),
             "mzo_compute_mixed_moment": (f64, [P(PvS), P(PiaS), i64, i64, C.c_int, C.c_int, C.c_int, f64, f64]),
@@ -454,6 +455,16 @@ def pv_array(pvs):
 def compute_props(pvs, pia, masses, props):
     m = np.ascontiguousarray(masses, dtype=np.float64)
     lib().mzo_compute_props(pv_array(pvs), pia.p, _dptr(m), props.ref)
+
+
+def compute_flux_props(pvs, pia, masses, props, inv_V, cells=None):
+    """[species, cell, 9] array: energy flux xyz, diagonal xyz, off-diagonal xy xz yz"""
+    m = np.ascontiguousarray(masses, dtype=np.float64)
+    iv = np.ascontiguousarray(inv_V, dtype=np.float64)
+    out = np.zeros((pia.n_species, pia.n_cells, 9))
+    lo, hi = (1, pia.n_cells) if cells is None else cells
+    lib().mzo_compute_flux_props(pv_array(pvs), pia.p, _dptr(m), props.ref, _dptr(iv), _dptr(out), lo, hi)
+    return out
 
 
 def compute_props_sorted(pvs, pia, masses, props, inv_V=None, cells=None):

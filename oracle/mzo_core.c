@@ -725,3 +725,36 @@ void mzo_convect_one(mzo_rng *rng, const mzo_grid1d *g, const mzo_walls *wl, mzo
     mzo_rng_set_stream(rng, (uint64_t)(id - 1));
     convect_single(rng, g, wl, PV_X(pv, id), PV_V(pv, id), species, dt);
 }
+
+/* compute_flux_props! (flux_props.jl:77-131), the reference's expressions
+ * left to right: (w C) |C|^2, w C_k^2, w (C_k C_l), then 0.5 q m / V and q m / V */
+void mzo_compute_flux_props(mzo_pv **pvs, const mzo_pia *pia, const double *masses, const mzo_props *pp,
+                            const double *inv_V, double *flux, int64_t cell_lo, int64_t cell_hi) {
+    const int64_t nc = pia->n_cells;
+    for (int64_t s = 1; s <= pia->n_species; ++s) {
+        mzo_pv *pv = pvs[s - 1];
+        for (int64_t c = cell_lo; c <= cell_hi; ++c) {
+            const mzo_ind *d = mzo_pia_at((mzo_pia *)pia, c, s);
+            const double *u = &pp->v[3 * ((s - 1) * nc + (c - 1))];
+            double q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int g = 0; g < 2; ++g) {
+                int64_t a = g == 0 ? d->start1 : d->start2, b = g == 0 ? d->end1 : d->end2;
+                if (g == 1 && d->n_group2 <= 0) continue;
+                for (int64_t i = a; i <= b; ++i) {
+                    const int64_t id = PID(pv, i);
+                    const double w = PV_W(pv, id), *v = PV_V(pv, id);
+                    const double c0 = v[0] - u[0], c1 = v[1] - u[1], c2 = v[2] - u[2];
+                    const double cx = c0 * c0, cy = c1 * c1, cz = c2 * c2;
+                    const double csq = cx + cy + cz;
+                    q[0] = q[0] + w * c0 * csq; q[1] = q[1] + w * c1 * csq; q[2] = q[2] + w * c2 * csq;
+                    q[3] = q[3] + w * cx; q[4] = q[4] + w * cy; q[5] = q[5] + w * cz;
+                    q[6] = q[6] + w * (c0 * c1); q[7] = q[7] + w * (c0 * c2); q[8] = q[8] + w * (c1 * c2);
+                }
+            }
+            double *o = &flux[9 * ((s - 1) * nc + (c - 1))];
+            const double m = masses[s - 1], iv = inv_V[c - 1];
+            for (int j = 0; j < 3; ++j) o[j] = 0.5 * q[j] * m * iv;
+            for (int j = 3; j < 9; ++j) o[j] = q[j] * m * iv;
+        }
+    }
+}
