@@ -945,7 +945,7 @@ def multi_index_moments(n):
 
 
 # NetCDF output in the reference's NCDataHolder layout (io.py), loaded on use
-_IO_NAMES = ("NCDataHolder", "NCDataHolderSurf", "write_netcdf", "close_netcdf")
+_IO_NAMES = ("NCDataHolder", "NCDataHolderSurf", "NCDataHolderFlux", "write_netcdf", "close_netcdf")
 
 
 def __getattr__(name):

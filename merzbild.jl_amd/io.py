@@ -132,6 +132,59 @@ class NCDataHolderSurf:
         self.n_written = 0
 
 
+_FLUX_NAMES = ("kinetic_energy_flux", "diagonal_momentum_flux", "off_diagonal_momentum_flux")
+
+
+class NCDataHolderFlux:
+    """NCDataHolderFlux(nc_filename[, names_skip_list], species_data,
+    flux_props; global_attributes) (io.jl:654-741) for FluxProps time series:
+        timestep                                        [timestep]
+        species_names                                   [n_species, name_strlen] (chars)
+        kinetic_energy_flux, diagonal_momentum_flux,
+        off_diagonal_momentum_flux                      [timestep, n_species, n_cells, vector_components]
+    (the reference's [3, n_cells, n_species, timestep] in column-major
+    order).  names_skip_list takes the names IOSkipListFlux takes
+    (io.jl:612-652); other names are ignored there and rejected here."""
+
+    def __init__(self, nc_filename, species_data, flux_props, names_skip_list=(), global_attributes=None):
+        for name in names_skip_list:
+            if name not in _FLUX_NAMES:
+                raise ValueError(f"cannot skip {name!r}; skippable: {list(_FLUX_NAMES)}")
+        if len(species_data) != flux_props.n_species:
+            raise ValueError("species_data and flux_props disagree on the number of species")
+        self.n_cells, self.n_species = flux_props.n_cells, flux_props.n_species
+        f = netcdf_file(nc_filename, "w", version=2)
+        for k, v in (global_attributes or {}).items():
+            setattr(f, str(k), v)
+        names = [s.name.encode() for s in species_data]
+        strlen = max(1, max(len(n) for n in names))
+        f.createDimension("timestep", None)
+        f.createDimension("vector_components", 3)
+        f.createDimension("n_cells", self.n_cells)
+        f.createDimension("n_species", self.n_species)
+        f.createDimension("name_strlen", strlen)
+        spn = f.createVariable("species_names", "c", ("n_species", "name_strlen"))
+        spn[:] = np.array([list(n.ljust(strlen, b"\0").decode("latin-1")) for n in names], dtype="S1")
+        self.v_timestep = f.createVariable("timestep", "d", ("timestep",))
+        dims = ("timestep", "n_species", "n_cells", "vector_components")
+        self.vars = {name: f.createVariable(name, "d", dims) for name in _FLUX_NAMES if name not in names_skip_list}
+        self.f = f
+        self.n_written = 0
+
+
+def _write_flux(ds, flux_props, timestep, sync_freq):
+    """write_netcdf(ds, flux_props::FluxProps, timestep; sync_freq) (io.jl:759-780)"""
+    if (flux_props.n_cells, flux_props.n_species) != (ds.n_cells, ds.n_species):
+        raise ValueError("flux_props does not match the file's dimensions")
+    t = ds.n_written
+    ds.v_timestep[t] = float(timestep)
+    for name, var in ds.vars.items():
+        var[t] = np.asarray(getattr(flux_props, name), dtype=np.float64)
+    ds.n_written += 1
+    if sync_freq > 0 and ds.n_written % sync_freq == 0:
+        ds.f.flush()
+
+
 def _write_surf(ds, surf_props, timestep, sync_freq):
     """write_netcdf(ds, surf_props::SurfProps, timestep; sync_freq) (io.jl:566-610)"""
     if (surf_props.n_elements, surf_props.n_species) != (ds.n_elements, ds.n_species):
@@ -153,6 +206,8 @@ def write_netcdf(ds, phys_props, timestep, sync_freq=0):
     surface-properties method (io.jl:566-610)."""
     if isinstance(ds, NCDataHolderSurf):
         return _write_surf(ds, phys_props, timestep, sync_freq)
+    if isinstance(ds, NCDataHolderFlux):
+        return _write_flux(ds, phys_props, timestep, sync_freq)
     if (phys_props.n_species, phys_props.n_cells) != (ds.n_species, ds.n_cells):
         raise ValueError("phys_props does not match the file's dimensions")
     t = ds.n_written

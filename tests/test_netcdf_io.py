@@ -143,3 +143,41 @@ def test_surf_skip_list(tmp_path):
         assert set(f.variables) == {"timestep", "species_names", "np", "force", "shear_pressure"}
     with pytest.raises(ValueError):
         io.NCDataHolderSurf(str(tmp_path / "bad.nc"), [_Sp("Ar")], sp, names_skip_list=["T"])
+
+
+def test_flux_round_trip_and_skip_list(tmp_path):
+    """NCDataHolderFlux and write_netcdf of FluxProps (src/io.jl:612-780),
+    as test/test_io_fluxes.jl exercises them: three outputs read back in the
+    reference's layout ([3, n_cells, n_species, timestep] column-major), and
+    the skip list dropping one or two of the three variables."""
+    pkg, io = _pkg(), _io()
+    rng = np.random.default_rng(2)
+    fp = pkg.FluxProps(5, 2)
+    path = str(tmp_path / "flux.nc")
+    ds = io.NCDataHolderFlux(path, [_Sp("Ar"), _Sp("He")], fp)
+    series = []
+    for t in (0, 10, 20):
+        for name in ("kinetic_energy_flux", "diagonal_momentum_flux", "off_diagonal_momentum_flux"):
+            getattr(fp, name)[...] = rng.normal(size=(2, 5, 3))
+        series.append({k: getattr(fp, k).copy() for k in ("kinetic_energy_flux", "diagonal_momentum_flux",
+                                                          "off_diagonal_momentum_flux")})
+        io.write_netcdf(ds, fp, t)
+    io.close_netcdf(ds)
+    with netcdf_file(path, "r", mmap=False) as f:
+        assert list(f.variables["timestep"][:]) == [0.0, 10.0, 20.0]
+        assert f.variables["kinetic_energy_flux"].dimensions == ("timestep", "n_species", "n_cells",
+                                                                 "vector_components")
+        for k, ser in enumerate(series):
+            for name, a in ser.items():
+                assert np.array_equal(f.variables[name][k], a)
+        names = [b"".join(r).rstrip(b"\0").decode() for r in f.variables["species_names"][:]]
+        assert names == ["Ar", "He"]
+    path = str(tmp_path / "skip.nc")
+    ds = io.NCDataHolderFlux(path, [_Sp("Ar"), _Sp("He")], fp, names_skip_list=["kinetic_energy_flux",
+                                                                               "off_diagonal_momentum_flux"])
+    io.write_netcdf(ds, fp, 0)
+    io.close_netcdf(ds)
+    with netcdf_file(path, "r", mmap=False) as f:
+        assert set(f.variables) == {"timestep", "species_names", "diagonal_momentum_flux"}
+    with pytest.raises(ValueError):
+        io.NCDataHolderFlux(str(tmp_path / "bad.nc"), [_Sp("Ar"), _Sp("He")], fp, names_skip_list=["force"])
