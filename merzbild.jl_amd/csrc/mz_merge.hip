@@ -797,7 +797,15 @@ __global__ void __launch_bounds__(256) k_grid_merge(GridArgs A) {
     for (int g = threadIdx.x; g < Nt; g += 256) {
         const int b = gstart[g], e = gstart[g + 1];
         double w = 0.0;
-        for (int j = b; j < e; ++j) w += A.sp.w[oct_slot(ord[j], s1, n1, s2)];
+        // four loads in flight, then the adds in particle order
+        for (int j = b; j < e; j += 4) {
+            double wb[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) wb[k] = j + k < e ? A.sp.w[oct_slot(ord[j + k], s1, n1, s2)] : 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (j + k < e) w += wb[k];
+        }
         const int np = w > 0.0 ? e - b : 0;
         gaux[g] = (np >= 2 ? 2 : np) | ((np > 2 ? 1 : 0) << 16);
     }
@@ -818,22 +826,46 @@ __global__ void __launch_bounds__(256) k_grid_merge(GridArgs A) {
         double ow[2], ov[2][3], ox[2][3];
         if (np > 2) {
             double w = 0.0, vm[3] = {0, 0, 0}, xm[3] = {0, 0, 0};
-            for (int j = b; j < e; ++j) {
-                const int64_t p = oct_slot(ord[j], s1, n1, s2);
-                const double pw = A.sp.w[p];
-                w += pw;
-                vm[0] = vm[0] + A.sp.v0[p] * pw; vm[1] = vm[1] + A.sp.v1[p] * pw; vm[2] = vm[2] + A.sp.v2[p] * pw;
-                xm[0] = xm[0] + A.sp.x0[p] * pw; xm[1] = xm[1] + A.sp.x1[p] * pw; xm[2] = xm[2] + A.sp.x2[p] * pw;
+            // four particles' loads in flight, then the sums in particle order
+            for (int j = b; j < e; j += 4) {
+                double pw[4], pv[4][3], px[4][3];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int64_t p = oct_slot(ord[j + k < e ? j + k : b], s1, n1, s2);
+                    pw[k] = A.sp.w[p];
+                    pv[k][0] = A.sp.v0[p]; pv[k][1] = A.sp.v1[p]; pv[k][2] = A.sp.v2[p];
+                    px[k][0] = A.sp.x0[p]; px[k][1] = A.sp.x1[p]; px[k][2] = A.sp.x2[p];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (j + k >= e) break;
+                    w += pw[k];
+                    for (int d = 0; d < 3; ++d) {
+                        vm[d] = vm[d] + pv[k][d] * pw[k];
+                        xm[d] = xm[d] + px[k][d] * pw[k];
+                    }
+                }
             }
             for (int d = 0; d < 3; ++d) { vm[d] = vm[d] / w; xm[d] = xm[d] / w; }
             double vs[3] = {0, 0, 0}, xs[3] = {0, 0, 0};
-            for (int j = b; j < e; ++j) {
-                const int64_t p = oct_slot(ord[j], s1, n1, s2);
-                const double pw = A.sp.w[p];
-                const double a0 = A.sp.v0[p] - vm[0], a1 = A.sp.v1[p] - vm[1], a2 = A.sp.v2[p] - vm[2];
-                const double c0 = A.sp.x0[p] - xm[0], c1 = A.sp.x1[p] - xm[1], c2 = A.sp.x2[p] - xm[2];
-                vs[0] = vs[0] + (a0 * a0) * pw; vs[1] = vs[1] + (a1 * a1) * pw; vs[2] = vs[2] + (a2 * a2) * pw;
-                xs[0] = xs[0] + (c0 * c0) * pw; xs[1] = xs[1] + (c1 * c1) * pw; xs[2] = xs[2] + (c2 * c2) * pw;
+            for (int j = b; j < e; j += 4) {
+                double pw[4], pv[4][3], px[4][3];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int64_t p = oct_slot(ord[j + k < e ? j + k : b], s1, n1, s2);
+                    pw[k] = A.sp.w[p];
+                    pv[k][0] = A.sp.v0[p]; pv[k][1] = A.sp.v1[p]; pv[k][2] = A.sp.v2[p];
+                    px[k][0] = A.sp.x0[p]; px[k][1] = A.sp.x1[p]; px[k][2] = A.sp.x2[p];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (j + k >= e) break;
+                    for (int d = 0; d < 3; ++d) {
+                        const double a = pv[k][d] - vm[d], c = px[k][d] - xm[d];
+                        vs[d] = vs[d] + (a * a) * pw[k];
+                        xs[d] = xs[d] + (c * c) * pw[k];
+                    }
+                }
             }
             for (int d = 0; d < 3; ++d) { vs[d] = sqrt(vs[d] / w); xs[d] = sqrt(xs[d] / w); }
             MzStream st = mz_stream((uint64_t)c);
