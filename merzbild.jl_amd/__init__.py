@@ -945,7 +945,8 @@ def multi_index_moments(n):
 
 
 # NetCDF output in the reference's NCDataHolder layout (io.py), loaded on use
-_IO_NAMES = ("NCDataHolder", "NCDataHolderSurf", "NCDataHolderFlux", "write_netcdf", "close_netcdf")
+_IO_NAMES = ("NCDataHolder", "NCDataHolderSurf", "NCDataHolderFlux", "write_netcdf", "write_particles_netcdf",
+             "close_netcdf")
 
 
 def __getattr__(name):

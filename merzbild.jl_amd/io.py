@@ -200,10 +200,72 @@ def _write_surf(ds, surf_props, timestep, sync_freq):
         ds.f.flush()
 
 
-def write_netcdf(ds, phys_props, timestep, sync_freq=0):
+def _species_particles(src, species):
+    """(x, v, w, pia table) of one species from a DeviceStore (downloaded)
+    or from a dict with those keys (pia: [n_cells, 7] rows of n_local,
+    start1, end1, n_group1, start2, end2, n_group2, positions 1-based)."""
+    d = src.download_species(species) if hasattr(src, "download_species") else src[species - 1]
+    return np.asarray(d["x"]), np.asarray(d["v"]), np.asarray(d["w"]), np.asarray(d["pia"])
+
+
+def _cellwise(x, v, w, pia):
+    """positions in the reference's output order: per cell, group 1 then
+    group 2 (io.jl:864-890); returns (0-based positions, 1-based cells)"""
+    pos, cells = [], []
+    for c, row in enumerate(pia, start=1):
+        _, s1, e1, _, s2, e2, n2 = (int(t) for t in row)
+        seg = list(range(s1 - 1, e1)) if e1 >= s1 > 0 else []
+        if n2 > 0:
+            seg += list(range(s2 - 1, e2))
+        pos += seg
+        cells += [c] * len(seg)
+    return np.array(pos, dtype=np.int64), np.array(cells, dtype=np.int32)
+
+
+def write_particles_netcdf(nc_filename, particles, species_data, species=None, global_attributes=None):
+    """write_netcdf(nc_filename, pv, pia, species, species_data) (io.jl:842-892)
+    when species is given, else write_netcdf(nc_filename, particles, pia,
+    species_data) (io.jl:906-984): the particles cell by cell (group 1, then
+    group 2).  particles is a DeviceStore or a list of per-species dicts.
+    Dimension and variable names are the reference's ("nparticles_[\"Ar\"]",
+    "w", "v", "x", "cell" for one species; "nparticles_Ar", "w_Ar", ... for
+    all); vectors are stored (3d, nparticles), the reference's (nparticles,
+    3d) Julia arrays in file order.  The cell index is int32: the classic
+    format has no 64-bit integers."""
+    f = netcdf_file(nc_filename, "w", version=2)
+    for k, val in (global_attributes or {}).items():
+        setattr(f, str(k), val)
+    single = species is not None
+    spec = [species] if single else list(range(1, len(species_data) + 1))
+    names = [species_data[s - 1].name.encode() for s in spec]
+    strlen = max(1, max(len(n) for n in names))
+    f.createDimension("3d", 3)
+    f.createDimension("1d" if single else "n_species", len(spec))
+    f.createDimension("name_strlen", strlen)
+    spn = f.createVariable("species_names", "c", ("1d" if single else "n_species", "name_strlen"))
+    spn[:] = np.array([list(n.ljust(strlen, b"\0").decode("latin-1")) for n in names], dtype="S1")
+    for s in spec:
+        x, v, w, pia = _species_particles(particles, s)
+        pos, cells = _cellwise(x, v, w, pia)
+        name = species_data[s - 1].name
+        dim = f'nparticles_["{name}"]' if single else f"nparticles_{name}"
+        sfx = "" if single else f"_{name}"
+        f.createDimension(dim, len(pos))
+        f.createVariable("w" + sfx, "d", (dim,))[:] = w[pos]
+        f.createVariable("v" + sfx, "d", ("3d", dim))[:] = v[pos].T
+        f.createVariable("x" + sfx, "d", ("3d", dim))[:] = x[pos].T
+        f.createVariable("cell" + sfx, "i", (dim,))[:] = cells
+    f.close()
+
+
+def write_netcdf(ds, phys_props, timestep=None, sync_freq=0, **kw):
     """write_netcdf(ds, phys_props, timestep; sync_freq) (io.jl:157-219):
     append one output time; with an NCDataHolderSurf and a SurfProps, the
-    surface-properties method (io.jl:566-610)."""
+    surface-properties method (io.jl:566-610).  With a file name first,
+    write_netcdf(nc_filename, particles, species_data[, species]) writes a
+    particle dump (write_particles_netcdf)."""
+    if isinstance(ds, str):
+        return write_particles_netcdf(ds, phys_props, timestep, **kw)
     if isinstance(ds, NCDataHolderSurf):
         return _write_surf(ds, phys_props, timestep, sync_freq)
     if isinstance(ds, NCDataHolderFlux):

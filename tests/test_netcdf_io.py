@@ -181,3 +181,34 @@ def test_flux_round_trip_and_skip_list(tmp_path):
         assert set(f.variables) == {"timestep", "species_names", "diagonal_momentum_flux"}
     with pytest.raises(ValueError):
         io.NCDataHolderFlux(str(tmp_path / "bad.nc"), [_Sp("Ar"), _Sp("He")], fp, names_skip_list=["force"])
+
+
+def test_particle_dump(tmp_path):
+    """write_netcdf(nc_filename, pv, pia, species, species_data) and the
+    all-species form (src/io.jl:842-984): particles cell by cell, group 1
+    then group 2, with their cells; the reference's dimension and variable
+    names."""
+    pkg, io = _pkg(), _io()
+    rng = np.random.default_rng(9)
+    n = 10
+    x, v, w = rng.normal(size=(n, 3)), rng.normal(size=(n, 3)), rng.uniform(1, 2, n)
+    # cell 1: positions 1-2 and 9-10; cell 2: none; cell 3: positions 3-8
+    pia = np.array([[4, 1, 2, 2, 9, 10, 2], [0, 0, -1, 0, 0, -1, 0], [6, 3, 8, 6, 0, -1, 0]])
+    sp = [{"x": x, "v": v, "w": w, "pia": pia}, {"x": x[:3], "v": v[:3], "w": w[:3],
+                                                  "pia": np.array([[0, 0, -1, 0, 0, -1, 0], [3, 1, 3, 3, 0, -1, 0],
+                                                                   [0, 0, -1, 0, 0, -1, 0]])}]
+    order = [0, 1, 8, 9, 2, 3, 4, 5, 6, 7]
+    path = str(tmp_path / "p1.nc")
+    io.write_netcdf(path, sp, [_Sp("Ar"), _Sp("He")], species=1)
+    with netcdf_file(path, "r", mmap=False) as f:
+        assert f.dimensions['nparticles_["Ar"]'] == n
+        assert np.array_equal(f.variables["w"][:], w[order])
+        assert np.array_equal(f.variables["v"][:], v[order].T) and np.array_equal(f.variables["x"][:], x[order].T)
+        assert list(f.variables["cell"][:]) == [1, 1, 1, 1, 3, 3, 3, 3, 3, 3]
+    path = str(tmp_path / "p2.nc")
+    io.write_particles_netcdf(path, sp, [_Sp("Ar"), _Sp("He")])
+    with netcdf_file(path, "r", mmap=False) as f:
+        assert f.dimensions["nparticles_Ar"] == n and f.dimensions["nparticles_He"] == 3
+        assert np.array_equal(f.variables["w_Ar"][:], w[order]) and list(f.variables["cell_He"][:]) == [2, 2, 2]
+        names = [b"".join(r).rstrip(b"\0").decode() for r in f.variables["species_names"][:]]
+        assert names == ["Ar", "He"]
