@@ -211,6 +211,27 @@ def load_interaction_data(path, species_data):
     return out
 
 
+def mean_free_path(interaction, species, n, T):
+    """mean_free_path(interaction, species, n, T)
+    (src/properties/collisional_props.jl:11-15): the VHS single-species
+    mean free path 1 / (sqrt(2) pi d^2 n (Tref / T)^(omega - 1/2))."""
+    it = interaction[species - 1][species - 1]
+    lam = np.sqrt(2.0) * np.pi * (it.vhs_d ** 2) * n
+    lam *= (it.vhs_Tref / T) ** (it.vhs_o - 0.5)
+    return 1.0 / lam
+
+
+def mean_collision_frequency(interaction, species_data, species, n, T):
+    """mean_collision_frequency(interaction, species_data, species, n, T)
+    (src/properties/collisional_props.jl:29-34): 4 d^2 n sqrt(pi k Tref / m)
+    (T / Tref)^(1 - omega)."""
+    it = interaction[species - 1][species - 1]
+    nu = 4.0 * (it.vhs_d ** 2) * n
+    nu *= np.sqrt(np.pi * K_B * it.vhs_Tref / species_data[species - 1].mass)
+    nu *= (T / it.vhs_Tref) ** (1.0 - it.vhs_o)
+    return nu
+
+
 def estimate_sigma_g_w_max(interaction, m1, m2, T1, T2, Fnum, mult_factor=1.0):
     """estimate_sigma_g_w_max (src/collisions/collision_utils.jl:418-424)"""
     g1 = np.sqrt(2 * T1 * K_B / m1)

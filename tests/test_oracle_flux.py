@@ -87,3 +87,25 @@ def test_flux_host_clear_and_avg(mzo):
     assert np.all(avg.off_diagonal_momentum_flux == 0.0)
     mz.clear_props(fp)
     assert not fp.kinetic_energy_flux.any() and not fp.off_diagonal_momentum_flux.any()
+
+
+def test_mean_free_path_and_collision_frequency():
+    """mean_free_path and mean_collision_frequency (collisional_props.jl):
+    at T = Tref the hard-sphere values 1 / (sqrt(2) pi d^2 n) and n pi d^2
+    <g> with <g> = sqrt(16 k T / (pi m)); their ratio is the mean speed
+    sqrt(8 k T / (pi m)); and the VHS temperature exponents."""
+    from test_oracle_surf import _pkg
+    mz = _pkg()
+    import os
+    data = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "merzbild.jl_amd", "data")
+    sp = mz.load_species_data(os.path.join(data, "particles.toml"), ["Ar"])
+    it = mz.load_interaction_data(os.path.join(data, "vhs.toml"), sp)
+    i0 = it[0][0]
+    d, Tref, om, m, n, kb = i0.vhs_d, i0.vhs_Tref, i0.vhs_o, sp[0].mass, 1e23, 1.380649e-23
+    lam = mz.mean_free_path(it, 1, n, Tref)
+    nu = mz.mean_collision_frequency(it, sp, 1, n, Tref)
+    assert abs(lam * np.sqrt(2) * np.pi * d * d * n - 1) < 1e-14
+    assert abs(nu / (n * np.pi * d * d * np.sqrt(16 * kb * Tref / (np.pi * m))) - 1) < 1e-14
+    assert abs(nu * lam / np.sqrt(8 * kb * Tref / (np.pi * m)) - 1) < 1e-14
+    assert abs(mz.mean_free_path(it, 1, n, 4 * Tref) / lam - 4.0 ** (om - 0.5)) < 1e-13
+    assert abs(mz.mean_collision_frequency(it, sp, 1, n, 4 * Tref) / nu - 4.0 ** (1 - om)) < 1e-13
