@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     __shared__ OctBin bins[OCT_MAX_BINS];
     __shared__ double red[4];
     __shared__ double vel_mid[3];
-    __shared__ int32_t cnt8[9], start8[9], run8[9], nonempty[9], ne_cnt[9];
+    __shared__ int32_t cnt8[9], start8[9], run8[9];
     __shared__ double ndens8[9];
     __shared__ int32_t Nbins_s, total_post_s, stop_s;
 
@@ -305,40 +305,51 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                 ndens8[o] = sw;
             }
             oct_wave_sync();
-            if (lane == 0) {
-                int n_ne = 0;
-                for (int o = 1; o <= 8; ++o)
-                    if (cnt8[o] > 0) { ++n_ne; nonempty[n_ne] = o; ne_cnt[n_ne] = cnt8[o]; }
+            // the children (split_bin!, :588-633), lane o for octant o: the
+            // k-th non-empty octant becomes bin b (k = 1) or Nb + k - 2
+            // (get_new_bin_id), its particles the octant's segment
+            {
                 const int Nb = Nbins_s;
                 double pmin[3], pmax[3];
                 for (int d = 0; d < 3; ++d) { pmin[d] = bins[b].vmin[d]; pmax[d] = bins[b].vmax[d]; }
-                int prev_end = bs - 1;
-                total_post_s -= 2;
-                for (int k = 1; k <= n_ne; ++k) {
-                    const int bi = k == 1 ? b : Nb + k - 2;   // child 1 keeps the id (get_new_bin_id)
-                    if (bi >= OCT_MAX_BINS) { atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE); stop_s = 1; break; }
-                    OctBin &nb = bins[bi];
-                    nb.start = prev_end + 1;
-                    nb.end = nb.start + ne_cnt[k] - 1;
-                    prev_end = nb.end;
-                    const int oc = nonempty[k];
-                    if (A.bounds_mode == 1) {   // bin_bounds_inherit! (:341-367)
-                        const int sx = (oc - 1) & 1, sy = ((oc - 1) >> 1) & 1, sz = ((oc - 1) >> 2) & 1;
-                        nb.vmin[0] = sx ? vel_mid[0] : pmin[0]; nb.vmax[0] = sx ? pmax[0] : vel_mid[0];
-                        nb.vmin[1] = sy ? vel_mid[1] : pmin[1]; nb.vmax[1] = sy ? pmax[1] : vel_mid[1];
-                        nb.vmin[2] = sz ? vel_mid[2] : pmin[2]; nb.vmax[2] = sz ? pmax[2] : vel_mid[2];
+                const int o = lane;
+                const bool ne = o >= 1 && o <= 8 && cnt8[o] > 0;
+                const uint64_t m = __ballot(ne);
+                const int n_ne = __popcll(m);
+                int contrib = 0;
+                oct_wave_sync();   // every lane has read bin b before child 1 overwrites it
+                if (ne) {
+                    const int k = __popcll(m & mz_lanemask_lt()) + 1;
+                    const int bi = k == 1 ? b : Nb + k - 2;
+                    if (bi < OCT_MAX_BINS) {
+                        OctBin &nb = bins[bi];
+                        nb.start = start8[o];
+                        nb.end = start8[o] + cnt8[o] - 1;
+                        if (A.bounds_mode == 1) {   // bin_bounds_inherit! (:341-367)
+                            const int sx = (o - 1) & 1, sy = ((o - 1) >> 1) & 1, sz = ((o - 1) >> 2) & 1;
+                            nb.vmin[0] = sx ? vel_mid[0] : pmin[0]; nb.vmax[0] = sx ? pmax[0] : vel_mid[0];
+                            nb.vmin[1] = sy ? vel_mid[1] : pmin[1]; nb.vmax[1] = sy ? pmax[1] : vel_mid[1];
+                            nb.vmin[2] = sz ? vel_mid[2] : pmin[2]; nb.vmax[2] = sz ? pmax[2] : vel_mid[2];
+                        }
+                        nb.np = cnt8[o];
+                        nb.w = ndens8[o];
+                        nb.depth = depth + 1;
+                        nb.refine = (cnt8[o] > 2) && (depth + 1 < A.max_depth);
+                    } else {
+                        atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
                     }
-                    nb.np = ne_cnt[k];
-                    nb.w = ndens8[oc];
-                    nb.depth = depth + 1;
-                    total_post_s += nb.np >= 2 ? 2 : nb.np;
-                    nb.refine = (nb.np > 2) && (nb.depth < A.max_depth);
+                    contrib = cnt8[o] >= 2 ? 2 : cnt8[o];
                 }
-                Nbins_s = Nb + n_ne - 1;
-                if (Nbins_s + 7 > A.max_Nbins) stop_s = 1;
-                else if (Nbins_s + 7 > max_bins) {   // LDS bin budget below the caller's max_Nbins
-                    atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
-                    stop_s = 1;
+                const int added = mz_wave_sum_i(contrib);
+                if (lane == 0) {
+                    total_post_s += added - 2;
+                    Nbins_s = Nb + n_ne - 1;
+                    if (Nb + n_ne - 2 >= OCT_MAX_BINS) stop_s = 1;
+                    else if (Nbins_s + 7 > A.max_Nbins) stop_s = 1;
+                    else if (Nbins_s + 7 > max_bins) {   // LDS bin budget below the caller's max_Nbins
+                        atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
+                        stop_s = 1;
+                    }
                 }
             }
             oct_wave_sync();
