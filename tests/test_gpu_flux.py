@@ -89,3 +89,22 @@ def test_flux_vs_oracle_two_species(hip, mzo, cells):
     st.flux_clear(1)
     assert not hip.FluxProps(nc, 2).fetch(st, 1).diagonal_momentum_flux.any()
     st.close()
+
+
+def test_particle_dump_from_store(hip, mzo, tmp_path):
+    """write_netcdf(nc_filename, pv, pia, species, species_data) (io.jl:842-892)
+    from a device store: the particles cell by cell as the store holds them."""
+    from scipy.io import netcdf_file
+    pv, pia = _kat(mzo)
+    st = hip.DeviceStore(8, 1, 64)
+    helpers.upload_from_oracle(st, mzo, pv, pia)
+    sp = [hip.Species("Ar", M_AR)]
+    path = str(tmp_path / "dump.nc")
+    hip.write_netcdf(path, st, sp, species=1)
+    got = st.download_species(1)
+    order = [0, 1, 2, 3, 4, 5]     # cell 1: group 1 (1-2) then group 2 (3-4); cell 6: 5-6
+    with netcdf_file(path, "r", mmap=False) as f:
+        assert np.array_equal(f.variables["w"][:], got["w"][order])
+        assert np.array_equal(f.variables["v"][:], got["v"][order].T)
+        assert list(f.variables["cell"][:]) == [1, 1, 1, 1, 6, 6]
+    st.close()
