@@ -712,7 +712,7 @@ __device__ int32_t gm_scan(int32_t *a, int n, int32_t *wsc) {
 }
 
 __global__ void __launch_bounds__(256) k_grid_merge(GridArgs A) {
-    __shared__ int16_t gidx[GM_MAX_N];     // velocity cell (0-based) of logical particle li
+    __shared__ uint32_t sk[GM_MAX_N];      // (velocity cell << 11) | li, sorted into velocity-cell order
     __shared__ int16_t ord[GM_MAX_N];      // logical particles grouped by velocity cell, in particle order
     __shared__ int32_t gstart[GM_MAX_CELLS + 1];
     __shared__ int32_t gaux[GM_MAX_CELLS + 1];   // counts, then running slots, then packed outputs/draws
@@ -771,7 +771,7 @@ __global__ void __launch_bounds__(256) k_grid_merge(GridArgs A) {
             k = Nt - 8 + (v0 > mid[0] ? 1 : 0) + (v1 > mid[1] ? 2 : 0) + (v2 > mid[2] ? 4 : 0);
         }
         if (k > Nt - 1) k = Nt - 1;
-        gidx[li] = (int16_t)k;
+        sk[li] = ((uint32_t)k << 11) | (uint32_t)li;
         atomicAdd(&gaux[k], 1);
     }
     __syncthreads();
@@ -779,29 +779,26 @@ __global__ void __launch_bounds__(256) k_grid_merge(GridArgs A) {
     __syncthreads();
     gm_scan(gstart, Nt, wsc);
     if (threadIdx.x == 0) gstart[Nt] = N;
-    for (int g = threadIdx.x; g < Nt; g += 256) gaux[g] = gstart[g];
+    // (velocity cell, particle) keys in ascending order: a bitonic sort over
+    // the next power of two, so each velocity cell's particles come out in
+    // particle order
+    int npad = 1;
+    while (npad < N) npad <<= 1;
+    for (int li = N + threadIdx.x; li < npad; li += 256) sk[li] = 0xFFFFFFFFu;
     __syncthreads();
-    // stable ranking by velocity cell (wave 0, 64 particles at a time)
-    if (threadIdx.x < 64) {
-        volatile int32_t *vrun = gaux;
-        const int lane = threadIdx.x;
-        const uint64_t lt = mz_lanemask_lt();
-        for (int b0 = 0; b0 < N; b0 += 64) {
-            const int li = b0 + lane;
-            const bool valid = li < N;
-            const int32_t key = valid ? gidx[li] : -1;
-            uint64_t active = __ballot(valid);
-            while (active) {
-                const int leader = __ffsll((unsigned long long)active) - 1;
-                const int32_t k = __shfl(key, leader, 64);
-                const uint64_t m = __ballot(valid && key == k);
-                const int32_t bk = vrun[k];
-                if (valid && key == k) ord[bk + __popcll(m & lt)] = (int16_t)li;
-                vrun[k] = bk + __popcll(m);
-                active &= ~m;
+    for (int len = 2; len <= npad; len <<= 1) {
+        for (int stride = len >> 1; stride > 0; stride >>= 1) {
+            for (int t = threadIdx.x; t < (npad >> 1); t += 256) {
+                const int lo = 2 * t - (t & (stride - 1));
+                const int hi = lo + stride;
+                const bool up = (lo & len) == 0;
+                const uint32_t a = sk[lo], c = sk[hi];
+                if ((a > c) == up) { sk[lo] = c; sk[hi] = a; }
             }
+            __syncthreads();
         }
     }
+    for (int j = threadIdx.x; j < N; j += 256) ord[j] = (int16_t)(sk[j] & 0x7FFu);
     __syncthreads();
     // pass A: each velocity cell's weight in particle order; outputs
     // min(np, 2) (none when the weight is not positive), draws 1 if np > 2
