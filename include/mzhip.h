@@ -268,9 +268,13 @@ int mz_get_nnls_info(mz_store *h, int32_t *info);
  * (test/test_1D_couette.jl:66-97).  Step t uses tags (t << 4) | 1 for NTC
  * and (t << 4) | 2 for convection.  mode 0 launches kernels one by one,
  * mode 1 replays a captured hipGraph of the step, mode 2 runs all steps in
- * one cooperative launch with each slab of cells resident in LDS (equal
- * weights; it falls back to mode 1 when the state is not the sorted state a
- * sort leaves, or when a particle crosses more than one slab in a step).
+ * one launch with each slab of cells resident in LDS (equal weights; it
+ * falls back to mode 1 when the state is not the sorted state a sort leaves,
+ * or when a particle crosses more than one slab in a step).  Mode 2 needs
+ * every slab resident at once: the caller keeps other kernels of this
+ * process off the device meanwhile (the library itself drains the device
+ * first when another store's stream is busy).  Per call the host enqueues a
+ * check-and-backup kernel, the launch and a report kernel, and waits once.
  * The chunked loop this replaces (collide, convect, sort per chunk, exchange
  * between chunks, re-sort and props) is simulations/1D/
  * couette_multithreaded.jl:97-161. */

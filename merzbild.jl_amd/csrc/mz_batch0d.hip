@@ -469,10 +469,9 @@ static int batch0d_launch(mz_store *h, int species, int32_t n_steps, uint64_t st
     h->b0_info[1] = maxc;
     h->b0_info[2] = (int32_t)target;
     h->b0_info[3] = grid;
-    mz_timing_begin(h, "batch0d");
+    MzTimed tm(h, "batch0d", n_steps);
     hipLaunchKernelGGL(k_batch0d, dim3(grid), dim3(256), 0, h->stream, A);
     MZ_HIP(hipGetLastError());
-    mz_timing_end_n(h, n_steps);
     return 0;
 }
 
@@ -485,6 +484,7 @@ extern "C" int mz_batch0d_steps(mz_store *h, int32_t species, int32_t n_steps, u
     MZ_CHECK(mode >= 0 && mode <= 2,
              "batch0d: mode must be 0 (per-step kernels), 1 (persistent) or 2 (one NTC launch per cell)");
     MZ_HIP(hipMemcpy(h->d_masses, masses, h->n_species * sizeof(double), hipMemcpyHostToDevice));
+    h->masses_valid = false;   // the couette drivers upload their mass again
     const double *dV = h->d_V;
     if (V) {
         double *v = (double *)mz_scratch(h, "b0_V", h->n_cells * sizeof(double));
@@ -499,7 +499,7 @@ extern "C" int mz_batch0d_steps(mz_store *h, int32_t species, int32_t n_steps, u
     // mode 0: per step the NTC kernel over all cells and the props kernel;
     // mode 2: per step one NTC launch per cell (the reference's loop of
     // ntc!(..., cell, ...) calls, each a drop-in launch), then the props kernel
-    mz_timing_begin(h, "batch0d");
+    MzTimed tm(h, "batch0d", n_steps);
     for (int32_t t = 0; t < n_steps; ++t) {
         const uint64_t step = step0 + (uint64_t)t;
         if (mode == 2) {
@@ -517,6 +517,5 @@ extern "C" int mz_batch0d_steps(mz_store *h, int32_t species, int32_t n_steps, u
             if (r) return r;
         }
     }
-    mz_timing_end_n(h, n_steps);
     return 0;
 }

@@ -171,6 +171,10 @@ struct MzSpecies {
     int64_t bufcap;
     MzPia pia;
     MzCounters *ctr;    // device
+    // host-side knowledge: true when ctr->contiguous is known to be 1 (set by
+    // the calls that leave the species sorted or squashed, cleared by every
+    // call that may clear it), so those calls need not read it back
+    bool contig_known = false;
 };
 
 // Plain-old-data view of a species passed to kernels by value.

@@ -49,6 +49,14 @@ struct MzScratch {
     size_t bytes = 0;
 };
 
+// The slab launch's shape, kept between calls (see mz_couette_slab).
+struct MzSlabShapeCache {
+    bool valid = false, avg = false;
+    int32_t req_spc = 0, req_ppt = 0;   // the requested shape it was picked for
+    int R = 0, spc = 0, n_slabs = 0, spx = 0, grid = 0;
+    const void *fn = nullptr;           // nullptr: no shape fits
+};
+
 struct mz_store {
     int n_cells = 0, n_species = 0;
     std::vector<MzSpecies> sp;
@@ -83,6 +91,13 @@ struct mz_store {
     // last launch's {particles per thread, cells per slab, slabs, outcome}
     int32_t slab_spc = 0, slab_ppt = 0;
     int32_t slab_info[4] = {0, 0, 0, 0};
+    MzSlabShapeCache slab_shape;
+    int32_t slab_min_ppt = 0;      // floor on particles per thread, raised after an aborted launch
+    int32_t *h_report = nullptr;   // pinned host words the device reports outcomes into
+    // host copies of what mz_couette_steps last uploaded (skips unchanged uploads)
+    double masses_up[MZ_MAX_SPECIES] = {0.0, 0.0, 0.0, 0.0};
+    bool masses_valid = false;
+    std::vector<double> V_up;
     int32_t last_path = -1;   // of mz_couette_steps: 0 eager, 1 graph, 2 slab
     int32_t n_moments = 0;    // moments of the last mz_compute_props_moments
     int32_t sort_path = 0;    // 0 automatic, 1 count matrix, 2 radix (mz_set_sort_path)
@@ -108,6 +123,26 @@ int mz_read_counters(mz_store *h, int species, MzCounters *c);
 MzRngParams mz_rng_params(mz_store *h, int kind, uint64_t tag, int mode);
 int mz_timing_begin(mz_store *h, const char *family);
 int mz_timing_end(mz_store *h);
+int mz_timing_end_n(mz_store *h, int64_t n);
+
+// A timing bracket closed on every path out of its scope, early returns
+// included, so an error between the two events cannot leave it open.
+struct MzTimed {
+    mz_store *h;
+    int64_t n;
+    bool open;
+    MzTimed(mz_store *h_, const char *family, int64_t n_ = 1) : h(h_), n(n_) {
+        open = mz_timing_begin(h, family) == 0 && h->timing;
+    }
+    ~MzTimed() {
+        if (open) mz_timing_end_n(h, n);
+    }
+    MzTimed(const MzTimed &) = delete;
+    MzTimed &operator=(const MzTimed &) = delete;
+};
+
+// true when another live store's stream still has work queued
+bool mz_other_stores_busy(mz_store *h);
 void mz_swap_buffers(MzSpecies &s);
 void mz_graph_release(mz_store *h);
 
@@ -124,6 +159,5 @@ int mz_surf_prealloc(mz_store *h, int species);
 int mz_launch_ntc_equal(mz_store *h, int s1, int s2, double dt, const double *dV, double dw_tol,
                         int equal_weight, uint64_t tag, int rng_mode, int cell_lo, int cell_hi,
                         const uint64_t *step_ptr = nullptr);
-int mz_timing_end_n(mz_store *h, int64_t n);
 int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *dV, int32_t props_every,
-                    int *ran, double avg_inv = 0.0, double mass = 0.0);
+                    int *ran, double avg_inv = 0.0, double mass = 0.0, int retry = 0);

@@ -601,6 +601,7 @@ extern "C" int mz_merge_octree(mz_store *h, int32_t species, int32_t threshold, 
                                int32_t init_bounds, int32_t bounds_mode, int32_t max_Nbins, int32_t max_depth,
                                int32_t grid_clamp, int32_t squash, uint64_t tag, int32_t rng_mode,
                                int64_t *n_merged) {
+    if (species >= 1 && species <= h->n_species) h->sp[species - 1].contig_known = false;
     MZ_CHECK(species >= 1 && species <= h->n_species, "merge_octree: bad species");
     MZ_CHECK(split == 1 || split == 2, "merge_octree: split must be 1 (mid) or 2 (mean)");
     MZ_CHECK(init_bounds >= 1 && init_bounds <= 3, "merge_octree: bad init_bounds");
@@ -632,13 +633,14 @@ extern "C" int mz_merge_octree(mz_store *h, int32_t species, int32_t threshold, 
     A.old_pia = old_pia;
     A.n_merged = cnt;
     A.n_cells = nc;
-    mz_timing_begin(h, "merge_octree");
-    hipLaunchKernelGGL(k_octree_merge, dim3(nc), dim3(256), 0, h->stream, A);
-    hipLaunchKernelGGL(k_merge_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, old_pia, nc, ndel);
-    hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
-    hipLaunchKernelGGL(k_merge_push_ids, dim3(nc), dim3(64), 0, h->stream, A.sp, old_pia, off, off + nc,
-                       snap.nbuffer, snap.n_total, nc);
-    mz_timing_end(h);
+    {
+        MzTimed tm(h, "merge_octree");
+        hipLaunchKernelGGL(k_octree_merge, dim3(nc), dim3(256), 0, h->stream, A);
+        hipLaunchKernelGGL(k_merge_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, old_pia, nc, ndel);
+        hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
+        hipLaunchKernelGGL(k_merge_push_ids, dim3(nc), dim3(64), 0, h->stream, A.sp, old_pia, off, off + nc,
+                           snap.nbuffer, snap.n_total, nc);
+    }
     MZ_HIP(hipGetLastError());
     if (n_merged) {
         int32_t v = 0;
@@ -910,6 +912,7 @@ extern "C" int mz_merge_grid(mz_store *h, int32_t species, int32_t threshold, in
                              const double *extent_multiplier, const double *extents, double mass,
                              int32_t grid_clamp, int32_t squash, uint64_t tag, int32_t rng_mode,
                              int64_t *n_merged) {
+    if (species >= 1 && species <= h->n_species) h->sp[species - 1].contig_known = false;
     MZ_CHECK(species >= 1 && species <= h->n_species, "merge_grid: bad species");
     MZ_CHECK(Nx >= 1 && Ny >= 1 && Nz >= 1 && (int64_t)Nx * Ny * Nz + 8 <= GM_MAX_CELLS,
              "merge_grid: Nx Ny Nz must be positive with Nx Ny Nz + 8 <= 4104");
@@ -951,13 +954,14 @@ extern "C" int mz_merge_grid(mz_store *h, int32_t species, int32_t threshold, in
     A.old_pia = old_pia;
     A.n_merged = cnt;
     A.n_cells = nc;
-    mz_timing_begin(h, "merge_grid");
-    hipLaunchKernelGGL(k_grid_merge, dim3(nc), dim3(256), 0, h->stream, A);
-    hipLaunchKernelGGL(k_merge_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, old_pia, nc, ndel);
-    hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
-    hipLaunchKernelGGL(k_merge_push_ids, dim3(nc), dim3(64), 0, h->stream, A.sp, old_pia, off, off + nc,
-                       snap.nbuffer, snap.n_total, nc);
-    mz_timing_end(h);
+    {
+        MzTimed tm(h, "merge_grid");
+        hipLaunchKernelGGL(k_grid_merge, dim3(nc), dim3(256), 0, h->stream, A);
+        hipLaunchKernelGGL(k_merge_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, old_pia, nc, ndel);
+        hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
+        hipLaunchKernelGGL(k_merge_push_ids, dim3(nc), dim3(64), 0, h->stream, A.sp, old_pia, off, off + nc,
+                           snap.nbuffer, snap.n_total, nc);
+    }
     MZ_HIP(hipGetLastError());
     if (n_merged) {
         int32_t v = 0;
@@ -2290,76 +2294,77 @@ static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, cons
         for (int c = 0; c < nc; ++c) init[(size_t)c * 5 + 4] = -1;
         MZ_HIP(hipMemcpy(old_pia, init.data(), init.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
-    mz_timing_begin(h, "merge_nnls");
-    if (!cells.empty() && method >= 1) {
-        int32_t *dcells = (int32_t *)mz_scratch(h, "nnls_cells", cells.size() * sizeof(int32_t));
-        MZ_CHECK(dcells, "merge_nnls: scratch allocation failed");
-        MZ_HIP(hipMemcpy(dcells, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        A.status = d_status;
-        A.old_pia = old_pia;
-        auto launch = [&](bool mixed, const int32_t *list, size_t count) {
-            A.cells = list;
-            const dim3 g((unsigned)count), b(NF_COLS);
-            if (m <= 35) {
-                if (mixed) hipLaunchKernelGGL((k_nnls_fast<35, float>), g, b, 0, h->stream, A);
-                else hipLaunchKernelGGL((k_nnls_fast<35, double>), g, b, 0, h->stream, A);
-            } else {   // more than 35 moments: FP64 only (the FP32 matrix area cannot hold the FP64 system)
-                hipLaunchKernelGGL((k_nnls_fast<NNLS_MAX_ROWS, double>), g, b, 0, h->stream, A);
-            }
-            return hipGetLastError();
-        };
-        const bool mixed = method == 2 && m <= 35;
-        MZ_HIP(launch(mixed, dcells, cells.size()));
-        h->nnls_info[0] = (int32_t)cells.size();
-        if (mixed) {
-            // cells the FP64 re-solve did not confirm run the FP64 method
-            std::vector<int32_t> st(nc);
-            MZ_HIP(hipStreamSynchronize(h->stream));
-            MZ_HIP(hipMemcpy(st.data(), d_status, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
-            std::vector<int32_t> redo;
-            for (int32_t cc : cells)
-                if (st[cc] == 2) redo.push_back(cc);
-            h->nnls_info[1] = (int32_t)redo.size();
-            if (!redo.empty()) {
-                int32_t *dredo = (int32_t *)mz_scratch(h, "nnls_cells_fp64", redo.size() * sizeof(int32_t));
-                MZ_CHECK(dredo, "merge_nnls: scratch allocation failed");
-                MZ_HIP(hipMemcpy(dredo, redo.data(), redo.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-                MZ_HIP(launch(false, dredo, redo.size()));
-            }
-        }
-    } else if (!cells.empty()) {
-        // batches bounded to ~1 GiB of matrix scratch
-        const int64_t budget = (int64_t)1 << 27;   // doubles
-        size_t i0 = 0;
-        while (i0 < cells.size()) {
-            size_t i1 = i0;
-            const int64_t base_off = offs[i0];
-            while (i1 < cells.size() && (i1 == i0 || offs[i1] - base_off + 1 < budget)) ++i1;
-            const int64_t span = (i1 < cells.size() ? offs[i1] : tot) - base_off;
-            double *scr = (double *)mz_scratch(h, "nnls_A", (size_t)span * sizeof(double));
-            int32_t *dcells = (int32_t *)mz_scratch(h, "nnls_cells", (i1 - i0) * sizeof(int32_t));
-            int64_t *doffs = (int64_t *)mz_scratch(h, "nnls_offs", (i1 - i0) * sizeof(int64_t));
-            MZ_CHECK(scr && dcells && doffs, "merge_nnls: scratch allocation failed");
-            std::vector<int64_t> lo(offs.begin() + i0, offs.begin() + i1);
-            for (auto &v : lo) v -= base_off;
-            MZ_HIP(hipMemcpy(dcells, cells.data() + i0, (i1 - i0) * sizeof(int32_t), hipMemcpyHostToDevice));
-            MZ_HIP(hipMemcpy(doffs, lo.data(), (i1 - i0) * sizeof(int64_t), hipMemcpyHostToDevice));
-            A.cells = dcells;
-            A.aoff = doffs;
-            A.scratch = scr;
+    {
+        MzTimed tm(h, "merge_nnls");
+        if (!cells.empty() && method >= 1) {
+            int32_t *dcells = (int32_t *)mz_scratch(h, "nnls_cells", cells.size() * sizeof(int32_t));
+            MZ_CHECK(dcells, "merge_nnls: scratch allocation failed");
+            MZ_HIP(hipMemcpy(dcells, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice));
             A.status = d_status;
             A.old_pia = old_pia;
-            hipLaunchKernelGGL(k_nnls_merge, dim3((unsigned)(i1 - i0)), dim3(256), 0, h->stream, A);
-            MZ_HIP(hipGetLastError());
-            MZ_HIP(hipStreamSynchronize(h->stream));
-            i0 = i1;
+            auto launch = [&](bool mixed, const int32_t *list, size_t count) {
+                A.cells = list;
+                const dim3 g((unsigned)count), b(NF_COLS);
+                if (m <= 35) {
+                    if (mixed) hipLaunchKernelGGL((k_nnls_fast<35, float>), g, b, 0, h->stream, A);
+                    else hipLaunchKernelGGL((k_nnls_fast<35, double>), g, b, 0, h->stream, A);
+                } else {   // more than 35 moments: FP64 only (the FP32 matrix area cannot hold the FP64 system)
+                    hipLaunchKernelGGL((k_nnls_fast<NNLS_MAX_ROWS, double>), g, b, 0, h->stream, A);
+                }
+                return hipGetLastError();
+            };
+            const bool mixed = method == 2 && m <= 35;
+            MZ_HIP(launch(mixed, dcells, cells.size()));
+            h->nnls_info[0] = (int32_t)cells.size();
+            if (mixed) {
+                // cells the FP64 re-solve did not confirm run the FP64 method
+                std::vector<int32_t> st(nc);
+                MZ_HIP(hipStreamSynchronize(h->stream));
+                MZ_HIP(hipMemcpy(st.data(), d_status, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
+                std::vector<int32_t> redo;
+                for (int32_t cc : cells)
+                    if (st[cc] == 2) redo.push_back(cc);
+                h->nnls_info[1] = (int32_t)redo.size();
+                if (!redo.empty()) {
+                    int32_t *dredo = (int32_t *)mz_scratch(h, "nnls_cells_fp64", redo.size() * sizeof(int32_t));
+                    MZ_CHECK(dredo, "merge_nnls: scratch allocation failed");
+                    MZ_HIP(hipMemcpy(dredo, redo.data(), redo.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+                    MZ_HIP(launch(false, dredo, redo.size()));
+                }
+            }
+        } else if (!cells.empty()) {
+            // batches bounded to ~1 GiB of matrix scratch
+            const int64_t budget = (int64_t)1 << 27;   // doubles
+            size_t i0 = 0;
+            while (i0 < cells.size()) {
+                size_t i1 = i0;
+                const int64_t base_off = offs[i0];
+                while (i1 < cells.size() && (i1 == i0 || offs[i1] - base_off + 1 < budget)) ++i1;
+                const int64_t span = (i1 < cells.size() ? offs[i1] : tot) - base_off;
+                double *scr = (double *)mz_scratch(h, "nnls_A", (size_t)span * sizeof(double));
+                int32_t *dcells = (int32_t *)mz_scratch(h, "nnls_cells", (i1 - i0) * sizeof(int32_t));
+                int64_t *doffs = (int64_t *)mz_scratch(h, "nnls_offs", (i1 - i0) * sizeof(int64_t));
+                MZ_CHECK(scr && dcells && doffs, "merge_nnls: scratch allocation failed");
+                std::vector<int64_t> lo(offs.begin() + i0, offs.begin() + i1);
+                for (auto &v : lo) v -= base_off;
+                MZ_HIP(hipMemcpy(dcells, cells.data() + i0, (i1 - i0) * sizeof(int32_t), hipMemcpyHostToDevice));
+                MZ_HIP(hipMemcpy(doffs, lo.data(), (i1 - i0) * sizeof(int64_t), hipMemcpyHostToDevice));
+                A.cells = dcells;
+                A.aoff = doffs;
+                A.scratch = scr;
+                A.status = d_status;
+                A.old_pia = old_pia;
+                hipLaunchKernelGGL(k_nnls_merge, dim3((unsigned)(i1 - i0)), dim3(256), 0, h->stream, A);
+                MZ_HIP(hipGetLastError());
+                MZ_HIP(hipStreamSynchronize(h->stream));
+                i0 = i1;
+            }
         }
+        hipLaunchKernelGGL(k_merge_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, old_pia, nc, ndel);
+        hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
+        hipLaunchKernelGGL(k_merge_push_ids, dim3(nc), dim3(64), 0, h->stream, A.sp, old_pia, off, off + nc,
+                           snap.nbuffer, snap.n_total, nc);
     }
-    hipLaunchKernelGGL(k_merge_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, old_pia, nc, ndel);
-    hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
-    hipLaunchKernelGGL(k_merge_push_ids, dim3(nc), dim3(64), 0, h->stream, A.sp, old_pia, off, off + nc,
-                       snap.nbuffer, snap.n_total, nc);
-    mz_timing_end(h);
     MZ_HIP(hipGetLastError());
     // the handle's stream does not synchronise with hipMemcpy: wait for it first
     MZ_HIP(hipStreamSynchronize(h->stream));
@@ -2372,6 +2377,7 @@ extern "C" int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, co
                              double vref, int32_t scaling_vref, int32_t centered_at_mean, const double *v_multipliers,
                              int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
                              int32_t squash, int32_t *status) {
+    if (species >= 1 && species <= h->n_species) h->sp[species - 1].contig_known = false;
     return nnls_merge_impl(h, species, threshold, mim, n_mim, vref, scaling_vref, centered_at_mean, v_multipliers,
                            n_vmult, max_err, iteration_mult, w_threshold, squash, -1, 0, status);
 }
@@ -2388,6 +2394,7 @@ extern "C" int mz_merge_nnls_cand(mz_store *h, int32_t species, int32_t threshol
                                   const double *v_multipliers, int32_t n_vmult, double max_err,
                                   int32_t iteration_mult, double w_threshold, int32_t squash, int32_t max_pcols,
                                   int32_t method, int32_t *status) {
+    if (species >= 1 && species <= h->n_species) h->sp[species - 1].contig_known = false;
     return nnls_merge_impl(h, species, threshold, mim, n_mim, vref, scaling_vref, centered_at_mean, v_multipliers,
                            n_vmult, max_err, iteration_mult, w_threshold, squash, max_pcols, method, status);
 }
@@ -2573,6 +2580,7 @@ __global__ void k_roulette_counters(MzSpView sp, const int64_t *__restrict__ tot
 
 extern "C" int mz_merge_roulette(mz_store *h, int32_t species, int32_t threshold, int32_t target_np,
                                  int32_t conservative, uint64_t tag, int32_t rng_mode, int64_t *n_merged) {
+    if (species >= 1 && species <= h->n_species) h->sp[species - 1].contig_known = false;
     MZ_CHECK(species >= 1 && species <= h->n_species, "merge_roulette: bad species");
     MZ_CHECK(target_np >= 0 && threshold >= 0, "merge_roulette: negative threshold or target");
     MZ_CHECK(rng_mode == 0 || (rng_mode == 1 && h->d_replay[2]), "merge_roulette: replay table not set");
@@ -2601,14 +2609,15 @@ extern "C" int mz_merge_roulette(mz_store *h, int32_t species, int32_t threshold
     A.off = off;
     A.nbuf0 = snap.nbuffer;
     A.n_merged = cnt;
-    mz_timing_begin(h, "merge_roulette");
-    hipLaunchKernelGGL(k_roulette_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, A.sp, nc, threshold,
-                       target_np, ndel);
-    hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
-    hipLaunchKernelGGL(k_roulette_merge, dim3(nc), dim3(256), 0, h->stream, A);
-    hipLaunchKernelGGL(k_roulette_counters, dim3(1), dim3(1), 0, h->stream, A.sp, off + nc, snap.nbuffer,
-                       snap.n_total);
-    mz_timing_end(h);
+    {
+        MzTimed tm(h, "merge_roulette");
+        hipLaunchKernelGGL(k_roulette_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, A.sp, nc, threshold,
+                           target_np, ndel);
+        hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
+        hipLaunchKernelGGL(k_roulette_merge, dim3(nc), dim3(256), 0, h->stream, A);
+        hipLaunchKernelGGL(k_roulette_counters, dim3(1), dim3(1), 0, h->stream, A.sp, off + nc, snap.nbuffer,
+                           snap.n_total);
+    }
     MZ_HIP(hipGetLastError());
     if (n_merged) {
         int32_t v = 0;

@@ -401,12 +401,9 @@ extern "C" int mz_ntc(mz_store *h, int32_t s1, int32_t s2, double dt, const doub
     } else {
         MZ_CHECK(h->has_grid, "ntc: V is NULL and no grid is set");
     }
-    mz_timing_begin(h, "ntc");
-    int r;
-    if (equal_weight) r = mz_launch_ntc_equal(h, s1, s2, dt, dV, dw_tol, 1, tag, rng_mode, cell_lo, cell_hi);
-    else r = launch_ntc_split(h, s1, s2, dt, dV, dw_tol, tag, rng_mode, cell_lo, cell_hi);
-    mz_timing_end(h);
-    return r;
+    MzTimed tm(h, "ntc");
+    if (equal_weight) return mz_launch_ntc_equal(h, s1, s2, dt, dV, dw_tol, 1, tag, rng_mode, cell_lo, cell_hi);
+    return launch_ntc_split(h, s1, s2, dt, dV, dw_tol, tag, rng_mode, cell_lo, cell_hi);
 }
 
 // ===========================================================================
@@ -544,34 +541,35 @@ extern "C" int mz_swpm(mz_store *h, int32_t species, double G, double dt, const 
     MZ_HIP(hipMemsetAsync(ncoll, 0, (size_t)nc * sizeof(int32_t), h->stream));
     MZ_HIP(hipMemsetAsync(ssz, 0, (size_t)nc * sizeof(int32_t), h->stream));
     MZ_HIP(hipMemsetAsync(created, 0, (size_t)2 * nc * sizeof(int32_t), h->stream));
-    mz_timing_begin(h, "swpm");
-    hipLaunchKernelGGL(k_swpm_count, dim3((ncells + 3) / 4), dim3(256), 0, h->stream, A, G, ncoll, ssz, wmax);
-    hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, h->stream, ssz, (int64_t)nc, soff, soff + nc);
-    int64_t total = 0;
-    MZ_HIP(hipMemcpyAsync(&total, soff + nc, sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
-    MZ_HIP(hipStreamSynchronize(h->stream));
-    const int64_t cap = std::max<int64_t>(total, 1);
-    double *sc = (double *)mz_scratch(h, "ntc_sc", (size_t)7 * cap * sizeof(double));
-    MZ_CHECK(sc != nullptr, "swpm: creation scratch allocation failed");
-    for (int f = 0; f < 7; ++f) A.sc[f] = sc + (size_t)f * cap;
-    A.sc_cap = cap;
-    A.ncoll_in = ncoll;
-    A.soff = soff;
-    A.created = created;
-    hipLaunchKernelGGL(k_swpm, dim3((ncells + 63) / 64), dim3(64), 0, h->stream, A, G, (const double *)wmax);
-    MZ_HIP(hipGetLastError());
-    double **scp = (double **)mz_scratch(h, "ntc_scp", 7 * sizeof(double *));
-    MZ_CHECK(scp != nullptr, "swpm: scratch allocation failed");
-    MZ_HIP(hipMemcpy(scp, A.sc, 7 * sizeof(double *), hipMemcpyHostToDevice));
-    MzCounters snap;
-    int r = mz_read_counters(h, species, &snap);
-    if (r) return r;
-    hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, h->stream, created, (int64_t)(A.cell_hi + 1), coff,
-                       coff + nc);
-    hipLaunchKernelGGL(k_ntc_append, dim3(ncells), dim3(64), 0, h->stream, mz_view(h->sp[species - 1]), A.cell_lo,
-                       A.cell_hi, created, coff, coff + nc, soff, (const double *const *)scp, (int64_t)0, snap);
-    hipLaunchKernelGGL(k_ntc_finish, dim3(1), dim3(256), 0, h->stream, mz_view(h->sp[species - 1]), coff + nc, snap);
-    mz_timing_end(h);
+    {
+        MzTimed tm(h, "swpm");
+        hipLaunchKernelGGL(k_swpm_count, dim3((ncells + 3) / 4), dim3(256), 0, h->stream, A, G, ncoll, ssz, wmax);
+        hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, h->stream, ssz, (int64_t)nc, soff, soff + nc);
+        int64_t total = 0;
+        MZ_HIP(hipMemcpyAsync(&total, soff + nc, sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
+        MZ_HIP(hipStreamSynchronize(h->stream));
+        const int64_t cap = std::max<int64_t>(total, 1);
+        double *sc = (double *)mz_scratch(h, "ntc_sc", (size_t)7 * cap * sizeof(double));
+        MZ_CHECK(sc != nullptr, "swpm: creation scratch allocation failed");
+        for (int f = 0; f < 7; ++f) A.sc[f] = sc + (size_t)f * cap;
+        A.sc_cap = cap;
+        A.ncoll_in = ncoll;
+        A.soff = soff;
+        A.created = created;
+        hipLaunchKernelGGL(k_swpm, dim3((ncells + 63) / 64), dim3(64), 0, h->stream, A, G, (const double *)wmax);
+        MZ_HIP(hipGetLastError());
+        double **scp = (double **)mz_scratch(h, "ntc_scp", 7 * sizeof(double *));
+        MZ_CHECK(scp != nullptr, "swpm: scratch allocation failed");
+        MZ_HIP(hipMemcpy(scp, A.sc, 7 * sizeof(double *), hipMemcpyHostToDevice));
+        MzCounters snap;
+        int r = mz_read_counters(h, species, &snap);
+        if (r) return r;
+        hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, h->stream, created, (int64_t)(A.cell_hi + 1), coff,
+                           coff + nc);
+        hipLaunchKernelGGL(k_ntc_append, dim3(ncells), dim3(64), 0, h->stream, mz_view(h->sp[species - 1]), A.cell_lo,
+                           A.cell_hi, created, coff, coff + nc, soff, (const double *const *)scp, (int64_t)0, snap);
+        hipLaunchKernelGGL(k_ntc_finish, dim3(1), dim3(256), 0, h->stream, mz_view(h->sp[species - 1]), coff + nc, snap);
+    }
     MZ_HIP(hipGetLastError());
     return 0;
 }

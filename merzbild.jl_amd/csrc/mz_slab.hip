@@ -7,9 +7,12 @@
 //   compute_props_sorted!.
 // Here one workgroup owns a slab of consecutive cells and keeps that slab's
 // particles in LDS for all steps of a launch; only particles that leave the
-// slab travel, through a mailbox in HBM, to the neighbouring slab.  The
-// launch is cooperative so every slab is resident and a slab waits only for
-// its two neighbours (a flag per slab side, bounded spins, no grid barrier).
+// slab travel, through a mailbox in HBM, to the neighbouring slab.  The grid
+// is no larger than the occupancy query allows, so on an otherwise idle
+// device every slab is resident, and a slab waits only for its two
+// neighbours (a flag per slab side, bounded spins, no grid barrier).  It is
+// an ordinary launch: every wait ends by itself (a spin limit, then the
+// abort path below), so nothing depends on cooperative-launch machinery.
 //
 // Bit-exact order.  After a sort, the particles of a slab occupy one
 // contiguous range of logical positions in cell order.  The stable counting
@@ -227,6 +230,9 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
     const int b = blockIdx.x;
     const int g = (b & 7) * A.spx + (b >> 3);   // consecutive slabs on one XCD
     if (g >= A.n_slabs) return;
+    // the check before the launch found a state this kernel cannot take, or
+    // another slab has already aborted: touch nothing
+    if (__hip_atomic_load(A.abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int clo = g * A.spc;
     const int chi = min(A.n_cells, clo + A.spc) - 1;
@@ -860,6 +866,106 @@ __global__ void __launch_bounds__(64) k_slab_surf_finalize(const double *__restr
     }
 }
 
+// Before a slab launch, on the store's stream: clears the flag words, keeps a
+// copy of what the launch overwrites before it can abort (collision factors,
+// PIA, collision counters, counters), and checks the launch's precondition,
+// the state a sort leaves: group 1 only, every cell's range starting where
+// the previous non-empty cell's ended, the ranges covering n_total, the
+// store contiguous and error-free.  A state that fails it sets the abort
+// word to SLAB_ABORT_STATE, and every slab then returns before it reads.
+#define SLAB_ABORT_STATE 2
+
+struct SlabPrepArgs {
+    MzPia pia;
+    const MzCounters *ctr;
+    const double *sgwm;
+    const int32_t *cnt;
+    int32_t n_cells;
+    char *bak;              // [nc] double | [7][nc] int32 PIA | [nc][5] int32 | MzCounters
+    uint64_t *flags;        // slab flags and fin lines, cleared
+    int64_t n_flag_words;
+    uint64_t *abort_line;   // SLAB_LINE words; the first 32 bits are the abort word
+};
+
+__global__ void __launch_bounds__(1024) k_slab_prep(SlabPrepArgs P) {
+    __shared__ int32_t wsum[16];
+    __shared__ int32_t carry, bad;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int nc = P.n_cells;
+    for (int64_t i = t; i < P.n_flag_words; i += 1024) P.flags[i] = 0;
+    double *__restrict__ bd = (double *)P.bak;
+    int32_t *__restrict__ bi = (int32_t *)(P.bak + (size_t)nc * sizeof(double));
+    int32_t *__restrict__ bc = bi + (size_t)7 * nc;
+    int32_t *__restrict__ bctr = bc + (size_t)5 * nc;
+    const int32_t *ctr = (const int32_t *)P.ctr;
+    if (t < (int)(sizeof(MzCounters) / sizeof(int32_t))) bctr[t] = ctr[t];
+    if (t == 0) {
+        carry = 0;
+        bad = (!P.ctr->contiguous || P.ctr->err) ? 1 : 0;
+    }
+    __syncthreads();
+    // one pass per 1024 cells: each thread loads all of its cell's words
+    // before it stores any, then the pass scans n_group1
+    for (int c0 = 0; c0 < nc; c0 += 1024) {
+        const int c = c0 + t;
+        const bool in = c < nc;
+        int32_t pv[7] = {0, 0, 0, 0, 0, 0, 0}, cv[5] = {0, 0, 0, 0, 0};
+        double sg = 0.0;
+        if (in) {
+            sg = P.sgwm[c];
+            pv[0] = P.pia.start1[c]; pv[1] = P.pia.end1[c]; pv[2] = P.pia.n1[c]; pv[3] = P.pia.start2[c];
+            pv[4] = P.pia.end2[c]; pv[5] = P.pia.n2[c]; pv[6] = P.pia.nloc[c];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) cv[k] = P.cnt[(size_t)5 * c + k];
+            bd[c] = sg;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) bi[(size_t)k * nc + c] = pv[k];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) bc[(size_t)5 * c + k] = cv[k];
+        }
+        const int32_t n1 = pv[2];
+        bool b = in && (n1 < 0 || pv[5] != 0);
+        const int32_t v = n1 > 0 ? n1 : 0;
+        int32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
+        __syncthreads();
+        if (t < 16) {
+            const int32_t a = wsum[t];
+            int32_t s = a;
+            for (int o = 1; o < 16; o <<= 1) {
+                const int32_t y = __shfl_up(s, o, 16);
+                if (t >= o) s += y;
+            }
+            wsum[t] = s - a;
+        }
+        __syncthreads();
+        const int32_t excl = carry + wsum[wv] + x - v;
+        if (in && n1 > 0 && pv[0] != excl + 1) b = true;
+        if (b) bad = 1;
+        __syncthreads();
+        if (t == 1023) carry = excl + v;
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (carry != P.ctr->n_total) bad = 1;
+        int32_t *ab = (int32_t *)P.abort_line;
+        ab[0] = bad ? SLAB_ABORT_STATE : 0;
+        ab[1] = 0;
+    }
+    if (t > 0 && t < SLAB_LINE) P.abort_line[t] = 0;
+}
+
+// After a slab launch: the abort word into host memory the caller reads
+// after the stream synchronises.
+__global__ void k_slab_report(const int32_t *__restrict__ abort_flag, int32_t *out) {
+    if (threadIdx.x == 0) *(volatile int32_t *)out = *abort_flag;
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
@@ -922,46 +1028,24 @@ extern "C" int mz_get_slab_info(mz_store *h, int32_t *info) {
     return 0;
 }
 
-// Runs n_steps on the slab kernel.  *ran = 1 when it did; 0 when the state or
-// the shape does not allow it, or when the launch aborted (state restored).
-int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *dV, int32_t props_every,
-                    int *ran, double avg_inv, double mass) {
-    const bool avg = avg_inv > 0.0;
-    *ran = 0;
+// Shape of the slab launch: the fewest particles per thread whose LDS slab
+// holds the current largest slab with room to spare, with as many slabs as
+// can be resident.  Kept per store until a launch aborts or the requested
+// shape changes, so a steady run reads nothing back from the device.
+static int slab_pick_shape(mz_store *h, bool avg, MzSlabShapeCache &out) {
     MzSpecies &s = h->sp[0];
     const int nc = h->n_cells;
-    // outcome codes of slab_info[3] when the slab kernel does not run
-    const int32_t NOT_SORTED = -2, NO_SHAPE = -3;
-    h->slab_info[0] = h->slab_info[1] = h->slab_info[2] = 0;
-    h->slab_info[3] = NOT_SORTED;
-    if (n_steps <= 0 || n_steps >= (1 << 28)) return 0;
-    // precondition: the state a sort leaves (group 1 only, consecutive in cell order)
-    MzCounters c;
-    int r = mz_read_counters(h, 1, &c);
-    if (r) return r;
-    if (!c.contiguous || c.err) return 0;
-    std::vector<int32_t> n1(nc), st1(nc), n2(nc);
+    out = MzSlabShapeCache{};
+    std::vector<int32_t> n1(nc);
+    MZ_HIP(hipStreamSynchronize(h->stream));
     MZ_HIP(hipMemcpy(n1.data(), s.pia.n1, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
-    MZ_HIP(hipMemcpy(st1.data(), s.pia.start1, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
-    MZ_HIP(hipMemcpy(n2.data(), s.pia.n2, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
-    int64_t pos = 1;
-    for (int k = 0; k < nc; ++k) {
-        if (n2[k] != 0 || n1[k] < 0) return 0;
-        if (n1[k] > 0 && st1[k] != pos) return 0;
-        pos += n1[k];
-    }
-    if (pos - 1 != c.n_total) return 0;
-    h->slab_info[3] = NO_SHAPE;
-
-    // shape: the fewest particles per thread whose LDS slab holds the current
-    // largest slab with room to spare, with as many slabs as can be resident
     int dev = 0, n_cu = 0;
     MZ_HIP(hipGetDevice(&dev));
     MZ_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
     const int Rs[4] = {1, 2, 4, 8};
-    SlabShape sh;
     for (int R : Rs) {
         if (h->slab_ppt && R != h->slab_ppt) continue;
+        if (R < h->slab_min_ppt) continue;
         SlabKernel fn = slab_kernel(R, avg);
         int per_cu = 0;
         MZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, SLAB_THREADS, 0));
@@ -977,40 +1061,72 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
         // deviations of a Poisson count); a slab that still overflows aborts
         // the launch and the steps run on the step graph
         const double mx = (double)max_slab_particles(n1, spc);
-        if (1.25 * mx + 4.0 * std::sqrt(mx) > (double)P && !h->slab_ppt) continue;
-        if ((int64_t)max_slab_particles(n1, spc) > P) continue;
-        sh.R = R; sh.spc = spc; sh.n_slabs = n_slabs; sh.spx = spx; sh.grid = 8 * spx; sh.fn = fn;
+        if (1.25 * mx + 4.0 * std::sqrt(mx) > (double)P) continue;
+        out.R = R; out.spc = spc; out.n_slabs = n_slabs; out.spx = spx; out.grid = 8 * spx;
+        out.fn = (const void *)fn;
         break;
     }
+    out.valid = true;
+    out.avg = avg;
+    out.req_spc = h->slab_spc;
+    out.req_ppt = h->slab_ppt;
+    return 0;
+}
+
+// Runs n_steps on the slab kernel.  *ran = 1 when it did; 0 when the state or
+// the shape does not allow it, or when the launch aborted (state restored).
+// Host work per call: one check-and-backup kernel, the launch, a one-word
+// report kernel and one stream synchronisation.
+int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, const double *dV, int32_t props_every,
+                    int *ran, double avg_inv, double mass, int retry) {
+    const bool avg = avg_inv > 0.0;
+    *ran = 0;
+    MzSpecies &s = h->sp[0];
+    const int nc = h->n_cells;
+    // outcome codes of slab_info[3] when the slab kernel does not run
+    const int32_t NOT_SORTED = -2, NO_SHAPE = -3;
+    h->slab_info[0] = h->slab_info[1] = h->slab_info[2] = 0;
+    h->slab_info[3] = NO_SHAPE;
+    if (n_steps <= 0 || n_steps >= (1 << 28)) return 0;
+    MzSlabShapeCache &sh = h->slab_shape;
+    if (!sh.valid || sh.avg != avg || sh.req_spc != h->slab_spc || sh.req_ppt != h->slab_ppt) {
+        int r = slab_pick_shape(h, avg, sh);
+        if (r) return r;
+    }
     if (!sh.fn) return 0;
+    if (!h->h_report) {
+        MZ_HIP(hipHostMalloc((void **)&h->h_report, 64, hipHostMallocDefault));
+    }
 
     // scratch: mailboxes, flags, abort word, and a copy of what the kernel
-    // overwrites before it can abort (PIA, counters, collision factors)
+    // overwrites before it can abort (collision factors, PIA, counters)
     const size_t mb_words = (size_t)sh.n_slabs * 2 * 2 * 8 * SLAB_MB_CAP;
     const size_t fl_words = (size_t)sh.n_slabs * 2 * SLAB_LINE;
     const size_t fin_words = (size_t)sh.n_slabs * SLAB_LINE;
     uint64_t *mb = (uint64_t *)mz_scratch(h, "slab_mb", mb_words * 8);
     uint64_t *fl = (uint64_t *)mz_scratch(h, "slab_flags", (fl_words + fin_words + SLAB_LINE) * 8);
-    const size_t bak_bytes = (size_t)nc * (7 * sizeof(int32_t) + sizeof(double) + 5 * sizeof(int32_t)) +
+    const size_t bak_bytes = (size_t)nc * (sizeof(double) + 7 * sizeof(int32_t) + 5 * sizeof(int32_t)) +
                              sizeof(MzCounters);
     char *bak = (char *)mz_scratch(h, "slab_backup", bak_bytes);
     double *surf_part = nullptr;
     if (avg) surf_part = (double *)mz_scratch(h, "slab_surf", (size_t)sh.n_slabs * 4 * MZ_SURF_Q * sizeof(double));
     MZ_CHECK(mb && fl && bak && (!avg || surf_part), "couette slab: scratch allocation failed");
     uint64_t *fin = fl + fl_words;
-    int32_t *abort_flag = (int32_t *)(fin + fin_words);
-    MZ_HIP(hipMemsetAsync(fl, 0, (fl_words + fin_words + SLAB_LINE) * 8, h->stream));
-    int32_t *pia_arr[7] = {s.pia.start1, s.pia.end1, s.pia.n1, s.pia.start2, s.pia.end2, s.pia.n2, s.pia.nloc};
-    size_t off = 0;
-    for (auto p : pia_arr) {
-        MZ_HIP(hipMemcpyAsync(bak + off, p, nc * sizeof(int32_t), hipMemcpyDeviceToDevice, h->stream));
-        off += nc * sizeof(int32_t);
-    }
-    MZ_HIP(hipMemcpyAsync(bak + off, h->d_sgwm, nc * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-    off += nc * sizeof(double);
-    MZ_HIP(hipMemcpyAsync(bak + off, h->d_cfcnt, nc * 5 * sizeof(int32_t), hipMemcpyDeviceToDevice, h->stream));
-    off += nc * 5 * sizeof(int32_t);
-    MZ_HIP(hipMemcpyAsync(bak + off, s.ctr, sizeof(MzCounters), hipMemcpyDeviceToDevice, h->stream));
+    uint64_t *abort_line = fin + fin_words;
+    int32_t *abort_flag = (int32_t *)abort_line;
+
+    SlabPrepArgs pa{};
+    pa.pia = s.pia;
+    pa.ctr = s.ctr;
+    pa.sgwm = h->d_sgwm;
+    pa.cnt = h->d_cfcnt;
+    pa.n_cells = nc;
+    pa.bak = bak;
+    pa.flags = fl;
+    pa.n_flag_words = (int64_t)(fl_words + fin_words);
+    pa.abort_line = abort_line;
+    hipLaunchKernelGGL(k_slab_prep, dim3(1), dim3(1024), 0, h->stream, pa);
+    MZ_HIP(hipGetLastError());
 
     SlabArgs A{};
     A.a = mz_view(s);
@@ -1022,7 +1138,7 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
     A.dt = dt;
     const MzInteraction &it = h->inter[0];
     A.mu1 = it.mu1; A.mu2 = it.mu2; A.factor = it.factor; A.expo = it.expo;
-    MZ_HIP(hipMemcpy(&A.mass, h->d_masses, sizeof(double), hipMemcpyDeviceToHost));
+    A.mass = mass;
     A.V = dV;
     A.sgwm = h->d_sgwm;
     A.cnt = h->d_cfcnt;
@@ -1045,33 +1161,52 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
     A.surf_part = surf_part;
     A.pavg = h->d_pavg;
     void *args[] = {&A};
-    // Every slab waits on its neighbours, so all workgroups must be resident
-    // at once.  Kernels still running on other streams of this process hold
-    // CUs and would start some slabs late, while their neighbours spin until
-    // the launch gives up and replays on the step graph: drain the device first.
-    MZ_HIP(hipDeviceSynchronize());
-    mz_timing_begin(h, "slab");
-    MZ_HIP(hipLaunchCooperativeKernel((const void *)sh.fn, dim3(sh.grid), dim3(SLAB_THREADS), args, 0, h->stream));
-    mz_timing_end_n(h, n_steps);
-    int32_t aborted = 0;
-    MZ_HIP(hipMemcpyAsync(&aborted, abort_flag, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+    // Every slab waits on its neighbours, so all workgroups should be
+    // resident at once: the grid is at most what the occupancy query allows
+    // on an otherwise idle device.  Work still running on another store's
+    // stream holds CUs and would start some slabs late, so the device is
+    // drained first when any other store is busy.  A slab whose neighbour
+    // never arrives gives up after a bounded spin and the launch aborts; the
+    // host then restores the state and runs the steps on the step graph.
+    if (mz_other_stores_busy(h)) MZ_HIP(hipDeviceSynchronize());
+    {
+        MzTimed tm(h, "slab", n_steps);
+        MZ_HIP(hipLaunchKernel(sh.fn, dim3(sh.grid), dim3(SLAB_THREADS), args, 0, h->stream));
+    }
+    hipLaunchKernelGGL(k_slab_report, dim3(1), dim3(64), 0, h->stream, abort_flag, h->h_report);
+    MZ_HIP(hipGetLastError());
     MZ_HIP(hipStreamSynchronize(h->stream));
+    const int32_t outcome = *(volatile int32_t *)h->h_report;
     h->slab_info[0] = sh.R;
     h->slab_info[1] = sh.spc;
     h->slab_info[2] = sh.n_slabs;
-    h->slab_info[3] = aborted ? -1 : 1;
-    if (aborted) {
-        off = 0;
-        for (auto p : pia_arr) {
-            MZ_HIP(hipMemcpyAsync(p, bak + off, nc * sizeof(int32_t), hipMemcpyDeviceToDevice, h->stream));
-            off += nc * sizeof(int32_t);
-        }
-        MZ_HIP(hipMemcpyAsync(h->d_sgwm, bak + off, nc * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-        off += nc * sizeof(double);
-        MZ_HIP(hipMemcpyAsync(h->d_cfcnt, bak + off, nc * 5 * sizeof(int32_t), hipMemcpyDeviceToDevice, h->stream));
-        off += nc * 5 * sizeof(int32_t);
-        MZ_HIP(hipMemcpyAsync(s.ctr, bak + off, sizeof(MzCounters), hipMemcpyDeviceToDevice, h->stream));
+    if (outcome == SLAB_ABORT_STATE) {
+        // precondition not met: no slab wrote anything
+        h->slab_info[3] = NOT_SORTED;
+        sh.valid = false;
+        return 0;
+    }
+    h->slab_info[3] = outcome ? -1 : 1;
+    if (outcome) {
+        sh.valid = false;
+        const double *bd = (const double *)bak;
+        const int32_t *bi = (const int32_t *)(bak + (size_t)nc * sizeof(double));
+        MZ_HIP(hipMemcpyAsync(h->d_sgwm, bd, nc * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        int32_t *pia_arr[7] = {s.pia.start1, s.pia.end1, s.pia.n1, s.pia.start2, s.pia.end2, s.pia.n2, s.pia.nloc};
+        for (int k = 0; k < 7; ++k)
+            MZ_HIP(hipMemcpyAsync(pia_arr[k], bi + (size_t)k * nc, nc * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                                  h->stream));
+        MZ_HIP(hipMemcpyAsync(h->d_cfcnt, bi + (size_t)7 * nc, nc * 5 * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                              h->stream));
+        MZ_HIP(hipMemcpyAsync(s.ctr, bi + (size_t)12 * nc, sizeof(MzCounters), hipMemcpyDeviceToDevice, h->stream));
         MZ_HIP(hipStreamSynchronize(h->stream));
+        // a slab that outgrew its LDS is the usual cause: with an automatic
+        // shape, try once more with twice the particles per thread (kept for
+        // later calls on this store)
+        if (!retry && h->slab_ppt == 0 && sh.R < 8) {
+            h->slab_min_ppt = 2 * sh.R;
+            return mz_couette_slab(h, n_steps, step0, dt, dV, props_every, ran, avg_inv, mass, 1);
+        }
         return 0;
     }
     if (avg) {
@@ -1080,6 +1215,7 @@ int mz_couette_slab(mz_store *h, int32_t n_steps, uint64_t step0, double dt, con
         MZ_HIP(hipGetLastError());
     }
     mz_swap_buffers(s);
+    s.contig_known = true;
     *ran = 1;
     return 0;
 }

@@ -10,6 +10,7 @@
 #include "mz_internal.h"
 
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 __global__ void k_set_step(uint64_t *step, uint64_t v) {
@@ -81,21 +82,25 @@ static int couette_run(mz_store *h, int32_t n_steps, uint64_t step0, double dt, 
     MZ_CHECK(h->inter[0].set, "couette_steps: interaction not set");
     MZ_CHECK(masses != nullptr, "couette_steps: masses is NULL");
     MZ_CHECK(n_steps >= 0, "couette_steps: negative step count");
-    MZ_HIP(hipMemcpy(h->d_masses, masses, sizeof(double), hipMemcpyHostToDevice));
+    // uploads only of what changed since the last call (the device copies
+    // are written by nothing else than these uploads)
+    if (!h->masses_valid || h->masses_up[0] != masses[0]) {
+        MZ_HIP(hipMemcpy(h->d_masses, masses, sizeof(double), hipMemcpyHostToDevice));
+        h->masses_up[0] = masses[0];
+        h->masses_valid = true;
+    }
     const double *dV = h->d_V;
     if (V) {
         double *v = (double *)mz_scratch(h, "step_V", h->n_cells * sizeof(double));
         MZ_CHECK(v != nullptr, "couette_steps: scratch allocation failed");
-        MZ_HIP(hipMemcpy(v, V, h->n_cells * sizeof(double), hipMemcpyHostToDevice));
+        if (h->V_up.size() != (size_t)h->n_cells || std::memcmp(h->V_up.data(), V, h->n_cells * sizeof(double))) {
+            MZ_HIP(hipMemcpy(v, V, h->n_cells * sizeof(double), hipMemcpyHostToDevice));
+            h->V_up.assign(V, V + h->n_cells);
+        }
         dV = v;
     }
-    MzCounters c;
-    int r = mz_read_counters(h, 1, &c);
+    int r = mz_squash(h, 1);   // returns at once when the species is known to be contiguous
     if (r) return r;
-    if (!c.contiguous) {
-        r = mz_squash(h, 1);
-        if (r) return r;
-    }
     const double mass = masses[0];
     if (avg_inv > 0.0) {
         MZ_CHECK(mass > 0.0 && dt > 0.0, "couette_avg_steps: wall properties need mass > 0 and dt > 0");
@@ -112,6 +117,13 @@ static int couette_run(mz_store *h, int32_t n_steps, uint64_t step0, double dt, 
             h->last_path = 2;
             return 0;
         }
+        if (h->slab_info[3] == -1 || h->slab_info[3] == -2) {
+            // the launch did not run or was undone: the replay reads the
+            // device's contiguity flag again
+            h->sp[0].contig_known = false;
+            r = mz_squash(h, 1);
+            if (r) return r;
+        }
         mode = 1;
     }
     if (mode == 2) mode = 1;
@@ -119,7 +131,7 @@ static int couette_run(mz_store *h, int32_t n_steps, uint64_t step0, double dt, 
     h->last_path = graph_ok ? 1 : 0;
     if (!graph_ok) {
         // eager launches (also the path for ntc! with weight splitting)
-        mz_timing_begin(h, "step");
+        MzTimed tm(h, "step", n_steps);
         for (int32_t t = 0; t < n_steps; ++t) {
             uint64_t step = step0 + (uint64_t)t;
             int do_props = props_every > 0 && ((t + 1) % props_every) == 0;
@@ -139,7 +151,7 @@ static int couette_run(mz_store *h, int32_t n_steps, uint64_t step0, double dt, 
                 }
             }
         }
-        mz_timing_end_n(h, n_steps);
+        h->sp[0].contig_known = true;
         return 0;
     }
     // graph replay
@@ -169,13 +181,13 @@ static int couette_run(mz_store *h, int32_t n_steps, uint64_t step0, double dt, 
         h->graph_key = keybuf;
     }
     hipLaunchKernelGGL(k_set_step, dim3(1), dim3(64), 0, h->stream, h->d_step, step0);
-    mz_timing_begin(h, "step");
+    MzTimed tm(h, "step", n_steps);
     for (int32_t t = 0; t + 1 < n_steps; t += 2) MZ_HIP(hipGraphLaunch(h->graph_exec, h->stream));
     if (n_steps & 1) {
         uint64_t step = step0 + (uint64_t)(n_steps - 1);
         r = enqueue_step(h, dV, dt, props_every == 1, step, nullptr, avg_inv, mass);
         if (r) return r;
     }
-    mz_timing_end_n(h, n_steps);
+    h->sp[0].contig_known = true;
     return 0;
 }

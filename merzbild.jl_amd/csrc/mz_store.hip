@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 // ===========================================================================
@@ -42,8 +43,17 @@ int mz_timing_begin(mz_store *h, const char *family) {
     if (!h->timing) return 0;
     MzEventPair ev;
     MZ_HIP(hipEventCreate(&ev.a));
-    MZ_HIP(hipEventCreate(&ev.b));
-    MZ_HIP(hipEventRecord(ev.a, h->stream));
+    if (hipEventCreate(&ev.b) != hipSuccess) {
+        hipEventDestroy(ev.a);
+        mz_set_error("timing: hipEventCreate failed");
+        return -1;
+    }
+    if (hipEventRecord(ev.a, h->stream) != hipSuccess) {
+        hipEventDestroy(ev.a);
+        hipEventDestroy(ev.b);
+        mz_set_error("timing: hipEventRecord failed");
+        return -1;
+    }
     h->open_events.push_back({std::string(family), ev});
     return 0;
 }
@@ -103,6 +113,18 @@ extern "C" int mz_timing_reset(mz_store *h) {
     if (r) return r;
     h->timing_acc.clear();
     return 0;
+}
+
+// ===========================================================================
+// live stores of the process (mz_other_stores_busy)
+static std::mutex g_stores_mu;
+static std::vector<mz_store *> g_stores;
+
+bool mz_other_stores_busy(mz_store *h) {
+    std::lock_guard<std::mutex> lk(g_stores_mu);
+    for (mz_store *o : g_stores)
+        if (o != h && hipStreamQuery(o->stream) == hipErrorNotReady) return true;
+    return false;
 }
 
 // ===========================================================================
@@ -182,6 +204,10 @@ extern "C" int mz_create(int32_t n_cells, int32_t n_species, const int64_t *capa
     MZ_HIP(hipMemset(h->d_pavg, 0, pavg_bytes));
     h->inter.resize(nss);
     MZ_HIP(hipStreamSynchronize(h->stream));
+    {
+        std::lock_guard<std::mutex> lk(g_stores_mu);
+        g_stores.push_back(h);
+    }
     *out = h;
     return 0;
 }
@@ -199,7 +225,24 @@ static void free_species(MzSpecies &s) {
 
 extern "C" int mz_destroy(mz_store *h) {
     if (!h) return 0;
+    {
+        std::lock_guard<std::mutex> lk(g_stores_mu);
+        g_stores.erase(std::remove(g_stores.begin(), g_stores.end(), h), g_stores.end());
+    }
     hipStreamSynchronize(h->stream);
+    // timing events still open or not yet read
+    for (auto &fe : h->open_events) {
+        hipEventDestroy(fe.second.a);
+        hipEventDestroy(fe.second.b);
+    }
+    for (auto &kv : h->pending)
+        for (auto &ev : kv.second) {
+            hipEventDestroy(ev.a);
+            hipEventDestroy(ev.b);
+        }
+    h->open_events.clear();
+    h->pending.clear();
+    if (h->h_report) hipHostFree(h->h_report);
     mz_graph_release(h);
     for (auto &s : h->sp) free_species(s);
     hipFree(h->d_sgwm); hipFree(h->d_cfcnt); hipFree(h->d_err); hipFree(h->d_ticket);
@@ -248,6 +291,9 @@ extern "C" int mz_upload_species(mz_store *h, int32_t species, int64_t n_pos, co
                                  const int64_t *buffer, int64_t nbuffer, int64_t length) {
     MZ_CHECK(species >= 1 && species <= h->n_species, "upload: bad species");
     MzSpecies &s = h->sp[species - 1];
+    s.contig_known = false;
+    h->slab_shape.valid = false;
+    h->slab_min_ppt = 0;
     MZ_CHECK(n_pos >= 0 && n_pos <= s.cap, "upload: n_pos exceeds capacity");
     MZ_CHECK(length >= 0 && length < (int64_t)INT32_MAX / 2, "upload: ParticleVector length out of range");
     MZ_CHECK(nbuffer >= 0 && nbuffer <= length, "upload: bad nbuffer");
@@ -315,6 +361,7 @@ extern "C" int mz_upload_species(mz_store *h, int32_t species, int64_t n_pos, co
     MzCounters c{};
     c.n_total = (int32_t)n_total;
     c.contiguous = contiguous ? 1 : 0;
+    s.contig_known = contiguous != 0;
     c.nbuffer = (int32_t)nbuffer;
     c.length = (int32_t)length;
     c.n_pos = (int32_t)std::max<int64_t>(max_end, contiguous ? n_total : max_end);
@@ -1017,18 +1064,23 @@ extern "C" int mz_set_sort_path(mz_store *h, int32_t path) {
 
 extern "C" int mz_sort(mz_store *h, int32_t species, int32_t compute_cells) {
     MZ_CHECK(species >= 1 && species <= h->n_species, "sort: bad species");
-    MzCounters c;
-    int r = mz_read_counters(h, species, &c);
-    if (r) return r;
-    if (!c.contiguous) {
-        r = mz_squash(h, species);
+    int r = 0;
+    if (!h->sp[species - 1].contig_known) {
+        MzCounters c;
+        r = mz_read_counters(h, species, &c);
         if (r) return r;
+        if (!c.contiguous) {
+            r = mz_squash(h, species);
+            if (r) return r;
+        }
     }
-    mz_timing_begin(h, "sort");
-    r = mz_launch_sort(h, species, compute_cells);
-    mz_timing_end(h);
+    {
+        MzTimed tm(h, "sort");
+        r = mz_launch_sort(h, species, compute_cells);
+    }
     if (r) return r;
     mz_swap_buffers(h->sp[species - 1]);
+    h->sp[species - 1].contig_known = true;
     return 0;
 }
 
@@ -1106,10 +1158,14 @@ __global__ void k_squash_pia(MzSpView sp, int32_t n_cells, const int32_t *__rest
 
 extern "C" int mz_squash(mz_store *h, int32_t species) {
     MZ_CHECK(species >= 1 && species <= h->n_species, "squash: bad species");
+    if (h->sp[species - 1].contig_known) return 0;
     MzCounters c;
     int r = mz_read_counters(h, species, &c);
     if (r) return r;
-    if (c.contiguous) return 0;
+    if (c.contiguous) {
+        h->sp[species - 1].contig_known = true;
+        return 0;
+    }
     MzSpecies &s = h->sp[species - 1];
     int nc = h->n_cells;
     int32_t *news = (int32_t *)mz_scratch(h, "squash", (size_t)2 * nc * sizeof(int32_t));
@@ -1122,6 +1178,7 @@ extern "C" int mz_squash(mz_store *h, int32_t species) {
     MZ_HIP(hipGetLastError());
     mz_swap_buffers(s);
     std::swap(s.key, s.keyb);
+    s.contig_known = true;
     return 0;
 }
 
@@ -1188,13 +1245,16 @@ int mz_launch_convect(mz_store *h, int species, double dt, uint64_t tag, int rng
 extern "C" int mz_convect(mz_store *h, int32_t species, double dt, uint64_t tag, int32_t rng_mode) {
     MZ_CHECK(species >= 1 && species <= h->n_species, "convect: bad species");
     MZ_CHECK(rng_mode == 0 || (rng_mode == 1 && h->d_replay[1]), "convect: replay table not set");
-    MzCounters c;
-    int r = mz_read_counters(h, species, &c);
-    if (r) return r;
-    mz_timing_begin(h, "convect");
-    r = mz_launch_convect(h, species, dt, tag, rng_mode, c.contiguous != 0, c.n_pos);
-    mz_timing_end(h);
-    return r;
+    // a species known to be contiguous uses positions [0, n_total), which the
+    // kernel reads on the device; otherwise the extent comes from the counters
+    MzCounters c{};
+    c.contiguous = 1;
+    if (!h->sp[species - 1].contig_known) {
+        int r = mz_read_counters(h, species, &c);
+        if (r) return r;
+    }
+    MzTimed tm(h, "convect");
+    return mz_launch_convect(h, species, dt, tag, rng_mode, c.contiguous != 0, c.n_pos);
 }
 
 // ===========================================================================
@@ -1399,13 +1459,16 @@ extern "C" int mz_convect_surf(mz_store *h, int32_t species, double dt, uint64_t
                                double mass) {
     MZ_CHECK(species >= 1 && species <= h->n_species, "convect: bad species");
     MZ_CHECK(rng_mode == 0 || (rng_mode == 1 && h->d_replay[1]), "convect: replay table not set");
-    MzCounters c;
-    int r = mz_read_counters(h, species, &c);
-    if (r) return r;
-    mz_timing_begin(h, "convect");
-    r = mz_launch_convect_surf(h, species, dt, tag, rng_mode, c.contiguous != 0, c.n_pos, mass, nullptr, 0.0);
-    mz_timing_end(h);
-    return r;
+    // a species known to be contiguous uses positions [0, n_total), which the
+    // kernel reads on the device; otherwise the extent comes from the counters
+    MzCounters c{};
+    c.contiguous = 1;
+    if (!h->sp[species - 1].contig_known) {
+        int r = mz_read_counters(h, species, &c);
+        if (r) return r;
+    }
+    MzTimed tm(h, "convect");
+    return mz_launch_convect_surf(h, species, dt, tag, rng_mode, c.contiguous != 0, c.n_pos, mass, nullptr, 0.0);
 }
 
 extern "C" int mz_set_surf_areas(mz_store *h, const double *areas) {
@@ -1570,9 +1633,12 @@ extern "C" int mz_compute_props(mz_store *h, int32_t variant, int32_t ndens_not_
     MZ_CHECK(variant == 0 || variant == 1, "props: bad variant");
     MZ_CHECK(masses != nullptr, "props: masses is NULL");
     MZ_HIP(hipMemcpyAsync(h->d_masses, masses, h->n_species * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    mz_timing_begin(h, "props");
-    int r = mz_launch_props(h, variant, ndens_not_Np);
-    mz_timing_end(h);
+    h->masses_valid = false;   // the couette drivers upload their mass again
+    int r;
+    {
+        MzTimed tm(h, "props");
+        r = mz_launch_props(h, variant, ndens_not_Np);
+    }
     if (r) return r;
     MZ_HIP(hipStreamSynchronize(h->stream));
     return 0;
@@ -1631,11 +1697,13 @@ extern "C" int mz_compute_flux_props(mz_store *h, const double *masses, int32_t 
     MZ_CHECK(h->has_grid, "flux_props: no grid set (the cell volumes come from it)");
     MZ_CHECK(cell_lo >= 1 && cell_hi <= h->n_cells && cell_lo <= cell_hi, "flux_props: bad cell range");
     MZ_HIP(hipMemcpyAsync(h->d_masses, masses, h->n_species * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    h->masses_valid = false;   // the couette drivers upload their mass again
     const int ncell = cell_hi - cell_lo + 1;
-    mz_timing_begin(h, "flux");
-    hipLaunchKernelGGL(k_flux, dim3((ncell + 3) / 4, h->n_species), dim3(256), 0, h->stream, mz_views(h), h->n_cells,
-                       cell_lo - 1, cell_hi - 1, h->d_masses, h->d_invV, h->d_v, h->d_flux);
-    mz_timing_end(h);
+    {
+        MzTimed tm(h, "flux");
+        hipLaunchKernelGGL(k_flux, dim3((ncell + 3) / 4, h->n_species), dim3(256), 0, h->stream, mz_views(h),
+                           h->n_cells, cell_lo - 1, cell_hi - 1, h->d_masses, h->d_invV, h->d_v, h->d_flux);
+    }
     MZ_HIP(hipGetLastError());
     MZ_HIP(hipStreamSynchronize(h->stream));
     return 0;
@@ -1762,12 +1830,14 @@ extern "C" int mz_compute_props_moments(mz_store *h, const int32_t *powers, int3
     MZ_HIP(hipMemcpyAsync(d_pow, powers, n_moments * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
     MZ_HIP(hipMemcpyAsync(d_sc, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     MZ_HIP(hipMemcpyAsync(h->d_masses, masses, ns * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    mz_timing_begin(h, "props");
-    dim3 grid((h->n_cells + 3) / 4, ns);
-    hipLaunchKernelGGL(k_props_moments, grid, dim3(256), 0, h->stream, mz_views(h), h->n_cells, n_moments, d_pow,
-                       d_sc, h->d_masses, h->d_np, h->d_n, h->d_v, h->d_T, d_m);
-    MZ_HIP(hipGetLastError());
-    mz_timing_end(h);
+    h->masses_valid = false;   // the couette drivers upload their mass again
+    {
+        MzTimed tm(h, "props");
+        dim3 grid((h->n_cells + 3) / 4, ns);
+        hipLaunchKernelGGL(k_props_moments, grid, dim3(256), 0, h->stream, mz_views(h), h->n_cells, n_moments,
+                           d_pow, d_sc, h->d_masses, h->d_np, h->d_n, h->d_v, h->d_T, d_m);
+        MZ_HIP(hipGetLastError());
+    }
     MZ_HIP(hipStreamSynchronize(h->stream));
     h->n_moments = n_moments;
     return 0;
