@@ -96,10 +96,23 @@ def setup_store(pkg, case):
     return st, sp, it
 
 
-def cpu_baseline(case, it, target_s=12.0):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, it, target_s=12.0, serial_s=6.0):
     """The C oracle (a restatement of the reference method, not the Julia
-    reference) on this host's cores with OpenMP, same workload, bounded to a
-    sample of steps.  Returns (particle-timesteps/s, threads, steps)."""
+    reference), -O3 with OpenMP over cells and a threaded counting sort, on
+    this host's cores, same workload, bounded to a sample of steps; also a
+    single-thread sample.  Returns (threaded particle-timesteps/s, threads,
+    steps, serial particle-timesteps/s, serial steps)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import mzo
     mzo.build()
@@ -126,16 +139,25 @@ def cpu_baseline(case, it, target_s=12.0):
     pp = mzo.Props(N_CELLS, 1)
     rng = mzo.RNG(seed=20000)
     threads = mzo.lib().mzo_omp_max_threads()
-    args = lambda k, s0: (rng.ref, C.byref(grid), C.byref(walls), C.byref(oit), cfs, pv.p, pia.p, DT,
-                          V.ctypes.data_as(C.POINTER(C.c_double)), case["mass"], pp.ref, k, s0, 1, threads, 1)
-    t0 = time.perf_counter()
-    mzo.lib().mzo_couette_steps(*args(5, 1))
-    t_probe = (time.perf_counter() - t0) / 5
-    steps = max(10, int(target_s / max(t_probe, 1e-6)))
-    t0 = time.perf_counter()
-    mzo.lib().mzo_couette_steps(*args(steps, 6))
-    el = time.perf_counter() - t0
-    return n * steps / el, threads, steps
+    args = lambda k, s0, nt: (rng.ref, C.byref(grid), C.byref(walls), C.byref(oit), cfs, pv.p, pia.p, DT,
+                              V.ctypes.data_as(C.POINTER(C.c_double)), case["mass"], pp.ref, k, s0, 1, nt, 1)
+    step = 1
+
+    def timed(nt, budget):
+        nonlocal step
+        t0 = time.perf_counter()
+        mzo.lib().mzo_couette_steps(*args(5, step, nt))
+        step += 5
+        k = max(10, int(budget / max((time.perf_counter() - t0) / 5, 1e-6)))
+        t0 = time.perf_counter()
+        mzo.lib().mzo_couette_steps(*args(k, step, nt))
+        el = time.perf_counter() - t0
+        step += k
+        return n * k / el, k
+
+    serial, serial_steps = timed(1, serial_s)
+    threaded, steps = timed(threads, target_s)
+    return threaded, threads, steps, serial, serial_steps
 
 
 # Batched uneven 0-D cells: 65,536 independent cells of V = 1 with log-uniform
@@ -246,6 +268,125 @@ def run_batched_0d(a, pkg):
                                   "compute_props! per step", "n_particles": n, "mode": mode,
                       "batch_shape": st.batch_info() if mode == "batched" else None, "autotune": tune},
            "compare": others}
+    print(json.dumps(out), flush=True)
+    st.close()
+
+
+def run_batched_0d_split(a, pkg):
+    """The batched uneven 0-D cells with ntc! itself: weight splitting
+    conserves momentum and energy with unequal weights, and the particle
+    count grows (about 2% per step at this collision rate), so the run is
+    kept to 20 steps (growth to about 1.5x) instead of merging; the rate
+    counts the particles each step actually holds."""
+    case = sample_batched_0d()
+    n = case["n"]
+    data = os.path.join(ROOT, "merzbild.jl_amd", "data")
+    sp = pkg.load_species_data(os.path.join(data, "particles.toml"), ["Ar"])
+    it = pkg.load_interaction_data(os.path.join(data, "pseudo_maxwell.toml"), sp)
+    steps = a.steps if a.steps != 20000 else 20
+    st = pkg.DeviceStore(B0_CELLS, 1, int(n * 1.02 ** (steps + 8)) + 4096)
+    st.upload_species(1, case["x"], case["v"], case["w"], case["ids"], case["pia"], n, True,
+                      np.zeros(0, dtype=np.int64), n)
+    del case["x"], case["v"], case["w"]
+    st.set_grid(pkg.Grid1DUniform(float(B0_CELLS), B0_CELLS))
+    st.set_interactions(it)
+    st.set_species_data(sp)
+    ia = it[0][0]
+    sg = np.array([pkg.estimate_sigma_g_w_max(ia, sp[0].mass, sp[0].mass, B0_T0, B0_T0, f) for f in case["fnum"]])
+    st.set_sigma_g_w_max(1, 1, sg)
+    st.set_seed(7)
+    tref = 1.0 / (B0_NDENS * np.pi * ia.vhs_d ** 2) / np.sqrt(2 * K_B * B0_T0 / sp[0].mass)
+    dt = 0.025 * tref
+    masses = np.array([sp[0].mass])
+    lib = pkg.load_library()
+
+    def step(k):
+        pkg.ntc(st, 1, 1, dt, V=1.0, equal_weight=False, tag=(k << 4) | 1)
+        pkg._check(lib.mz_compute_props(st.h, 0, 0, pkg._d(masses)))
+
+    step(1)   # warm-up (sizes the creation scratch)
+    st.sync()
+    counts = []
+    st.timing_reset()
+    st.timing(True)
+    t0 = time.perf_counter()
+    for k in range(2, 2 + steps):
+        counts.append(st.scalars(1)["n_total"])
+        step(k)
+    st.sync()
+    el = time.perf_counter() - t0
+    ntc_ms, _ = st.timing_get("ntc")
+    props_ms, _ = st.timing_get("props")
+    st.timing(False)
+    sc = st.scalars(1)
+    assert sc["err"] == 0, sc
+    out = {"metric": "particle-timesteps/s (NTC with splitting + props), batched uneven 0-D cells",
+           "value": float(np.sum(counts)) / el, "unit": "particle-timesteps/s", "n_gpus": 1, "steps": steps,
+           "ms_per_step": el * 1e3 / steps, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "65536 0-D cells, log-uniform 20..20000 particles (Philox seed 7), "
+                                  "pseudo-Maxwell Ar, BKW at 273 K, w ~ U(0.5, 2) Fnum; ntc! with weight "
+                                  "splitting + compute_props! per step, no merging (20 steps)",
+                      "n_particles_start": n, "n_particles_end": sc["n_total"],
+                      "kernel_ms_per_step": {"ntc": ntc_ms, "props": props_ms}}}
+    print(json.dumps(out), flush=True)
+    st.close()
+
+
+def run_couette_varweight(a, pkg):
+    """The variable-weight Couette flow with octree merging
+    (test/test_1D_couette_varweight.jl:78-105 at the headline size): 500 cells
+    x 200 particles, weights Fnum lognormal(0, 1) / e^0.5, per step ntc! with
+    splitting, octree N:2 merging (midpoint split, min/max bounds, max depth
+    8, grid-clamped) of every cell above 400 particles down to 64, squash,
+    convection, sort and compute_props_sorted!.  The rate counts the
+    particles each step holds; merge time is reported on its own."""
+    case = sample_case()
+    n = case["n"]
+    g = np.random.default_rng(99)
+    case["w"] = case["Fnum"] * g.lognormal(0.0, 1.0, size=n) / np.exp(0.5)
+    st, sp, it = setup_store(pkg, case)
+    steps = a.steps if a.steps != 20000 else 4000
+    warm = a.warmup if a.warmup != 200 else 200
+    masses = np.array([sp[0].mass])
+    lib = pkg.load_library()
+    merged = [0]
+
+    def step(k):
+        pkg.ntc(st, 1, 1, DT, V=L / N_CELLS, equal_weight=False, tag=(k << 4) | 1)
+        merged[0] += pkg.merge_octree_N2_based(st, 1, threshold=400, target_np=64, max_depth=8, grid_clamp=True,
+                                               squash=True, tag=(k << 4) | 3)
+        pkg.convect_particles(st, 1, DT, tag=(k << 4) | 2, fetch=False)
+        pkg.sort_particles(st, 1)
+        pkg._check(lib.mz_compute_props(st.h, 1, 0, pkg._d(masses)))
+
+    for k in range(1, warm + 1):
+        step(k)
+    st.sync()
+    merged[0] = 0
+    n0 = st.scalars(1)["n_total"]
+    st.timing_reset()
+    st.timing(True)
+    t0 = time.perf_counter()
+    for k in range(warm + 1, warm + 1 + steps):
+        step(k)
+    st.sync()
+    el = time.perf_counter() - t0
+    fam = {f: st.timing_get(f)[0] for f in ("ntc", "merge_octree", "convect", "sort", "props")}
+    st.timing(False)
+    sc = st.scalars(1)
+    assert sc["err"] == 0, sc
+    n_mean = 0.5 * (n0 + sc["n_total"])
+    merge_ms = fam["merge_octree"] * steps
+    out = {"metric": "particle-timesteps/s (ntc! with splitting + octree merge + convection + sort + props), "
+                     "variable-weight Couette",
+           "value": n_mean * steps / el, "unit": "particle-timesteps/s", "n_gpus": 1, "steps": steps,
+           "warmup": warm, "ms_per_step": el * 1e3 / steps, "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "1D Couette, argon VHS, 500 cells x 200 particles, lognormal(0,1)/e^0.5 weights, "
+                                  "dt 2e-10 s; octree N:2 merge above 400 to 64 (mid split, max depth 8, grid clamp)",
+                      "n_particles_start": n0, "n_particles_end": sc["n_total"], "cells_merged": merged[0],
+                      "kernel_ms_per_step": fam,
+                      "value_excluding_merge": n_mean * steps / max(el - merge_ms * 1e-3, 1e-9)}}
     print(json.dumps(out), flush=True)
     st.close()
 
@@ -717,11 +858,14 @@ def main():
                          "eager: kernel by kernel")
     ap.add_argument("--graph-steps", type=int, default=400, help="steps of the step-graph path timed alongside")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-nnls", action="store_true", help="couette: skip the NNLS merges/s half of the metric")
+    ap.add_argument("--replicas", type=int, default=16, help="couette: independent replicas run side by side (0: skip)")
+    ap.add_argument("--replica-steps", type=int, default=400, help="couette: steps of the replica run")
     ap.add_argument("--no-compare", action="store_true",
                     help="batched-0d: skip timing the plain and per-cell launch patterns")
     ap.add_argument("--no-autotune", action="store_true", help="slab mode: keep the library's default shape")
-    ap.add_argument("--workload", choices=("couette", "batched-0d", "nnls-merge", "couette-surf", "roulette-merge",
-                                           "bkw-swpm", "grid-merge"),
+    ap.add_argument("--workload", choices=("couette", "batched-0d", "batched-0d-split", "couette-varweight",
+                                           "nnls-merge", "couette-surf", "roulette-merge", "bkw-swpm", "grid-merge"),
                     default="couette",
                     help="couette: the headline 1-D Couette line; batched-0d: 65,536 uneven 0-D cells; "
                          "nnls-merge: 32,768 cells merged by NNLS; couette-surf: the averaging phase of the "
@@ -748,6 +892,12 @@ def main():
     pkg = load_pkg()
     if a.workload == "batched-0d":
         run_batched_0d(a, pkg)
+        return
+    if a.workload == "batched-0d-split":
+        run_batched_0d_split(a, pkg)
+        return
+    if a.workload == "couette-varweight":
+        run_couette_varweight(a, pkg)
         return
     if a.workload == "nnls-merge":
         run_nnls(a, pkg)
@@ -832,13 +982,70 @@ def main():
             "kernel_ms_per_step": launch_ms, "steps_per_launch": steps_per_launch,
             "alg_bytes_per_particle_step": ALG_BYTES["step"]}
 
+    # 16 independent replicas of the same Couette case (seeds differ), each in
+    # its own store on its own stream, run side by side on the step graph: the
+    # ensemble the survey's C2 row asks for.  All replicas' particles (1.6M)
+    # do not fit in LDS at once, so this is the HBM-resident step graph.
+    replicas = None
+    if a.replicas > 0:
+        stores = []
+        for r in range(a.replicas):
+            rc = sample_case(seed=12345 + 1000 * (r + 1))
+            rs, _, _ = setup_store(pkg, rc)
+            rs.set_seed(20000 + r)
+            pkg.couette_steps(rs, 4, 1, DT, props_every=1, mode="graph")
+            stores.append(rs)
+        for rs in stores:
+            rs.sync()
+        k = a.replica_steps
+        t0 = time.perf_counter()
+        for rs in stores:
+            pkg.couette_steps(rs, k, 5, DT, props_every=1, mode="graph")
+        for rs in stores:
+            rs.sync()
+        el_r = time.perf_counter() - t0
+        n_r = sum(rs.scalars(1)["n_total"] for rs in stores)
+        for rs in stores:
+            assert rs.scalars(1)["err"] == 0
+            rs.close()
+        ach = ALG_BYTES["step"] * n_r * k / el_r / 1e9
+        replicas = {"replicas": a.replicas, "steps": k, "n_particles": n_r, "value": n_r * k / el_r,
+                    "unit": "particle-timesteps/s", "ms_per_step": el_r * 1e3 / k, "path": "graph, one stream per replica",
+                    "roofline": {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0,
+                                 "alg_bytes_per_particle_step": ALG_BYTES["step"],
+                                 "note": "wall clock of all replicas' steps (host enqueue included)"}}
+
+    # the second half of the metric: NNLS merges/s at <= 1e-10 moment error on
+    # the batched NNLS workload, the mixed method beside the plain FP64 one
+    nnls = None
+    if not a.no_nnls:
+        cs = sample_nnls_cells()
+        mix = nnls_merge_once(pkg, cs, NN_PCOLS, "mixed")
+        lds = nnls_merge_once(pkg, cs, NN_PCOLS, "lds")
+        nnls = {"metric": "NNLS merges/s at <= 1e-10 moment error", "value": mix["merges_per_s"],
+                "unit": "merges/s", "method": "mixed (FP32 active-set search, FP64 passive-set solve)",
+                "kernel_ms": mix["kernel_ms"], "merged": mix["merged"], "failed": mix["failed"],
+                "refined_in_fp64": mix["fp64_redo"],
+                "max_rel_moment_error_sampled": mix["max_rel_moment_error_sampled"],
+                "plain_fp64": {"merges_per_s": lds["merges_per_s"], "kernel_ms": lds["kernel_ms"],
+                               "max_rel_moment_error_sampled": lds["max_rel_moment_error_sampled"]},
+                "speedup_vs_plain_fp64": mix["merges_per_s"] / lds["merges_per_s"],
+                "workload": "32768 cells x U{200..3000} bi-Maxwellian particles, lognormal(0.5) weights, 35 "
+                            "moments (order <= 4), :variance, columns = first 239 particles + 17 fictitious",
+                "n_particles": cs["n"]}
+        del cs
     cpu = None
     if not a.no_cpu_baseline:
-        v_cpu, threads, steps = cpu_baseline(case, it)
+        v_cpu, threads, steps, v_serial, serial_steps = cpu_baseline(case, it)
         cpu = {"value": v_cpu, "unit": "particle-timesteps/s", "cores": threads, "kind": "port",
-               "sample": f"{steps} Couette steps x {n} particles (C oracle, OpenMP)"}
+               "sample": f"{steps} Couette steps x {n} particles (C oracle -O3, OpenMP over cells, threaded "
+                         f"counting sort); serial: {serial_steps} steps on 1 thread",
+               "serial_value": v_serial, "cpu_model": _cpu_model(),
+               "reference_published": {"serial_i9_13900K_50k": 8.8e7, "threads32_epyc_9374F": 1.2e8,
+                                       "source": "BENCHMARKS.md:23,240 (derived particle-timesteps/s)"}}
     out = {
-        "metric": "particle-timesteps/s (sort+NTC+props)",
+        # the metric's first half is "value"; its second half is in "nnls"
+        "metric": "particle-timesteps/s (sort+NTC+props); NNLS merges/s at \u22641e-10 moment error",
         "value": value,
         "unit": "particle-timesteps/s",
         "n_gpus": 1,
@@ -856,6 +1063,8 @@ def main():
                    "graph_ms_per_step": graph_ms, "step_kernels_ms": kern},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "nnls": nnls,
+        "replicas": replicas,
     }
     print(json.dumps(out), flush=True)
     st.close()

@@ -118,6 +118,12 @@ int mz_set_sort_path(mz_store *h, int32_t path);
 
 /* squash_pia!(pv, pia, species) (src/particles.jl:622-701) */
 int mz_squash(mz_store *h, int32_t species);
+/* restore_particle_ordering!(pv, inv_map) (src/particles.jl:1086-1137):
+ * afterwards the index is the identity and the free-id buffer holds
+ * length, length - 1, ... ; the particles at the logical positions and the
+ * PIA are unchanged (the store already keeps data in logical order, so no
+ * data moves). */
+int mz_restore_particle_ordering(mz_store *h, int32_t species);
 
 /* convect_particles!(rng, grid, walls, pv, pia, species, species_data, dt)
  * (src/convection/convection_1D.jl:130-157). */
@@ -157,6 +163,13 @@ int mz_set_surf(mz_store *h, int32_t which, const double *in);
  * to group 2 in cell order, as a sequential loop over cells would. */
 int mz_ntc(mz_store *h, int32_t s1, int32_t s2, double dt, const double *V, double dw_tol,
            int32_t equal_weight, uint64_t tag, int32_t rng_mode, int32_t cell_lo, int32_t cell_hi);
+/* The caller's loop of per-cell ntc! / ntc_equal_weight! calls as one call
+ * (the loop of test/test_1D_couette.jl:67-71; collision_ntc.jl:338,479 per
+ * cell): cell cells[k] (1-based) with volume V[k], in list order, all with
+ * the pair (s1, s2), dt, dw_tol and tag.  Runs of consecutive ascending
+ * cells launch together; the result equals the calls one by one. */
+int mz_ntc_cells(mz_store *h, int32_t s1, int32_t s2, double dt, const int32_t *cells, const double *V,
+                 int32_t n, double dw_tol, int32_t equal_weight, uint64_t tag, int32_t rng_mode);
 
 /* swpm!(rng, collision_factors_swpm, collision_data, interaction, pv, pia,
  * cell, species, G, dt, V) (src/collisions/collision_swpm.jl:201-288) for
@@ -202,15 +215,30 @@ int mz_get_props_avg(mz_store *h, double *lpa, double *np, double *n, double *v,
 int mz_merge_octree(mz_store *h, int32_t species, int32_t threshold, int32_t target_np, int32_t split,
                     int32_t init_bounds, int32_t bounds_mode, int32_t max_Nbins, int32_t max_depth,
                     int32_t grid_clamp, int32_t squash, uint64_t tag, int32_t rng_mode, int64_t *n_merged);
+/* The caller's loop of per-cell merge_octree_N2_based!(rng, octree, pv, pia,
+ * cell, species, target_np[, grid]) calls as one call
+ * (merging_octree_N2.jl:1060,1088; the loop of
+ * test/test_1D_couette_varweight.jl:78-87): cells[k] (1-based) in list
+ * order, whatever their occupancy; no squash (the caller's squash_pia!
+ * follows, as in the reference).  Runs of ascending cells launch together;
+ * the result equals the calls one by one. */
+int mz_merge_octree_cells(mz_store *h, int32_t species, const int32_t *cells, int32_t n, int32_t target_np,
+                          int32_t split, int32_t init_bounds, int32_t bounds_mode, int32_t max_Nbins,
+                          int32_t max_depth, int32_t grid_clamp, uint64_t tag, int32_t rng_mode, int64_t *n_merged);
 
 /* merge_nnls_based! for every cell whose n_local exceeds `threshold`
  * (src/merging/merging_nnls.jl:1457-1510).  mim[n_mim][3] are the requested
- * multi-indices (base moments are added as the reference does).
- * status[n_cells] receives 1, -1, or 0 for cells not merged; NULL allowed. */
+ * multi-indices (base moments are added as the reference does);
+ * mim_pos[n_mim_pos][3] the spatial ones (NNLSMerge's
+ * multi_index_moments_pos, :128; at most 16).  n_rand_pairs random-pair
+ * columns follow the fictitious ones (:1004-1030), drawn from the cell's
+ * stream of the merge replay table or Philox (seed, tag).  status[n_cells]
+ * receives 1, -1, or 0 for cells not merged; NULL allowed. */
 int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim, int32_t n_mim,
-                  double vref, int32_t scaling_vref, int32_t centered_at_mean, const double *v_multipliers,
-                  int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
-                  int32_t squash, int32_t *status);
+                  const int32_t *mim_pos, int32_t n_mim_pos, double vref, int32_t scaling_vref, int32_t n_rand_pairs,
+                  int32_t centered_at_mean, const double *v_multipliers, int32_t n_vmult, double max_err,
+                  int32_t iteration_mult, double w_threshold, int32_t squash, uint64_t tag, int32_t rng_mode,
+                  int32_t *status);
 
 /* merge_roulette!(rng, pv, pia, cell, species, target_np; conservative)
  * (src/merging/merging_roulette.jl:25-184) on every cell holding more than
@@ -253,11 +281,24 @@ int mz_merge_grid(mz_store *h, int32_t species, int32_t threshold, int32_t Nx, i
  * matrix stored in FP32 and solves the final passive set again in FP64
  * (normal equations, one refinement step), re-running in method 1 every
  * cell whose FP64 solution is not positive or misses max_err (at most 35
- * moments; more run method 1). */
+ * moments; more run method 1).  Spatial moments and random pairs as
+ * mz_merge_nnls. */
 int mz_merge_nnls_cand(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim, int32_t n_mim,
-                       double vref, int32_t scaling_vref, int32_t centered_at_mean, const double *v_multipliers,
-                       int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
-                       int32_t squash, int32_t max_pcols, int32_t method, int32_t *status);
+                       const int32_t *mim_pos, int32_t n_mim_pos, double vref, int32_t scaling_vref,
+                       int32_t n_rand_pairs, int32_t centered_at_mean, const double *v_multipliers, int32_t n_vmult,
+                       double max_err, int32_t iteration_mult, double w_threshold, int32_t squash, int32_t max_pcols,
+                       int32_t method, uint64_t tag, int32_t rng_mode, int32_t *status);
+/* The caller's loop of per-cell merge_nnls_based!(rng, nnls_merging, pv, pia,
+ * cell, species; ...) calls as one call (merging_nnls.jl:1457-1510):
+ * cells[k] (1-based) in list order; status_out[k] is the reference's return
+ * value, 1 (merged) or -1 (failed).  No squash.  max_pcols < 0 takes every
+ * particle as a column (the reference's candidate set, method 0);
+ * otherwise max_pcols and method as in mz_merge_nnls_cand. */
+int mz_merge_nnls_cells(mz_store *h, int32_t species, const int32_t *cells, int32_t n, const int32_t *mim,
+                        int32_t n_mim, const int32_t *mim_pos, int32_t n_mim_pos, double vref, int32_t scaling_vref,
+                        int32_t n_rand_pairs, int32_t centered_at_mean, const double *v_multipliers,
+                        int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
+                        int32_t max_pcols, int32_t method, uint64_t tag, int32_t rng_mode, int32_t *status_out);
 /* info[0] = cells the last mz_merge_nnls_cand call merged or tried,
  * info[1] = cells its method 2 handed to the FP64 method. */
 int mz_get_nnls_info(mz_store *h, int32_t *info);

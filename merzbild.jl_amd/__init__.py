@@ -45,6 +45,12 @@ _SIGS = {
     "mz_set_seed": (C.c_int, [C.c_void_p, u64]),
     "mz_set_replay": (C.c_int, [C.c_void_p, i32, P(f64), i64, i64]),
     "mz_sort": (C.c_int, [C.c_void_p, i32, i32]),
+    "mz_restore_particle_ordering": (C.c_int, [C.c_void_p, i32]),
+    "mz_ntc_cells": (C.c_int, [C.c_void_p, i32, i32, f64, P(i32), P(f64), i32, f64, i32, u64, i32]),
+    "mz_merge_octree_cells": (C.c_int, [C.c_void_p, i32, P(i32), i32, i32, i32, i32, i32, i32, i32, i32, u64, i32,
+                                        P(i64)]),
+    "mz_merge_nnls_cells": (C.c_int, [C.c_void_p, i32, P(i32), i32, P(i32), i32, P(i32), i32, f64, i32, i32, i32,
+                                      P(f64), i32, f64, i32, f64, i32, i32, u64, i32, P(i32)]),
     "mz_compute_flux_props": (C.c_int, [C.c_void_p, P(f64), i32, i32]),
     "mz_flux_avg": (C.c_int, [C.c_void_p, f64]),
     "mz_flux_clear": (C.c_int, [C.c_void_p, i32]),
@@ -68,12 +74,12 @@ _SIGS = {
     "mz_get_props_moments": (C.c_int, [C.c_void_p, P(f64)]),
     "mz_merge_octree": (C.c_int, [C.c_void_p, i32, i32, i32, i32, i32, i32, i32, i32, i32, i32, u64, i32,
                                   P(i64)]),
-    "mz_merge_nnls": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, f64, i32, i32, P(f64), i32, f64, i32, f64,
-                                i32, P(i32)]),
+    "mz_merge_nnls": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, P(i32), i32, f64, i32, i32, i32, P(f64), i32,
+                                f64, i32, f64, i32, u64, i32, P(i32)]),
     "mz_get_nnls_info": (C.c_int, [C.c_void_p, P(i32)]),
     "mz_merge_roulette": (C.c_int, [C.c_void_p, i32, i32, i32, i32, u64, i32, P(i64)]),
-    "mz_merge_nnls_cand": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, f64, i32, i32, P(f64), i32, f64, i32,
-                                     f64, i32, i32, i32, P(i32)]),
+    "mz_merge_nnls_cand": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, P(i32), i32, f64, i32, i32, i32, P(f64),
+                                     i32, f64, i32, f64, i32, i32, i32, u64, i32, P(i32)]),
     "mz_couette_steps": (C.c_int, [C.c_void_p, i32, u64, f64, P(f64), i32, i32, P(f64), i32]),
     "mz_couette_avg_steps": (C.c_int, [C.c_void_p, i32, u64, f64, P(f64), P(f64), i32, f64]),
     "mz_props_avg": (C.c_int, [C.c_void_p, f64]),
@@ -252,6 +258,30 @@ class Grid1DUniform:
         self.min_x = self.dx * wall_offset
         self.max_x = self.L - self.dx * wall_offset
         self.V = np.full(nx, self.dx)
+        self.cells = _OneBased([_GridCell(i * self.dx, (i + 1) * self.dx, self.dx) for i in range(nx)])
+
+
+class _GridCell:
+    """Cell1D(xlo, xhi, V) of Grid1DUniform (src/grids/grid_uniform1D.jl:14-18)."""
+    __slots__ = ("xlo", "xhi", "V", "inv_V")
+
+    def __init__(self, xlo, xhi, V):
+        self.xlo, self.xhi, self.V, self.inv_V = xlo, xhi, V, 1.0 / V
+
+
+class _OneBased:
+    """grid.cells[cell] with the reference's 1-based cell numbers."""
+
+    def __init__(self, items):
+        self._items = items
+
+    def __getitem__(self, i):
+        if not 1 <= i <= len(self._items):
+            raise IndexError(f"cell {i} out of range 1..{len(self._items)}")
+        return self._items[i - 1]
+
+    def __len__(self):
+        return len(self._items)
 
 
 class MaxwellWalls1D:
@@ -468,6 +498,12 @@ def sort_particles(store, species, compute_cells=True):
 def squash_pia(store, species):
     """squash_pia!(pv, pia, species) (src/particles.jl:622-701)"""
     _check(load_library().mz_squash(store.h, species))
+
+
+def restore_particle_ordering(store, species):
+    """restore_particle_ordering!(pv, inv_map) (src/particles.jl:1086-1137):
+    the index becomes the identity, the free-id buffer length, length - 1, ..."""
+    _check(load_library().mz_restore_particle_ordering(store.h, species))
 
 
 def convect_particles(store, species, dt, tag=0, rng_mode=RNG_PHILOX, species_data=None, surf_props=None,
@@ -722,6 +758,43 @@ def merge_octree_N2_based(store, species, threshold, target_np, split=OCTREE_MID
     return n.value
 
 
+def merge_octree_cells(store, species, cells, target_np, split=OCTREE_MID, init_bin_bounds=OCTREE_INIT_MINMAX,
+                       bin_bounds_compute=OCTREE_BOUNDS_INHERIT, max_Nbins=4096, max_depth=10, grid_clamp=False,
+                       tag=0, rng_mode=RNG_PHILOX):
+    """merge_octree_N2_based! on each listed cell (1-based), in list order,
+    without squash (src/merging/merging_octree_N2.jl:1060-1094); returns
+    the merged-cell count."""
+    cl = np.ascontiguousarray(cells, dtype=np.int32)
+    n = i64(0)
+    _check(load_library().mz_merge_octree_cells(store.h, species, cl.ctypes.data_as(P(i32)), len(cl), target_np,
+                                                split, init_bin_bounds, bin_bounds_compute, max_Nbins, max_depth,
+                                                1 if grid_clamp else 0, tag, rng_mode, C.byref(n)))
+    return n.value
+
+
+def _mim3(m):
+    return np.ascontiguousarray(np.asarray(m, dtype=np.int32).reshape(-1, 3))
+
+
+def merge_nnls_cells(store, species, cells, multi_index_moments, vref=1.0, scaling="variance", n_rand_pairs=0,
+                     centered_at_mean=False, v_multipliers=(), max_err=1e-11, iteration_mult=2, w_threshold=0.0,
+                     max_pcols=-1, method="reference", multi_index_moments_pos=(), tag=0, rng_mode=RNG_PHILOX):
+    """merge_nnls_based! on each listed cell (1-based), in list order,
+    without squash (src/merging/merging_nnls.jl:1457-1510); returns the
+    reference's return value per listed cell (1 merged, -1 failed)."""
+    cl = np.ascontiguousarray(cells, dtype=np.int32)
+    mim, mp = _mim3(multi_index_moments), _mim3(multi_index_moments_pos)
+    vm = np.ascontiguousarray(v_multipliers, dtype=np.float64)
+    st = np.zeros(len(cl), dtype=np.int32)
+    _check(load_library().mz_merge_nnls_cells(store.h, species, cl.ctypes.data_as(P(i32)), len(cl),
+                                              mim.ctypes.data_as(P(i32)), len(mim), mp.ctypes.data_as(P(i32)),
+                                              len(mp), vref, 1 if scaling == "vref" else 0, n_rand_pairs,
+                                              1 if centered_at_mean else 0, _d(vm), len(vm), max_err,
+                                              iteration_mult, w_threshold, max_pcols, NNLS_METHODS[method], tag,
+                                              rng_mode, st.ctypes.data_as(P(i32))))
+    return st
+
+
 class FluxProps:
     """FluxProps (src/properties/flux_props.jl:27-58), host copies shaped
     [species, cell, component] (the reference's arrays are (component, cell,
@@ -807,16 +880,20 @@ def merge_roulette(store, species, threshold, target_np, conservative=False, tag
 
 def merge_nnls_based(store, species, threshold, multi_index_moments, vref=1.0, scaling="variance",
                      centered_at_mean=True, v_multipliers=(0.5, 1.0), max_err=1e-11, iteration_mult=2,
-                     w_threshold=0.0, squash=True):
+                     w_threshold=0.0, squash=True, multi_index_moments_pos=(), n_rand_pairs=0, tag=0,
+                     rng_mode=RNG_PHILOX):
     """merge_nnls_based! on every cell with n_local > threshold
-    (src/merging/merging_nnls.jl:1457-1510); returns the per-cell status
-    (1 merged, -1 failed, 0 not attempted)."""
-    mim = np.ascontiguousarray(np.asarray(multi_index_moments, dtype=np.int32).reshape(-1, 3))
+    (src/merging/merging_nnls.jl:1457-1510), with the spatial moments of
+    NNLSMerge's multi_index_moments_pos and n_rand_pairs random-pair
+    columns (drawn with tag); returns the per-cell status (1 merged, -1
+    failed, 0 not attempted)."""
+    mim, mp = _mim3(multi_index_moments), _mim3(multi_index_moments_pos)
     vm = np.ascontiguousarray(v_multipliers, dtype=np.float64)
     status = np.zeros(store.n_cells, dtype=np.int32)
-    _check(load_library().mz_merge_nnls(store.h, species, threshold, mim.ctypes.data_as(P(i32)), len(mim), vref,
-                                        1 if scaling == "vref" else 0, 1 if centered_at_mean else 0, _d(vm),
-                                        len(vm), max_err, iteration_mult, w_threshold, 1 if squash else 0,
+    _check(load_library().mz_merge_nnls(store.h, species, threshold, mim.ctypes.data_as(P(i32)), len(mim),
+                                        mp.ctypes.data_as(P(i32)), len(mp), vref, 1 if scaling == "vref" else 0,
+                                        n_rand_pairs, 1 if centered_at_mean else 0, _d(vm), len(vm), max_err,
+                                        iteration_mult, w_threshold, 1 if squash else 0, tag, rng_mode,
                                         status.ctypes.data_as(P(i32))))
     return status
 
@@ -826,7 +903,8 @@ NNLS_METHODS = {"reference": 0, "lds": 1, "mixed": 2}
 
 def merge_nnls_candidates(store, species, threshold, multi_index_moments, max_pcols, method="lds", vref=1.0,
                           scaling="variance", centered_at_mean=True, v_multipliers=(0.5, 1.0), max_err=1e-11,
-                          iteration_mult=2, w_threshold=0.0, squash=True):
+                          iteration_mult=2, w_threshold=0.0, squash=True, multi_index_moments_pos=(),
+                          n_rand_pairs=0, tag=0, rng_mode=RNG_PHILOX):
     """merge_nnls_based! with the columns cut to the first max_pcols particles
     plus the fictitious ones; the moments matched are those of all the cell's
     particles.  method "reference" keeps the reference's layout (bit-exact
@@ -834,14 +912,15 @@ def merge_nnls_candidates(store, species, threshold, multi_index_moments, max_pc
     matrix in LDS, "mixed" runs the active-set search on the matrix stored in
     FP32 and solves the final passive set again in FP64 (cells it cannot
     confirm run "lds").  Returns the per-cell status as merge_nnls_based."""
-    mim = np.ascontiguousarray(np.asarray(multi_index_moments, dtype=np.int32).reshape(-1, 3))
+    mim, mp = _mim3(multi_index_moments), _mim3(multi_index_moments_pos)
     vm = np.ascontiguousarray(v_multipliers, dtype=np.float64)
     status = np.zeros(store.n_cells, dtype=np.int32)
     _check(load_library().mz_merge_nnls_cand(store.h, species, threshold, mim.ctypes.data_as(P(i32)), len(mim),
-                                             vref, 1 if scaling == "vref" else 0, 1 if centered_at_mean else 0,
-                                             _d(vm), len(vm), max_err, iteration_mult, w_threshold,
-                                             1 if squash else 0, max_pcols, NNLS_METHODS[method],
-                                             status.ctypes.data_as(P(i32))))
+                                             mp.ctypes.data_as(P(i32)), len(mp), vref,
+                                             1 if scaling == "vref" else 0, n_rand_pairs,
+                                             1 if centered_at_mean else 0, _d(vm), len(vm), max_err,
+                                             iteration_mult, w_threshold, 1 if squash else 0, max_pcols,
+                                             NNLS_METHODS[method], tag, rng_mode, status.ctypes.data_as(P(i32))))
     return status
 
 
@@ -974,4 +1053,7 @@ def __getattr__(name):
     if name in _IO_NAMES:
         from . import io as _io
         return getattr(_io, name)
+    if name == "dropin":
+        import importlib
+        return importlib.import_module(__name__ + ".dropin")
     raise AttributeError(name)

@@ -98,6 +98,8 @@ struct mz_store {
     double masses_up[MZ_MAX_SPECIES] = {0.0, 0.0, 0.0, 0.0};
     bool masses_valid = false;
     std::vector<double> V_up;
+    std::vector<double> ntc_V_up;    // of the NTC calls, at ntc_V_dev
+    const void *ntc_V_dev = nullptr;
     int32_t last_path = -1;   // of mz_couette_steps: 0 eager, 1 graph, 2 slab
     int32_t n_moments = 0;    // moments of the last mz_compute_props_moments
     int32_t sort_path = 0;    // 0 automatic, 1 count matrix, 2 radix (mz_set_sort_path)
@@ -141,6 +143,7 @@ struct MzTimed {
     MzTimed &operator=(const MzTimed &) = delete;
 };
 
+// device address of the fixed scratch array on the current device
 // true when another live store's stream still has work queued
 bool mz_other_stores_busy(mz_store *h);
 void mz_swap_buffers(MzSpecies &s);

@@ -33,6 +33,7 @@ struct OctArgs {
     int32_t *old_pia;         // [n_cells][5]: start1, n1, start2, n2, n_new (n_new = -1: not merged)
     int32_t *n_merged;
     int32_t n_cells;
+    const uint8_t *mask;      // cells to merge (nullptr: every cell above the threshold)
 };
 
 struct OctBin {
@@ -145,11 +146,14 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     __shared__ int32_t cnt8[9], start8[9], run8[9];
     __shared__ double ndens8[9];
     __shared__ int32_t Nbins_s, total_post_s, stop_s;
+    // the cell's velocities and weights by logical index: the split loop
+    // reads them through the particle lists many times, from LDS
+    __shared__ double s_v0[OCT_MAX_N], s_v1[OCT_MAX_N], s_v2[OCT_MAX_N], s_w[OCT_MAX_N];
 
     const int c = blockIdx.x;
     A.rp = mz_rng_resolve(A.rp);
     const int32_t N = A.sp.pia.nloc[c];
-    if (N <= A.threshold) {
+    if (N <= A.threshold || (A.mask && !A.mask[c])) {
         if (threadIdx.x == 0) A.old_pia[c * 5 + 4] = -1;
         return;
     }
@@ -163,7 +167,14 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     const int32_t s1 = A.sp.pia.start1[c], n1 = A.sp.pia.n1[c], s2 = A.sp.pia.start2[c], n2 = A.sp.pia.n2[c];
     const int max_bins = min(A.max_Nbins, OCT_MAX_BINS);
     // init_octree! (merging_octree_N2.jl:947-984)
-    for (int i = threadIdx.x; i < N; i += blockDim.x) pis[i] = (int16_t)i;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        pis[i] = (int16_t)i;
+        const int64_t p = oct_slot(i, s1, n1, s2);
+        s_v0[i] = A.sp.v0[p];
+        s_v1[i] = A.sp.v1[p];
+        s_v2[i] = A.sp.v2[p];
+        s_w[i] = A.sp.w[p];
+    }
     double mn[3], mx[3];
     if (A.init_bounds == 3) {
         for (int d = 0; d < 3; ++d) { mn[d] = -299792458.0; mx[d] = 299792458.0; }
@@ -217,10 +228,10 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
             if (A.bounds_mode == 2) {
                 double bmn[3], bmx[3];
                 for (int d = 0; d < 3; ++d) {
-                    const double *vv = d == 0 ? A.sp.v0 : (d == 1 ? A.sp.v1 : A.sp.v2);
+                    const double *vv = d == 0 ? s_v0 : (d == 1 ? s_v1 : s_v2);
                     double lo = 9299792458.0, hi = -9299792458.0;
                     for (int i = bs + lane; i <= be; i += 64) {
-                        const double x = vv[oct_slot(pis[i], s1, n1, s2)];
+                        const double x = vv[pis[i]];
                         lo = fmin(lo, x);
                         hi = fmax(hi, x);
                     }
@@ -238,12 +249,12 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                     // compute_v_mean! keeps the previous middle velocity in the sum
                     double nt = 0.0;
                     for (int i = bs; i <= be; ++i) {
-                        int64_t p = oct_slot(pis[i], s1, n1, s2);
-                        double w = A.sp.w[p];
+                        const int li = pis[i];
+                        double w = s_w[li];
                         nt += w;
-                        vel_mid[0] = vel_mid[0] + w * A.sp.v0[p];
-                        vel_mid[1] = vel_mid[1] + w * A.sp.v1[p];
-                        vel_mid[2] = vel_mid[2] + w * A.sp.v2[p];
+                        vel_mid[0] = vel_mid[0] + w * s_v0[li];
+                        vel_mid[1] = vel_mid[1] + w * s_v1[li];
+                        vel_mid[2] = vel_mid[2] + w * s_v2[li];
                     }
                     for (int d = 0; d < 3; ++d) vel_mid[d] = vel_mid[d] / nt;
                 }
@@ -251,11 +262,11 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
             }
             oct_wave_sync();
             for (int i = bs + lane; i <= be; i += 64) {
-                int64_t p = oct_slot(pis[i], s1, n1, s2);
+                const int li = pis[i];
                 int o = 1;                        // compute_octant (:304-316): strict >
-                if (A.sp.v0[p] > vel_mid[0]) o += 1;
-                if (A.sp.v1[p] > vel_mid[1]) o += 2;
-                if (A.sp.v2[p] > vel_mid[2]) o += 4;
+                if (s_v0[li] > vel_mid[0]) o += 1;
+                if (s_v1[li] > vel_mid[1]) o += 2;
+                if (s_v2[li] > vel_mid[2]) o += 4;
                 poct[i - bs] = (uint8_t)o;
                 atomicAdd(&cnt8[o], 1);
             }
@@ -297,7 +308,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                     const int m = min(8, i - lo + 1);
                     double wb[8];
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) wb[j] = j < m ? A.sp.w[oct_slot(pis[i - j], s1, n1, s2)] : 0.0;
+                    for (int j = 0; j < 8; ++j) wb[j] = j < m ? s_w[pis[i - j]] : 0.0;
 #pragma unroll
                     for (int j = 0; j < 8; ++j)
                         if (j < m) sw += wb[j];
@@ -360,7 +371,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     const int Nbins = Nbins_s;
     if (Nbins == 1 && threadIdx.x == 0) {
         double s = 0.0;
-        for (int i = 0; i < N; ++i) s += A.sp.w[oct_slot(pis[i], s1, n1, s2)];
+        for (int i = 0; i < N; ++i) s += s_w[pis[i]];
         bins[0].w = s;
     }
     __syncthreads();
@@ -597,10 +608,60 @@ __global__ void __launch_bounds__(1024) k_scan_i32m(const int32_t *__restrict__ 
     if (t == 1023) *total = run;
 }
 
+static int octree_merge_launch(mz_store *h, int32_t species, int32_t threshold, const uint8_t *mask,
+                               int32_t target_np, int32_t split, int32_t init_bounds, int32_t bounds_mode,
+                               int32_t max_Nbins, int32_t max_depth, int32_t grid_clamp, uint64_t tag,
+                               int32_t rng_mode, int64_t *n_merged);
+
 extern "C" int mz_merge_octree(mz_store *h, int32_t species, int32_t threshold, int32_t target_np, int32_t split,
                                int32_t init_bounds, int32_t bounds_mode, int32_t max_Nbins, int32_t max_depth,
                                int32_t grid_clamp, int32_t squash, uint64_t tag, int32_t rng_mode,
                                int64_t *n_merged) {
+    int r = octree_merge_launch(h, species, threshold, nullptr, target_np, split, init_bounds, bounds_mode, max_Nbins,
+                                max_depth, grid_clamp, tag, rng_mode, n_merged);
+    if (r) return r;
+    if (squash) return mz_squash(h, species);
+    return 0;
+}
+
+// The caller's loop of per-cell merge_octree_N2_based! calls
+// (test/test_1D_couette_varweight.jl:78-87) as one call: cells[k] (1-based)
+// in list order.  Each run of ascending cells is one launch over a cell
+// mask; the runs go one after another, so freed ids reach the buffer in the
+// calls' order.  n_merged adds the runs' counts.
+extern "C" int mz_merge_octree_cells(mz_store *h, int32_t species, const int32_t *cells, int32_t n, int32_t target_np,
+                                     int32_t split, int32_t init_bounds, int32_t bounds_mode, int32_t max_Nbins,
+                                     int32_t max_depth, int32_t grid_clamp, uint64_t tag, int32_t rng_mode,
+                                     int64_t *n_merged) {
+    MZ_CHECK(n >= 0 && (n == 0 || cells != nullptr), "merge_octree_cells: bad cell list");
+    for (int32_t k = 0; k < n; ++k)
+        MZ_CHECK(cells[k] >= 1 && cells[k] <= h->n_cells, "merge_octree_cells: cell out of range");
+    if (n_merged) *n_merged = 0;
+    uint8_t *mask = (uint8_t *)mz_scratch(h, "oct_mask", (size_t)h->n_cells);
+    MZ_CHECK(mask != nullptr, "merge_octree_cells: scratch allocation failed");
+    std::vector<uint8_t> hm(h->n_cells);
+    int32_t k0 = 0;
+    while (k0 < n) {
+        int32_t k1 = k0 + 1;
+        while (k1 < n && cells[k1] > cells[k1 - 1]) ++k1;
+        std::fill(hm.begin(), hm.end(), 0);
+        for (int32_t k = k0; k < k1; ++k) hm[cells[k] - 1] = 1;
+        MZ_HIP(hipStreamSynchronize(h->stream));
+        MZ_HIP(hipMemcpy(mask, hm.data(), h->n_cells, hipMemcpyHostToDevice));
+        int64_t cnt = 0;
+        int r = octree_merge_launch(h, species, -1, mask, target_np, split, init_bounds, bounds_mode, max_Nbins,
+                                    max_depth, grid_clamp, tag, rng_mode, n_merged ? &cnt : nullptr);
+        if (r) return r;
+        if (n_merged) *n_merged += cnt;
+        k0 = k1;
+    }
+    return 0;
+}
+
+static int octree_merge_launch(mz_store *h, int32_t species, int32_t threshold, const uint8_t *mask,
+                               int32_t target_np, int32_t split, int32_t init_bounds, int32_t bounds_mode,
+                               int32_t max_Nbins, int32_t max_depth, int32_t grid_clamp, uint64_t tag,
+                               int32_t rng_mode, int64_t *n_merged) {
     if (species >= 1 && species <= h->n_species) h->sp[species - 1].contig_known = false;
     MZ_CHECK(species >= 1 && species <= h->n_species, "merge_octree: bad species");
     MZ_CHECK(split == 1 || split == 2, "merge_octree: split must be 1 (mid) or 2 (mean)");
@@ -633,6 +694,7 @@ extern "C" int mz_merge_octree(mz_store *h, int32_t species, int32_t threshold, 
     A.old_pia = old_pia;
     A.n_merged = cnt;
     A.n_cells = nc;
+    A.mask = mask;
     {
         MzTimed tm(h, "merge_octree");
         hipLaunchKernelGGL(k_octree_merge, dim3(nc), dim3(256), 0, h->stream, A);
@@ -648,7 +710,6 @@ extern "C" int mz_merge_octree(mz_store *h, int32_t species, int32_t threshold, 
         MZ_HIP(hipStreamSynchronize(h->stream));
         *n_merged = v;
     }
-    if (squash) return mz_squash(h, species);
     return 0;
 }
 
@@ -791,11 +852,11 @@ __global__ void __launch_bounds__(256) k_grid_merge(GridArgs A) {
     for (int len = 2; len <= npad; len <<= 1) {
         for (int stride = len >> 1; stride > 0; stride >>= 1) {
             for (int t = threadIdx.x; t < (npad >> 1); t += 256) {
-                const int lo = 2 * t - (t & (stride - 1));
-                const int hi = lo + stride;
-                const bool up = (lo & len) == 0;
-                const uint32_t a = sk[lo], c = sk[hi];
-                if ((a > c) == up) { sk[lo] = c; sk[hi] = a; }
+                const int ia = 2 * t - (t & (stride - 1));
+                const int ib = ia + stride;
+                const bool asc = (ia & len) == 0;
+                const uint32_t ka = sk[ia], kb = sk[ib];
+                if ((ka > kb) == asc) { sk[ia] = kb; sk[ib] = ka; }
             }
             __syncthreads();
         }
@@ -989,11 +1050,18 @@ extern "C" int mz_merge_grid(mz_store *h, int32_t species, int32_t threshold, in
 #define NNLS_MAX_ROWS 64
 #define NNLS_MAX_VMULT 4
 
+#define NNLS_MAX_POS 16
+
 struct NnlsArgs {
     MzSpView sp, tmp;
-    int32_t m;                     // rows (velocity moments; no spatial moments)
+    int32_t m;                     // rows: m_v velocity moments, then m_p spatial moments
+    int32_t m_v, m_p;
     int32_t mim[NNLS_MAX_ROWS][3];
     int32_t tot_order[NNLS_MAX_ROWS];
+    int32_t mim_pos[NNLS_MAX_POS][3];
+    int32_t pos_ix, pos_iy, pos_iz;   // 1 when the first spatial moment in x, y, z is conserved
+    int32_t n_rand;                // random-pair columns after the fictitious ones
+    MzRngParams rp;                // their draws: the cell's stream, 2 per pair
     double vref, inv_vref;
     int32_t scaling_vref, centered, n_vmult, iteration_mult;
     double vmult[NNLS_MAX_VMULT];
@@ -1016,6 +1084,64 @@ __device__ inline double nnls_ipow(double a, int p) {
     double r = 1.0;
     for (int i = 0; i < p; ++i) r *= a;
     return r;
+}
+
+// A random-pair column (compute_lhs_particles_additional!,
+// merging_nnls.jl:1004-1030): pair r draws i = floor(R n_local), then k the
+// same way (draws 2r and 2r + 1 of the cell's stream); i == k leaves the
+// column zero.  Otherwise the candidate sits at the pair's mean velocity
+// and mean position.  Returns false for a zero column.
+__device__ inline bool nnls_pair(const NnlsArgs &A, const MzRngParams &rp, int c, int r, int32_t N, int32_t s1,
+                                 int32_t n1, int32_t s2, double vm[3], double xm[3]) {
+    MzStream st = mz_stream((uint64_t)c, (uint64_t)(2 * r));
+    const int64_t i = (int64_t)floor(mz_rand(rp, st) * (double)N);
+    const int64_t k = (int64_t)floor(mz_rand(rp, st) * (double)N);
+    if (i == k) return false;
+    const int64_t pi = oct_slot((int)i, s1, n1, s2), pk = oct_slot((int)k, s1, n1, s2);
+    vm[0] = 0.5 * (A.sp.v0[pi] + A.sp.v0[pk]);
+    vm[1] = 0.5 * (A.sp.v1[pi] + A.sp.v1[pk]);
+    vm[2] = 0.5 * (A.sp.v2[pi] + A.sp.v2[pk]);
+    xm[0] = 0.5 * (A.sp.x0[pi] + A.sp.x0[pk]);
+    xm[1] = 0.5 * (A.sp.x1[pi] + A.sp.x1[pk]);
+    xm[2] = 0.5 * (A.sp.x2[pi] + A.sp.x2[pk]);
+    return true;
+}
+
+// Row q of a candidate column before scaling: velocity rows are the central
+// moments of (cx, cy, cz), the velocity relative to the mean; spatial rows
+// those of (px, py, pz), the position relative to the mean, with the
+// spatial multi-indices for particles and (as the reference writes it,
+// merging_nnls.jl:1023-1026) the velocity multi-indices for random pairs;
+// kind 0 particle, 1 zero column, 2 octant particle (zero spatial rows),
+// 3 random pair.
+__device__ inline double nnls_entry(const NnlsArgs &A, int kind, int q, double cx, double cy, double cz, double px,
+                                    double py, double pz) {
+    if (kind == 1) return 0.0;
+    if (q < A.m_v) return nnls_ipow(cx, A.mim[q][0]) * nnls_ipow(cy, A.mim[q][1]) * nnls_ipow(cz, A.mim[q][2]);
+    const int r = q - A.m_v;
+    if (kind == 0) return nnls_ipow(px, A.mim_pos[r][0]) * nnls_ipow(py, A.mim_pos[r][1]) * nnls_ipow(pz, A.mim_pos[r][2]);
+    if (kind == 3) return nnls_ipow(px, A.mim[r][0]) * nnls_ipow(py, A.mim[r][1]) * nnls_ipow(pz, A.mim[r][2]);
+    return 0.0;
+}
+
+// Row scaling (scale_lhs_rhs!, merging_nnls.jl:1178-1276): velocity rows by
+// the variances (or vref), spatial rows by the spatial variances.
+__device__ inline double nnls_rowscale(const NnlsArgs &A, int q, double Ex, double Ey, double Ez, double Epx,
+                                       double Epy, double Epz) {
+    if (q < A.m_v) {
+        if (!A.scaling_vref) {
+            const double iex = Ex > 1e-6 ? 1.0 / Ex : A.inv_vref;
+            const double iey = Ey > 1e-6 ? 1.0 / Ey : A.inv_vref;
+            const double iez = Ez > 1e-6 ? 1.0 / Ez : A.inv_vref;
+            return nnls_ipow(iex, A.mim[q][0]) * nnls_ipow(iey, A.mim[q][1]) * nnls_ipow(iez, A.mim[q][2]);
+        }
+        return nnls_ipow(A.inv_vref, A.tot_order[q]);
+    }
+    const int r = q - A.m_v;
+    const double ipx = Epx > 1e-6 ? 1.0 / Epx : 1.0;
+    const double ipy = Epy > 1e-6 ? 1.0 / Epy : 1.0;
+    const double ipz = Epz > 1e-6 ? 1.0 / Epz : 1.0;
+    return nnls_ipow(ipx, A.mim_pos[r][0]) * nnls_ipow(ipy, A.mim_pos[r][1]) * nnls_ipow(ipz, A.mim_pos[r][2]);
 }
 
 #pragma clang fp contract(off)
@@ -1053,7 +1179,8 @@ __global__ void __launch_bounds__(256) k_nnls_merge(NnlsArgs A) {
     const int32_t Np = nnls_pcols(A, N);                // particle columns
     const int m = A.m;
     const int n_add = (A.centered ? 1 : 0) + 8 * A.n_vmult;
-    const int n = Np + n_add;
+    const int n = Np + n_add + A.n_rand;
+    const MzRngParams rp = mz_rng_resolve(A.rp);
     double *Am = A.scratch + A.aoff[blockIdx.x];       // [m][n] row-major
     double *xs = Am + (int64_t)m * n;                   // x[n]
     double *ws = xs + n;                                // w[n] (dual)
@@ -1090,58 +1217,62 @@ __global__ void __launch_bounds__(256) k_nnls_merge(NnlsArgs A) {
     __syncthreads();
     const double W = st_d[12];
     const double v0x = st_d[0], v0y = st_d[1], v0z = st_d[2];
-    // ---- variances (compute_lhs_and_rhs!, :539-548): one accumulator per thread
-    if (tid < 3) {
+    // ---- variances of velocity and position (compute_lhs_and_rhs!, :539-548):
+    // one accumulator per thread
+    if (tid < 6) {
         double E = 0.0;
+        const int d = tid % 3;
         for (int li = 0; li < N; ++li) {
             int64_t p = oct_slot(li, s1, n1, s2);
-            double v = tid == 0 ? A.sp.v0[p] : (tid == 1 ? A.sp.v1[p] : A.sp.v2[p]);
+            double v = tid < 3 ? (d == 0 ? A.sp.v0[p] : (d == 1 ? A.sp.v1[p] : A.sp.v2[p]))
+                               : (d == 0 ? A.sp.x0[p] : (d == 1 ? A.sp.x1[p] : A.sp.x2[p]));
             double a = v - st_d[tid];
             E = E + A.sp.w[p] * (a * a);
         }
-        st_d[13 + tid] = sqrt(E / W);
+        st_d[tid < 3 ? 13 + tid : 18 + tid] = sqrt(E / W);
     }
     // ---- right-hand side: one row per thread, particles in order
     for (int q = tid; q < m; q += nt) {
         double r = 0.0;
         for (int li = 0; li < N; ++li) {
             int64_t p = oct_slot(li, s1, n1, s2);
-            double t = nnls_ipow(A.sp.v0[p] - v0x, A.mim[q][0]) * nnls_ipow(A.sp.v1[p] - v0y, A.mim[q][1]) *
-                       nnls_ipow(A.sp.v2[p] - v0z, A.mim[q][2]);
+            double t = nnls_entry(A, 0, q, A.sp.v0[p] - v0x, A.sp.v1[p] - v0y, A.sp.v2[p] - v0z,
+                                  A.sp.x0[p] - st_d[3], A.sp.x1[p] - st_d[4], A.sp.x2[p] - st_d[5]);
             r = r + A.sp.w[p] * t;
         }
         rhs[q] = r / W;
     }
     __syncthreads();
     const double Ex = st_d[13], Ey = st_d[14], Ez = st_d[15];
-    // ---- row scaling (scale_lhs_rhs_variance! / _vref!, :1178-1215)
+    // ---- row scaling (scale_lhs_rhs!, :1178-1276)
     if (tid < m) {
-        double rv;
-        if (!A.scaling_vref) {
-            double iex = Ex > 1e-6 ? 1.0 / Ex : A.inv_vref;
-            double iey = Ey > 1e-6 ? 1.0 / Ey : A.inv_vref;
-            double iez = Ez > 1e-6 ? 1.0 / Ez : A.inv_vref;
-            rv = nnls_ipow(iex, A.mim[tid][0]) * nnls_ipow(iey, A.mim[tid][1]) * nnls_ipow(iez, A.mim[tid][2]);
-        } else {
-            rv = nnls_ipow(A.inv_vref, A.tot_order[tid]);
-        }
+        const double rv = nnls_rowscale(A, tid, Ex, Ey, Ez, st_d[21], st_d[22], st_d[23]);
         rowscale[tid] = rv;
         b[tid] = rhs[tid] * rv;
     }
     __syncthreads();
-    // ---- columns: particles, centre, octant particles; scaled, unit norm
+    // ---- columns: particles, centre, octant particles, random pairs;
+    // scaled, unit norm
     for (int j = tid; j < n; j += nt) {
-        double cx, cy, cz;
-        bool zero_col = false;
+        double cx, cy, cz, px = 0.0, py = 0.0, pz = 0.0;
+        int kind = 0;
         if (j < Np) {
             int64_t p = oct_slot(j, s1, n1, s2);
             cx = A.sp.v0[p] - v0x; cy = A.sp.v1[p] - v0y; cz = A.sp.v2[p] - v0z;
+            px = A.sp.x0[p] - st_d[3]; py = A.sp.x1[p] - st_d[4]; pz = A.sp.x2[p] - st_d[5];
+        } else if (j >= Np + n_add) {
+            double vm[3], xm[3];
+            kind = nnls_pair(A, rp, c, j - Np - n_add, N, s1, n1, s2, vm, xm) ? 3 : 1;
+            cx = kind == 3 ? vm[0] - v0x : 0.0; cy = kind == 3 ? vm[1] - v0y : 0.0; cz = kind == 3 ? vm[2] - v0z : 0.0;
+            px = kind == 3 ? xm[0] - st_d[3] : 0.0; py = kind == 3 ? xm[1] - st_d[4] : 0.0;
+            pz = kind == 3 ? xm[2] - st_d[5] : 0.0;
         } else {
             int f = j - Np;
             if (A.centered && f == 0) {
-                zero_col = true;
+                kind = 1;
                 cx = cy = cz = 0.0;
             } else {
+                kind = 2;
                 f -= A.centered ? 1 : 0;
                 const int vm = f / 8, o = f % 8 + 1;
                 const double mult = A.vmult[vm];
@@ -1158,8 +1289,7 @@ __global__ void __launch_bounds__(256) k_nnls_merge(NnlsArgs A) {
         }
         double nn = 0.0;
         for (int q = 0; q < m; ++q) {
-            double t = zero_col ? 0.0
-                                : nnls_ipow(cx, A.mim[q][0]) * nnls_ipow(cy, A.mim[q][1]) * nnls_ipow(cz, A.mim[q][2]);
+            double t = nnls_entry(A, kind, q, cx, cy, cz, px, py, pz);
             t *= rowscale[q];
             AM(q + 1, j + 1) = t;
             nn = nn + t * t;
@@ -1471,10 +1601,16 @@ __global__ void __launch_bounds__(256) k_nnls_merge(NnlsArgs A) {
                 if (!(xs[j] > A.w_threshold)) continue;
                 const int64_t dst = oct_slot(cur, s1, n1, s2);
                 ++cur;
-                double vx, vy, vz;
+                double vx, vy, vz, xx = st_d[3], xy = st_d[4], xz = st_d[5];
                 if (j < Np) {
                     int64_t p = oct_slot(j, s1, n1, s2);
                     vx = A.sp.v0[p]; vy = A.sp.v1[p]; vz = A.sp.v2[p];
+                    xx = A.sp.x0[p]; xy = A.sp.x1[p]; xz = A.sp.x2[p];
+                } else if (j >= Np + n_add) {
+                    double vm[3] = {0, 0, 0}, xm[3] = {0, 0, 0};
+                    nnls_pair(A, rp, c, j - Np - n_add, N, s1, n1, s2, vm, xm);
+                    vx = vm[0]; vy = vm[1]; vz = vm[2];
+                    xx = xm[0]; xy = xm[1]; xz = xm[2];
                 } else {
                     int f = j - Np;
                     if (A.centered && f == 0) {
@@ -1496,7 +1632,11 @@ __global__ void __launch_bounds__(256) k_nnls_merge(NnlsArgs A) {
                 }
                 A.tmp.w[dst] = xs[j] * wt;
                 A.tmp.v0[dst] = vx; A.tmp.v1[dst] = vy; A.tmp.v2[dst] = vz;
-                A.tmp.x0[dst] = st_d[3]; A.tmp.x1[dst] = st_d[4]; A.tmp.x2[dst] = st_d[5];
+                // positions: the candidate's where the first spatial moment is
+                // conserved, else the mean (compute_post_merge_particles_nnls!)
+                A.tmp.x0[dst] = A.pos_ix ? xx : st_d[3];
+                A.tmp.x1[dst] = A.pos_iy ? xy : st_d[4];
+                A.tmp.x2[dst] = A.pos_iz ? xz : st_d[5];
             }
         }
         __syncthreads();
@@ -1597,7 +1737,7 @@ __global__ void __launch_bounds__(NF_COLS, sizeof(T) == 4 ? 4 : (ROWS <= 36 ? 2 
     const int32_t Np = nnls_pcols(A, N);
     const int m = A.m;
     const int n_add = (A.centered ? 1 : 0) + 8 * A.n_vmult;
-    const int n = Np + n_add;
+    const int n = Np + n_add + A.n_rand;
 #define AL(r, cc) Al[((r) - 1) * NF_COLS + ((cc) - 1)]
 #define RED(w, k) red[(w) * (ROWS + 8) + (k)]
     // ---- compute_w_total_v0!: W, mean velocity and position, velocity bounds
@@ -1650,26 +1790,31 @@ __global__ void __launch_bounds__(NF_COLS, sizeof(T) == 4 ? 4 : (ROWS <= 36 ? 2 
     const double v0x = sd[1], v0y = sd[2], v0z = sd[3];
     // ---- variances and right-hand side (compute_lhs_and_rhs!, :509-613)
     {
-        double acc[ROWS], e[3] = {0, 0, 0};
+        double acc[ROWS], e[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int q = 0; q < ROWS; ++q) acc[q] = 0.0;
         for (int li = tid; li < N; li += NF_COLS) {
             const int64_t p = oct_slot(li, s1, n1, s2);
             const double w = A.sp.w[p];
             const double cx = A.sp.v0[p] - v0x, cy = A.sp.v1[p] - v0y, cz = A.sp.v2[p] - v0z;
+            const double px = A.sp.x0[p] - sd[4], py = A.sp.x1[p] - sd[5], pz = A.sp.x2[p] - sd[6];
             e[0] += w * (cx * cx);
             e[1] += w * (cy * cy);
             e[2] += w * (cz * cz);
+            e[3] += w * (px * px);
+            e[4] += w * (py * py);
+            e[5] += w * (pz * pz);
 #pragma unroll
             for (int q = 0; q < ROWS; ++q) {
-                if (q < m)
-                    acc[q] += w * (nnls_ipow(cx, A.mim[q][0]) * nnls_ipow(cy, A.mim[q][1]) * nnls_ipow(cz, A.mim[q][2]));
+                if (q < m) acc[q] += w * nnls_entry(A, 0, q, cx, cy, cz, px, py, pz);
             }
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const double r = mz_wave_sum(e[k]);
             if (lane == 0) RED(wave, k) = r;
+            const double rp_ = mz_wave_sum(e[3 + k]);
+            if (lane == 0) RED(wave, ROWS + 3 + k) = rp_;
         }
 #pragma unroll
         for (int q = 0; q < ROWS; ++q) {
@@ -1682,6 +1827,9 @@ __global__ void __launch_bounds__(NF_COLS, sizeof(T) == 4 ? 4 : (ROWS <= 36 ? 2 
         if (tid < 3) {
             const double E = ((RED(0, tid) + RED(1, tid)) + RED(2, tid)) + RED(3, tid);
             sd[13 + tid] = sqrt(E / W);
+            const int kp = ROWS + 3 + tid;
+            const double Ep = ((RED(0, kp) + RED(1, kp)) + RED(2, kp)) + RED(3, kp);
+            sd[20 + tid] = sqrt(Ep / W);
         }
         if (tid >= 32 && tid < 32 + m) {
             const int q = tid - 32;
@@ -1693,15 +1841,7 @@ __global__ void __launch_bounds__(NF_COLS, sizeof(T) == 4 ? 4 : (ROWS <= 36 ? 2 
     // ---- row scaling (scale_lhs_rhs_variance! / _vref!, :1178-1215); zz holds the moments
     __shared__ double rowscale[ROWS], b0[ROWS];
     if (tid < m) {
-        double rv;
-        if (!A.scaling_vref) {
-            const double iex = Ex > 1e-6 ? 1.0 / Ex : A.inv_vref;
-            const double iey = Ey > 1e-6 ? 1.0 / Ey : A.inv_vref;
-            const double iez = Ez > 1e-6 ? 1.0 / Ez : A.inv_vref;
-            rv = nnls_ipow(iex, A.mim[tid][0]) * nnls_ipow(iey, A.mim[tid][1]) * nnls_ipow(iez, A.mim[tid][2]);
-        } else {
-            rv = nnls_ipow(A.inv_vref, A.tot_order[tid]);
-        }
+        const double rv = nnls_rowscale(A, tid, Ex, Ey, Ez, sd[20], sd[21], sd[22]);
         rowscale[tid] = rv;
         bq[tid] = zz[tid] * rv;
         b0[tid] = bq[tid];
@@ -1710,25 +1850,37 @@ __global__ void __launch_bounds__(NF_COLS, sizeof(T) == 4 ? 4 : (ROWS <= 36 ? 2 
     // ---- column j on thread j: moments of the candidate, row-scaled, unit norm
     // (scale_columns!, utils.jl:62-77); the candidate's velocity stays in
     // registers for the post-merge write
+    // and position (relative to the means, and absolute for the write)
     double cvx = 0.0, cvy = 0.0, cvz = 0.0, cinv = 1.0;
-    double cx = 0.0, cy = 0.0, cz = 0.0;   // relative to the mean velocity
-    bool zero_col = false;
+    double cpx = sd[4], cpy = sd[5], cpz = sd[6];
+    double cx = 0.0, cy = 0.0, cz = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+    int kind = 0;   // see nnls_entry
     if (tid < n) {
         const int j = tid;
         if (j < Np) {
             const int64_t p = oct_slot(j, s1, n1, s2);
             cvx = A.sp.v0[p]; cvy = A.sp.v1[p]; cvz = A.sp.v2[p];
+            cpx = A.sp.x0[p]; cpy = A.sp.x1[p]; cpz = A.sp.x2[p];
             cx = cvx - v0x; cy = cvy - v0y; cz = cvz - v0z;
+            px = cpx - sd[4]; py = cpy - sd[5]; pz = cpz - sd[6];
+        } else if (j >= Np + n_add) {
+            double vm[3] = {0, 0, 0}, xm[3] = {0, 0, 0};
+            kind = nnls_pair(A, mz_rng_resolve(A.rp), c, j - Np - n_add, N, s1, n1, s2, vm, xm) ? 3 : 1;
+            cvx = vm[0]; cvy = vm[1]; cvz = vm[2];
+            cpx = xm[0]; cpy = xm[1]; cpz = xm[2];
+            if (kind == 3) {
+                cx = cvx - v0x; cy = cvy - v0y; cz = cvz - v0z;
+                px = cpx - sd[4]; py = cpy - sd[5]; pz = cpz - sd[6];
+            }
         } else {
             const double e3[3] = {Ex, Ey, Ez};
             const double mnv[3] = {sd[7], sd[8], sd[9]}, mxv[3] = {sd[10], sd[11], sd[12]};
-            zero_col = nf_fict(A, j - Np, e3, mnv, mxv, cx, cy, cz);
+            kind = nf_fict(A, j - Np, e3, mnv, mxv, cx, cy, cz) ? 1 : 2;
             cvx = v0x + cx; cvy = v0y + cy; cvz = v0z + cz;
         }
         double nn = 0.0;
         for (int q = 0; q < m; ++q) {
-            double t = zero_col ? 0.0
-                                : nnls_ipow(cx, A.mim[q][0]) * nnls_ipow(cy, A.mim[q][1]) * nnls_ipow(cz, A.mim[q][2]);
+            double t = nnls_entry(A, kind, q, cx, cy, cz, px, py, pz);
             t *= rowscale[q];
             AL(q + 1, j + 1) = t;
             nn = nn + t * t;
@@ -2075,9 +2227,7 @@ __global__ void __launch_bounds__(NF_COLS, sizeof(T) == 4 ? 4 : (ROWS <= 36 ? 2 
         const int pc = tid < n ? ppos[tid] : -1;
         if (pc >= 0) {
             for (int q = 0; q < m; ++q) {
-                const double t = zero_col ? 0.0
-                                          : nnls_ipow(cx, A.mim[q][0]) * nnls_ipow(cy, A.mim[q][1]) *
-                                                nnls_ipow(cz, A.mim[q][2]);
+                const double t = nnls_entry(A, kind, q, cx, cy, cz, px, py, pz);
                 AP[q * NPP + pc] = (t * rowscale[q]) * cinv;
             }
         }
@@ -2182,7 +2332,9 @@ __global__ void __launch_bounds__(NF_COLS, sizeof(T) == 4 ? 4 : (ROWS <= 36 ? 2 
             const int64_t dst = oct_slot(cur, s1, n1, s2);
             A.sp.w[dst] = xj * (W / ndw);
             A.sp.v0[dst] = cvx; A.sp.v1[dst] = cvy; A.sp.v2[dst] = cvz;
-            A.sp.x0[dst] = sd[4]; A.sp.x1[dst] = sd[5]; A.sp.x2[dst] = sd[6];
+            A.sp.x0[dst] = A.pos_ix ? cpx : sd[4];
+            A.sp.x1[dst] = A.pos_iy ? cpy : sd[5];
+            A.sp.x2[dst] = A.pos_iz ? cpz : sd[6];
         }
         for (int li = n_new + tid; li < N; li += NF_COLS) A.sp.w[oct_slot(li, s1, n1, s2)] = 0.0;
     }
@@ -2211,11 +2363,18 @@ __global__ void __launch_bounds__(NF_COLS, sizeof(T) == 4 ? 4 : (ROWS <= 36 ? 2 
 
 #pragma clang fp contract(on)
 
+// cell_list (0-based, ascending) replaces the threshold's selection when given.
 static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim, int32_t n_mim,
                            double vref, int32_t scaling_vref, int32_t centered_at_mean, const double *v_multipliers,
                            int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
-                           int32_t squash, int32_t max_pcols, int32_t method, int32_t *status) {
+                           int32_t squash, int32_t max_pcols, int32_t method, int32_t *status,
+                           const int32_t *cell_list = nullptr, int32_t n_list = 0, const int32_t *mim_pos = nullptr,
+                           int32_t n_mim_pos = 0, int32_t n_rand_pairs = 0, uint64_t tag = 0, int32_t rng_mode = 0) {
     MZ_CHECK(species >= 1 && species <= h->n_species, "merge_nnls: bad species");
+    MZ_CHECK(n_mim_pos >= 0 && n_mim_pos <= NNLS_MAX_POS && (n_mim_pos == 0 || mim_pos != nullptr),
+             "merge_nnls: at most 16 spatial multi-indices");
+    MZ_CHECK(n_rand_pairs >= 0, "merge_nnls: negative n_rand_pairs");
+    MZ_CHECK(rng_mode == 0 || (rng_mode == 1 && h->d_replay[2]), "merge_nnls: replay table not set");
     MZ_CHECK(method >= 0 && method <= 2,
              "merge_nnls: method must be 0 (reference layout), 1 (LDS-resident) or 2 (FP32 search, FP64 solve)");
     MZ_CHECK(n_vmult >= 0 && n_vmult <= NNLS_MAX_VMULT, "merge_nnls: too many v_multipliers");
@@ -2240,8 +2399,29 @@ static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, cons
         }
         ++m;
     }
-    A.m = m;
     for (int q = 0; q < m; ++q) A.tot_order[q] = A.mim[q][0] + A.mim[q][1] + A.mim[q][2];
+    // spatial moments follow the velocity rows (merging_nnls.jl:128-141); the
+    // first spatial moments present decide where merged particles sit
+    A.m_v = m;
+    A.m_p = n_mim_pos;
+    A.pos_ix = A.pos_iy = A.pos_iz = 0;
+    for (int i = 0; i < n_mim_pos; ++i) {
+        for (int d = 0; d < 3; ++d) {
+            MZ_CHECK(mim_pos[3 * i + d] >= 0 && mim_pos[3 * i + d] <= 16, "merge_nnls: bad spatial multi-index");
+            A.mim_pos[i][d] = mim_pos[3 * i + d];
+        }
+        const int32_t *q = &mim_pos[3 * i];
+        if (q[0] == 1 && q[1] == 0 && q[2] == 0) A.pos_ix = 1;
+        else if (q[0] == 0 && q[1] == 1 && q[2] == 0) A.pos_iy = 1;
+        else if (q[0] == 0 && q[1] == 0 && q[2] == 1) A.pos_iz = 1;
+    }
+    // a random pair's spatial rows take the velocity multi-indices (see nnls_entry)
+    MZ_CHECK(n_rand_pairs == 0 || n_mim_pos <= m, "merge_nnls: more spatial than velocity moments with random pairs");
+    m += n_mim_pos;
+    MZ_CHECK(m <= NNLS_MAX_ROWS, "merge_nnls: too many moments");
+    A.m = m;
+    A.n_rand = n_rand_pairs;
+    A.rp = mz_rng_params(h, 2, tag, rng_mode);
     A.vref = vref;
     A.inv_vref = 1.0 / vref;
     A.scaling_vref = scaling_vref;
@@ -2259,8 +2439,8 @@ static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, cons
     A.n_cells = nc;
     A.max_pcols = max_pcols < 0 ? -1 : max_pcols;
     if (method >= 1) {
-        MZ_CHECK(max_pcols >= 0 && max_pcols + A.centered + 8 * n_vmult <= NF_COLS,
-                 "merge_nnls: the LDS-resident methods need max_pcols + fictitious columns <= 256");
+        MZ_CHECK(max_pcols >= 0 && max_pcols + A.centered + 8 * n_vmult + n_rand_pairs <= NF_COLS,
+                 "merge_nnls: the LDS-resident methods need max_pcols + fictitious columns + random pairs <= 256");
     }
     h->nnls_info[0] = 0;
     h->nnls_info[1] = 0;
@@ -2270,12 +2450,13 @@ static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, cons
     if (r) return r;
     std::vector<int32_t> nloc(nc);
     MZ_HIP(hipMemcpy(nloc.data(), s.pia.nloc, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
-    const int n_add = A.centered + 8 * n_vmult;
+    const int n_add = A.centered + 8 * n_vmult + n_rand_pairs;
     std::vector<int32_t> cells;
     std::vector<int64_t> offs;
     int64_t tot = 0;
-    for (int c = 0; c < nc; ++c) {
-        if (nloc[c] > threshold) {
+    for (int k = 0; k < (cell_list ? n_list : nc); ++k) {
+        const int c = cell_list ? cell_list[k] : k;
+        if (cell_list || nloc[c] > threshold) {
             const int64_t n = (A.max_pcols >= 0 ? std::min<int64_t>(nloc[c], A.max_pcols) : nloc[c]) + n_add;
             cells.push_back(c);
             offs.push_back(tot);
@@ -2374,12 +2555,14 @@ static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, cons
 }
 
 extern "C" int mz_merge_nnls(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim, int32_t n_mim,
-                             double vref, int32_t scaling_vref, int32_t centered_at_mean, const double *v_multipliers,
+                             const int32_t *mim_pos, int32_t n_mim_pos, double vref, int32_t scaling_vref,
+                             int32_t n_rand_pairs, int32_t centered_at_mean, const double *v_multipliers,
                              int32_t n_vmult, double max_err, int32_t iteration_mult, double w_threshold,
-                             int32_t squash, int32_t *status) {
+                             int32_t squash, uint64_t tag, int32_t rng_mode, int32_t *status) {
     if (species >= 1 && species <= h->n_species) h->sp[species - 1].contig_known = false;
     return nnls_merge_impl(h, species, threshold, mim, n_mim, vref, scaling_vref, centered_at_mean, v_multipliers,
-                           n_vmult, max_err, iteration_mult, w_threshold, squash, -1, 0, status);
+                           n_vmult, max_err, iteration_mult, w_threshold, squash, -1, 0, status, nullptr, 0, mim_pos,
+                           n_mim_pos, n_rand_pairs, tag, rng_mode);
 }
 
 extern "C" int mz_get_nnls_info(mz_store *h, int32_t *info) {
@@ -2390,13 +2573,49 @@ extern "C" int mz_get_nnls_info(mz_store *h, int32_t *info) {
 }
 
 extern "C" int mz_merge_nnls_cand(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim,
-                                  int32_t n_mim, double vref, int32_t scaling_vref, int32_t centered_at_mean,
+                                  int32_t n_mim, const int32_t *mim_pos, int32_t n_mim_pos, double vref,
+                                  int32_t scaling_vref, int32_t n_rand_pairs, int32_t centered_at_mean,
                                   const double *v_multipliers, int32_t n_vmult, double max_err,
                                   int32_t iteration_mult, double w_threshold, int32_t squash, int32_t max_pcols,
-                                  int32_t method, int32_t *status) {
+                                  int32_t method, uint64_t tag, int32_t rng_mode, int32_t *status) {
     if (species >= 1 && species <= h->n_species) h->sp[species - 1].contig_known = false;
     return nnls_merge_impl(h, species, threshold, mim, n_mim, vref, scaling_vref, centered_at_mean, v_multipliers,
-                           n_vmult, max_err, iteration_mult, w_threshold, squash, max_pcols, method, status);
+                           n_vmult, max_err, iteration_mult, w_threshold, squash, max_pcols, method, status, nullptr,
+                           0, mim_pos, n_mim_pos, n_rand_pairs, tag, rng_mode);
+}
+
+// The caller's loop of per-cell merge_nnls_based! calls as one call: cells[k]
+// (1-based) in list order, status_out[k] = 1 (merged) or -1 (failed), the
+// reference's return value (merging_nnls.jl:1457-1510).  Each run of
+// ascending cells is one launch, runs in order; method as
+// mz_merge_nnls_cand (max_pcols < 0: every particle a column, method 0).
+extern "C" int mz_merge_nnls_cells(mz_store *h, int32_t species, const int32_t *cells, int32_t n,
+                                   const int32_t *mim, int32_t n_mim, const int32_t *mim_pos, int32_t n_mim_pos,
+                                   double vref, int32_t scaling_vref, int32_t n_rand_pairs, int32_t centered_at_mean,
+                                   const double *v_multipliers, int32_t n_vmult, double max_err,
+                                   int32_t iteration_mult, double w_threshold, int32_t max_pcols, int32_t method,
+                                   uint64_t tag, int32_t rng_mode, int32_t *status_out) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "merge_nnls_cells: bad species");
+    MZ_CHECK(n >= 0 && (n == 0 || (cells != nullptr && status_out != nullptr)), "merge_nnls_cells: bad cell list");
+    for (int32_t k = 0; k < n; ++k)
+        MZ_CHECK(cells[k] >= 1 && cells[k] <= h->n_cells, "merge_nnls_cells: cell out of range");
+    h->sp[species - 1].contig_known = false;
+    std::vector<int32_t> st(h->n_cells), run;
+    int32_t k0 = 0;
+    while (k0 < n) {
+        int32_t k1 = k0 + 1;
+        while (k1 < n && cells[k1] > cells[k1 - 1]) ++k1;
+        run.clear();
+        for (int32_t k = k0; k < k1; ++k) run.push_back(cells[k] - 1);
+        int r = nnls_merge_impl(h, species, 0, mim, n_mim, vref, scaling_vref, centered_at_mean, v_multipliers,
+                                n_vmult, max_err, iteration_mult, w_threshold, 0, max_pcols < 0 ? -1 : max_pcols,
+                                max_pcols < 0 ? 0 : method, st.data(), run.data(), (int32_t)run.size(), mim_pos,
+                                n_mim_pos, n_rand_pairs, tag, rng_mode);
+        if (r) return r;
+        for (int32_t k = k0; k < k1; ++k) status_out[k] = st[cells[k] - 1];
+        k0 = k1;
+    }
+    return 0;
 }
 
 // ===========================================================================

@@ -25,6 +25,7 @@
 #include "mz_phys.h"
 
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 struct NtcArgs {
@@ -386,6 +387,23 @@ static int launch_ntc_split(mz_store *h, int s1, int s2, double dt, const double
     return 0;
 }
 
+// The cell volumes V[n_cells] of an NTC call on the device.  They upload
+// only when they differ from the last upload; an upload first waits for the
+// store's stream, whose queued kernels may still read the previous values.
+static int upload_ntc_volumes(mz_store *h, const double *V, const double **out) {
+    double *v = (double *)mz_scratch(h, "ntc_V", h->n_cells * sizeof(double));
+    MZ_CHECK(v != nullptr, "ntc: scratch allocation failed");
+    if (h->ntc_V_up.size() != (size_t)h->n_cells ||
+        std::memcmp(h->ntc_V_up.data(), V, h->n_cells * sizeof(double)) != 0 || h->ntc_V_dev != v) {
+        MZ_HIP(hipStreamSynchronize(h->stream));
+        MZ_HIP(hipMemcpy(v, V, h->n_cells * sizeof(double), hipMemcpyHostToDevice));
+        h->ntc_V_up.assign(V, V + h->n_cells);
+        h->ntc_V_dev = v;
+    }
+    *out = v;
+    return 0;
+}
+
 extern "C" int mz_ntc(mz_store *h, int32_t s1, int32_t s2, double dt, const double *V, double dw_tol,
                       int32_t equal_weight, uint64_t tag, int32_t rng_mode, int32_t cell_lo, int32_t cell_hi) {
     MZ_CHECK(s1 >= 1 && s1 <= h->n_species && s2 >= 1 && s2 <= h->n_species, "ntc: bad species");
@@ -394,16 +412,47 @@ extern "C" int mz_ntc(mz_store *h, int32_t s1, int32_t s2, double dt, const doub
     MZ_CHECK(rng_mode == 0 || (rng_mode == 1 && h->d_replay[0]), "ntc: replay table not set");
     const double *dV = h->d_V;
     if (V) {
-        double *v = (double *)mz_scratch(h, "ntc_V", h->n_cells * sizeof(double));
-        MZ_CHECK(v != nullptr, "ntc: scratch allocation failed");
-        MZ_HIP(hipMemcpy(v, V, h->n_cells * sizeof(double), hipMemcpyHostToDevice));
-        dV = v;
+        int r = upload_ntc_volumes(h, V, &dV);
+        if (r) return r;
     } else {
         MZ_CHECK(h->has_grid, "ntc: V is NULL and no grid is set");
     }
     MzTimed tm(h, "ntc");
     if (equal_weight) return mz_launch_ntc_equal(h, s1, s2, dt, dV, dw_tol, 1, tag, rng_mode, cell_lo, cell_hi);
     return launch_ntc_split(h, s1, s2, dt, dV, dw_tol, tag, rng_mode, cell_lo, cell_hi);
+}
+
+// The reference's loop of per-cell ntc! calls (test/test_1D_couette.jl:67-71)
+// as one call: cell cells[k] with volume V[k], in list order.  A maximal run
+// of consecutive ascending cells launches as one range (its cells are
+// independent and its splits append in cell order), runs launch one after
+// another, so the result is the sequential loop's for any list.
+extern "C" int mz_ntc_cells(mz_store *h, int32_t s1, int32_t s2, double dt, const int32_t *cells, const double *V,
+                            int32_t n, double dw_tol, int32_t equal_weight, uint64_t tag, int32_t rng_mode) {
+    MZ_CHECK(s1 >= 1 && s1 <= h->n_species && s2 >= 1 && s2 <= h->n_species, "ntc_cells: bad species");
+    MZ_CHECK(h->inter[(s1 - 1) * h->n_species + (s2 - 1)].set, "ntc_cells: interaction not set");
+    MZ_CHECK(rng_mode == 0 || (rng_mode == 1 && h->d_replay[0]), "ntc_cells: replay table not set");
+    MZ_CHECK(n >= 0 && (n == 0 || (cells != nullptr && V != nullptr)), "ntc_cells: bad cell list");
+    for (int32_t k = 0; k < n; ++k) MZ_CHECK(cells[k] >= 1 && cells[k] <= h->n_cells, "ntc_cells: cell out of range");
+    if (n == 0) return 0;
+    std::vector<double> full = h->ntc_V_up.size() == (size_t)h->n_cells ? h->ntc_V_up
+                                                                           : std::vector<double>(h->n_cells, 1.0);
+    MzTimed tm(h, "ntc");
+    int32_t k0 = 0;
+    while (k0 < n) {
+        int32_t k1 = k0 + 1;
+        while (k1 < n && cells[k1] == cells[k1 - 1] + 1) ++k1;
+        for (int32_t k = k0; k < k1; ++k) full[cells[k] - 1] = V[k];
+        const double *dv = nullptr;
+        int r = upload_ntc_volumes(h, full.data(), &dv);
+        if (r) return r;
+        const int32_t lo = cells[k0], hi = cells[k1 - 1];
+        r = equal_weight ? mz_launch_ntc_equal(h, s1, s2, dt, dv, dw_tol, 1, tag, rng_mode, lo, hi)
+                         : launch_ntc_split(h, s1, s2, dt, dv, dw_tol, tag, rng_mode, lo, hi);
+        if (r) return r;
+        k0 = k1;
+    }
+    return 0;
 }
 
 // ===========================================================================

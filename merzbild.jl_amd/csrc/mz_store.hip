@@ -1033,6 +1033,23 @@ static MzSpView view_b(const MzSpecies &s) {
 }
 
 // launches the three sort kernels on the current A buffers; the caller swaps
+// The count-matrix sort's three kernels over the count matrix hist
+// [n_cells][chunks] and the cell starts cstart [n_cells]; every launch
+// overwrites both in full, so neither needs clearing.
+static int launch_count_sort(mz_store *h, MzSpecies &s, int compute_cells, int32_t *hist, int32_t *cstart) {
+    const int nc = h->n_cells;
+    const int32_t nchunks = (int32_t)((s.cap + MZ_SORT_CHUNK - 1) / MZ_SORT_CHUNK);
+    MzSpView a = mz_view(s), b = view_b(s);
+    size_t lds = (size_t)nc * sizeof(int32_t);
+    hipLaunchKernelGGL(k_sort_hist, dim3(nchunks), dim3(256), lds, h->stream, a, h->grid, compute_cells, nc,
+                       nchunks, hist);
+    hipLaunchKernelGGL(k_sort_rows, dim3((nc + 3) / 4), dim3(256), 0, h->stream, a, nc, nchunks, hist, cstart,
+                       h->d_ticket);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nchunks), dim3(256), lds, h->stream, a, b, nc, nchunks, hist, cstart);
+    MZ_HIP(hipGetLastError());
+    return 0;
+}
+
 int mz_launch_sort(mz_store *h, int species, int compute_cells) {
     MzSpecies &s = h->sp[species - 1];
     int nc = h->n_cells;
@@ -1044,15 +1061,7 @@ int mz_launch_sort(mz_store *h, int species, int compute_cells) {
     int32_t *hist = (int32_t *)mz_scratch(h, "sort_hist", (size_t)nc * nchunks * sizeof(int32_t));
     int32_t *cstart = (int32_t *)mz_scratch(h, "sort_cstart", (size_t)nc * sizeof(int32_t));
     MZ_CHECK(hist != nullptr && cstart != nullptr, "sort: scratch allocation failed");
-    MzSpView a = mz_view(s), b = view_b(s);
-    size_t lds = (size_t)nc * sizeof(int32_t);
-    hipLaunchKernelGGL(k_sort_hist, dim3(nchunks), dim3(256), lds, h->stream, a, h->grid, compute_cells, nc,
-                       nchunks, hist);
-    hipLaunchKernelGGL(k_sort_rows, dim3((nc + 3) / 4), dim3(256), 0, h->stream, a, nc, nchunks, hist, cstart,
-                       h->d_ticket);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(nchunks), dim3(256), lds, h->stream, a, b, nc, nchunks, hist, cstart);
-    MZ_HIP(hipGetLastError());
-    return 0;
+    return launch_count_sort(h, s, compute_cells, hist, cstart);
 }
 
 extern "C" int mz_set_sort_path(mz_store *h, int32_t path) {
@@ -1156,6 +1165,22 @@ __global__ void k_squash_pia(MzSpView sp, int32_t n_cells, const int32_t *__rest
     }
 }
 
+// The squash kernels over the new group starts news [2][n_cells] (written
+// in full by k_squash_plan); the species' buffers swap, and it is contiguous.
+static int launch_squash(mz_store *h, MzSpecies &s, int32_t *news) {
+    const int nc = h->n_cells;
+    MzSpView a = mz_view(s), b = view_b(s);
+    b.key = s.keyb;
+    hipLaunchKernelGGL(k_squash_plan, dim3(1), dim3(1024), 0, h->stream, a, nc, news);
+    hipLaunchKernelGGL(k_squash_move, dim3(nc), dim3(256), 0, h->stream, a, b, nc, news);
+    hipLaunchKernelGGL(k_squash_pia, dim3((nc + 255) / 256), dim3(256), 0, h->stream, a, nc, news);
+    MZ_HIP(hipGetLastError());
+    mz_swap_buffers(s);
+    std::swap(s.key, s.keyb);
+    s.contig_known = true;
+    return 0;
+}
+
 extern "C" int mz_squash(mz_store *h, int32_t species) {
     MZ_CHECK(species >= 1 && species <= h->n_species, "squash: bad species");
     if (h->sp[species - 1].contig_known) return 0;
@@ -1166,19 +1191,30 @@ extern "C" int mz_squash(mz_store *h, int32_t species) {
         h->sp[species - 1].contig_known = true;
         return 0;
     }
-    MzSpecies &s = h->sp[species - 1];
-    int nc = h->n_cells;
-    int32_t *news = (int32_t *)mz_scratch(h, "squash", (size_t)2 * nc * sizeof(int32_t));
+    int32_t *news = (int32_t *)mz_scratch(h, "squash", (size_t)2 * h->n_cells * sizeof(int32_t));
     MZ_CHECK(news != nullptr, "squash: scratch allocation failed");
-    MzSpView a = mz_view(s), b = view_b(s);
-    b.key = s.keyb;
-    hipLaunchKernelGGL(k_squash_plan, dim3(1), dim3(1024), 0, h->stream, a, nc, news);
-    hipLaunchKernelGGL(k_squash_move, dim3(nc), dim3(256), 0, h->stream, a, b, nc, news);
-    hipLaunchKernelGGL(k_squash_pia, dim3((nc + 255) / 256), dim3(256), 0, h->stream, a, nc, news);
+    return launch_squash(h, h->sp[species - 1], news);
+}
+
+// ===========================================================================
+// restore_particle_ordering! (particles.jl:1086-1137).  Afterwards index[i]
+// == i for every position and the free-id buffer reads length, length - 1,
+// ...; every logical position keeps its particle.  The store already holds
+// each particle's data at its logical position, so the reference's swaps
+// of particle data have nothing to move here: only the identities change.
+__global__ void k_restore_ordering(MzSpView sp) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t len = sp.ctr->length, nb = sp.ctr->nbuffer;
+    if (p < sp.cap && p < len) sp.id[p] = (int32_t)(p + 1);
+    if (p < sp.bufcap && p < nb) sp.buffer[p] = (int32_t)(len - p);
+}
+
+extern "C" int mz_restore_particle_ordering(mz_store *h, int32_t species) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "restore_particle_ordering: bad species");
+    MzSpecies &s = h->sp[species - 1];
+    const int64_t n = std::max<int64_t>(s.cap, s.bufcap);
+    hipLaunchKernelGGL(k_restore_ordering, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, mz_view(s));
     MZ_HIP(hipGetLastError());
-    mz_swap_buffers(s);
-    std::swap(s.key, s.keyb);
-    s.contig_known = true;
     return 0;
 }
 

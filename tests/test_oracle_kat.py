@@ -607,3 +607,73 @@ def test_squash_pia_kat(mzo):
         d = pia.ind(c)
         got = [int(pv.index[p - 1]) for p in list(range(d.start1, d.end1 + 1)) + list(range(d.start2, d.end2 + 1))]
         assert got == ids_before[c]
+
+
+def nnls_particles2(mult=0.0, mult2=1.0):
+    """The 16 particles of test/test_nnls_merging.jl:14-40 (create_particles2)."""
+    v = [[1, 1, 1], [1, 1, -1], [1, -1, 1], [1, -1, -1], [-1, 1, 1], [-1, 1, -1], [-1, -1, 1], [-1, -1, -1],
+         [2, 2, 2], [2, 2, -2], [2, -2, 2], [2, -2, -2], [-2, 2, 2], [-2, 2, -2], [-2, -2, 2], [-2, -2, -2]]
+    x = [[mult, 0, 0], [0, mult, 0], [0, -mult, 0], [0, 0, mult], [0, 0.5 * mult, 0], [0.5 * mult, 0, 0],
+         [0, -0.5 * mult, 0], [0, 0, -0.5 * mult]] + [[0, 0, 0]] * 5 + \
+        [[0, 1.5 * mult, 0], [0, 0, 1.5 * mult], [-2.0 * mult, 0, 0]]
+    w = [1.0] * 13 + [mult2] * 3
+    return [(w[i], [float(a) for a in v[i]], [float(a) for a in x[i]]) for i in range(16)]
+
+
+def test_nnls_spatial_moments_kat(mzo):
+    """test/test_nnls_merging.jl:331-408: first spatial moments conserved
+    with multi_index_moments_pos = [[0,0,1],[1,0,0],[0,1,0]]; default
+    keywords (variance scaling, centre particle, v_multipliers 0.5 and 1)."""
+    parts = nnls_particles2(mult=2.0, mult2=2.0)
+    pv = make_pv(mzo, parts)
+    pia = single_cell_pia(mzo, 16)
+    nm = mzo.NNLSMerge([], mim_pos=[[0, 0, 1], [1, 0, 0], [0, 1, 0]])
+    assert (nm.s.pos_i_x, nm.s.pos_i_y, nm.s.pos_i_z) == (2, 3, 1)
+    w = np.array([p[0] for p in parts])
+    v = np.array([p[1] for p in parts])
+    x = np.array([p[2] for p in parts])
+    tw = w.sum()
+    x_mean = (w[:, None] * x).sum(0) / tw
+    v_mean = (w[:, None] * v).sum(0) / tw
+    x_std = np.sqrt((w[:, None] * (x - x_mean) ** 2).sum(0) / tw)
+    res = nm.merge(mzo.RNG(seed=1234), pv, pia, 1, 1)
+    assert nm.s.n_total_conserved == 10 and nm.s.n_moments_vel + nm.s.n_moments_pos == 10
+    assert res == 1
+    n_new = pia.ind(1).n_local
+    assert n_new < 16
+    assert np.max(np.abs(np.array(nm.s.x0) - x_mean)) < 4 * EPS
+    assert np.max(np.abs(np.array([nm.s.Epx, nm.s.Epy, nm.s.Epz]) - x_std)) < 4 * EPS
+    assert np.max(np.abs(np.array([nm.s.scalex, nm.s.scaley, nm.s.scalez]) - x_std)) < 4 * EPS
+    assert np.max(np.abs(np.array(nm.s.rhs[7:10]))) < 4 * EPS
+    ids = pv.index[:n_new]
+    nw = pv.w[ids - 1]
+    assert abs(nw.sum() - tw) < 3.75e-15
+    assert np.max(np.abs((nw[:, None] * pv.v[ids - 1]).sum(0) / nw.sum() - v_mean)) < 1e-15
+    assert np.max(np.abs((nw[:, None] * pv.x[ids - 1]).sum(0) / nw.sum() - x_mean)) < 1e-15
+    # the merged particles sit at their columns' positions, not all at x0
+    assert len({tuple(r) for r in pv.x[ids - 1]}) > 1
+
+
+def test_nnls_random_pairs_draws(mzo):
+    """Random-pair columns (merging_nnls.jl:1004-1030): pair r uses draws 2r
+    and 2r + 1 of the cell's stream, i = floor(R n_local); the merge with
+    pairs draws exactly 2 n_rand_pairs numbers and, with a replay table in
+    which every pair picks one particle twice, equals the merge without
+    pairs (those columns stay zero and never enter the solution)."""
+    parts = nnls_particles2(mult=2.0, mult2=2.0)
+    n_pairs = 4
+    # every pair draws the same particle twice: (0.5, 0.5) -> i = k = 8
+    table = np.full((1, 2 * n_pairs), 0.5)
+    pv_a, pia_a = make_pv(mzo, parts), single_cell_pia(mzo, 16)
+    pv_b, pia_b = make_pv(mzo, parts), single_cell_pia(mzo, 16)
+    nm_a, nm_b = mzo.NNLSMerge([[3, 0, 0], [0, 3, 0]]), mzo.NNLSMerge([[3, 0, 0], [0, 3, 0]])
+    ra = mzo.RNG(table=table)
+    res_a = nm_a.merge(ra, pv_a, pia_a, 1, 1, n_rand_pairs=n_pairs)
+    res_b = nm_b.merge(mzo.RNG(seed=3), pv_b, pia_b, 1, 1)
+    assert ra.s.k == 2 * n_pairs and ra.s.err == 0
+    assert nm_a.s.lhs_ncols == nm_b.s.lhs_ncols + n_pairs
+    assert res_a == res_b == 1
+    n = pia_a.ind(1).n_local
+    assert n == pia_b.ind(1).n_local
+    ia, ib = pv_a.index[:n], pv_b.index[:n]
+    assert np.array_equal(pv_a.w[ia - 1], pv_b.w[ib - 1]) and np.array_equal(pv_a.v[ia - 1], pv_b.v[ib - 1])
