@@ -110,6 +110,18 @@ int mz_set_replay(mz_store *h, int32_t kind, const double *table, int64_t rows, 
  * when 0 (src/grids/grid_sorting.jl:58-113, 128-182).  Squashes first if the
  * PIA is not contiguous. */
 int mz_sort(mz_store *h, int32_t species, int32_t compute_cells);
+/* mz_sort without allocating: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The count matrix, the
+ * cell starts and (when the species is not contiguous) the squash's group
+ * starts use a fixed device array of 2^21 int32 words (8 MiB) that each GPU
+ * holds until the process ends; no hipMalloc, hipFree or allocating library
+ * call is made.  It rejects with an error the radix path (more than 14336
+ * cells, or a count matrix over 8 x capacity words) and shapes whose
+ * n_cells x (ceil(capacity / 1024) + 1) exceeds 2^21.  Results equal
+ * mz_sort's.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (several stores' streams at once) race. */
+int mz_sort_noalloc(mz_store *h, int32_t species, int32_t compute_cells);
 /* Which sort runs (a tuning knob; the reference has one sort): 0 chooses by
  * size, 1 the per-chunk count matrix over cells (at most 14,336 cells), 2 the
  * 8-bit-digit radix sort, whose traffic does not grow with the cell count.

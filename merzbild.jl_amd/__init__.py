@@ -45,6 +45,7 @@ _SIGS = {
     "mz_set_seed": (C.c_int, [C.c_void_p, u64]),
     "mz_set_replay": (C.c_int, [C.c_void_p, i32, P(f64), i64, i64]),
     "mz_sort": (C.c_int, [C.c_void_p, i32, i32]),
+    "mz_sort_noalloc": (C.c_int, [C.c_void_p, i32, i32]),
     "mz_restore_particle_ordering": (C.c_int, [C.c_void_p, i32]),
     "mz_ntc_cells": (C.c_int, [C.c_void_p, i32, i32, f64, P(i32), P(f64), i32, f64, i32, u64, i32]),
     "mz_merge_octree_cells": (C.c_int, [C.c_void_p, i32, P(i32), i32, i32, i32, i32, i32, i32, i32, i32, u64, i32,
@@ -493,6 +494,13 @@ def sort_particles(store, species, compute_cells=True):
     """sort_particles!(gridsort, grid, pv, pia, species) (compute_cells) or
     sort_particles!(gridsort, pv, pia, species) (src/grids/grid_sorting.jl:58-182)."""
     _check(load_library().mz_sort(store.h, species, 1 if compute_cells else 0))
+
+
+def sort_particles_noalloc(store, species, compute_cells=True):
+    """sort_particles! as sort_particles, with the sort's scratch in a fixed
+    device array instead of allocated memory (for callers that cannot
+    allocate); same results."""
+    _check(load_library().mz_sort_noalloc(store.h, species, 1 if compute_cells else 0))
 
 
 def squash_pia(store, species):

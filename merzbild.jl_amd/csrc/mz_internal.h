@@ -16,6 +16,7 @@
 #define MZ_MAX_MOMENTS 8   // total-moment powers per compute_props_with_total_moments! call
 #define MZ_NTC_MAX_CANDIDATES 100000000  // per cell and call; larger counts flag an error
 #define MZ_KB 1.380649e-23  // src/constants.jl:4
+#define MZ_NOALLOC_SCRATCH_WORDS (1 << 21)   // int32 words of the fixed scratch array (8 MiB per GPU)
 
 struct MzGrid {
     double L, dx, inv_dx, min_x, max_x;
@@ -144,6 +145,7 @@ struct MzTimed {
 };
 
 // device address of the fixed scratch array on the current device
+int mz_noalloc_scratch_init(int32_t **out);
 // true when another live store's stream still has work queued
 bool mz_other_stores_busy(mz_store *h);
 void mz_swap_buffers(MzSpecies &s);

@@ -204,6 +204,9 @@ extern "C" int mz_create(int32_t n_cells, int32_t n_species, const int64_t *capa
     MZ_HIP(hipMemset(h->d_pavg, 0, pavg_bytes));
     h->inter.resize(nss);
     MZ_HIP(hipStreamSynchronize(h->stream));
+    // the fixed scratch array of mz_sort_noalloc: its address on this device
+    int rs = mz_noalloc_scratch_init(nullptr);
+    if (rs) return rs;
     {
         std::lock_guard<std::mutex> lk(g_stores_mu);
         g_stores.push_back(h);
@@ -1090,6 +1093,73 @@ extern "C" int mz_sort(mz_store *h, int32_t species, int32_t compute_cells) {
     if (r) return r;
     mz_swap_buffers(h->sp[species - 1]);
     h->sp[species - 1].contig_known = true;
+    return 0;
+}
+
+// ===========================================================================
+// The sort without allocating: the count matrix, the cell starts and the
+// squash's group starts live in `scratch`, a fixed device array that each GPU
+// holds from the library's load to the process's end.  Host code never
+// passes `scratch` itself to a kernel (in host code the name is a stand-in,
+// not a device address): the device address comes from hipGetSymbolAddress,
+// once per device, and is kept in g_scratch_dev.  Every call on a GPU shares
+// that GPU's array, so calls that overlap on one GPU race.
+static __device__ __align__(256) int32_t scratch[MZ_NOALLOC_SCRATCH_WORDS];
+static int launch_squash(mz_store *h, MzSpecies &s, int32_t *news);
+#define MZ_MAX_DEVICES 64
+static int32_t *g_scratch_dev[MZ_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// Looks up (first call on the current device) or returns the device address
+// of `scratch` on the current device.
+int mz_noalloc_scratch_init(int32_t **out) {
+    int dev = 0;
+    MZ_HIP(hipGetDevice(&dev));
+    MZ_CHECK(dev >= 0 && dev < MZ_MAX_DEVICES, "no-alloc scratch: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            mz_set_error(std::string("no-alloc scratch: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+            return -1;
+        }
+        g_scratch_dev[dev] = (int32_t *)p;
+    }
+    if (out) *out = g_scratch_dev[dev];
+    return 0;
+}
+
+extern "C" int mz_sort_noalloc(mz_store *h, int32_t species, int32_t compute_cells) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "sort_noalloc: bad species");
+    MzSpecies &s = h->sp[species - 1];
+    MZ_CHECK(!compute_cells || h->has_grid, "sort_noalloc: no grid set");
+    MZ_CHECK(!sort_uses_radix(h, s),
+             "sort_noalloc: only the count-matrix sort runs without allocating (at most 14336 cells, n_cells x "
+             "chunks <= 8 x capacity, sort path not radix)");
+    const int64_t nc = h->n_cells;
+    const int64_t nchunks = (s.cap + MZ_SORT_CHUNK - 1) / MZ_SORT_CHUNK;
+    const int64_t need = std::max<int64_t>(nc * nchunks + nc, 2 * nc);
+    MZ_CHECK(need <= MZ_NOALLOC_SCRATCH_WORDS,
+             "sort_noalloc: n_cells x (chunks + 1) exceeds the fixed scratch of 2^21 int32 words (8 MiB); "
+             "chunks = ceil(capacity / 1024)");
+    int32_t *scr = nullptr;
+    int r = mz_noalloc_scratch_init(&scr);
+    if (r) return r;
+    if (!s.contig_known) {
+        MzCounters c;
+        r = mz_read_counters(h, species, &c);
+        if (r) return r;
+        if (!c.contiguous) {
+            r = launch_squash(h, s, scr);
+            if (r) return r;
+        }
+        s.contig_known = true;
+    }
+    // no timing bracket: its events would be allocations
+    r = launch_count_sort(h, s, compute_cells, scr, scr + nc * nchunks);
+    if (r) return r;
+    mz_swap_buffers(s);
     return 0;
 }
 
