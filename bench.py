@@ -570,24 +570,27 @@ NN_PCOLS = 239
 
 def run_nnls(a, pkg):
     cs = sample_nnls_cells()
+    wav = nnls_merge_once(pkg, cs, NN_PCOLS, "wave")
     mix = nnls_merge_once(pkg, cs, NN_PCOLS, "mixed")
     lds = nnls_merge_once(pkg, cs, NN_PCOLS, "lds")
     ref = nnls_merge_once(pkg, cs, NN_PCOLS, "reference")
-    k64 = nnls_merge_once(pkg, cs, 47, "lds", n_check=0)
-    out = {"metric": "NNLS merges/s at <= 1e-10 moment error", "value": mix["merges_per_s"],
+    k64 = {m: nnls_merge_once(pkg, cs, 47, m, n_check=0) for m in ("lds", "wave")}
+    best = wav
+    out = {"metric": "NNLS merges/s at <= 1e-10 moment error", "value": best["merges_per_s"],
            "unit": "merges/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32 search, f64 solve",
-           "data": "synthetic", "wall_s": mix["wall_s"], "kernel_ms": mix["kernel_ms"], "vs_baseline": None,
+           "data": "synthetic", "wall_s": best["wall_s"], "kernel_ms": best["kernel_ms"], "vs_baseline": None,
            "config": {"workload": "32768 cells x U{200..3000} bi-Maxwellian particles, lognormal(0.5) weights; "
                                   "merge_nnls_based! with base + order <= 4 multi-index moments (35 rows), "
                                   ":variance, columns = first 239 particles + 17 fictitious",
-                      "n_particles": cs["n"], "max_pcols": NN_PCOLS, "method": "mixed",
-                      "merged": mix["merged"], "failed": mix["failed"], "refined_in_fp64": mix["fp64_redo"],
-                      "max_rel_moment_error_sampled": mix["max_rel_moment_error_sampled"],
+                      "n_particles": cs["n"], "max_pcols": NN_PCOLS, "method": "wave",
+                      "merged": best["merged"], "failed": best["failed"], "refined_in_fp64": best["fp64_redo"],
+                      "max_rel_moment_error_sampled": best["max_rel_moment_error_sampled"],
                       "rows": len(set(_mim(4)) | {(0, 0, 0)})},
-           "lds_fp64": lds, "reference_layout_fp64": ref,
-           "speedup_vs_reference_layout": mix["merges_per_s"] / ref["merges_per_s"],
-           "speedup_vs_lds_fp64": mix["merges_per_s"] / lds["merges_per_s"],
-           "k64_columns": {"merged": k64["merged"], "failed": k64["failed"]}}
+           "mixed_lds": mix, "lds_fp64": lds, "reference_layout_fp64": ref,
+           "speedup_vs_lds_fp64": best["merges_per_s"] / lds["merges_per_s"],
+           "speedup_vs_mixed_lds": best["merges_per_s"] / mix["merges_per_s"],
+           "speedup_vs_reference_layout": best["merges_per_s"] / ref["merges_per_s"],
+           "k64_columns": {m: {"merged": r["merged"], "failed": r["failed"]} for m, r in k64.items()}}
     print(json.dumps(out), flush=True)
 
 

@@ -906,7 +906,7 @@ def merge_nnls_based(store, species, threshold, multi_index_moments, vref=1.0, s
     return status
 
 
-NNLS_METHODS = {"reference": 0, "lds": 1, "mixed": 2}
+NNLS_METHODS = {"reference": 0, "lds": 1, "mixed": 2, "wave": 3}
 
 
 def merge_nnls_candidates(store, species, threshold, multi_index_moments, max_pcols, method="lds", vref=1.0,
@@ -919,7 +919,9 @@ def merge_nnls_candidates(store, species, threshold, multi_index_moments, max_pc
     with the CPU restatement), "lds" runs one workgroup per cell with the
     matrix in LDS, "mixed" runs the active-set search on the matrix stored in
     FP32 and solves the final passive set again in FP64 (cells it cannot
-    confirm run "lds").  Returns the per-cell status as merge_nnls_based."""
+    confirm run "lds"), "wave" does the same with one wavefront per cell and
+    the FP32 matrix in registers.  Returns the per-cell status as
+    merge_nnls_based."""
     mim, mp = _mim3(multi_index_moments), _mim3(multi_index_moments_pos)
     vm = np.ascontiguousarray(v_multipliers, dtype=np.float64)
     status = np.zeros(store.n_cells, dtype=np.int32)

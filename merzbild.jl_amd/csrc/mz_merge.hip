@@ -146,9 +146,6 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     __shared__ int32_t cnt8[9], start8[9], run8[9];
     __shared__ double ndens8[9];
     __shared__ int32_t Nbins_s, total_post_s, stop_s;
-    // the cell's velocities and weights by logical index: the split loop
-    // reads them through the particle lists many times, from LDS
-    __shared__ double s_v0[OCT_MAX_N], s_v1[OCT_MAX_N], s_v2[OCT_MAX_N], s_w[OCT_MAX_N];
 
     const int c = blockIdx.x;
     A.rp = mz_rng_resolve(A.rp);
@@ -167,14 +164,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     const int32_t s1 = A.sp.pia.start1[c], n1 = A.sp.pia.n1[c], s2 = A.sp.pia.start2[c], n2 = A.sp.pia.n2[c];
     const int max_bins = min(A.max_Nbins, OCT_MAX_BINS);
     // init_octree! (merging_octree_N2.jl:947-984)
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        pis[i] = (int16_t)i;
-        const int64_t p = oct_slot(i, s1, n1, s2);
-        s_v0[i] = A.sp.v0[p];
-        s_v1[i] = A.sp.v1[p];
-        s_v2[i] = A.sp.v2[p];
-        s_w[i] = A.sp.w[p];
-    }
+    for (int i = threadIdx.x; i < N; i += blockDim.x) pis[i] = (int16_t)i;
     double mn[3], mx[3];
     if (A.init_bounds == 3) {
         for (int d = 0; d < 3; ++d) { mn[d] = -299792458.0; mx[d] = 299792458.0; }
@@ -228,10 +218,10 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
             if (A.bounds_mode == 2) {
                 double bmn[3], bmx[3];
                 for (int d = 0; d < 3; ++d) {
-                    const double *vv = d == 0 ? s_v0 : (d == 1 ? s_v1 : s_v2);
+                    const double *vv = d == 0 ? A.sp.v0 : (d == 1 ? A.sp.v1 : A.sp.v2);
                     double lo = 9299792458.0, hi = -9299792458.0;
                     for (int i = bs + lane; i <= be; i += 64) {
-                        const double x = vv[pis[i]];
+                        const double x = vv[oct_slot(pis[i], s1, n1, s2)];
                         lo = fmin(lo, x);
                         hi = fmax(hi, x);
                     }
@@ -249,12 +239,12 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                     // compute_v_mean! keeps the previous middle velocity in the sum
                     double nt = 0.0;
                     for (int i = bs; i <= be; ++i) {
-                        const int li = pis[i];
-                        double w = s_w[li];
+                        int64_t p = oct_slot(pis[i], s1, n1, s2);
+                        double w = A.sp.w[p];
                         nt += w;
-                        vel_mid[0] = vel_mid[0] + w * s_v0[li];
-                        vel_mid[1] = vel_mid[1] + w * s_v1[li];
-                        vel_mid[2] = vel_mid[2] + w * s_v2[li];
+                        vel_mid[0] = vel_mid[0] + w * A.sp.v0[p];
+                        vel_mid[1] = vel_mid[1] + w * A.sp.v1[p];
+                        vel_mid[2] = vel_mid[2] + w * A.sp.v2[p];
                     }
                     for (int d = 0; d < 3; ++d) vel_mid[d] = vel_mid[d] / nt;
                 }
@@ -262,11 +252,11 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
             }
             oct_wave_sync();
             for (int i = bs + lane; i <= be; i += 64) {
-                const int li = pis[i];
+                int64_t p = oct_slot(pis[i], s1, n1, s2);
                 int o = 1;                        // compute_octant (:304-316): strict >
-                if (s_v0[li] > vel_mid[0]) o += 1;
-                if (s_v1[li] > vel_mid[1]) o += 2;
-                if (s_v2[li] > vel_mid[2]) o += 4;
+                if (A.sp.v0[p] > vel_mid[0]) o += 1;
+                if (A.sp.v1[p] > vel_mid[1]) o += 2;
+                if (A.sp.v2[p] > vel_mid[2]) o += 4;
                 poct[i - bs] = (uint8_t)o;
                 atomicAdd(&cnt8[o], 1);
             }
@@ -308,7 +298,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                     const int m = min(8, i - lo + 1);
                     double wb[8];
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) wb[j] = j < m ? s_w[pis[i - j]] : 0.0;
+                    for (int j = 0; j < 8; ++j) wb[j] = j < m ? A.sp.w[oct_slot(pis[i - j], s1, n1, s2)] : 0.0;
 #pragma unroll
                     for (int j = 0; j < 8; ++j)
                         if (j < m) sw += wb[j];
@@ -371,7 +361,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     const int Nbins = Nbins_s;
     if (Nbins == 1 && threadIdx.x == 0) {
         double s = 0.0;
-        for (int i = 0; i < N; ++i) s += s_w[pis[i]];
+        for (int i = 0; i < N; ++i) s += A.sp.w[oct_slot(pis[i], s1, n1, s2)];
         bins[0].w = s;
     }
     __syncthreads();
@@ -2361,6 +2351,714 @@ __global__ void __launch_bounds__(NF_COLS, sizeof(T) == 4 ? 4 : (ROWS <= 36 ? 2 
 #undef AL
 }
 
+// ===========================================================================
+// Candidate NNLS merging with one wavefront per cell (method 3).  The
+// moments, the right-hand side and the candidate columns are built in FP64
+// as in k_nnls_fast.  The Lawson-Hanson search (nnls.jl:273-548) then runs
+// on the scaled moment matrix rounded to FP32 and held in registers: lane l
+// keeps columns l, l + 64, l + 128 and l + 192, NW_ROWS values each, so the
+// loops over columns (duals, Householder updates, Givens rotations) are
+// register arithmetic with the Householder vector, the right-hand side and
+// the rotations broadcast from LDS.  The triangular factor R of the passive
+// columns sits in LDS, one slot of NW_RS floats per passive column; the
+// right-hand side, the solution and the Householder scalars are FP64 with
+// one lane per row.  The final passive set P is solved again in FP64 from
+// the exact columns (normal equations, Cholesky, one refinement step); a
+// cell whose FP64 solution is not positive, whose residual exceeds max_err
+// or which would keep all its particles goes to the FP64 method (status 2).
+// Two choices differ from the reference's sequence: a coefficient found
+// infeasible after a step leaves P from its own position (Lawson and
+// Hanson's step 11), and ties between duals go to the smaller column.
+#define NW_ROWS 35
+#define NW_RS 36          // LDS stride of an R slot and length of the broadcast vectors
+#define NW_K 4            // columns per lane
+#define NW_RB 4           // rows per block of the register loops
+
+struct NwCand {           // a candidate: where it sits, and its moments' arguments
+    double vx, vy, vz, xx, xy, xz;   // velocity and position written for a merged particle
+    double cx, cy, cz, px, py, pz;   // relative to the cell's mean velocity and position
+    int kind;                        // as nnls_entry
+};
+
+// Candidate column j of a cell (particle, fictitious particle or random
+// pair), as the column set-up of k_nnls_fast; st holds W, the mean
+// velocity, the mean position, the velocity bounds and the deviations.
+__device__ inline NwCand nw_candidate(const NnlsArgs &A, int j, int c, int32_t N, int32_t Np, int n_add, int32_t s1,
+                                      int32_t n1, int32_t s2, const double *st) {
+    NwCand r;
+    r.vx = r.vy = r.vz = 0.0;
+    r.xx = st[4]; r.xy = st[5]; r.xz = st[6];
+    r.cx = r.cy = r.cz = r.px = r.py = r.pz = 0.0;
+    r.kind = 0;
+    if (j < Np) {
+        const int64_t p = oct_slot(j, s1, n1, s2);
+        r.vx = A.sp.v0[p]; r.vy = A.sp.v1[p]; r.vz = A.sp.v2[p];
+        r.xx = A.sp.x0[p]; r.xy = A.sp.x1[p]; r.xz = A.sp.x2[p];
+        r.cx = r.vx - st[1]; r.cy = r.vy - st[2]; r.cz = r.vz - st[3];
+        r.px = r.xx - st[4]; r.py = r.xy - st[5]; r.pz = r.xz - st[6];
+    } else if (j >= Np + n_add) {
+        double vm[3] = {0, 0, 0}, xm[3] = {0, 0, 0};
+        r.kind = nnls_pair(A, mz_rng_resolve(A.rp), c, j - Np - n_add, N, s1, n1, s2, vm, xm) ? 3 : 1;
+        r.vx = vm[0]; r.vy = vm[1]; r.vz = vm[2];
+        r.xx = xm[0]; r.xy = xm[1]; r.xz = xm[2];
+        if (r.kind == 3) {
+            r.cx = r.vx - st[1]; r.cy = r.vy - st[2]; r.cz = r.vz - st[3];
+            r.px = r.xx - st[4]; r.py = r.xy - st[5]; r.pz = r.xz - st[6];
+        }
+    } else {
+        const double e3[3] = {st[13], st[14], st[15]};
+        const double mnv[3] = {st[7], st[8], st[9]}, mxv[3] = {st[10], st[11], st[12]};
+        r.kind = nf_fict(A, j - Np, e3, mnv, mxv, r.cx, r.cy, r.cz) ? 1 : 2;
+        r.vx = st[1] + r.cx; r.vy = st[2] + r.cy; r.vz = st[3] + r.cz;
+    }
+    return r;
+}
+
+__device__ inline float nw_readlane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Row exponents of the search matrix are read from LDS, packed a | b << 8 |
+// c << 16 per row: ev for the velocity rows, ep for the spatial rows of
+// particles, er for the spatial rows of random pairs (the velocity
+// multi-indices, as nnls_entry); indexing the argument block by a row number
+// inside the unrolled row loops would make the compiler copy the whole block
+// to scratch.
+__device__ inline int32_t nw_pack3(const int32_t *e) { return e[0] | (e[1] << 8) | (e[2] << 16); }
+__device__ inline double nw_pow3(int32_t e, double x, double y, double z) {
+    e = __builtin_amdgcn_readfirstlane(e);
+    return nnls_ipow(x, e & 255) * nnls_ipow(y, (e >> 8) & 255) * nnls_ipow(z, e >> 16);
+}
+// nnls_entry with the exponents from the LDS tables
+__device__ inline double nw_entry(const int32_t *ev, const int32_t *ep, const int32_t *er, int m_v, int kind, int q,
+                                  double cx, double cy, double cz, double px, double py, double pz) {
+    if (kind == 1) return 0.0;
+    if (q < m_v) return nw_pow3(ev[q], cx, cy, cz);
+    if (kind == 0) return nw_pow3(ep[q], px, py, pz);
+    if (kind == 3) return nw_pow3(er[q], px, py, pz);
+    return 0.0;
+}
+
+// Column j of the search matrix: the candidate's moments, row-scaled by rs,
+// unit norm (scale_columns!, utils.jl:62-77), rounded to FP32; zero for
+// j >= n.  Clears the column's x.
+__device__ __forceinline__ void nw_build_column(const NnlsArgs &A, float (&col)[NW_ROWS], int j, int n, int c,
+                                                int32_t N, int32_t Np, int n_add, int32_t s1, int32_t n1,
+                                                int32_t s2, const double *st, const double *rs, double *xs,
+                                                const int32_t *ev, const int32_t *ep, const int32_t *er, int m,
+                                                int m_v) {
+    double nn = 0.0;
+    if (j < n) {
+        const NwCand cd = nw_candidate(A, j, c, N, Np, n_add, s1, n1, s2, st);
+#pragma unroll
+        for (int q = 0; q < NW_ROWS; ++q) {
+            double t = 0.0;
+            if (q < m) t = nw_entry(ev, ep, er, m_v, cd.kind, q, cd.cx, cd.cy, cd.cz, cd.px, cd.py, cd.pz) * rs[q];
+            col[q] = (float)t;
+            nn = nn + t * t;
+        }
+        xs[j] = 0.0;
+    } else {
+#pragma unroll
+        for (int q = 0; q < NW_ROWS; ++q) col[q] = 0.0f;
+    }
+    nn = sqrt(nn);
+    nn = nn > 1e-15 ? nn : 1.0;
+    const float ci = (float)(1.0 / nn);
+#pragma unroll
+    for (int q = 0; q < NW_ROWS; ++q) col[q] *= ci;
+}
+
+__global__ void __launch_bounds__(64) k_nnls_wave(NnlsArgs A) {
+    constexpr int R = NW_ROWS, RS = NW_RS;
+    // the FP64 passive-set system (A_P [row][position], packed lower G) aliases
+    // the search's x by column (FP64) and R by slot (FP32)
+    constexpr int POL = R * R + R * (R + 1) / 2;
+    constexpr int SRCH = NF_COLS + (R * RS + 1) / 2;
+    __shared__ __align__(16) double s_area[POL > SRCH ? POL : SRCH];
+    __shared__ __align__(16) float s_u[RS], s_bz[RS], s_col[RS];
+    __shared__ float2 s_cs[RS];
+    __shared__ int32_t s_pcol[RS], s_pslot[RS];
+    __shared__ double s_rs[RS], s_gy[RS], s_gr[RS], s_st[16];
+    __shared__ int32_t s_ev[RS], s_ep[RS], s_er[RS];
+    double *s_xs = s_area;
+    float *s_R = reinterpret_cast<float *>(s_area + NF_COLS);
+    const int c = A.cells[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int32_t s1 = A.sp.pia.start1[c], n1 = A.sp.pia.n1[c], s2 = A.sp.pia.start2[c], n2 = A.sp.pia.n2[c];
+    const int32_t N = n1 + n2;
+    const int32_t Np = nnls_pcols(A, N);
+    const int m = A.m;
+    const int n_add = (A.centered ? 1 : 0) + 8 * A.n_vmult;
+    const int n = Np + n_add + A.n_rand;
+    const int m_v = A.m_v;
+    if (lane < m) {
+        if (lane < m_v) {
+            s_ev[lane] = nw_pack3(A.mim[lane]);
+        } else {
+            s_ep[lane] = nw_pack3(A.mim_pos[lane - m_v]);
+            s_er[lane] = nw_pack3(A.mim[lane - m_v]);
+        }
+    }
+    // ---- compute_w_total_v0!: W, mean velocity and position, velocity bounds
+    {
+        double Ws = 0.0, sv[3] = {0, 0, 0}, sx[3] = {0, 0, 0};
+        double mn[3] = {299792458.0, 299792458.0, 299792458.0};
+        double mx[3] = {-299792458.0, -299792458.0, -299792458.0};
+        for (int li = lane; li < N; li += 64) {
+            const int64_t p = oct_slot(li, s1, n1, s2);
+            const double w = A.sp.w[p], v[3] = {A.sp.v0[p], A.sp.v1[p], A.sp.v2[p]};
+            const double xx[3] = {A.sp.x0[p], A.sp.x1[p], A.sp.x2[p]};
+            Ws += w;
+            for (int d = 0; d < 3; ++d) {
+                sv[d] += w * v[d];
+                sx[d] += w * xx[d];
+                mn[d] = fmin(mn[d], v[d]);
+                mx[d] = fmax(mx[d], v[d]);
+            }
+        }
+        const double Wt = mz_wave_sum(Ws);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double v0 = mz_wave_sum(sv[d]) / Wt, x0 = mz_wave_sum(sx[d]) / Wt;
+            const double lo = mz_wave_min(mn[d]) - v0, hi = mz_wave_max(mx[d]) - v0;
+            if (lane == 0) { s_st[1 + d] = v0; s_st[4 + d] = x0; s_st[7 + d] = lo; s_st[10 + d] = hi; }
+        }
+        if (lane == 0) s_st[0] = Wt;
+    }
+    oct_wave_sync();
+    const double W = s_st[0];
+    // ---- variances and right-hand side (compute_lhs_and_rhs!, :509-613), row scaling (:1178-1215)
+    double bq = 0.0;   // lane q < m: row q of the scaled right-hand side, then of Q^T b
+    {
+        const double v0x = s_st[1], v0y = s_st[2], v0z = s_st[3], x0x = s_st[4], x0y = s_st[5], x0z = s_st[6];
+        double acc[R], e[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < R; ++q) acc[q] = 0.0;
+        for (int li = lane; li < N; li += 64) {
+            const int64_t p = oct_slot(li, s1, n1, s2);
+            const double w = A.sp.w[p];
+            const double cx = A.sp.v0[p] - v0x, cy = A.sp.v1[p] - v0y, cz = A.sp.v2[p] - v0z;
+            const double px = A.sp.x0[p] - x0x, py = A.sp.x1[p] - x0y, pz = A.sp.x2[p] - x0z;
+            e[0] += w * (cx * cx);
+            e[1] += w * (cy * cy);
+            e[2] += w * (cz * cz);
+            e[3] += w * (px * px);
+            e[4] += w * (py * py);
+            e[5] += w * (pz * pz);
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if (q < m) acc[q] += w * nw_entry(s_ev, s_ep, s_er, m_v, 0, q, cx, cy, cz, px, py, pz);
+            }
+        }
+        double Es[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) Es[k] = sqrt(mz_wave_sum(e[k]) / W);
+        double mom = 0.0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            if (q < m) {
+                const double t = mz_wave_sum(acc[q]);
+                if (lane == q) mom = t / W;
+            }
+        }
+        if (lane == 0) { s_st[13] = Es[0]; s_st[14] = Es[1]; s_st[15] = Es[2]; }
+        if (lane < m) {
+            const double rv = nnls_rowscale(A, lane, Es[0], Es[1], Es[2], Es[3], Es[4], Es[5]);
+            s_rs[lane] = rv;
+            bq = mom * rv;
+        }
+    }
+    const double b0 = bq;
+    oct_wave_sync();
+    // ---- the candidate columns: row-scaled, unit norm (scale_columns!, utils.jl:62-77), rounded to FP32
+    float a[NW_K][R];
+    float wd[NW_K];     // dual of each column
+    int ppos[NW_K];     // position in P; -1 in Z; -2 no column
+#pragma unroll
+    for (int k = 0; k < NW_K; ++k) {
+        wd[k] = 0.0f;
+        ppos[k] = k * 64 + lane < n ? -1 : -2;
+    }
+    // one call per register slot (a loop over the slots is too long for the
+    // compiler to unroll, and a[k] must keep constant indices)
+    static_assert(NW_K == 4, "one call per register slot");
+    nw_build_column(A, a[0], lane, n, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_xs, s_ev, s_ep, s_er, m,
+                    m_v);
+    nw_build_column(A, a[1], 64 + lane, n, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_xs, s_ev, s_ep, s_er, m,
+                    m_v);
+    nw_build_column(A, a[2], 128 + lane, n, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_xs, s_ev, s_ep, s_er, m,
+                    m_v);
+    nw_build_column(A, a[3], 192 + lane, n, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_xs, s_ev, s_ep, s_er, m,
+                    m_v);
+    if (lane < RS) {
+        s_pslot[lane] = lane;
+        s_pcol[lane] = 0;
+        s_u[lane] = 0.0f;
+        s_bz[lane] = 0.0f;
+        s_col[lane] = 0.0f;
+    }
+    oct_wave_sync();
+
+    // ---- Lawson-Hanson solve!(work, iteration_mult * n, 1e-14)
+    const double kkt_tol = 1e-14, factor = 0.01;
+    const int64_t max_iter = (int64_t)A.iteration_mult * n;
+    int nsetp = 0;
+    int64_t iter = 0;
+    double up = 0.0;
+    double z = 0.0;   // lane t < nsetp: solution of R z = (Q^T b)[0, nsetp) at position t
+    bool duals_ok = false;
+    // z <- R^-1 (Q^T b) over the nsetp positions, one lane per row; each step's
+    // column of R is loaded one step ahead
+    auto back_subst = [&]() {
+        const int ns = nsetp;
+        const int sl = lane < ns ? s_pslot[lane] : 0;
+        const double rdk = lane < ns ? 1.0 / (double)s_R[sl * RS + lane] : 0.0;
+        double y = lane < ns ? bq : 0.0;
+        auto rcol = [&](int ip) {
+            const int sp = __builtin_amdgcn_readlane(sl, ip >= 1 ? ip : 0);
+            return (ip >= 1 && lane < ip) ? (double)s_R[sp * RS + lane] : 0.0;
+        };
+        double p1 = rcol(ns - 1), p2 = rcol(ns - 2);
+        for (int ip = ns - 1; ip >= 0; --ip) {
+            if (lane == ip) y = y * rdk;
+            const double zip = mz_readlane_d(y, ip);
+            if (lane < ip) y = y - p1 * zip;
+            p1 = p2;
+            p2 = rcol(ip - 2);
+        }
+        z = y;
+    };
+    // duals of the zero set over rows >= lo: w = A_Z^T (Q^T b) with rows < lo zero
+    auto duals = [&](int lo) {
+        float d[NW_K] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int qb = 0; qb < RS; qb += NW_RB) {
+            if (qb + NW_RB > lo) {
+                const float4 bz = *reinterpret_cast<const float4 *>(s_bz + qb);
+                const float bv[NW_RB] = {bz.x, bz.y, bz.z, bz.w};
+#pragma unroll
+                for (int k = 0; k < NW_K; ++k)
+#pragma unroll
+                    for (int r = 0; r < NW_RB; ++r)
+                        if (qb + r < R) d[k] = __builtin_fmaf(a[k][qb + r], bv[r], d[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NW_K; ++k)
+            if (ppos[k] == -1) wd[k] = d[k];
+    };
+    for (;;) {
+        if (nsetp >= n || nsetp >= m) break;
+        if (!duals_ok) {
+            if (lane < RS) s_bz[lane] = (lane >= nsetp && lane < m) ? (float)bq : 0.0f;
+            oct_wave_sync();
+            duals(nsetp);
+        }
+        // the largest positive dual; construct_householder! on rows nsetp..m-1 of
+        // its column, the column test and apply_householder! to a copy of b
+        int bj = -1;
+        double top = 0.0, av = 0.0;
+        bool stop = false;
+        for (;;) {
+            uint64_t key = 0;
+#pragma unroll
+            for (int k = 0; k < NW_K; ++k) {
+                if (ppos[k] == -1 && wd[k] > 0.0f) {
+                    const uint64_t kk = ((uint64_t)__float_as_uint(wd[k]) << 32) | (uint32_t)(0xffff - (k * 64 + lane));
+                    key = kk > key ? kk : key;
+                }
+            }
+            key = mz_wave_max_u64(key);
+            if (key == 0 || !((double)__uint_as_float((uint32_t)(key >> 32)) > kkt_tol)) {
+                stop = true;
+                break;
+            }
+            bj = 0xffff - (int)(key & 0xffffu);
+            const int bk = bj >> 6, bl = bj & 63;
+            // the column's register slot as a mask: a test k == bk would let the
+            // compiler index the register array by bk
+            const uint32_t bsel = 1u << bk;
+#pragma unroll
+            for (int k = 0; k < NW_K; ++k) {
+                if (((bsel >> k) & 1u) && lane == bl) {
+#pragma unroll
+                    for (int q = 0; q < R; ++q) s_col[q] = a[k][q];
+                }
+            }
+            oct_wave_sync();
+            av = lane < m ? (double)s_col[lane] : 0.0;
+            const int len = m - nsetp;
+            const bool hh = lane >= nsetp && lane < m;
+            const double asave = mz_readlane_d(av, nsetp);
+            double upn = up;
+            top = asave;
+            if (len > 1) {
+                double cl = mz_wave_max(hh ? fabs(av) : 0.0);
+                const double clinv = 1.0 / cl;
+                const double t = hh ? av * clinv : 0.0;
+                cl *= sqrt(mz_wave_sum(t * t));
+                if (asave > 0) cl = -cl;
+                upn = asave - cl;
+                top = cl;
+            }
+            const double unorm = sqrt(mz_wave_sum(lane < nsetp ? av * av : 0.0));
+            bool accept = false;
+            double zr = lane < m ? bq : 0.0;
+            if (((unorm + fabs(top) * factor) - unorm) > 0) {
+                if (len > 1) {
+                    double bb = upn * top;
+                    if (bb < 0) {
+                        bb = 1.0 / bb;
+                        const double ur = lane == nsetp ? upn : (hh ? av : 0.0);
+                        double sm = mz_wave_sum(zr * ur);
+                        if (sm != 0) {
+                            sm *= bb;
+                            if (hh) zr = zr + sm * ur;
+                        }
+                    }
+                }
+                const double ztest = mz_readlane_d(zr, nsetp) / top;
+                accept = ztest > 0;
+            }
+            up = upn;
+            if (accept) {
+                bq = zr;
+                break;
+            }
+#pragma unroll
+            for (int k = 0; k < NW_K; ++k)
+                if (((bsel >> k) & 1u) && lane == bl) wd[k] = 0.0f;
+        }
+        if (stop) break;
+        // move column bj from Z to P at position p, transform the remaining columns
+        const int p = nsetp;
+        const int bl = bj & 63;
+        const uint32_t bsel = 1u << (bj >> 6);
+        nsetp = p + 1;
+#pragma unroll
+        for (int k = 0; k < NW_K; ++k)
+            if (((bsel >> k) & 1u) && lane == bl) ppos[k] = p;
+        if (lane < RS) {
+            s_u[lane] = lane == p ? (float)up : ((lane > p && lane < m) ? (float)av : 0.0f);
+            s_bz[lane] = (lane >= nsetp && lane < m) ? (float)bq : 0.0f;
+        }
+        if (lane == 0) s_pcol[p] = bj;
+        oct_wave_sync();
+        {
+            const double bbd = up * top;
+            const bool apply = m - p > 1 && bbd < 0;
+            const float bbf = (float)(1.0 / bbd);
+            float sm[NW_K] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (apply) {
+#pragma unroll
+                for (int qb = 0; qb < RS; qb += NW_RB) {
+                    if (qb + NW_RB > p) {
+                        const float4 u4 = *reinterpret_cast<const float4 *>(s_u + qb);
+                        const float uv[NW_RB] = {u4.x, u4.y, u4.z, u4.w};
+#pragma unroll
+                        for (int k = 0; k < NW_K; ++k)
+#pragma unroll
+                            for (int r = 0; r < NW_RB; ++r)
+                                if (qb + r < R) sm[k] = __builtin_fmaf(a[k][qb + r], uv[r], sm[k]);
+                    }
+                }
+            }
+            float f[NW_K], d[NW_K];
+#pragma unroll
+            for (int k = 0; k < NW_K; ++k) {
+                f[k] = (apply && ppos[k] == -1 && sm[k] != 0.0f) ? sm[k] * bbf : 0.0f;
+                d[k] = 0.0f;
+            }
+#pragma unroll
+            for (int qb = 0; qb < RS; qb += NW_RB) {
+                if (qb + NW_RB > p) {
+                    const float4 u4 = *reinterpret_cast<const float4 *>(s_u + qb);
+                    const float4 b4 = *reinterpret_cast<const float4 *>(s_bz + qb);
+                    const float uv[NW_RB] = {u4.x, u4.y, u4.z, u4.w};
+                    const float bv[NW_RB] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+                    for (int k = 0; k < NW_K; ++k)
+#pragma unroll
+                        for (int r = 0; r < NW_RB; ++r)
+                            if (qb + r < R) {
+                                a[k][qb + r] = __builtin_fmaf(f[k], uv[r], a[k][qb + r]);
+                                d[k] = __builtin_fmaf(a[k][qb + r], bv[r], d[k]);
+                            }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < NW_K; ++k)
+                if (ppos[k] == -1) wd[k] = d[k];
+        }
+        duals_ok = true;
+        // column bj: top at row p, zero below; its R column into a free slot
+        const int slot = s_pslot[p];
+#pragma unroll
+        for (int k = 0; k < NW_K; ++k) {
+            if (((bsel >> k) & 1u) && lane == bl) {
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const float v = q < p ? a[k][q] : (q == p ? (float)top : 0.0f);
+                    a[k][q] = v;
+                    s_R[slot * RS + q] = v;
+                }
+                wd[k] = 0.0f;
+            }
+        }
+        oct_wave_sync();
+        back_subst();
+        // secondary loop
+        bool capped = false;
+        for (;;) {
+            ++iter;
+            if (iter > max_iter) {
+                capped = true;
+                break;
+            }
+            // alpha = the first minimum of -x / (z - x) over the non-positive z
+            // in P order, if below 2
+            double t = 2.0;
+            int l = 0;
+            if (lane < nsetp) {
+                l = s_pcol[lane];
+                if (z <= 0) t = -s_xs[l] / (z - s_xs[l]);
+            }
+            const bool cand = t < 2.0;
+            const double alpha = mz_wave_min(cand ? t : 2.0);
+            const uint64_t hit = __ballot(cand && t == alpha);
+            if (!hit) break;
+            if (lane < nsetp) s_xs[l] = s_xs[l] + alpha * (z - s_xs[l]);
+            int rp = __ffsll((unsigned long long)hit) - 1;
+            // move the coefficient at position rp from P to Z (Givens rotations
+            // close the gap), then any remaining one that is not positive
+            for (;;) {
+                oct_wave_sync();
+                const int i = s_pcol[rp];
+                const int last = nsetp - 1;
+                if (lane == 0) s_xs[i] = 0.0;
+                const int sl = lane < nsetp ? s_pslot[lane] : 0;
+                const int pc = lane < nsetp ? s_pcol[lane] : 0;
+                if (rp < last) {
+                    // rotation pp acts on rows pp - 1 and pp and takes the column at
+                    // position pp to (sig, 0) there; lane t > pp rotates its column's
+                    // rows, carrying the lower one to the next step
+                    float carry = (lane > rp && lane <= last) ? s_R[sl * RS + rp] : 0.0f;
+                    for (int pp = rp + 1; pp <= last; ++pp) {
+                        const float r1 = (lane >= pp && lane <= last) ? s_R[sl * RS + pp] : 0.0f;
+                        const double ra = (double)nw_readlane_f(carry, pp), rb = (double)nw_readlane_f(r1, pp);
+                        double cc, ss, sig;
+                        if (fabs(ra) > fabs(rb)) {
+                            const double xr = rb / ra, yr = sqrt(1 + xr * xr);
+                            cc = (1 / yr) * (ra > 0 ? 1.0 : (ra < 0 ? -1.0 : 0.0));
+                            ss = cc * xr;
+                            sig = fabs(ra) * yr;
+                        } else if (rb != 0) {
+                            const double xr = ra / rb, yr = sqrt(1 + xr * xr);
+                            ss = (1 / yr) * (rb > 0 ? 1.0 : -1.0);
+                            cc = ss * xr;
+                            sig = fabs(rb) * yr;
+                        } else {
+                            sig = 0.0; cc = 0.0; ss = 1.0;
+                        }
+                        if (lane > pp && lane <= last) {
+                            const double c0 = carry, c1 = r1;
+                            s_R[sl * RS + pp - 1] = (float)(cc * c0 + ss * c1);
+                            carry = (float)(-ss * c0 + cc * c1);
+                        }
+                        if (lane == pp) s_R[sl * RS + pp - 1] = (float)sig;
+                        if (lane == 0) s_cs[pp - 1] = make_float2((float)cc, (float)ss);
+                        const double t0 = mz_readlane_d(bq, pp - 1), t1 = mz_readlane_d(bq, pp);
+                        if (lane == pp - 1) bq = cc * t0 + ss * t1;
+                        if (lane == pp) bq = -ss * t0 + cc * t1;
+                    }
+                    oct_wave_sync();
+                    // the same rotations on every column held in registers
+#pragma unroll
+                    for (int q = 0; q + 1 < R; ++q) {
+                        if (q >= rp && q < last) {
+                            const float2 cs = s_cs[q];
+#pragma unroll
+                            for (int k = 0; k < NW_K; ++k) {
+                                const float t0 = a[k][q], t1 = a[k][q + 1];
+                                a[k][q] = cs.x * t0 + cs.y * t1;
+                                a[k][q + 1] = -cs.y * t0 + cs.x * t1;
+                            }
+                        }
+                    }
+                }
+                // positions after rp move down by one; rp's slot goes to the end
+                const int pc_next = __shfl(pc, lane < 63 ? lane + 1 : 63, 64);
+                const int sl_next = __shfl(sl, lane < 63 ? lane + 1 : 63, 64);
+                const int sl_free = __builtin_amdgcn_readlane(sl, rp);
+                oct_wave_sync();
+                if (lane >= rp && lane < last) {
+                    s_pcol[lane] = pc_next;
+                    s_pslot[lane] = sl_next;
+                }
+                if (lane == last) s_pslot[lane] = sl_free;
+#pragma unroll
+                for (int k = 0; k < NW_K; ++k) {
+                    if (ppos[k] == rp) ppos[k] = -1;
+                    else if (ppos[k] > rp) ppos[k] -= 1;
+                }
+                nsetp = last;
+                duals_ok = false;
+                oct_wave_sync();
+                const bool bad = lane < nsetp && s_xs[s_pcol[lane]] <= 0.0;
+                const uint64_t badm = __ballot(bad);
+                if (!badm) break;
+                rp = __ffsll((unsigned long long)badm) - 1;
+            }
+            back_subst();
+        }
+        if (capped) break;
+        if (lane < nsetp) s_xs[s_pcol[lane]] = z;
+        oct_wave_sync();
+    }
+
+    // ---- the passive set P solved again in FP64 from the exact columns:
+    // normal equations G y = A_P^T b by Cholesky, one refinement step with the
+    // FP64 residual; compute_post_merge_particles_nnls! (:1334-1404)
+    oct_wave_sync();
+    const int p = nsetp;
+    double *AP = s_area;              // [row][position]
+    double *G = s_area + R * R;       // lower triangle, (i, j <= i) at i (i + 1) / 2 + j
+    int jt = NF_COLS;
+    double cinv = 0.0;
+    NwCand cd{};
+    if (lane < p) {
+        jt = s_pcol[lane];
+        cd = nw_candidate(A, jt, c, N, Np, n_add, s1, n1, s2, s_st);
+        double nn = 0.0;
+        for (int q = 0; q < m; ++q) {
+            const double t = nw_entry(s_ev, s_ep, s_er, m_v, cd.kind, q, cd.cx, cd.cy, cd.cz, cd.px, cd.py, cd.pz) *
+                             s_rs[q];
+            AP[q * R + lane] = t;
+            nn = nn + t * t;
+        }
+        nn = sqrt(nn);
+        nn = nn > 1e-15 ? nn : 1.0;
+        cinv = 1.0 / nn;
+        for (int q = 0; q < m; ++q) AP[q * R + lane] *= cinv;
+    }
+    oct_wave_sync();
+    for (int e = lane; e < p * (p + 1) / 2; e += 64) {
+        int i = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+        while ((i + 1) * (i + 2) / 2 <= e) ++i;
+        while (i * (i + 1) / 2 > e) --i;
+        const int jx = e - i * (i + 1) / 2;
+        double sm = 0.0;
+        for (int q = 0; q < m; ++q) sm = sm + AP[q * R + i] * AP[q * R + jx];
+        G[e] = sm;
+    }
+    oct_wave_sync();
+#define GL(i, j) G[(i) * ((i) + 1) / 2 + (j)]
+    bool spd = true;
+    for (int k = 0; k < p; ++k) {
+        const double dkk = GL(k, k);
+        if (!(dkk > 0.0)) spd = false;
+        const double d = sqrt(dkk);
+        if (lane > k && lane < p) {
+            const double lik = GL(lane, k) / d;
+            GL(lane, k) = lik;
+            for (int jx = k + 1; jx <= lane; ++jx) GL(lane, jx) -= lik * GL(jx, k);
+        }
+        oct_wave_sync();
+        if (lane == k) GL(k, k) = d;
+        oct_wave_sync();
+    }
+    // G y = rhs by the two triangular solves (lane t < p holds rhs_t)
+    auto solve = [&](double rhs) {
+        double zr = lane < p ? rhs : 0.0;
+        for (int k = 0; k < p; ++k) {
+            const double zk = mz_readlane_d(zr, k) / GL(k, k);
+            if (lane == k) zr = zk;
+            if (lane > k && lane < p) zr -= GL(lane, k) * zk;
+        }
+        for (int k = p - 1; k >= 0; --k) {
+            const double yk = mz_readlane_d(zr, k) / GL(k, k);
+            if (lane == k) zr = yk;
+            if (lane < k) zr -= GL(k, lane) * yk;
+        }
+        return zr;
+    };
+    // residual b - A_P y (lane = row) and A_P^T of it (lane = position)
+    auto resid = [&](double yl, double &rq) {
+        if (lane < p) s_gy[lane] = yl;
+        oct_wave_sync();
+        rq = 0.0;
+        if (lane < m) {
+            double sm = b0;
+            for (int t = 0; t < p; ++t) sm = sm - AP[lane * R + t] * s_gy[t];
+            rq = sm;
+            s_gr[lane] = sm;
+        }
+        oct_wave_sync();
+        double g = 0.0;
+        if (lane < p)
+            for (int q = 0; q < m; ++q) g = g + AP[q * R + lane] * s_gr[q];
+        oct_wave_sync();
+        return g;
+    };
+#undef GL
+    double rq;
+    double y = 0.0;
+    if (spd) {
+        y = solve(resid(0.0, rq));
+        const double dy = solve(resid(y, rq));
+        y = y + dy;
+    }
+    (void)resid(y, rq);
+    const double rnorm = sqrt(mz_wave_sum(rq * rq));
+    const bool pos = __ballot(lane < p && !(y > 0.0)) == 0;
+    int ok = spd && pos && !(rnorm > A.max_err);
+    const double xj = lane < p ? y * cinv : 0.0;
+    const bool keep = ok && lane < p && xj > A.w_threshold;
+    const uint64_t km = __ballot(keep);
+    const int nonzero = __popcll(km);
+    const double ndw = mz_wave_sum(keep ? xj : 0.0);
+    if (nonzero >= N) ok = 0;
+    const int32_t n_new = ok ? nonzero : N;
+    if (ok) {
+        // merged particles in column order
+        int rank = 0;
+        for (int u = 0; u < p; ++u) {
+            const int ju = __builtin_amdgcn_readlane(jt, u);
+            if (((km >> u) & 1ull) && ju < jt) ++rank;
+        }
+        if (keep) {
+            const int64_t dst = oct_slot(rank, s1, n1, s2);
+            A.sp.w[dst] = xj * (W / ndw);
+            A.sp.v0[dst] = cd.vx; A.sp.v1[dst] = cd.vy; A.sp.v2[dst] = cd.vz;
+            A.sp.x0[dst] = A.pos_ix ? cd.xx : s_st[4];
+            A.sp.x1[dst] = A.pos_iy ? cd.xy : s_st[5];
+            A.sp.x2[dst] = A.pos_iz ? cd.xz : s_st[6];
+        }
+        for (int li = n_new + lane; li < N; li += 64) A.sp.w[oct_slot(li, s1, n1, s2)] = 0.0;
+    }
+    if (lane == 0) {
+        if (!ok) {
+            A.status[c] = 2;   // the FP64 method decides
+        } else {
+            A.status[c] = 1;
+            int32_t *op = A.old_pia + c * 5;
+            op[0] = s1; op[1] = n1; op[2] = s2; op[3] = n2; op[4] = n_new;
+            const int32_t n_del = N - n_new;
+            const int32_t d2 = min(n_del, n2), d1 = n_del - d2;
+            const int32_t nn2 = n2 - d2, nn1 = n1 - d1;
+            A.sp.pia.n2[c] = nn2;
+            A.sp.pia.end2[c] = nn2 > 0 ? s2 + nn2 - 1 : -1;
+            if (nn2 == 0) A.sp.pia.start2[c] = 0;
+            A.sp.pia.n1[c] = nn1;
+            A.sp.pia.end1[c] = nn1 > 0 ? s1 + nn1 - 1 : -1;
+            if (nn1 == 0) A.sp.pia.start1[c] = 0;
+            A.sp.pia.nloc[c] = nn1 + nn2;
+            if (!(c == A.n_cells - 1) || (n_del > n2)) A.sp.ctr->contiguous = 0;
+        }
+    }
+}
+
 #pragma clang fp contract(on)
 
 // cell_list (0-based, ascending) replaces the threshold's selection when given.
@@ -2375,8 +3073,9 @@ static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, cons
              "merge_nnls: at most 16 spatial multi-indices");
     MZ_CHECK(n_rand_pairs >= 0, "merge_nnls: negative n_rand_pairs");
     MZ_CHECK(rng_mode == 0 || (rng_mode == 1 && h->d_replay[2]), "merge_nnls: replay table not set");
-    MZ_CHECK(method >= 0 && method <= 2,
-             "merge_nnls: method must be 0 (reference layout), 1 (LDS-resident) or 2 (FP32 search, FP64 solve)");
+    MZ_CHECK(method >= 0 && method <= 3,
+             "merge_nnls: method must be 0 (reference layout), 1 (LDS-resident), 2 (FP32 search in LDS, FP64 "
+             "solve) or 3 (FP32 search in registers, one wavefront per cell, FP64 solve)");
     MZ_CHECK(n_vmult >= 0 && n_vmult <= NNLS_MAX_VMULT, "merge_nnls: too many v_multipliers");
     MZ_CHECK(n_mim >= 0, "merge_nnls: bad multi-index count");
     MzSpecies &s = h->sp[species - 1];
@@ -2483,19 +3182,21 @@ static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, cons
             MZ_HIP(hipMemcpy(dcells, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice));
             A.status = d_status;
             A.old_pia = old_pia;
-            auto launch = [&](bool mixed, const int32_t *list, size_t count) {
+            // kind 0: FP64 in LDS, 1: FP32 search in LDS, 2: FP32 search in registers
+            auto launch = [&](int kind, const int32_t *list, size_t count) {
                 A.cells = list;
                 const dim3 g((unsigned)count), b(NF_COLS);
                 if (m <= 35) {
-                    if (mixed) hipLaunchKernelGGL((k_nnls_fast<35, float>), g, b, 0, h->stream, A);
+                    if (kind == 2) hipLaunchKernelGGL(k_nnls_wave, g, dim3(64), 0, h->stream, A);
+                    else if (kind == 1) hipLaunchKernelGGL((k_nnls_fast<35, float>), g, b, 0, h->stream, A);
                     else hipLaunchKernelGGL((k_nnls_fast<35, double>), g, b, 0, h->stream, A);
                 } else {   // more than 35 moments: FP64 only (the FP32 matrix area cannot hold the FP64 system)
                     hipLaunchKernelGGL((k_nnls_fast<NNLS_MAX_ROWS, double>), g, b, 0, h->stream, A);
                 }
                 return hipGetLastError();
             };
-            const bool mixed = method == 2 && m <= 35;
-            MZ_HIP(launch(mixed, dcells, cells.size()));
+            const bool mixed = method >= 2 && m <= 35;
+            MZ_HIP(launch(mixed ? method - 1 : 0, dcells, cells.size()));
             h->nnls_info[0] = (int32_t)cells.size();
             if (mixed) {
                 // cells the FP64 re-solve did not confirm run the FP64 method
@@ -2510,7 +3211,7 @@ static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, cons
                     int32_t *dredo = (int32_t *)mz_scratch(h, "nnls_cells_fp64", redo.size() * sizeof(int32_t));
                     MZ_CHECK(dredo, "merge_nnls: scratch allocation failed");
                     MZ_HIP(hipMemcpy(dredo, redo.data(), redo.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-                    MZ_HIP(launch(false, dredo, redo.size()));
+                    MZ_HIP(launch(0, dredo, redo.size()));
                 }
             }
         } else if (!cells.empty()) {
