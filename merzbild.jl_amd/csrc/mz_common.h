@@ -218,6 +218,7 @@ extern "C" __device__ double __ockl_wfred_add_f64(double);
 extern "C" __device__ double __ockl_wfred_max_f64(double);
 extern "C" __device__ double __ockl_wfred_min_f64(double);
 extern "C" __device__ unsigned long long __ockl_wfred_max_u64(unsigned long long);
+extern "C" __device__ float __ockl_wfred_max_f32(float);
 // Last-block reductions publish their partials with agent-scope relaxed
 // atomic stores, which the hardware keeps coherent across the XCDs' L2
 // caches; the writer then only has to wait for its stores to complete

@@ -19,6 +19,7 @@
 #include "mz_internal.h"
 
 #include <algorithm>
+#include <utility>
 #include <vector>
 
 #define OCT_MAX_N 2048
@@ -39,7 +40,8 @@ struct OctArgs {
 struct OctBin {
     double w;
     double vmin[3], vmax[3];
-    int32_t np, start, end, depth, refine, pad;
+    int16_t np, start, end;     // particles (a cell holds at most OCT_MAX_N = 2048)
+    int8_t depth, refine;
 };
 
 // orders LDS accesses among the lanes of one wave (a barrier for code that a
@@ -143,7 +145,10 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     __shared__ OctBin bins[OCT_MAX_BINS];
     __shared__ double red[4];
     __shared__ double vel_mid[3];
-    __shared__ int32_t cnt8[9], start8[9], run8[9];
+    __shared__ int32_t cnt8[9], start8[9];
+    // the refine pick's keys: bins[b].w when bin b may be refined, else -1
+    // (contiguous, so the pick's scan reads without bank conflicts)
+    __shared__ double bkey[OCT_MAX_BINS];
     __shared__ double ndens8[9];
     __shared__ int32_t Nbins_s, total_post_s, stop_s;
 
@@ -185,6 +190,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
         for (int d = 0; d < 3; ++d) { b.vmin[d] = mn[d]; b.vmax[d] = mx[d]; }
         b.np = N; b.start = 0; b.end = N - 1; b.depth = 0;
         b.refine = (N > 2) && (A.max_depth > 0);
+        bkey[0] = b.refine ? b.w : -1.0;
         Nbins_s = 1;
         total_post_s = N >= 2 ? 2 : N;
         for (int d = 0; d < 3; ++d) vel_mid[d] = 0.0;
@@ -202,7 +208,8 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
             double bw = -1.0;
             int bid = -1;
             for (int b = lane; b < Nbins_s; b += 64) {
-                if (bins[b].refine && bins[b].w > bw) { bw = bins[b].w; bid = b; }
+                const double kb = bkey[b];
+                if (kb > bw) { bw = kb; bid = b; }
             }
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) {
@@ -251,38 +258,51 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                 for (int o = 0; o < 9; ++o) { cnt8[o] = 0; ndens8[o] = 0.0; }
             }
             oct_wave_sync();
-            for (int i = bs + lane; i <= be; i += 64) {
-                int64_t p = oct_slot(pis[i], s1, n1, s2);
-                int o = 1;                        // compute_octant (:304-316): strict >
-                if (A.sp.v0[p] > vel_mid[0]) o += 1;
-                if (A.sp.v1[p] > vel_mid[1]) o += 2;
-                if (A.sp.v2[p] > vel_mid[2]) o += 4;
-                poct[i - bs] = (uint8_t)o;
-                atomicAdd(&cnt8[o], 1);
+            // each particle's octant (compute_octant, :304-316: strict >), and
+            // the octants' counts from one ballot per octant and chunk of 64
+            int cq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            const double mid0 = vel_mid[0], mid1 = vel_mid[1], mid2 = vel_mid[2];
+            for (int i0 = bs; i0 <= be; i0 += 64) {
+                const int i = i0 + lane;
+                int o = 0;
+                if (i <= be) {
+                    const int64_t p = oct_slot(pis[i], s1, n1, s2);
+                    o = 1;
+                    if (A.sp.v0[p] > mid0) o += 1;
+                    if (A.sp.v1[p] > mid1) o += 2;
+                    if (A.sp.v2[p] > mid2) o += 4;
+                    poct[i - bs] = (uint8_t)o;
+                }
+#pragma unroll
+                for (int q = 1; q <= 8; ++q) cq[q] += __popcll(__ballot(o == q));
             }
-            oct_wave_sync();
-            if (lane == 0) {
-                int acc = 0;
-                for (int o = 1; o <= 8; ++o) { start8[o] = bs + acc; run8[o] = 0; acc += cnt8[o]; }
-            }
-            oct_wave_sync();
+            int sq[9];
+            sq[0] = bs;
+#pragma unroll
+            for (int q = 1; q <= 8; ++q) sq[q] = q == 1 ? bs : sq[q - 1] + cq[q - 1];
+#pragma unroll
+            for (int q = 1; q <= 8; ++q)
+                if (lane == q) { cnt8[q] = cq[q]; start8[q] = sq[q]; }
             // Scatter by octant.  The reference walks the bin forward and fills
             // each octant from its end (merging_octree_N2.jl:621-626), so every
-            // child holds its particles in reverse parent order.
+            // child holds its particles in reverse parent order: a particle's
+            // place is its octant's end less the octant's particles before it
+            // (earlier chunks, then earlier lanes of its chunk).
             {
                 const uint64_t lt = mz_lanemask_lt();
-                volatile int32_t *vrun = run8;
+                int rq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
                 for (int i0 = bs; i0 <= be; i0 += 64) {
                     const int i = i0 + lane;
                     const bool valid = i <= be;
                     const int o = valid ? poct[i - bs] : 0;
+                    int dst = 0;
+#pragma unroll
                     for (int q = 1; q <= 8; ++q) {
-                        uint64_t m = __ballot(valid && o == q);
-                        if (m == 0) continue;
-                        const int32_t r0 = vrun[q];
-                        if (valid && o == q) psort[start8[q] + cnt8[q] - 1 - (r0 + __popcll(m & lt))] = pis[i];
-                        vrun[q] = r0 + __popcll(m);
+                        const uint64_t m = __ballot(o == q);
+                        if (o == q) dst = sq[q] + cq[q] - 1 - (rq[q] + __popcll(m & lt));
+                        rq[q] += __popcll(m);
                     }
+                    if (valid) psort[dst] = pis[i];
                 }
             }
             oct_wave_sync();
@@ -336,6 +356,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                         nb.w = ndens8[o];
                         nb.depth = depth + 1;
                         nb.refine = (cnt8[o] > 2) && (depth + 1 < A.max_depth);
+                        bkey[bi] = nb.refine ? nb.w : -1.0;
                     } else {
                         atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
                     }
@@ -1063,6 +1084,7 @@ struct NnlsArgs {
     int32_t *old_pia;              // [n_cells][5]
     int32_t n_cells;
     int32_t max_pcols;             // particle columns: the first max_pcols (< 0: all)
+    int32_t canon;                 // 1: the rows are the 35 multi-indices of total order <= 4 in graded order
 };
 
 // Particle columns of a cell with N particles.
@@ -2414,6 +2436,66 @@ __device__ inline NwCand nw_candidate(const NnlsArgs &A, int j, int c, int32_t N
     return r;
 }
 
+// Multi-index q of the 35 of total order <= 4 in graded order: order 0, then
+// for each order i the pairs (a, b) with a, then b, ascending, c = i - a - b.
+__host__ __device__ constexpr int nw_canon(int q, int d) {
+    if (q == 0) return 0;
+    int k = 1;
+    for (int i = 1; i <= 4; ++i)
+        for (int a = 0; a <= i; ++a)
+            for (int b = 0; b <= i - a; ++b) {
+                if (k == q) return d == 0 ? a : (d == 1 ? b : i - a - b);
+                ++k;
+            }
+    return -1;
+}
+
+template <typename T, int... Q>
+__device__ __forceinline__ void nw_mono4_rows(std::integer_sequence<int, Q...>, const T (&px)[5], const T (&py)[5],
+                                              const T (&pz)[5], T (&out)[NW_ROWS]) {
+    ((out[Q] = (px[std::integral_constant<int, nw_canon(Q, 0)>::value] *
+                py[std::integral_constant<int, nw_canon(Q, 1)>::value]) *
+               pz[std::integral_constant<int, nw_canon(Q, 2)>::value]),
+     ...);
+}
+
+// The 35 monomials x^a y^b z^c in graded order with constant exponents, each
+// formed as nnls_entry forms it: powers by repeated multiplication, then
+// (x^a y^b) z^c.
+__device__ __forceinline__ void nw_mono4(double x, double y, double z, double (&out)[NW_ROWS]) {
+    static_assert(NW_ROWS == 35, "35 multi-indices of total order <= 4");
+    const double x2 = x * x, y2 = y * y, z2 = z * z;
+    const double x3 = x2 * x, y3 = y2 * y, z3 = z2 * z;
+    const double px[5] = {1.0, x, x2, x3, x3 * x}, py[5] = {1.0, y, y2, y3, y3 * y}, pz[5] = {1.0, z, z2, z3, z3 * z};
+    nw_mono4_rows(std::make_integer_sequence<int, NW_ROWS>{}, px, py, pz, out);
+}
+template <int... Q>
+__device__ __forceinline__ void nw_mono4_acc_rows(std::integer_sequence<int, Q...>, const double (&px)[5],
+                                                  const double (&py)[5], const double (&pz)[5], double w,
+                                                  double (&acc)[NW_ROWS]) {
+    ((acc[Q] += w * ((px[std::integral_constant<int, nw_canon(Q, 0)>::value] *
+                      py[std::integral_constant<int, nw_canon(Q, 1)>::value]) *
+                     pz[std::integral_constant<int, nw_canon(Q, 2)>::value])),
+     ...);
+}
+// acc[q] += w x^a y^b z^c over the 35 multi-indices, each monomial used as it
+// is formed
+__device__ __forceinline__ void nw_mono4_acc(double x, double y, double z, double w, double (&acc)[NW_ROWS]) {
+    const double x2 = x * x, y2 = y * y, z2 = z * z;
+    const double x3 = x2 * x, y3 = y2 * y, z3 = z2 * z;
+    const double px[5] = {1.0, x, x2, x3, x3 * x}, py[5] = {1.0, y, y2, y3, y3 * y}, pz[5] = {1.0, z, z2, z3, z3 * z};
+    nw_mono4_acc_rows(std::make_integer_sequence<int, NW_ROWS>{}, px, py, pz, w, acc);
+}
+// the same in FP32 (the search's copy of the matrix)
+__device__ __forceinline__ void nw_mono4(double xd, double yd, double zd, float (&out)[NW_ROWS]) {
+    const float x = (float)xd, y = (float)yd, z = (float)zd;
+    const float x2 = x * x, y2 = y * y, z2 = z * z;
+    const float x3 = x2 * x, y3 = y2 * y, z3 = z2 * z;
+    const float px[5] = {1.0f, x, x2, x3, x3 * x}, py[5] = {1.0f, y, y2, y3, y3 * y},
+                pz[5] = {1.0f, z, z2, z3, z3 * z};
+    nw_mono4_rows(std::make_integer_sequence<int, NW_ROWS>{}, px, py, pz, out);
+}
+
 __device__ inline float nw_readlane_f(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
@@ -2439,6 +2521,37 @@ __device__ inline double nw_entry(const int32_t *ev, const int32_t *ep, const in
     return 0.0;
 }
 
+// The exact FP64 column of a candidate, row-scaled and of unit norm (the
+// column the FP32 search holds rounded); returns 1 / its norm before
+// normalisation.
+__device__ __forceinline__ double nw_exact_column(const NnlsArgs &A, const NwCand &cd, const double *rs,
+                                                  const int32_t *ev, const int32_t *ep, const int32_t *er, int m,
+                                                  int m_v, double (&col)[NW_ROWS]) {
+    double nn = 0.0;
+    if (A.canon) {
+        double mo[NW_ROWS];
+        nw_mono4(cd.cx, cd.cy, cd.cz, mo);
+#pragma unroll
+        for (int q = 0; q < NW_ROWS; ++q) {
+            col[q] = (cd.kind == 1 ? 0.0 : mo[q]) * rs[q];
+            nn = nn + col[q] * col[q];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NW_ROWS; ++q) {
+            col[q] = q < m ? nw_entry(ev, ep, er, m_v, cd.kind, q, cd.cx, cd.cy, cd.cz, cd.px, cd.py, cd.pz) * rs[q]
+                           : 0.0;
+            nn = nn + col[q] * col[q];
+        }
+    }
+    nn = sqrt(nn);
+    nn = nn > 1e-15 ? nn : 1.0;
+    const double ci = 1.0 / nn;
+#pragma unroll
+    for (int q = 0; q < NW_ROWS; ++q) col[q] *= ci;
+    return ci;
+}
+
 // Column j of the search matrix: the candidate's moments, row-scaled by rs,
 // unit norm (scale_columns!, utils.jl:62-77), rounded to FP32; zero for
 // j >= n.  Clears the column's x.
@@ -2450,12 +2563,26 @@ __device__ __forceinline__ void nw_build_column(const NnlsArgs &A, float (&col)[
     double nn = 0.0;
     if (j < n) {
         const NwCand cd = nw_candidate(A, j, c, N, Np, n_add, s1, n1, s2, st);
+        if (A.canon) {
+            // the search's copy in FP32 throughout (the FP64 solve rebuilds its columns)
+            float mo[NW_ROWS];
+            nw_mono4(cd.cx, cd.cy, cd.cz, mo);
+            float nf = 0.0f;
 #pragma unroll
-        for (int q = 0; q < NW_ROWS; ++q) {
-            double t = 0.0;
-            if (q < m) t = nw_entry(ev, ep, er, m_v, cd.kind, q, cd.cx, cd.cy, cd.cz, cd.px, cd.py, cd.pz) * rs[q];
-            col[q] = (float)t;
-            nn = nn + t * t;
+            for (int q = 0; q < NW_ROWS; ++q) {
+                const float t = (cd.kind == 1 ? 0.0f : mo[q]) * (float)rs[q];
+                col[q] = t;
+                nf = __builtin_fmaf(t, t, nf);
+            }
+            nn = nf;
+        } else {
+#pragma unroll
+            for (int q = 0; q < NW_ROWS; ++q) {
+                double t = 0.0;
+                if (q < m) t = nw_entry(ev, ep, er, m_v, cd.kind, q, cd.cx, cd.cy, cd.cz, cd.px, cd.py, cd.pz) * rs[q];
+                col[q] = (float)t;
+                nn = nn + t * t;
+            }
         }
         xs[j] = 0.0;
     } else {
@@ -2469,11 +2596,31 @@ __device__ __forceinline__ void nw_build_column(const NnlsArgs &A, float (&col)[
     for (int q = 0; q < NW_ROWS; ++q) col[q] *= ci;
 }
 
-__global__ void __launch_bounds__(64) k_nnls_wave(NnlsArgs A) {
+// Dual r . a_j of candidate column j in the zero set (pos == -1): the largest
+// goes to wmax, the largest norm before normalisation (zero columns aside) to
+// nmax.
+__device__ __forceinline__ void nw_kkt_dual(const NnlsArgs &A, int pos, int j, int c, int32_t N, int32_t Np,
+                                            int n_add, int32_t s1, int32_t n1, int32_t s2, const double *st,
+                                            const double *rs, const int32_t *ev, const int32_t *ep,
+                                            const int32_t *er, int m, int m_v, const double *r, double &wmax,
+                                            double &nmax) {
+    if (pos != -1) return;
+    const NwCand cd = nw_candidate(A, j, c, N, Np, n_add, s1, n1, s2, st);
+    double col[NW_ROWS];
+    const double ci = nw_exact_column(A, cd, rs, ev, ep, er, m, m_v, col);
+    double w = 0.0;
+#pragma unroll
+    for (int q = 0; q < NW_ROWS; ++q)
+        if (q < m) w = fma(col[q], r[q], w);
+    wmax = fmax(wmax, w);
+    if (cd.kind != 1) nmax = fmax(nmax, 1.0 / ci);
+}
+
+__global__ void __launch_bounds__(64, 2) k_nnls_wave(NnlsArgs A) {
     constexpr int R = NW_ROWS, RS = NW_RS;
-    // the FP64 passive-set system (A_P [row][position], packed lower G) aliases
+    // the FP64 passive-set system (A_P [row][position], two vectors) aliases
     // the search's x by column (FP64) and R by slot (FP32)
-    constexpr int POL = R * R + R * (R + 1) / 2;
+    constexpr int POL = R * R + 2 * RS;
     constexpr int SRCH = NF_COLS + (R * RS + 1) / 2;
     __shared__ __align__(16) double s_area[POL > SRCH ? POL : SRCH];
     __shared__ __align__(16) float s_u[RS], s_bz[RS], s_col[RS];
@@ -2482,7 +2629,7 @@ __global__ void __launch_bounds__(64) k_nnls_wave(NnlsArgs A) {
     __shared__ double s_rs[RS], s_gy[RS], s_gr[RS], s_st[16];
     __shared__ int32_t s_ev[RS], s_ep[RS], s_er[RS];
     double *s_xs = s_area;
-    float *s_R = reinterpret_cast<float *>(s_area + NF_COLS);
+    float *s_R = reinterpret_cast<float *>(s_area + NF_COLS);   // R by slot: [slot][row]
     const int c = A.cells[blockIdx.x];
     const int lane = threadIdx.x;
     const int32_t s1 = A.sp.pia.start1[c], n1 = A.sp.pia.n1[c], s2 = A.sp.pia.start2[c], n2 = A.sp.pia.n2[c];
@@ -2535,6 +2682,7 @@ __global__ void __launch_bounds__(64) k_nnls_wave(NnlsArgs A) {
         double acc[R], e[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int q = 0; q < R; ++q) acc[q] = 0.0;
+#pragma unroll 1
         for (int li = lane; li < N; li += 64) {
             const int64_t p = oct_slot(li, s1, n1, s2);
             const double w = A.sp.w[p];
@@ -2546,9 +2694,13 @@ __global__ void __launch_bounds__(64) k_nnls_wave(NnlsArgs A) {
             e[3] += w * (px * px);
             e[4] += w * (py * py);
             e[5] += w * (pz * pz);
+            if (A.canon) {
+                nw_mono4_acc(cx, cy, cz, w, acc);
+            } else {
 #pragma unroll
-            for (int q = 0; q < R; ++q) {
-                if (q < m) acc[q] += w * nw_entry(s_ev, s_ep, s_er, m_v, 0, q, cx, cy, cz, px, py, pz);
+                for (int q = 0; q < R; ++q) {
+                    if (q < m) acc[q] += w * nw_entry(s_ev, s_ep, s_er, m_v, 0, q, cx, cy, cz, px, py, pz);
+                }
             }
         }
         double Es[6];
@@ -2608,26 +2760,42 @@ __global__ void __launch_bounds__(64) k_nnls_wave(NnlsArgs A) {
     double up = 0.0;
     double z = 0.0;   // lane t < nsetp: solution of R z = (Q^T b)[0, nsetp) at position t
     bool duals_ok = false;
-    // z <- R^-1 (Q^T b) over the nsetp positions, one lane per row; each step's
-    // column of R is loaded one step ahead
+    // R by rows in registers: lane r holds row r of R over the positions of P
+    // (Rrow[t] = R[r][t]), so that the back substitution reads no LDS and, with
+    // its steps unrolled, indexes the row by constants.  LDS keeps R by slot
+    // for the Givens rotations of a removal, after which the rows are read
+    // again.
+    float Rrow[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) Rrow[t] = 0.0f;
     auto back_subst = [&]() {
         const int ns = nsetp;
-        const int sl = lane < ns ? s_pslot[lane] : 0;
-        const double rdk = lane < ns ? 1.0 / (double)s_R[sl * RS + lane] : 0.0;
+        float dg = 1.0f;   // this lane's diagonal entry R[lane][lane]
+        const uint64_t lbit = 1ull << lane;
+#pragma unroll
+        for (int t = 0; t < R; ++t)
+            if ((lbit >> t) & 1ull) dg = Rrow[t];
+        const double rdk = lane < ns ? 1.0 / (double)dg : 0.0;
         double y = lane < ns ? bq : 0.0;
-        auto rcol = [&](int ip) {
-            const int sp = __builtin_amdgcn_readlane(sl, ip >= 1 ? ip : 0);
-            return (ip >= 1 && lane < ip) ? (double)s_R[sp * RS + lane] : 0.0;
-        };
-        double p1 = rcol(ns - 1), p2 = rcol(ns - 2);
-        for (int ip = ns - 1; ip >= 0; --ip) {
-            if (lane == ip) y = y * rdk;
-            const double zip = mz_readlane_d(y, ip);
-            if (lane < ip) y = y - p1 * zip;
-            p1 = p2;
-            p2 = rcol(ip - 2);
+#pragma unroll
+        for (int ip = R - 1; ip >= 0; --ip) {
+            if (ip < ns) {
+                if (lane == ip) y = y * rdk;
+                const double zip = mz_readlane_d(y, ip);
+                if (lane < ip) y = y - (double)Rrow[ip] * zip;
+            }
         }
         z = y;
+    };
+    auto reload_rows = [&]() {
+        const int ns = nsetp;
+        const int sl = lane < ns ? s_pslot[lane] : 0;
+        const int rl = lane < R ? lane : 0;
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            const float v = s_R[__builtin_amdgcn_readlane(sl, t) * RS + rl];
+            Rrow[t] = (t < ns && lane < R) ? v : 0.0f;
+        }
     };
     // duals of the zero set over rows >= lo: w = A_Z^T (Q^T b) with rows < lo zero
     auto duals = [&](int lo) {
@@ -2661,20 +2829,25 @@ __global__ void __launch_bounds__(64) k_nnls_wave(NnlsArgs A) {
         double top = 0.0, av = 0.0;
         bool stop = false;
         for (;;) {
-            uint64_t key = 0;
+            // the lane's best (first maximum over its slots), the wave's maximum,
+            // the lowest lane holding it
+            float bwl = 0.0f;
+            int bkl = -1;
 #pragma unroll
             for (int k = 0; k < NW_K; ++k) {
-                if (ppos[k] == -1 && wd[k] > 0.0f) {
-                    const uint64_t kk = ((uint64_t)__float_as_uint(wd[k]) << 32) | (uint32_t)(0xffff - (k * 64 + lane));
-                    key = kk > key ? kk : key;
+                if (ppos[k] == -1 && wd[k] > bwl) {
+                    bwl = wd[k];
+                    bkl = k;
                 }
             }
-            key = mz_wave_max_u64(key);
-            if (key == 0 || !((double)__uint_as_float((uint32_t)(key >> 32)) > kkt_tol)) {
+            const float bw = __ockl_wfred_max_f32(bwl);
+            if (!((double)bw > kkt_tol)) {
                 stop = true;
                 break;
             }
-            bj = 0xffff - (int)(key & 0xffffu);
+            const uint64_t hitm = __ballot(bkl >= 0 && bwl == bw);
+            const int bl0 = __ffsll((unsigned long long)hitm) - 1;
+            bj = __builtin_amdgcn_readlane(bkl, bl0) * 64 + bl0;
             const int bk = bj >> 6, bl = bj & 63;
             // the column's register slot as a mask: a test k == bk would let the
             // compiler index the register array by bk
@@ -2693,16 +2866,15 @@ __global__ void __launch_bounds__(64) k_nnls_wave(NnlsArgs A) {
             const double asave = mz_readlane_d(av, nsetp);
             double upn = up;
             top = asave;
+            // the column's norm below and above row nsetp (two independent sums)
+            const double sq_lo = mz_wave_sum(hh ? av * av : 0.0);
+            const double unorm = sqrt(mz_wave_sum(lane < nsetp ? av * av : 0.0));
             if (len > 1) {
-                double cl = mz_wave_max(hh ? fabs(av) : 0.0);
-                const double clinv = 1.0 / cl;
-                const double t = hh ? av * clinv : 0.0;
-                cl *= sqrt(mz_wave_sum(t * t));
+                double cl = sqrt(sq_lo);
                 if (asave > 0) cl = -cl;
                 upn = asave - cl;
                 top = cl;
             }
-            const double unorm = sqrt(mz_wave_sum(lane < nsetp ? av * av : 0.0));
             bool accept = false;
             double zr = lane < m ? bq : 0.0;
             if (((unorm + fabs(top) * factor) - unorm) > 0) {
@@ -2807,6 +2979,13 @@ __global__ void __launch_bounds__(64) k_nnls_wave(NnlsArgs A) {
             }
         }
         oct_wave_sync();
+        {
+            const float v = s_R[slot * RS + (lane < R ? lane : 0)];
+            const uint64_t pbit = 1ull << p;
+#pragma unroll
+            for (int t = 0; t < R; ++t)
+                if ((pbit >> t) & 1ull) Rrow[t] = lane < R ? v : 0.0f;
+        }
         back_subst();
         // secondary loop
         bool capped = false;
@@ -2910,6 +3089,7 @@ __global__ void __launch_bounds__(64) k_nnls_wave(NnlsArgs A) {
                 if (!badm) break;
                 rp = __ffsll((unsigned long long)badm) - 1;
             }
+            reload_rows();
             back_subst();
         }
         if (capped) break;
@@ -2918,101 +3098,138 @@ __global__ void __launch_bounds__(64) k_nnls_wave(NnlsArgs A) {
     }
 
     // ---- the passive set P solved again in FP64 from the exact columns:
-    // normal equations G y = A_P^T b by Cholesky, one refinement step with the
-    // FP64 residual; compute_post_merge_particles_nnls! (:1334-1404)
+    // Householder QR with one lane per column of P (lane 63 carries b), the
+    // back substitution with one lane per row; the residual norm is the
+    // solver's, that of the rows of Q^T b below P.
+    // compute_post_merge_particles_nnls! (:1334-1404)
     oct_wave_sync();
     const int p = nsetp;
-    double *AP = s_area;              // [row][position]
-    double *G = s_area + R * R;       // lower triangle, (i, j <= i) at i (i + 1) / 2 + j
+    double *AP = s_area;              // [row][position]: the exact columns of P
+    double *s_v = s_area + R * R;     // a column of R, broadcast to the rows
+    double *s_uh = s_v + RS;          // the Householder vector
     int jt = NF_COLS;
     double cinv = 0.0;
     NwCand cd{};
+    double col[R];                    // lane t < p: column t; lane 63: b
+    if (lane < m) s_gr[lane] = b0;
     if (lane < p) {
         jt = s_pcol[lane];
         cd = nw_candidate(A, jt, c, N, Np, n_add, s1, n1, s2, s_st);
-        double nn = 0.0;
-        for (int q = 0; q < m; ++q) {
-            const double t = nw_entry(s_ev, s_ep, s_er, m_v, cd.kind, q, cd.cx, cd.cy, cd.cz, cd.px, cd.py, cd.pz) *
-                             s_rs[q];
-            AP[q * R + lane] = t;
-            nn = nn + t * t;
-        }
-        nn = sqrt(nn);
-        nn = nn > 1e-15 ? nn : 1.0;
-        cinv = 1.0 / nn;
-        for (int q = 0; q < m; ++q) AP[q * R + lane] *= cinv;
+        cinv = nw_exact_column(A, cd, s_rs, s_ev, s_ep, s_er, m, m_v, col);
+#pragma unroll
+        for (int q = 0; q < R; ++q) AP[q * R + lane] = col[q];
     }
     oct_wave_sync();
-    for (int e = lane; e < p * (p + 1) / 2; e += 64) {
-        int i = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-        while ((i + 1) * (i + 2) / 2 <= e) ++i;
-        while (i * (i + 1) / 2 > e) --i;
-        const int jx = e - i * (i + 1) / 2;
-        double sm = 0.0;
-        for (int q = 0; q < m; ++q) sm = sm + AP[q * R + i] * AP[q * R + jx];
-        G[e] = sm;
+    if (lane == 63) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) col[q] = q < m ? s_gr[q] : 0.0;
+    } else if (lane >= p) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) col[q] = 0.0;
     }
-    oct_wave_sync();
-#define GL(i, j) G[(i) * ((i) + 1) / 2 + (j)]
-    bool spd = true;
+    bool rank_ok = true;
     for (int k = 0; k < p; ++k) {
-        const double dkk = GL(k, k);
-        if (!(dkk > 0.0)) spd = false;
-        const double d = sqrt(dkk);
-        if (lane > k && lane < p) {
-            const double lik = GL(lane, k) / d;
-            GL(lane, k) = lik;
-            for (int jx = k + 1; jx <= lane; ++jx) GL(lane, jx) -= lik * GL(jx, k);
+        if (lane == k) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) s_v[q] = col[q];
         }
         oct_wave_sync();
-        if (lane == k) GL(k, k) = d;
+        const double vk = (lane >= k && lane < m) ? s_v[lane] : 0.0;
+        const double nrm = sqrt(mz_wave_sum(vk * vk));
+        const double akk = s_v[k];
+        const double alpha = akk > 0 ? -nrm : nrm;
+        const double uk = akk - alpha;
+        if (!(nrm > 0.0)) rank_ok = false;
+        if (lane < RS) s_uh[lane] = lane == k ? uk : ((lane > k && lane < m) ? vk : 0.0);
         oct_wave_sync();
+        const double beta = rank_ok ? -1.0 / (alpha * uk) : 0.0;
+        if ((lane > k && lane < p) || lane == 63) {
+            double d = 0.0;
+#pragma unroll
+            for (int q = 0; q < R; ++q) d = fma(s_uh[q], col[q], d);
+            d *= beta;
+#pragma unroll
+            for (int q = 0; q < R; ++q) col[q] = fma(-d, s_uh[q], col[q]);
+        }
+        if (lane == k) {
+            // row k becomes alpha (a mask, not q == k: the row stays a constant index)
+            const uint64_t kbit = 1ull << k;
+#pragma unroll
+            for (int q = 0; q < R; ++q)
+                if ((kbit >> q) & 1ull) col[q] = alpha;
+        }
     }
-    // G y = rhs by the two triangular solves (lane t < p holds rhs_t)
-    auto solve = [&](double rhs) {
-        double zr = lane < p ? rhs : 0.0;
-        for (int k = 0; k < p; ++k) {
-            const double zk = mz_readlane_d(zr, k) / GL(k, k);
-            if (lane == k) zr = zk;
-            if (lane > k && lane < p) zr -= GL(lane, k) * zk;
+    double rr = 0.0;
+    if (lane == 63) {
+        const uint64_t inP = p >= 64 ? ~0ull : ((1ull << p) - 1);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            if (!((inP >> q) & 1ull) && q < m) rr = fma(col[q], col[q], rr);
+            s_gr[q] = col[q];
         }
-        for (int k = p - 1; k >= 0; --k) {
-            const double yk = mz_readlane_d(zr, k) / GL(k, k);
-            if (lane == k) zr = yk;
-            if (lane < k) zr -= GL(k, lane) * yk;
-        }
-        return zr;
-    };
-    // residual b - A_P y (lane = row) and A_P^T of it (lane = position)
-    auto resid = [&](double yl, double &rq) {
-        if (lane < p) s_gy[lane] = yl;
-        oct_wave_sync();
-        rq = 0.0;
-        if (lane < m) {
-            double sm = b0;
-            for (int t = 0; t < p; ++t) sm = sm - AP[lane * R + t] * s_gy[t];
-            rq = sm;
-            s_gr[lane] = sm;
-        }
-        oct_wave_sync();
-        double g = 0.0;
-        if (lane < p)
-            for (int q = 0; q < m; ++q) g = g + AP[q * R + lane] * s_gr[q];
-        oct_wave_sync();
-        return g;
-    };
-#undef GL
-    double rq;
+    }
+    oct_wave_sync();
+    const double rnorm = sqrt(mz_readlane_d(rr, 63));
+    // R y = (Q^T b)[0, p): one lane per row, column ip of R broadcast per step
+    double cr = lane < p ? s_gr[lane] : 0.0;
     double y = 0.0;
-    if (spd) {
-        y = solve(resid(0.0, rq));
-        const double dy = solve(resid(y, rq));
-        y = y + dy;
+    for (int ip = p - 1; ip >= 0; --ip) {
+        if (lane == ip) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) s_v[q] = col[q];
+        }
+        oct_wave_sync();
+        const double yi = mz_readlane_d(cr, ip) / s_v[ip];
+        if (lane == ip) y = yi;
+        if (lane < ip) cr = fma(-s_v[lane], yi, cr);
+        oct_wave_sync();
     }
-    (void)resid(y, rq);
-    const double rnorm = sqrt(mz_wave_sum(rq * rq));
-    const bool pos = __ballot(lane < p && !(y > 0.0)) == 0;
-    int ok = spd && pos && !(rnorm > A.max_err);
+    const bool pos = __ballot(lane < p && !(y > 0.0 && y < 1e300)) == 0;
+    int ok = rank_ok && pos && !(rnorm > A.max_err);
+    // A merge that fails on the residual is decided here when the FP64
+    // solution certifies that no non-negative solution reaches max_err
+    // (otherwise the FP64 method decides).  With r = b - A_P y and the duals
+    // w = A^T r, convexity gives for every x* >= 0
+    //   |A x* - b|^2 >= |r|^2 - 2 (max(0, max_{j outside P} w_j) |x*|_1 + |w_P . y|),
+    // and the mass row bounds |x*|_1 of a minimiser by (b_0 + |r|) max_j |a_j| / rs_0
+    // (its entry in column j is rs_0 / |a_j| before normalisation).
+    bool certified_fail = false;
+    if (!ok && rank_ok && p > 0) {
+        if (lane < p) s_gy[lane] = y;
+        oct_wave_sync();
+        double r = 0.0;
+        if (lane < m) {
+            r = b0;
+            for (int t = 0; t < p; ++t) r = fma(-AP[lane * R + t], s_gy[t], r);
+        }
+        oct_wave_sync();
+        if (lane < m) s_gr[lane] = r;
+        oct_wave_sync();
+        const double r2 = mz_wave_sum(r * r);
+        double wP = 0.0;
+        if (lane < p) {
+            double w = 0.0;
+            for (int q = 0; q < m; ++q) w = fma(AP[q * R + lane], s_gr[q], w);
+            wP = w * y;
+        }
+        double wZ = 0.0, nmax = lane < p ? 1.0 / cinv : 0.0;
+        static_assert(NW_K == 4, "one call per register slot");
+        nw_kkt_dual(A, ppos[0], lane, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_ev, s_ep, s_er, m, m_v, s_gr, wZ,
+                    nmax);
+        nw_kkt_dual(A, ppos[1], 64 + lane, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_ev, s_ep, s_er, m, m_v, s_gr,
+                    wZ, nmax);
+        nw_kkt_dual(A, ppos[2], 128 + lane, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_ev, s_ep, s_er, m, m_v, s_gr,
+                    wZ, nmax);
+        nw_kkt_dual(A, ppos[3], 192 + lane, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_ev, s_ep, s_er, m, m_v, s_gr,
+                    wZ, nmax);
+        const double wZmax = mz_wave_max(wZ);
+        const double amax = mz_wave_max(nmax);
+        const double wPy = fabs(mz_wave_sum(wP));
+        const double b00 = mz_readlane_d(b0, 0);
+        const double x1 = (b00 + sqrt(r2)) * amax / s_rs[0];
+        const double gap = wZmax * x1 + wPy;
+        certified_fail = r2 - 2.0 * gap > A.max_err * A.max_err;
+    }
     const double xj = lane < p ? y * cinv : 0.0;
     const bool keep = ok && lane < p && xj > A.w_threshold;
     const uint64_t km = __ballot(keep);
@@ -3038,7 +3255,11 @@ __global__ void __launch_bounds__(64) k_nnls_wave(NnlsArgs A) {
         for (int li = n_new + lane; li < N; li += 64) A.sp.w[oct_slot(li, s1, n1, s2)] = 0.0;
     }
     if (lane == 0) {
-        if (!ok) {
+        if (!ok && certified_fail) {
+            A.status[c] = -1;
+            int32_t *op = A.old_pia + c * 5;
+            op[0] = s1; op[1] = n1; op[2] = s2; op[3] = n2; op[4] = -1;
+        } else if (!ok) {
             A.status[c] = 2;   // the FP64 method decides
         } else {
             A.status[c] = 1;
@@ -3060,6 +3281,29 @@ __global__ void __launch_bounds__(64) k_nnls_wave(NnlsArgs A) {
 }
 
 #pragma clang fp contract(on)
+
+// The wave method evaluates its rows with constant exponents when they are
+// exactly the 35 multi-indices of total order <= 4 and there are no spatial
+// rows: the rows are then listed in graded order (the order of a system's
+// rows changes neither its solutions nor the merged particles' moments).
+static void nnls_wave_rows(NnlsArgs &A) {
+    A.canon = 0;
+    if (A.m != NW_ROWS || A.m_p != 0) return;
+    bool seen[NW_ROWS] = {};
+    for (int q = 0; q < A.m; ++q) {
+        int hit = -1;
+        for (int k = 0; k < NW_ROWS; ++k)
+            if (A.mim[q][0] == nw_canon(k, 0) && A.mim[q][1] == nw_canon(k, 1) && A.mim[q][2] == nw_canon(k, 2))
+                hit = k;
+        if (hit < 0 || seen[hit]) return;
+        seen[hit] = true;
+    }
+    for (int q = 0; q < NW_ROWS; ++q) {
+        for (int d = 0; d < 3; ++d) A.mim[q][d] = nw_canon(q, d);
+        A.tot_order[q] = nw_canon(q, 0) + nw_canon(q, 1) + nw_canon(q, 2);
+    }
+    A.canon = 1;
+}
 
 // cell_list (0-based, ascending) replaces the threshold's selection when given.
 static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, const int32_t *mim, int32_t n_mim,
@@ -3187,7 +3431,11 @@ static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, cons
                 A.cells = list;
                 const dim3 g((unsigned)count), b(NF_COLS);
                 if (m <= 35) {
-                    if (kind == 2) hipLaunchKernelGGL(k_nnls_wave, g, dim3(64), 0, h->stream, A);
+                    if (kind == 2) {
+                        NnlsArgs Aw = A;
+                        nnls_wave_rows(Aw);
+                        hipLaunchKernelGGL(k_nnls_wave, g, dim3(64), 0, h->stream, Aw);
+                    }
                     else if (kind == 1) hipLaunchKernelGGL((k_nnls_fast<35, float>), g, b, 0, h->stream, A);
                     else hipLaunchKernelGGL((k_nnls_fast<35, double>), g, b, 0, h->stream, A);
                 } else {   // more than 35 moments: FP64 only (the FP32 matrix area cannot hold the FP64 system)

@@ -61,6 +61,31 @@ def test_slab_couette_vs_oracle(hip, mzo, shape):
     st.close()
 
 
+def test_slab_couette_full_size_vs_oracle(hip, mzo):
+    """The benchmarked configuration itself, 500 cells x 200 particles
+    (test/test_1D_couette.jl's 100k-particle flow, equal weights, the
+    collision call the bench runs), 20 steps in one slab launch with the
+    library's shape, against the
+    oracle: index and PIA bit-exact, positions, velocities and weights to
+    1e-11, the sorted properties' T to 1e-11."""
+    case = helpers.couette_case(mzo, n_cells=500, ppc=200, seed=5, dt=2e-10)
+    st = helpers.store_from_case(hip, case)
+    path = hip.couette_steps(st, 20, 1, case["dt"], props_every=1, mode="slab")
+    assert path == "slab", st.slab_info()
+    assert st.slab_info()[3] == 1
+    for step in range(1, 21):
+        helpers.oracle_couette_step(mzo, case, step, equal_weight=True)
+    got = st.download_species(1)
+    assert got["err"] == 0
+    assert got["n_total"] == case["pia"].n_total()
+    helpers.assert_state_matches(mzo, got, case["pv"], case["pia"], rtol=1e-11)
+    pp = hip.PhysProps(case["n_cells"], 1).fetch(st)
+    assert np.allclose(pp.T[0], helpers.oracle_props_sorted(mzo, case), rtol=1e-11)
+    cf = st.collision_factors(1, 1)
+    assert sum(cf["n_coll_performed"]) > 0
+    st.close()
+
+
 # a requested shape runs only with room for its largest slab to grow by 25%
 # plus four Poisson deviations: (2, 4) holds 600 particles in 1,024 places
 @pytest.mark.parametrize("n_cells,ppc,steps,shape", [(500, 200, 3000, (0, 0)), (97, 300, 400, (2, 4)),
