@@ -512,10 +512,35 @@ def sample_nnls_cells():
     return dict(counts=counts, starts=starts, n=n, x=x, v=v, w=w)
 
 
+def _nnls_warm(pkg, cs, max_pcols, method, n_cells=64):
+    """One untimed merge of the first cells with the same method, so that the
+    timed call does not include loading the kernels' code."""
+    counts = cs["counts"][:n_cells]
+    n = int(counts.sum())
+    starts = np.concatenate(([0], np.cumsum(counts)[:-1]))
+    pia = np.zeros((n_cells, 7), dtype=np.int64)
+    pia[:, 0] = counts
+    pia[:, 1] = starts + 1
+    pia[:, 2] = starts + counts
+    pia[:, 3] = counts
+    pia[:, 5] = -1
+    st = pkg.DeviceStore(n_cells, 1, n + 1024)
+    st.upload_species(1, cs["x"][:n], cs["v"][:n], cs["w"][:n], np.arange(1, n + 1, dtype=np.int64), pia, n, True,
+                      np.zeros(0, dtype=np.int64), n)
+    if max_pcols is None:
+        pkg.merge_nnls_based(st, 1, 1, _mim(4), scaling="variance", squash=False)
+    else:
+        pkg.merge_nnls_candidates(st, 1, 1, _mim(4), max_pcols, method=method, scaling="variance", squash=False)
+    st.sync()
+    st.close()
+
+
 def nnls_merge_once(pkg, cs, max_pcols, method, n_check=256):
     """Upload the cells, merge every cell once (timed by the library's event
-    timer around the merge kernels), then the largest relative moment error
+    timer around the merge kernels; an untimed merge of a few cells with the
+    same method runs first), then the largest relative moment error
     |M_after - M_before| / sum w |v^k| over a sample of merged cells."""
+    _nnls_warm(pkg, cs, max_pcols, method)
     n, counts, starts = cs["n"], cs["counts"], cs["starts"]
     pia = np.zeros((NN_CELLS, 7), dtype=np.int64)
     pia[:, 0] = counts
@@ -1019,20 +1044,21 @@ def main():
                                  "note": "wall clock of all replicas' steps (host enqueue included)"}}
 
     # the second half of the metric: NNLS merges/s at <= 1e-10 moment error on
-    # the batched NNLS workload, the mixed method beside the plain FP64 one
+    # the batched NNLS workload, the wave method beside the plain FP64 one
     nnls = None
     if not a.no_nnls:
         cs = sample_nnls_cells()
-        mix = nnls_merge_once(pkg, cs, NN_PCOLS, "mixed")
+        wav = nnls_merge_once(pkg, cs, NN_PCOLS, "wave")
         lds = nnls_merge_once(pkg, cs, NN_PCOLS, "lds")
-        nnls = {"metric": "NNLS merges/s at <= 1e-10 moment error", "value": mix["merges_per_s"],
-                "unit": "merges/s", "method": "mixed (FP32 active-set search, FP64 passive-set solve)",
-                "kernel_ms": mix["kernel_ms"], "merged": mix["merged"], "failed": mix["failed"],
-                "refined_in_fp64": mix["fp64_redo"],
-                "max_rel_moment_error_sampled": mix["max_rel_moment_error_sampled"],
+        nnls = {"metric": "NNLS merges/s at <= 1e-10 moment error", "value": wav["merges_per_s"],
+                "unit": "merges/s",
+                "method": "wave (one wavefront per cell: FP32 active-set search in registers, FP64 passive-set solve)",
+                "kernel_ms": wav["kernel_ms"], "merged": wav["merged"], "failed": wav["failed"],
+                "refined_in_fp64": wav["fp64_redo"],
+                "max_rel_moment_error_sampled": wav["max_rel_moment_error_sampled"],
                 "plain_fp64": {"merges_per_s": lds["merges_per_s"], "kernel_ms": lds["kernel_ms"],
                                "max_rel_moment_error_sampled": lds["max_rel_moment_error_sampled"]},
-                "speedup_vs_plain_fp64": mix["merges_per_s"] / lds["merges_per_s"],
+                "speedup_vs_plain_fp64": wav["merges_per_s"] / lds["merges_per_s"],
                 "workload": "32768 cells x U{200..3000} bi-Maxwellian particles, lognormal(0.5) weights, 35 "
                             "moments (order <= 4), :variance, columns = first 239 particles + 17 fictitious",
                 "n_particles": cs["n"]}

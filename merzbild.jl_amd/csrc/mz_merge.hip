@@ -24,6 +24,7 @@
 
 #define OCT_MAX_N 2048
 #define OCT_MAX_BINS 512
+#define OCT_WB 32          // weight loads in flight per octant in a split
 
 struct OctArgs {
     MzSpView sp;
@@ -262,6 +263,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
             // the octants' counts from one ballot per octant and chunk of 64
             int cq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             const double mid0 = vel_mid[0], mid1 = vel_mid[1], mid2 = vel_mid[2];
+#pragma unroll 2
             for (int i0 = bs; i0 <= be; i0 += 64) {
                 const int i = i0 + lane;
                 int o = 0;
@@ -313,14 +315,14 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
             if (lane >= 1 && lane <= 8) {
                 const int o = lane, lo = start8[o];
                 double sw = 0.0;
-                // eight loads in flight, then the adds in the same order
-                for (int i = start8[o] + cnt8[o] - 1; i >= lo; i -= 8) {
-                    const int m = min(8, i - lo + 1);
-                    double wb[8];
+                // OCT_WB loads in flight, then the adds in the same order
+                for (int i = start8[o] + cnt8[o] - 1; i >= lo; i -= OCT_WB) {
+                    const int m = min(OCT_WB, i - lo + 1);
+                    double wb[OCT_WB];
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) wb[j] = j < m ? A.sp.w[oct_slot(pis[i - j], s1, n1, s2)] : 0.0;
+                    for (int j = 0; j < OCT_WB; ++j) wb[j] = j < m ? A.sp.w[oct_slot(pis[i - j], s1, n1, s2)] : 0.0;
 #pragma unroll
-                    for (int j = 0; j < 8; ++j)
+                    for (int j = 0; j < OCT_WB; ++j)
                         if (j < m) sw += wb[j];
                 }
                 ndens8[o] = sw;
@@ -2486,6 +2488,47 @@ __device__ __forceinline__ void nw_mono4_acc(double x, double y, double z, doubl
     const double px[5] = {1.0, x, x2, x3, x3 * x}, py[5] = {1.0, y, y2, y3, y3 * y}, pz[5] = {1.0, z, z2, z3, z3 * z};
     nw_mono4_acc_rows(std::make_integer_sequence<int, NW_ROWS>{}, px, py, pz, w, acc);
 }
+template <int... Q>
+__device__ __forceinline__ void nw_mono4_scaled_rows(std::integer_sequence<int, Q...>, const double (&px)[5],
+                                                     const double (&py)[5], const double (&pz)[5], const double *rs,
+                                                     double (&out)[NW_ROWS]) {
+    ((out[Q] = ((px[std::integral_constant<int, nw_canon(Q, 0)>::value] *
+                 py[std::integral_constant<int, nw_canon(Q, 1)>::value]) *
+                pz[std::integral_constant<int, nw_canon(Q, 2)>::value]) *
+               rs[Q]),
+     ...);
+}
+// out[q] = x^a y^b z^c rs[q] over the 35 multi-indices
+__device__ __forceinline__ void nw_mono4_scaled(double x, double y, double z, const double *rs,
+                                                double (&out)[NW_ROWS]) {
+    const double x2 = x * x, y2 = y * y, z2 = z * z;
+    const double x3 = x2 * x, y3 = y2 * y, z3 = z2 * z;
+    const double px[5] = {1.0, x, x2, x3, x3 * x}, py[5] = {1.0, y, y2, y3, y3 * y}, pz[5] = {1.0, z, z2, z3, z3 * z};
+    nw_mono4_scaled_rows(std::make_integer_sequence<int, NW_ROWS>{}, px, py, pz, rs, out);
+}
+template <int... Q>
+__device__ __forceinline__ void nw_mono4_dual_rows(std::integer_sequence<int, Q...>, const double (&px)[5],
+                                                   const double (&py)[5], const double (&pz)[5], const double *rs,
+                                                   const double *r, double &nn, double &wr) {
+    (([&] {
+         const double t = ((px[std::integral_constant<int, nw_canon(Q, 0)>::value] *
+                            py[std::integral_constant<int, nw_canon(Q, 1)>::value]) *
+                           pz[std::integral_constant<int, nw_canon(Q, 2)>::value]) *
+                          rs[Q];
+         nn = fma(t, t, nn);
+         wr = fma(t, r[Q], wr);
+     }()),
+     ...);
+}
+// sum of squares and of products with r of the row-scaled monomials, each
+// used as it is formed
+__device__ __forceinline__ void nw_mono4_dual(double x, double y, double z, const double *rs, const double *r,
+                                              double &nn, double &wr) {
+    const double x2 = x * x, y2 = y * y, z2 = z * z;
+    const double x3 = x2 * x, y3 = y2 * y, z3 = z2 * z;
+    const double px[5] = {1.0, x, x2, x3, x3 * x}, py[5] = {1.0, y, y2, y3, y3 * y}, pz[5] = {1.0, z, z2, z3, z3 * z};
+    nw_mono4_dual_rows(std::make_integer_sequence<int, NW_ROWS>{}, px, py, pz, rs, r, nn, wr);
+}
 // the same in FP32 (the search's copy of the matrix)
 __device__ __forceinline__ void nw_mono4(double xd, double yd, double zd, float (&out)[NW_ROWS]) {
     const float x = (float)xd, y = (float)yd, z = (float)zd;
@@ -2529,12 +2572,13 @@ __device__ __forceinline__ double nw_exact_column(const NnlsArgs &A, const NwCan
                                                   int m_v, double (&col)[NW_ROWS]) {
     double nn = 0.0;
     if (A.canon) {
-        double mo[NW_ROWS];
-        nw_mono4(cd.cx, cd.cy, cd.cz, mo);
+        if (cd.kind == 1) {
 #pragma unroll
-        for (int q = 0; q < NW_ROWS; ++q) {
-            col[q] = (cd.kind == 1 ? 0.0 : mo[q]) * rs[q];
-            nn = nn + col[q] * col[q];
+            for (int q = 0; q < NW_ROWS; ++q) col[q] = 0.0;
+        } else {
+            nw_mono4_scaled(cd.cx, cd.cy, cd.cz, rs, col);
+#pragma unroll
+            for (int q = 0; q < NW_ROWS; ++q) nn = nn + col[q] * col[q];
         }
     } else {
 #pragma unroll
@@ -2596,24 +2640,35 @@ __device__ __forceinline__ void nw_build_column(const NnlsArgs &A, float (&col)[
     for (int q = 0; q < NW_ROWS; ++q) col[q] *= ci;
 }
 
-// Dual r . a_j of candidate column j in the zero set (pos == -1): the largest
-// goes to wmax, the largest norm before normalisation (zero columns aside) to
-// nmax.
-__device__ __forceinline__ void nw_kkt_dual(const NnlsArgs &A, int pos, int j, int c, int32_t N, int32_t Np,
+// Dual r . a_j of candidate column j when in_z: the largest goes to wmax
+// (its column to jmax), the largest norm before normalisation (zero columns
+// aside) to nmax.
+__device__ __forceinline__ void nw_kkt_dual(const NnlsArgs &A, bool in_z, int j, int c, int32_t N, int32_t Np,
                                             int n_add, int32_t s1, int32_t n1, int32_t s2, const double *st,
                                             const double *rs, const int32_t *ev, const int32_t *ep,
                                             const int32_t *er, int m, int m_v, const double *r, double &wmax,
-                                            double &nmax) {
-    if (pos != -1) return;
+                                            int &jmax, double &nmax) {
+    if (!in_z) return;
     const NwCand cd = nw_candidate(A, j, c, N, Np, n_add, s1, n1, s2, st);
-    double col[NW_ROWS];
-    const double ci = nw_exact_column(A, cd, rs, ev, ep, er, m, m_v, col);
-    double w = 0.0;
-#pragma unroll
-    for (int q = 0; q < NW_ROWS; ++q)
-        if (q < m) w = fma(col[q], r[q], w);
-    wmax = fmax(wmax, w);
-    if (cd.kind != 1) nmax = fmax(nmax, 1.0 / ci);
+    if (cd.kind == 1) return;   // a zero column: dual 0
+    double nn = 0.0, wr = 0.0;
+    if (A.canon) {
+        nw_mono4_dual(cd.cx, cd.cy, cd.cz, rs, r, nn, wr);
+    } else {
+        for (int q = 0; q < m; ++q) {
+            const double t = nw_entry(ev, ep, er, m_v, cd.kind, q, cd.cx, cd.cy, cd.cz, cd.px, cd.py, cd.pz) * rs[q];
+            nn = fma(t, t, nn);
+            wr = fma(t, r[q], wr);
+        }
+    }
+    nn = sqrt(nn);
+    nn = nn > 1e-15 ? nn : 1.0;
+    const double w = wr / nn;
+    if (w > wmax) {
+        wmax = w;
+        jmax = j;
+    }
+    nmax = fmax(nmax, nn);
 }
 
 __global__ void __launch_bounds__(64, 2) k_nnls_wave(NnlsArgs A) {
@@ -3144,12 +3199,31 @@ __global__ void __launch_bounds__(64, 2) k_nnls_wave(NnlsArgs A) {
         oct_wave_sync();
         const double beta = rank_ok ? -1.0 / (alpha * uk) : 0.0;
         if ((lane > k && lane < p) || lane == 63) {
+            // the Householder vector read eight entries at a time (bounds the registers)
             double d = 0.0;
 #pragma unroll
-            for (int q = 0; q < R; ++q) d = fma(s_uh[q], col[q], d);
+            for (int qb = 0; qb < R; qb += 8) {
+                double u8[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (qb + i < R) u8[i] = s_uh[qb + i];
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (qb + i < R) d = fma(u8[i], col[qb + i], d);
+                __builtin_amdgcn_sched_barrier(0);
+            }
             d *= beta;
 #pragma unroll
-            for (int q = 0; q < R; ++q) col[q] = fma(-d, s_uh[q], col[q]);
+            for (int qb = 0; qb < R; qb += 8) {
+                double u8[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (qb + i < R) u8[i] = s_uh[qb + i];
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (qb + i < R) col[qb + i] = fma(-d, u8[i], col[qb + i]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
         if (lane == k) {
             // row k becomes alpha (a mask, not q == k: the row stays a constant index)
@@ -3185,7 +3259,7 @@ __global__ void __launch_bounds__(64, 2) k_nnls_wave(NnlsArgs A) {
         oct_wave_sync();
     }
     const bool pos = __ballot(lane < p && !(y > 0.0 && y < 1e300)) == 0;
-    int ok = rank_ok && pos && !(rnorm > A.max_err);
+    bool ok = rank_ok && pos && !(rnorm > A.max_err);
     // A merge that fails on the residual is decided here when the FP64
     // solution certifies that no non-negative solution reaches max_err
     // (otherwise the FP64 method decides).  With r = b - A_P y and the duals
@@ -3213,15 +3287,16 @@ __global__ void __launch_bounds__(64, 2) k_nnls_wave(NnlsArgs A) {
             wP = w * y;
         }
         double wZ = 0.0, nmax = lane < p ? 1.0 / cinv : 0.0;
+        int jz = -1;
         static_assert(NW_K == 4, "one call per register slot");
-        nw_kkt_dual(A, ppos[0], lane, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_ev, s_ep, s_er, m, m_v, s_gr, wZ,
-                    nmax);
-        nw_kkt_dual(A, ppos[1], 64 + lane, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_ev, s_ep, s_er, m, m_v, s_gr,
-                    wZ, nmax);
-        nw_kkt_dual(A, ppos[2], 128 + lane, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_ev, s_ep, s_er, m, m_v, s_gr,
-                    wZ, nmax);
-        nw_kkt_dual(A, ppos[3], 192 + lane, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_ev, s_ep, s_er, m, m_v, s_gr,
-                    wZ, nmax);
+        nw_kkt_dual(A, ppos[0] == -1, lane, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_ev, s_ep, s_er, m, m_v, s_gr,
+                    wZ, jz, nmax);
+        nw_kkt_dual(A, ppos[1] == -1, 64 + lane, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_ev, s_ep, s_er, m, m_v,
+                    s_gr, wZ, jz, nmax);
+        nw_kkt_dual(A, ppos[2] == -1, 128 + lane, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_ev, s_ep, s_er, m, m_v,
+                    s_gr, wZ, jz, nmax);
+        nw_kkt_dual(A, ppos[3] == -1, 192 + lane, c, N, Np, n_add, s1, n1, s2, s_st, s_rs, s_ev, s_ep, s_er, m, m_v,
+                    s_gr, wZ, jz, nmax);
         const double wZmax = mz_wave_max(wZ);
         const double amax = mz_wave_max(nmax);
         const double wPy = fabs(mz_wave_sum(wP));
@@ -3235,7 +3310,7 @@ __global__ void __launch_bounds__(64, 2) k_nnls_wave(NnlsArgs A) {
     const uint64_t km = __ballot(keep);
     const int nonzero = __popcll(km);
     const double ndw = mz_wave_sum(keep ? xj : 0.0);
-    if (nonzero >= N) ok = 0;
+    if (nonzero >= N) ok = false;
     const int32_t n_new = ok ? nonzero : N;
     if (ok) {
         // merged particles in column order
@@ -3418,12 +3493,18 @@ static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, cons
         for (int c = 0; c < nc; ++c) init[(size_t)c * 5 + 4] = -1;
         MZ_HIP(hipMemcpy(old_pia, init.data(), init.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     }
+    // the LDS-resident methods' cell lists (allocated and uploaded before the
+    // timed region: staging of the input, not merging)
+    int32_t *dcells = nullptr, *dredo = nullptr;
+    if (!cells.empty() && method >= 1) {
+        dcells = (int32_t *)mz_scratch(h, "nnls_cells", cells.size() * sizeof(int32_t));
+        dredo = (int32_t *)mz_scratch(h, "nnls_cells_fp64", cells.size() * sizeof(int32_t));
+        MZ_CHECK(dcells && dredo, "merge_nnls: scratch allocation failed");
+        MZ_HIP(hipMemcpy(dcells, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
     {
         MzTimed tm(h, "merge_nnls");
         if (!cells.empty() && method >= 1) {
-            int32_t *dcells = (int32_t *)mz_scratch(h, "nnls_cells", cells.size() * sizeof(int32_t));
-            MZ_CHECK(dcells, "merge_nnls: scratch allocation failed");
-            MZ_HIP(hipMemcpy(dcells, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice));
             A.status = d_status;
             A.old_pia = old_pia;
             // kind 0: FP64 in LDS, 1: FP32 search in LDS, 2: FP32 search in registers
@@ -3456,8 +3537,6 @@ static int nnls_merge_impl(mz_store *h, int32_t species, int32_t threshold, cons
                     if (st[cc] == 2) redo.push_back(cc);
                 h->nnls_info[1] = (int32_t)redo.size();
                 if (!redo.empty()) {
-                    int32_t *dredo = (int32_t *)mz_scratch(h, "nnls_cells_fp64", redo.size() * sizeof(int32_t));
-                    MZ_CHECK(dredo, "merge_nnls: scratch allocation failed");
                     MZ_HIP(hipMemcpy(dredo, redo.data(), redo.size() * sizeof(int32_t), hipMemcpyHostToDevice));
                     MZ_HIP(launch(0, dredo, redo.size()));
                 }
