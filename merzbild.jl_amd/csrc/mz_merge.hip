@@ -24,7 +24,6 @@
 
 #define OCT_MAX_N 2048
 #define OCT_MAX_BINS 512
-#define OCT_WB 32          // weight loads in flight per octant in a split
 
 struct OctArgs {
     MzSpView sp;
@@ -259,16 +258,21 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                 for (int o = 0; o < 9; ++o) { cnt8[o] = 0; ndens8[o] = 0.0; }
             }
             oct_wave_sync();
-            // each particle's octant (compute_octant, :304-316: strict >), and
-            // the octants' counts from one ballot per octant and chunk of 64
+            // each particle's octant (compute_octant, :304-316: strict >), the
+            // octants' counts from one ballot per octant and chunk of 64, and the
+            // octants' weights in the reference's accumulation order (the bin's
+            // particles in order, each weight added to its octant's sum: the
+            // chunk's lanes are walked in order, lane o keeping octant o's sum)
             int cq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            double sw = 0.0;
             const double mid0 = vel_mid[0], mid1 = vel_mid[1], mid2 = vel_mid[2];
-#pragma unroll 2
             for (int i0 = bs; i0 <= be; i0 += 64) {
                 const int i = i0 + lane;
                 int o = 0;
+                double wi = 0.0;
                 if (i <= be) {
                     const int64_t p = oct_slot(pis[i], s1, n1, s2);
+                    wi = A.sp.w[p];
                     o = 1;
                     if (A.sp.v0[p] > mid0) o += 1;
                     if (A.sp.v1[p] > mid1) o += 2;
@@ -277,7 +281,14 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                 }
 #pragma unroll
                 for (int q = 1; q <= 8; ++q) cq[q] += __popcll(__ballot(o == q));
+                const int nl = min(64, be - i0 + 1);
+                for (int l = 0; l < nl; ++l) {
+                    const int ol = __builtin_amdgcn_readlane(o, l);
+                    const double wl = mz_readlane_d(wi, l);
+                    if (lane == ol) sw += wl;
+                }
             }
+            if (lane >= 1 && lane <= 8) ndens8[lane] = sw;
             int sq[9];
             sq[0] = bs;
 #pragma unroll
@@ -309,24 +320,6 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
             }
             oct_wave_sync();
             for (int i = bs + lane; i <= be; i += 64) pis[i] = psort[i];
-            oct_wave_sync();
-            // per-octant weights in the reference's accumulation order, i.e. parent
-            // order, which is each child's segment read backwards (one lane per octant)
-            if (lane >= 1 && lane <= 8) {
-                const int o = lane, lo = start8[o];
-                double sw = 0.0;
-                // OCT_WB loads in flight, then the adds in the same order
-                for (int i = start8[o] + cnt8[o] - 1; i >= lo; i -= OCT_WB) {
-                    const int m = min(OCT_WB, i - lo + 1);
-                    double wb[OCT_WB];
-#pragma unroll
-                    for (int j = 0; j < OCT_WB; ++j) wb[j] = j < m ? A.sp.w[oct_slot(pis[i - j], s1, n1, s2)] : 0.0;
-#pragma unroll
-                    for (int j = 0; j < OCT_WB; ++j)
-                        if (j < m) sw += wb[j];
-                }
-                ndens8[o] = sw;
-            }
             oct_wave_sync();
             // the children (split_bin!, :588-633), lane o for octant o: the
             // k-th non-empty octant becomes bin b (k = 1) or Nb + k - 2
