@@ -227,6 +227,17 @@ int mz_get_props_avg(mz_store *h, double *lpa, double *np, double *n, double *v,
 int mz_merge_octree(mz_store *h, int32_t species, int32_t threshold, int32_t target_np, int32_t split,
                     int32_t init_bounds, int32_t bounds_mode, int32_t max_Nbins, int32_t max_depth,
                     int32_t grid_clamp, int32_t squash, uint64_t tag, int32_t rng_mode, int64_t *n_merged);
+/* The caller's loop "if n_local of cell c exceeds thresholds[c], call
+ * merge_octree_N2_based!(rng, octree, pv, pia, c, species, targets[c])"
+ * (merging_octree_N2.jl:1060-1094; a per-cell target such as each cell's
+ * initial count) as one call: thresholds and targets are host arrays of
+ * n_cells entries, the other arguments as mz_merge_octree.  Cells of more
+ * than 2048 particles or more than 512 bins run on a device-memory path
+ * with the same results. */
+int mz_merge_octree_targets(mz_store *h, int32_t species, const int32_t *thresholds, const int32_t *targets,
+                            int32_t split, int32_t init_bounds, int32_t bounds_mode, int32_t max_Nbins,
+                            int32_t max_depth, int32_t grid_clamp, int32_t squash, uint64_t tag, int32_t rng_mode,
+                            int64_t *n_merged);
 /* The caller's loop of per-cell merge_octree_N2_based!(rng, octree, pv, pia,
  * cell, species, target_np[, grid]) calls as one call
  * (merging_octree_N2.jl:1060,1088; the loop of

@@ -75,6 +75,8 @@ _SIGS = {
     "mz_get_props_moments": (C.c_int, [C.c_void_p, P(f64)]),
     "mz_merge_octree": (C.c_int, [C.c_void_p, i32, i32, i32, i32, i32, i32, i32, i32, i32, i32, u64, i32,
                                   P(i64)]),
+    "mz_merge_octree_targets": (C.c_int, [C.c_void_p, i32, P(i32), P(i32), i32, i32, i32, i32, i32, i32, i32, u64,
+                                          i32, P(i64)]),
     "mz_merge_nnls": (C.c_int, [C.c_void_p, i32, i32, P(i32), i32, P(i32), i32, f64, i32, i32, i32, P(f64), i32,
                                 f64, i32, f64, i32, u64, i32, P(i32)]),
     "mz_get_nnls_info": (C.c_int, [C.c_void_p, P(i32)]),
@@ -763,6 +765,25 @@ def merge_octree_N2_based(store, species, threshold, target_np, split=OCTREE_MID
     _check(load_library().mz_merge_octree(store.h, species, threshold, target_np, split, init_bin_bounds,
                                           bin_bounds_compute, max_Nbins, max_depth, 1 if grid_clamp else 0,
                                           1 if squash else 0, tag, rng_mode, C.byref(n)))
+    return n.value
+
+
+def merge_octree_targets(store, species, thresholds, targets, split=OCTREE_MID, init_bin_bounds=OCTREE_INIT_MINMAX,
+                         bin_bounds_compute=OCTREE_BOUNDS_INHERIT, max_Nbins=4096, max_depth=10, grid_clamp=False,
+                         squash=True, tag=0, rng_mode=RNG_PHILOX):
+    """merge_octree_N2_based!(rng, octree, pv, pia, cell, species,
+    targets[cell]) on every cell whose n_local exceeds thresholds[cell]
+    (per-cell arrays of n_cells entries; src/merging/merging_octree_N2.jl:1060-1094);
+    returns the merged-cell count."""
+    th = np.ascontiguousarray(thresholds, dtype=np.int32)
+    tg = np.ascontiguousarray(targets, dtype=np.int32)
+    if th.shape != (store.n_cells,) or tg.shape != (store.n_cells,):
+        raise MzError("merge_octree_targets: thresholds and targets need one entry per cell")
+    n = i64(0)
+    _check(load_library().mz_merge_octree_targets(store.h, species, th.ctypes.data_as(P(i32)),
+                                                  tg.ctypes.data_as(P(i32)), split, init_bin_bounds,
+                                                  bin_bounds_compute, max_Nbins, max_depth, 1 if grid_clamp else 0,
+                                                  1 if squash else 0, tag, rng_mode, C.byref(n)))
     return n.value
 
 

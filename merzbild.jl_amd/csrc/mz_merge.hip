@@ -16,6 +16,13 @@
 //   * the refine pick is the first maximum over bins 1..Nbins.
 // Bin means and variances are block reductions (they only feed the new
 // particle values, compared to 1e-12).
+//
+// A cell of more than OCT_MAX_N particles, or one whose split loop would
+// need more than OCT_MAX_BINS bins, is handed on: the LDS kernel lists it and
+// a second kernel merges the listed cells with the same code on arrays in
+// device memory (one slice per workgroup, one wavefront per workgroup, the
+// workgroups taking the listed cells in turn).  The split loop keeps its
+// state in LDS either way; the cell's arrays are then read through L2.
 #include "mz_internal.h"
 
 #include <algorithm>
@@ -35,13 +42,52 @@ struct OctArgs {
     int32_t *n_merged;
     int32_t n_cells;
     const uint8_t *mask;      // cells to merge (nullptr: every cell above the threshold)
+    const int32_t *thresholds, *targets;   // per cell (nullptr: threshold, target_np for every cell)
+    // cells handed to the device-memory path (big_list nullptr: such a cell
+    // raises MZ_ERR_MERGE_SIZE); big_info[0] counts them, [1] is their largest N
+    int32_t *big_list, *big_info;
+    // that path's arrays, one slice of big_ncap particles and big_bcap bins per workgroup
+    int32_t *big_pis, *big_psort, *big_bslot;
+    uint8_t *big_poct;
+    double *big_bkey;
+    void *big_bins;
+    int32_t big_ncap, big_bcap;
 };
 
-struct OctBin {
+// A bin of the octree; IdxT holds particle counts and positions (int16_t in
+// LDS, where a cell holds at most OCT_MAX_N = 2048 particles, int32_t in
+// device memory).
+template <class IdxT>
+struct OctBinT {
     double w;
     double vmin[3], vmax[3];
-    int16_t np, start, end;     // particles (a cell holds at most OCT_MAX_N = 2048)
+    IdxT np, start, end;
     int8_t depth, refine;
+};
+using OctBin = OctBinT<int16_t>;
+
+// The working arrays of one cell's merge: the particle order, its scatter
+// target (after the splits: per-bin output slots and draw offsets), each
+// particle's octant, the bins and the refine pick's keys.
+template <class IdxT>
+struct OctWork {
+    IdxT *pis, *psort;
+    uint8_t *poct;
+    OctBinT<IdxT> *bins;
+    double *bkey;
+    int32_t *bslot, *bdraw;
+    int32_t bins_cap;
+};
+
+// The small per-cell state of the split loop and the output scan, in LDS.
+struct OctShared {
+    double red[4];
+    double vel_mid[3];
+    int32_t cnt8[9], start8[9];
+    double ndens8[9];
+    int32_t Nbins_s, total_post_s, stop_s, defer_s;
+    int64_t wsc[4];
+    int32_t n_new_s;
 };
 
 // orders LDS accesses among the lanes of one wave (a barrier for code that a
@@ -84,7 +130,8 @@ __device__ inline double blk_sum(double v, double *red) {
 }
 
 // min/max of v over pis[bs..be] (bin_bounds_recompute!, merging_octree_N2.jl:382-418)
-__device__ void oct_bounds(const OctArgs &A, const int16_t *pis, int bs, int be, int32_t s1, int32_t n1, int32_t s2,
+template <class IdxT, int NW>
+__device__ void oct_bounds(const OctArgs &A, const IdxT *pis, int bs, int be, int32_t s1, int32_t n1, int32_t s2,
                            double *mn, double *mx, double *red) {
     for (int d = 0; d < 3; ++d) {
         const double *vv = d == 0 ? A.sp.v0 : (d == 1 ? A.sp.v1 : A.sp.v2);
@@ -94,8 +141,13 @@ __device__ void oct_bounds(const OctArgs &A, const int16_t *pis, int bs, int be,
             a = fmin(a, x);
             b = fmax(b, x);
         }
-        mn[d] = blk_min(a, red);
-        mx[d] = blk_max(b, red);
+        if (NW == 1) {
+            mn[d] = mz_wave_min(a);
+            mx[d] = mz_wave_max(b);
+        } else {
+            mn[d] = blk_min(a, red);
+            mx[d] = blk_max(b, red);
+        }
     }
 }
 
@@ -138,44 +190,38 @@ __device__ void merge_finish(const MzSpView &sp, const MzSpView &tmp, int c, int
     }
 }
 
-__global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
-    __shared__ int16_t pis[OCT_MAX_N];
-    __shared__ __align__(16) int16_t psort[OCT_MAX_N];   // after the splits: per-bin slots and draws
-    __shared__ uint8_t poct[OCT_MAX_N];
-    __shared__ OctBin bins[OCT_MAX_BINS];
-    __shared__ double red[4];
-    __shared__ double vel_mid[3];
-    __shared__ int32_t cnt8[9], start8[9];
+// Merges cell c (N particles, target_np) with the working arrays W; NW
+// waves per workgroup.  With BIG false a cell that would need more bins than
+// W holds is listed for the device-memory path (or raises
+// MZ_ERR_MERGE_SIZE when there is none) and left unchanged.
+template <class IdxT, int NW, bool BIG>
+__device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, const int32_t N,
+                                               const int32_t target_np, const OctWork<IdxT> &W, OctShared &S) {
+    IdxT *const pis = W.pis;
+    IdxT *const psort = W.psort;
+    uint8_t *const poct = W.poct;
+    OctBinT<IdxT> *const bins = W.bins;
     // the refine pick's keys: bins[b].w when bin b may be refined, else -1
     // (contiguous, so the pick's scan reads without bank conflicts)
-    __shared__ double bkey[OCT_MAX_BINS];
-    __shared__ double ndens8[9];
-    __shared__ int32_t Nbins_s, total_post_s, stop_s;
-
-    const int c = blockIdx.x;
-    A.rp = mz_rng_resolve(A.rp);
-    const int32_t N = A.sp.pia.nloc[c];
-    if (N <= A.threshold || (A.mask && !A.mask[c])) {
-        if (threadIdx.x == 0) A.old_pia[c * 5 + 4] = -1;
-        return;
-    }
-    if (N > OCT_MAX_N) {
-        if (threadIdx.x == 0) {
-            A.old_pia[c * 5 + 4] = -1;
-            atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
-        }
-        return;
-    }
+    double *const bkey = W.bkey;
+    double *const red = S.red;
+    double *const vel_mid = S.vel_mid;
+    int32_t *const cnt8 = S.cnt8;
+    int32_t *const start8 = S.start8;
+    double *const ndens8 = S.ndens8;
+    int32_t &Nbins_s = S.Nbins_s;
+    int32_t &total_post_s = S.total_post_s;
+    int32_t &stop_s = S.stop_s;
     const int32_t s1 = A.sp.pia.start1[c], n1 = A.sp.pia.n1[c], s2 = A.sp.pia.start2[c], n2 = A.sp.pia.n2[c];
-    const int max_bins = min(A.max_Nbins, OCT_MAX_BINS);
+    const int max_bins = min(A.max_Nbins, W.bins_cap);
     // init_octree! (merging_octree_N2.jl:947-984)
-    for (int i = threadIdx.x; i < N; i += blockDim.x) pis[i] = (int16_t)i;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) pis[i] = (IdxT)i;
     double mn[3], mx[3];
     if (A.init_bounds == 3) {
         for (int d = 0; d < 3; ++d) { mn[d] = -299792458.0; mx[d] = 299792458.0; }
     } else {
         __syncthreads();
-        oct_bounds(A, pis, 0, N - 1, s1, n1, s2, mn, mx, red);
+        oct_bounds<IdxT, NW>(A, pis, 0, N - 1, s1, n1, s2, mn, mx, red);
         if (A.init_bounds == 2) {
             for (int d = 0; d < 3; ++d) {
                 double m = fmax(fabs(mn[d]), fabs(mx[d]));
@@ -185,7 +231,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
         }
     }
     if (threadIdx.x == 0) {
-        OctBin &b = bins[0];
+        OctBinT<IdxT> &b = bins[0];
         b.w = 1e50;
         for (int d = 0; d < 3; ++d) { b.vmin[d] = mn[d]; b.vmax[d] = mx[d]; }
         b.np = N; b.start = 0; b.end = N - 1; b.depth = 0;
@@ -195,6 +241,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
         total_post_s = N >= 2 ? 2 : N;
         for (int d = 0; d < 3; ++d) vel_mid[d] = 0.0;
         stop_s = 0;
+        S.defer_s = 0;
     }
     __syncthreads();
 
@@ -217,7 +264,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                 int oi = __shfl_xor(bid, o, 64);
                 if (oi >= 0 && (bid < 0 || ow > bw || (ow == bw && oi < bid))) { bw = ow; bid = oi; }
             }
-            if (bid < 0 || total_post_s + 14 > A.target_np) break;
+            if (bid < 0 || total_post_s + 14 > target_np) break;
             // split_bin! (merging_octree_N2.jl:503-633)
             const int b = bid;
             const int bs = bins[b].start, be = bins[b].end;
@@ -337,8 +384,8 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                 if (ne) {
                     const int k = __popcll(m & mz_lanemask_lt()) + 1;
                     const int bi = k == 1 ? b : Nb + k - 2;
-                    if (bi < OCT_MAX_BINS) {
-                        OctBin &nb = bins[bi];
+                    if (bi < W.bins_cap) {
+                        OctBinT<IdxT> &nb = bins[bi];
                         nb.start = start8[o];
                         nb.end = start8[o] + cnt8[o] - 1;
                         if (A.bounds_mode == 1) {   // bin_bounds_inherit! (:341-367)
@@ -352,8 +399,6 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                         nb.depth = depth + 1;
                         nb.refine = (cnt8[o] > 2) && (depth + 1 < A.max_depth);
                         bkey[bi] = nb.refine ? nb.w : -1.0;
-                    } else {
-                        atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
                     }
                     contrib = cnt8[o] >= 2 ? 2 : cnt8[o];
                 }
@@ -361,11 +406,13 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
                 if (lane == 0) {
                     total_post_s += added - 2;
                     Nbins_s = Nb + n_ne - 1;
-                    if (Nb + n_ne - 2 >= OCT_MAX_BINS) stop_s = 1;
-                    else if (Nbins_s + 7 > A.max_Nbins) stop_s = 1;
-                    else if (Nbins_s + 7 > max_bins) {   // LDS bin budget below the caller's max_Nbins
-                        atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
+                    if (Nbins_s + 7 > A.max_Nbins) {
                         stop_s = 1;
+                    } else if (Nbins_s + 7 > max_bins) {
+                        // the arrays' bin budget is below the caller's max_Nbins:
+                        // the cell goes to the device-memory path (or fails)
+                        stop_s = 1;
+                        S.defer_s = 1;
                     }
                 }
             }
@@ -374,6 +421,18 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
         }
     }
     __syncthreads();
+    if (S.defer_s) {
+        if (threadIdx.x == 0) {
+            A.old_pia[c * 5 + 4] = -1;
+            if (!BIG && A.big_list) {
+                A.big_list[atomicAdd(&A.big_info[0], 1)] = c;
+                atomicMax(&A.big_info[1], N);
+            } else {
+                atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
+            }
+        }
+        return;
+    }
     const int Nbins = Nbins_s;
     if (Nbins == 1 && threadIdx.x == 0) {
         double s = 0.0;
@@ -390,44 +449,54 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     // block reduction over its particles (lane l holds the block threads l,
     // l + 64, l + 128, l + 192), so the values do not depend on which wave
     // runs the bin.
-    int32_t *bslot = reinterpret_cast<int32_t *>(psort);
-    int32_t *bdraw = bslot + OCT_MAX_BINS;
-    __shared__ int32_t wsc[4];
-    __shared__ int32_t n_new_s;
+    int32_t *const bslot = W.bslot;
+    int32_t *const bdraw = W.bdraw;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // each bin's output slots (2, or its particles when it has at most 2)
+    // and draw offset (6 sign draws per bin of more than 2 particles): an
+    // exclusive scan over the bins, two bins per thread and pass, slots in
+    // the low and draws in the high 32 bits
     {
-        int32_t oc[2], dr[2];
-        for (int j = 0; j < 2; ++j) {
-            const int b = 2 * threadIdx.x + j;
-            const int32_t bnp = b < Nbins ? (bins[b].w == 0.0 ? 0 : bins[b].np) : 0;
-            oc[j] = bnp <= 0 ? 0 : (bnp <= 2 ? bnp : 2);
-            dr[j] = bnp > 2 ? 1 : 0;
-        }
-        const int32_t v = (oc[0] + oc[1]) | ((dr[0] + dr[1]) << 16);
-        int32_t x = v;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int32_t y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) wsc[wv] = x;
-        __syncthreads();
-        int32_t pre = 0;
-        for (int k = 0; k < wv; ++k) pre += wsc[k];
-        const int32_t ex = pre + x - v;
-        const int b = 2 * threadIdx.x;
-        if (b < OCT_MAX_BINS) {
-            bslot[b] = ex & 0xFFFF;
-            bdraw[b] = ex >> 16;
-            if (b + 1 < OCT_MAX_BINS) {
-                bslot[b + 1] = (ex & 0xFFFF) + oc[0];
-                bdraw[b + 1] = (ex >> 16) + dr[0];
+        int64_t carry = 0;
+        for (int b0 = 0; b0 < Nbins; b0 += 2 * NW * 64) {
+            int64_t oc[2], dr[2];
+            for (int j = 0; j < 2; ++j) {
+                const int b = b0 + 2 * threadIdx.x + j;
+                const int32_t bnp = b < Nbins ? (bins[b].w == 0.0 ? 0 : (int32_t)bins[b].np) : 0;
+                oc[j] = bnp <= 0 ? 0 : (bnp <= 2 ? bnp : 2);
+                dr[j] = bnp > 2 ? 1 : 0;
             }
+            const int64_t v = (oc[0] + oc[1]) | ((dr[0] + dr[1]) << 32);
+            int64_t x = v;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int64_t y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            if (lane == 63) S.wsc[wv] = x;
+            __syncthreads();
+            int64_t pre = carry, tot = carry;
+            for (int k = 0; k < NW; ++k) {
+                if (k < wv) pre += S.wsc[k];
+                tot += S.wsc[k];
+            }
+            const int64_t ex = pre + x - v;
+            const int b = b0 + 2 * threadIdx.x;
+            if (b < Nbins) {
+                bslot[b] = (int32_t)(ex & 0xFFFFFFFF);
+                bdraw[b] = (int32_t)(ex >> 32);
+                if (b + 1 < Nbins) {
+                    bslot[b + 1] = (int32_t)(ex & 0xFFFFFFFF) + (int32_t)oc[0];
+                    bdraw[b + 1] = (int32_t)(ex >> 32) + (int32_t)dr[0];
+                }
+            }
+            carry = tot;
+            __syncthreads();
         }
-        if (threadIdx.x == 255) n_new_s = (pre + x) & 0xFFFF;
+        if (threadIdx.x == 0) S.n_new_s = (int32_t)(carry & 0xFFFFFFFF);
         __syncthreads();
     }
-    for (int b = wv; b < Nbins; b += 4) {
+    for (int b = wv; b < Nbins; b += NW) {
         const int32_t bnp = bins[b].w == 0.0 ? 0 : bins[b].np;
         if (bnp <= 0) continue;
         const int bs = bins[b].start, be = bins[b].end;
@@ -542,8 +611,79 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
         }
     }
     __syncthreads();
-    merge_finish(A.sp, A.tmp, c, N, n_new_s, s1, n1, s2, n2, A.old_pia, A.n_merged, A.n_cells);
+    merge_finish(A.sp, A.tmp, c, N, S.n_new_s, s1, n1, s2, n2, A.old_pia, A.n_merged, A.n_cells);
 }
+
+// The cell's threshold and target (per cell when the caller gives them).
+__device__ inline bool oct_pick(const OctArgs &A, int c, int32_t &N, int32_t &target) {
+    N = A.sp.pia.nloc[c];
+    const int32_t thr = A.thresholds ? A.thresholds[c] : A.threshold;
+    target = A.targets ? A.targets[c] : A.target_np;
+    return !(N <= thr || (A.mask && !A.mask[c]));
+}
+
+// One workgroup per cell, the working arrays in LDS.
+__global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
+    __shared__ int16_t pis[OCT_MAX_N];
+    __shared__ __align__(16) int16_t psort[OCT_MAX_N];   // after the splits: per-bin slots and draws
+    __shared__ uint8_t poct[OCT_MAX_N];
+    __shared__ OctBin bins[OCT_MAX_BINS];
+    __shared__ double bkey[OCT_MAX_BINS];
+    __shared__ OctShared S;
+    const int c = blockIdx.x;
+    A.rp = mz_rng_resolve(A.rp);
+    int32_t N, target;
+    if (!oct_pick(A, c, N, target)) {
+        if (threadIdx.x == 0) A.old_pia[c * 5 + 4] = -1;
+        return;
+    }
+    if (N > OCT_MAX_N) {
+        if (threadIdx.x == 0) {
+            A.old_pia[c * 5 + 4] = -1;
+            if (A.big_list) {
+                A.big_list[atomicAdd(&A.big_info[0], 1)] = c;
+                atomicMax(&A.big_info[1], N);
+            } else {
+                atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
+            }
+        }
+        return;
+    }
+    OctWork<int16_t> W;
+    W.pis = pis; W.psort = psort; W.poct = poct; W.bins = bins; W.bkey = bkey;
+    W.bslot = reinterpret_cast<int32_t *>(psort);
+    W.bdraw = W.bslot + OCT_MAX_BINS;
+    W.bins_cap = OCT_MAX_BINS;
+    oct_merge_cell<int16_t, 4, false>(A, c, N, target, W, S);
+}
+
+// The listed cells, one wavefront per workgroup, the working arrays in the
+// workgroup's slice of device memory; workgroup g takes the cells g, g + G,
+// ... of the list.
+__global__ void __launch_bounds__(64) k_octree_merge_big(OctArgs A) {
+    __shared__ OctShared S;
+    A.rp = mz_rng_resolve(A.rp);
+    const int32_t n_list = A.big_info[0];
+    const size_t g = blockIdx.x;
+    OctWork<int32_t> W;
+    W.pis = A.big_pis + g * (size_t)A.big_ncap;
+    W.psort = A.big_psort + g * (size_t)A.big_ncap;
+    W.poct = A.big_poct + g * (size_t)A.big_ncap;
+    W.bins = reinterpret_cast<OctBinT<int32_t> *>(A.big_bins) + g * (size_t)A.big_bcap;
+    W.bkey = A.big_bkey + g * (size_t)A.big_bcap;
+    W.bslot = A.big_bslot + g * (size_t)A.big_bcap * 2;
+    W.bdraw = W.bslot + A.big_bcap;
+    W.bins_cap = A.big_bcap;
+    for (int32_t j = blockIdx.x; j < n_list; j += gridDim.x) {
+        const int c = A.big_list[j];
+        int32_t N, target;
+        oct_pick(A, c, N, target);
+        if (N <= A.big_ncap) oct_merge_cell<int32_t, 1, true>(A, c, N, target, W, S);
+        else if (threadIdx.x == 0) atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
+        __syncthreads();
+    }
+}
+
 
 // Free-id pushes in cell order: cell c pushes the ids of its deleted
 // positions, last first (delete_particle_end! pushes pv.index[end]).
@@ -617,7 +757,8 @@ __global__ void __launch_bounds__(1024) k_scan_i32m(const int32_t *__restrict__ 
 static int octree_merge_launch(mz_store *h, int32_t species, int32_t threshold, const uint8_t *mask,
                                int32_t target_np, int32_t split, int32_t init_bounds, int32_t bounds_mode,
                                int32_t max_Nbins, int32_t max_depth, int32_t grid_clamp, uint64_t tag,
-                               int32_t rng_mode, int64_t *n_merged);
+                               int32_t rng_mode, int64_t *n_merged, const int32_t *d_thresholds = nullptr,
+                               const int32_t *d_targets = nullptr);
 
 extern "C" int mz_merge_octree(mz_store *h, int32_t species, int32_t threshold, int32_t target_np, int32_t split,
                                int32_t init_bounds, int32_t bounds_mode, int32_t max_Nbins, int32_t max_depth,
@@ -625,6 +766,27 @@ extern "C" int mz_merge_octree(mz_store *h, int32_t species, int32_t threshold, 
                                int64_t *n_merged) {
     int r = octree_merge_launch(h, species, threshold, nullptr, target_np, split, init_bounds, bounds_mode, max_Nbins,
                                 max_depth, grid_clamp, tag, rng_mode, n_merged);
+    if (r) return r;
+    if (squash) return mz_squash(h, species);
+    return 0;
+}
+
+// The caller's loop "merge cell c when its n_local exceeds thresholds[c],
+// down to targets[c]" (per-cell merge_octree_N2_based!(rng, octree, pv, pia,
+// cell, species, target_np) calls, merging_octree_N2.jl:1060-1094) as one
+// call; thresholds and targets are host arrays of n_cells entries.
+extern "C" int mz_merge_octree_targets(mz_store *h, int32_t species, const int32_t *thresholds,
+                                       const int32_t *targets, int32_t split, int32_t init_bounds,
+                                       int32_t bounds_mode, int32_t max_Nbins, int32_t max_depth, int32_t grid_clamp,
+                                       int32_t squash, uint64_t tag, int32_t rng_mode, int64_t *n_merged) {
+    MZ_CHECK(thresholds != nullptr && targets != nullptr, "merge_octree_targets: thresholds and targets needed");
+    const int nc = h->n_cells;
+    int32_t *d = (int32_t *)mz_scratch(h, "oct_targets", (size_t)nc * 2 * sizeof(int32_t));
+    MZ_CHECK(d != nullptr, "merge_octree_targets: scratch allocation failed");
+    MZ_HIP(hipMemcpyAsync(d, thresholds, (size_t)nc * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    MZ_HIP(hipMemcpyAsync(d + nc, targets, (size_t)nc * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    int r = octree_merge_launch(h, species, 0, nullptr, 0, split, init_bounds, bounds_mode, max_Nbins, max_depth,
+                                grid_clamp, tag, rng_mode, n_merged, d, d + nc);
     if (r) return r;
     if (squash) return mz_squash(h, species);
     return 0;
@@ -667,7 +829,8 @@ extern "C" int mz_merge_octree_cells(mz_store *h, int32_t species, const int32_t
 static int octree_merge_launch(mz_store *h, int32_t species, int32_t threshold, const uint8_t *mask,
                                int32_t target_np, int32_t split, int32_t init_bounds, int32_t bounds_mode,
                                int32_t max_Nbins, int32_t max_depth, int32_t grid_clamp, uint64_t tag,
-                               int32_t rng_mode, int64_t *n_merged) {
+                               int32_t rng_mode, int64_t *n_merged, const int32_t *d_thresholds,
+                               const int32_t *d_targets) {
     if (species >= 1 && species <= h->n_species) h->sp[species - 1].contig_known = false;
     MZ_CHECK(species >= 1 && species <= h->n_species, "merge_octree: bad species");
     MZ_CHECK(split == 1 || split == 2, "merge_octree: split must be 1 (mid) or 2 (mean)");
@@ -679,14 +842,16 @@ static int octree_merge_launch(mz_store *h, int32_t species, int32_t threshold, 
     MzSpecies &s = h->sp[species - 1];
     const int nc = h->n_cells;
     int32_t *old_pia = (int32_t *)mz_scratch(h, "oct_oldpia", (size_t)nc * 5 * sizeof(int32_t));
-    int32_t *cnt = (int32_t *)mz_scratch(h, "oct_cnt", sizeof(int32_t));
+    int32_t *cnt = (int32_t *)mz_scratch(h, "oct_cnt", 3 * sizeof(int32_t));
     int32_t *ndel = (int32_t *)mz_scratch(h, "oct_ndel", (size_t)nc * sizeof(int32_t));
     int64_t *off = (int64_t *)mz_scratch(h, "oct_off", (size_t)(nc + 1) * sizeof(int64_t));
-    MZ_CHECK(old_pia && cnt && ndel && off, "merge_octree: scratch allocation failed");
+    int32_t *big_list = (int32_t *)mz_scratch(h, "oct_big_list", (size_t)nc * sizeof(int32_t));
+    MZ_CHECK(old_pia && cnt && ndel && off && big_list, "merge_octree: scratch allocation failed");
     MzCounters snap;
     int r = mz_read_counters(h, species, &snap);
     if (r) return r;
-    MZ_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t), h->stream));
+    // cnt[0] counts merges; cnt[1], cnt[2] are the big-cell list's count and largest N
+    MZ_HIP(hipMemsetAsync(cnt, 0, 3 * sizeof(int32_t), h->stream));
     OctArgs A{};
     A.sp = mz_view(s);
     A.tmp = A.sp;
@@ -701,9 +866,40 @@ static int octree_merge_launch(mz_store *h, int32_t species, int32_t threshold, 
     A.n_merged = cnt;
     A.n_cells = nc;
     A.mask = mask;
+    A.thresholds = d_thresholds;
+    A.targets = d_targets;
+    A.big_list = big_list;
+    A.big_info = cnt + 1;
     {
         MzTimed tm(h, "merge_octree");
         hipLaunchKernelGGL(k_octree_merge, dim3(nc), dim3(256), 0, h->stream, A);
+        // cells above the LDS kernel's sizes: one more launch with their
+        // working arrays in device memory, sized by what the list needs
+        int32_t info[2] = {0, 0};
+        MZ_HIP(hipMemcpyAsync(info, cnt + 1, sizeof(info), hipMemcpyDeviceToHost, h->stream));
+        MZ_HIP(hipStreamSynchronize(h->stream));
+        if (info[0] > 0) {
+            const int64_t ncap = ((int64_t)info[1] + 255) / 256 * 256;
+            const int64_t bcap = std::min<int64_t>(max_Nbins, ncap) + 8;
+            const int64_t per = ncap * (2 * sizeof(int32_t) + 1) + bcap * (sizeof(OctBinT<int32_t>) + 3 * sizeof(double));
+            const int64_t budget = (int64_t)2 << 30;
+            const int64_t G = std::max<int64_t>(1, std::min<int64_t>({(int64_t)info[0], (int64_t)2048, budget / per}));
+            int32_t *ib = (int32_t *)mz_scratch(h, "oct_big_idx", (size_t)(G * ncap * 2 * sizeof(int32_t)));
+            uint8_t *ob = (uint8_t *)mz_scratch(h, "oct_big_oct", (size_t)(G * ncap));
+            void *bb = mz_scratch(h, "oct_big_bins", (size_t)(G * bcap * sizeof(OctBinT<int32_t>)));
+            double *kb = (double *)mz_scratch(h, "oct_big_key", (size_t)(G * bcap * sizeof(double)));
+            int32_t *sb = (int32_t *)mz_scratch(h, "oct_big_slot", (size_t)(G * bcap * 2 * sizeof(int32_t)));
+            MZ_CHECK(ib && ob && bb && kb && sb, "merge_octree: scratch allocation for large cells failed");
+            A.big_pis = ib;
+            A.big_psort = ib + G * ncap;
+            A.big_poct = ob;
+            A.big_bins = bb;
+            A.big_bkey = kb;
+            A.big_bslot = sb;
+            A.big_ncap = (int32_t)ncap;
+            A.big_bcap = (int32_t)bcap;
+            hipLaunchKernelGGL(k_octree_merge_big, dim3((unsigned)G), dim3(64), 0, h->stream, A);
+        }
         hipLaunchKernelGGL(k_merge_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, old_pia, nc, ndel);
         hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
         hipLaunchKernelGGL(k_merge_push_ids, dim3(nc), dim3(64), 0, h->stream, A.sp, old_pia, off, off + nc,

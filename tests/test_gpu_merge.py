@@ -91,6 +91,64 @@ def test_octree_merge_vs_oracle(hip, mzo, split, init, bounds, clamp):
     st.close()
 
 
+def _octree_vs_oracle(hip, mzo, counts, thresholds, targets, seed, split=1, init=1, bounds=1, per_cell=False):
+    nc = len(counts)
+    pv, pia = _var_weight_cells(mzo, counts, seed=seed)
+    st = hip.DeviceStore(nc, 1, sum(counts) + 512)
+    helpers.upload_from_oracle(st, mzo, pv, pia)
+    st.set_grid(hip.Grid1DUniform(1e-6 * nc, nc))
+    st.set_seed(977)
+    tag = (5 << 4) | 3
+    before = st.download_species(1)
+    n = before["n_total"]
+    m0, p0, e0 = _moments(before["w"][:n], before["v"][:n])
+    kw = dict(split=split, init_bin_bounds=init, bin_bounds_compute=bounds, max_Nbins=4096, max_depth=10,
+              squash=False, tag=tag)
+    if per_cell:
+        n_merged = hip.merge_octree_targets(st, 1, thresholds, targets, **kw)
+    else:
+        n_merged = hip.merge_octree_N2_based(st, 1, threshold=thresholds[0], target_np=targets[0], **kw)
+    assert n_merged == sum(1 for k, t in zip(counts, thresholds) if k > t)
+    r = mzo.RNG(seed=977, tag=tag)
+    for c in range(1, nc + 1):
+        if pia.ind(c).n_local > thresholds[c - 1]:
+            oc = mzo.Octree(split=split, init_bin_bounds=init, bin_bounds_compute=bounds, max_Nbins=4096,
+                            max_depth=10)
+            mzo.lib().mzo_merge_octree_N2_based(r.ref, oc.p, pv.p, pia.p, c, 1, int(targets[c - 1]), None)
+    got = st.download_species(1)
+    assert got["err"] == 0
+    helpers.assert_state_matches(mzo, got, pv, pia, rtol=1e-12)
+    tab = got["pia"]
+    sel = np.concatenate([np.arange(r_[1] - 1, r_[2]) for r_ in tab if r_[3] > 0] +
+                         [np.arange(r_[4] - 1, r_[5]) for r_ in tab if r_[6] > 0])
+    m1, p1, e1 = _moments(got["w"][sel], got["v"][sel])
+    assert abs(m1 / m0 - 1) < 1e-12
+    assert np.all(np.abs(p1 - p0) <= 1e-12 * np.abs(p0).max())
+    assert abs(e1 / e0 - 1) < 1e-12
+    st.close()
+
+
+@pytest.mark.parametrize("target,counts,split,bounds", [
+    (64, [3000, 5000, 2049, 900, 2100, 401], 1, 1),
+    (1500, [1800, 2048, 4000, 600, 12000], 1, 1),
+    (700, [2500, 1900, 30000], 2, 2)])
+def test_octree_merge_large_cells_vs_oracle(hip, mzo, target, counts, split, bounds):
+    """Cells above 2048 particles, and cells whose split loop needs more than
+    512 bins (targets of 700 and 1500 with max_Nbins 4096), go to the
+    device-memory path: the same results as the oracle's per-cell merges."""
+    nc = len(counts)
+    _octree_vs_oracle(hip, mzo, counts, [400] * nc, [target] * nc, seed=61 + target, split=split, bounds=bounds)
+
+
+def test_octree_merge_per_cell_targets_vs_oracle(hip, mzo):
+    """The batched form of "merge cell c above thresholds[c] down to
+    targets[c]" against the oracle's per-cell calls with each cell's target."""
+    counts = [500, 2500, 300, 1200, 3100, 90, 6000]
+    thresholds = [400, 2000, 350, 1000, 3000, 60, 4500]
+    targets = [100, 1500, 64, 700, 2000, 30, 3000]
+    _octree_vs_oracle(hip, mzo, counts, thresholds, targets, seed=73, per_cell=True)
+
+
 def test_octree_merge_reference_kat(hip, mzo):
     """test/test_octree_merging.jl:76-163 on the device: 8 octants x 3
     particles merged to 16 (one split, depth 1), then to 2 (no split)."""
