@@ -45,6 +45,7 @@ import numpy as np
 
 from . import (MzError, PhysProps as _PhysProps, _check, _d, estimate_sigma_g_w_max, i32, load_library)
 from . import merge_octree_cells as _merge_octree_cells
+from . import merge_octree_targets as _merge_octree_targets
 from . import merge_nnls_cells as _merge_nnls_cells
 
 P = __import__("ctypes").POINTER
@@ -142,12 +143,32 @@ class _Queue:
         _invalidate(self.store)
 
     def _octree(self, key, run, rng):
-        oc, species, target_np, use_grid = key
+        oc, species, use_grid = key
         self._seed(rng)
         cells = [it[2] for it in run]
-        _merge_octree_cells(self.store, species, cells, target_np, oc.split, oc.init_bin_bounds,
-                            oc.bin_bounds_compute, oc.max_Nbins, oc.max_depth, use_grid,
-                            tag=rng.tag("merge", ("octree", species), cells))
+        targets = [it[3] for it in run]
+        tag = rng.tag("merge", ("octree", species), cells)
+        if len(set(targets)) == 1:
+            _merge_octree_cells(self.store, species, cells, targets[0], oc.split, oc.init_bin_bounds,
+                                oc.bin_bounds_compute, oc.max_Nbins, oc.max_depth, use_grid, tag=tag)
+        else:
+            # per-cell targets: each run of ascending cells is one launch that
+            # merges exactly its cells (threshold -1), each down to its own target
+            nc = self.store.n_cells
+            k0 = 0
+            while k0 < len(cells):
+                k1 = k0 + 1
+                while k1 < len(cells) and cells[k1] > cells[k1 - 1]:
+                    k1 += 1
+                thr = np.full(nc, np.iinfo(np.int32).max, dtype=np.int32)
+                tgt = np.zeros(nc, dtype=np.int32)
+                for c, t in zip(cells[k0:k1], targets[k0:k1]):
+                    thr[c - 1] = -1
+                    tgt[c - 1] = t
+                _merge_octree_targets(self.store, species, thr, tgt, oc.split, oc.init_bin_bounds,
+                                      oc.bin_bounds_compute, oc.max_Nbins, oc.max_depth, use_grid, squash=False,
+                                      tag=tag)
+                k0 = k1
         _invalidate(self.store)
 
     def _nnls(self, key, run, rng):
@@ -513,7 +534,7 @@ def merge_octree_N2_based(rng, octree, pv, pia, cell, species, target_np, grid=N
     target_np[, grid]) (src/merging/merging_octree_N2.jl:1060, 1088):
     queues the cell (grid: the grid-clamped variant)."""
     store = _store_of(pv)
-    _queue(store).push("octree", (octree, int(species), int(target_np), grid is not None), int(cell), None, rng)
+    _queue(store).push("octree", (octree, int(species), grid is not None), int(cell), int(target_np), rng)
 
 
 def merge_nnls_based(rng, nnls_merging, pv, pia, cell, species, vref=1.0, scaling="variance", n_rand_pairs=0,

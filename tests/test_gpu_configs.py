@@ -32,10 +32,13 @@ def _interactions(hip, st, its):
                                              vhs_o=its[i][k].vhs_o))() for k in range(n)] for i in range(n)])
 
 
-def test_batched_0d_cells_with_splitting(hip, mzo):
-    rng = np.random.default_rng(7)
-    nc = 192
-    counts = np.floor(np.exp(rng.uniform(math.log(20), math.log(2000), size=nc))).astype(int)
+def _batched_0d(hip, mzo, nc, nmax, seed=7, cap_mult=2, centred=False):
+    """Cells of log-uniform occupancy in [20, nmax), BKW velocities at 273 K,
+    w ~ U(0.5, 2) Fnum; the device store and the oracle's state.  centred:
+    every particle at its cell's centre (a merge's new positions, mean +- the
+    standard deviation of its bin's positions, then stay in the cell)."""
+    rng = np.random.default_rng(seed)
+    counts = np.floor(np.exp(rng.uniform(math.log(20), math.log(nmax), size=nc))).astype(int)
     n = int(counts.sum())
     T0, m = 273.0, ac.M_AR
     grid = mzo.grid1d(float(nc), nc)              # one unit-length cell per 0-D cell, V = 1
@@ -47,8 +50,8 @@ def test_batched_0d_cells_with_splitting(hip, mzo):
         k = int(counts[c - 1])
         for j in range(k):
             pv.pop_to(pos + j)
-            pv.set(pos + j, Fnum[c - 1] * rng.uniform(0.5, 2.0), [0.0, 0.0, 0.0],
-                   [c - 1 + rng.uniform(0.01, 0.99), 0.5, 0.5])
+            w, xr = Fnum[c - 1] * rng.uniform(0.5, 2.0), rng.uniform(0.01, 0.99)
+            pv.set(pos + j, w, [0.0, 0.0, 0.0], [c - 0.5 if centred else c - 1 + xr, 0.5, 0.5])
         r.s.stream, r.s.k = c, 0
         mzo.lib().mzo_sample_bkw(r.ref, pv.p, k, pos - 1, m, T0, (C.c_double * 3)(0.0, 0.0, 0.0))
         pia.set(c, 1, k, pos, pos + k - 1, k)
@@ -63,12 +66,19 @@ def test_batched_0d_cells_with_splitting(hip, mzo):
     sg = np.array([mzo.lib().mzo_estimate_sigma_g_w_max(C.byref(it), m, m, T0, T0, float(f), 1.0) for f in Fnum])
     for c in range(nc):
         cfs[c].sigma_g_w_max = sg[c]
-    st = hip.DeviceStore(nc, 1, 2 * n + 1024)
+    st = hip.DeviceStore(nc, 1, cap_mult * n + 1024)
     helpers.upload_from_oracle(st, mzo, pv, pia)
     st.set_grid(hip.Grid1DUniform(float(nc), nc))
     _interactions(hip, st, its)
     st.set_sigma_g_w_max(1, 1, sg)
     st.set_seed(7)
+    return dict(st=st, pv=pv, pia=pia, cfs=cfs, it=it, grid=grid, dt=dt, m=m, n=n, counts=counts)
+
+
+def test_batched_0d_cells_with_splitting(hip, mzo):
+    b = _batched_0d(hip, mzo, 192, 2000)
+    st, pv, pia, cfs, it, grid, dt, m, n = (b[k] for k in ("st", "pv", "pia", "cfs", "it", "grid", "dt", "m", "n"))
+    nc = 192
     sp = [hip.Species("Ar", m)]
     V = np.ones(nc)
     ref = mzo.Props(nc, 1)
@@ -96,6 +106,50 @@ def test_batched_0d_cells_with_splitting(hip, mzo):
     assert np.allclose(pp.n, ref.n, rtol=1e-12) and np.allclose(pp.T, ref.T, rtol=1e-10)
     # weight splitting conserves each cell's total weight (up to rounding)
     assert np.allclose(pp.n[0], n_cell0, rtol=1e-12)
+    st.close()
+
+
+def test_batched_0d_cells_split_sort_merge(hip, mzo):
+    """The batched 0-D step with the growth remedy: ntc! with splitting, the
+    sort, octree N:2 merging of each cell above its own threshold down to its
+    own initial count (merge_octree_N2_based! with a per-cell target_np, the
+    cells above 2,048 particles on the device-memory path), squash_pia!, five
+    steps against the oracle: index and PIA bit-exact, values to 1e-11."""
+    nc = 96
+    b = _batched_0d(hip, mzo, nc, 6000, seed=11, centred=True)
+    st, pv, pia, cfs, it, grid, dt, m, counts = (b[k] for k in ("st", "pv", "pia", "cfs", "it", "grid", "dt", "m",
+                                                                 "counts"))
+    # thresholds a little above the start, so merges begin within the five steps
+    thresholds = (counts + counts // 40 + 1).astype(np.int32)
+    targets = counts.astype(np.int32)
+    V = np.ones(nc)
+    merged = 0
+    for step in range(1, 6):
+        tag = (step << 4) | 1
+        hip.ntc(st, 1, 1, dt, V=V, tag=tag)
+        rr = mzo.RNG(seed=7, tag=tag)
+        for c in range(1, nc + 1):
+            mzo.lib().mzo_ntc_single(rr.ref, C.byref(cfs[c - 1]), C.byref(it), pv.p, pia.p, c, 1, dt, 1.0, 1e-16, 0)
+        hip.sort_particles(st, 1)
+        mzo.lib().mzo_sort_particles(C.byref(grid), pv.p, pia.p, 1)
+        mtag = (step << 4) | 3
+        got_m = hip.merge_octree_targets(st, 1, thresholds, targets, max_Nbins=16384, max_depth=10, squash=True,
+                                         tag=mtag)
+        rm = mzo.RNG(seed=7, tag=mtag)
+        exp_m = 0
+        for c in range(1, nc + 1):
+            if pia.ind(c).n_local > thresholds[c - 1]:
+                oc = mzo.Octree(max_Nbins=16384, max_depth=10)
+                mzo.lib().mzo_merge_octree_N2_based(rm.ref, oc.p, pv.p, pia.p, c, 1, int(targets[c - 1]), None)
+                exp_m += 1
+        mzo.lib().mzo_squash_pia(pv.p, pia.p, 1)
+        assert got_m == exp_m
+        merged += got_m
+        got = st.download_species(1)
+        assert got["err"] == 0
+        helpers.assert_state_matches(mzo, got, pv, pia, rtol=1e-11)
+    assert merged > nc // 2
+    assert any(k > 2048 for k in counts)
     st.close()
 
 

@@ -122,3 +122,60 @@ def test_dropin_varweight_loop_vs_oracle(hip, mzo):
     assert got["err"] == 0
     helpers.assert_state_matches(mzo, got, pv, opia, rtol=1e-12)
     st.close()
+
+
+def test_dropin_per_cell_targets_vs_oracle(hip, mzo):
+    """A loop that merges every cell above its own threshold down to its own
+    target_np (merge_octree_N2_based!(rng, octree, pv, pia, cell, 1,
+    targets[cell], grid) per cell, then one squash_pia!): the queued calls
+    with different targets go to the device as one launch, and the state
+    matches the oracle's per-cell calls."""
+    nc = 6
+    thresholds = [0] + [118 + 3 * c for c in range(1, nc + 1)]
+    targets = [0] + [28 + 6 * c for c in range(1, nc + 1)]
+    case = helpers.couette_case(mzo, n_cells=nc, ppc=125, seed=14, dt=2e-10, sgwm_mult=40.0,
+                                weights=lambda rng, n: rng.lognormal(0.0, 1.0, n) / np.exp(0.5))
+    D, st, particles, pia, grid, species_data, boundaries = _setup(hip, mzo, case)
+    rng = D.DeviceRNG(case["seed"])
+    collision_factors = D.CollisionFactorsArray(st)
+    collision_data = D.CollisionData()
+    interaction_data = [[case["it"]]]
+    gridsorter = D.GridSortInPlace(grid, nc * 125)
+    oc = D.OctreeN2Merge("OctreeBinMidSplit", init_bin_bounds="OctreeInitBinMinMaxVel", max_Nbins=6000)
+    Δt = case["dt"]
+    n_timesteps = 6
+    merges = 0
+    for t in range(1, n_timesteps + 1):
+        for cell in range(1, grid.n_cells + 1):
+            D.ntc(rng, collision_factors[1, 1, cell],
+                  collision_data, interaction_data, particles[0], pia, cell, 1, Δt, grid.cells[cell].V)
+        to_merge = [cell for cell in range(1, grid.n_cells + 1) if pia.indexer[cell, 1].n_local > thresholds[cell]]
+        for cell in to_merge:
+            D.merge_octree_N2_based(rng, oc, particles[0], pia, cell, 1, targets[cell], grid)
+        merges += len(to_merge)
+        D.squash_pia(particles, pia)
+        D.convect_particles(rng, grid, boundaries, particles[0], pia, 1, species_data, Δt)
+        D.sort_particles(gridsorter, grid, particles[0], pia, 1)
+    assert merges > 2
+    pv, opia, g = case["pv"], case["pia"], case["grid"]
+    og = mzo.grid1d(case["L"], case["n_cells"])
+    o_merges = 0
+    for step in range(1, n_timesteps + 1):
+        r_ntc = mzo.RNG(seed=case["seed"], tag=(step << 4) | 1)
+        r_merge = mzo.RNG(seed=case["seed"], tag=(step << 4) | 3)
+        for c in range(1, nc + 1):
+            mzo.lib().mzo_ntc_single(r_ntc.ref, C.byref(case["cfs"][c - 1]), C.byref(case["it"]), pv.p, opia.p, c,
+                                     1, Δt, g.dx, 1e-16, 0)
+        for c in [c for c in range(1, nc + 1) if opia.ind(c).n_local > thresholds[c]]:
+            octree = mzo.Octree(split=1, init_bin_bounds=1, bin_bounds_compute=1, max_Nbins=6000, max_depth=10)
+            mzo.lib().mzo_merge_octree_N2_based(r_merge.ref, octree.p, pv.p, opia.p, c, 1, targets[c], C.byref(og))
+            o_merges += 1
+        mzo.lib().mzo_squash_pia(pv.p, opia.p, 1)
+        r = mzo.RNG(seed=case["seed"], tag=(step << 4) | 2)
+        mzo.lib().mzo_convect_particles(r.ref, C.byref(g), C.byref(case["walls"]), pv.p, opia.p, 1, Δt)
+        mzo.lib().mzo_sort_particles(C.byref(g), pv.p, opia.p, 1)
+    assert o_merges == merges
+    got = st.download_species(1)
+    assert got["err"] == 0
+    helpers.assert_state_matches(mzo, got, pv, opia, rtol=1e-12)
+    st.close()
