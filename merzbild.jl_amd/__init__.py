@@ -1009,8 +1009,9 @@ def couette_autotune(store, step0, dt, species_data=None, steps=200,
     """Pick the slab shape (cells per slab, particles per thread) of the
     slab-resident Couette step by running `steps` steps with each candidate
     on the store itself (the steps advance the simulation, as warmup steps
-    would) and keeping the fastest one that ran without falling back.
-    Returns (best shape, {shape: ms per step}, next step number)."""
+    would) and keeping the fastest one that ran without falling back; then
+    two steps with that shape.  Returns (best shape, {shape: ms per step},
+    next step number)."""
     times = {}
     s = step0
     for shp in shapes:
@@ -1025,12 +1026,12 @@ def couette_autotune(store, step0, dt, species_data=None, steps=200,
         s += steps
         if path == "slab" and ms > 0:
             times[tuple(shp)] = ms
-    if not times:
-        store.set_slab_shape(0, 0)
-        return (0, 0), times, s
-    best = min(times, key=times.get)
+    best = min(times, key=times.get) if times else (0, 0)
     store.set_slab_shape(*best)
-    return best, times, s
+    # two more steps with the chosen shape: the shape's launch set-up
+    # (occupancy query, scratch sizes) is then done before the caller's run
+    couette_steps(store, 2, s, dt, species_data, mode="slab")
+    return best, times, s + 2
 
 
 def batch0d_autotune(store, step0, dt, species_data=None, steps=3,

@@ -83,6 +83,7 @@ struct mz_store {
     std::vector<std::pair<std::string, MzEventPair>> open_events;
     std::map<std::string, std::vector<MzEventPair>> pending;
     std::map<std::string, std::pair<double, int64_t>> timing_acc;
+    std::vector<hipEvent_t> event_pool;   // read events, reused by later brackets
     // captured step graphs
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;

@@ -31,6 +31,7 @@
 
 #define OCT_MAX_N 2048
 #define OCT_MAX_BINS 512
+#define OCT_BIG_BLOCKS 1024   // 64-bin blocks whose maxima the device-memory path keeps in LDS
 
 struct OctArgs {
     MzSpView sp;
@@ -77,6 +78,10 @@ struct OctWork {
     double *bkey;
     int32_t *bslot, *bdraw;
     int32_t bins_cap;
+    // per block of 64 bins: the largest key and its lowest bin (nullptr:
+    // the pick scans the keys themselves)
+    double *bmv;
+    int32_t *bmi;
 };
 
 // The small per-cell state of the split loop and the output scan, in LDS.
@@ -237,6 +242,10 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
         b.np = N; b.start = 0; b.end = N - 1; b.depth = 0;
         b.refine = (N > 2) && (A.max_depth > 0);
         bkey[0] = b.refine ? b.w : -1.0;
+        if (W.bmv) {
+            W.bmv[0] = bkey[0];
+            W.bmi[0] = 0;
+        }
         Nbins_s = 1;
         total_post_s = N >= 2 ? 2 : N;
         for (int d = 0; d < 3; ++d) vel_mid[d] = 0.0;
@@ -254,9 +263,17 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
             // refine pick: first maximum of w over refinable bins (lowest id on ties)
             double bw = -1.0;
             int bid = -1;
-            for (int b = lane; b < Nbins_s; b += 64) {
-                const double kb = bkey[b];
-                if (kb > bw) { bw = kb; bid = b; }
+            if (W.bmv) {
+                // the blocks' maxima (each the lowest bin of its block's maximum)
+                for (int k = lane; k < (Nbins_s + 63) >> 6; k += 64) {
+                    const double kb = W.bmv[k];
+                    if (kb > bw) { bw = kb; bid = W.bmi[k]; }
+                }
+            } else {
+                for (int b = lane; b < Nbins_s; b += 64) {
+                    const double kb = bkey[b];
+                    if (kb > bw) { bw = kb; bid = b; }
+                }
             }
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) {
@@ -371,8 +388,8 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
             // the children (split_bin!, :588-633), lane o for octant o: the
             // k-th non-empty octant becomes bin b (k = 1) or Nb + k - 2
             // (get_new_bin_id), its particles the octant's segment
+            const int Nb = Nbins_s;
             {
-                const int Nb = Nbins_s;
                 double pmin[3], pmax[3];
                 for (int d = 0; d < 3; ++d) { pmin[d] = bins[b].vmin[d]; pmax[d] = bins[b].vmax[d]; }
                 const int o = lane;
@@ -417,6 +434,30 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
                 }
             }
             oct_wave_sync();
+            if (W.bmv) {
+                // the keys of bin b and of the new bins Nb .. Nbins - 1 changed:
+                // their blocks' maxima again (at most three blocks)
+                const int nb_now = Nbins_s;
+                const int k0 = b >> 6, k1 = nb_now > Nb ? Nb >> 6 : k0, k2 = nb_now > Nb ? (nb_now - 1) >> 6 : k0;
+                for (int k = k1; k <= k2 + 1; ++k) {
+                    const int blk = k == k2 + 1 ? k0 : k;
+                    if (k == k2 + 1 && k0 >= k1 && k0 <= k2) break;   // b's block was among them
+                    const int i = (blk << 6) + lane;
+                    double v = i < nb_now ? bkey[i] : -2.0;
+                    int vi = i;
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) {
+                        const double ov = __shfl_xor(v, o, 64);
+                        const int oi = __shfl_xor(vi, o, 64);
+                        if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
+                    }
+                    if (lane == 0) {
+                        W.bmv[blk] = v;
+                        W.bmi[blk] = vi;
+                    }
+                }
+                oct_wave_sync();
+            }
             if (stop_s) break;
         }
     }
@@ -654,6 +695,8 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
     W.bslot = reinterpret_cast<int32_t *>(psort);
     W.bdraw = W.bslot + OCT_MAX_BINS;
     W.bins_cap = OCT_MAX_BINS;
+    W.bmv = nullptr;
+    W.bmi = nullptr;
     oct_merge_cell<int16_t, 4, false>(A, c, N, target, W, S);
 }
 
@@ -662,6 +705,8 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
 // ... of the list.
 __global__ void __launch_bounds__(64) k_octree_merge_big(OctArgs A) {
     __shared__ OctShared S;
+    __shared__ double s_bmv[OCT_BIG_BLOCKS];
+    __shared__ int32_t s_bmi[OCT_BIG_BLOCKS];
     A.rp = mz_rng_resolve(A.rp);
     const int32_t n_list = A.big_info[0];
     const size_t g = blockIdx.x;
@@ -674,6 +719,9 @@ __global__ void __launch_bounds__(64) k_octree_merge_big(OctArgs A) {
     W.bslot = A.big_bslot + g * (size_t)A.big_bcap * 2;
     W.bdraw = W.bslot + A.big_bcap;
     W.bins_cap = A.big_bcap;
+    const bool blocks = A.big_bcap <= OCT_BIG_BLOCKS * 64;
+    W.bmv = blocks ? s_bmv : nullptr;
+    W.bmi = blocks ? s_bmi : nullptr;
     for (int32_t j = blockIdx.x; j < n_list; j += gridDim.x) {
         const int c = A.big_list[j];
         int32_t N, target;

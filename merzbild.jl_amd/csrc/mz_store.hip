@@ -39,18 +39,28 @@ extern "C" int mz_device_info(char *name, int name_len, int *n_cu, double *hbm_g
 
 // ===========================================================================
 // timing
+// an event from the store's pool (created when the pool is empty)
+static int timing_event(mz_store *h, hipEvent_t *e) {
+    if (!h->event_pool.empty()) {
+        *e = h->event_pool.back();
+        h->event_pool.pop_back();
+        return 0;
+    }
+    MZ_HIP(hipEventCreate(e));
+    return 0;
+}
+
 int mz_timing_begin(mz_store *h, const char *family) {
     if (!h->timing) return 0;
     MzEventPair ev;
-    MZ_HIP(hipEventCreate(&ev.a));
-    if (hipEventCreate(&ev.b) != hipSuccess) {
-        hipEventDestroy(ev.a);
-        mz_set_error("timing: hipEventCreate failed");
+    if (timing_event(h, &ev.a)) return -1;
+    if (timing_event(h, &ev.b)) {
+        h->event_pool.push_back(ev.a);
         return -1;
     }
     if (hipEventRecord(ev.a, h->stream) != hipSuccess) {
-        hipEventDestroy(ev.a);
-        hipEventDestroy(ev.b);
+        h->event_pool.push_back(ev.a);
+        h->event_pool.push_back(ev.b);
         mz_set_error("timing: hipEventRecord failed");
         return -1;
     }
@@ -81,8 +91,8 @@ static int resolve_timing(mz_store *h) {
             MZ_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
             acc.first += ms;
             acc.second += ev.n;
-            hipEventDestroy(ev.a);
-            hipEventDestroy(ev.b);
+            h->event_pool.push_back(ev.a);
+            h->event_pool.push_back(ev.b);
         }
     }
     h->pending.clear();
@@ -243,6 +253,8 @@ extern "C" int mz_destroy(mz_store *h) {
             hipEventDestroy(ev.a);
             hipEventDestroy(ev.b);
         }
+    for (hipEvent_t e : h->event_pool) hipEventDestroy(e);
+    h->event_pool.clear();
     h->open_events.clear();
     h->pending.clear();
     if (h->h_report) hipHostFree(h->h_report);
