@@ -330,26 +330,55 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
             int cq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             double sw = 0.0;
             const double mid0 = vel_mid[0], mid1 = vel_mid[1], mid2 = vel_mid[2];
+            // the next chunk's particle loads are issued before this chunk's
+            // ballots and sums, so their latency overlaps that work
+            double nw = 0.0, nv0 = 0.0, nv1 = 0.0, nv2 = 0.0;
+            if (bs + lane <= be) {
+                const int64_t p = oct_slot(pis[bs + lane], s1, n1, s2);
+                nw = A.sp.w[p]; nv0 = A.sp.v0[p]; nv1 = A.sp.v1[p]; nv2 = A.sp.v2[p];
+            }
             for (int i0 = bs; i0 <= be; i0 += 64) {
                 const int i = i0 + lane;
+                const double wi = nw, c0 = nv0, c1 = nv1, c2 = nv2;
+                if (i + 64 <= be) {
+                    const int64_t p = oct_slot(pis[i + 64], s1, n1, s2);
+                    nw = A.sp.w[p]; nv0 = A.sp.v0[p]; nv1 = A.sp.v1[p]; nv2 = A.sp.v2[p];
+                }
                 int o = 0;
-                double wi = 0.0;
                 if (i <= be) {
-                    const int64_t p = oct_slot(pis[i], s1, n1, s2);
-                    wi = A.sp.w[p];
                     o = 1;
-                    if (A.sp.v0[p] > mid0) o += 1;
-                    if (A.sp.v1[p] > mid1) o += 2;
-                    if (A.sp.v2[p] > mid2) o += 4;
+                    if (c0 > mid0) o += 1;
+                    if (c1 > mid1) o += 2;
+                    if (c2 > mid2) o += 4;
                     poct[i - bs] = (uint8_t)o;
                 }
+                uint64_t mq = 0;   // lane q: the chunk's lanes in octant q
 #pragma unroll
-                for (int q = 1; q <= 8; ++q) cq[q] += __popcll(__ballot(o == q));
-                const int nl = min(64, be - i0 + 1);
-                for (int l = 0; l < nl; ++l) {
-                    const int ol = __builtin_amdgcn_readlane(o, l);
-                    const double wl = mz_readlane_d(wi, l);
-                    if (lane == ol) sw += wl;
+                for (int q = 1; q <= 8; ++q) {
+                    const uint64_t m = __ballot(o == q);
+                    cq[q] += __popcll(m);
+                    if (lane == q) mq = m;
+                }
+                // lane q adds its octant's weights in lane order, four at a
+                // time, each fetched from its lane with ds_bpermute
+                const int wlo = __double2loint(wi), whi = __double2hiint(wi);
+                while (__ballot(mq != 0)) {
+                    int src[4];
+                    bool has[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        has[u] = mq != 0;
+                        src[u] = has[u] ? (int)__builtin_ctzll(mq) : 0;
+                        mq &= mq - 1;
+                    }
+                    double wv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        wv[u] = __hiloint2double(__builtin_amdgcn_ds_bpermute(src[u] << 2, whi),
+                                                 __builtin_amdgcn_ds_bpermute(src[u] << 2, wlo));
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (has[u]) sw += wv[u];
                 }
             }
             if (lane >= 1 && lane <= 8) ndens8[lane] = sw;
