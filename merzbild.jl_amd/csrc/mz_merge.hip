@@ -862,7 +862,9 @@ extern "C" int mz_merge_octree_targets(mz_store *h, int32_t species, const int32
     MZ_CHECK(d != nullptr, "merge_octree_targets: scratch allocation failed");
     MZ_HIP(hipMemcpyAsync(d, thresholds, (size_t)nc * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
     MZ_HIP(hipMemcpyAsync(d + nc, targets, (size_t)nc * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
-    int r = octree_merge_launch(h, species, 0, nullptr, 0, split, init_bounds, bounds_mode, max_Nbins, max_depth,
+    int32_t tmax = 2;
+    for (int c = 0; c < nc; ++c) tmax = std::max(tmax, targets[c]);
+    int r = octree_merge_launch(h, species, 0, nullptr, tmax, split, init_bounds, bounds_mode, max_Nbins, max_depth,
                                 grid_clamp, tag, rng_mode, n_merged, d, d + nc);
     if (r) return r;
     if (squash) return mz_squash(h, species);
@@ -956,11 +958,24 @@ static int octree_merge_launch(mz_store *h, int32_t species, int32_t threshold, 
         MZ_HIP(hipMemcpyAsync(info, cnt + 1, sizeof(info), hipMemcpyDeviceToHost, h->stream));
         MZ_HIP(hipStreamSynchronize(h->stream));
         if (info[0] > 0) {
+            // a cell's bins hold at least one particle each and count at least
+            // one each towards the post-merge count, which stays at most the
+            // target: at most min(max_Nbins, N, target) bins (target_np is the
+            // largest per-cell target when there are several)
             const int64_t ncap = ((int64_t)info[1] + 255) / 256 * 256;
-            const int64_t bcap = std::min<int64_t>(max_Nbins, ncap) + 8;
+            const int64_t bcap = std::min<int64_t>({(int64_t)max_Nbins, ncap, (int64_t)std::max(target_np, 2)}) + 8;
             const int64_t per = ncap * (2 * sizeof(int32_t) + 1) + bcap * (sizeof(OctBinT<int32_t>) + 3 * sizeof(double));
-            const int64_t budget = (int64_t)2 << 30;
-            const int64_t G = std::max<int64_t>(1, std::min<int64_t>({(int64_t)info[0], (int64_t)2048, budget / per}));
+            // workgroups: up to 12 per CU (three one-wave workgroups per SIMD),
+            // within a quarter of the free memory (at least 2 GiB, at most 16 GiB)
+            size_t mfree = 0, mtot = 0;
+            MZ_HIP(hipMemGetInfo(&mfree, &mtot));
+            const int64_t budget = std::min<int64_t>((int64_t)16 << 30, std::max<int64_t>((int64_t)2 << 30,
+                                                                                          (int64_t)(mfree / 4)));
+            int dev = 0, n_cu = 0;
+            MZ_HIP(hipGetDevice(&dev));
+            MZ_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+            const int64_t G = std::max<int64_t>(1, std::min<int64_t>({(int64_t)info[0], (int64_t)12 * n_cu,
+                                                                      budget / per}));
             int32_t *ib = (int32_t *)mz_scratch(h, "oct_big_idx", (size_t)(G * ncap * 2 * sizeof(int32_t)));
             uint8_t *ob = (uint8_t *)mz_scratch(h, "oct_big_oct", (size_t)(G * ncap));
             void *bb = mz_scratch(h, "oct_big_bins", (size_t)(G * bcap * sizeof(OctBinT<int32_t>)));
