@@ -36,10 +36,12 @@ def _assert_same(a, b):
         assert np.array_equal(np.asarray(ca[k]), np.asarray(cb[k])), k
 
 
-@pytest.mark.parametrize("shape", [(0, 0), (3, 2), (1, 1)])
+@pytest.mark.parametrize("shape", [(0, 0), (3, 2), (1, 1), (6, 4)])
 def test_slab_couette_vs_oracle(hip, mzo, shape):
     """Thirty Couette steps (CFL 0.17) on the slab kernel against the oracle:
-    index and PIA bit-exact, velocities and T to 1e-11 (as the graph path)."""
+    index and PIA bit-exact, velocities and T to 1e-11 (as the graph path).
+    Slabs of up to four cells collide each cell on a whole wavefront, larger
+    ones (6 cells here) one lane per cell."""
     case = helpers.couette_case(mzo, n_cells=20, ppc=100, seed=3, dt=2e-10, sgwm_mult=50.0)
     st = helpers.store_from_case(hip, case)
     st.set_slab_shape(*shape)
