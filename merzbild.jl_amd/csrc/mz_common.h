@@ -219,6 +219,7 @@ extern "C" __device__ double __ockl_wfred_max_f64(double);
 extern "C" __device__ double __ockl_wfred_min_f64(double);
 extern "C" __device__ unsigned long long __ockl_wfred_max_u64(unsigned long long);
 extern "C" __device__ float __ockl_wfred_max_f32(float);
+extern "C" __device__ int __ockl_wfred_min_i32(int);
 // Last-block reductions publish their partials with agent-scope relaxed
 // atomic stores, which the hardware keeps coherent across the XCDs' L2
 // caches; the writer then only has to wait for its stores to complete
@@ -235,6 +236,7 @@ __device__ inline double mz_wave_sum(double v) { return __ockl_wfred_add_f64(v);
 __device__ inline double mz_wave_max(double v) { return __ockl_wfred_max_f64(v); }
 __device__ inline double mz_wave_min(double v) { return __ockl_wfred_min_f64(v); }
 __device__ inline uint64_t mz_wave_max_u64(uint64_t v) { return __ockl_wfred_max_u64(v); }
+__device__ inline int32_t mz_wave_min_i32(int32_t v) { return __ockl_wfred_min_i32(v); }
 __device__ inline int mz_wave_sum_i(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

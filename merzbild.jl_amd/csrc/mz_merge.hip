@@ -275,11 +275,10 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
                     if (kb > bw) { bw = kb; bid = b; }
                 }
             }
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                double ow = __shfl_xor(bw, o, 64);
-                int oi = __shfl_xor(bid, o, 64);
-                if (oi >= 0 && (bid < 0 || ow > bw || (ow == bw && oi < bid))) { bw = ow; bid = oi; }
+            // the largest key, then the lowest bin holding it (two wave reductions)
+            {
+                const double mw = mz_wave_max(bw);
+                bid = mw > -1.0 ? mz_wave_min_i32(bw == mw ? bid : 0x7FFFFFFF) : -1;
             }
             if (bid < 0 || total_post_s + 14 > target_np) break;
             // split_bin! (merging_octree_N2.jl:503-633)
@@ -473,13 +472,8 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
                     if (k == k2 + 1 && k0 >= k1 && k0 <= k2) break;   // b's block was among them
                     const int i = (blk << 6) + lane;
                     double v = i < nb_now ? bkey[i] : -2.0;
-                    int vi = i;
-#pragma unroll
-                    for (int o = 32; o > 0; o >>= 1) {
-                        const double ov = __shfl_xor(v, o, 64);
-                        const int oi = __shfl_xor(vi, o, 64);
-                        if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
-                    }
+                    v = mz_wave_max(v);
+                    const int vi = mz_wave_min_i32(i < nb_now && bkey[i] == v ? i : 0x7FFFFFFF);
                     if (lane == 0) {
                         W.bmv[blk] = v;
                         W.bmi[blk] = vi;
