@@ -220,6 +220,10 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
     constexpr int WC = SNAP ? SLAB_WALL_CAP : 1;
     __shared__ double swl[3][WC];                  // reflected velocities drawn ahead (wall slab)
     __shared__ int32_t swf[WC];                    // 1 where swl holds the entry
+    // the flight with that one bounce worked out ahead: the velocity it
+    // assumed and the end position (NaN where the flight bounces again)
+    constexpr int WX = (SNAP && !AVG) ? SLAB_WALL_CAP : 1;
+    __shared__ double swv[WX], swx[WX];
     __shared__ int32_t s_abort, s_n, s_na[2];
     __shared__ int64_t s_red[4][4];
     constexpr int SC = AVG ? SLAB_SURF_CAP : 1;
@@ -377,6 +381,18 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
             double u0, u1, u2;
             mz_diffuse_reflect(rpw, sr, u0, u1, u2, A.wl.vsq[wall_side][0], wn, A.wl.v[wall_side]);
             swl[0][j] = u0; swl[1][j] = u1; swl[2][j] = u2;
+            if (SNAP && !AVG) {
+                // mz_convect_impl's first bounce at this wall with the velocity
+                // before NTC; convection takes the result when NTC left that
+                // velocity unchanged (the same operations on the same values)
+                const double x0 = sx0[lo + j], v0 = ss0[lo + j];
+                const double t_rest = A.dt - (wall_side == 1 ? fabs((A.g.L - x0) / v0) : fabs(x0 / v0));
+                const double x_new = fma(u0, t_rest, wall_side == 1 ? A.g.L : 0.0);
+                const bool again = (x_new >= A.g.L) || (x_new <= 0.0);
+                swv[j] = v0;
+                swx[j] = again ? __builtin_nan("") : (x_new < A.g.min_x ? A.g.min_x
+                                                      : (x_new > A.g.max_x ? A.g.max_x : x_new));
+            }
         }
     };
     if (A.n_steps > 0) fill_draws(A.step0, tid, SLAB_THREADS);
@@ -455,14 +471,24 @@ __global__ void __launch_bounds__(SLAB_THREADS, R <= 2 ? 2 : 1) k_slab_couette(S
                     } else {
                         double p0 = 0.0, p1 = 0.0, p2 = 0.0;
                         int pre_bc = -1;
+                        bool done = false;
                         if (wall_side >= 0) {
                             const int j = i - cst[wall_k];
                             if (j >= 0 && j < min(ccn[wall_k], SLAB_WALL_CAP) && swf[j]) {
                                 p0 = swl[0][j]; p1 = swl[1][j]; p2 = swl[2][j];
                                 pre_bc = wall_side;
+                                if (SNAP && !AVG) {
+                                    const double xe = swx[j];
+                                    if (swv[j] == pv0[r] && xe == xe) {
+                                        px0[r] = xe;
+                                        pv0[r] = p0; pv1[r] = p1; pv2[r] = p2;
+                                        done = true;
+                                    }
+                                }
                             }
                         }
-                        if (AVG) {
+                        if (done) {
+                        } else if (AVG) {
                             SlabSurfRec sr{srec[wave], &scnt[wave], &s_abort, pw[r], 0.0, 0.0, 0.0};
                             mz_convect_one_surf(sr, rp, A.g, A.wl, 0, A.dt, pid[r], px0[r], pv0[r], pv1[r], pv2[r],
                                                 &A.a.ctr->err, pre_bc, p0, p1, p2);
