@@ -68,21 +68,20 @@ __device__ inline PRef pref_scratch(const NtcArgs &A, int half, int64_t e) {
     return PRef{A.sc[0] + o, A.sc[1] + o, A.sc[2] + o, A.sc[3] + o, A.sc[4] + o, A.sc[5] + o, A.sc[6] + o, e};
 }
 
-__global__ void k_ntc(NtcArgs A) {
-    const int c = A.cell_lo + blockIdx.x * blockDim.x + threadIdx.x;
-    if (c > A.cell_hi) return;
-    A.rp = mz_rng_resolve(A.rp);
+// The candidate loop of cell c with the draws of its stream from dr (next(),
+// seek(k), exhausted()).
+template <class Draw>
+__device__ __forceinline__ void ntc_cell(const NtcArgs &A, const int c, Draw &dr) {
     const int32_t n1a = A.a.pia.n1[c], n2a = A.a.pia.n2[c];
     const int32_t n1b = A.b.pia.n1[c], n2b = A.b.pia.n2[c];
     const int32_t nla = A.a.pia.nloc[c], nlb = A.b.pia.nloc[c];
     double sgwm = A.sgwm[c];
-    MzStream st = mz_stream((uint64_t)c);
     int64_t n_coll;
     if (A.ncoll_in) {
         n_coll = A.ncoll_in[c];
-        st.k = 1;
+        dr.seek(1);
     } else {
-        double R = mz_rand(A.rp, st);
+        double R = dr.next();
         n_coll = (int64_t)floor(mz_ncoll_est(A.single, A.dt, nla, nlb, sgwm, A.V[c], R));
     }
     if (!(n_coll <= MZ_NTC_MAX_CANDIDATES)) {   // also catches NaN estimates
@@ -94,12 +93,12 @@ __global__ void k_ntc(NtcArgs A) {
     int32_t performed = 0, eqw = 0;
     const int64_t s_base = A.may_split ? A.soff[c] : 0;
     for (int64_t cc = 0; cc < n_coll; ++cc) {
-        int32_t i = (int32_t)floor(mz_rand(A.rp, st) * (double)na);
-        int32_t k = (int32_t)floor(mz_rand(A.rp, st) * (double)(A.single ? na : nb));
+        int32_t i = (int32_t)floor(dr.next() * (double)na);
+        int32_t k = (int32_t)floor(dr.next() * (double)(A.single ? na : nb));
         if (A.single) {
             while (i == k) {
-                k = (int32_t)floor(mz_rand(A.rp, st) * (double)na);
-                if (A.rp.mode == 1 && (int64_t)st.k > A.rp.cols) return;   // replay row exhausted
+                k = (int32_t)floor(dr.next() * (double)na);
+                if (dr.exhausted()) return;   // replay row exhausted
             }
         }
         // resolve both candidates: existing positions or this call's scratch entries
@@ -118,7 +117,7 @@ __global__ void k_ntc(NtcArgs A) {
         double wi = pi.w[pi.i], wk = pk.w[pk.i];
         double sgw = mz_vhs_sgw(A.factor, A.expo, g, wi, wk);
         sgwm = sgw > sgwm ? sgw : sgwm;
-        if (!(mz_rand(A.rp, st) < sgw / sgwm)) continue;
+        if (!(dr.next() < sgw / sgwm)) continue;
         ++performed;
         double vc0, vc1, vc2;
         mz_com(A.mu1, A.mu2, vi0, vi1, vi2, vk0, vk1, vk2, vc0, vc1, vc2);
@@ -158,7 +157,9 @@ __global__ void k_ntc(NtcArgs A) {
         }
         // scatter_vhs!: phi first, then cos(theta)
         double ui0, ui1, ui2, uk0, uk1, uk2;
-        mz_vhs_scatter(A.rp, st, A.mu1, A.mu2, g, vc0, vc1, vc2, ui0, ui1, ui2, uk0, uk1, uk2);
+        const double R_phi = dr.next();
+        const double R_ct = dr.next();
+        mz_vhs_post(A.mu1, A.mu2, g, vc0, vc1, vc2, R_phi, R_ct, ui0, ui1, ui2, uk0, uk1, uk2);
         pi.v0[pi.i] = ui0; pi.v1[pi.i] = ui1; pi.v2[pi.i] = ui2;
         pk.v0[pk.i] = uk0; pk.v1[pk.i] = uk1; pk.v2[pk.i] = uk2;
     }
@@ -173,6 +174,61 @@ __global__ void k_ntc(NtcArgs A) {
         A.created[c] = A.single ? ca + cb : ca;
         A.created[(int64_t)(A.cell_hi + 1) + c] = A.single ? 0 : cb;
     }
+}
+
+
+// Draws of a cell's stream for one lane: mz_rand.
+struct NtcLaneDraw {
+    const MzRngParams &rp;
+    MzStream st;
+    __device__ double next() { return mz_rand(rp, st); }
+    __device__ bool exhausted() const { return rp.mode == 1 && (int64_t)st.k > rp.cols; }
+    __device__ void seek(uint64_t k) { st.k = k; }
+};
+
+// Draws of a cell's stream for a whole wavefront running the cell (Philox
+// draws only): lane L holds Philox block base / 2 + L, i.e. draws base + 2 L
+// and base + 2 L + 1, and every lane reads draw k from its lane; the block
+// of 128 draws is computed again, all lanes at once, when k leaves it.
+struct NtcWaveDraw {
+    const MzRngParams &rp;
+    uint64_t stream, k, base;
+    double h0, h1;
+    __device__ double next() {
+        const uint64_t kk = k++;
+        if (kk < base || kk >= base + 128) {
+            base = kk & ~(uint64_t)1;
+            mz_philox_pair(rp, stream, (base >> 1) + (uint64_t)(threadIdx.x & 63), h0, h1);
+        }
+        const uint64_t idx = kk - base;
+        const int l = (int)(idx >> 1);
+        const double a = mz_readlane_d(h0, l), b = mz_readlane_d(h1, l);
+        return (idx & 1) ? b : a;
+    }
+    __device__ bool exhausted() const { return false; }
+    __device__ void seek(uint64_t kk) { k = kk; }
+};
+
+// One lane per cell; cells run in parallel (64-lane workgroups, so the
+// loop keeps its state in registers).
+__global__ void __launch_bounds__(64) k_ntc(NtcArgs A) {
+    const int c = A.cell_lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > A.cell_hi) return;
+    A.rp = mz_rng_resolve(A.rp);
+    NtcLaneDraw dr{A.rp, mz_stream((uint64_t)c)};
+    ntc_cell(A, c, dr);
+}
+
+// One wavefront per cell (Philox draws): every lane runs the cell's loop on
+// the same values, so there is no divergence between cells and the draws
+// come from registers; for calls over few cells, which one lane per cell
+// would spread over a handful of wavefronts.
+__global__ void __launch_bounds__(64) k_ntc_wave(NtcArgs A) {
+    const int c = A.cell_lo + blockIdx.x;
+    if (c > A.cell_hi) return;
+    A.rp = mz_rng_resolve(A.rp);
+    NtcWaveDraw dr{A.rp, (uint64_t)c, 0, ~(uint64_t)0, 0.0, 0.0};
+    ntc_cell(A, c, dr);
 }
 
 // Count pass of the split path: n_coll of every cell (draw 0 of its stream).
@@ -324,6 +380,16 @@ static NtcArgs base_args(mz_store *h, int s1, int s2, double dt, const double *d
     return A;
 }
 
+// The candidate loops over ncells cells: a wavefront per cell for calls over
+// few cells (Philox draws), one lane per cell otherwise.
+#define NTC_WAVE_MAX_CELLS 4096
+static void launch_k_ntc(mz_store *h, const NtcArgs &A, int ncells) {
+    if (ncells <= NTC_WAVE_MAX_CELLS && A.rp.mode == 0)
+        hipLaunchKernelGGL(k_ntc_wave, dim3(ncells), dim3(64), 0, h->stream, A);
+    else
+        hipLaunchKernelGGL(k_ntc, dim3((ncells + 63) / 64), dim3(64), 0, h->stream, A);
+}
+
 int mz_launch_ntc_equal(mz_store *h, int s1, int s2, double dt, const double *dV, double dw_tol, int equal_weight,
                         uint64_t tag, int rng_mode, int cell_lo, int cell_hi, const uint64_t *step_ptr) {
     NtcArgs A = base_args(h, s1, s2, dt, dV, dw_tol, equal_weight, tag, rng_mode, cell_lo, cell_hi);
@@ -331,7 +397,7 @@ int mz_launch_ntc_equal(mz_store *h, int s1, int s2, double dt, const double *dV
     A.rp.step_ptr = step_ptr;
     A.rp.phase = 1;
     int ncells = cell_hi - cell_lo + 1;
-    hipLaunchKernelGGL(k_ntc, dim3((ncells + 63) / 64), dim3(64), 0, h->stream, A);
+    launch_k_ntc(h, A, ncells);
     MZ_HIP(hipGetLastError());
     return 0;
 }
@@ -363,7 +429,7 @@ static int launch_ntc_split(mz_store *h, int s1, int s2, double dt, const double
     A.soff = soff;
     // created is indexed [species half][cell]; the kernel writes half 1 at cell_hi + 1 + c
     A.created = created;
-    hipLaunchKernelGGL(k_ntc, dim3((ncells + 63) / 64), dim3(64), 0, h->stream, A);
+    launch_k_ntc(h, A, ncells);
     MZ_HIP(hipGetLastError());
     // append per species half
     double **scp = (double **)mz_scratch(h, "ntc_scp", 7 * sizeof(double *));
