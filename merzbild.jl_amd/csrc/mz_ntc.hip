@@ -561,7 +561,7 @@ __global__ void __launch_bounds__(256) k_swpm_count(NtcArgs A, double G, int32_t
     wmax[c] = w_max;
 }
 
-__global__ void k_swpm(NtcArgs A, double G, const double *__restrict__ wmax) {
+__global__ void __launch_bounds__(64) k_swpm(NtcArgs A, double G, const double *__restrict__ wmax) {
     const int c = A.cell_lo + blockIdx.x * blockDim.x + threadIdx.x;
     if (c > A.cell_hi) return;
     A.rp = mz_rng_resolve(A.rp);
