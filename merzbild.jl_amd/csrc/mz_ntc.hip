@@ -68,114 +68,173 @@ __device__ inline PRef pref_scratch(const NtcArgs &A, int half, int64_t e) {
     return PRef{A.sc[0] + o, A.sc[1] + o, A.sc[2] + o, A.sc[3] + o, A.sc[4] + o, A.sc[5] + o, A.sc[6] + o, e};
 }
 
-// The candidate loop of cell c with the draws of its stream from dr (next(),
-// seek(k), exhausted()).
+// A cell's fixed values and the candidate loop's running state.
+struct NtcCell {
+    int32_t n1a, n2a, n1b, n2b, nla, nlb;
+    int64_t s_base, n_coll;
+};
+struct NtcState {
+    int32_t na, nb;       // current n_local (grows with splits)
+    int32_t ca, cb;       // created per species
+    int32_t performed, eqw;
+    double sgwm;
+};
+
+// Particle i of species half 0 (k of half 1) of cell c: a position present
+// before the call or this call's scratch entry.
+__device__ inline PRef ntc_ref_a(const NtcArgs &A, int c, const NtcCell &C, int32_t i) {
+    return i < C.nla ? pref_slot(A.a, slot_of(A.a, c, i, C.n1a, C.n2a)) : pref_scratch(A, 0, C.s_base + (i - C.nla));
+}
+__device__ inline PRef ntc_ref_b(const NtcArgs &A, int c, const NtcCell &C, int32_t k) {
+    return k < C.nlb ? pref_slot(A.b, slot_of(A.b, c, k, C.n1b, C.n2b)) : pref_scratch(A, 1, C.s_base + (k - C.nlb));
+}
+
+// A cell's setup: counts, (sigma g w)_max and n_coll (draw 0 of its stream,
+// or the count pass's value).
 template <class Draw>
-__device__ __forceinline__ void ntc_cell(const NtcArgs &A, const int c, Draw &dr) {
-    const int32_t n1a = A.a.pia.n1[c], n2a = A.a.pia.n2[c];
-    const int32_t n1b = A.b.pia.n1[c], n2b = A.b.pia.n2[c];
-    const int32_t nla = A.a.pia.nloc[c], nlb = A.b.pia.nloc[c];
-    double sgwm = A.sgwm[c];
+__device__ __forceinline__ void ntc_setup(const NtcArgs &A, const int c, Draw &dr, NtcCell &C, NtcState &S) {
+    C.n1a = A.a.pia.n1[c]; C.n2a = A.a.pia.n2[c];
+    C.n1b = A.b.pia.n1[c]; C.n2b = A.b.pia.n2[c];
+    C.nla = A.a.pia.nloc[c]; C.nlb = A.b.pia.nloc[c];
+    S.sgwm = A.sgwm[c];
     int64_t n_coll;
     if (A.ncoll_in) {
         n_coll = A.ncoll_in[c];
         dr.seek(1);
     } else {
         double R = dr.next();
-        n_coll = (int64_t)floor(mz_ncoll_est(A.single, A.dt, nla, nlb, sgwm, A.V[c], R));
+        n_coll = (int64_t)floor(mz_ncoll_est(A.single, A.dt, C.nla, C.nlb, S.sgwm, A.V[c], R));
     }
     if (!(n_coll <= MZ_NTC_MAX_CANDIDATES)) {   // also catches NaN estimates
         atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH);
         n_coll = 0;
     }
-    int32_t na = nla, nb = nlb;     // current n_local (grows with splits)
-    int32_t ca = 0, cb = 0;         // created per species
-    int32_t performed = 0, eqw = 0;
-    const int64_t s_base = A.may_split ? A.soff[c] : 0;
-    for (int64_t cc = 0; cc < n_coll; ++cc) {
-        int32_t i = (int32_t)floor(dr.next() * (double)na);
-        int32_t k = (int32_t)floor(dr.next() * (double)(A.single ? na : nb));
-        if (A.single) {
-            while (i == k) {
-                k = (int32_t)floor(dr.next() * (double)na);
-                if (dr.exhausted()) return;   // replay row exhausted
-            }
-        }
-        // resolve both candidates: existing positions or this call's scratch entries
-        PRef pi = i < nla ? pref_slot(A.a, slot_of(A.a, c, i, n1a, n2a)) : pref_scratch(A, 0, s_base + (i - nla));
-        PRef pk;
-        if (A.single) {
-            pk = k < nla ? pref_slot(A.a, slot_of(A.a, c, k, n1a, n2a)) : pref_scratch(A, 0, s_base + (k - nla));
-        } else {
-            pk = k < nlb ? pref_slot(A.b, slot_of(A.b, c, k, n1b, n2b)) : pref_scratch(A, 1, s_base + (k - nlb));
-        }
-        double vi0 = pi.v0[pi.i], vi1 = pi.v1[pi.i], vi2 = pi.v2[pi.i];
-        double vk0 = pk.v0[pk.i], vk1 = pk.v1[pk.i], vk2 = pk.v2[pk.i];
-        double g0, g1, g2;
-        double g = mz_rel_speed(vi0, vi1, vi2, vk0, vk1, vk2, g0, g1, g2);
-        if (!(g > MZ_G_EPS)) continue;
-        double wi = pi.w[pi.i], wk = pk.w[pk.i];
-        double sgw = mz_vhs_sgw(A.factor, A.expo, g, wi, wk);
-        sgwm = sgw > sgwm ? sgw : sgwm;
-        if (!(dr.next() < sgw / sgwm)) continue;
-        ++performed;
-        double vc0, vc1, vc2;
-        mz_com(A.mu1, A.mu2, vi0, vi1, vi2, vk0, vk1, vk2, vc0, vc1, vc2);
-        if (A.equal_weight || fabs(wi - wk) < A.dw_tol) {
-            ++eqw;
-        } else if (!A.may_split) {
-            atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH);
-        } else if (wi > wk) {
-            // split i: the new particle keeps i's pre-collision v, x and weight wi - wk
-            int32_t q = A.single ? ca + cb : ca;
-            if (q >= n_coll) { atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH); continue; }
-            PRef nw = pref_scratch(A, 0, s_base + q);
-            nw.x0[nw.i] = pi.x0[pi.i]; nw.x1[nw.i] = pi.x1[pi.i]; nw.x2[nw.i] = pi.x2[pi.i];
-            nw.v0[nw.i] = vi0; nw.v1[nw.i] = vi1; nw.v2[nw.i] = vi2;
-            nw.w[nw.i] = wi - wk;
-            pi.w[pi.i] = wk;
-            ++ca; ++na;
-        } else {
-            if (A.single) {
-                int32_t q = ca + cb;
-                if (q >= n_coll) { atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH); continue; }
-                PRef nw = pref_scratch(A, 0, s_base + q);
-                nw.x0[nw.i] = pk.x0[pk.i]; nw.x1[nw.i] = pk.x1[pk.i]; nw.x2[nw.i] = pk.x2[pk.i];
-                nw.v0[nw.i] = vk0; nw.v1[nw.i] = vk1; nw.v2[nw.i] = vk2;
-                nw.w[nw.i] = wk - wi;
-                pk.w[pk.i] = wi;
-                ++ca; ++na;
-            } else {
-                if (cb >= n_coll) { atomicOr(&A.b.ctr->err, MZ_ERR_SCRATCH); continue; }
-                PRef nw = pref_scratch(A, 1, s_base + cb);
-                nw.x0[nw.i] = pk.x0[pk.i]; nw.x1[nw.i] = pk.x1[pk.i]; nw.x2[nw.i] = pk.x2[pk.i];
-                nw.v0[nw.i] = vk0; nw.v1[nw.i] = vk1; nw.v2[nw.i] = vk2;
-                nw.w[nw.i] = wk - wi;
-                pk.w[pk.i] = wi;
-                ++cb; ++nb;
-            }
-        }
-        // scatter_vhs!: phi first, then cos(theta)
-        double ui0, ui1, ui2, uk0, uk1, uk2;
-        const double R_phi = dr.next();
-        const double R_ct = dr.next();
-        mz_vhs_post(A.mu1, A.mu2, g, vc0, vc1, vc2, R_phi, R_ct, ui0, ui1, ui2, uk0, uk1, uk2);
-        pi.v0[pi.i] = ui0; pi.v1[pi.i] = ui1; pi.v2[pi.i] = ui2;
-        pk.v0[pk.i] = uk0; pk.v1[pk.i] = uk1; pk.v2[pk.i] = uk2;
+    C.n_coll = n_coll;
+    S.na = C.nla; S.nb = C.nlb;
+    S.ca = 0; S.cb = 0;
+    S.performed = 0; S.eqw = 0;
+    C.s_base = A.may_split ? A.soff[c] : 0;
+}
+
+// Whether this call's scratch has room for the split a collision of the
+// pair makes (always true when it makes none).  Without room the scratch
+// error is raised and the candidate ends before its scattering draws, as a
+// performed collision.
+__device__ inline bool ntc_split_room(const NtcArgs &A, const NtcCell &C, const NtcState &S, double wi, double wk) {
+    if (A.equal_weight || fabs(wi - wk) < A.dw_tol || !A.may_split) return true;
+    const bool on_a = wi > wk || A.single;
+    const int32_t q = on_a ? (A.single ? S.ca + S.cb : S.ca) : S.cb;
+    if (q < C.n_coll) return true;
+    atomicOr(&(on_a ? A.a : A.b).ctr->err, MZ_ERR_SCRATCH);
+    return false;
+}
+
+// The collision of a pair that passed the acceptance test, with room for
+// its split (collide_2particles_vhs! with weight splitting,
+// collision_ntc.jl:223-270): the split (the heavier particle leaves a copy of
+// itself with the weight difference and takes the lighter's weight), then
+// the post-collision velocities u.
+__device__ __forceinline__ void ntc_collide(const NtcArgs &A, const NtcCell &C, NtcState &S, const PRef &pi,
+                                            const PRef &pk, double vi0, double vi1, double vi2, double vk0,
+                                            double vk1, double vk2, double wi, double wk, double ui0, double ui1,
+                                            double ui2, double uk0, double uk1, double uk2) {
+    ++S.performed;
+    if (A.equal_weight || fabs(wi - wk) < A.dw_tol) {
+        ++S.eqw;
+    } else if (!A.may_split) {
+        atomicOr(&A.a.ctr->err, MZ_ERR_SCRATCH);
+    } else if (wi > wk) {
+        PRef nw = pref_scratch(A, 0, C.s_base + (A.single ? S.ca + S.cb : S.ca));
+        nw.x0[nw.i] = pi.x0[pi.i]; nw.x1[nw.i] = pi.x1[pi.i]; nw.x2[nw.i] = pi.x2[pi.i];
+        nw.v0[nw.i] = vi0; nw.v1[nw.i] = vi1; nw.v2[nw.i] = vi2;
+        nw.w[nw.i] = wi - wk;
+        pi.w[pi.i] = wk;
+        ++S.ca; ++S.na;
+    } else if (A.single) {
+        PRef nw = pref_scratch(A, 0, C.s_base + S.ca + S.cb);
+        nw.x0[nw.i] = pk.x0[pk.i]; nw.x1[nw.i] = pk.x1[pk.i]; nw.x2[nw.i] = pk.x2[pk.i];
+        nw.v0[nw.i] = vk0; nw.v1[nw.i] = vk1; nw.v2[nw.i] = vk2;
+        nw.w[nw.i] = wk - wi;
+        pk.w[pk.i] = wi;
+        ++S.ca; ++S.na;
+    } else {
+        PRef nw = pref_scratch(A, 1, C.s_base + S.cb);
+        nw.x0[nw.i] = pk.x0[pk.i]; nw.x1[nw.i] = pk.x1[pk.i]; nw.x2[nw.i] = pk.x2[pk.i];
+        nw.v0[nw.i] = vk0; nw.v1[nw.i] = vk1; nw.v2[nw.i] = vk2;
+        nw.w[nw.i] = wk - wi;
+        pk.w[pk.i] = wi;
+        ++S.cb; ++S.nb;
     }
-    A.sgwm[c] = sgwm;
+    pi.v0[pi.i] = ui0; pi.v1[pi.i] = ui1; pi.v2[pi.i] = ui2;
+    pk.v0[pk.i] = uk0; pk.v1[pk.i] = uk1; pk.v2[pk.i] = uk2;
+}
+
+// One candidate of cell c's loop (collision_ntc.jl:338-380); false when a
+// replay row ran out.
+template <class Draw>
+__device__ __forceinline__ bool ntc_candidate(const NtcArgs &A, const int c, const NtcCell &C, NtcState &S,
+                                              Draw &dr) {
+    int32_t i = (int32_t)floor(dr.next() * (double)S.na);
+    int32_t k = (int32_t)floor(dr.next() * (double)(A.single ? S.na : S.nb));
+    if (A.single) {
+        while (i == k) {
+            k = (int32_t)floor(dr.next() * (double)S.na);
+            if (dr.exhausted()) return false;   // replay row exhausted
+        }
+    }
+    const PRef pi = ntc_ref_a(A, c, C, i);
+    const PRef pk = A.single ? ntc_ref_a(A, c, C, k) : ntc_ref_b(A, c, C, k);
+    const double vi0 = pi.v0[pi.i], vi1 = pi.v1[pi.i], vi2 = pi.v2[pi.i];
+    const double vk0 = pk.v0[pk.i], vk1 = pk.v1[pk.i], vk2 = pk.v2[pk.i];
+    double g0, g1, g2;
+    const double g = mz_rel_speed(vi0, vi1, vi2, vk0, vk1, vk2, g0, g1, g2);
+    if (!(g > MZ_G_EPS)) return true;
+    const double wi = pi.w[pi.i], wk = pk.w[pk.i];
+    const double sgw = mz_vhs_sgw(A.factor, A.expo, g, wi, wk);
+    S.sgwm = sgw > S.sgwm ? sgw : S.sgwm;
+    if (!(dr.next() < sgw / S.sgwm)) return true;
+    if (!ntc_split_room(A, C, S, wi, wk)) {
+        ++S.performed;
+        return true;
+    }
+    double vc0, vc1, vc2;
+    mz_com(A.mu1, A.mu2, vi0, vi1, vi2, vk0, vk1, vk2, vc0, vc1, vc2);
+    // scatter_vhs!: phi first, then cos(theta)
+    const double R_phi = dr.next();
+    const double R_ct = dr.next();
+    double ui0, ui1, ui2, uk0, uk1, uk2;
+    mz_vhs_post(A.mu1, A.mu2, g, vc0, vc1, vc2, R_phi, R_ct, ui0, ui1, ui2, uk0, uk1, uk2);
+    ntc_collide(A, C, S, pi, pk, vi0, vi1, vi2, vk0, vk1, vk2, wi, wk, ui0, ui1, ui2, uk0, uk1, uk2);
+    return true;
+}
+
+// The end of a cell's loop: (sigma g w)_max, the counters, created counts.
+__device__ __forceinline__ void ntc_finish_cell(const NtcArgs &A, const int c, const NtcCell &C,
+                                                const NtcState &S) {
+    A.sgwm[c] = S.sgwm;
     int32_t *cn = A.cnt + (int64_t)c * 5;
-    cn[0] = nla;
-    cn[1] = A.single ? nla : nlb;
-    cn[2] = (int32_t)n_coll;
-    cn[3] = performed;
-    cn[4] = eqw;
+    cn[0] = C.nla;
+    cn[1] = A.single ? C.nla : C.nlb;
+    cn[2] = (int32_t)C.n_coll;
+    cn[3] = S.performed;
+    cn[4] = S.eqw;
     if (A.may_split) {
-        A.created[c] = A.single ? ca + cb : ca;
-        A.created[(int64_t)(A.cell_hi + 1) + c] = A.single ? 0 : cb;
+        A.created[c] = A.single ? S.ca + S.cb : S.ca;
+        A.created[(int64_t)(A.cell_hi + 1) + c] = A.single ? 0 : S.cb;
     }
 }
 
+// The candidate loop of cell c with the draws of its stream from dr (next(),
+// seek(k), exhausted()).
+template <class Draw>
+__device__ __forceinline__ void ntc_cell(const NtcArgs &A, const int c, Draw &dr) {
+    NtcCell C;
+    NtcState S;
+    ntc_setup(A, c, dr, C, S);
+    for (int64_t cc = 0; cc < C.n_coll; ++cc)
+        if (!ntc_candidate(A, c, C, S, dr)) return;
+    ntc_finish_cell(A, c, C, S);
+}
 
 // Draws of a cell's stream for one lane: mz_rand.
 struct NtcLaneDraw {
@@ -207,7 +266,128 @@ struct NtcWaveDraw {
     }
     __device__ bool exhausted() const { return false; }
     __device__ void seek(uint64_t kk) { k = kk; }
+    // the block holds draws D .. D + 67 afterwards
+    __device__ void cover(uint64_t D) {
+        if (D < base || D + 68 > base + 128) {
+            base = D & ~(uint64_t)1;
+            mz_philox_pair(rp, stream, (base >> 1) + (uint64_t)(threadIdx.x & 63), h0, h1);
+        }
+    }
+    // draw s for this lane (s may differ between lanes; base <= s < base + 128)
+    __device__ double at(uint64_t s) const {
+        const uint64_t idx = s - base;
+        const int l = (int)(idx >> 1);
+        const double a = __shfl(h0, l, 64), b = __shfl(h1, l, 64);
+        return (idx & 1) ? b : a;
+    }
 };
+
+// ntc_cell for a single-species cell on a whole wavefront with Philox
+// draws, with the candidates evaluated ahead: at a window's start at draw D,
+// lane L evaluates the candidate that would start at draw D + L (pair,
+// relative speed, (sigma g w), the acceptance test under the window's
+// (sigma g w)_max, post-collision velocities); the wave then walks the
+// candidates in the reference's order on uniform values.  Each starts where
+// the previous one ended, (sigma g w)_max is updated before its acceptance
+// test, and a lane's test is taken only while (sigma g w)_max is unchanged
+// (the same operands, so the same result).  A candidate whose k is redrawn,
+// or that shares a particle with a collision earlier in the window, is
+// evaluated directly and a new window starts after it; so does a split,
+// which changes n_local and with it every later pair.  The results are
+// ntc_cell's.
+__device__ void ntc_cell_spec(const NtcArgs &A, const int c, NtcWaveDraw &dr) {
+    const int lane = threadIdx.x & 63;
+    NtcCell C;
+    NtcState S;
+    ntc_setup(A, c, dr, C, S);
+    int64_t cc = 0;
+    uint64_t o = dr.k;   // the next draw of the stream
+    while (cc < C.n_coll) {
+        const uint64_t D = o;
+        dr.cover(D);
+        const int32_t na0 = S.na;
+        const double sgwm0 = S.sgwm;
+        const uint64_t sL = D + (uint64_t)lane;
+        const double d0 = dr.at(sL), d1 = dr.at(sL + 1), d2 = dr.at(sL + 2), d3 = dr.at(sL + 3), d4 = dr.at(sL + 4);
+        const int32_t si = (int32_t)floor(d0 * (double)na0), sk = (int32_t)floor(d1 * (double)na0);
+        double swi, swk, sgw, vi0, vi1, vi2, vk0, vk1, vk2, ui0, ui1, ui2, uk0, uk1, uk2;
+        int32_t f;
+        {
+            const PRef pi = ntc_ref_a(A, c, C, si), pk = ntc_ref_a(A, c, C, sk);
+            vi0 = pi.v0[pi.i]; vi1 = pi.v1[pi.i]; vi2 = pi.v2[pi.i];
+            vk0 = pk.v0[pk.i]; vk1 = pk.v1[pk.i]; vk2 = pk.v2[pk.i];
+            swi = pi.w[pi.i];
+            swk = pk.w[pk.i];
+            double g0, g1, g2;
+            const double g = mz_rel_speed(vi0, vi1, vi2, vk0, vk1, vk2, g0, g1, g2);
+            sgw = mz_vhs_sgw(A.factor, A.expo, g, swi, swk);
+            f = (si == sk ? 1 : 0) | (g > MZ_G_EPS ? 2 : 0) | (sgw > sgwm0 ? 4 : 0) | (d2 < sgw / sgwm0 ? 8 : 0);
+            double vc0, vc1, vc2;
+            mz_com(A.mu1, A.mu2, vi0, vi1, vi2, vk0, vk1, vk2, vc0, vc1, vc2);
+            mz_vhs_post(A.mu1, A.mu2, g, vc0, vc1, vc2, d3, d4, ui0, ui1, ui2, uk0, uk1, uk2);
+        }
+        // walk the window; lane j < nd holds a particle that collided in it
+        int32_t dirty = -1;
+        int nd = 0;
+        bool direct = false;
+        while (cc < C.n_coll) {
+            const int L = (int)(o - D);
+            if (L >= 60 || nd >= 62) break;
+            const int32_t i = __builtin_amdgcn_readlane(si, L), k = __builtin_amdgcn_readlane(sk, L);
+            const int32_t fl = __builtin_amdgcn_readlane(f, L);
+            if ((fl & 1) || __ballot(lane < nd && (dirty == i || dirty == k))) {
+                direct = true;
+                break;
+            }
+            if (!(fl & 2)) {   // g <= eps: no acceptance draw
+                o += 2;
+                ++cc;
+                continue;
+            }
+            bool acc;
+            if (!(fl & 4) && S.sgwm == sgwm0) {
+                acc = (fl & 8) != 0;
+            } else {
+                const double sl = mz_readlane_d(sgw, L);
+                S.sgwm = sl > S.sgwm ? sl : S.sgwm;
+                acc = mz_readlane_d(d2, L) < sl / S.sgwm;
+            }
+            if (!acc) {
+                o += 3;
+                ++cc;
+                continue;
+            }
+            const double wi = mz_readlane_d(swi, L), wk = mz_readlane_d(swk, L);
+            if (!ntc_split_room(A, C, S, wi, wk)) {
+                ++S.performed;
+                o += 3;
+                ++cc;
+                continue;
+            }
+            const int32_t na_before = S.na;
+            ntc_collide(A, C, S, ntc_ref_a(A, c, C, i), ntc_ref_a(A, c, C, k), mz_readlane_d(vi0, L),
+                        mz_readlane_d(vi1, L), mz_readlane_d(vi2, L), mz_readlane_d(vk0, L), mz_readlane_d(vk1, L),
+                        mz_readlane_d(vk2, L), wi, wk, mz_readlane_d(ui0, L), mz_readlane_d(ui1, L),
+                        mz_readlane_d(ui2, L), mz_readlane_d(uk0, L), mz_readlane_d(uk1, L), mz_readlane_d(uk2, L));
+            if (lane == nd) dirty = i;
+            if (lane == nd + 1) dirty = k;
+            nd += 2;
+            o += 5;
+            ++cc;
+            if (S.na != na_before) break;   // a split changed n_local
+        }
+        // the stores of the window before the loads that follow
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        if (direct) {
+            dr.seek(o);
+            ntc_candidate(A, c, C, S, dr);   // Philox draws: never exhausted
+            o = dr.k;
+            ++cc;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        }
+    }
+    ntc_finish_cell(A, c, C, S);
+}
 
 // One lane per cell; cells run in parallel (64-lane workgroups, so the
 // loop keeps its state in registers).
@@ -219,16 +399,18 @@ __global__ void __launch_bounds__(64) k_ntc(NtcArgs A) {
     ntc_cell(A, c, dr);
 }
 
-// One wavefront per cell (Philox draws): every lane runs the cell's loop on
-// the same values, so there is no divergence between cells and the draws
-// come from registers; for calls over few cells, which one lane per cell
-// would spread over a handful of wavefronts.
+// One wavefront per cell (Philox draws): single-species cells take their
+// candidates a window at a time (ntc_cell_spec), two-species cells run the
+// loop on every lane on the same values; the draws come from registers.  For
+// calls over few cells, which one lane per cell would spread over a handful
+// of wavefronts.
 __global__ void __launch_bounds__(64) k_ntc_wave(NtcArgs A) {
     const int c = A.cell_lo + blockIdx.x;
     if (c > A.cell_hi) return;
     A.rp = mz_rng_resolve(A.rp);
     NtcWaveDraw dr{A.rp, (uint64_t)c, 0, ~(uint64_t)0, 0.0, 0.0};
-    ntc_cell(A, c, dr);
+    if (A.single) ntc_cell_spec(A, c, dr);
+    else ntc_cell(A, c, dr);
 }
 
 // Count pass of the split path: n_coll of every cell (draw 0 of its stream).
