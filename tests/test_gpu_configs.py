@@ -32,13 +32,16 @@ def _interactions(hip, st, its):
                                              vhs_o=its[i][k].vhs_o))() for k in range(n)] for i in range(n)])
 
 
-def _batched_0d(hip, mzo, nc, nmax, seed=7, cap_mult=2, centred=False):
-    """Cells of log-uniform occupancy in [20, nmax), BKW velocities at 273 K,
-    w ~ U(0.5, 2) Fnum; the device store and the oracle's state.  centred:
-    every particle at its cell's centre (a merge's new positions, mean +- the
-    standard deviation of its bin's positions, then stay in the cell)."""
+def _batched_0d(hip, mzo, nc, nmax, seed=7, cap_mult=2, centred=False, counts=None):
+    """Cells of log-uniform occupancy in [20, nmax) (or the given counts),
+    BKW velocities at 273 K, w ~ U(0.5, 2) Fnum; the device store and the
+    oracle's state.  centred: every particle at its cell's centre (a merge's
+    new positions, mean +- the standard deviation of its bin's positions,
+    then stay in the cell)."""
     rng = np.random.default_rng(seed)
-    counts = np.floor(np.exp(rng.uniform(math.log(20), math.log(nmax), size=nc))).astype(int)
+    if counts is None:
+        counts = np.floor(np.exp(rng.uniform(math.log(20), math.log(nmax), size=nc))).astype(int)
+    counts = np.asarray(counts, dtype=int)
     n = int(counts.sum())
     T0, m = 273.0, ac.M_AR
     grid = mzo.grid1d(float(nc), nc)              # one unit-length cell per 0-D cell, V = 1
@@ -106,6 +109,33 @@ def test_batched_0d_cells_with_splitting(hip, mzo):
     assert np.allclose(pp.n, ref.n, rtol=1e-12) and np.allclose(pp.T, ref.T, rtol=1e-10)
     # weight splitting conserves each cell's total weight (up to rounding)
     assert np.allclose(pp.n[0], n_cell0, rtol=1e-12)
+    st.close()
+
+
+def test_batched_0d_many_cells_splitting(hip, mzo):
+    """ntc! with splitting over more than 4,096 cells (one lane per cell;
+    4,100 cells of 20 to 40 particles and four of 8,000 to 16,000, whose
+    candidate loops run hundreds of candidates): three steps of NTC and the
+    sort against the oracle, index and PIA bit-exact, values to 1e-11."""
+    rng = np.random.default_rng(3)
+    counts = list(rng.integers(20, 41, size=4100)) + [8000, 12000, 16000, 9000]
+    nc = len(counts)
+    b = _batched_0d(hip, mzo, nc, 0, seed=5, counts=counts)
+    st, pv, pia, cfs, it, grid, dt = (b[k] for k in ("st", "pv", "pia", "cfs", "it", "grid", "dt"))
+    V = np.ones(nc)
+    for step in range(1, 4):
+        tag = (step << 4) | 1
+        hip.ntc(st, 1, 1, dt, V=V, tag=tag)
+        rr = mzo.RNG(seed=7, tag=tag)
+        for c in range(1, nc + 1):
+            mzo.lib().mzo_ntc_single(rr.ref, C.byref(cfs[c - 1]), C.byref(it), pv.p, pia.p, c, 1, dt, 1.0, 1e-16, 0)
+        got = st.download_species(1)
+        assert got["err"] == 0
+        helpers.assert_state_matches(mzo, got, pv, pia, rtol=1e-11)
+        hip.sort_particles(st, 1)
+        mzo.lib().mzo_sort_particles(C.byref(grid), pv.p, pia.p, 1)
+    cf = st.collision_factors(1, 1)
+    assert max(cf["n_coll"][-4:]) >= 256
     st.close()
 
 
