@@ -31,7 +31,7 @@
 
 #define OCT_MAX_N 2048
 #define OCT_MAX_BINS 512
-#define OCT_BIG_BLOCKS 1024   // 64-bin blocks whose maxima the device-memory path keeps in LDS
+#define OCT_BIG_BLOCKS 4096   // most 64-bin blocks whose maxima the device-memory path keeps in LDS
 
 struct OctArgs {
     MzSpView sp;
@@ -45,7 +45,10 @@ struct OctArgs {
     const uint8_t *mask;      // cells to merge (nullptr: every cell above the threshold)
     const int32_t *thresholds, *targets;   // per cell (nullptr: threshold, target_np for every cell)
     // cells handed to the device-memory path (big_list nullptr: such a cell
-    // raises MZ_ERR_MERGE_SIZE); big_info[0] counts them, [1] is their largest N
+    // raises MZ_ERR_MERGE_SIZE): four lists of n_cells entries, one per size
+    // class (more than 16384, 8192, 4096 particles, the rest); big_info[0]
+    // counts the listed cells, [1] is their largest N, [2..5] count each
+    // class, [6] is the device-memory kernel's queue
     int32_t *big_list, *big_info;
     // that path's arrays, one slice of big_ncap particles and big_bcap bins per workgroup
     int32_t *big_pis, *big_psort, *big_bslot;
@@ -53,6 +56,7 @@ struct OctArgs {
     double *big_bkey;
     void *big_bins;
     int32_t big_ncap, big_bcap;
+    int32_t big_blocks;       // 1: the 64-bin blocks' maxima are kept in dynamic LDS
 };
 
 // A bin of the octree; IdxT holds particle counts and positions (int16_t in
@@ -92,7 +96,9 @@ struct OctShared {
     double ndens8[9];
     int32_t Nbins_s, total_post_s, stop_s, defer_s;
     int64_t wsc[4];
-    int32_t n_new_s;
+    int32_t n_new_s, ticket_s;
+    double ckey[8];           // a split's children: key and bin id, by child
+    int32_t cbi[8];
 };
 
 // orders LDS accesses among the lanes of one wave (a barrier for code that a
@@ -199,6 +205,53 @@ __device__ void merge_finish(const MzSpView &sp, const MzSpView &tmp, int c, int
 // waves per workgroup.  With BIG false a cell that would need more bins than
 // W holds is listed for the device-memory path (or raises
 // MZ_ERR_MERGE_SIZE when there is none) and left unchanged.
+// Copies particle src to slot dst of the second buffer.
+__device__ inline void oct_copy(const OctArgs &A, int64_t src, int64_t dst) {
+    A.tmp.x0[dst] = A.sp.x0[src]; A.tmp.x1[dst] = A.sp.x1[src]; A.tmp.x2[dst] = A.sp.x2[src];
+    A.tmp.v0[dst] = A.sp.v0[src]; A.tmp.v1[dst] = A.sp.v1[src]; A.tmp.v2[dst] = A.sp.v2[src];
+    A.tmp.w[dst] = A.sp.w[src];
+}
+
+// compute_new_particles! for one bin (merging_octree_N2.jl:736-813): its two
+// particles at slots slot, slot + 1 of the second buffer, the 3 + 3 sign
+// draws at the bin's place (draw) in the cell's sequence.
+__device__ inline void oct_emit(const OctArgs &A, int c, int32_t draw, int slot, double Wb, const double *vm,
+                                const double *xm, const double *sdv, const double *sdx, int32_t s1, int32_t n1,
+                                int32_t s2) {
+    MzStream st = mz_stream((uint64_t)c);
+    st.k = 6 * (uint64_t)draw;
+    double dv[3], dx[3];
+    for (int d = 0; d < 3; ++d) dv[d] = mz_rand(A.rp, st) < 0.5 ? -1.0 : 1.0;
+    for (int d = 0; d < 3; ++d) dx[d] = mz_rand(A.rp, st) < 0.5 ? -1.0 : 1.0;
+    const double w1 = 0.5 * Wb;
+    for (int k = 0; k < 2; ++k) {
+        const double sg = k == 0 ? 1.0 : -1.0;
+        const int64_t dst = oct_slot(slot + k, s1, n1, s2);
+        double x0 = xm[0] + sg * dx[0] * sdx[0];
+        if (A.grid_clamp) x0 = x0 < A.min_x ? A.min_x : (x0 > A.max_x ? A.max_x : x0);
+        A.tmp.x0[dst] = x0;
+        A.tmp.x1[dst] = xm[1] + sg * dx[1] * sdx[1];
+        A.tmp.x2[dst] = xm[2] + sg * dx[2] * sdx[2];
+        A.tmp.v0[dst] = vm[0] + sg * dv[0] * sdv[0];
+        A.tmp.v1[dst] = vm[1] + sg * dv[1] * sdv[1];
+        A.tmp.v2[dst] = vm[2] + sg * dv[2] * sdv[2];
+        A.tmp.w[dst] = w1;
+    }
+}
+
+// Lists cell c (N particles) for the device-memory path, in its size class.
+__device__ inline void oct_defer(const OctArgs &A, int c, int32_t N) {
+    A.old_pia[c * 5 + 4] = -1;
+    if (A.big_list) {
+        const int cls = N > 16384 ? 0 : (N > 8192 ? 1 : (N > 4096 ? 2 : 3));
+        A.big_list[(size_t)cls * A.n_cells + atomicAdd(&A.big_info[2 + cls], 1)] = c;
+        atomicAdd(&A.big_info[0], 1);
+        atomicMax(&A.big_info[1], N);
+    } else {
+        atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
+    }
+}
+
 template <class IdxT, int NW, bool BIG>
 __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, const int32_t N,
                                                const int32_t target_np, const OctWork<IdxT> &W, OctShared &S) {
@@ -283,6 +336,10 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
             if (bid < 0 || total_post_s + 14 > target_np) break;
             // split_bin! (merging_octree_N2.jl:503-633)
             const int b = bid;
+            // the keys of b's block, loaded with bin b: the block's maximum is
+            // rebuilt from them after the split
+            double k0key = -2.0;
+            if (W.bmv && (b & ~63) + lane < Nbins_s) k0key = bkey[(b & ~63) + lane];
             const int bs = bins[b].start, be = bins[b].end;
             const int depth = bins[b].depth;
             if (A.bounds_mode == 2) {
@@ -302,9 +359,12 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
                     for (int d = 0; d < 3; ++d) { bins[b].vmin[d] = bmn[d]; bins[b].vmax[d] = bmx[d]; }
                 oct_wave_sync();
             }
+            // the parent's bounds, read once by every lane
+            double pmin[3], pmax[3];
+            for (int d = 0; d < 3; ++d) { pmin[d] = bins[b].vmin[d]; pmax[d] = bins[b].vmax[d]; }
             if (lane == 0) {
                 if (A.split == 1) {
-                    for (int d = 0; d < 3; ++d) vel_mid[d] = 0.5 * (bins[b].vmin[d] + bins[b].vmax[d]);
+                    for (int d = 0; d < 3; ++d) vel_mid[d] = 0.5 * (pmin[d] + pmax[d]);
                 } else {
                     // compute_v_mean! keeps the previous middle velocity in the sum
                     double nt = 0.0;
@@ -331,26 +391,37 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
             const double mid0 = vel_mid[0], mid1 = vel_mid[1], mid2 = vel_mid[2];
             // the next chunk's particle loads are issued before this chunk's
             // ballots and sums, so their latency overlaps that work
+            // A bin of at most 64 particles is one chunk: its ids and octants
+            // stay in registers and the scatter writes pis directly.
+            const bool one = be - bs < 64;
             double nw = 0.0, nv0 = 0.0, nv1 = 0.0, nv2 = 0.0;
+            IdxT pi1 = 0;
             if (bs + lane <= be) {
-                const int64_t p = oct_slot(pis[bs + lane], s1, n1, s2);
+                pi1 = pis[bs + lane];
+                const int64_t p = oct_slot(pi1, s1, n1, s2);
                 nw = A.sp.w[p]; nv0 = A.sp.v0[p]; nv1 = A.sp.v1[p]; nv2 = A.sp.v2[p];
             }
+            // the next chunk's ids, loaded a chunk ahead of its values
+            IdxT pin = 0;
+            if (bs + 64 + lane <= be) pin = pis[bs + 64 + lane];
+            int o1 = 0;
             for (int i0 = bs; i0 <= be; i0 += 64) {
                 const int i = i0 + lane;
                 const double wi = nw, c0 = nv0, c1 = nv1, c2 = nv2;
                 if (i + 64 <= be) {
-                    const int64_t p = oct_slot(pis[i + 64], s1, n1, s2);
+                    const int64_t p = oct_slot(pin, s1, n1, s2);
                     nw = A.sp.w[p]; nv0 = A.sp.v0[p]; nv1 = A.sp.v1[p]; nv2 = A.sp.v2[p];
                 }
+                if (i + 128 <= be) pin = pis[i + 128];
                 int o = 0;
                 if (i <= be) {
                     o = 1;
                     if (c0 > mid0) o += 1;
                     if (c1 > mid1) o += 2;
                     if (c2 > mid2) o += 4;
-                    poct[i - bs] = (uint8_t)o;
+                    if (!one) poct[i - bs] = (uint8_t)o;
                 }
+                o1 = o;
                 uint64_t mq = 0;   // lane q: the chunk's lanes in octant q
 #pragma unroll
                 for (int q = 1; q <= 8; ++q) {
@@ -393,13 +464,30 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
             // child holds its particles in reverse parent order: a particle's
             // place is its octant's end less the octant's particles before it
             // (earlier chunks, then earlier lanes of its chunk).
-            {
+            if (one) {
+                const uint64_t lt = mz_lanemask_lt();
+                int dst = 0;
+#pragma unroll
+                for (int q = 1; q <= 8; ++q) {
+                    const uint64_t m = __ballot(o1 == q);
+                    if (o1 == q) dst = sq[q] + cq[q] - 1 - __popcll(m & lt);
+                }
+                if (bs + lane <= be) pis[dst] = pi1;
+                oct_wave_sync();
+            } else {
                 const uint64_t lt = mz_lanemask_lt();
                 int rq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                // each chunk's octants and ids are loaded a chunk ahead
+                int on = bs + lane <= be ? poct[lane] : 0;
+                IdxT pn = bs + lane <= be ? pis[bs + lane] : (IdxT)0;
                 for (int i0 = bs; i0 <= be; i0 += 64) {
                     const int i = i0 + lane;
                     const bool valid = i <= be;
-                    const int o = valid ? poct[i - bs] : 0;
+                    const int o = on;
+                    const IdxT pv = pn;
+                    const bool vn = i + 64 <= be;
+                    on = vn ? poct[i + 64 - bs] : 0;
+                    pn = vn ? pis[i + 64] : (IdxT)0;
                     int dst = 0;
 #pragma unroll
                     for (int q = 1; q <= 8; ++q) {
@@ -407,23 +495,32 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
                         if (o == q) dst = sq[q] + cq[q] - 1 - (rq[q] + __popcll(m & lt));
                         rq[q] += __popcll(m);
                     }
-                    if (valid) psort[dst] = pis[i];
+                    if (valid) psort[dst] = pv;
                 }
+                oct_wave_sync();
+                for (int i = bs + lane; i <= be; i += 256) {
+                    IdxT t[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) t[u] = i + 64 * u <= be ? psort[i + 64 * u] : (IdxT)0;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (i + 64 * u <= be) pis[i + 64 * u] = t[u];
+                }
+                oct_wave_sync();
             }
-            oct_wave_sync();
-            for (int i = bs + lane; i <= be; i += 64) pis[i] = psort[i];
-            oct_wave_sync();
             // the children (split_bin!, :588-633), lane o for octant o: the
             // k-th non-empty octant becomes bin b (k = 1) or Nb + k - 2
             // (get_new_bin_id), its particles the octant's segment
             const int Nb = Nbins_s;
+            // a new bin's key and id on its octant's lane (for the blocks' maxima)
+            double nkey = -2.0;
+            int nbi = 0x7FFFFFFF;
+            int n_ne = 0;
             {
-                double pmin[3], pmax[3];
-                for (int d = 0; d < 3; ++d) { pmin[d] = bins[b].vmin[d]; pmax[d] = bins[b].vmax[d]; }
                 const int o = lane;
                 const bool ne = o >= 1 && o <= 8 && cnt8[o] > 0;
                 const uint64_t m = __ballot(ne);
-                const int n_ne = __popcll(m);
+                n_ne = __popcll(m);
                 int contrib = 0;
                 oct_wave_sync();   // every lane has read bin b before child 1 overwrites it
                 if (ne) {
@@ -443,7 +540,11 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
                         nb.w = ndens8[o];
                         nb.depth = depth + 1;
                         nb.refine = (cnt8[o] > 2) && (depth + 1 < A.max_depth);
-                        bkey[bi] = nb.refine ? nb.w : -1.0;
+                        const double key = nb.refine ? nb.w : -1.0;
+                        bkey[bi] = key;
+                        S.ckey[k - 1] = key;
+                        S.cbi[k - 1] = bi;
+                        if (k > 1) { nkey = key; nbi = bi; }
                     }
                     contrib = cnt8[o] >= 2 ? 2 : cnt8[o];
                 }
@@ -466,17 +567,36 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
                 // the keys of bin b and of the new bins Nb .. Nbins - 1 changed:
                 // their blocks' maxima again (at most three blocks)
                 const int nb_now = Nbins_s;
-                const int k0 = b >> 6, k1 = nb_now > Nb ? Nb >> 6 : k0, k2 = nb_now > Nb ? (nb_now - 1) >> 6 : k0;
-                for (int k = k1; k <= k2 + 1; ++k) {
-                    const int blk = k == k2 + 1 ? k0 : k;
-                    if (k == k2 + 1 && k0 >= k1 && k0 <= k2) break;   // b's block was among them
-                    const int i = (blk << 6) + lane;
-                    double v = i < nb_now ? bkey[i] : -2.0;
-                    v = mz_wave_max(v);
-                    const int vi = mz_wave_min_i32(i < nb_now && bkey[i] == v ? i : 0x7FFFFFFF);
+                const int k0 = b >> 6;
+                // b's block from its keys as loaded, the children's keys put
+                // in (b's key fell; new bins may lie in the block)
+                {
+                    const int i = (k0 << 6) + lane;
+                    double v = i < nb_now ? k0key : -2.0;
+                    for (int j = 0; j < n_ne; ++j)
+                        if (S.cbi[j] == i) v = S.ckey[j];
+                    const double mv = mz_wave_max(v);
+                    const int vi = mz_wave_min_i32(i < nb_now && v == mv ? i : 0x7FFFFFFF);
+                    v = mv;
                     if (lane == 0) {
-                        W.bmv[blk] = v;
-                        W.bmi[blk] = vi;
+                        W.bmv[k0] = v;
+                        W.bmi[k0] = vi;
+                    }
+                }
+                // the other blocks that gained new bins Nb .. nb_now - 1: their
+                // old maximum against the new keys (an old bin wins a tie, its id
+                // being lower)
+                if (nb_now > Nb) {
+                    for (int k = Nb >> 6; k <= (nb_now - 1) >> 6; ++k) {
+                        if (k == k0) continue;
+                        const bool in = (nbi >> 6) == k && nbi != 0x7FFFFFFF;
+                        const double mv = mz_wave_max(in ? nkey : -2.0);
+                        const int mi = mz_wave_min_i32(in && nkey == mv ? nbi : 0x7FFFFFFF);
+                        const bool had = (k << 6) < Nb;
+                        if (lane == 0 && (!had || mv > W.bmv[k])) {
+                            W.bmv[k] = mv;
+                            W.bmi[k] = mi;
+                        }
                     }
                 }
                 oct_wave_sync();
@@ -487,11 +607,10 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
     __syncthreads();
     if (S.defer_s) {
         if (threadIdx.x == 0) {
-            A.old_pia[c * 5 + 4] = -1;
-            if (!BIG && A.big_list) {
-                A.big_list[atomicAdd(&A.big_info[0], 1)] = c;
-                atomicMax(&A.big_info[1], N);
+            if (!BIG) {
+                oct_defer(A, c, N);
             } else {
+                A.old_pia[c * 5 + 4] = -1;
                 atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
             }
         }
@@ -509,10 +628,10 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
     // 830-933).  The reference walks the bins in order; here a scan over the
     // bins gives each its output slot and its place in the cell's draw
     // sequence (six sign draws per bin of more than two particles), and the
-    // four waves take the bins in turn.  A bin's sums are those of a 256-wide
-    // block reduction over its particles (lane l holds the block threads l,
-    // l + 64, l + 128, l + 192), so the values do not depend on which wave
-    // runs the bin.
+    // waves take the bins 64 at a time, a lane per bin.  A bin of more than 64
+    // particles is summed by its whole wave as a 256-wide block reduction
+    // (lane l holds the block threads l, l + 64, l + 128, l + 192), so the
+    // values do not depend on which wave runs the bin.
     int32_t *const bslot = W.bslot;
     int32_t *const bdraw = W.bdraw;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -560,48 +679,65 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
         if (threadIdx.x == 0) S.n_new_s = (int32_t)(carry & 0xFFFFFFFF);
         __syncthreads();
     }
-    for (int b = wv; b < Nbins; b += NW) {
-        const int32_t bnp = bins[b].w == 0.0 ? 0 : bins[b].np;
-        if (bnp <= 0) continue;
-        const int bs = bins[b].start, be = bins[b].end;
-        const int slot = bslot[b];
-        if (bnp <= 2) {
-            if (lane < bnp) {
-                const int64_t src = oct_slot(pis[bs + lane], s1, n1, s2);
-                const int64_t dst = oct_slot(slot + lane, s1, n1, s2);
-                A.tmp.x0[dst] = A.sp.x0[src]; A.tmp.x1[dst] = A.sp.x1[src]; A.tmp.x2[dst] = A.sp.x2[src];
-                A.tmp.v0[dst] = A.sp.v0[src]; A.tmp.v1[dst] = A.sp.v1[src]; A.tmp.v2[dst] = A.sp.v2[src];
-                A.tmp.w[dst] = A.sp.w[src];
-            }
-            continue;
+    // compute_bin_props! and compute_new_particles! per bin.  A bin of at most
+    // 64 particles takes one lane, its sums in particle order (the
+    // reference's order); a larger bin takes the whole wavefront.
+    for (int b0 = wv * 64; b0 < Nbins; b0 += NW * 64) {
+        const int bl = b0 + lane;
+        int32_t bnp = 0;
+        int bs = 0;
+        double Wb = 0.0;
+        if (bl < Nbins) {
+            Wb = bins[bl].w;
+            bnp = Wb == 0.0 ? 0 : (int32_t)bins[bl].np;
+            bs = bins[bl].start;
         }
-        const double W = bins[b].w, inv_w = 1.0 / W;
-        double vm[3], xm[3], sdv[3], sdx[3];
-        if (bnp <= 64) {
-            // one particle per lane: the block reduction's other waves add zeros
-            double w = 0.0, pv0 = 0.0, pv1 = 0.0, pv2 = 0.0, px0 = 0.0, px1 = 0.0, px2 = 0.0;
-            if (lane < bnp) {
-                const int64_t p = oct_slot(pis[bs + lane], s1, n1, s2);
-                w = A.sp.w[p];
-                pv0 = A.sp.v0[p]; pv1 = A.sp.v1[p]; pv2 = A.sp.v2[p];
-                px0 = A.sp.x0[p]; px1 = A.sp.x1[p]; px2 = A.sp.x2[p];
+        const bool lone = bnp > 0 && bnp <= 64;
+        const int slot = lone ? bslot[bl] : 0;
+        if (lone && bnp <= 2)
+            for (int j = 0; j < bnp; ++j) oct_copy(A, oct_slot(pis[bs + j], s1, n1, s2), oct_slot(slot + j, s1, n1, s2));
+        const bool sums = lone && bnp > 2;
+        const int jn = (int)mz_wave_max(sums ? (double)bnp : 0.0);
+        if (jn > 0) {
+            double vm[3] = {0.0, 0.0, 0.0}, xm[3] = {0.0, 0.0, 0.0};
+            for (int j = 0; j < jn; ++j) {
+                if (sums && j < bnp) {
+                    const int64_t p = oct_slot(pis[bs + j], s1, n1, s2);
+                    const double w = A.sp.w[p];
+                    vm[0] = vm[0] + w * A.sp.v0[p]; vm[1] = vm[1] + w * A.sp.v1[p]; vm[2] = vm[2] + w * A.sp.v2[p];
+                    xm[0] = xm[0] + w * A.sp.x0[p]; xm[1] = xm[1] + w * A.sp.x1[p]; xm[2] = xm[2] + w * A.sp.x2[p];
+                }
             }
-            const bool on = lane < bnp;
-            vm[0] = mz_wave_sum(on ? fma(w, pv0, 0.0) : 0.0) * inv_w;
-            vm[1] = mz_wave_sum(on ? fma(w, pv1, 0.0) : 0.0) * inv_w;
-            vm[2] = mz_wave_sum(on ? fma(w, pv2, 0.0) : 0.0) * inv_w;
-            xm[0] = mz_wave_sum(on ? fma(w, px0, 0.0) : 0.0) * inv_w;
-            xm[1] = mz_wave_sum(on ? fma(w, px1, 0.0) : 0.0) * inv_w;
-            xm[2] = mz_wave_sum(on ? fma(w, px2, 0.0) : 0.0) * inv_w;
-            const double a0 = pv0 - vm[0], a1 = pv1 - vm[1], a2 = pv2 - vm[2];
-            const double b0 = px0 - xm[0], b1 = px1 - xm[1], b2 = px2 - xm[2];
-            sdv[0] = sqrt(mz_wave_sum(on ? fma(w, a0 * a0, 0.0) : 0.0) * inv_w);
-            sdv[1] = sqrt(mz_wave_sum(on ? fma(w, a1 * a1, 0.0) : 0.0) * inv_w);
-            sdv[2] = sqrt(mz_wave_sum(on ? fma(w, a2 * a2, 0.0) : 0.0) * inv_w);
-            sdx[0] = sqrt(mz_wave_sum(on ? fma(w, b0 * b0, 0.0) : 0.0) * inv_w);
-            sdx[1] = sqrt(mz_wave_sum(on ? fma(w, b1 * b1, 0.0) : 0.0) * inv_w);
-            sdx[2] = sqrt(mz_wave_sum(on ? fma(w, b2 * b2, 0.0) : 0.0) * inv_w);
-        } else {
+            const double inv_w = 1.0 / (sums ? Wb : 1.0);
+            for (int d = 0; d < 3; ++d) { vm[d] = vm[d] * inv_w; xm[d] = xm[d] * inv_w; }
+            double vs[3] = {0.0, 0.0, 0.0}, xs[3] = {0.0, 0.0, 0.0};
+            for (int j = 0; j < jn; ++j) {
+                if (sums && j < bnp) {
+                    const int64_t p = oct_slot(pis[bs + j], s1, n1, s2);
+                    const double w = A.sp.w[p];
+                    const double a0 = A.sp.v0[p] - vm[0], a1 = A.sp.v1[p] - vm[1], a2 = A.sp.v2[p] - vm[2];
+                    const double e0 = A.sp.x0[p] - xm[0], e1 = A.sp.x1[p] - xm[1], e2 = A.sp.x2[p] - xm[2];
+                    vs[0] = vs[0] + w * (a0 * a0); vs[1] = vs[1] + w * (a1 * a1); vs[2] = vs[2] + w * (a2 * a2);
+                    xs[0] = xs[0] + w * (e0 * e0); xs[1] = xs[1] + w * (e1 * e1); xs[2] = xs[2] + w * (e2 * e2);
+                }
+            }
+            if (sums) {
+                double sdv[3], sdx[3];
+                for (int d = 0; d < 3; ++d) {
+                    sdv[d] = sqrt(vs[d] * inv_w);
+                    sdx[d] = sqrt(xs[d] * inv_w);
+                }
+                oct_emit(A, c, bdraw[bl], slot, Wb, vm, xm, sdv, sdx, s1, n1, s2);
+            }
+        }
+        // bins of more than 64 particles, one at a time on the wavefront
+        uint64_t bigm = __ballot(bnp > 64);
+        while (bigm) {
+            const int b = b0 + (int)__builtin_ctzll(bigm);
+            bigm &= bigm - 1;
+            const int bs = bins[b].start, be = bins[b].end;
+            const double Wb = bins[b].w, inv_w = 1.0 / Wb;
+            double vm[3], xm[3], sdv[3], sdx[3];
             // lane l stands for the block threads l + 64 q, q = 0..3
             double sv[4][3], sx[4][3];
 #pragma unroll
@@ -636,11 +772,11 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
                     const int64_t p = oct_slot(pis[i], s1, n1, s2);
                     const double w = A.sp.w[p];
                     const double a0 = A.sp.v0[p] - vm[0], a1 = A.sp.v1[p] - vm[1], a2 = A.sp.v2[p] - vm[2];
-                    const double b0 = A.sp.x0[p] - xm[0], b1 = A.sp.x1[p] - xm[1], b2 = A.sp.x2[p] - xm[2];
+                    const double e0 = A.sp.x0[p] - xm[0], e1 = A.sp.x1[p] - xm[1], e2 = A.sp.x2[p] - xm[2];
                     sv[q][0] = fma(w, a0 * a0, sv[q][0]); sv[q][1] = fma(w, a1 * a1, sv[q][1]);
                     sv[q][2] = fma(w, a2 * a2, sv[q][2]);
-                    sx[q][0] = fma(w, b0 * b0, sx[q][0]); sx[q][1] = fma(w, b1 * b1, sx[q][1]);
-                    sx[q][2] = fma(w, b2 * b2, sx[q][2]);
+                    sx[q][0] = fma(w, e0 * e0, sx[q][0]); sx[q][1] = fma(w, e1 * e1, sx[q][1]);
+                    sx[q][2] = fma(w, e2 * e2, sx[q][2]);
                 }
             }
 #pragma unroll
@@ -650,28 +786,7 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
                 sdx[d] = sqrt(((mz_wave_sum(sx[0][d]) + mz_wave_sum(sx[1][d])) +
                                (mz_wave_sum(sx[2][d]) + mz_wave_sum(sx[3][d]))) * inv_w);
             }
-        }
-        if (lane == 0) {
-            // 3 + 3 sign draws, at this bin's place in the cell's sequence
-            MzStream st = mz_stream((uint64_t)c);
-            st.k = 6 * (uint64_t)bdraw[b];
-            double dv[3], dx[3];
-            for (int d = 0; d < 3; ++d) dv[d] = mz_rand(A.rp, st) < 0.5 ? -1.0 : 1.0;
-            for (int d = 0; d < 3; ++d) dx[d] = mz_rand(A.rp, st) < 0.5 ? -1.0 : 1.0;
-            const double w1 = 0.5 * W;
-            for (int k = 0; k < 2; ++k) {
-                const double sg = k == 0 ? 1.0 : -1.0;
-                const int64_t dst = oct_slot(slot + k, s1, n1, s2);
-                double x0 = xm[0] + sg * dx[0] * sdx[0];
-                if (A.grid_clamp) x0 = x0 < A.min_x ? A.min_x : (x0 > A.max_x ? A.max_x : x0);
-                A.tmp.x0[dst] = x0;
-                A.tmp.x1[dst] = xm[1] + sg * dx[1] * sdx[1];
-                A.tmp.x2[dst] = xm[2] + sg * dx[2] * sdx[2];
-                A.tmp.v0[dst] = vm[0] + sg * dv[0] * sdv[0];
-                A.tmp.v1[dst] = vm[1] + sg * dv[1] * sdv[1];
-                A.tmp.v2[dst] = vm[2] + sg * dv[2] * sdv[2];
-                A.tmp.w[dst] = w1;
-            }
+            if (lane == 0) oct_emit(A, c, bdraw[b], bslot[b], Wb, vm, xm, sdv, sdx, s1, n1, s2);
         }
     }
     __syncthreads();
@@ -702,15 +817,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
         return;
     }
     if (N > OCT_MAX_N) {
-        if (threadIdx.x == 0) {
-            A.old_pia[c * 5 + 4] = -1;
-            if (A.big_list) {
-                A.big_list[atomicAdd(&A.big_info[0], 1)] = c;
-                atomicMax(&A.big_info[1], N);
-            } else {
-                atomicOr(&A.sp.ctr->err, MZ_ERR_MERGE_SIZE);
-            }
-        }
+        if (threadIdx.x == 0) oct_defer(A, c, N);
         return;
     }
     OctWork<int16_t> W;
@@ -724,15 +831,17 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
 }
 
 // The listed cells, one wavefront per workgroup, the working arrays in the
-// workgroup's slice of device memory; workgroup g takes the cells g, g + G,
-// ... of the list.
-__global__ void __launch_bounds__(64) k_octree_merge_big(OctArgs A) {
+// workgroup's slice of device memory.  The workgroups take cells from a queue,
+// the largest size class first, so the longest merges start first.
+__global__ void __launch_bounds__(64, 4) k_octree_merge_big(OctArgs A) {
     __shared__ OctShared S;
-    __shared__ double s_bmv[OCT_BIG_BLOCKS];
-    __shared__ int32_t s_bmi[OCT_BIG_BLOCKS];
+    // per 64-bin block: its largest key (nblk doubles), then its lowest bin
+    // holding that key (nblk int32s)
+    extern __shared__ double s_blk[];
     A.rp = mz_rng_resolve(A.rp);
     const int32_t n_list = A.big_info[0];
     const size_t g = blockIdx.x;
+    const int nblk = (A.big_bcap + 63) >> 6;
     OctWork<int32_t> W;
     W.pis = A.big_pis + g * (size_t)A.big_ncap;
     W.psort = A.big_psort + g * (size_t)A.big_ncap;
@@ -742,11 +851,17 @@ __global__ void __launch_bounds__(64) k_octree_merge_big(OctArgs A) {
     W.bslot = A.big_bslot + g * (size_t)A.big_bcap * 2;
     W.bdraw = W.bslot + A.big_bcap;
     W.bins_cap = A.big_bcap;
-    const bool blocks = A.big_bcap <= OCT_BIG_BLOCKS * 64;
-    W.bmv = blocks ? s_bmv : nullptr;
-    W.bmi = blocks ? s_bmi : nullptr;
-    for (int32_t j = blockIdx.x; j < n_list; j += gridDim.x) {
-        const int c = A.big_list[j];
+    W.bmv = A.big_blocks ? s_blk : nullptr;
+    W.bmi = A.big_blocks ? reinterpret_cast<int32_t *>(s_blk + nblk) : nullptr;
+    for (;;) {
+        if (threadIdx.x == 0) S.ticket_s = atomicAdd(&A.big_info[6], 1);
+        __syncthreads();
+        int32_t t = S.ticket_s;
+        __syncthreads();
+        if (t >= n_list) break;
+        int cls = 0;
+        while (cls < 3 && t >= A.big_info[2 + cls]) t -= A.big_info[2 + cls++];
+        const int c = A.big_list[(size_t)cls * A.n_cells + t];
         int32_t N, target;
         oct_pick(A, c, N, target);
         if (N <= A.big_ncap) oct_merge_cell<int32_t, 1, true>(A, c, N, target, W, S);
@@ -915,16 +1030,16 @@ static int octree_merge_launch(mz_store *h, int32_t species, int32_t threshold, 
     MzSpecies &s = h->sp[species - 1];
     const int nc = h->n_cells;
     int32_t *old_pia = (int32_t *)mz_scratch(h, "oct_oldpia", (size_t)nc * 5 * sizeof(int32_t));
-    int32_t *cnt = (int32_t *)mz_scratch(h, "oct_cnt", 3 * sizeof(int32_t));
+    int32_t *cnt = (int32_t *)mz_scratch(h, "oct_cnt", 8 * sizeof(int32_t));
     int32_t *ndel = (int32_t *)mz_scratch(h, "oct_ndel", (size_t)nc * sizeof(int32_t));
     int64_t *off = (int64_t *)mz_scratch(h, "oct_off", (size_t)(nc + 1) * sizeof(int64_t));
-    int32_t *big_list = (int32_t *)mz_scratch(h, "oct_big_list", (size_t)nc * sizeof(int32_t));
+    int32_t *big_list = (int32_t *)mz_scratch(h, "oct_big_list", (size_t)nc * 4 * sizeof(int32_t));
     MZ_CHECK(old_pia && cnt && ndel && off && big_list, "merge_octree: scratch allocation failed");
     MzCounters snap;
     int r = mz_read_counters(h, species, &snap);
     if (r) return r;
-    // cnt[0] counts merges; cnt[1], cnt[2] are the big-cell list's count and largest N
-    MZ_HIP(hipMemsetAsync(cnt, 0, 3 * sizeof(int32_t), h->stream));
+    // cnt[0] counts merges; cnt[1..7] are the big-cell lists' counts and queue
+    MZ_HIP(hipMemsetAsync(cnt, 0, 8 * sizeof(int32_t), h->stream));
     OctArgs A{};
     A.sp = mz_view(s);
     A.tmp = A.sp;
@@ -959,16 +1074,23 @@ static int octree_merge_launch(mz_store *h, int32_t species, int32_t threshold, 
             const int64_t ncap = ((int64_t)info[1] + 255) / 256 * 256;
             const int64_t bcap = std::min<int64_t>({(int64_t)max_Nbins, ncap, (int64_t)std::max(target_np, 2)}) + 8;
             const int64_t per = ncap * (2 * sizeof(int32_t) + 1) + bcap * (sizeof(OctBinT<int32_t>) + 3 * sizeof(double));
-            // workgroups: up to 12 per CU (three one-wave workgroups per SIMD),
-            // within a quarter of the free memory (at least 2 GiB, at most 16 GiB)
+            // the blocks' maxima in LDS: 12 bytes per 64-bin block
+            const int64_t nblk = (bcap + 63) / 64;
+            const bool blocks = nblk <= OCT_BIG_BLOCKS;
+            const size_t dyn = blocks ? (size_t)((nblk * 12 + 15) / 16 * 16) : 0;
+            // workgroups: as many as can be resident (the loop is latency-bound,
+            // so every resident wave helps), within a quarter of the free memory
+            // (at least 2 GiB, at most 16 GiB)
             size_t mfree = 0, mtot = 0;
             MZ_HIP(hipMemGetInfo(&mfree, &mtot));
             const int64_t budget = std::min<int64_t>((int64_t)16 << 30, std::max<int64_t>((int64_t)2 << 30,
                                                                                           (int64_t)(mfree / 4)));
-            int dev = 0, n_cu = 0;
+            int dev = 0, n_cu = 0, per_cu = 0;
             MZ_HIP(hipGetDevice(&dev));
             MZ_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-            const int64_t G = std::max<int64_t>(1, std::min<int64_t>({(int64_t)info[0], (int64_t)12 * n_cu,
+            MZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_octree_merge_big, 64, dyn));
+            const int64_t G = std::max<int64_t>(1, std::min<int64_t>({(int64_t)info[0],
+                                                                      (int64_t)std::max(per_cu, 1) * n_cu,
                                                                       budget / per}));
             int32_t *ib = (int32_t *)mz_scratch(h, "oct_big_idx", (size_t)(G * ncap * 2 * sizeof(int32_t)));
             uint8_t *ob = (uint8_t *)mz_scratch(h, "oct_big_oct", (size_t)(G * ncap));
@@ -984,7 +1106,8 @@ static int octree_merge_launch(mz_store *h, int32_t species, int32_t threshold, 
             A.big_bslot = sb;
             A.big_ncap = (int32_t)ncap;
             A.big_bcap = (int32_t)bcap;
-            hipLaunchKernelGGL(k_octree_merge_big, dim3((unsigned)G), dim3(64), 0, h->stream, A);
+            A.big_blocks = blocks ? 1 : 0;
+            hipLaunchKernelGGL(k_octree_merge_big, dim3((unsigned)G), dim3(64), dyn, h->stream, A);
         }
         hipLaunchKernelGGL(k_merge_ndel, dim3((nc + 255) / 256), dim3(256), 0, h->stream, old_pia, nc, ndel);
         hipLaunchKernelGGL(k_scan_i32m, dim3(1), dim3(1024), 0, h->stream, ndel, (int64_t)nc, off, off + nc);
