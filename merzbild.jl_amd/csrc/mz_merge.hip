@@ -52,6 +52,7 @@ struct OctArgs {
     int32_t *big_list, *big_info;
     // that path's arrays, one slice of big_ncap particles and big_bcap bins per workgroup
     int32_t *big_pis, *big_psort, *big_bslot;
+    double *big_stage;        // per workgroup: w, v0, v1, v2 of two tree-order buffers
     uint8_t *big_poct;
     double *big_bkey;
     void *big_bins;
@@ -86,7 +87,15 @@ struct OctWork {
     // the pick scans the keys themselves)
     double *bmv;
     int32_t *bmi;
+    // device-memory path: the particles' ids and values in tree order, in the
+    // buffer of the parity of their bin's depth (a split moves its bin's
+    // particles to the other buffer, so a pass reads a contiguous segment)
+    int32_t *sid0;            // buffer 0's ids; buffer 1's at sid0 + sid_q
+    int64_t sid_q;
+    double *stg;              // w, v0, v1, v2 of buffer 0, then of buffer 1, n_stg each
+    int64_t n_stg;
 };
+
 
 // The small per-cell state of the split loop and the output scan, in LDS.
 struct OctShared {
@@ -111,6 +120,37 @@ __device__ inline void oct_wave_sync() {
 
 __device__ inline int64_t oct_slot(int li, int32_t s1, int32_t n1, int32_t s2) {
     return li < n1 ? (int64_t)s1 - 1 + li : (int64_t)s2 - 1 + (li - n1);
+}
+
+// Buffer q's ids, and its values k (0: w, 1-3: v0-v2).
+template <class IdxT>
+__device__ __forceinline__ int32_t *oct_sid(const OctWork<IdxT> &W, int q) {
+    return W.sid0 + q * W.sid_q;
+}
+template <class IdxT>
+__device__ __forceinline__ double *oct_stg(const OctWork<IdxT> &W, int q, int k) {
+    return W.stg + (int64_t)(4 * q + k) * W.n_stg;
+}
+
+// The values of the particle at tree position i of a bin of depth parity par:
+// through the particle order (LDS path) or from the staged buffer.
+template <class IdxT, bool BIG>
+__device__ __forceinline__ void oct_ld(const OctArgs &A, const OctWork<IdxT> &W, const IdxT *pis, int32_t s1,
+                                       int32_t n1, int32_t s2, int par, int i, double &w, double &a0, double &a1,
+                                       double &a2) {
+    if constexpr (BIG) {
+        w = oct_stg(W, par, 0)[i]; a0 = oct_stg(W, par, 1)[i]; a1 = oct_stg(W, par, 2)[i]; a2 = oct_stg(W, par, 3)[i];
+    } else {
+        const int64_t p = oct_slot(pis[i], s1, n1, s2);
+        w = A.sp.w[p]; a0 = A.sp.v0[p]; a1 = A.sp.v1[p]; a2 = A.sp.v2[p];
+    }
+}
+
+// The id (place in the cell) of the particle at tree position i.
+template <class IdxT, bool BIG>
+__device__ __forceinline__ int oct_id(const OctWork<IdxT> &W, const IdxT *pis, int par, int i) {
+    if constexpr (BIG) return oct_sid(W, par)[i];
+    else return pis[i];
 }
 
 __device__ inline double blk_min(double v, double *red) {
@@ -273,7 +313,18 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
     const int32_t s1 = A.sp.pia.start1[c], n1 = A.sp.pia.n1[c], s2 = A.sp.pia.start2[c], n2 = A.sp.pia.n2[c];
     const int max_bins = min(A.max_Nbins, W.bins_cap);
     // init_octree! (merging_octree_N2.jl:947-984)
-    for (int i = threadIdx.x; i < N; i += blockDim.x) pis[i] = (IdxT)i;
+    if constexpr (BIG) {
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+            const int64_t p = oct_slot(i, s1, n1, s2);
+            W.sid0[i] = i;
+            W.stg[i] = A.sp.w[p];
+            W.stg[W.n_stg + i] = A.sp.v0[p];
+            W.stg[2 * W.n_stg + i] = A.sp.v1[p];
+            W.stg[3 * W.n_stg + i] = A.sp.v2[p];
+        }
+    } else {
+        for (int i = threadIdx.x; i < N; i += blockDim.x) pis[i] = (IdxT)i;
+    }
     double mn[3], mx[3];
     if (A.init_bounds == 3) {
         for (int d = 0; d < 3; ++d) { mn[d] = -299792458.0; mx[d] = 299792458.0; }
@@ -342,13 +393,18 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
             if (W.bmv && (b & ~63) + lane < Nbins_s) k0key = bkey[(b & ~63) + lane];
             const int bs = bins[b].start, be = bins[b].end;
             const int depth = bins[b].depth;
+            const int par = depth & 1;
             if (A.bounds_mode == 2) {
                 double bmn[3], bmx[3];
                 for (int d = 0; d < 3; ++d) {
-                    const double *vv = d == 0 ? A.sp.v0 : (d == 1 ? A.sp.v1 : A.sp.v2);
+                    const double *vv;
+                    if constexpr (BIG) vv = d == 0 ? oct_stg(W, par, 1) : (d == 1 ? oct_stg(W, par, 2) : oct_stg(W, par, 3));
+                    else vv = d == 0 ? A.sp.v0 : (d == 1 ? A.sp.v1 : A.sp.v2);
                     double lo = 9299792458.0, hi = -9299792458.0;
                     for (int i = bs + lane; i <= be; i += 64) {
-                        const double x = vv[oct_slot(pis[i], s1, n1, s2)];
+                        double x;
+                        if constexpr (BIG) x = vv[i];
+                        else x = vv[oct_slot(pis[i], s1, n1, s2)];
                         lo = fmin(lo, x);
                         hi = fmax(hi, x);
                     }
@@ -369,12 +425,12 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
                     // compute_v_mean! keeps the previous middle velocity in the sum
                     double nt = 0.0;
                     for (int i = bs; i <= be; ++i) {
-                        int64_t p = oct_slot(pis[i], s1, n1, s2);
-                        double w = A.sp.w[p];
+                        double w, a0, a1, a2;
+                        oct_ld<IdxT, BIG>(A, W, pis, s1, n1, s2, par, i, w, a0, a1, a2);
                         nt += w;
-                        vel_mid[0] = vel_mid[0] + w * A.sp.v0[p];
-                        vel_mid[1] = vel_mid[1] + w * A.sp.v1[p];
-                        vel_mid[2] = vel_mid[2] + w * A.sp.v2[p];
+                        vel_mid[0] = vel_mid[0] + w * a0;
+                        vel_mid[1] = vel_mid[1] + w * a1;
+                        vel_mid[2] = vel_mid[2] + w * a2;
                     }
                     for (int d = 0; d < 3; ++d) vel_mid[d] = vel_mid[d] / nt;
                 }
@@ -397,22 +453,27 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
             double nw = 0.0, nv0 = 0.0, nv1 = 0.0, nv2 = 0.0;
             IdxT pi1 = 0;
             if (bs + lane <= be) {
-                pi1 = pis[bs + lane];
-                const int64_t p = oct_slot(pi1, s1, n1, s2);
-                nw = A.sp.w[p]; nv0 = A.sp.v0[p]; nv1 = A.sp.v1[p]; nv2 = A.sp.v2[p];
+                pi1 = oct_id<IdxT, BIG>(W, pis, par, bs + lane);
+                oct_ld<IdxT, BIG>(A, W, pis, s1, n1, s2, par, bs + lane, nw, nv0, nv1, nv2);
             }
-            // the next chunk's ids, loaded a chunk ahead of its values
+            // the next chunk's ids, loaded a chunk ahead of its values (the
+            // staged values need no ids)
             IdxT pin = 0;
-            if (bs + 64 + lane <= be) pin = pis[bs + 64 + lane];
+            if (!BIG && bs + 64 + lane <= be) pin = pis[bs + 64 + lane];
             int o1 = 0;
             for (int i0 = bs; i0 <= be; i0 += 64) {
                 const int i = i0 + lane;
                 const double wi = nw, c0 = nv0, c1 = nv1, c2 = nv2;
                 if (i + 64 <= be) {
-                    const int64_t p = oct_slot(pin, s1, n1, s2);
-                    nw = A.sp.w[p]; nv0 = A.sp.v0[p]; nv1 = A.sp.v1[p]; nv2 = A.sp.v2[p];
+                    if constexpr (BIG) {
+                        nw = oct_stg(W, par, 0)[i + 64]; nv0 = oct_stg(W, par, 1)[i + 64];
+                        nv1 = oct_stg(W, par, 2)[i + 64]; nv2 = oct_stg(W, par, 3)[i + 64];
+                    } else {
+                        const int64_t p = oct_slot(pin, s1, n1, s2);
+                        nw = A.sp.w[p]; nv0 = A.sp.v0[p]; nv1 = A.sp.v1[p]; nv2 = A.sp.v2[p];
+                    }
                 }
-                if (i + 128 <= be) pin = pis[i + 128];
+                if (!BIG && i + 128 <= be) pin = pis[i + 128];
                 int o = 0;
                 if (i <= be) {
                     o = 1;
@@ -472,22 +533,35 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
                     const uint64_t m = __ballot(o1 == q);
                     if (o1 == q) dst = sq[q] + cq[q] - 1 - __popcll(m & lt);
                 }
-                if (bs + lane <= be) pis[dst] = pi1;
+                if (bs + lane <= be) {
+                    if constexpr (BIG) {
+                        // the values are still those of the bin's one chunk
+                        const int q = par ^ 1;
+                        oct_sid(W, q)[dst] = pi1;
+                        oct_stg(W, q, 0)[dst] = nw; oct_stg(W, q, 1)[dst] = nv0; oct_stg(W, q, 2)[dst] = nv1; oct_stg(W, q, 3)[dst] = nv2;
+                    } else {
+                        pis[dst] = pi1;
+                    }
+                }
                 oct_wave_sync();
             } else {
                 const uint64_t lt = mz_lanemask_lt();
                 int rq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
                 // each chunk's octants and ids are loaded a chunk ahead
                 int on = bs + lane <= be ? poct[lane] : 0;
-                IdxT pn = bs + lane <= be ? pis[bs + lane] : (IdxT)0;
+                IdxT pn = bs + lane <= be ? oct_id<IdxT, BIG>(W, pis, par, bs + lane) : (IdxT)0;
+                double qw = 0.0, qa = 0.0, qb = 0.0, qc = 0.0;
+                if (BIG && bs + lane <= be) oct_ld<IdxT, BIG>(A, W, pis, s1, n1, s2, par, bs + lane, qw, qa, qb, qc);
                 for (int i0 = bs; i0 <= be; i0 += 64) {
                     const int i = i0 + lane;
                     const bool valid = i <= be;
                     const int o = on;
                     const IdxT pv = pn;
+                    const double vw = qw, va = qa, vb = qb, vc = qc;
                     const bool vn = i + 64 <= be;
                     on = vn ? poct[i + 64 - bs] : 0;
-                    pn = vn ? pis[i + 64] : (IdxT)0;
+                    pn = vn ? oct_id<IdxT, BIG>(W, pis, par, i + 64) : (IdxT)0;
+                    if (BIG && vn) oct_ld<IdxT, BIG>(A, W, pis, s1, n1, s2, par, i + 64, qw, qa, qb, qc);
                     int dst = 0;
 #pragma unroll
                     for (int q = 1; q <= 8; ++q) {
@@ -495,18 +569,28 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
                         if (o == q) dst = sq[q] + cq[q] - 1 - (rq[q] + __popcll(m & lt));
                         rq[q] += __popcll(m);
                     }
-                    if (valid) psort[dst] = pv;
+                    if (valid) {
+                        if constexpr (BIG) {
+                            const int q = par ^ 1;
+                            oct_sid(W, q)[dst] = pv;
+                            oct_stg(W, q, 0)[dst] = vw; oct_stg(W, q, 1)[dst] = va; oct_stg(W, q, 2)[dst] = vb; oct_stg(W, q, 3)[dst] = vc;
+                        } else {
+                            psort[dst] = pv;
+                        }
+                    }
                 }
                 oct_wave_sync();
-                for (int i = bs + lane; i <= be; i += 256) {
-                    IdxT t[4];
+                if constexpr (!BIG) {
+                    for (int i = bs + lane; i <= be; i += 256) {
+                        IdxT t[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) t[u] = i + 64 * u <= be ? psort[i + 64 * u] : (IdxT)0;
+                        for (int u = 0; u < 4; ++u) t[u] = i + 64 * u <= be ? psort[i + 64 * u] : (IdxT)0;
 #pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        if (i + 64 * u <= be) pis[i + 64 * u] = t[u];
+                        for (int u = 0; u < 4; ++u)
+                            if (i + 64 * u <= be) pis[i + 64 * u] = t[u];
+                    }
+                    oct_wave_sync();
                 }
-                oct_wave_sync();
             }
             // the children (split_bin!, :588-633), lane o for octant o: the
             // k-th non-empty octant becomes bin b (k = 1) or Nb + k - 2
@@ -619,7 +703,8 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
     const int Nbins = Nbins_s;
     if (Nbins == 1 && threadIdx.x == 0) {
         double s = 0.0;
-        for (int i = 0; i < N; ++i) s += A.sp.w[oct_slot(pis[i], s1, n1, s2)];
+        const int par0 = bins[0].depth & 1;
+        for (int i = 0; i < N; ++i) s += A.sp.w[oct_slot(oct_id<IdxT, BIG>(W, pis, par0, i), s1, n1, s2)];
         bins[0].w = s;
     }
     __syncthreads();
@@ -685,26 +770,29 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
     for (int b0 = wv * 64; b0 < Nbins; b0 += NW * 64) {
         const int bl = b0 + lane;
         int32_t bnp = 0;
-        int bs = 0;
+        int bs = 0, par = 0;
         double Wb = 0.0;
         if (bl < Nbins) {
             Wb = bins[bl].w;
             bnp = Wb == 0.0 ? 0 : (int32_t)bins[bl].np;
             bs = bins[bl].start;
+            par = bins[bl].depth & 1;
         }
         const bool lone = bnp > 0 && bnp <= 64;
         const int slot = lone ? bslot[bl] : 0;
         if (lone && bnp <= 2)
-            for (int j = 0; j < bnp; ++j) oct_copy(A, oct_slot(pis[bs + j], s1, n1, s2), oct_slot(slot + j, s1, n1, s2));
+            for (int j = 0; j < bnp; ++j)
+                oct_copy(A, oct_slot(oct_id<IdxT, BIG>(W, pis, par, bs + j), s1, n1, s2), oct_slot(slot + j, s1, n1, s2));
         const bool sums = lone && bnp > 2;
         const int jn = (int)mz_wave_max(sums ? (double)bnp : 0.0);
         if (jn > 0) {
             double vm[3] = {0.0, 0.0, 0.0}, xm[3] = {0.0, 0.0, 0.0};
             for (int j = 0; j < jn; ++j) {
                 if (sums && j < bnp) {
-                    const int64_t p = oct_slot(pis[bs + j], s1, n1, s2);
-                    const double w = A.sp.w[p];
-                    vm[0] = vm[0] + w * A.sp.v0[p]; vm[1] = vm[1] + w * A.sp.v1[p]; vm[2] = vm[2] + w * A.sp.v2[p];
+                    const int64_t p = oct_slot(oct_id<IdxT, BIG>(W, pis, par, bs + j), s1, n1, s2);
+                    double w, a0, a1, a2;
+                    oct_ld<IdxT, BIG>(A, W, pis, s1, n1, s2, par, bs + j, w, a0, a1, a2);
+                    vm[0] = vm[0] + w * a0; vm[1] = vm[1] + w * a1; vm[2] = vm[2] + w * a2;
                     xm[0] = xm[0] + w * A.sp.x0[p]; xm[1] = xm[1] + w * A.sp.x1[p]; xm[2] = xm[2] + w * A.sp.x2[p];
                 }
             }
@@ -713,9 +801,10 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
             double vs[3] = {0.0, 0.0, 0.0}, xs[3] = {0.0, 0.0, 0.0};
             for (int j = 0; j < jn; ++j) {
                 if (sums && j < bnp) {
-                    const int64_t p = oct_slot(pis[bs + j], s1, n1, s2);
-                    const double w = A.sp.w[p];
-                    const double a0 = A.sp.v0[p] - vm[0], a1 = A.sp.v1[p] - vm[1], a2 = A.sp.v2[p] - vm[2];
+                    const int64_t p = oct_slot(oct_id<IdxT, BIG>(W, pis, par, bs + j), s1, n1, s2);
+                    double w, u0, u1, u2;
+                    oct_ld<IdxT, BIG>(A, W, pis, s1, n1, s2, par, bs + j, w, u0, u1, u2);
+                    const double a0 = u0 - vm[0], a1 = u1 - vm[1], a2 = u2 - vm[2];
                     const double e0 = A.sp.x0[p] - xm[0], e1 = A.sp.x1[p] - xm[1], e2 = A.sp.x2[p] - xm[2];
                     vs[0] = vs[0] + w * (a0 * a0); vs[1] = vs[1] + w * (a1 * a1); vs[2] = vs[2] + w * (a2 * a2);
                     xs[0] = xs[0] + w * (e0 * e0); xs[1] = xs[1] + w * (e1 * e1); xs[2] = xs[2] + w * (e2 * e2);
@@ -735,7 +824,7 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
         while (bigm) {
             const int b = b0 + (int)__builtin_ctzll(bigm);
             bigm &= bigm - 1;
-            const int bs = bins[b].start, be = bins[b].end;
+            const int bs = bins[b].start, be = bins[b].end, par = bins[b].depth & 1;
             const double Wb = bins[b].w, inv_w = 1.0 / Wb;
             double vm[3], xm[3], sdv[3], sdx[3];
             // lane l stands for the block threads l + 64 q, q = 0..3
@@ -747,10 +836,11 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 for (int i = bs + lane + 64 * q; i <= be; i += 256) {
-                    const int64_t p = oct_slot(pis[i], s1, n1, s2);
-                    const double w = A.sp.w[p];
-                    sv[q][0] = fma(w, A.sp.v0[p], sv[q][0]); sv[q][1] = fma(w, A.sp.v1[p], sv[q][1]);
-                    sv[q][2] = fma(w, A.sp.v2[p], sv[q][2]);
+                    const int64_t p = oct_slot(oct_id<IdxT, BIG>(W, pis, par, i), s1, n1, s2);
+                    double w, u0, u1, u2;
+                    oct_ld<IdxT, BIG>(A, W, pis, s1, n1, s2, par, i, w, u0, u1, u2);
+                    sv[q][0] = fma(w, u0, sv[q][0]); sv[q][1] = fma(w, u1, sv[q][1]);
+                    sv[q][2] = fma(w, u2, sv[q][2]);
                     sx[q][0] = fma(w, A.sp.x0[p], sx[q][0]); sx[q][1] = fma(w, A.sp.x1[p], sx[q][1]);
                     sx[q][2] = fma(w, A.sp.x2[p], sx[q][2]);
                 }
@@ -769,9 +859,10 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 for (int i = bs + lane + 64 * q; i <= be; i += 256) {
-                    const int64_t p = oct_slot(pis[i], s1, n1, s2);
-                    const double w = A.sp.w[p];
-                    const double a0 = A.sp.v0[p] - vm[0], a1 = A.sp.v1[p] - vm[1], a2 = A.sp.v2[p] - vm[2];
+                    const int64_t p = oct_slot(oct_id<IdxT, BIG>(W, pis, par, i), s1, n1, s2);
+                    double w, u0, u1, u2;
+                    oct_ld<IdxT, BIG>(A, W, pis, s1, n1, s2, par, i, w, u0, u1, u2);
+                    const double a0 = u0 - vm[0], a1 = u1 - vm[1], a2 = u2 - vm[2];
                     const double e0 = A.sp.x0[p] - xm[0], e1 = A.sp.x1[p] - xm[1], e2 = A.sp.x2[p] - xm[2];
                     sv[q][0] = fma(w, a0 * a0, sv[q][0]); sv[q][1] = fma(w, a1 * a1, sv[q][1]);
                     sv[q][2] = fma(w, a2 * a2, sv[q][2]);
@@ -833,7 +924,7 @@ __global__ void __launch_bounds__(256) k_octree_merge(OctArgs A) {
 // The listed cells, one wavefront per workgroup, the working arrays in the
 // workgroup's slice of device memory.  The workgroups take cells from a queue,
 // the largest size class first, so the longest merges start first.
-__global__ void __launch_bounds__(64, 4) k_octree_merge_big(OctArgs A) {
+__global__ void __launch_bounds__(64) k_octree_merge_big(OctArgs A) {
     __shared__ OctShared S;
     // per 64-bin block: its largest key (nblk doubles), then its lowest bin
     // holding that key (nblk int32s)
@@ -845,6 +936,10 @@ __global__ void __launch_bounds__(64, 4) k_octree_merge_big(OctArgs A) {
     OctWork<int32_t> W;
     W.pis = A.big_pis + g * (size_t)A.big_ncap;
     W.psort = A.big_psort + g * (size_t)A.big_ncap;
+    W.sid0 = W.pis;
+    W.sid_q = W.psort - W.pis;
+    W.stg = A.big_stage + g * (size_t)A.big_ncap * 8;
+    W.n_stg = A.big_ncap;
     W.poct = A.big_poct + g * (size_t)A.big_ncap;
     W.bins = reinterpret_cast<OctBinT<int32_t> *>(A.big_bins) + g * (size_t)A.big_bcap;
     W.bkey = A.big_bkey + g * (size_t)A.big_bcap;
@@ -1073,7 +1168,8 @@ static int octree_merge_launch(mz_store *h, int32_t species, int32_t threshold, 
             // largest per-cell target when there are several)
             const int64_t ncap = ((int64_t)info[1] + 255) / 256 * 256;
             const int64_t bcap = std::min<int64_t>({(int64_t)max_Nbins, ncap, (int64_t)std::max(target_np, 2)}) + 8;
-            const int64_t per = ncap * (2 * sizeof(int32_t) + 1) + bcap * (sizeof(OctBinT<int32_t>) + 3 * sizeof(double));
+            const int64_t per = ncap * (2 * sizeof(int32_t) + 8 * sizeof(double) + 1) +
+                                bcap * (sizeof(OctBinT<int32_t>) + 3 * sizeof(double));
             // the blocks' maxima in LDS: 12 bytes per 64-bin block
             const int64_t nblk = (bcap + 63) / 64;
             const bool blocks = nblk <= OCT_BIG_BLOCKS;
@@ -1094,13 +1190,15 @@ static int octree_merge_launch(mz_store *h, int32_t species, int32_t threshold, 
                                                                       budget / per}));
             int32_t *ib = (int32_t *)mz_scratch(h, "oct_big_idx", (size_t)(G * ncap * 2 * sizeof(int32_t)));
             uint8_t *ob = (uint8_t *)mz_scratch(h, "oct_big_oct", (size_t)(G * ncap));
+            double *stg = (double *)mz_scratch(h, "oct_big_stage", (size_t)(G * ncap * 8 * sizeof(double)));
             void *bb = mz_scratch(h, "oct_big_bins", (size_t)(G * bcap * sizeof(OctBinT<int32_t>)));
             double *kb = (double *)mz_scratch(h, "oct_big_key", (size_t)(G * bcap * sizeof(double)));
             int32_t *sb = (int32_t *)mz_scratch(h, "oct_big_slot", (size_t)(G * bcap * 2 * sizeof(int32_t)));
-            MZ_CHECK(ib && ob && bb && kb && sb, "merge_octree: scratch allocation for large cells failed");
+            MZ_CHECK(ib && ob && stg && bb && kb && sb, "merge_octree: scratch allocation for large cells failed");
             A.big_pis = ib;
             A.big_psort = ib + G * ncap;
             A.big_poct = ob;
+            A.big_stage = stg;
             A.big_bins = bb;
             A.big_bkey = kb;
             A.big_bslot = sb;
