@@ -31,6 +31,7 @@
 
 #define OCT_MAX_N 2048
 #define OCT_MAX_BINS 512
+#define OCT_LANE_NP 16        // bins of at most this many particles get a lane each for their moments
 #define OCT_BIG_BLOCKS 4096   // most 64-bin blocks whose maxima the device-memory path keeps in LDS
 
 struct OctArgs {
@@ -764,10 +765,109 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
         if (threadIdx.x == 0) S.n_new_s = (int32_t)(carry & 0xFFFFFFFF);
         __syncthreads();
     }
-    // compute_bin_props! and compute_new_particles! per bin.  A bin of at most
-    // 64 particles takes one lane, its sums in particle order (the
-    // reference's order); a larger bin takes the whole wavefront.
-    for (int b0 = wv * 64; b0 < Nbins; b0 += NW * 64) {
+    // one bin of more than two particles on the whole wavefront: one particle
+    // per lane up to 64, else the 256-wide block layout
+    auto wave_bin = [&](const int b) {
+        const int bs = bins[b].start, be = bins[b].end, par = bins[b].depth & 1;
+        const double Wb = bins[b].w, inv_w = 1.0 / Wb;
+        double vm[3], xm[3], sdv[3], sdx[3];
+        if (be - bs < 64) {
+            // one particle per lane
+            const bool on = bs + lane <= be;
+            double w = 0.0, u0 = 0.0, u1 = 0.0, u2 = 0.0, y0 = 0.0, y1 = 0.0, y2 = 0.0;
+            if (on) {
+                const int64_t p = oct_slot(oct_id<IdxT, BIG>(W, pis, par, bs + lane), s1, n1, s2);
+                oct_ld<IdxT, BIG>(A, W, pis, s1, n1, s2, par, bs + lane, w, u0, u1, u2);
+                y0 = A.sp.x0[p]; y1 = A.sp.x1[p]; y2 = A.sp.x2[p];
+            }
+            vm[0] = mz_wave_sum(on ? w * u0 : 0.0) * inv_w;
+            vm[1] = mz_wave_sum(on ? w * u1 : 0.0) * inv_w;
+            vm[2] = mz_wave_sum(on ? w * u2 : 0.0) * inv_w;
+            xm[0] = mz_wave_sum(on ? w * y0 : 0.0) * inv_w;
+            xm[1] = mz_wave_sum(on ? w * y1 : 0.0) * inv_w;
+            xm[2] = mz_wave_sum(on ? w * y2 : 0.0) * inv_w;
+            const double a0 = u0 - vm[0], a1 = u1 - vm[1], a2 = u2 - vm[2];
+            const double e0 = y0 - xm[0], e1 = y1 - xm[1], e2 = y2 - xm[2];
+            sdv[0] = sqrt(mz_wave_sum(on ? w * (a0 * a0) : 0.0) * inv_w);
+            sdv[1] = sqrt(mz_wave_sum(on ? w * (a1 * a1) : 0.0) * inv_w);
+            sdv[2] = sqrt(mz_wave_sum(on ? w * (a2 * a2) : 0.0) * inv_w);
+            sdx[0] = sqrt(mz_wave_sum(on ? w * (e0 * e0) : 0.0) * inv_w);
+            sdx[1] = sqrt(mz_wave_sum(on ? w * (e1 * e1) : 0.0) * inv_w);
+            sdx[2] = sqrt(mz_wave_sum(on ? w * (e2 * e2) : 0.0) * inv_w);
+            if (lane == 0) oct_emit(A, c, bdraw[b], bslot[b], Wb, vm, xm, sdv, sdx, s1, n1, s2);
+            return;
+        }
+        // lane l stands for the block threads l + 64 q, q = 0..3
+        double sv[4][3], sx[4][3];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) sv[q][d] = sx[q][d] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            for (int i = bs + lane + 64 * q; i <= be; i += 256) {
+                const int64_t p = oct_slot(oct_id<IdxT, BIG>(W, pis, par, i), s1, n1, s2);
+                double w, u0, u1, u2;
+                oct_ld<IdxT, BIG>(A, W, pis, s1, n1, s2, par, i, w, u0, u1, u2);
+                sv[q][0] = fma(w, u0, sv[q][0]); sv[q][1] = fma(w, u1, sv[q][1]);
+                sv[q][2] = fma(w, u2, sv[q][2]);
+                sx[q][0] = fma(w, A.sp.x0[p], sx[q][0]); sx[q][1] = fma(w, A.sp.x1[p], sx[q][1]);
+                sx[q][2] = fma(w, A.sp.x2[p], sx[q][2]);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            vm[d] = ((mz_wave_sum(sv[0][d]) + mz_wave_sum(sv[1][d])) +
+                     (mz_wave_sum(sv[2][d]) + mz_wave_sum(sv[3][d]))) * inv_w;
+            xm[d] = ((mz_wave_sum(sx[0][d]) + mz_wave_sum(sx[1][d])) +
+                     (mz_wave_sum(sx[2][d]) + mz_wave_sum(sx[3][d]))) * inv_w;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) sv[q][d] = sx[q][d] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            for (int i = bs + lane + 64 * q; i <= be; i += 256) {
+                const int64_t p = oct_slot(oct_id<IdxT, BIG>(W, pis, par, i), s1, n1, s2);
+                double w, u0, u1, u2;
+                oct_ld<IdxT, BIG>(A, W, pis, s1, n1, s2, par, i, w, u0, u1, u2);
+                const double a0 = u0 - vm[0], a1 = u1 - vm[1], a2 = u2 - vm[2];
+                const double e0 = A.sp.x0[p] - xm[0], e1 = A.sp.x1[p] - xm[1], e2 = A.sp.x2[p] - xm[2];
+                sv[q][0] = fma(w, a0 * a0, sv[q][0]); sv[q][1] = fma(w, a1 * a1, sv[q][1]);
+                sv[q][2] = fma(w, a2 * a2, sv[q][2]);
+                sx[q][0] = fma(w, e0 * e0, sx[q][0]); sx[q][1] = fma(w, e1 * e1, sx[q][1]);
+                sx[q][2] = fma(w, e2 * e2, sx[q][2]);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            sdv[d] = sqrt(((mz_wave_sum(sv[0][d]) + mz_wave_sum(sv[1][d])) +
+                           (mz_wave_sum(sv[2][d]) + mz_wave_sum(sv[3][d]))) * inv_w);
+            sdx[d] = sqrt(((mz_wave_sum(sx[0][d]) + mz_wave_sum(sx[1][d])) +
+                           (mz_wave_sum(sx[2][d]) + mz_wave_sum(sx[3][d]))) * inv_w);
+        }
+        if (lane == 0) oct_emit(A, c, bdraw[b], bslot[b], Wb, vm, xm, sdv, sdx, s1, n1, s2);
+    };
+    // compute_bin_props! and compute_new_particles! per bin.  With at least 32
+    // bins per wave a bin of at most OCT_LANE_NP particles takes one lane, its
+    // sums in particle order (the reference's order), and a larger bin the whole
+    // wavefront; with fewer bins the waves take the bins in turn.
+    if (Nbins < 32 * NW) {
+        for (int b = wv; b < Nbins; b += NW) {
+            const int32_t bnp = bins[b].w == 0.0 ? 0 : (int32_t)bins[b].np;
+            if (bnp <= 0) continue;
+            if (bnp <= 2) {
+                const int bs = bins[b].start, par = bins[b].depth & 1;
+                if (lane < bnp)
+                    oct_copy(A, oct_slot(oct_id<IdxT, BIG>(W, pis, par, bs + lane), s1, n1, s2),
+                             oct_slot(bslot[b] + lane, s1, n1, s2));
+                continue;
+            }
+            wave_bin(b);
+        }
+    }
+    for (int b0 = wv * 64; Nbins >= 32 * NW && b0 < Nbins; b0 += NW * 64) {
         const int bl = b0 + lane;
         int32_t bnp = 0;
         int bs = 0, par = 0;
@@ -778,7 +878,7 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
             bs = bins[bl].start;
             par = bins[bl].depth & 1;
         }
-        const bool lone = bnp > 0 && bnp <= 64;
+        const bool lone = bnp > 0 && bnp <= OCT_LANE_NP;
         const int slot = lone ? bslot[bl] : 0;
         if (lone && bnp <= 2)
             for (int j = 0; j < bnp; ++j)
@@ -819,65 +919,12 @@ __device__ __forceinline__ void oct_merge_cell(const OctArgs &A, const int c, co
                 oct_emit(A, c, bdraw[bl], slot, Wb, vm, xm, sdv, sdx, s1, n1, s2);
             }
         }
-        // bins of more than 64 particles, one at a time on the wavefront
-        uint64_t bigm = __ballot(bnp > 64);
+        // larger bins, one at a time on the wavefront
+        uint64_t bigm = __ballot(bnp > OCT_LANE_NP);
         while (bigm) {
             const int b = b0 + (int)__builtin_ctzll(bigm);
             bigm &= bigm - 1;
-            const int bs = bins[b].start, be = bins[b].end, par = bins[b].depth & 1;
-            const double Wb = bins[b].w, inv_w = 1.0 / Wb;
-            double vm[3], xm[3], sdv[3], sdx[3];
-            // lane l stands for the block threads l + 64 q, q = 0..3
-            double sv[4][3], sx[4][3];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int d = 0; d < 3; ++d) sv[q][d] = sx[q][d] = 0.0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                for (int i = bs + lane + 64 * q; i <= be; i += 256) {
-                    const int64_t p = oct_slot(oct_id<IdxT, BIG>(W, pis, par, i), s1, n1, s2);
-                    double w, u0, u1, u2;
-                    oct_ld<IdxT, BIG>(A, W, pis, s1, n1, s2, par, i, w, u0, u1, u2);
-                    sv[q][0] = fma(w, u0, sv[q][0]); sv[q][1] = fma(w, u1, sv[q][1]);
-                    sv[q][2] = fma(w, u2, sv[q][2]);
-                    sx[q][0] = fma(w, A.sp.x0[p], sx[q][0]); sx[q][1] = fma(w, A.sp.x1[p], sx[q][1]);
-                    sx[q][2] = fma(w, A.sp.x2[p], sx[q][2]);
-                }
-            }
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                vm[d] = ((mz_wave_sum(sv[0][d]) + mz_wave_sum(sv[1][d])) +
-                         (mz_wave_sum(sv[2][d]) + mz_wave_sum(sv[3][d]))) * inv_w;
-                xm[d] = ((mz_wave_sum(sx[0][d]) + mz_wave_sum(sx[1][d])) +
-                         (mz_wave_sum(sx[2][d]) + mz_wave_sum(sx[3][d]))) * inv_w;
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int d = 0; d < 3; ++d) sv[q][d] = sx[q][d] = 0.0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                for (int i = bs + lane + 64 * q; i <= be; i += 256) {
-                    const int64_t p = oct_slot(oct_id<IdxT, BIG>(W, pis, par, i), s1, n1, s2);
-                    double w, u0, u1, u2;
-                    oct_ld<IdxT, BIG>(A, W, pis, s1, n1, s2, par, i, w, u0, u1, u2);
-                    const double a0 = u0 - vm[0], a1 = u1 - vm[1], a2 = u2 - vm[2];
-                    const double e0 = A.sp.x0[p] - xm[0], e1 = A.sp.x1[p] - xm[1], e2 = A.sp.x2[p] - xm[2];
-                    sv[q][0] = fma(w, a0 * a0, sv[q][0]); sv[q][1] = fma(w, a1 * a1, sv[q][1]);
-                    sv[q][2] = fma(w, a2 * a2, sv[q][2]);
-                    sx[q][0] = fma(w, e0 * e0, sx[q][0]); sx[q][1] = fma(w, e1 * e1, sx[q][1]);
-                    sx[q][2] = fma(w, e2 * e2, sx[q][2]);
-                }
-            }
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                sdv[d] = sqrt(((mz_wave_sum(sv[0][d]) + mz_wave_sum(sv[1][d])) +
-                               (mz_wave_sum(sv[2][d]) + mz_wave_sum(sv[3][d]))) * inv_w);
-                sdx[d] = sqrt(((mz_wave_sum(sx[0][d]) + mz_wave_sum(sx[1][d])) +
-                               (mz_wave_sum(sx[2][d]) + mz_wave_sum(sx[3][d]))) * inv_w);
-            }
-            if (lane == 0) oct_emit(A, c, bdraw[b], bslot[b], Wb, vm, xm, sdv, sdx, s1, n1, s2);
+            wave_bin(b);
         }
     }
     __syncthreads();
