@@ -207,6 +207,12 @@ int mz_compute_props_moments(mz_store *h, const int32_t *powers, int32_t n_momen
                              const double *masses);
 /* moments[n_species][n_cells][n_moments] of the last mz_compute_props_moments. */
 int mz_get_props_moments(mz_store *h, double *moments);
+/* compute_mixed_moment(particles, pia, cell, species, powers; sum_scaler,
+ * res_scaler) (src/properties/physical_props.jl:473-501) for every cell in
+ * cell_lo..cell_hi (1-based, inclusive): out[cell - cell_lo] =
+ * res_scaler * sum over groups 1 and 2 of w sum_scaler vx^px vy^py vz^pz. */
+int mz_compute_mixed_moment(mz_store *h, int32_t species, int32_t cell_lo, int32_t cell_hi, int32_t px,
+                            int32_t py, int32_t pz, double sum_scaler, double res_scaler, double *out);
 /* PhysProps arrays [n_species][n_cells] (v: [n_species][n_cells][3]); any may be NULL. */
 int mz_get_props(mz_store *h, double *lpa, double *np, double *n, double *v, double *T);
 

@@ -73,6 +73,7 @@ _SIGS = {
     "mz_get_props": (C.c_int, [C.c_void_p, P(f64), P(f64), P(f64), P(f64), P(f64)]),
     "mz_compute_props_moments": (C.c_int, [C.c_void_p, P(i32), i32, f64, P(f64)]),
     "mz_get_props_moments": (C.c_int, [C.c_void_p, P(f64)]),
+    "mz_compute_mixed_moment": (C.c_int, [C.c_void_p, i32, i32, i32, i32, i32, i32, f64, f64, P(f64)]),
     "mz_merge_octree": (C.c_int, [C.c_void_p, i32, i32, i32, i32, i32, i32, i32, i32, i32, i32, u64, i32,
                                   P(i64)]),
     "mz_merge_octree_targets": (C.c_int, [C.c_void_p, i32, P(i32), P(i32), i32, i32, i32, i32, i32, i32, i32, u64,
@@ -737,6 +738,20 @@ def compute_props_with_total_moments(store, species_data, phys_props, fetch=True
     _check(load_library().mz_compute_props_moments(store.h, phys_props.moment_powers.ctypes.data_as(P(i32)),
                                                     phys_props.n_moments, phys_props.Tref, _d(m)))
     return phys_props.fetch(store, moments=True) if fetch else None
+
+
+def compute_mixed_moment(store, species, powers, cells=None, sum_scaler=1.0, res_scaler=1.0):
+    """compute_mixed_moment(particles, pia, cell, species, powers; sum_scaler,
+    res_scaler) (src/properties/physical_props.jl:473-501).  cells is one
+    1-based cell (returns a float) or an inclusive (lo, hi) range (returns an
+    array, one value per cell); None means every cell."""
+    px, py, pz = (int(p) for p in powers)
+    single = cells is not None and np.isscalar(cells)
+    lo, hi = (int(cells), int(cells)) if single else ((1, store.n_cells) if cells is None else map(int, cells))
+    out = np.zeros(max(hi - lo + 1, 1))
+    _check(load_library().mz_compute_mixed_moment(store.h, int(species), lo, hi, px, py, pz, float(sum_scaler),
+                                                   float(res_scaler), _d(out)))
+    return float(out[0]) if single else out
 
 
 def compute_props_sorted(store, species_data=None, phys_props=None, ndens_not_Np=False, fetch=True):

@@ -1971,6 +1971,51 @@ extern "C" int mz_get_props_moments(mz_store *h, double *moments) {
     return 0;
 }
 
+// compute_mixed_moment (physical_props.jl:473-501) over a range of cells:
+// sum over groups 1 and 2 of w * sum_scaler * vx^px * vy^py * vz^pz, times
+// res_scaler.  One wave per cell; the sum is a wave reduction, not the
+// reference's left-to-right loop, so the last bits can differ.
+__global__ void __launch_bounds__(256) k_mixed_moment(MzViews views, int32_t sidx, int32_t c0, int32_t n_out,
+                                                      int32_t px, int32_t py, int32_t pz, double sum_scaler,
+                                                      double res_scaler, double *__restrict__ out) {
+    const MzSpView &sp = views.v[sidx];
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= n_out) return;
+    const int c = c0 + k;
+    const int32_t st1 = sp.pia.start1[c] - 1, n1 = sp.pia.n1[c] > 0 ? sp.pia.n1[c] : 0;
+    const int32_t st2 = sp.pia.start2[c] - 1, n2 = sp.pia.n2[c] > 0 ? sp.pia.n2[c] : 0;
+    const int32_t tot = n1 + n2;
+    const double ex = (double)px, ey = (double)py, ez = (double)pz;
+    double r = 0.0;
+    for (int32_t i = lane; i < tot; i += 64) {
+        const int64_t p = i < n1 ? (int64_t)st1 + i : (int64_t)st2 + (i - n1);
+        r += sp.w[p] * sum_scaler * pow(sp.v0[p], ex) * pow(sp.v1[p], ey) * pow(sp.v2[p], ez);
+    }
+    r = mz_wave_sum(r);
+    if (lane == 0) out[k] = r * res_scaler;
+}
+
+extern "C" int mz_compute_mixed_moment(mz_store *h, int32_t species, int32_t cell_lo, int32_t cell_hi, int32_t px,
+                                       int32_t py, int32_t pz, double sum_scaler, double res_scaler,
+                                       double *out) {
+    MZ_CHECK(species >= 1 && species <= h->n_species, "mixed moment: species out of range");
+    MZ_CHECK(cell_lo >= 1 && cell_hi <= h->n_cells && cell_lo <= cell_hi, "mixed moment: cells out of range");
+    MZ_CHECK(out != nullptr, "mixed moment: NULL output");
+    const int32_t n_out = cell_hi - cell_lo + 1;
+    double *d_out = (double *)mz_scratch(h, "mixed_moment", (size_t)h->n_cells * sizeof(double));
+    MZ_CHECK(d_out, "mixed moment: scratch allocation failed");
+    {
+        MzTimed tm(h, "mixed_moment");
+        hipLaunchKernelGGL(k_mixed_moment, dim3((n_out + 3) / 4), dim3(256), 0, h->stream, mz_views(h),
+                           species - 1, cell_lo - 1, n_out, px, py, pz, sum_scaler, res_scaler, d_out);
+        MZ_HIP(hipGetLastError());
+    }
+    MZ_HIP(hipMemcpyAsync(out, d_out, (size_t)n_out * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    MZ_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
 extern "C" int mz_get_props(mz_store *h, double *lpa, double *np, double *n, double *v, double *T) {
     MZ_HIP(hipStreamSynchronize(h->stream));
     size_t pn = (size_t)h->n_species * h->n_cells;

@@ -53,6 +53,7 @@ P = __import__("ctypes").POINTER
 __all__ = ["DeviceRNG", "DeviceParticleVector", "DevicePIA", "CollisionFactorsArray", "CollisionData",
            "GridSortInPlace", "OctreeN2Merge", "NNLSMerge", "create_collision_factors_array", "ntc",
            "ntc_equal_weight", "convect_particles", "sort_particles", "compute_props", "compute_props_sorted",
+           "compute_mixed_moment",
            "merge_octree_N2_based", "merge_nnls_based", "squash_pia", "restore_particle_ordering", "PhysProps", "flush", "MergeStatus"]
 
 PHASE_NTC, PHASE_CONVECT, PHASE_MERGE = 1, 2, 3
@@ -506,6 +507,15 @@ def compute_props_sorted(particles, pia, species_data, phys_props):
     store = pia.store
     flush(store)
     _cps(store, species_data, phys_props, ndens_not_Np=phys_props.ndens_not_Np)
+
+
+def compute_mixed_moment(particles, pia, cell, species, powers, sum_scaler=1.0, res_scaler=1.0):
+    """compute_mixed_moment(particles, pia, cell, species, powers; sum_scaler,
+    res_scaler) (src/properties/physical_props.jl:473-501); returns the moment."""
+    from . import compute_mixed_moment as _cmm
+    store = pia.store
+    flush(store)
+    return _cmm(store, species, powers, cells=int(cell), sum_scaler=sum_scaler, res_scaler=res_scaler)
 
 
 def squash_pia(particles, pia, species=None):

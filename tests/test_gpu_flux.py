@@ -93,18 +93,31 @@ def test_flux_vs_oracle_two_species(hip, mzo, cells):
 
 def test_particle_dump_from_store(hip, mzo, tmp_path):
     """write_netcdf(nc_filename, pv, pia, species, species_data) (io.jl:842-892)
-    from a device store: the particles cell by cell as the store holds them."""
+    from a device store.  The layout is the one particle creation leaves
+    (io.jl:864-890 walks it): cell 1's group 2 sits after cell 6's particles
+    and the index array is a permutation, so the file order differs from
+    both the position order and the id order.  The expected file is the
+    oracle's own walk over its ParticleVector and PIA."""
     from scipy.io import netcdf_file
-    pv, pia = _kat(mzo)
+    pv, _ = _kat(mzo)
+    pv.index[:6] = [3, 1, 6, 5, 2, 4]          # position p holds particle id index[p]
+    for i in range(6):
+        pv.x[i] = [0.25 + 0.5 * i, 1.0 + i, -2.0 * i]
+    pia = mzo.PIA(8, 1)
+    pia.set(1, 1, 4, 1, 2, 2, 5, 6, 2)        # cell 1: group 1 at 1-2, group 2 at 5-6
+    pia.set(6, 1, 2, 3, 4, 2)                 # cell 6 between them, at 3-4
+    pia.set_n_total(1, 6)
+    pia.set_contiguous(1, False)
     st = hip.DeviceStore(8, 1, 64)
     helpers.upload_from_oracle(st, mzo, pv, pia)
     sp = [hip.Species("Ar", M_AR)]
     path = str(tmp_path / "dump.nc")
     hip.write_netcdf(path, st, sp, species=1)
-    got = st.download_species(1)
-    order = [0, 1, 2, 3, 4, 5]     # cell 1: group 1 (1-2) then group 2 (3-4); cell 6: 5-6
+    walk = [(1, p) for p in (1, 2, 5, 6)] + [(6, p) for p in (3, 4)]
+    ids = [int(pv.index[p - 1]) - 1 for _, p in walk]
     with netcdf_file(path, "r", mmap=False) as f:
-        assert np.array_equal(f.variables["w"][:], got["w"][order])
-        assert np.array_equal(f.variables["v"][:], got["v"][order].T)
-        assert list(f.variables["cell"][:]) == [1, 1, 1, 1, 6, 6]
+        assert list(f.variables["cell"][:]) == [c for c, _ in walk]
+        assert np.array_equal(f.variables["w"][:], pv.w[ids])
+        assert np.array_equal(f.variables["v"][:], pv.v[ids].T)
+        assert np.array_equal(f.variables["x"][:], pv.x[ids].T)
     st.close()

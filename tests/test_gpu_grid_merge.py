@@ -92,3 +92,57 @@ def test_grid_merge_conserves_at_size(hip, mzo):
         eb, ea = (wb * (vb ** 2).sum(1)).sum(), (wa * (va ** 2).sum(1)).sum()
         assert abs(ea / eb - 1) < 1e-12
     st.close()
+
+
+def _grid_case(hip, mzo, counts, N, seed):
+    pv, pia = _grouped_cells(mzo, counts, seed=seed)
+    nc = len(counts)
+    st = hip.DeviceStore(nc, 1, sum(counts) + 512)
+    helpers.upload_from_oracle(st, mzo, pv, pia)
+    st.set_seed(1234)
+    sp = [hip.Species("Ar", helpers.MASS_AR)]
+    pp = hip.compute_props(st, sp)
+    return pv, pia, st, sp, pp
+
+
+def test_grid_merge_at_kernel_limits(hip, mzo):
+    """The kernel's largest case: a 16 x 16 x 16 velocity grid (4,104 bins
+    with the 8 outer ones, its bin limit) and a cell of exactly 2,048
+    particles (its particle limit, the bitonic sort over 2,048 keys), next
+    to smaller cells: bit for bit against the oracle."""
+    counts = [2048, 0, 700, 2047, 33]
+    pv, pia, st, sp, pp = _grid_case(hip, mzo, counts, (16, 16, 16), seed=31)
+    tag = (5 << 4) | 2
+    merged = hip.merge_grid_based(st, 1, 30, hip.GridN2Merge(16, 16, 16, extent_multiplier=3.0),
+                                  species_data=sp, tag=tag)
+    assert merged == 4
+    _oracle(mzo, pv, pia, 30, mzo.GridMerge(16, 16, 16, 3.0), props=pp, tag=tag)
+    got = st.download_species(1)
+    assert got["err"] == 0
+    helpers.assert_state_matches(mzo, got, pv, pia, rtol=RTOL)
+    st.close()
+
+
+def test_grid_merge_rejects_oversize_cell(hip, mzo):
+    """A cell of 2,049 particles is beyond the kernel: the call flags the
+    merge-size error and leaves that cell as it was, while the other cells
+    merge exactly as the oracle merges them."""
+    counts = [2049, 400, 90]
+    pv, pia, st, sp, pp = _grid_case(hip, mzo, counts, (4, 4, 4), seed=32)
+    tag = 11
+    before = st.download_species(1)
+    hip.merge_grid_based(st, 1, 30, hip.GridN2Merge(4, extent_multiplier=3.5), species_data=sp, squash=False,
+                         tag=tag)
+    got = st.download_species(1)
+    assert got["err"] & 32
+    assert np.array_equal(got["pia"][0], before["pia"][0])
+    s1, e1 = int(before["pia"][0][1]), int(before["pia"][0][2])
+    assert np.array_equal(got["w"][s1 - 1:e1], before["w"][s1 - 1:e1])
+    assert np.array_equal(got["v"][s1 - 1:e1], before["v"][s1 - 1:e1])
+    r = mzo.RNG(seed=1234, tag=tag)
+    omg = mzo.GridMerge(4, 4, 4, 3.5)
+    for c in (2, 3):
+        omg.extent_props(pp.T[0, c - 1], pp.v[0, c - 1], helpers.MASS_AR)
+        mzo.lib().mzo_merge_grid_based(r.ref, omg.p, pv.p, pia.p, c, 1, None)
+    helpers.assert_state_matches(mzo, got, pv, pia, rtol=RTOL)
+    st.close()
